@@ -1,0 +1,160 @@
+/*
+ * admp_amd -- C ABI of the MI355X (gfx950) electrostatics hot path.
+ *
+ * Every entry point takes plain device pointers, sizes, a host copy of the
+ * 3x3 box (row vectors, row-major doubles) and a hipStream_t passed as void*.
+ * The caller owns every buffer: inputs, outputs and a scratch workspace whose
+ * size the matching *_workspace_bytes query returns.  No entry point allocates
+ * device memory, and only admp_nblist_count synchronises the stream (to return
+ * the pair count the caller must allocate for).
+ *
+ * Return value: ADMP_OK (0); ADMP_OVERFLOW (>0) when an output capacity is too
+ * small (the required size is written to the out-param); a negative code for an
+ * invalid argument or a HIP error.  No C++ exception crosses this boundary.
+ *
+ * Precision codes: ADMP_F64 = float64 throughout; ADMP_F32 = float32 bulk
+ * arithmetic (mesh, FFT, pair terms) with float64 accumulation of energies,
+ * virials and per-atom charge sums.
+ *
+ * Reference interfaces are cited as path:line relative to the torch-admp 1.1.5
+ * source tree.  The reference is pure PyTorch; each entry point replaces the
+ * torch-op sequence named in its comment.
+ */
+#ifndef ADMP_AMD_H
+#define ADMP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ADMP_OK 0
+#define ADMP_OVERFLOW 1
+#define ADMP_ERR_INVALID (-1)
+#define ADMP_ERR_HIP (-2)
+#define ADMP_ERR_UNSUPPORTED (-3)
+
+#define ADMP_F64 0
+#define ADMP_F32 1
+
+/* k-vector convention: ADMP_KCONV_REFERENCE reproduces k = 2 pi m A (rows of
+ * A = box^-1, src/torch_admp/recip.py:304); ADMP_KCONV_COLUMNS uses the
+ * reciprocal lattice vectors proper, k = 2 pi A m.  Identical for orthorhombic
+ * and symmetric boxes. */
+#define ADMP_KCONV_REFERENCE 0
+#define ADMP_KCONV_COLUMNS 1
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int admp_version(void);
+
+/* Human-readable text for a status code (static storage). */
+const char* admp_status_string(int status);
+
+/* ------------------------------------------------------------------------ */
+/* Neighbour list: replaces TorchNeighborList.forward / forward_pbc /
+ * wrap_positions (src/torch_admp/nblist.py:304-470).  Pairs (i, j) are kept
+ * when 0.01 < |image_j - wrapped_i| < cutoff with both orderings listed, as
+ * nblist.py:439-445.  box == NULL selects open boundaries (nblist.py:472-490,
+ * same set without the 0.01 floor). Pairs are int64 (P, 2), grouped by i.  */
+
+/* Workspace for a neighbour-list build of n atoms. */
+int admp_nblist_workspace_bytes(int64_t n, const double* box, double cutoff, size_t* bytes);
+
+/* Bins atoms into cells and counts pairs; synchronises `stream` and writes the
+ * total number of ordered pairs to *n_pairs (host).  The workspace must be
+ * passed unchanged to the following admp_nblist_fill. */
+int admp_nblist_count(const double* pos, int64_t n, const double* box, double cutoff,
+                      void* workspace, size_t ws_bytes, int64_t* n_pairs, void* stream);
+
+/* Writes the pairs counted by admp_nblist_count; capacity is in pairs. */
+int admp_nblist_fill(const double* pos, int64_t n, const double* box, double cutoff,
+                     void* workspace, size_t ws_bytes, int64_t* pairs, int64_t capacity,
+                     void* stream);
+
+/* Pair distances |pbc_shift(r_i - r_j)| (src/torch_admp/nblist.py:517-551,
+ * src/torch_admp/spatial.py:15-36); box == NULL gives |r_j - r_i|.  Also writes
+ * the minimum-image displacement dr (P, 3) used by the backward. */
+int admp_pair_ds(const double* pos, const int64_t* pairs, int64_t n_pairs, const double* box,
+                 double* ds, double* dr, void* stream);
+
+/* Backward of admp_pair_ds: grad_pos (n, 3) += dL/dds * d ds / d r;
+ * grad_box (9) += dL/dds * d ds / d box (may be NULL).  grad_pos and grad_box
+ * must be zeroed by the caller; accumulation uses float64 atomics. */
+int admp_pair_ds_backward(const double* grad_ds, const int64_t* pairs, int64_t n_pairs,
+                          const double* pos, const double* box, const double* dr,
+                          const double* ds, double* grad_pos, double* grad_box, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Pair energy over a caller-supplied pair list, nf frames of P pairs each.
+ * mode 0: Ewald real space eps*erfc(kappa d) qi qj safe_inv(d, 1e-4) bs
+ *         (src/torch_admp/pme.py:312-349)
+ * mode 1: open boundary eps*qi qj safe_inv(d, 1e-8) bs (src/torch_admp/pme.py:529-561)
+ * mode 2: Gaussian damping -eps*erfc(d / (2 eta_ij)) qi qj safe_inv(d, 1e-4) bs,
+ *         eta_ij = sqrt((eta_i^2 + eta_j^2)/2) (src/torch_admp/qeq.py:106-121)
+ * q is (nf, n); eta (nf, n) for mode 2 else NULL; pairs (nf, P, 2); ds, bs (nf, P).
+ * Writes energy (nf) and, when non-NULL, dE/dds (nf, P) and dE/dq (nf, n)
+ * (dE/dq is overwritten).  bs is float64. */
+int admp_pair_energy(int32_t mode, int64_t nf, int64_t n, int64_t n_pairs, const double* q,
+                     const double* eta, const int64_t* pairs, const double* ds,
+                     const double* bs, double kappa, double dielectric, double* energy,
+                     double* grad_ds, double* grad_q, void* workspace, size_t ws_bytes,
+                     void* stream);
+int admp_pair_energy_workspace_bytes(int64_t nf, size_t* bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Smooth PME reciprocal space, order-6 B-splines, one frame per call.
+ * kmesh = (K1, K2, K3); the mesh is real (K1*K2*K3) of the precision type.  */
+
+int admp_pme_workspace_bytes(const int32_t* kmesh, int32_t precision, int64_t n,
+                             size_t* bytes);
+
+/* Charge spreading Q = sum_i q_i theta_i (src/torch_admp/recip.py:216-252).
+ * Overwrites mesh. */
+int admp_pme_spread(const double* pos, const double* q, int64_t n, const double* box,
+                    const int32_t* kmesh, int32_t precision, void* mesh, void* workspace,
+                    size_t ws_bytes, void* stream);
+
+/* Reciprocal energy and potential mesh (src/torch_admp/pme.py:410-443,
+ * 661-683; torch.fft.fftn at pme.py:434 replaced by an in-library r2c/c2r FFT).
+ * energy[0] = eps sum_k G(k) |S(k)|^2; dEdA (9, may be NULL) = dE/dA through
+ * the k vectors at fixed A-independent structure factor; phi (may be NULL) =
+ * dE/dQ on the mesh, scaled by phi_scale.  energy and dEdA are overwritten. */
+int admp_pme_convolve(const void* mesh, const double* box, const int32_t* kmesh,
+                      int32_t precision, int32_t convention, double kappa, double dielectric,
+                      double phi_scale, void* phi, double* energy, double* dEdA,
+                      void* workspace, size_t ws_bytes, void* stream);
+
+/* Back-interpolation, the transpose of admp_pme_spread (the autograd
+ * transpose of recip.py:94-99,206).  For a mesh field phi = dE/dQ:
+ *   grad_pos (n,3) = dE/dr,  dEdq (n) = dE/dq,  dEdA (9) = dE/dA via s = r A.
+ * Any output may be NULL.  accumulate != 0 adds into grad_pos / dEdq instead
+ * of overwriting; dEdA is always overwritten. */
+int admp_pme_gather(const double* pos, const double* q, int64_t n, const double* box,
+                    const int32_t* kmesh, int32_t precision, const void* phi,
+                    double* grad_pos, double* dEdq, double* dEdA, int32_t accumulate,
+                    void* workspace, size_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Fused Ewald/PME energy + forces for one periodic frame straight from
+ * positions: cell-list real space (no pair list materialised), spread, FFT,
+ * influence function, gather, self and non-neutral terms.  Equivalent to
+ * CoulombForceModule.forward on the TorchNeighborList pair list followed by
+ * calc_grads (src/torch_admp/pme.py:261-310, src/torch_admp/utils.py:84-112).
+ *   energy (5, device): real, reciprocal, self, non-neutral, total
+ *   forces (n, 3, device): -dE/dr (overwritten)
+ *   dEdbox (9, device, may be NULL): dE/dh at fixed Cartesian positions
+ * Workspace from admp_pme_fused_workspace_bytes. */
+int admp_pme_fused_workspace_bytes(int64_t n, const double* box, double cutoff,
+                                   const int32_t* kmesh, int32_t precision, size_t* bytes);
+int admp_pme_fused(const double* pos, const double* q, int64_t n, const double* box,
+                   double cutoff, double kappa, const int32_t* kmesh, int32_t precision,
+                   int32_t convention, double dielectric, double* energy, double* forces,
+                   double* dEdbox, void* workspace, size_t ws_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADMP_AMD_H */

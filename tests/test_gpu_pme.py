@@ -1,0 +1,238 @@
+"""GPU parity of the HIP path against the float64 CPU oracle.
+
+Tolerances: float64 kernels agree with the oracle to 1e-10 relative energy and
+1e-8 eV/A force RMSE (rounding only); float32-bulk kernels to 1e-6 relative
+energy and 1e-4 eV/A force RMSE.  Pair sets are compared exactly.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import nblist_ref, pme_ref
+from torch_admp_amd import ops
+from torch_admp_amd.nblist import TorchNeighborList
+from torch_admp_amd.pme import CoulombForceModule
+from torch_admp_amd.utils import calc_grads
+
+pytestmark = pytest.mark.gpu
+
+F64 = torch.float64
+
+
+def _system(n, box, seed, neutral=True):
+    rng = np.random.default_rng(seed)
+    box = torch.as_tensor(box, dtype=F64)
+    frac = torch.tensor(rng.random((n, 3)))
+    pos = frac @ box
+    q = torch.tensor(rng.uniform(-1, 1, n))
+    if neutral:
+        q -= q.mean()
+    return pos, q, box
+
+
+TRICLINIC = [[12.0, 0.0, 0.0], [2.5, 13.0, 0.0], [-1.5, 1.0, 14.0]]
+SYMMETRIC = [[12.0, 0.4, 0.3], [0.4, 13.0, 0.2], [0.3, 0.2, 14.0]]
+
+
+def _oracle_ef(pos, box, q, rc, kmesh, **kw):
+    pairs = nblist_ref.nblist_pbc(pos, box, rc)
+    bs = nblist_ref.buffer_scales(pairs)
+    orc = pme_ref.CoulombOracle(rc, 1e-5, kmesh=kmesh, **kw)
+    e, f, gb = pme_ref.energy_forces(
+        pos, box, q, pairs, lambda p, b: nblist_ref.pairs_ds(p, pairs, b), bs, orc)
+    return e, f, gb, orc
+
+
+@pytest.mark.parametrize("box", [np.eye(3) * 13.0, TRICLINIC, SYMMETRIC])
+def test_nblist_pair_set_exact(dev, box):
+    pos, _, box = _system(300, box, 1)
+    # positions partly outside the box exercise the wrapping
+    pos = pos + torch.tensor([[30.0, -25.0, 41.0]], dtype=F64) * (torch.arange(300) % 3 == 0)[:, None]
+    rc = 5.0
+    nbl = TorchNeighborList(rc)
+    pairs = nbl(pos.to(dev), box.to(dev)).cpu()
+    got = set(map(tuple, pairs.tolist()))
+    ref = set(map(tuple, nblist_ref.nblist_pbc(pos, box, rc).tolist()))
+    brute, amb = nblist_ref.brute_force_pairs(pos.numpy(), box.numpy(), rc)
+    assert not amb
+    assert got == ref == brute
+    assert len(got) == pairs.shape[0]  # no duplicates
+    # grouped by i, buffer scales i < j
+    assert torch.all(pairs[1:, 0] >= pairs[:-1, 0])
+    assert torch.equal(nbl.get_buffer_scales().cpu(), (pairs[:, 0] < pairs[:, 1]).long())
+
+
+def test_nblist_obc(dev):
+    pos, _, _ = _system(200, np.eye(3) * 10.0, 2)
+    rc = 3.0
+    pairs = TorchNeighborList(rc)(pos.to(dev), None).cpu()
+    ref = set(map(tuple, nblist_ref.nblist_obc(pos, rc).tolist()))
+    assert set(map(tuple, pairs.tolist())) == ref
+
+
+@pytest.mark.parametrize("box", [np.eye(3) * 13.0, TRICLINIC])
+def test_pair_ds_and_backward(dev, box):
+    pos, _, box = _system(150, box, 3)
+    pairs = nblist_ref.nblist_pbc(pos, box, 5.0)
+    p = pos.clone().requires_grad_(True)
+    b = box.clone().requires_grad_(True)
+    ds_ref = nblist_ref.pairs_ds(p, pairs, b)
+    w = torch.tensor(np.random.default_rng(4).normal(size=ds_ref.shape[0]))
+    gp_ref, gb_ref = torch.autograd.grad((ds_ref * w).sum(), [p, b])
+    pg = pos.to(dev).requires_grad_(True)
+    bg = box.to(dev).requires_grad_(True)
+    ds = ops.PairDistance.apply(pg, bg, pairs.to(dev))
+    gp, gb = torch.autograd.grad((ds * w.to(dev)).sum(), [pg, bg])
+    assert torch.allclose(ds.detach().cpu(), ds_ref.detach(), rtol=0, atol=1e-12)
+    assert torch.allclose(gp.cpu(), gp_ref, rtol=0, atol=1e-10)
+    assert torch.allclose(gb.cpu(), gb_ref, rtol=0, atol=1e-9)
+
+
+def test_real_space_pair_energy(dev):
+    pos, q, box = _system(300, np.eye(3) * 14.0, 5)
+    rc, kappa = 6.0, 0.4
+    pairs = nblist_ref.nblist_pbc(pos, box, rc)
+    bs = nblist_ref.buffer_scales(pairs).to(F64)
+    ds = nblist_ref.pairs_ds(pos, pairs, box)
+    qr = q[None].clone().requires_grad_(True)
+    dr = ds[None].clone().requires_grad_(True)
+    e_ref = pme_ref.real_energy(qr, pairs[None], dr, bs[None], kappa)
+    gq_ref, gd_ref = torch.autograd.grad(e_ref.sum(), [qr, dr])
+    qg = q[None].to(dev).requires_grad_(True)
+    dg = ds[None].to(dev).requires_grad_(True)
+    e = ops.PairEnergy.apply(qg, dg, pairs[None].to(dev), bs[None].to(dev), None,
+                             ops.PAIR_EWALD, kappa, pme_ref.DIELECTRIC)
+    gq, gd = torch.autograd.grad(e.sum(), [qg, dg])
+    assert abs(float(e) - float(e_ref)) <= 1e-11 * abs(float(e_ref))
+    assert torch.allclose(gq.cpu(), gq_ref, atol=1e-11, rtol=1e-10)
+    assert torch.allclose(gd.cpu(), gd_ref, atol=1e-11, rtol=1e-10)
+
+
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-12), ("float32", 2e-6)])
+@pytest.mark.parametrize("kmesh", [[16, 18, 20], [25, 12, 27], [74, 30, 22]])
+def test_spread_mesh(dev, precision, tol, kmesh):
+    import ctypes
+
+    from torch_admp_amd import _lib
+
+    pos, q, box = _system(120, TRICLINIC, 6)
+    pos = pos - 7.0  # negative fractional coordinates
+    ref = pme_ref.spread_charges(pos, torch.linalg.inv(box), q, kmesh)
+    lib = _lib.load()
+    dt = torch.float32 if precision == "float32" else F64
+    mesh = torch.empty(int(np.prod(kmesh)), dtype=dt, device=dev)
+    pd, qd = pos.to(dev).contiguous(), q.to(dev).contiguous()
+    _lib.check(lib.admp_pme_spread(_lib.ptr(pd), _lib.ptr(qd), pos.shape[0], _lib.box_array(box),
+                                   _lib.kmesh_array(kmesh), _lib.PRECISION[precision],
+                                   _lib.ptr(mesh), None, 0, _lib.stream_of(pd)), "spread")
+    got = mesh.double().cpu().reshape(kmesh)
+    assert (got - ref).abs().max() <= tol * ref.abs().max()
+    assert abs(float(got.sum()) - float(q.sum())) < 1e-9 + tol * len(q)
+
+
+@pytest.mark.parametrize("conv", ["reference", "columns"])
+@pytest.mark.parametrize("box,kmesh", [(np.eye(3) * 13.0, [24, 24, 24]),
+                                       (TRICLINIC, [30, 32, 36]),
+                                       (np.diag([11.0, 12.5, 13.0]), [25, 27, 33]),
+                                       (np.eye(3) * 14.0, [74, 37, 22])])
+def test_reciprocal_energy_grads_fp64(dev, box, kmesh, conv):
+    pos, q, box = _system(160, box, 8, neutral=False)
+    kappa = 0.45
+    p = pos.clone().requires_grad_(True)
+    b = box.clone().requires_grad_(True)
+    qq = q.clone().requires_grad_(True)
+    e_ref = pme_ref.reciprocal_energy_frame(p, b, qq, kappa, kmesh, conv)
+    gp_ref, gb_ref, gq_ref = torch.autograd.grad(e_ref, [p, b, qq])
+    pg = pos.to(dev).requires_grad_(True)
+    bg = box.to(dev).requires_grad_(True)
+    qg = q.to(dev).requires_grad_(True)
+    from torch_admp_amd import _lib
+
+    e = ops.PMEReciprocal.apply(pg, bg, qg, kmesh, kappa, pme_ref.DIELECTRIC, _lib.ADMP_F64,
+                                _lib.KCONV[conv])
+    gp, gb, gq = torch.autograd.grad(e, [pg, bg, qg])
+    assert abs(float(e) - float(e_ref)) <= 1e-10 * abs(float(e_ref))
+    assert float((gp.cpu() - gp_ref).abs().max()) <= 1e-9 * float(gp_ref.abs().max())
+    assert float((gq.cpu() - gq_ref).abs().max()) <= 1e-9 * float(gq_ref.abs().max())
+    assert float((gb.cpu() - gb_ref).abs().max()) <= 1e-8 * float(gb_ref.abs().max())
+
+
+@pytest.mark.parametrize("box,kmesh,conv", [(np.eye(3) * 13.0, 24, "reference"),
+                                            (TRICLINIC, [30, 32, 36], "reference"),
+                                            (TRICLINIC, [30, 32, 36], "columns")])
+def test_module_energy_forces(dev, box, kmesh, conv):
+    pos, q, box = _system(250, box, 9)
+    rc = 5.5
+    km = [kmesh] * 3 if isinstance(kmesh, int) else kmesh
+    e_ref, f_ref, gb_ref, orc = _oracle_ef(pos, box, q, rc, km, convention=conv)
+    p = pos.to(dev).requires_grad_(True)
+    b = box.to(dev).requires_grad_(True)
+    nbl = TorchNeighborList(rc)
+    pairs = nbl(p, b)
+    mod = CoulombForceModule(rc, 1e-5, kmesh=kmesh, convention=conv)
+    e = mod(p, b, pairs, nbl.get_ds(), nbl.get_buffer_scales(), {"charge": q.to(dev)})
+    gp, gb = torch.autograd.grad(e.sum(), [p, b])
+    assert abs(float(e) - float(e_ref)) <= 1e-10 * abs(float(e_ref))
+    assert float(((-gp.cpu() - f_ref) ** 2).mean().sqrt()) < 1e-8
+    assert float((gb.cpu() - gb_ref).abs().max()) <= 1e-8 * float(gb_ref.abs().max())
+    assert abs(float(mod.real_energy) - float(orc.real)) < 1e-9 * abs(float(orc.real))
+    assert abs(float(mod.reciprocal_energy) - float(orc.recip)) < 1e-9 * abs(float(orc.recip))
+
+
+@pytest.mark.parametrize("precision,etol,ftol", [("float64", 1e-10, 1e-8),
+                                                 ("float32", 1e-6, 1e-4)])
+@pytest.mark.parametrize("box,kmesh", [(np.eye(3) * 16.0, [32, 32, 32]),
+                                       (TRICLINIC, [30, 32, 36])])
+def test_fused_energy_forces(dev, precision, etol, ftol, box, kmesh):
+    pos, q, box = _system(400, box, 10)
+    rc = 5.5
+    e_ref, f_ref, gb_ref, orc = _oracle_ef(pos, box, q, rc, kmesh)
+    mod = CoulombForceModule(rc, 1e-5, kmesh=kmesh, precision=precision)
+    e, f, gb = mod.energy_forces(pos.to(dev), box.to(dev), q.to(dev))
+    assert abs(float(e) - float(e_ref)) <= etol * abs(float(e_ref))
+    assert float(((f.cpu() - f_ref) ** 2).mean().sqrt()) < ftol
+    assert float((gb.cpu() - gb_ref).abs().max()) <= 100 * etol * float(gb_ref.abs().max())
+
+
+def test_module_batched_frames(dev):
+    frames = [_system(120, np.eye(3) * 12.0, s) for s in (11, 12, 13)]
+    pos = torch.stack([f[0] for f in frames])
+    q = torch.stack([f[1] for f in frames])
+    box = torch.stack([f[2] for f in frames])
+    rc = 5.0
+    # same pair count per frame is required by the batched layout: pad with zero-scale rows
+    plist = [nblist_ref.nblist_pbc(pos[i], box[i], rc) for i in range(3)]
+    npmax = max(p.shape[0] for p in plist)
+    pairs = torch.zeros(3, npmax, 2, dtype=torch.long)
+    bs = torch.zeros(3, npmax, dtype=F64)
+    for i, p in enumerate(plist):
+        pairs[i, :p.shape[0]] = p
+        bs[i, :p.shape[0]] = nblist_ref.buffer_scales(p).to(F64)
+    ds = torch.stack([nblist_ref.pairs_ds(pos[i], pairs[i], box[i]) for i in range(3)])
+    orc = pme_ref.CoulombOracle(rc, 1e-5, kmesh=20)
+    e_ref = orc.energy(pos, box, pairs, ds, bs, q)
+    mod = CoulombForceModule(rc, 1e-5, kmesh=20)
+    e = mod(pos.to(dev), box.to(dev), pairs.to(dev), ds.to(dev), bs.to(dev), {"charge": q.to(dev)})
+    assert torch.allclose(e.cpu(), e_ref, rtol=1e-10, atol=0)
+    assert mod._kmesh.tolist() == [20, 20, 20]
+
+
+def test_kmesh_rule_and_obc(dev):
+    pos, q, box = _system(100, np.eye(3) * 10.0, 14)
+    rc = 4.0
+    nbl = TorchNeighborList(rc)
+    p = pos.to(dev).requires_grad_(True)
+    pairs = nbl(p, None)
+    mod = CoulombForceModule(rc + 1.0, 1e-5)
+    e = mod(p, None, pairs, nbl.get_ds(), nbl.get_buffer_scales(), {"charge": q.to(dev)})
+    f = -calc_grads(e, p)
+    pr = nblist_ref.nblist_obc(pos, rc)
+    pp = pos.clone().requires_grad_(True)
+    ds = nblist_ref.pairs_ds(pp, pr, None)
+    e_ref = pme_ref.obc_energy(q[None], pr[None], ds[None], nblist_ref.buffer_scales(pr)[None].to(F64))
+    f_ref = -torch.autograd.grad(e_ref.sum(), pp)[0]
+    assert abs(float(e) - float(e_ref)) < 1e-10 * abs(float(e_ref))
+    assert torch.allclose(f.cpu(), f_ref, atol=1e-9)

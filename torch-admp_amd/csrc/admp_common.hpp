@@ -1,0 +1,155 @@
+// Shared device helpers for the admp_amd HIP library (gfx950 / CDNA4).
+//
+// Conventions
+//   * box h is row-major with the lattice vectors as rows; A = h^-1.
+//   * fractional coordinates s = r A (row vector times matrix).
+//   * mesh index of grid point (i0, i1, i2) is (i0 * K1 + i1) * K2 + i2.
+//   * "real" is the bulk arithmetic type (float or double); every energy,
+//     virial and charge reduction is accumulated in double.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/admp_amd.h"
+
+namespace admp {
+
+constexpr int kOrder = 6;              // order-6 cardinal B-spline only
+constexpr int kBlock = 256;            // default work-group size (4 wavefronts)
+constexpr int kRedBlocks = 512;        // fixed grid for deterministic reductions
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kSqrtPi = 1.77245385090551602730;
+
+struct Geom {
+  double h[9];    // box rows
+  double a[9];    // inverse box
+  double vol;     // det(h)
+  int k[3];       // mesh size
+};
+
+template <typename T> struct cplx_t;
+template <> struct cplx_t<float> { using type = float2; };
+template <> struct cplx_t<double> { using type = double2; };
+template <typename T> using cplx = typename cplx_t<T>::type;
+
+template <typename C> __device__ __forceinline__ C cmk(double re, double im) {
+  C c; c.x = re; c.y = im; return c;
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { a.x += b.x; a.y += b.y; return a; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { a.x -= b.x; a.y -= b.y; return a; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  C c; c.x = a.x * b.x - a.y * b.y; c.y = a.x * b.y + a.y * b.x; return c;
+}
+
+__host__ __device__ inline int pmod(long long m, int k) {
+  long long r = m % k;
+  return (int)(r < 0 ? r + k : r);
+}
+
+// Order-6 cardinal B-spline on the uniform knots by the Cox-de Boor
+// recursion: w[j] = M6(f + j), dw[j] = M6'(f + j) for f in [0, 1), j = 0..5.
+// M6'(u) = M5(u) - M5(u - 1).
+template <typename T>
+__device__ __forceinline__ void bspline6(T f, T w[6], T dw[6]) {
+  T m[6];
+  m[0] = f;
+  m[1] = T(1) - f;
+#pragma unroll
+  for (int k = 3; k <= 5; ++k) {
+    const T inv = T(1) / T(k - 1);
+    T prev = T(0);
+#pragma unroll
+    for (int j = 0; j < k; ++j) {
+      const T cur = (j < k - 1) ? m[j] : T(0);
+      const T nv = ((f + T(j)) * cur + (T(k) - f - T(j)) * prev) * inv;
+      prev = cur;
+      m[j] = nv;
+    }
+  }
+  // m[0..4] = M5(f + j)
+  T prev = T(0);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const T cur = (j < 5) ? m[j] : T(0);
+    dw[j] = cur - prev;
+    w[j] = ((f + T(j)) * cur + (T(6) - f - T(j)) * prev) * T(0.2);
+    prev = cur;
+  }
+}
+
+// Reference point of an atom on the mesh: R = K (r A); m0 = ceil(R);
+// f = m0 - R in [0, 1).  Grid point m0 - 3 + j carries weight M6(f + j).
+__device__ __forceinline__ void mesh_ref(const Geom& g, double x, double y, double z,
+                                         int m0[3], double f[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double s = x * g.a[0 * 3 + d] + y * g.a[1 * 3 + d] + z * g.a[2 * 3 + d];
+    const double R = s * g.k[d];
+    const double c = ceil(R);
+    m0[d] = (int)c;
+    f[d] = c - R;
+  }
+}
+
+// ---- wavefront / block reductions (64-wide wavefronts) ---------------------
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum NV doubles over the block; result valid in thread 0.  `smem` holds
+// NV * (blockDim/64) doubles.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* smem) {
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) smem[wid * NV + i] = v[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < nw; ++w) {
+#pragma unroll
+      for (int i = 0; i < NV; ++i) v[i] += smem[w * NV + i];
+    }
+  }
+  __syncthreads();
+}
+
+// Deterministic reduction of per-block partials in a fixed order:
+// value 0 -> first[0], values 1.. -> rest[v - 1]; either pointer may be null.
+template <int UNUSED = 0>
+__global__ void __launch_bounds__(kBlock) k_reduce_partials_t(const double* __restrict__ p,
+                                                              int nblocks, int nv, double scale,
+                                                              double* __restrict__ first,
+                                                              double* __restrict__ rest) {
+  __shared__ double red[kBlock / 64];
+  for (int v = 0; v < nv; ++v) {
+    double s = 0.0;
+    for (int bi = threadIdx.x; bi < nblocks; bi += blockDim.x) s += p[(int64_t)bi * nv + v];
+    double arr[1] = {s};
+    block_sum<1>(arr, red);
+    if (threadIdx.x == 0) {
+      if (v == 0 && first) first[0] = scale * arr[0];
+      if (v > 0 && rest) rest[v - 1] = scale * arr[0];
+    }
+  }
+}
+#define k_reduce_partials k_reduce_partials_t<0>
+
+// Fill kernel (graph-safe replacement for memset).
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+}  // namespace admp

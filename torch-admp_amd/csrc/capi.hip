@@ -1,0 +1,388 @@
+// C ABI of the admp_amd library: argument validation, workspace carving and
+// kernel sequencing.  No entry point allocates device memory; only
+// admp_nblist_count synchronises its stream.
+#include <cmath>
+#include <cstring>
+
+#include "nblist_kernels.hpp"
+#include "pme_kernels.hpp"
+
+using namespace admp;
+
+namespace {
+
+inline int hip_status(hipError_t e) { return e == hipSuccess ? ADMP_OK : ADMP_ERR_HIP; }
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+bool box_ok(const double* box) {
+  if (!box) return false;
+  for (int i = 0; i < 9; ++i)
+    if (!std::isfinite(box[i])) return false;
+  const double* h = box;
+  const double det = h[0] * (h[4] * h[8] - h[5] * h[7]) - h[1] * (h[3] * h[8] - h[5] * h[6]) +
+                     h[2] * (h[3] * h[7] - h[4] * h[6]);
+  return det != 0.0 && std::isfinite(det);
+}
+
+bool kmesh_ok(const int32_t* k) {
+  if (!k) return false;
+  for (int d = 0; d < 3; ++d)
+    if (k[d] < 1 || k[d] > kMaxMesh) return false;
+  return true;
+}
+
+Geom make_geom(const double* box, const int32_t* kmesh) {
+  Geom g;
+  const CellGrid cg = make_grid(box, 1.0, false);
+  for (int i = 0; i < 9; ++i) {
+    g.h[i] = cg.h[i];
+    g.a[i] = cg.a[i];
+  }
+  const double* h = box;
+  g.vol = h[0] * (h[4] * h[8] - h[5] * h[7]) - h[1] * (h[3] * h[8] - h[5] * h[6]) +
+          h[2] * (h[3] * h[7] - h[4] * h[6]);
+  for (int d = 0; d < 3; ++d) g.k[d] = kmesh ? kmesh[d] : 1;
+  return g;
+}
+
+size_t elem_size(int32_t precision) { return precision == ADMP_F32 ? 4 : 8; }
+
+// PME workspace: spectrum (K1*K2*(K3/2+1) complex) + partials.
+struct PmeLayout {
+  size_t spec, partials, bytes;
+};
+
+PmeLayout pme_layout(const int32_t* k, int32_t precision) {
+  PmeLayout L;
+  const size_t nspec = (size_t)k[0] * k[1] * (k[2] / 2 + 1);
+  L.spec = 0;
+  size_t off = align256(nspec * 2 * elem_size(precision));
+  L.partials = off;
+  const size_t np = std::max(convolve_partials_count(k), (size_t)kRedBlocks * 9 + 16);
+  off += align256(np * sizeof(double));
+  L.bytes = off;
+  return L;
+}
+
+// Fused workspace: neighbour list + sorted (x, y, z, q) + gradient + mesh +
+// PME workspace + real-space partials + reduced scalars.
+struct FusedLayout {
+  NbLayout nb;
+  size_t ps, grad, mesh, pme, rpart, red, bytes;
+  PmeLayout pl;
+};
+
+FusedLayout fused_layout(int64_t n, const CellGrid& cg, const int32_t* k, int32_t precision) {
+  FusedLayout F;
+  nb_layout(n, cg, &F.nb);
+  size_t off = align256(F.nb.bytes);
+  F.ps = off;
+  off += align256((size_t)std::max<int64_t>(n, 1) * 4 * elem_size(precision));
+  F.grad = off;
+  off += align256((size_t)std::max<int64_t>(n, 1) * 3 * sizeof(double));
+  F.mesh = off;
+  off += align256((size_t)k[0] * k[1] * k[2] * elem_size(precision));
+  F.pl = pme_layout(k, precision);
+  F.pme = off;
+  off += align256(F.pl.bytes);
+  F.rpart = off;
+  off += align256((size_t)((n + kBlock - 1) / kBlock + 1) * kRealNV * sizeof(double));
+  F.red = off;
+  off += align256(64 * sizeof(double));
+  F.bytes = off;
+  return F;
+}
+
+// Combine the reduced partial sums into energies and dE/dh on the device.
+//   red[0..17]  real space: E, D(6), sum r (x) g (9), sum q^2, sum q
+//   red[18..27] reciprocal: E, dE/dA through k (9)
+//   red[28..36] gather: dE/dA through s = r A (9)
+__global__ void k_finalize(const double* __restrict__ red, Geom g, double kappa, double eps,
+                           double* __restrict__ energy, double* __restrict__ dEdbox) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double* rr = red;
+  const double* rk = red + 18;
+  const double* rg = red + 28;
+  const double e_real = rr[0];
+  const double e_recip = rk[0];
+  const double e_self = -eps * kappa / kSqrtPi * rr[16];
+  const double e_nn = -eps * kPi / (2.0 * g.vol * kappa * kappa) * rr[17] * rr[17];
+  energy[0] = e_real;
+  energy[1] = e_recip;
+  energy[2] = e_self;
+  energy[3] = e_nn;
+  energy[4] = e_real + e_recip + e_self + e_nn;
+  if (!dEdbox) return;
+  const double* A = g.a;
+  // real: G = A^T (D - R^T g)
+  double D[9] = {rr[1], rr[4], rr[5], rr[4], rr[2], rr[6], rr[5], rr[6], rr[3]};
+  double M[9];
+  for (int i = 0; i < 9; ++i) M[i] = D[i] - rr[7 + i];
+  double dA[9];
+  for (int i = 0; i < 9; ++i) dA[i] = rk[1 + i] + rg[i];
+  for (int c = 0; c < 3; ++c)
+    for (int b = 0; b < 3; ++b) {
+      double gr = 0.0, gk = 0.0;
+      for (int a = 0; a < 3; ++a) gr += A[a * 3 + c] * M[a * 3 + b];
+      // reciprocal: -(A^T dA A^T)[c][b] = -sum_{a,e} A[a][c] dA[a][e] A[b][e]
+      for (int a = 0; a < 3; ++a)
+        for (int e = 0; e < 3; ++e) gk -= A[a * 3 + c] * dA[a * 3 + e] * A[b * 3 + e];
+      // volume terms: -E A^T for the reciprocal and non-neutral energies
+      const double at = A[b * 3 + c];
+      dEdbox[c * 3 + b] = gr + gk - (e_recip + e_nn) * at;
+    }
+}
+
+__global__ void k_negate(const double* __restrict__ g, int64_t m, double* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = -g[i];
+}
+
+template <typename T>
+static hipError_t fused_impl(const double* pos, const double* q, int64_t n, const double* box,
+                             double cutoff, double kappa, const int32_t* kmesh, int32_t conv,
+                             double eps, double* energy, double* forces, double* dEdbox,
+                             char* ws, const FusedLayout& F, const CellGrid& cg, hipStream_t s) {
+  const Geom g = make_geom(box, kmesh);
+  double* grad = (double*)(ws + F.grad);
+  double* red = (double*)(ws + F.red);
+  T* mesh = (T*)(ws + F.mesh);
+  char* pws = ws + F.pme;
+  double* ppart = (double*)(pws + F.pl.partials);
+  hipError_t e = launch_bin(pos, n, cg, ws, F.nb, s);
+  if (e != hipSuccess) return e;
+  e = launch_real_fused<T>(pos, q, n, cg, kappa, eps, ws, F.nb, (P4<T>*)(ws + F.ps), grad,
+                           (double*)(ws + F.rpart), red, s);
+  if (e != hipSuccess) return e;
+  e = launch_spread<T>(pos, q, n, g, mesh, s);
+  if (e != hipSuccess) return e;
+  e = launch_convolve<T>(mesh, g, conv, kappa, eps, 1.0, mesh, red + 18, red + 19,
+                         (cplx<T>*)(pws + F.pl.spec), ppart, s);
+  if (e != hipSuccess) return e;
+  e = launch_gather<T>(pos, q, n, g, mesh, grad, nullptr, red + 28, 1, ppart, s);
+  if (e != hipSuccess) return e;
+  k_finalize<<<1, 64, 0, s>>>(red, g, kappa, eps, energy, dEdbox);
+  if (n > 0)
+    k_negate<<<(unsigned)std::min<int64_t>((3 * n + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(
+        grad, 3 * n, forces);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+int admp_version(void) { return 100; }
+
+const char* admp_status_string(int status) {
+  switch (status) {
+    case ADMP_OK: return "ok";
+    case ADMP_OVERFLOW: return "output capacity too small";
+    case ADMP_ERR_INVALID: return "invalid argument";
+    case ADMP_ERR_HIP: return "HIP runtime error";
+    case ADMP_ERR_UNSUPPORTED: return "unsupported configuration";
+    default: return "unknown status";
+  }
+}
+
+// ------------------------------------------------------------------ nblist
+int admp_nblist_workspace_bytes(int64_t n, const double* box, double cutoff, size_t* bytes) {
+  if (n < 0 || !(cutoff > 0.0) || !bytes) return ADMP_ERR_INVALID;
+  if (box && !box_ok(box)) return ADMP_ERR_INVALID;
+  NbLayout L;
+  nb_layout(n, make_grid(box, cutoff, box == nullptr), &L);
+  *bytes = L.bytes;
+  return ADMP_OK;
+}
+
+int admp_nblist_count(const double* pos, int64_t n, const double* box, double cutoff,
+                      void* workspace, size_t ws_bytes, int64_t* n_pairs, void* stream) {
+  if (n < 0 || !(cutoff > 0.0) || !n_pairs || (n > 0 && (!pos || !workspace)))
+    return ADMP_ERR_INVALID;
+  if (n > (int64_t)1 << 30) return ADMP_ERR_UNSUPPORTED;
+  if (box && !box_ok(box)) return ADMP_ERR_INVALID;
+  const CellGrid cg = make_grid(box, cutoff, box == nullptr);
+  NbLayout L;
+  nb_layout(n, cg, &L);
+  if (ws_bytes < L.bytes) return ADMP_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  hipError_t e = launch_bin(pos, n, cg, ws, L, s);
+  if (e != hipSuccess) return ADMP_ERR_HIP;
+  e = launch_pairs(0, n, cg, ws, L, nullptr, 0, s);
+  if (e != hipSuccess) return ADMP_ERR_HIP;
+  int64_t total = 0;
+  if (n > 0) {
+    e = hipMemcpyAsync(&total, ws + L.offsets + sizeof(int64_t) * n, sizeof(int64_t),
+                       hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return ADMP_ERR_HIP;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return ADMP_ERR_HIP;
+  }
+  *n_pairs = total;
+  return ADMP_OK;
+}
+
+int admp_nblist_fill(const double* pos, int64_t n, const double* box, double cutoff,
+                     void* workspace, size_t ws_bytes, int64_t* pairs, int64_t capacity,
+                     void* stream) {
+  if (n < 0 || !(cutoff > 0.0) || capacity < 0 || (n > 0 && (!pos || !workspace)))
+    return ADMP_ERR_INVALID;
+  if (capacity > 0 && !pairs) return ADMP_ERR_INVALID;
+  if (box && !box_ok(box)) return ADMP_ERR_INVALID;
+  const CellGrid cg = make_grid(box, cutoff, box == nullptr);
+  NbLayout L;
+  nb_layout(n, cg, &L);
+  if (ws_bytes < L.bytes) return ADMP_ERR_INVALID;
+  (void)pos;
+  return hip_status(launch_pairs(1, n, cg, (char*)workspace, L, pairs, capacity,
+                                 (hipStream_t)stream));
+}
+
+int admp_pair_ds(const double* pos, const int64_t* pairs, int64_t n_pairs, const double* box,
+                 double* ds, double* dr, void* stream) {
+  if (n_pairs < 0 || (n_pairs > 0 && (!pos || !pairs || !ds))) return ADMP_ERR_INVALID;
+  if (box && !box_ok(box)) return ADMP_ERR_INVALID;
+  const CellGrid cg = make_grid(box, 1.0, box == nullptr);
+  return hip_status(launch_pair_ds(pos, pairs, n_pairs, cg, ds, dr, (hipStream_t)stream));
+}
+
+int admp_pair_ds_backward(const double* grad_ds, const int64_t* pairs, int64_t n_pairs,
+                          const double* pos, const double* box, const double* dr,
+                          const double* ds, double* grad_pos, double* grad_box, void* stream) {
+  if (n_pairs < 0 || (n_pairs > 0 && (!grad_ds || !pairs || !pos || !dr || !ds || !grad_pos)))
+    return ADMP_ERR_INVALID;
+  if (box && !box_ok(box)) return ADMP_ERR_INVALID;
+  const CellGrid cg = make_grid(box, 1.0, box == nullptr);
+  return hip_status(launch_pair_ds_bwd(grad_ds, pairs, n_pairs, pos, cg, dr, ds, grad_pos,
+                                       grad_box, (hipStream_t)stream));
+}
+
+// ------------------------------------------------------------- pair energy
+int admp_pair_energy_workspace_bytes(int64_t nf, size_t* bytes) {
+  if (nf < 0 || !bytes) return ADMP_ERR_INVALID;
+  *bytes = align256((size_t)std::max<int64_t>(nf, 1) * kPairBlocksPerFrame * sizeof(double));
+  return ADMP_OK;
+}
+
+int admp_pair_energy(int32_t mode, int64_t nf, int64_t n, int64_t n_pairs, const double* q,
+                     const double* eta, const int64_t* pairs, const double* ds,
+                     const double* bs, double kappa, double dielectric, double* energy,
+                     double* grad_ds, double* grad_q, void* workspace, size_t ws_bytes,
+                     void* stream) {
+  if (mode < 0 || mode > 2 || nf < 1 || nf > 65535 || n < 0 || n_pairs < 0 || !energy ||
+      !workspace)
+    return ADMP_ERR_INVALID;
+  if (n_pairs > 0 && (!q || !pairs || !ds || !bs)) return ADMP_ERR_INVALID;
+  if (mode == 2 && n_pairs > 0 && !eta) return ADMP_ERR_INVALID;
+  size_t need;
+  admp_pair_energy_workspace_bytes(nf, &need);
+  if (ws_bytes < need) return ADMP_ERR_INVALID;
+  return hip_status(launch_pair_energy(mode, nf, n, n_pairs, q, eta, pairs, ds, bs, kappa,
+                                       dielectric, energy, grad_ds, grad_q, (double*)workspace,
+                                       (hipStream_t)stream));
+}
+
+// --------------------------------------------------------------------- PME
+int admp_pme_workspace_bytes(const int32_t* kmesh, int32_t precision, int64_t n, size_t* bytes) {
+  if (!kmesh_ok(kmesh) || (precision != ADMP_F32 && precision != ADMP_F64) || n < 0 || !bytes)
+    return ADMP_ERR_INVALID;
+  *bytes = pme_layout(kmesh, precision).bytes;
+  return ADMP_OK;
+}
+
+int admp_pme_spread(const double* pos, const double* q, int64_t n, const double* box,
+                    const int32_t* kmesh, int32_t precision, void* mesh, void* workspace,
+                    size_t ws_bytes, void* stream) {
+  (void)workspace;
+  (void)ws_bytes;
+  if (!kmesh_ok(kmesh) || !box_ok(box) || n < 0 || !mesh || (n > 0 && (!pos || !q)))
+    return ADMP_ERR_INVALID;
+  const Geom g = make_geom(box, kmesh);
+  hipStream_t s = (hipStream_t)stream;
+  if (precision == ADMP_F32) return hip_status(launch_spread<float>(pos, q, n, g, (float*)mesh, s));
+  if (precision == ADMP_F64) return hip_status(launch_spread<double>(pos, q, n, g, (double*)mesh, s));
+  return ADMP_ERR_INVALID;
+}
+
+int admp_pme_convolve(const void* mesh, const double* box, const int32_t* kmesh,
+                      int32_t precision, int32_t convention, double kappa, double dielectric,
+                      double phi_scale, void* phi, double* energy, double* dEdA,
+                      void* workspace, size_t ws_bytes, void* stream) {
+  if (!kmesh_ok(kmesh) || !box_ok(box) || !mesh || !(kappa > 0.0) || !workspace)
+    return ADMP_ERR_INVALID;
+  if (convention != ADMP_KCONV_REFERENCE && convention != ADMP_KCONV_COLUMNS)
+    return ADMP_ERR_INVALID;
+  if (precision != ADMP_F32 && precision != ADMP_F64) return ADMP_ERR_INVALID;
+  const PmeLayout L = pme_layout(kmesh, precision);
+  if (ws_bytes < L.bytes) return ADMP_ERR_INVALID;
+  const Geom g = make_geom(box, kmesh);
+  char* ws = (char*)workspace;
+  hipStream_t s = (hipStream_t)stream;
+  if (precision == ADMP_F32)
+    return hip_status(launch_convolve<float>((const float*)mesh, g, convention, kappa, dielectric,
+                                             phi_scale, (float*)phi, energy, dEdA,
+                                             (float2*)(ws + L.spec), (double*)(ws + L.partials),
+                                             s));
+  return hip_status(launch_convolve<double>((const double*)mesh, g, convention, kappa, dielectric,
+                                            phi_scale, (double*)phi, energy, dEdA,
+                                            (double2*)(ws + L.spec), (double*)(ws + L.partials),
+                                            s));
+}
+
+int admp_pme_gather(const double* pos, const double* q, int64_t n, const double* box,
+                    const int32_t* kmesh, int32_t precision, const void* phi,
+                    double* grad_pos, double* dEdq, double* dEdA, int32_t accumulate,
+                    void* workspace, size_t ws_bytes, void* stream) {
+  if (!kmesh_ok(kmesh) || !box_ok(box) || n < 0 || !phi || (n > 0 && (!pos || !q)))
+    return ADMP_ERR_INVALID;
+  if (precision != ADMP_F32 && precision != ADMP_F64) return ADMP_ERR_INVALID;
+  const PmeLayout L = pme_layout(kmesh, precision);
+  if (dEdA && (!workspace || ws_bytes < L.bytes)) return ADMP_ERR_INVALID;
+  const Geom g = make_geom(box, kmesh);
+  double* partials = workspace ? (double*)((char*)workspace + L.partials) : nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  if (precision == ADMP_F32)
+    return hip_status(launch_gather<float>(pos, q, n, g, (const float*)phi, grad_pos, dEdq, dEdA,
+                                           accumulate, partials, s));
+  return hip_status(launch_gather<double>(pos, q, n, g, (const double*)phi, grad_pos, dEdq, dEdA,
+                                          accumulate, partials, s));
+}
+
+// ------------------------------------------------------------------- fused
+int admp_pme_fused_workspace_bytes(int64_t n, const double* box, double cutoff,
+                                   const int32_t* kmesh, int32_t precision, size_t* bytes) {
+  if (n < 0 || !box_ok(box) || !(cutoff > 0.0) || !kmesh_ok(kmesh) || !bytes)
+    return ADMP_ERR_INVALID;
+  if (precision != ADMP_F32 && precision != ADMP_F64) return ADMP_ERR_INVALID;
+  *bytes = fused_layout(n, make_grid(box, cutoff, false), kmesh, precision).bytes;
+  return ADMP_OK;
+}
+
+int admp_pme_fused(const double* pos, const double* q, int64_t n, const double* box,
+                   double cutoff, double kappa, const int32_t* kmesh, int32_t precision,
+                   int32_t convention, double dielectric, double* energy, double* forces,
+                   double* dEdbox, void* workspace, size_t ws_bytes, void* stream) {
+  if (n < 0 || !box_ok(box) || !(cutoff > 0.0) || !(kappa > 0.0) || !kmesh_ok(kmesh) ||
+      !energy || !workspace || (n > 0 && (!pos || !q || !forces)))
+    return ADMP_ERR_INVALID;
+  if (convention != ADMP_KCONV_REFERENCE && convention != ADMP_KCONV_COLUMNS)
+    return ADMP_ERR_INVALID;
+  if (n > (int64_t)1 << 30) return ADMP_ERR_UNSUPPORTED;
+  const CellGrid cg = make_grid(box, cutoff, false);
+  if (precision != ADMP_F32 && precision != ADMP_F64) return ADMP_ERR_INVALID;
+  const FusedLayout F = fused_layout(n, cg, kmesh, precision);
+  if (ws_bytes < F.bytes) return ADMP_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  if (precision == ADMP_F32)
+    return hip_status(fused_impl<float>(pos, q, n, box, cutoff, kappa, kmesh, convention,
+                                        dielectric, energy, forces, dEdbox, (char*)workspace, F,
+                                        cg, s));
+  return hip_status(fused_impl<double>(pos, q, n, box, cutoff, kappa, kmesh, convention,
+                                       dielectric, energy, forces, dEdbox, (char*)workspace, F,
+                                       cg, s));
+}
+
+}  // extern "C"

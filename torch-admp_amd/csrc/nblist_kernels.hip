@@ -1,0 +1,662 @@
+// Cell-list neighbour search, pair distances, pair energies and the fused
+// cell-list real-space Ewald sum for gfx950.
+//
+// Replaces TorchNeighborList (src/torch_admp/nblist.py:304-551,
+// src/torch_admp/spatial.py:15-36) and the P-sized torch chains of
+// src/torch_admp/pme.py:312-349, 529-561 and src/torch_admp/qeq.py:106-121.
+#include "nblist_kernels.hpp"
+
+namespace admp {
+
+// ---------------------------------------------------------------------------
+// Cell grid in fractional space.  Along dimension d the grid has nc[d] cells;
+// with nc >= 3 the stencil is the 3 neighbouring cells (periodic wrap with an
+// image shift), otherwise the whole box is one cell and the stencil runs over
+// the image shifts -nimg..nimg.  Cells are at least rc + 1e-6 h wide, so no
+// pair within rc is missed.
+CellGrid make_grid(const double* box, double cutoff, bool obc) {
+  CellGrid cg;
+  for (int i = 0; i < 9; ++i) cg.h[i] = box ? box[i] : (i % 4 == 0 ? 1.0 : 0.0);
+  cg.obc = obc ? 1 : 0;
+  cg.rc2 = cutoff * cutoff;
+  if (obc) {
+    for (int d = 0; d < 3; ++d) {
+      cg.nc[d] = 1;
+      cg.nimg[d] = 0;
+    }
+    for (int i = 0; i < 9; ++i) cg.a[i] = (i % 4 == 0 ? 1.0 : 0.0);
+    cg.ncell = 1;
+    return cg;
+  }
+  const double* h = box;
+  const double c0[3] = {h[4] * h[8] - h[5] * h[7], h[5] * h[6] - h[3] * h[8], h[3] * h[7] - h[4] * h[6]};
+  const double det = h[0] * c0[0] + h[1] * c0[1] + h[2] * c0[2];
+  // inverse (A = h^-1) by cofactors
+  cg.a[0] = c0[0] / det;
+  cg.a[3] = c0[1] / det;
+  cg.a[6] = c0[2] / det;
+  cg.a[1] = (h[2] * h[7] - h[1] * h[8]) / det;
+  cg.a[4] = (h[0] * h[8] - h[2] * h[6]) / det;
+  cg.a[7] = (h[1] * h[6] - h[0] * h[7]) / det;
+  cg.a[2] = (h[1] * h[5] - h[2] * h[4]) / det;
+  cg.a[5] = (h[2] * h[3] - h[0] * h[5]) / det;
+  cg.a[8] = (h[0] * h[4] - h[1] * h[3]) / det;
+  const double vol = fabs(det);
+  int64_t ncell = 1;
+  for (int d = 0; d < 3; ++d) {
+    const int e = (d + 1) % 3, f = (d + 2) % 3;
+    const double cx = h[e * 3 + 1] * h[f * 3 + 2] - h[e * 3 + 2] * h[f * 3 + 1];
+    const double cy = h[e * 3 + 2] * h[f * 3 + 0] - h[e * 3 + 0] * h[f * 3 + 2];
+    const double cz = h[e * 3 + 0] * h[f * 3 + 1] - h[e * 3 + 1] * h[f * 3 + 0];
+    const double height = vol / sqrt(cx * cx + cy * cy + cz * cz);
+    int nc = (int)floor(height / (cutoff + 1e-6 * height));
+    if (nc > 1024) nc = 1024;
+    if (nc < 3) {
+      cg.nc[d] = 1;
+      cg.nimg[d] = (int)ceil(cutoff / height);
+    } else {
+      cg.nc[d] = nc;
+      cg.nimg[d] = 1;
+    }
+    ncell *= cg.nc[d];
+  }
+  cg.ncell = (int)ncell;
+  return cg;
+}
+
+// Stencil entries along one dimension: cell index and image shift.
+__device__ __forceinline__ int stencil_len(const CellGrid& cg, int d) {
+  return cg.nc[d] >= 3 ? 3 : 2 * cg.nimg[d] + 1;
+}
+__device__ __forceinline__ void stencil_at(const CellGrid& cg, int d, int ci, int o, int* cell,
+                                           int* shift) {
+  if (cg.nc[d] >= 3) {
+    int c = ci + o - 1;
+    int s = 0;
+    if (c < 0) { c += cg.nc[d]; s = -1; }
+    else if (c >= cg.nc[d]) { c -= cg.nc[d]; s = 1; }
+    *cell = c;
+    *shift = s;
+  } else {
+    *cell = 0;
+    *shift = o - cg.nimg[d];
+  }
+}
+
+// Bin: wrapped fractional s = (r A + 1e-7) mod 1 - 1e-7 and wrapped
+// position w = s h (src/torch_admp/nblist.py:447-470); cell of s.
+__global__ void __launch_bounds__(kBlock) k_nb_bin(const double* __restrict__ pos, int64_t n,
+                                                   CellGrid cg, double* __restrict__ w,
+                                                   int* __restrict__ cell_of,
+                                                   int* __restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  int cid = 0;
+  if (cg.obc) {
+    for (int d = 0; d < 3; ++d) w[3 * i + d] = r[d];
+  } else {
+    double s[3];
+    int c[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double v = r[0] * cg.a[d] + r[1] * cg.a[3 + d] + r[2] * cg.a[6 + d];
+      v = v + 1e-7;
+      v = v - floor(v);
+      s[d] = v - 1e-7;
+      int cc = (int)floor(s[d] * cg.nc[d]);
+      c[d] = cc < 0 ? 0 : (cc >= cg.nc[d] ? cg.nc[d] - 1 : cc);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) w[3 * i + j] = s[0] * cg.h[j] + s[1] * cg.h[3 + j] + s[2] * cg.h[6 + j];
+    cid = (c[0] * cg.nc[1] + c[1]) * cg.nc[2] + c[2];
+  }
+  cell_of[i] = cid;
+  atomicAdd(&count[cid], 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_nb_scatter(int64_t n, const int* __restrict__ cell_of,
+                                                       const int* __restrict__ start,
+                                                       int* __restrict__ cursor,
+                                                       int* __restrict__ sorted) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell_of[i];
+  sorted[start[c] + atomicAdd(&cursor[c], 1)] = (int)i;
+}
+
+// Order the atoms of every cell by index so traversal order is deterministic.
+__global__ void __launch_bounds__(kBlock) k_nb_sort_cells(int ncell, const int* __restrict__ start,
+                                                          int* __restrict__ sorted) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int b = start[c], e = start[c + 1];
+  for (int x = b + 1; x < e; ++x) {
+    const int v = sorted[x];
+    int y = x - 1;
+    while (y >= b && sorted[y] > v) {
+      sorted[y + 1] = sorted[y];
+      --y;
+    }
+    sorted[y + 1] = v;
+  }
+}
+
+// Count (MODE 0) or write (MODE 1) the ordered pairs of atom i:
+// PBC keeps 0.01 < |w_j + n h - w_i| < rc (src/torch_admp/nblist.py:433-445);
+// OBC keeps |r_j - r_i| < rc, j != i (nblist.py:472-490).
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) k_nb_pairs(int64_t n, CellGrid cg,
+                                                     const double* __restrict__ w,
+                                                     const int* __restrict__ cell_of,
+                                                     const int* __restrict__ start,
+                                                     const int* __restrict__ sorted,
+                                                     int64_t* __restrict__ counts,
+                                                     const int64_t* __restrict__ offsets,
+                                                     int64_t* __restrict__ pairs,
+                                                     int64_t capacity) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = sorted[t];
+  const double wi[3] = {w[3 * i], w[3 * i + 1], w[3 * i + 2]};
+  const int cid = cell_of[i];
+  const int ci[3] = {cid / (cg.nc[1] * cg.nc[2]), (cid / cg.nc[2]) % cg.nc[1], cid % cg.nc[2]};
+  const double rmin2 = 0.01 * 0.01;
+  const double rc = sqrt(cg.rc2);
+  int64_t cnt = 0;
+  int64_t out = MODE == 1 ? offsets[i] : 0;
+  const int l0 = stencil_len(cg, 0), l1 = stencil_len(cg, 1), l2 = stencil_len(cg, 2);
+  for (int o0 = 0; o0 < l0; ++o0) {
+    int c0, s0;
+    stencil_at(cg, 0, ci[0], o0, &c0, &s0);
+    for (int o1 = 0; o1 < l1; ++o1) {
+      int c1, s1;
+      stencil_at(cg, 1, ci[1], o1, &c1, &s1);
+      for (int o2 = 0; o2 < l2; ++o2) {
+        int c2, s2;
+        stencil_at(cg, 2, ci[2], o2, &c2, &s2);
+        double sh[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) sh[j] = s0 * cg.h[j] + s1 * cg.h[3 + j] + s2 * cg.h[6 + j];
+        const int cc = (c0 * cg.nc[1] + c1) * cg.nc[2] + c2;
+        for (int x = start[cc]; x < start[cc + 1]; ++x) {
+          const int j = sorted[x];
+          const double dx = (w[3 * j] + sh[0]) - wi[0];
+          const double dy = (w[3 * j + 1] + sh[1]) - wi[1];
+          const double dz = (w[3 * j + 2] + sh[2]) - wi[2];
+          const double d = sqrt(dx * dx + dy * dy + dz * dz);
+          bool keep;
+          if (cg.obc) keep = (j != i) && d < rc;
+          else keep = d < rc && d * d > rmin2 && d > 0.01;
+          if (keep) {
+            if (MODE == 1 && out + cnt < capacity) {
+              pairs[2 * (out + cnt)] = i;
+              pairs[2 * (out + cnt) + 1] = j;
+            }
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+  if (MODE == 0) counts[i] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan (3 kernels): block sums of 2048 items, single-block scan of
+// the block sums, then the per-block scan with its offset.
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+template <typename I>
+__device__ I block_exclusive_scan(I v, I* lds, I* total) {
+  lds[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    const I add = threadIdx.x >= off ? lds[threadIdx.x - off] : I(0);
+    __syncthreads();
+    lds[threadIdx.x] += add;
+    __syncthreads();
+  }
+  const I incl = lds[threadIdx.x];
+  if (total) *total = lds[kBlock - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+template <typename I>
+__global__ void __launch_bounds__(kBlock) k_scan_sums(const I* __restrict__ in, int64_t n,
+                                                      I* __restrict__ sums) {
+  __shared__ I lds[kBlock];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  I s = 0;
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < n) s += in[base + k];
+  I tot;
+  block_exclusive_scan<I>(s, lds, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+template <typename I>
+__global__ void __launch_bounds__(kBlock) k_scan_top(I* __restrict__ sums, int nb,
+                                                     I* __restrict__ total) {
+  __shared__ I lds[kBlock];
+  I carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += kBlock) {
+    const int i = b0 + threadIdx.x;
+    const I v = i < nb ? sums[i] : I(0);
+    I tot;
+    const I ex = block_exclusive_scan<I>(v, lds, &tot);
+    if (i < nb) sums[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+template <typename I>
+__global__ void __launch_bounds__(kBlock) k_scan_apply(const I* __restrict__ in, int64_t n,
+                                                       const I* __restrict__ sums,
+                                                       I* __restrict__ out) {
+  __shared__ I lds[kBlock];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  I vals[kScanItems];
+  I s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    vals[k] = base + k < n ? in[base + k] : I(0);
+    s += vals[k];
+  }
+  I run = block_exclusive_scan<I>(s, lds, (I*)nullptr) + sums[blockIdx.x];
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += vals[k];
+  }
+}
+
+template <typename I>
+size_t scan_scratch_count(int64_t n) {
+  return (size_t)((n + kScanTile - 1) / kScanTile) + 1;
+}
+
+// out[0..n] = exclusive scan of in[0..n-1] with out[n] = total (n + 1 entries).
+template <typename I>
+void launch_scan(const I* in, int64_t n, I* out, I* scratch, hipStream_t s) {
+  const int nb = (int)((n + kScanTile - 1) / kScanTile);
+  if (nb == 0) {
+    k_fill<I><<<1, 64, 0, s>>>(out, 1, I(0));
+    return;
+  }
+  k_scan_sums<I><<<nb, kBlock, 0, s>>>(in, n, scratch);
+  k_scan_top<I><<<1, kBlock, 0, s>>>(scratch, nb, out + n);
+  k_scan_apply<I><<<nb, kBlock, 0, s>>>(in, n, scratch, out);
+}
+
+// ---------------------------------------------------------------------------
+// Pair distances: dr = pbc_shift(r_i - r_j) (src/torch_admp/spatial.py:15-36)
+__global__ void __launch_bounds__(kBlock) k_pair_ds(const double* __restrict__ pos,
+                                                    const int64_t* __restrict__ pairs,
+                                                    int64_t np, CellGrid cg,
+                                                    double* __restrict__ ds,
+                                                    double* __restrict__ dr) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int64_t i = pairs[2 * p], j = pairs[2 * p + 1];
+  double v[3];
+  if (cg.obc) {
+    for (int d = 0; d < 3; ++d) v[d] = pos[3 * j + d] - pos[3 * i + d];
+  } else {
+    double x[3];
+    for (int d = 0; d < 3; ++d) x[d] = pos[3 * i + d] - pos[3 * j + d];
+    double s[3];
+    for (int d = 0; d < 3; ++d) {
+      s[d] = x[0] * cg.a[d] + x[1] * cg.a[3 + d] + x[2] * cg.a[6 + d];
+      s[d] -= floor(s[d] + 0.5);
+    }
+    for (int d = 0; d < 3; ++d) v[d] = s[0] * cg.h[d] + s[1] * cg.h[3 + d] + s[2] * cg.h[6 + d];
+  }
+  ds[p] = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (dr)
+    for (int d = 0; d < 3; ++d) dr[3 * p + d] = v[d];
+}
+
+__global__ void __launch_bounds__(kBlock) k_pair_ds_bwd(const double* __restrict__ gds,
+                                                        const int64_t* __restrict__ pairs,
+                                                        int64_t np, const double* __restrict__ pos,
+                                                        CellGrid cg, const double* __restrict__ dr,
+                                                        const double* __restrict__ ds,
+                                                        double* __restrict__ gpos,
+                                                        double* __restrict__ gbox) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const double d = ds[p];
+  const double g = gds[p];
+  if (d <= 0.0 || g == 0.0) return;
+  const int64_t i = pairs[2 * p], j = pairs[2 * p + 1];
+  const double u[3] = {dr[3 * p] / d, dr[3 * p + 1] / d, dr[3 * p + 2] / d};
+  // PBC: dr = r_i - r_j - n h; OBC: dr = r_j - r_i
+  const double si = cg.obc ? -1.0 : 1.0;
+  for (int k = 0; k < 3; ++k) {
+    unsafeAtomicAdd(&gpos[3 * i + k], si * g * u[k]);
+    unsafeAtomicAdd(&gpos[3 * j + k], -si * g * u[k]);
+  }
+  if (gbox && !cg.obc) {
+    double x[3], nimg[3];
+    for (int k = 0; k < 3; ++k) x[k] = pos[3 * i + k] - pos[3 * j + k];
+    for (int k = 0; k < 3; ++k) {
+      const double s = x[0] * cg.a[k] + x[1] * cg.a[3 + k] + x[2] * cg.a[6 + k];
+      nimg[k] = floor(s + 0.5);
+    }
+    for (int c = 0; c < 3; ++c)
+      if (nimg[c] != 0.0)
+        for (int b = 0; b < 3; ++b) unsafeAtomicAdd(&gbox[c * 3 + b], -g * nimg[c] * u[b]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pair energies over a caller-supplied list; grid (blocks per frame, nf).
+__global__ void __launch_bounds__(kBlock) k_pair_energy(int mode, int64_t n, int64_t np,
+                                                        const double* __restrict__ q,
+                                                        const double* __restrict__ eta,
+                                                        const int64_t* __restrict__ pairs,
+                                                        const double* __restrict__ ds,
+                                                        const double* __restrict__ bs,
+                                                        double kappa, double eps,
+                                                        double* __restrict__ partials,
+                                                        double* __restrict__ gds,
+                                                        double* __restrict__ gq) {
+  __shared__ double red[kBlock / 64];
+  const int64_t f = blockIdx.y;
+  const double* qf = q + f * n;
+  const double* etaf = eta ? eta + f * n : nullptr;
+  const int64_t* pf = pairs + f * np * 2;
+  double e_acc = 0.0;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = pf[2 * p], j = pf[2 * p + 1];
+    const double d = ds[f * np + p];
+    const double b = bs[f * np + p];
+    const double qi = qf[i], qj = qf[j];
+    const double thr = mode == 1 ? 1e-8 : 1e-4;
+    const bool ok = fabs(d) > thr;
+    const double inv = ok ? 1.0 / d : 0.0;
+    const double dinv = ok ? -inv * inv : 0.0;
+    double s, ds_s;  // screening function and its derivative in d
+    if (mode == 0) {
+      s = erfc(kappa * d);
+      ds_s = -2.0 / kSqrtPi * kappa * exp(-kappa * kappa * d * d);
+    } else if (mode == 1) {
+      s = 1.0;
+      ds_s = 0.0;
+    } else {
+      const double eij = sqrt((etaf[i] * etaf[i] + etaf[j] * etaf[j]) * 0.5);
+      const double x = d / (2.0 * eij);
+      s = -erfc(x);
+      ds_s = 2.0 / kSqrtPi * exp(-x * x) / (2.0 * eij);
+    }
+    const double pre = eps * b;
+    const double e = pre * s * qi * qj * inv;
+    e_acc += e;
+    if (gds) gds[f * np + p] = pre * qi * qj * (ds_s * inv + s * dinv);
+    if (gq && pre != 0.0) {
+      unsafeAtomicAdd(&gq[f * n + i], pre * s * qj * inv);
+      unsafeAtomicAdd(&gq[f * n + j], pre * s * qi * inv);
+    }
+  }
+  double arr[1] = {e_acc};
+  block_sum<1>(arr, red);
+  if (threadIdx.x == 0) partials[f * gridDim.x + blockIdx.x] = arr[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_reduce_frames(const double* __restrict__ p, int nb,
+                                                          double* __restrict__ out) {
+  __shared__ double red[kBlock / 64];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += p[(int64_t)blockIdx.x * nb + b];
+  double arr[1] = {s};
+  block_sum<1>(arr, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = arr[0];
+}
+
+// ---------------------------------------------------------------------------
+// Fused real space: one thread per atom in cell order, the neighbour cells'
+// atoms read from a cell-sorted (x, y, z, q) copy.  Per block partials (17):
+//   0 E_real, 1..6 strain derivative D (xx yy zz xy xz yz), 7..15 sum r_i (x) g_i,
+//   16 sum q^2 ; the total charge is kept in slot 17.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_pack_sorted(int64_t n, const int* __restrict__ sorted,
+                                                        const double* __restrict__ w,
+                                                        const double* __restrict__ q,
+                                                        P4<T>* __restrict__ ps) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = sorted[t];
+  P4<T> v;
+  v.x = (T)w[3 * i];
+  v.y = (T)w[3 * i + 1];
+  v.z = (T)w[3 * i + 2];
+  v.q = (T)q[i];
+  ps[t] = v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_real_fused(int64_t n, CellGrid cg, T kappa, T eps,
+                                                       const P4<T>* __restrict__ ps,
+                                                       const int* __restrict__ sorted,
+                                                       const int* __restrict__ cell_of,
+                                                       const int* __restrict__ start,
+                                                       const double* __restrict__ pos,
+                                                       double* __restrict__ grad,
+                                                       double* __restrict__ partials) {
+  __shared__ double red[kRealNV * (kBlock / 64)];
+  double acc[kRealNV];
+#pragma unroll
+  for (int k = 0; k < kRealNV; ++k) acc[k] = 0.0;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    const int i = sorted[t];
+    const P4<T> pi = ps[t];
+    const int cid = cell_of[i];
+    const int ci[3] = {cid / (cg.nc[1] * cg.nc[2]), (cid / cg.nc[2]) % cg.nc[1], cid % cg.nc[2]};
+    const T rc2 = (T)cg.rc2;
+    const T rmin2 = (T)1e-4;
+    const T two_over_sqrtpi_k = (T)(2.0 / kSqrtPi) * kappa;
+    const T k2 = kappa * kappa;
+    T gx = 0, gy = 0, gz = 0;
+    double e = 0.0, dxx = 0.0, dyy = 0.0, dzz = 0.0, dxy = 0.0, dxz = 0.0, dyz = 0.0;
+    const int l0 = stencil_len(cg, 0), l1 = stencil_len(cg, 1), l2 = stencil_len(cg, 2);
+    for (int o0 = 0; o0 < l0; ++o0) {
+      int c0, s0;
+      stencil_at(cg, 0, ci[0], o0, &c0, &s0);
+      for (int o1 = 0; o1 < l1; ++o1) {
+        int c1, s1;
+        stencil_at(cg, 1, ci[1], o1, &c1, &s1);
+        for (int o2 = 0; o2 < l2; ++o2) {
+          int c2, s2;
+          stencil_at(cg, 2, ci[2], o2, &c2, &s2);
+          const T shx = (T)(s0 * cg.h[0] + s1 * cg.h[3] + s2 * cg.h[6]) - pi.x;
+          const T shy = (T)(s0 * cg.h[1] + s1 * cg.h[4] + s2 * cg.h[7]) - pi.y;
+          const T shz = (T)(s0 * cg.h[2] + s1 * cg.h[5] + s2 * cg.h[8]) - pi.z;
+          const int cc = (c0 * cg.nc[1] + c1) * cg.nc[2] + c2;
+          const int xe = start[cc + 1];
+          for (int x = start[cc]; x < xe; ++x) {
+            const P4<T> pj = ps[x];
+            const T dx = pj.x + shx, dy = pj.y + shy, dz = pj.z + shz;
+            const T r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < rc2 && r2 > rmin2) {
+              const T r = sqrt(r2);
+              const T ir = T(1) / r;
+              const T qq = eps * pi.q * pj.q;
+              const T ec = erfc(kappa * r);
+              const T ev = qq * ec * ir;
+              // de/dr = -qq (2 kappa / sqrt(pi) exp(-kappa^2 r^2) + erfc / r) / r
+              const T dedr = -qq * (two_over_sqrtpi_k * exp(-k2 * r2) + ec * ir) * ir;
+              const T fs = dedr * ir;  // (de/dr) / r
+              e += (double)ev;
+              gx -= fs * dx;
+              gy -= fs * dy;
+              gz -= fs * dz;
+              dxx += (double)(fs * dx * dx);
+              dyy += (double)(fs * dy * dy);
+              dzz += (double)(fs * dz * dz);
+              dxy += (double)(fs * dx * dy);
+              dxz += (double)(fs * dx * dz);
+              dyz += (double)(fs * dy * dz);
+            }
+          }
+        }
+      }
+    }
+    grad[3 * i] = (double)gx;
+    grad[3 * i + 1] = (double)gy;
+    grad[3 * i + 2] = (double)gz;
+    acc[0] = 0.5 * e;
+    acc[1] = 0.5 * dxx;
+    acc[2] = 0.5 * dyy;
+    acc[3] = 0.5 * dzz;
+    acc[4] = 0.5 * dxy;
+    acc[5] = 0.5 * dxz;
+    acc[6] = 0.5 * dyz;
+    const double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    const double g[3] = {(double)gx, (double)gy, (double)gz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) acc[7 + a * 3 + b] = r[a] * g[b];
+    acc[16] = (double)pi.q * (double)pi.q;
+    acc[17] = (double)pi.q;
+  }
+  block_sum<kRealNV>(acc, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kRealNV; ++k) partials[(int64_t)blockIdx.x * kRealNV + k] = acc[k];
+}
+
+template __global__ void k_pack_sorted<float>(int64_t, const int*, const double*, const double*,
+                                              P4<float>*);
+template __global__ void k_pack_sorted<double>(int64_t, const int*, const double*,
+                                               const double*, P4<double>*);
+
+// ---------------------------------------------------------------------------
+// host launchers
+
+void nb_layout(int64_t n, const CellGrid& cg, NbLayout* L) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~size_t(255);
+    return o;
+  };
+  L->w = take(sizeof(double) * 3 * (size_t)std::max<int64_t>(n, 1));
+  L->cell_of = take(sizeof(int) * (size_t)std::max<int64_t>(n, 1));
+  L->count = take(sizeof(int) * (size_t)(cg.ncell + 1));
+  L->start = take(sizeof(int) * (size_t)(cg.ncell + 1));
+  L->cursor = take(sizeof(int) * (size_t)(cg.ncell + 1));
+  L->sorted = take(sizeof(int) * (size_t)std::max<int64_t>(n, 1));
+  L->pcount = take(sizeof(int64_t) * (size_t)(n + 1));
+  L->offsets = take(sizeof(int64_t) * (size_t)(n + 1));
+  const size_t sc = std::max(scan_scratch_count<int64_t>(n + 1), scan_scratch_count<int>(cg.ncell + 1));
+  L->scan = take(sizeof(int64_t) * sc);
+  L->bytes = off;
+}
+
+hipError_t launch_bin(const double* pos, int64_t n, const CellGrid& cg, char* ws,
+                      const NbLayout& L, hipStream_t s) {
+  int* count = (int*)(ws + L.count);
+  int* start = (int*)(ws + L.start);
+  int* cursor = (int*)(ws + L.cursor);
+  const int nfill = cg.ncell + 1;
+  k_fill<int><<<std::min((nfill + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(count, nfill, 0);
+  k_fill<int><<<std::min((nfill + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(cursor, nfill, 0);
+  const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
+  if (n > 0)
+    k_nb_bin<<<nb, kBlock, 0, s>>>(pos, n, cg, (double*)(ws + L.w), (int*)(ws + L.cell_of), count);
+  launch_scan<int>(count, cg.ncell, start, (int*)(ws + L.scan), s);
+  if (n > 0) {
+    k_nb_scatter<<<nb, kBlock, 0, s>>>(n, (int*)(ws + L.cell_of), start, cursor,
+                                       (int*)(ws + L.sorted));
+    k_nb_sort_cells<<<(cg.ncell + kBlock - 1) / kBlock, kBlock, 0, s>>>(cg.ncell, start,
+                                                                      (int*)(ws + L.sorted));
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pairs(int mode, int64_t n, const CellGrid& cg, char* ws, const NbLayout& L,
+                        int64_t* pairs, int64_t capacity, hipStream_t s) {
+  const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
+  if (n == 0) return hipGetLastError();
+  const double* w = (const double*)(ws + L.w);
+  const int* cell_of = (const int*)(ws + L.cell_of);
+  const int* start = (const int*)(ws + L.start);
+  const int* sorted = (const int*)(ws + L.sorted);
+  int64_t* pc = (int64_t*)(ws + L.pcount);
+  int64_t* off = (int64_t*)(ws + L.offsets);
+  if (mode == 0) {
+    k_nb_pairs<0><<<nb, kBlock, 0, s>>>(n, cg, w, cell_of, start, sorted, pc, nullptr, nullptr,
+                                        0);
+    launch_scan<int64_t>(pc, n, off, (int64_t*)(ws + L.scan), s);
+  } else {
+    k_nb_pairs<1><<<nb, kBlock, 0, s>>>(n, cg, w, cell_of, start, sorted, nullptr, off, pairs,
+                                        capacity);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_ds(const double* pos, const int64_t* pairs, int64_t np, const CellGrid& cg,
+                          double* ds, double* dr, hipStream_t s) {
+  if (np > 0)
+    k_pair_ds<<<(unsigned)((np + kBlock - 1) / kBlock), kBlock, 0, s>>>(pos, pairs, np, cg, ds, dr);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_ds_bwd(const double* gds, const int64_t* pairs, int64_t np,
+                              const double* pos, const CellGrid& cg, const double* dr,
+                              const double* ds, double* gpos, double* gbox, hipStream_t s) {
+  if (np > 0)
+    k_pair_ds_bwd<<<(unsigned)((np + kBlock - 1) / kBlock), kBlock, 0, s>>>(gds, pairs, np, pos, cg,
+                                                                          dr, ds, gpos, gbox);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const double* q,
+                              const double* eta, const int64_t* pairs, const double* ds,
+                              const double* bs, double kappa, double eps, double* energy,
+                              double* gds, double* gq, double* partials, hipStream_t s) {
+  const int bpf = kPairBlocksPerFrame;
+  if (gq) {
+    const int64_t m = nf * n;
+    if (m > 0)
+      k_fill<double><<<(unsigned)std::min<int64_t>((m + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(
+          gq, m, 0.0);
+  }
+  dim3 grid(bpf, (unsigned)nf);
+  k_pair_energy<<<grid, kBlock, 0, s>>>(mode, n, np, q, eta, pairs, ds, bs, kappa, eps, partials,
+                                        gds, gq);
+  k_reduce_frames<<<(unsigned)nf, kBlock, 0, s>>>(partials, bpf, energy);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_real_fused(const double* pos, const double* q, int64_t n, const CellGrid& cg,
+                             double kappa, double eps, char* ws, const NbLayout& L, P4<T>* ps,
+                             double* grad, double* partials, double* reduced, hipStream_t s) {
+  const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
+  if (n > 0) {
+    k_pack_sorted<T><<<nb, kBlock, 0, s>>>(n, (const int*)(ws + L.sorted), (const double*)(ws + L.w),
+                                           q, ps);
+    k_real_fused<T><<<nb, kBlock, 0, s>>>(n, cg, (T)kappa, (T)eps, ps, (const int*)(ws + L.sorted),
+                                          (const int*)(ws + L.cell_of), (const int*)(ws + L.start),
+                                          pos, grad, partials);
+  }
+  k_reduce_partials<<<1, kBlock, 0, s>>>(partials, std::max<int>(nb, 1) * (n > 0), kRealNV, 1.0,
+                                         reduced, reduced + 1);
+  return hipGetLastError();
+}
+
+template hipError_t launch_real_fused<float>(const double*, const double*, int64_t,
+                                             const CellGrid&, double, double, char*,
+                                             const NbLayout&, P4<float>*, double*, double*,
+                                             double*, hipStream_t);
+template hipError_t launch_real_fused<double>(const double*, const double*, int64_t,
+                                              const CellGrid&, double, double, char*,
+                                              const NbLayout&, P4<double>*, double*, double*,
+                                              double*, hipStream_t);
+
+}  // namespace admp

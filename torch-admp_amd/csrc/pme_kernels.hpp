@@ -1,0 +1,40 @@
+// Internal declarations shared by the kernel files and the C-ABI layer.
+#pragma once
+
+#include <algorithm>
+
+#include "admp_common.hpp"
+
+namespace admp {
+
+constexpr int kMaxPasses = 24;
+constexpr int kMaxMesh = 1024;  // largest mesh dimension (LDS line limit)
+
+// Factorisation of an FFT length into Stockham passes: radices 8, 4, 2, 3, 5
+// first, then any remaining primes (handled by a direct-sum pass).
+struct Radix {
+  int n;
+  int r[kMaxPasses];
+};
+
+Radix factorize(int L);
+bool radix_ok(const Radix& rd);
+
+template <typename T>
+hipError_t launch_spread(const double* pos, const double* q, int64_t n, const Geom& g, T* mesh,
+                         hipStream_t s);
+
+template <typename T>
+hipError_t launch_gather(const double* pos, const double* q, int64_t n, const Geom& g,
+                         const T* phi, double* grad_pos, double* dEdq, double* dEdA,
+                         int accumulate, double* partials, hipStream_t s);
+
+template <typename T>
+hipError_t launch_convolve(const T* mesh, const Geom& g, int conv, double kappa, double eps,
+                           double phi_scale, T* phi, double* energy, double* dEdA,
+                           cplx<T>* spec, double* partials, hipStream_t s);
+
+// number of doubles the convolution needs for its per-block partials
+size_t convolve_partials_count(const int32_t* k);
+
+}  // namespace admp

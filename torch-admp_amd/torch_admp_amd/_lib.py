@@ -1,0 +1,130 @@
+"""ctypes binding of libadmp_amd.so (the C ABI declared in include/admp_amd.h).
+
+The library is the only compute path of this package: importing works
+everywhere (so that shape validation and host logic can be used and tested on
+a CPU-only machine), but every call that reaches device code raises when the
+library is missing or the tensors are not on a ROCm GPU.  There is no CPU
+fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+from typing import Optional
+
+import torch
+
+_LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libadmp_amd.so"
+
+ADMP_OK = 0
+ADMP_OVERFLOW = 1
+ADMP_F64 = 0
+ADMP_F32 = 1
+KCONV = {"reference": 0, "columns": 1}
+PRECISION = {"float64": ADMP_F64, "float32": ADMP_F32}
+
+_c_i32 = ctypes.c_int32
+_c_i64 = ctypes.c_int64
+_c_dbl = ctypes.c_double
+_c_vp = ctypes.c_void_p
+_c_sz = ctypes.c_size_t
+_P = ctypes.POINTER
+
+# name -> (restype, argtypes); mirrors include/admp_amd.h
+_SIGNATURES = {
+    "admp_version": (ctypes.c_int, []),
+    "admp_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "admp_nblist_workspace_bytes": (ctypes.c_int, [_c_i64, _c_vp, _c_dbl, _P(_c_sz)]),
+    "admp_nblist_count": (ctypes.c_int, [_c_vp, _c_i64, _c_vp, _c_dbl, _c_vp, _c_sz,
+                                         _P(_c_i64), _c_vp]),
+    "admp_nblist_fill": (ctypes.c_int, [_c_vp, _c_i64, _c_vp, _c_dbl, _c_vp, _c_sz, _c_vp,
+                                        _c_i64, _c_vp]),
+    "admp_pair_ds": (ctypes.c_int, [_c_vp, _c_vp, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "admp_pair_ds_backward": (ctypes.c_int, [_c_vp, _c_vp, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp,
+                                             _c_vp, _c_vp, _c_vp]),
+    "admp_pair_energy_workspace_bytes": (ctypes.c_int, [_c_i64, _P(_c_sz)]),
+    "admp_pair_energy": (ctypes.c_int, [_c_i32, _c_i64, _c_i64, _c_i64, _c_vp, _c_vp, _c_vp,
+                                        _c_vp, _c_vp, _c_dbl, _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp,
+                                        _c_sz, _c_vp]),
+    "admp_pme_workspace_bytes": (ctypes.c_int, [_c_vp, _c_i32, _c_i64, _P(_c_sz)]),
+    "admp_pme_spread": (ctypes.c_int, [_c_vp, _c_vp, _c_i64, _c_vp, _c_vp, _c_i32, _c_vp, _c_vp,
+                                       _c_sz, _c_vp]),
+    "admp_pme_convolve": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_i32, _c_i32, _c_dbl, _c_dbl,
+                                         _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp, _c_sz, _c_vp]),
+    "admp_pme_gather": (ctypes.c_int, [_c_vp, _c_vp, _c_i64, _c_vp, _c_vp, _c_i32, _c_vp, _c_vp,
+                                       _c_vp, _c_vp, _c_i32, _c_vp, _c_sz, _c_vp]),
+    "admp_pme_fused_workspace_bytes": (ctypes.c_int, [_c_i64, _c_vp, _c_dbl, _c_vp, _c_i32,
+                                                      _P(_c_sz)]),
+    "admp_pme_fused": (ctypes.c_int, [_c_vp, _c_vp, _c_i64, _c_vp, _c_dbl, _c_dbl, _c_vp, _c_i32,
+                                      _c_i32, _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp, _c_sz, _c_vp]),
+}
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+class AdmpError(RuntimeError):
+    """A non-OK status returned by the HIP library."""
+
+
+def library_path() -> Path:
+    return Path(os.environ.get("ADMP_AMD_LIB", str(_LIB_PATH)))
+
+
+def load() -> ctypes.CDLL:
+    """Load (once) and return the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not path.exists():
+        raise ImportError(
+            f"torch_admp_amd: HIP library not found at {path}; build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` or `make -C torch-admp_amd/csrc`")
+    lib = ctypes.CDLL(str(path))
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int, what: str) -> None:
+    if status != ADMP_OK:
+        msg = load().admp_status_string(status).decode()
+        raise AdmpError(f"{what} failed with status {status}: {msg}")
+
+
+def ptr(t: Optional[torch.Tensor]):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_of(t: torch.Tensor):
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def require_gpu(*tensors: Optional[torch.Tensor]) -> None:
+    """Fail loudly unless every tensor lives on a ROCm GPU."""
+    for t in tensors:
+        if t is not None and t.device.type != "cuda":
+            raise RuntimeError(
+                "torch_admp_amd runs only on a ROCm GPU (no CPU fallback); got a tensor on "
+                f"{t.device}")
+
+
+def box_array(box: Optional[torch.Tensor]):
+    """Host copy of a (3, 3) box as a ctypes double[9] (None for open boundaries)."""
+    if box is None:
+        return None
+    vals = box.detach().reshape(9).to("cpu", torch.float64).tolist()
+    return (ctypes.c_double * 9)(*vals)
+
+
+def kmesh_array(kmesh) -> ctypes.Array:
+    return (ctypes.c_int32 * 3)(*[int(k) for k in kmesh])
+
+
+def workspace(nbytes: int, device) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
