@@ -153,6 +153,18 @@ int admp_pme_fused(const double* pos, const double* q, int64_t n, const double* 
                    int32_t convention, double dielectric, double* energy, double* forces,
                    double* dEdbox, void* workspace, size_t ws_bytes, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Stage timing of admp_pme_fused (a profiling aid with no counterpart in the
+ * reference; single-threaded use).  admp_profile_begin arms hipEvent pairs
+ * around the stages of the next max_calls fused calls; admp_profile_collect
+ * synchronises, writes the summed GPU milliseconds per stage and the number of
+ * calls recorded, and disarms.  Stages: 0 cell binning and packing, 1 real-space
+ * pair kernel, 2 charge spreading, 3 FFT passes with the influence function,
+ * 4 gather, 5 reductions and finalisation. */
+#define ADMP_PROFILE_STAGES 6
+int admp_profile_begin(int32_t max_calls);
+int admp_profile_collect(double* stage_ms, int32_t* n_calls);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,9 @@ function               reference                                  pinned by (tes
 =====================  =========================================  ==============================
 wrap_positions         src/torch_admp/nblist.py:447-470           closed form (fractional in [0,1))
 nblist_pbc             src/torch_admp/nblist.py:335-445           brute_force_pairs (set equality)
+nblist_pbc_cells       src/torch_admp/nblist.py:335-445 (cell     nblist_pbc and brute_force_pairs
+                       table, step by step; used as the timed     (set equality)
+                       CPU baseline)
 nblist_obc             src/torch_admp/nblist.py:472-490           brute_force_pairs
 buffer_scales          src/torch_admp/nblist.py:492-515           closed form (i < j)
 pbc_shift / pairs_ds   src/torch_admp/spatial.py:15-36,           brute-force minimum image
@@ -74,6 +77,49 @@ def nblist_pbc(pos: torch.Tensor, box: torch.Tensor, cutoff: float) -> torch.Ten
     pairs = torch.cat(out, 0)
     order = torch.argsort(pairs[:, 0] * pos.shape[0] + pairs[:, 1], stable=True)
     return pairs[order]
+
+
+def nblist_pbc_cells(pos: torch.Tensor, box: torch.Tensor, cutoff: float) -> torch.Tensor:
+    """The reference's own cell-table search (src/torch_admp/nblist.py:335-445),
+    restated step by step so its cost can be timed on the CPU:
+    image padding, Cartesian cells of edge rc, a stable sort into a dense
+    (cells, max occupancy) table, a gather of the 27 neighbour cells of every
+    atom, and the 0.01 < d < rc mask."""
+    pos = pos.detach()
+    box = box.detach()
+    w = wrap_positions(pos, box)
+    vol = torch.sum(box[0] * torch.cross(box[1], box[2], dim=-1))
+    pads = [torch.ceil(cutoff * torch.cross(box[a], box[b], dim=-1).norm() / vol).long().item()
+            for a, b in ((1, 2), (2, 0), (0, 1))]
+    pad_mat = torch.cartesian_prod(*[torch.arange(-p, p + 1) for p in pads]).to(box.dtype)
+    n_img = pad_mat.shape[0]
+    padded = (w.unsqueeze(1) + pad_mat @ box).view(-1, 3)
+    cpos = torch.floor(padded / cutoff).long()
+    corner = torch.min(cpos, dim=0)[0]
+    cpos -= corner
+    shape = torch.max(cpos, dim=0)[0] + 1
+    n_cells = int(torch.prod(shape).item())
+    stride = torch.stack([shape[1] * shape[2], shape[2], torch.ones_like(shape[0])])
+    cind = torch.sum(cpos * stride, dim=1)
+    gind = torch.arange(cind.shape[0]) + 1                       # 1-based padded index
+    rind = torch.arange(pos.shape[0]).repeat_interleave(n_img)   # owning atom
+    disp = torch.cartesian_prod(*[torch.arange(-1, 2)] * 3)
+    atom_cn = ((torch.floor(w / cutoff).long() - corner).unsqueeze(1) + disp)
+    atom_cnind = torch.sum(atom_cn * stride, dim=-1)
+    cind_sorted, order = torch.sort(cind, stable=True)
+    _, counts = torch.unique_consecutive(cind_sorted, return_counts=True)
+    max_occ = int(counts.max().item())
+    table = torch.zeros((n_cells, max_occ, 2), dtype=torch.long)
+    slot = torch.nonzero(torch.arange(max_occ).repeat(counts.shape[0], 1) < counts.unsqueeze(-1))[:, 1]
+    table[cind_sorted, slot, 0] = gind[order]
+    table[cind_sorted, slot, 1] = rind[order]
+    nb = table[atom_cnind]
+    pi, nn, jj = torch.where(nb[:, :, :, 0])
+    pj = nb[pi, nn, jj, 1]
+    pj_pad = nb[pi, nn, jj, 0] - 1
+    d = torch.norm(padded[pj_pad] - w[pi], dim=1)
+    mask = (d < cutoff) & (d > 0.01)
+    return torch.stack((pi, pj), 1)[mask]
 
 
 def nblist_obc(pos: torch.Tensor, cutoff: float) -> torch.Tensor:

@@ -1,0 +1,72 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every symbol that
+include/*.h declares, and rejects invalid arguments without touching a GPU."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        txt = h.read_text()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        names |= set(re.findall(r"^\s*(?:int|const char\*)\s+(admp_\w+)\s*\(", txt, flags=re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from torch_admp_amd import _lib
+
+    lib = _lib.load()
+    declared = _declared()
+    assert len(declared) >= 15
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert declared == set(_lib._SIGNATURES), "ctypes signature table out of sync with the header"
+
+
+def test_status_strings_and_version():
+    from torch_admp_amd import _lib
+
+    lib = _lib.load()
+    assert lib.admp_version() >= 100
+    assert lib.admp_status_string(0) == b"ok"
+    assert lib.admp_status_string(-1) == b"invalid argument"
+
+
+def test_invalid_arguments_rejected_on_host():
+    from torch_admp_amd import _lib
+
+    lib = _lib.load()
+    nbytes = ctypes.c_size_t()
+    bad_box = (ctypes.c_double * 9)(*([0.0] * 9))
+    assert lib.admp_nblist_workspace_bytes(10, bad_box, 5.0, ctypes.byref(nbytes)) == -1
+    assert lib.admp_nblist_workspace_bytes(10, None, -1.0, ctypes.byref(nbytes)) == -1
+    km = (ctypes.c_int32 * 3)(0, 16, 16)
+    assert lib.admp_pme_workspace_bytes(km, 0, 10, ctypes.byref(nbytes)) == -1
+    km = (ctypes.c_int32 * 3)(16, 16, 16)
+    assert lib.admp_pme_workspace_bytes(km, 7, 10, ctypes.byref(nbytes)) == -1
+    assert lib.admp_pme_workspace_bytes(km, 1, 10, ctypes.byref(nbytes)) == 0
+    assert nbytes.value >= 16 * 16 * 9 * 8
+    box = (ctypes.c_double * 9)(10.0, 0, 0, 0, 10.0, 0, 0, 0, 10.0)
+    assert lib.admp_nblist_workspace_bytes(1000, box, 3.0, ctypes.byref(nbytes)) == 0
+    assert lib.admp_pme_fused_workspace_bytes(1000, box, 3.0, km, 0, ctypes.byref(nbytes)) == 0
+    # profiling: collect without begin is an invalid call
+    stage = (ctypes.c_double * 6)()
+    calls = ctypes.c_int32()
+    assert lib.admp_profile_collect(stage, ctypes.byref(calls)) == -1
+
+
+def test_cpu_tensors_fail_loudly():
+    import torch
+
+    from torch_admp_amd.nblist import TorchNeighborList
+
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        TorchNeighborList(3.0)(torch.rand(10, 3, dtype=torch.float64) * 10,
+                               torch.eye(3, dtype=torch.float64) * 10)

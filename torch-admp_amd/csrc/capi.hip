@@ -140,33 +140,70 @@ __global__ void k_negate(const double* __restrict__ g, int64_t m, double* __rest
     out[i] = -g[i];
 }
 
+// Stage timing (profiling aid): hipEvent pairs around the fused stages.
+struct Profiler {
+  bool on = false;
+  int max_calls = 0;
+  int calls = 0;
+  hipEvent_t* ev = nullptr;
+};
+Profiler g_prof;
+
+struct StageMarks {
+  hipStream_t s;
+  hipEvent_t* ev = nullptr;
+  void mark(int k) {
+    if (ev) hipEventRecord(ev[k], s);
+  }
+};
+
+StageMarks stage_marks(hipStream_t s) {
+  StageMarks m;
+  m.s = s;
+  if (g_prof.on && g_prof.calls < g_prof.max_calls) {
+    m.ev = g_prof.ev + (size_t)g_prof.calls * (ADMP_PROFILE_STAGES + 1);
+    ++g_prof.calls;
+  }
+  return m;
+}
+
 template <typename T>
 static hipError_t fused_impl(const double* pos, const double* q, int64_t n, const double* box,
                              double cutoff, double kappa, const int32_t* kmesh, int32_t conv,
                              double eps, double* energy, double* forces, double* dEdbox,
                              char* ws, const FusedLayout& F, const CellGrid& cg, hipStream_t s) {
+  (void)cutoff;
   const Geom g = make_geom(box, kmesh);
   double* grad = (double*)(ws + F.grad);
   double* red = (double*)(ws + F.red);
   T* mesh = (T*)(ws + F.mesh);
   char* pws = ws + F.pme;
   double* ppart = (double*)(pws + F.pl.partials);
+  P4<T>* ps = (P4<T>*)(ws + F.ps);
+  StageMarks pm = stage_marks(s);
+  pm.mark(0);
   hipError_t e = launch_bin(pos, n, cg, ws, F.nb, s);
+  if (e == hipSuccess) e = launch_pack<T>(q, n, ws, F.nb, ps, s);
+  pm.mark(1);
+  if (e == hipSuccess)
+    e = launch_real_fused<T>(pos, n, cg, kappa, eps, ws, F.nb, ps, grad, (double*)(ws + F.rpart), s);
+  pm.mark(2);
+  if (e == hipSuccess) e = launch_spread<T>(pos, q, n, g, mesh, s);
+  pm.mark(3);
+  if (e == hipSuccess)
+    e = launch_convolve<T>(mesh, g, conv, kappa, eps, 1.0, mesh, red + 18, red + 19,
+                           (cplx<T>*)(pws + F.pl.spec), ppart, s);
+  pm.mark(4);
+  if (e == hipSuccess) e = launch_gather<T>(pos, q, n, g, mesh, grad, nullptr, red + 28, 1, ppart, s);
+  pm.mark(5);
   if (e != hipSuccess) return e;
-  e = launch_real_fused<T>(pos, q, n, cg, kappa, eps, ws, F.nb, (P4<T>*)(ws + F.ps), grad,
-                           (double*)(ws + F.rpart), red, s);
-  if (e != hipSuccess) return e;
-  e = launch_spread<T>(pos, q, n, g, mesh, s);
-  if (e != hipSuccess) return e;
-  e = launch_convolve<T>(mesh, g, conv, kappa, eps, 1.0, mesh, red + 18, red + 19,
-                         (cplx<T>*)(pws + F.pl.spec), ppart, s);
-  if (e != hipSuccess) return e;
-  e = launch_gather<T>(pos, q, n, g, mesh, grad, nullptr, red + 28, 1, ppart, s);
-  if (e != hipSuccess) return e;
+  const int nbr = n > 0 ? (int)((n + kBlock - 1) / kBlock) : 0;
+  k_reduce_partials<<<1, kBlock, 0, s>>>((double*)(ws + F.rpart), nbr, kRealNV, 1.0, red, red + 1);
   k_finalize<<<1, 64, 0, s>>>(red, g, kappa, eps, energy, dEdbox);
   if (n > 0)
     k_negate<<<(unsigned)std::min<int64_t>((3 * n + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(
         grad, 3 * n, forces);
+  pm.mark(6);
   return hipGetLastError();
 }
 
@@ -175,6 +212,48 @@ static hipError_t fused_impl(const double* pos, const double* q, int64_t n, cons
 extern "C" {
 
 int admp_version(void) { return 100; }
+
+int admp_profile_begin(int32_t max_calls) {
+  if (max_calls < 1 || max_calls > 100000) return ADMP_ERR_INVALID;
+  if (g_prof.ev) {
+    for (int i = 0; i < g_prof.max_calls * (ADMP_PROFILE_STAGES + 1); ++i)
+      hipEventDestroy(g_prof.ev[i]);
+    delete[] g_prof.ev;
+  }
+  const int nev = max_calls * (ADMP_PROFILE_STAGES + 1);
+  g_prof.ev = new hipEvent_t[nev];
+  for (int i = 0; i < nev; ++i)
+    if (hipEventCreate(&g_prof.ev[i]) != hipSuccess) return ADMP_ERR_HIP;
+  g_prof.max_calls = max_calls;
+  g_prof.calls = 0;
+  g_prof.on = true;
+  return ADMP_OK;
+}
+
+int admp_profile_collect(double* stage_ms, int32_t* n_calls) {
+  if (!stage_ms || !n_calls) return ADMP_ERR_INVALID;
+  for (int k = 0; k < ADMP_PROFILE_STAGES; ++k) stage_ms[k] = 0.0;
+  *n_calls = 0;
+  if (!g_prof.ev) return ADMP_ERR_INVALID;
+  g_prof.on = false;
+  int status = ADMP_OK;
+  for (int c = 0; c < g_prof.calls; ++c) {
+    hipEvent_t* ev = g_prof.ev + (size_t)c * (ADMP_PROFILE_STAGES + 1);
+    if (hipEventSynchronize(ev[ADMP_PROFILE_STAGES]) != hipSuccess) status = ADMP_ERR_HIP;
+    for (int k = 0; k < ADMP_PROFILE_STAGES; ++k) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) status = ADMP_ERR_HIP;
+      stage_ms[k] += ms;
+    }
+  }
+  *n_calls = g_prof.calls;
+  for (int i = 0; i < g_prof.max_calls * (ADMP_PROFILE_STAGES + 1); ++i)
+    hipEventDestroy(g_prof.ev[i]);
+  delete[] g_prof.ev;
+  g_prof.ev = nullptr;
+  g_prof.calls = g_prof.max_calls = 0;
+  return status;
+}
 
 const char* admp_status_string(int status) {
   switch (status) {

@@ -125,21 +125,20 @@ __global__ void __launch_bounds__(kBlock) k_nb_scatter(int64_t n, const int* __r
   sorted[start[c] + atomicAdd(&cursor[c], 1)] = (int)i;
 }
 
-// Order the atoms of every cell by index so traversal order is deterministic.
-__global__ void __launch_bounds__(kBlock) k_nb_sort_cells(int ncell, const int* __restrict__ start,
-                                                          int* __restrict__ sorted) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
+// Deterministic in-cell order: the rank of atom i among the atoms of its cell
+// is the number of cell members with a smaller index (one thread per atom).
+__global__ void __launch_bounds__(kBlock) k_nb_rank(int64_t n, const int* __restrict__ cell_of,
+                                                    const int* __restrict__ start,
+                                                    const int* __restrict__ unsorted,
+                                                    int* __restrict__ sorted) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const int i = unsorted[x];
+  const int c = cell_of[i];
   const int b = start[c], e = start[c + 1];
-  for (int x = b + 1; x < e; ++x) {
-    const int v = sorted[x];
-    int y = x - 1;
-    while (y >= b && sorted[y] > v) {
-      sorted[y + 1] = sorted[y];
-      --y;
-    }
-    sorted[y + 1] = v;
-  }
+  int rank = 0;
+  for (int y = b; y < e; ++y) rank += unsorted[y] < i;
+  sorted[b + rank] = i;
 }
 
 // Count (MODE 0) or write (MODE 1) the ordered pairs of atom i:
@@ -550,6 +549,7 @@ void nb_layout(int64_t n, const CellGrid& cg, NbLayout* L) {
   L->start = take(sizeof(int) * (size_t)(cg.ncell + 1));
   L->cursor = take(sizeof(int) * (size_t)(cg.ncell + 1));
   L->sorted = take(sizeof(int) * (size_t)std::max<int64_t>(n, 1));
+  L->unsorted = take(sizeof(int) * (size_t)std::max<int64_t>(n, 1));
   L->pcount = take(sizeof(int64_t) * (size_t)(n + 1));
   L->offsets = take(sizeof(int64_t) * (size_t)(n + 1));
   const size_t sc = std::max(scan_scratch_count<int64_t>(n + 1), scan_scratch_count<int>(cg.ncell + 1));
@@ -571,9 +571,9 @@ hipError_t launch_bin(const double* pos, int64_t n, const CellGrid& cg, char* ws
   launch_scan<int>(count, cg.ncell, start, (int*)(ws + L.scan), s);
   if (n > 0) {
     k_nb_scatter<<<nb, kBlock, 0, s>>>(n, (int*)(ws + L.cell_of), start, cursor,
-                                       (int*)(ws + L.sorted));
-    k_nb_sort_cells<<<(cg.ncell + kBlock - 1) / kBlock, kBlock, 0, s>>>(cg.ncell, start,
-                                                                      (int*)(ws + L.sorted));
+                                       (int*)(ws + L.unsorted));
+    k_nb_rank<<<nb, kBlock, 0, s>>>(n, (int*)(ws + L.cell_of), start, (int*)(ws + L.unsorted),
+                                    (int*)(ws + L.sorted));
   }
   return hipGetLastError();
 }
@@ -634,29 +634,34 @@ hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const
 }
 
 template <typename T>
-hipError_t launch_real_fused(const double* pos, const double* q, int64_t n, const CellGrid& cg,
-                             double kappa, double eps, char* ws, const NbLayout& L, P4<T>* ps,
-                             double* grad, double* partials, double* reduced, hipStream_t s) {
-  const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
-  if (n > 0) {
-    k_pack_sorted<T><<<nb, kBlock, 0, s>>>(n, (const int*)(ws + L.sorted), (const double*)(ws + L.w),
-                                           q, ps);
-    k_real_fused<T><<<nb, kBlock, 0, s>>>(n, cg, (T)kappa, (T)eps, ps, (const int*)(ws + L.sorted),
-                                          (const int*)(ws + L.cell_of), (const int*)(ws + L.start),
-                                          pos, grad, partials);
-  }
-  k_reduce_partials<<<1, kBlock, 0, s>>>(partials, std::max<int>(nb, 1) * (n > 0), kRealNV, 1.0,
-                                         reduced, reduced + 1);
+hipError_t launch_pack(const double* q, int64_t n, char* ws, const NbLayout& L, P4<T>* ps,
+                       hipStream_t s) {
+  if (n > 0)
+    k_pack_sorted<T><<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        n, (const int*)(ws + L.sorted), (const double*)(ws + L.w), q, ps);
   return hipGetLastError();
 }
 
-template hipError_t launch_real_fused<float>(const double*, const double*, int64_t,
-                                             const CellGrid&, double, double, char*,
-                                             const NbLayout&, P4<float>*, double*, double*,
-                                             double*, hipStream_t);
-template hipError_t launch_real_fused<double>(const double*, const double*, int64_t,
-                                              const CellGrid&, double, double, char*,
-                                              const NbLayout&, P4<double>*, double*, double*,
-                                              double*, hipStream_t);
+template <typename T>
+hipError_t launch_real_fused(const double* pos, int64_t n, const CellGrid& cg, double kappa,
+                             double eps, char* ws, const NbLayout& L, const P4<T>* ps,
+                             double* grad, double* partials, hipStream_t s) {
+  if (n > 0)
+    k_real_fused<T><<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        n, cg, (T)kappa, (T)eps, ps, (const int*)(ws + L.sorted), (const int*)(ws + L.cell_of),
+        (const int*)(ws + L.start), pos, grad, partials);
+  return hipGetLastError();
+}
+
+template hipError_t launch_pack<float>(const double*, int64_t, char*, const NbLayout&, P4<float>*,
+                                       hipStream_t);
+template hipError_t launch_pack<double>(const double*, int64_t, char*, const NbLayout&,
+                                        P4<double>*, hipStream_t);
+template hipError_t launch_real_fused<float>(const double*, int64_t, const CellGrid&, double,
+                                             double, char*, const NbLayout&, const P4<float>*,
+                                             double*, double*, hipStream_t);
+template hipError_t launch_real_fused<double>(const double*, int64_t, const CellGrid&, double,
+                                              double, char*, const NbLayout&, const P4<double>*,
+                                              double*, double*, hipStream_t);
 
 }  // namespace admp

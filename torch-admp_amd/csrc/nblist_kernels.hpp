@@ -27,7 +27,7 @@ struct alignas(4 * sizeof(T)) P4 {
 
 // byte offsets of the neighbour-list workspace regions
 struct NbLayout {
-  size_t w, cell_of, count, start, cursor, sorted, pcount, offsets, scan, bytes;
+  size_t w, cell_of, count, start, cursor, sorted, unsorted, pcount, offsets, scan, bytes;
 };
 
 CellGrid make_grid(const double* box, double cutoff, bool obc);
@@ -49,8 +49,12 @@ hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const
                               const double* bs, double kappa, double eps, double* energy,
                               double* gds, double* gq, double* partials, hipStream_t s);
 template <typename T>
-hipError_t launch_real_fused(const double* pos, const double* q, int64_t n, const CellGrid& cg,
-                             double kappa, double eps, char* ws, const NbLayout& L, P4<T>* ps,
-                             double* grad, double* partials, double* reduced, hipStream_t s);
+hipError_t launch_pack(const double* q, int64_t n, char* ws, const NbLayout& L, P4<T>* ps,
+                       hipStream_t s);
+// per-block partials of the fused real-space kernel: ceil(n / kBlock) rows of kRealNV
+template <typename T>
+hipError_t launch_real_fused(const double* pos, int64_t n, const CellGrid& cg, double kappa,
+                             double eps, char* ws, const NbLayout& L, const P4<T>* ps,
+                             double* grad, double* partials, hipStream_t s);
 
 }  // namespace admp
