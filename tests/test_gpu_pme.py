@@ -182,12 +182,19 @@ def test_module_energy_forces(dev, box, kmesh, conv):
     assert abs(float(mod.reciprocal_energy) - float(orc.recip)) < 1e-9 * abs(float(orc.recip))
 
 
+BIG_TRICLINIC = [[24.0, 0.0, 0.0], [3.0, 23.0, 0.0], [-2.0, 1.5, 25.0]]
+
+
 @pytest.mark.parametrize("precision,etol,ftol", [("float64", 1e-10, 1e-8),
                                                  ("float32", 1e-6, 1e-4)])
-@pytest.mark.parametrize("box,kmesh", [(np.eye(3) * 16.0, [32, 32, 32]),
-                                       (TRICLINIC, [30, 32, 36])])
-def test_fused_energy_forces(dev, precision, etol, ftol, box, kmesh):
-    pos, q, box = _system(400, box, 10)
+@pytest.mark.parametrize("box,kmesh,n", [(np.eye(3) * 16.0, [32, 32, 32], 400),
+                                         (TRICLINIC, [30, 32, 36], 400),
+                                         (np.eye(3) * 24.0, [32, 30, 27], 1200),
+                                         (BIG_TRICLINIC, [25, 24, 40], 1200)])
+def test_fused_energy_forces(dev, precision, etol, ftol, box, kmesh, n):
+    """Small boxes use one cell with periodic images; the 24 A boxes use the
+    cell grid (nc >= 4) and the LDS-tiled spread."""
+    pos, q, box = _system(n, box, 10)
     rc = 5.5
     e_ref, f_ref, gb_ref, orc = _oracle_ef(pos, box, q, rc, kmesh)
     mod = CoulombForceModule(rc, 1e-5, kmesh=kmesh, precision=precision)

@@ -123,22 +123,23 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* smem) {
   __syncthreads();
 }
 
-// Deterministic reduction of per-block partials in a fixed order:
-// value 0 -> first[0], values 1.. -> rest[v - 1]; either pointer may be null.
+// Deterministic reduction of per-block partials in a fixed order, one
+// wavefront per value: value 0 -> first[0], values 1.. -> rest[v - 1]; either
+// pointer may be null.  Launch with kReduceThreads threads.
+constexpr int kReduceThreads = 1024;
 template <int UNUSED = 0>
-__global__ void __launch_bounds__(kBlock) k_reduce_partials_t(const double* __restrict__ p,
-                                                              int nblocks, int nv, double scale,
-                                                              double* __restrict__ first,
-                                                              double* __restrict__ rest) {
-  __shared__ double red[kBlock / 64];
-  for (int v = 0; v < nv; ++v) {
+__global__ void __launch_bounds__(kReduceThreads) k_reduce_partials_t(
+    const double* __restrict__ p, int nblocks, int nv, double scale, double* __restrict__ first,
+    double* __restrict__ rest) {
+  const int lane = threadIdx.x & 63;
+  const int nw = blockDim.x >> 6;
+  for (int v = threadIdx.x >> 6; v < nv; v += nw) {
     double s = 0.0;
-    for (int bi = threadIdx.x; bi < nblocks; bi += blockDim.x) s += p[(int64_t)bi * nv + v];
-    double arr[1] = {s};
-    block_sum<1>(arr, red);
-    if (threadIdx.x == 0) {
-      if (v == 0 && first) first[0] = scale * arr[0];
-      if (v > 0 && rest) rest[v - 1] = scale * arr[0];
+    for (int bi = lane; bi < nblocks; bi += 64) s += p[(int64_t)bi * nv + v];
+    s = wave_sum(s);
+    if (lane == 0) {
+      if (v == 0 && first) first[0] = scale * s;
+      if (v > 0 && rest) rest[v - 1] = scale * s;
     }
   }
 }

@@ -6,6 +6,7 @@
 
 #include "nblist_kernels.hpp"
 #include "pme_kernels.hpp"
+#include "tiles.hpp"
 
 using namespace admp;
 
@@ -69,8 +70,9 @@ PmeLayout pme_layout(const int32_t* k, int32_t precision) {
 // PME workspace + real-space partials + reduced scalars.
 struct FusedLayout {
   NbLayout nb;
-  size_t ps, grad, mesh, pme, rpart, red, bytes;
+  size_t ps, grad, mesh, pme, rpart, red, tiles, bytes;
   PmeLayout pl;
+  TileGeom tg;
 };
 
 FusedLayout fused_layout(int64_t n, const CellGrid& cg, const int32_t* k, int32_t precision) {
@@ -78,18 +80,22 @@ FusedLayout fused_layout(int64_t n, const CellGrid& cg, const int32_t* k, int32_
   nb_layout(n, cg, &F.nb);
   size_t off = align256(F.nb.bytes);
   F.ps = off;
-  off += align256((size_t)std::max<int64_t>(n, 1) * 4 * elem_size(precision));
+  off += align256((size_t)std::max<int64_t>(n, 1) * 4 * sizeof(double));
   F.grad = off;
-  off += align256((size_t)std::max<int64_t>(n, 1) * 3 * sizeof(double));
+  off += align256((size_t)kRealSplit * std::max<int64_t>(n, 1) * 3 * sizeof(double));
   F.mesh = off;
   off += align256((size_t)k[0] * k[1] * k[2] * elem_size(precision));
   F.pl = pme_layout(k, precision);
   F.pme = off;
   off += align256(F.pl.bytes);
   F.rpart = off;
-  off += align256((size_t)((n + kBlock - 1) / kBlock + 1) * kRealNV * sizeof(double));
+  off += align256((size_t)(std::max<int64_t>(real_fused_blocks(n), (int64_t)kRealSplit * cg.ncell) + 1) *
+                  kRealNV * sizeof(double));
   F.red = off;
   off += align256(64 * sizeof(double));
+  F.tg = make_tile_geom(cg, k, elem_size(precision));
+  F.tiles = off;
+  if (F.tg.ok) off += align256((size_t)cg.ncell * F.tg.ntile * elem_size(precision));
   F.bytes = off;
   return F;
 }
@@ -134,10 +140,15 @@ __global__ void k_finalize(const double* __restrict__ red, Geom g, double kappa,
     }
 }
 
-__global__ void k_negate(const double* __restrict__ g, int64_t m, double* __restrict__ out) {
+// forces = -(sum of the nsplit gradient rows)
+__global__ void k_forces(const double* __restrict__ g, int nsplit, int64_t m,
+                         double* __restrict__ out) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
-       i += (int64_t)gridDim.x * blockDim.x)
-    out[i] = -g[i];
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int k = 0; k < nsplit; ++k) s += g[k * m + i];
+    out[i] = -s;
+  }
 }
 
 // Stage timing (profiling aid): hipEvent pairs around the fused stages.
@@ -179,16 +190,27 @@ static hipError_t fused_impl(const double* pos, const double* q, int64_t n, cons
   T* mesh = (T*)(ws + F.mesh);
   char* pws = ws + F.pme;
   double* ppart = (double*)(pws + F.pl.partials);
-  P4<T>* ps = (P4<T>*)(ws + F.ps);
+  P4<double>* ps = (P4<double>*)(ws + F.ps);
   StageMarks pm = stage_marks(s);
   pm.mark(0);
   hipError_t e = launch_bin(pos, n, cg, ws, F.nb, s);
-  if (e == hipSuccess) e = launch_pack<T>(q, n, ws, F.nb, ps, s);
+  if (e == hipSuccess) e = launch_pack(q, n, ws, F.nb, ps, s);
   pm.mark(1);
-  if (e == hipSuccess)
-    e = launch_real_fused<T>(pos, n, cg, kappa, eps, ws, F.nb, ps, grad, (double*)(ws + F.rpart), s);
+  const bool cells = real_cells_ok(cg);
+  if (e == hipSuccess) {
+    if (cells)
+      e = launch_real_cells<T>(pos, n, cg, kappa, eps, ws, F.nb, ps, grad, (double*)(ws + F.rpart), s);
+    else
+      e = launch_real_fused<T>(pos, n, cg, kappa, eps, ws, F.nb, ps, grad, (double*)(ws + F.rpart), s);
+  }
   pm.mark(2);
-  if (e == hipSuccess) e = launch_spread<T>(pos, q, n, g, mesh, s);
+  if (e == hipSuccess) {
+    if (F.tg.ok)
+      e = launch_spread_tiled<T>(pos, q, n, g, F.tg, cg.ncell, (const int*)(ws + F.nb.sorted),
+                                 (const int*)(ws + F.nb.start), (T*)(ws + F.tiles), mesh, s);
+    else
+      e = launch_spread<T>(pos, q, n, g, mesh, s);
+  }
   pm.mark(3);
   if (e == hipSuccess)
     e = launch_convolve<T>(mesh, g, conv, kappa, eps, 1.0, mesh, red + 18, red + 19,
@@ -197,12 +219,12 @@ static hipError_t fused_impl(const double* pos, const double* q, int64_t n, cons
   if (e == hipSuccess) e = launch_gather<T>(pos, q, n, g, mesh, grad, nullptr, red + 28, 1, ppart, s);
   pm.mark(5);
   if (e != hipSuccess) return e;
-  const int nbr = n > 0 ? (int)((n + kBlock - 1) / kBlock) : 0;
-  k_reduce_partials<<<1, kBlock, 0, s>>>((double*)(ws + F.rpart), nbr, kRealNV, 1.0, red, red + 1);
+  const int nbr = n > 0 ? (cells ? kRealSplit * cg.ncell : (int)real_fused_blocks(n)) : 0;
+  k_reduce_partials<<<1, kReduceThreads, 0, s>>>((double*)(ws + F.rpart), nbr, kRealNV, 1.0, red, red + 1);
   k_finalize<<<1, 64, 0, s>>>(red, g, kappa, eps, energy, dEdbox);
   if (n > 0)
-    k_negate<<<(unsigned)std::min<int64_t>((3 * n + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(
-        grad, 3 * n, forces);
+    k_forces<<<(unsigned)std::min<int64_t>((3 * n + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(
+        grad, cells ? kRealSplit : 1, 3 * n, forces);
   pm.mark(6);
   return hipGetLastError();
 }

@@ -60,6 +60,15 @@ CellGrid make_grid(const double* box, double cutoff, bool obc) {
     }
     ncell *= cg.nc[d];
   }
+  // bound the cell count (cells wider than rc stay correct, only slower)
+  while (ncell > kMaxCells) {
+    int d = 0;
+    for (int k = 1; k < 3; ++k)
+      if (cg.nc[k] > cg.nc[d]) d = k;
+    ncell /= cg.nc[d];
+    cg.nc[d] = std::max(3, cg.nc[d] / 2);
+    ncell *= cg.nc[d];
+  }
   cg.ncell = (int)ncell;
   return cg;
 }
@@ -112,7 +121,7 @@ __global__ void __launch_bounds__(kBlock) k_nb_bin(const double* __restrict__ po
     cid = (c[0] * cg.nc[1] + c[1]) * cg.nc[2] + c[2];
   }
   cell_of[i] = cid;
-  atomicAdd(&count[cid], 1);
+  atomicAdd(&count[cid * kSub + (int)(i & (kSub - 1))], 1);
 }
 
 __global__ void __launch_bounds__(kBlock) k_nb_scatter(int64_t n, const int* __restrict__ cell_of,
@@ -121,8 +130,8 @@ __global__ void __launch_bounds__(kBlock) k_nb_scatter(int64_t n, const int* __r
                                                        int* __restrict__ sorted) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int c = cell_of[i];
-  sorted[start[c] + atomicAdd(&cursor[c], 1)] = (int)i;
+  const int key = cell_of[i] * kSub + (int)(i & (kSub - 1));
+  sorted[start[key] + atomicAdd(&cursor[key], 1)] = (int)i;
 }
 
 // Deterministic in-cell order: the rank of atom i among the atoms of its cell
@@ -135,7 +144,7 @@ __global__ void __launch_bounds__(kBlock) k_nb_rank(int64_t n, const int* __rest
   if (x >= n) return;
   const int i = unsorted[x];
   const int c = cell_of[i];
-  const int b = start[c], e = start[c + 1];
+  const int b = start[c * kSub], e = start[(c + 1) * kSub];
   int rank = 0;
   for (int y = b; y < e; ++y) rank += unsorted[y] < i;
   sorted[b + rank] = i;
@@ -178,7 +187,8 @@ __global__ void __launch_bounds__(kBlock) k_nb_pairs(int64_t n, CellGrid cg,
 #pragma unroll
         for (int j = 0; j < 3; ++j) sh[j] = s0 * cg.h[j] + s1 * cg.h[3 + j] + s2 * cg.h[6 + j];
         const int cc = (c0 * cg.nc[1] + c1) * cg.nc[2] + c2;
-        for (int x = start[cc]; x < start[cc + 1]; ++x) {
+        const int xe = start[(cc + 1) * kSub];
+        for (int x = start[cc * kSub]; x < xe; ++x) {
           const int j = sorted[x];
           const double dx = (w[3 * j] + sh[0]) - wi[0];
           const double dy = (w[3 * j + 1] + sh[1]) - wi[1];
@@ -420,25 +430,30 @@ __global__ void __launch_bounds__(kBlock) k_reduce_frames(const double* __restri
 // atoms read from a cell-sorted (x, y, z, q) copy.  Per block partials (17):
 //   0 E_real, 1..6 strain derivative D (xx yy zz xy xz yz), 7..15 sum r_i (x) g_i,
 //   16 sum q^2 ; the total charge is kept in slot 17.
-template <typename T>
+// Cell-sorted copy of the wrapped positions and charges.  Positions stay in
+// float64 so that pair displacements are formed without cancellation; the
+// pair arithmetic itself runs in the bulk type.
 __global__ void __launch_bounds__(kBlock) k_pack_sorted(int64_t n, const int* __restrict__ sorted,
                                                         const double* __restrict__ w,
                                                         const double* __restrict__ q,
-                                                        P4<T>* __restrict__ ps) {
+                                                        P4<double>* __restrict__ ps) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int i = sorted[t];
-  P4<T> v;
-  v.x = (T)w[3 * i];
-  v.y = (T)w[3 * i + 1];
-  v.z = (T)w[3 * i + 2];
-  v.q = (T)q[i];
+  P4<double> v;
+  v.x = w[3 * i];
+  v.y = w[3 * i + 1];
+  v.z = w[3 * i + 2];
+  v.q = q[i];
   ps[t] = v;
 }
 
+// 32 lanes per atom (atoms taken in cell order, 8 per block); lane s owns the
+// stencil entries s, s + 32, ...  Per-lane sums are combined with shuffles
+// inside the 32-lane half-wave, then per block.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_real_fused(int64_t n, CellGrid cg, T kappa, T eps,
-                                                       const P4<T>* __restrict__ ps,
+                                                       const P4<double>* __restrict__ ps,
                                                        const int* __restrict__ sorted,
                                                        const int* __restrict__ cell_of,
                                                        const int* __restrict__ start,
@@ -446,80 +461,82 @@ __global__ void __launch_bounds__(kBlock) k_real_fused(int64_t n, CellGrid cg, T
                                                        double* __restrict__ grad,
                                                        double* __restrict__ partials) {
   __shared__ double red[kRealNV * (kBlock / 64)];
-  double acc[kRealNV];
-#pragma unroll
-  for (int k = 0; k < kRealNV; ++k) acc[k] = 0.0;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) {
-    const int i = sorted[t];
-    const P4<T> pi = ps[t];
+  const int l32 = threadIdx.x & 31;
+  const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 5) + (threadIdx.x >> 5);
+  const bool valid = t < n;
+  T gx = 0, gy = 0, gz = 0, e = 0;
+  T vxx = 0, vyy = 0, vzz = 0, vxy = 0, vxz = 0, vyz = 0;
+  int i = 0;
+  P4<double> pi;
+  pi.x = pi.y = pi.z = pi.q = 0.0;
+  if (valid) {
+    i = sorted[t];
+    pi = ps[t];
     const int cid = cell_of[i];
     const int ci[3] = {cid / (cg.nc[1] * cg.nc[2]), (cid / cg.nc[2]) % cg.nc[1], cid % cg.nc[2]};
     const T rc2 = (T)cg.rc2;
     const T rmin2 = (T)1e-4;
-    const T two_over_sqrtpi_k = (T)(2.0 / kSqrtPi) * kappa;
+    const T c_exp = (T)(2.0 / kSqrtPi) * kappa;
     const T k2 = kappa * kappa;
-    T gx = 0, gy = 0, gz = 0;
-    double e = 0.0, dxx = 0.0, dyy = 0.0, dzz = 0.0, dxy = 0.0, dxz = 0.0, dyz = 0.0;
     const int l0 = stencil_len(cg, 0), l1 = stencil_len(cg, 1), l2 = stencil_len(cg, 2);
-    for (int o0 = 0; o0 < l0; ++o0) {
-      int c0, s0;
+    const int nst = l0 * l1 * l2;
+    for (int st = l32; st < nst; st += 32) {
+      const int o2 = st % l2, o1 = (st / l2) % l1, o0 = st / (l1 * l2);
+      int c0, s0, c1, s1, c2, s2;
       stencil_at(cg, 0, ci[0], o0, &c0, &s0);
-      for (int o1 = 0; o1 < l1; ++o1) {
-        int c1, s1;
-        stencil_at(cg, 1, ci[1], o1, &c1, &s1);
-        for (int o2 = 0; o2 < l2; ++o2) {
-          int c2, s2;
-          stencil_at(cg, 2, ci[2], o2, &c2, &s2);
-          const T shx = (T)(s0 * cg.h[0] + s1 * cg.h[3] + s2 * cg.h[6]) - pi.x;
-          const T shy = (T)(s0 * cg.h[1] + s1 * cg.h[4] + s2 * cg.h[7]) - pi.y;
-          const T shz = (T)(s0 * cg.h[2] + s1 * cg.h[5] + s2 * cg.h[8]) - pi.z;
-          const int cc = (c0 * cg.nc[1] + c1) * cg.nc[2] + c2;
-          const int xe = start[cc + 1];
-          for (int x = start[cc]; x < xe; ++x) {
-            const P4<T> pj = ps[x];
-            const T dx = pj.x + shx, dy = pj.y + shy, dz = pj.z + shz;
-            const T r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < rc2 && r2 > rmin2) {
-              const T r = sqrt(r2);
-              const T ir = T(1) / r;
-              const T qq = eps * pi.q * pj.q;
-              const T ec = erfc(kappa * r);
-              const T ev = qq * ec * ir;
-              // de/dr = -qq (2 kappa / sqrt(pi) exp(-kappa^2 r^2) + erfc / r) / r
-              const T dedr = -qq * (two_over_sqrtpi_k * exp(-k2 * r2) + ec * ir) * ir;
-              const T fs = dedr * ir;  // (de/dr) / r
-              e += (double)ev;
-              gx -= fs * dx;
-              gy -= fs * dy;
-              gz -= fs * dz;
-              dxx += (double)(fs * dx * dx);
-              dyy += (double)(fs * dy * dy);
-              dzz += (double)(fs * dz * dz);
-              dxy += (double)(fs * dx * dy);
-              dxz += (double)(fs * dx * dz);
-              dyz += (double)(fs * dy * dz);
-            }
-          }
+      stencil_at(cg, 1, ci[1], o1, &c1, &s1);
+      stencil_at(cg, 2, ci[2], o2, &c2, &s2);
+      const double shx = (s0 * cg.h[0] + s1 * cg.h[3] + s2 * cg.h[6]) - pi.x;
+      const double shy = (s0 * cg.h[1] + s1 * cg.h[4] + s2 * cg.h[7]) - pi.y;
+      const double shz = (s0 * cg.h[2] + s1 * cg.h[5] + s2 * cg.h[8]) - pi.z;
+      const T qi = (T)pi.q;
+      const int cc = (c0 * cg.nc[1] + c1) * cg.nc[2] + c2;
+      const int xe = start[(cc + 1) * kSub];
+      for (int x = start[cc * kSub]; x < xe; ++x) {
+        const P4<double> pj = ps[x];
+        const T dx = (T)(pj.x + shx), dy = (T)(pj.y + shy), dz = (T)(pj.z + shz);
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < rc2 && r2 > rmin2) {
+          const T r = sqrt(r2);
+          const T ir = T(1) / r;
+          const T qq = eps * qi * (T)pj.q;
+          const T ec = erfc(kappa * r);
+          // (de/dr) / r with e = qq erfc(kappa r) / r
+          const T fs = -qq * (c_exp * exp(-k2 * r2) + ec * ir) * ir * ir;
+          e += qq * ec * ir;
+          gx -= fs * dx;
+          gy -= fs * dy;
+          gz -= fs * dz;
+          vxx += fs * dx * dx;
+          vyy += fs * dy * dy;
+          vzz += fs * dz * dz;
+          vxy += fs * dx * dy;
+          vxz += fs * dx * dz;
+          vyz += fs * dy * dz;
         }
       }
     }
-    grad[3 * i] = (double)gx;
-    grad[3 * i + 1] = (double)gy;
-    grad[3 * i + 2] = (double)gz;
-    acc[0] = 0.5 * e;
-    acc[1] = 0.5 * dxx;
-    acc[2] = 0.5 * dyy;
-    acc[3] = 0.5 * dzz;
-    acc[4] = 0.5 * dxy;
-    acc[5] = 0.5 * dxz;
-    acc[6] = 0.5 * dyz;
+  }
+  double v[10] = {(double)e, (double)vxx, (double)vyy, (double)vzz, (double)vxy,
+                  (double)vxz, (double)vyz, (double)gx, (double)gy, (double)gz};
+#pragma unroll
+  for (int off = 16; off > 0; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < 10; ++k) v[k] += __shfl_xor(v[k], off, 64);
+  double acc[kRealNV];
+#pragma unroll
+  for (int k = 0; k < kRealNV; ++k) acc[k] = 0.0;
+  if (valid && l32 == 0) {
+    grad[3 * i] = v[7];
+    grad[3 * i + 1] = v[8];
+    grad[3 * i + 2] = v[9];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) acc[k] = 0.5 * v[k];
     const double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-    const double g[3] = {(double)gx, (double)gy, (double)gz};
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-      for (int b = 0; b < 3; ++b) acc[7 + a * 3 + b] = r[a] * g[b];
+      for (int b = 0; b < 3; ++b) acc[7 + a * 3 + b] = r[a] * v[7 + b];
     acc[16] = (double)pi.q * (double)pi.q;
     acc[17] = (double)pi.q;
   }
@@ -528,10 +545,6 @@ __global__ void __launch_bounds__(kBlock) k_real_fused(int64_t n, CellGrid cg, T
     for (int k = 0; k < kRealNV; ++k) partials[(int64_t)blockIdx.x * kRealNV + k] = acc[k];
 }
 
-template __global__ void k_pack_sorted<float>(int64_t, const int*, const double*, const double*,
-                                              P4<float>*);
-template __global__ void k_pack_sorted<double>(int64_t, const int*, const double*,
-                                               const double*, P4<double>*);
 
 // ---------------------------------------------------------------------------
 // host launchers
@@ -545,14 +558,15 @@ void nb_layout(int64_t n, const CellGrid& cg, NbLayout* L) {
   };
   L->w = take(sizeof(double) * 3 * (size_t)std::max<int64_t>(n, 1));
   L->cell_of = take(sizeof(int) * (size_t)std::max<int64_t>(n, 1));
-  L->count = take(sizeof(int) * (size_t)(cg.ncell + 1));
-  L->start = take(sizeof(int) * (size_t)(cg.ncell + 1));
-  L->cursor = take(sizeof(int) * (size_t)(cg.ncell + 1));
+  const size_t nkeys = (size_t)cg.ncell * kSub + 1;
+  L->count = take(sizeof(int) * nkeys);
+  L->start = take(sizeof(int) * nkeys);
+  L->cursor = take(sizeof(int) * nkeys);
   L->sorted = take(sizeof(int) * (size_t)std::max<int64_t>(n, 1));
   L->unsorted = take(sizeof(int) * (size_t)std::max<int64_t>(n, 1));
   L->pcount = take(sizeof(int64_t) * (size_t)(n + 1));
   L->offsets = take(sizeof(int64_t) * (size_t)(n + 1));
-  const size_t sc = std::max(scan_scratch_count<int64_t>(n + 1), scan_scratch_count<int>(cg.ncell + 1));
+  const size_t sc = std::max(scan_scratch_count<int64_t>(n + 1), scan_scratch_count<int>(nkeys));
   L->scan = take(sizeof(int64_t) * sc);
   L->bytes = off;
 }
@@ -562,13 +576,13 @@ hipError_t launch_bin(const double* pos, int64_t n, const CellGrid& cg, char* ws
   int* count = (int*)(ws + L.count);
   int* start = (int*)(ws + L.start);
   int* cursor = (int*)(ws + L.cursor);
-  const int nfill = cg.ncell + 1;
+  const int nfill = cg.ncell * kSub + 1;
   k_fill<int><<<std::min((nfill + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(count, nfill, 0);
   k_fill<int><<<std::min((nfill + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(cursor, nfill, 0);
   const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
   if (n > 0)
     k_nb_bin<<<nb, kBlock, 0, s>>>(pos, n, cg, (double*)(ws + L.w), (int*)(ws + L.cell_of), count);
-  launch_scan<int>(count, cg.ncell, start, (int*)(ws + L.scan), s);
+  launch_scan<int>(count, (int64_t)cg.ncell * kSub, start, (int*)(ws + L.scan), s);
   if (n > 0) {
     k_nb_scatter<<<nb, kBlock, 0, s>>>(n, (int*)(ws + L.cell_of), start, cursor,
                                        (int*)(ws + L.unsorted));
@@ -633,32 +647,258 @@ hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Fused real space, cell-staged.  Work-group (c, s) takes home cell c and every
+// S-th candidate (split s) of its stencil.  The candidates, shifted to the
+// right periodic image, are staged once in LDS in float64; home atoms map onto
+// lanes 2-D (Hg atoms x 64/Hg candidate phases), and each wavefront compacts
+// the in-cutoff (home, candidate) pairs into an LDS queue so the erfc / exp
+// arithmetic always runs on 64 useful lanes.  Forces of the home atoms gather
+// in LDS (atomics) and leave as one row per split: gsplit[s][i][3].
+// Per-block partials as in k_real_fused (kRealNV values).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T>
-hipError_t launch_pack(const double* q, int64_t n, char* ws, const NbLayout& L, P4<T>* ps,
+struct PairAcc {
+  double e;
+  T vxx, vyy, vzz, vxy, vxz, vyz;
+};
+
+template <typename T>
+__device__ __forceinline__ void pair_eval(const P4<double>& pi, const P4<double>& pj, T kappa,
+                                          T eps, T c_exp, T k2, double* facc_i, PairAcc<T>& acc) {
+  const T dx = (T)(pj.x - pi.x), dy = (T)(pj.y - pi.y), dz = (T)(pj.z - pi.z);
+  const T r2 = dx * dx + dy * dy + dz * dz;
+  const T r = sqrt(r2);
+  const T ir = T(1) / r;
+  const T qq = eps * (T)pi.q * (T)pj.q;
+  const T ec = erfc(kappa * r);
+  const T fs = -qq * (c_exp * exp(-k2 * r2) + ec * ir) * ir * ir;  // (de/dr) / r
+  acc.e += (double)(qq * ec * ir);
+  atomicAdd(facc_i + 0, (double)(-fs * dx));
+  atomicAdd(facc_i + 1, (double)(-fs * dy));
+  atomicAdd(facc_i + 2, (double)(-fs * dz));
+  acc.vxx += fs * dx * dx;
+  acc.vyy += fs * dy * dy;
+  acc.vzz += fs * dz * dz;
+  acc.vxy += fs * dx * dy;
+  acc.vxz += fs * dx * dz;
+  acc.vyz += fs * dy * dz;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_real_cells(int64_t n, CellGrid cg, T kappa, T eps,
+                                                       const P4<double>* __restrict__ ps,
+                                                       const int* __restrict__ sorted,
+                                                       const int* __restrict__ start,
+                                                       const double* __restrict__ pos,
+                                                       double* __restrict__ gsplit,
+                                                       double* __restrict__ partials) {
+  __shared__ P4<double> cand[kCandCap];
+  __shared__ P4<double> home[kHomeCap];
+  __shared__ double facc[kHomeCap * 3];  // float64 LDS atomics in both precisions
+  __shared__ int queue[(kBlock / 64) * 128];
+  __shared__ int st_pre[kMaxStencil + 1];
+  __shared__ int st_beg[kMaxStencil];
+  __shared__ double st_sh[kMaxStencil * 3];
+  __shared__ double red[kRealNV * (kBlock / 64)];
+
+  const int c = blockIdx.x;
+  const int split = blockIdx.y, S = gridDim.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ci[3] = {c / (cg.nc[1] * cg.nc[2]), (c / cg.nc[2]) % cg.nc[1], c % cg.nc[2]};
+  const int l0 = stencil_len(cg, 0), l1 = stencil_len(cg, 1), l2 = stencil_len(cg, 2);
+  const int nst = l0 * l1 * l2;
+  if (threadIdx.x < nst) {
+    const int st = threadIdx.x;
+    const int o2 = st % l2, o1 = (st / l2) % l1, o0 = st / (l1 * l2);
+    int c0, s0, c1, s1, c2, s2;
+    stencil_at(cg, 0, ci[0], o0, &c0, &s0);
+    stencil_at(cg, 1, ci[1], o1, &c1, &s1);
+    stencil_at(cg, 2, ci[2], o2, &c2, &s2);
+    const int cc = (c0 * cg.nc[1] + c1) * cg.nc[2] + c2;
+    st_beg[st] = start[cc * kSub];
+    st_pre[st + 1] = start[(cc + 1) * kSub] - start[cc * kSub];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) st_sh[st * 3 + d] = s0 * cg.h[d] + s1 * cg.h[3 + d] + s2 * cg.h[6 + d];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    st_pre[0] = 0;
+    for (int st = 1; st <= nst; ++st) st_pre[st] += st_pre[st - 1];
+  }
+  __syncthreads();
+  const int ctot = st_pre[nst];
+  const int cs = ctot > split ? (ctot - split + S - 1) / S : 0;
+  const int hb = start[c * kSub], he = start[(c + 1) * kSub];
+  const T rc2 = (T)cg.rc2, rmin2 = (T)1e-4;
+  const T c_exp = (T)(2.0 / kSqrtPi) * kappa, k2 = kappa * kappa;
+  PairAcc<T> pacc;
+  pacc.e = 0.0;
+  pacc.vxx = pacc.vyy = pacc.vzz = pacc.vxy = pacc.vxz = pacc.vyz = T(0);
+  double acc[kRealNV];
+#pragma unroll
+  for (int k = 0; k < kRealNV; ++k) acc[k] = 0.0;
+  int* q = queue + wave * 128;
+
+  for (int h0 = hb; h0 < he; h0 += kHomeCap) {
+    const int hc = min(kHomeCap, he - h0);
+    for (int t = threadIdx.x; t < hc; t += blockDim.x) {
+      home[t] = ps[h0 + t];
+      facc[3 * t] = facc[3 * t + 1] = facc[3 * t + 2] = 0.0;
+    }
+    for (int k0 = 0; k0 < cs; k0 += kCandCap) {
+      const int cc = min(kCandCap, cs - k0);
+      for (int kk = threadIdx.x; kk < cc; kk += blockDim.x) {
+        const int kg = split + S * (k0 + kk);
+        int lo = 0, hi = nst;  // last st with st_pre[st] <= kg
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (st_pre[mid] <= kg) lo = mid;
+          else hi = mid;
+        }
+        const P4<double> p = ps[st_beg[lo] + (kg - st_pre[lo])];
+        P4<double> v;
+        v.x = p.x + st_sh[lo * 3];
+        v.y = p.y + st_sh[lo * 3 + 1];
+        v.z = p.z + st_sh[lo * 3 + 2];
+        v.q = p.q;
+        cand[kk] = v;
+      }
+      __syncthreads();
+      for (int g0 = 0; g0 < hc; g0 += 64) {
+        const int hg = min(64, hc - g0);
+        const int nph = 64 / hg;
+        const int hl = lane % hg, ph = lane / hg;
+        const bool active = ph < nph;
+        const P4<double> pi = home[g0 + hl];
+        const int stride = nph * (kBlock / 64);
+        const int nit = (cc + stride - 1) / stride;
+        int qn = 0;
+        for (int j = 0; j < nit; ++j) {
+          const int kk = ph + nph * wave + stride * j;
+          bool pred = false;
+          if (active && kk < cc) {
+            const P4<double> pj = cand[kk];
+            const T dx = (T)(pj.x - pi.x), dy = (T)(pj.y - pi.y), dz = (T)(pj.z - pi.z);
+            const T r2 = dx * dx + dy * dy + dz * dz;
+            pred = r2 < rc2 && r2 > rmin2;
+          }
+          const unsigned long long m = __ballot(pred);
+          if (pred) {
+            const int at = qn + __popcll(m & ((1ull << lane) - 1ull));
+            q[at] = ((g0 + hl) << 16) | kk;
+          }
+          qn += __popcll(m);
+          if (qn >= 64) {
+            wave_sync();
+            const int ent = q[lane];
+            const int hx = ent >> 16, kx = ent & 0xffff;
+            pair_eval<T>(home[hx], cand[kx], kappa, eps, c_exp, k2, facc + 3 * hx, pacc);
+            const int rem = qn - 64;
+            const int tail = lane < rem ? q[64 + lane] : 0;
+            wave_sync();
+            if (lane < rem) q[lane] = tail;
+            wave_sync();
+            qn = rem;
+          }
+        }
+        wave_sync();
+        if (lane < qn) {
+          const int ent = q[lane];
+          const int hx = ent >> 16, kx = ent & 0xffff;
+          pair_eval<T>(home[hx], cand[kx], kappa, eps, c_exp, k2, facc + 3 * hx, pacc);
+        }
+        wave_sync();
+      }
+      __syncthreads();
+    }
+    // home forces of this split
+    for (int t = threadIdx.x; t < hc; t += blockDim.x) {
+      const int i = sorted[h0 + t];
+      const double g[3] = {facc[3 * t], facc[3 * t + 1], facc[3 * t + 2]};
+      double* dst = gsplit + ((int64_t)split * n + i) * 3;
+      dst[0] = g[0];
+      dst[1] = g[1];
+      dst[2] = g[2];
+      const double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+#pragma unroll
+      for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+        for (int b2 = 0; b2 < 3; ++b2) acc[7 + a2 * 3 + b2] += r[a2] * g[b2];
+      if (split == 0) {
+        const double qi = home[t].q;
+        acc[16] += qi * qi;
+        acc[17] += qi;
+      }
+    }
+    __syncthreads();
+  }
+  acc[0] = 0.5 * pacc.e;
+  acc[1] = 0.5 * (double)pacc.vxx;
+  acc[2] = 0.5 * (double)pacc.vyy;
+  acc[3] = 0.5 * (double)pacc.vzz;
+  acc[4] = 0.5 * (double)pacc.vxy;
+  acc[5] = 0.5 * (double)pacc.vxz;
+  acc[6] = 0.5 * (double)pacc.vyz;
+  block_sum<kRealNV>(acc, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kRealNV; ++k)
+      partials[((int64_t)split * gridDim.x + blockIdx.x) * kRealNV + k] = acc[k];
+}
+
+bool real_cells_ok(const CellGrid& cg) {
+  int nst = 1;
+  for (int d = 0; d < 3; ++d) nst *= cg.nc[d] >= 3 ? 3 : 2 * cg.nimg[d] + 1;
+  return !cg.obc && nst <= kMaxStencil;
+}
+
+template <typename T>
+hipError_t launch_real_cells(const double* pos, int64_t n, const CellGrid& cg, double kappa,
+                             double eps, const char* ws, const NbLayout& L,
+                             const P4<double>* ps, double* gsplit, double* partials,
+                             hipStream_t s) {
+  if (n > 0) {
+    dim3 grid(cg.ncell, kRealSplit);
+    k_real_cells<T><<<grid, kBlock, 0, s>>>(n, cg, (T)kappa, (T)eps, ps,
+                                            (const int*)(ws + L.sorted),
+                                            (const int*)(ws + L.start), pos, gsplit, partials);
+  }
+  return hipGetLastError();
+}
+
+template hipError_t launch_real_cells<float>(const double*, int64_t, const CellGrid&, double,
+                                             double, const char*, const NbLayout&,
+                                             const P4<double>*, double*, double*, hipStream_t);
+template hipError_t launch_real_cells<double>(const double*, int64_t, const CellGrid&, double,
+                                              double, const char*, const NbLayout&,
+                                              const P4<double>*, double*, double*, hipStream_t);
+
+hipError_t launch_pack(const double* q, int64_t n, char* ws, const NbLayout& L, P4<double>* ps,
                        hipStream_t s) {
   if (n > 0)
-    k_pack_sorted<T><<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+    k_pack_sorted<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(
         n, (const int*)(ws + L.sorted), (const double*)(ws + L.w), q, ps);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_real_fused(const double* pos, int64_t n, const CellGrid& cg, double kappa,
-                             double eps, char* ws, const NbLayout& L, const P4<T>* ps,
+                             double eps, char* ws, const NbLayout& L, const P4<double>* ps,
                              double* grad, double* partials, hipStream_t s) {
   if (n > 0)
-    k_real_fused<T><<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+    k_real_fused<T><<<(unsigned)real_fused_blocks(n), kBlock, 0, s>>>(
         n, cg, (T)kappa, (T)eps, ps, (const int*)(ws + L.sorted), (const int*)(ws + L.cell_of),
         (const int*)(ws + L.start), pos, grad, partials);
   return hipGetLastError();
 }
 
-template hipError_t launch_pack<float>(const double*, int64_t, char*, const NbLayout&, P4<float>*,
-                                       hipStream_t);
-template hipError_t launch_pack<double>(const double*, int64_t, char*, const NbLayout&,
-                                        P4<double>*, hipStream_t);
 template hipError_t launch_real_fused<float>(const double*, int64_t, const CellGrid&, double,
-                                             double, char*, const NbLayout&, const P4<float>*,
+                                             double, char*, const NbLayout&, const P4<double>*,
                                              double*, double*, hipStream_t);
 template hipError_t launch_real_fused<double>(const double*, int64_t, const CellGrid&, double,
                                               double, char*, const NbLayout&, const P4<double>*,

@@ -9,6 +9,18 @@ namespace admp {
 
 constexpr int kRealNV = 18;               // fused real-space partial values per block
 constexpr int kPairBlocksPerFrame = 256;  // fixed grid of the pair-energy reduction
+constexpr int kSub = 8;
+constexpr int64_t kMaxCells = 1 << 21;    // cap on the cell-grid size                   // sub-counters per cell (atomic contention)
+constexpr int kRealAtomsPerBlock = kBlock / 32;
+constexpr int kCandCap = 768;     // staged candidates per pass of the cell kernel (24 KiB)
+constexpr int kHomeCap = 128;     // home atoms per pass of the cell kernel
+constexpr int kMaxStencil = 128;  // stencil entries the cell kernel supports
+constexpr int kRealSplit = 4;     // candidate splits (work-groups) per home cell
+
+// cell c holds the sorted slots [start[c * kSub], start[(c + 1) * kSub])
+inline int64_t real_fused_blocks(int64_t n) {
+  return (n + kRealAtomsPerBlock - 1) / kRealAtomsPerBlock;
+}
 
 struct CellGrid {
   double h[9];
@@ -48,13 +60,21 @@ hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const
                               const double* eta, const int64_t* pairs, const double* ds,
                               const double* bs, double kappa, double eps, double* energy,
                               double* gds, double* gq, double* partials, hipStream_t s);
-template <typename T>
-hipError_t launch_pack(const double* q, int64_t n, char* ws, const NbLayout& L, P4<T>* ps,
+hipError_t launch_pack(const double* q, int64_t n, char* ws, const NbLayout& L, P4<double>* ps,
                        hipStream_t s);
-// per-block partials of the fused real-space kernel: ceil(n / kBlock) rows of kRealNV
+// per-block partials of the fused real-space kernel: real_fused_blocks(n) rows of kRealNV
 template <typename T>
 hipError_t launch_real_fused(const double* pos, int64_t n, const CellGrid& cg, double kappa,
-                             double eps, char* ws, const NbLayout& L, const P4<T>* ps,
+                             double eps, char* ws, const NbLayout& L, const P4<double>* ps,
                              double* grad, double* partials, hipStream_t s);
+
+// cell-staged real space (grid ncell x kRealSplit): force rows gsplit[kRealSplit][n][3],
+// partials kRealSplit * ncell rows of kRealNV
+bool real_cells_ok(const CellGrid& cg);
+template <typename T>
+hipError_t launch_real_cells(const double* pos, int64_t n, const CellGrid& cg, double kappa,
+                             double eps, const char* ws, const NbLayout& L,
+                             const P4<double>* ps, double* gsplit, double* partials,
+                             hipStream_t s);
 
 }  // namespace admp

@@ -614,7 +614,7 @@ hipError_t launch_gather(const double* pos, const double* q, int64_t n, const Ge
   if (blocks < 1) blocks = 1;
   k_gather<T><<<blocks, kBlock, 0, s>>>(pos, q, n, g, phi, grad_pos, dEdq,
                                         dEdA ? partials : nullptr, accumulate);
-  if (dEdA) k_reduce_partials<<<1, kBlock, 0, s>>>(partials, blocks, 9, 1.0, dEdA, dEdA + 1);
+  if (dEdA) k_reduce_partials<<<1, kReduceThreads, 0, s>>>(partials, blocks, 9, 1.0, dEdA, dEdA + 1);
   return hipGetLastError();
 }
 
@@ -655,7 +655,7 @@ hipError_t launch_convolve(const T* mesh, const Geom& g, int conv, double kappa,
     k_fft_x_convolve<T><<<grid3, kBlock, sh, s>>>(spec, C3, r0, P, partials);
   }
   const int nb3 = grid3.x * grid3.y;
-  k_reduce_partials<<<1, kBlock, 0, s>>>(partials, nb3, 10, eps, energy, dEdA);
+  k_reduce_partials<<<1, kReduceThreads, 0, s>>>(partials, nb3, 10, eps, energy, dEdA);
   if (phi) {
     // pass 4: inverse along y
     {

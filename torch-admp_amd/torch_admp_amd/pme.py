@@ -189,14 +189,18 @@ class CoulombForceModule(BaseForceModule):
             raise ValueError("energy_forces requires kspace=True, rspace=True, slab_corr=False")
         lc = self.const_lib.length_coeff
         ec = self.const_lib.energy_coeff
-        km = self.mesh_for_box(box * lc)
+        if lc != 1.0:
+            positions, box = positions * lc, box * lc
+        km = self.mesh_for_box(box)
         self._kmesh = torch.tensor(km, dtype=torch.long)
-        e, f, gbox = pme_fused(positions * lc, box * lc, charges, self.rcut * lc,
-                               float(self.kappa), km, self.precision, self.convention,
-                               self.const_lib.dielectric, workspace)
-        self.real_energy, self.reciprocal_energy = e[0:1] / ec, e[1:2] / ec
-        self.self_energy, self.non_neutral_energy = e[2:3] / ec, e[3:4] / ec
-        return e[4] / ec, f * (lc / ec), gbox * (lc / ec)
+        e, f, gbox = pme_fused(positions, box, charges, self.rcut * lc, float(self.kappa), km,
+                               self.precision, self.convention, self.const_lib.dielectric,
+                               workspace)
+        if ec != 1.0 or lc != 1.0:
+            e, f, gbox = e / ec, f * (lc / ec), gbox * (lc / ec)
+        self.real_energy, self.reciprocal_energy = e[0:1], e[1:2]
+        self.self_energy, self.non_neutral_energy = e[2:3], e[3:4]
+        return e[4], f, gbox
 
 
 def setup_ewald_parameters(rcut: float, box: Union[torch.Tensor, np.ndarray, None] = None,
