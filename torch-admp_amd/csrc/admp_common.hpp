@@ -47,6 +47,18 @@ __host__ __device__ inline int pmod(long long m, int k) {
   return (int)(r < 0 ? r + k : r);
 }
 
+// 32-bit positive modulo (one integer division)
+__host__ __device__ inline int pmod32(int m, int k) {
+  const int r = m % k;
+  return r < 0 ? r + k : r;
+}
+
+// m >= 0 reduced into [0, k) by subtraction (m is a few k at most)
+__host__ __device__ inline int wrapk(int m, int k) {
+  while (m >= k) m -= k;
+  return m;
+}
+
 // Order-6 cardinal B-spline on the uniform knots by the Cox-de Boor
 // recursion: w[j] = M6(f + j), dw[j] = M6'(f + j) for f in [0, 1), j = 0..5.
 // M6'(u) = M5(u) - M5(u - 1).

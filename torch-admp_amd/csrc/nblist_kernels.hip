@@ -281,6 +281,40 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(const I* __restrict__ in,
   }
 }
 
+// Single-block exclusive scan for short arrays (n <= kScanSingleMax); writes
+// out[0..n] with out[n] = total.
+constexpr int kScanSingleThreads = 1024;
+constexpr int kScanSingleItems = 16;
+constexpr int64_t kScanSingleMax = (int64_t)kScanSingleThreads * kScanSingleItems;
+
+template <typename I>
+__global__ void __launch_bounds__(kScanSingleThreads) k_scan_single(const I* __restrict__ in,
+                                                                    int64_t n,
+                                                                    I* __restrict__ out) {
+  __shared__ I lds[kScanSingleThreads];
+  const int per = (int)((n + kScanSingleThreads - 1) / kScanSingleThreads);
+  const int64_t base = (int64_t)threadIdx.x * per;
+  I s = 0;
+  for (int k = 0; k < per; ++k)
+    if (base + k < n) s += in[base + k];
+  lds[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < kScanSingleThreads; off <<= 1) {
+    const I add = threadIdx.x >= off ? lds[threadIdx.x - off] : I(0);
+    __syncthreads();
+    lds[threadIdx.x] += add;
+    __syncthreads();
+  }
+  I run = lds[threadIdx.x] - s;
+  for (int k = 0; k < per; ++k)
+    if (base + k < n) {
+      const I v = in[base + k];
+      out[base + k] = run;
+      run += v;
+    }
+  if (threadIdx.x == kScanSingleThreads - 1) out[n] = lds[threadIdx.x];
+}
+
 template <typename I>
 size_t scan_scratch_count(int64_t n) {
   return (size_t)((n + kScanTile - 1) / kScanTile) + 1;
@@ -292,6 +326,10 @@ void launch_scan(const I* in, int64_t n, I* out, I* scratch, hipStream_t s) {
   const int nb = (int)((n + kScanTile - 1) / kScanTile);
   if (nb == 0) {
     k_fill<I><<<1, 64, 0, s>>>(out, 1, I(0));
+    return;
+  }
+  if (n <= kScanSingleMax) {
+    k_scan_single<I><<<1, kScanSingleThreads, 0, s>>>(in, n, out);
     return;
   }
   k_scan_sums<I><<<nb, kBlock, 0, s>>>(in, n, scratch);
@@ -559,9 +597,9 @@ void nb_layout(int64_t n, const CellGrid& cg, NbLayout* L) {
   L->w = take(sizeof(double) * 3 * (size_t)std::max<int64_t>(n, 1));
   L->cell_of = take(sizeof(int) * (size_t)std::max<int64_t>(n, 1));
   const size_t nkeys = (size_t)cg.ncell * kSub + 1;
-  L->count = take(sizeof(int) * nkeys);
+  L->count = take(sizeof(int) * 2 * nkeys);  // counts, then the scatter cursors
+  L->cursor = L->count + sizeof(int) * nkeys;
   L->start = take(sizeof(int) * nkeys);
-  L->cursor = take(sizeof(int) * nkeys);
   L->sorted = take(sizeof(int) * (size_t)std::max<int64_t>(n, 1));
   L->unsorted = take(sizeof(int) * (size_t)std::max<int64_t>(n, 1));
   L->pcount = take(sizeof(int64_t) * (size_t)(n + 1));
@@ -576,9 +614,8 @@ hipError_t launch_bin(const double* pos, int64_t n, const CellGrid& cg, char* ws
   int* count = (int*)(ws + L.count);
   int* start = (int*)(ws + L.start);
   int* cursor = (int*)(ws + L.cursor);
-  const int nfill = cg.ncell * kSub + 1;
+  const int nfill = 2 * (cg.ncell * kSub + 1);
   k_fill<int><<<std::min((nfill + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(count, nfill, 0);
-  k_fill<int><<<std::min((nfill + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(cursor, nfill, 0);
   const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
   if (n > 0)
     k_nb_bin<<<nb, kBlock, 0, s>>>(pos, n, cg, (double*)(ws + L.w), (int*)(ws + L.cell_of), count);
@@ -668,14 +705,38 @@ struct PairAcc {
   T vxx, vyy, vzz, vxy, vxz, vyz;
 };
 
+// Staged atom: position relative to the home-cell origin as an unevaluated sum
+// hi + lo of the bulk type (exact to float64 for float), plus the charge.
 template <typename T>
-__device__ __forceinline__ void pair_eval(const P4<double>& pi, const P4<double>& pj, T kappa,
+struct Staged {
+  T xh, yh, zh, q, xl, yl, zl, pad;
+};
+
+template <typename T>
+__device__ __forceinline__ Staged<T> stage(double x, double y, double z, double q) {
+  Staged<T> s;
+  s.xh = (T)x;
+  s.yh = (T)y;
+  s.zh = (T)z;
+  s.xl = (T)(x - (double)s.xh);
+  s.yl = (T)(y - (double)s.yh);
+  s.zl = (T)(z - (double)s.zh);
+  s.q = (T)q;
+  s.pad = T(0);
+  return s;
+}
+
+template <typename T>
+__device__ __forceinline__ void pair_eval(const Staged<T>& pi, const Staged<T>& pj, T kappa,
                                           T eps, T c_exp, T k2, double* facc_i, PairAcc<T>& acc) {
-  const T dx = (T)(pj.x - pi.x), dy = (T)(pj.y - pi.y), dz = (T)(pj.z - pi.z);
+  // hi parts of nearby atoms subtract exactly; the lo parts restore float64 accuracy
+  const T dx = (pj.xh - pi.xh) + (pj.xl - pi.xl);
+  const T dy = (pj.yh - pi.yh) + (pj.yl - pi.yl);
+  const T dz = (pj.zh - pi.zh) + (pj.zl - pi.zl);
   const T r2 = dx * dx + dy * dy + dz * dz;
   const T r = sqrt(r2);
   const T ir = T(1) / r;
-  const T qq = eps * (T)pi.q * (T)pj.q;
+  const T qq = eps * pi.q * pj.q;
   const T ec = erfc(kappa * r);
   const T fs = -qq * (c_exp * exp(-k2 * r2) + ec * ir) * ir * ir;  // (de/dr) / r
   acc.e += (double)(qq * ec * ir);
@@ -698,10 +759,11 @@ __global__ void __launch_bounds__(kBlock) k_real_cells(int64_t n, CellGrid cg, T
                                                        const double* __restrict__ pos,
                                                        double* __restrict__ gsplit,
                                                        double* __restrict__ partials) {
-  __shared__ P4<double> cand[kCandCap];
-  __shared__ P4<double> home[kHomeCap];
+  __shared__ Staged<T> cand[kCandCap];
+  __shared__ Staged<T> home[kHomeCap];
+  __shared__ double origin[3];
   __shared__ double facc[kHomeCap * 3];  // float64 LDS atomics in both precisions
-  __shared__ int queue[(kBlock / 64) * 128];
+  __shared__ int queue[(kBlock / 64) * kQueueCap];
   __shared__ int st_pre[kMaxStencil + 1];
   __shared__ int st_beg[kMaxStencil];
   __shared__ double st_sh[kMaxStencil * 3];
@@ -727,11 +789,32 @@ __global__ void __launch_bounds__(kBlock) k_real_cells(int64_t n, CellGrid cg, T
     for (int d = 0; d < 3; ++d) st_sh[st * 3 + d] = s0 * cg.h[d] + s1 * cg.h[3 + d] + s2 * cg.h[6 + d];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    st_pre[0] = 0;
-    for (int st = 1; st <= nst; ++st) st_pre[st] += st_pre[st - 1];
+  if (wave == 0) {
+    // inclusive prefix of the counts st_pre[1..nst] by wavefront scan, 64 at a time
+    int carry = 0;
+    for (int b0 = 1; b0 <= nst; b0 += 64) {
+      const int idx = b0 + lane;
+      int v = idx <= nst ? st_pre[idx] : 0;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+      }
+      if (idx <= nst) st_pre[idx] = v + carry;
+      carry += __shfl(v, 63, 64);
+    }
+    if (lane == 0) st_pre[0] = 0;
   }
   __syncthreads();
+  if (threadIdx.x < 3) {
+    const int d = threadIdx.x;
+    double o = 0.0;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) o += ((double)ci[e] / cg.nc[e]) * cg.h[e * 3 + d];
+    origin[d] = o;
+  }
+  __syncthreads();
+  const double ox = origin[0], oy = origin[1], oz = origin[2];
   const int ctot = st_pre[nst];
   const int cs = ctot > split ? (ctot - split + S - 1) / S : 0;
   const int hb = start[c * kSub], he = start[(c + 1) * kSub];
@@ -743,12 +826,13 @@ __global__ void __launch_bounds__(kBlock) k_real_cells(int64_t n, CellGrid cg, T
   double acc[kRealNV];
 #pragma unroll
   for (int k = 0; k < kRealNV; ++k) acc[k] = 0.0;
-  int* q = queue + wave * 128;
+  int* q = queue + wave * kQueueCap;
 
   for (int h0 = hb; h0 < he; h0 += kHomeCap) {
     const int hc = min(kHomeCap, he - h0);
     for (int t = threadIdx.x; t < hc; t += blockDim.x) {
-      home[t] = ps[h0 + t];
+      const P4<double> p = ps[h0 + t];
+      home[t] = stage<T>(p.x - ox, p.y - oy, p.z - oz, p.q);
       facc[3 * t] = facc[3 * t + 1] = facc[3 * t + 2] = 0.0;
     }
     for (int k0 = 0; k0 < cs; k0 += kCandCap) {
@@ -762,12 +846,8 @@ __global__ void __launch_bounds__(kBlock) k_real_cells(int64_t n, CellGrid cg, T
           else hi = mid;
         }
         const P4<double> p = ps[st_beg[lo] + (kg - st_pre[lo])];
-        P4<double> v;
-        v.x = p.x + st_sh[lo * 3];
-        v.y = p.y + st_sh[lo * 3 + 1];
-        v.z = p.z + st_sh[lo * 3 + 2];
-        v.q = p.q;
-        cand[kk] = v;
+        cand[kk] = stage<T>((p.x + st_sh[lo * 3]) - ox, (p.y + st_sh[lo * 3 + 1]) - oy,
+                            (p.z + st_sh[lo * 3 + 2]) - oz, p.q);
       }
       __syncthreads();
       for (int g0 = 0; g0 < hc; g0 += 64) {
@@ -775,34 +855,50 @@ __global__ void __launch_bounds__(kBlock) k_real_cells(int64_t n, CellGrid cg, T
         const int nph = 64 / hg;
         const int hl = lane % hg, ph = lane / hg;
         const bool active = ph < nph;
-        const P4<double> pi = home[g0 + hl];
+        const Staged<T> pi = home[g0 + hl];
         const int stride = nph * (kBlock / 64);
         const int nit = (cc + stride - 1) / stride;
         int qn = 0;
-        for (int j = 0; j < nit; ++j) {
-          const int kk = ph + nph * wave + stride * j;
-          bool pred = false;
-          if (active && kk < cc) {
-            const P4<double> pj = cand[kk];
-            const T dx = (T)(pj.x - pi.x), dy = (T)(pj.y - pi.y), dz = (T)(pj.z - pi.z);
-            const T r2 = dx * dx + dy * dy + dz * dz;
-            pred = r2 < rc2 && r2 > rmin2;
+        for (int j0 = 0; j0 < nit; j0 += kFilterUnroll) {
+          bool pred[kFilterUnroll];
+          int kks[kFilterUnroll];
+#pragma unroll
+          for (int u = 0; u < kFilterUnroll; ++u) {
+            const int kk = ph + nph * wave + stride * (j0 + u);
+            kks[u] = kk;
+            pred[u] = false;
+            if (active && kk < cc) {
+              // cutoff test on the hi parts only (a 1e-6 A band at rc in float32)
+              const Staged<T>& pj = cand[kk];
+              const T dx = pj.xh - pi.xh, dy = pj.yh - pi.yh, dz = pj.zh - pi.zh;
+              const T r2 = dx * dx + dy * dy + dz * dz;
+              pred[u] = r2 < rc2 && r2 > rmin2;
+            }
           }
-          const unsigned long long m = __ballot(pred);
-          if (pred) {
-            const int at = qn + __popcll(m & ((1ull << lane) - 1ull));
-            q[at] = ((g0 + hl) << 16) | kk;
+#pragma unroll
+          for (int u = 0; u < kFilterUnroll; ++u) {
+            const unsigned long long m = __ballot(pred[u]);
+            if (pred[u]) {
+              const int at = qn + __popcll(m & ((1ull << lane) - 1ull));
+              q[at] = ((g0 + hl) << 16) | kks[u];
+            }
+            qn += __popcll(m);
           }
-          qn += __popcll(m);
-          if (qn >= 64) {
+          while (qn >= 64) {
             wave_sync();
             const int ent = q[lane];
             const int hx = ent >> 16, kx = ent & 0xffff;
             pair_eval<T>(home[hx], cand[kx], kappa, eps, c_exp, k2, facc + 3 * hx, pacc);
             const int rem = qn - 64;
-            const int tail = lane < rem ? q[64 + lane] : 0;
+            // move the tail (at most kQueueCap - 64 entries) to the front
+            int tails[kFilterUnroll];
+#pragma unroll
+            for (int u = 0; u < kFilterUnroll; ++u)
+              tails[u] = (64 * u + lane < rem) ? q[64 + 64 * u + lane] : 0;
             wave_sync();
-            if (lane < rem) q[lane] = tail;
+#pragma unroll
+            for (int u = 0; u < kFilterUnroll; ++u)
+              if (64 * u + lane < rem) q[64 * u + lane] = tails[u];
             wave_sync();
             qn = rem;
           }
@@ -831,7 +927,7 @@ __global__ void __launch_bounds__(kBlock) k_real_cells(int64_t n, CellGrid cg, T
 #pragma unroll
         for (int b2 = 0; b2 < 3; ++b2) acc[7 + a2 * 3 + b2] += r[a2] * g[b2];
       if (split == 0) {
-        const double qi = home[t].q;
+        const double qi = ps[h0 + t].q;
         acc[16] += qi * qi;
         acc[17] += qi;
       }

@@ -12,10 +12,12 @@ constexpr int kPairBlocksPerFrame = 256;  // fixed grid of the pair-energy reduc
 constexpr int kSub = 8;
 constexpr int64_t kMaxCells = 1 << 21;    // cap on the cell-grid size                   // sub-counters per cell (atomic contention)
 constexpr int kRealAtomsPerBlock = kBlock / 32;
-constexpr int kCandCap = 768;     // staged candidates per pass of the cell kernel (24 KiB)
+constexpr int kCandCap = 512;     // staged candidates per pass of the cell kernel (16 KiB)
 constexpr int kHomeCap = 128;     // home atoms per pass of the cell kernel
 constexpr int kMaxStencil = 128;  // stencil entries the cell kernel supports
-constexpr int kRealSplit = 4;     // candidate splits (work-groups) per home cell
+constexpr int kRealSplit = 8;     // candidate splits (work-groups) per home cell
+constexpr int kFilterUnroll = 4;  // candidates tested per lane between queue checks
+constexpr int kQueueCap = 64 + 64 * kFilterUnroll;  // per-wavefront pair queue entries
 
 // cell c holds the sorted slots [start[c * kSub], start[(c + 1) * kSub])
 inline int64_t real_fused_blocks(int64_t n) {

@@ -29,13 +29,14 @@ __global__ void __launch_bounds__(kBlock) k_spread(const double* __restrict__ po
   bspline6<T>((T)f[2], wz, dd);
   const int K0 = g.k[0], K1 = g.k[1], K2 = g.k[2];
   const T qa = (T)q[atom] * wx[a];
-  const int ix = pmod((long long)m0[0] - 3 + a, K0);
+  const int ix = wrapk(pmod32(m0[0] - 3, K0) + a, K0);
+  const int by = pmod32(m0[1] - 3, K1), bz = pmod32(m0[2] - 3, K2);
   int iz[6];
 #pragma unroll
-  for (int c = 0; c < 6; ++c) iz[c] = pmod((long long)m0[2] - 3 + c, K2);
+  for (int c = 0; c < 6; ++c) iz[c] = wrapk(bz + c, K2);
 #pragma unroll
   for (int b = 0; b < 6; ++b) {
-    const int iy = pmod((long long)m0[1] - 3 + b, K1);
+    const int iy = wrapk(by + b, K1);
     T* row = mesh + ((int64_t)ix * K1 + iy) * K2;
     const T qab = qa * wy[b];
 #pragma unroll
@@ -74,13 +75,14 @@ __global__ void __launch_bounds__(kBlock) k_gather(const double* __restrict__ po
     bspline6<T>((T)f[2], wz, dz);
     T sx = 0, sy = 0, sz = 0, sp = 0;
     if (a < kOrder) {
-      const int ix = pmod((long long)m0[0] - 3 + a, K0);
+      const int ix = wrapk(pmod32(m0[0] - 3, K0) + a, K0);
+      const int by = pmod32(m0[1] - 3, K1), bz = pmod32(m0[2] - 3, K2);
       int iz[6];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) iz[c] = pmod((long long)m0[2] - 3 + c, K2);
+      for (int c = 0; c < 6; ++c) iz[c] = wrapk(bz + c, K2);
 #pragma unroll
       for (int b = 0; b < 6; ++b) {
-        const int iy = pmod((long long)m0[1] - 3 + b, K1);
+        const int iy = wrapk(by + b, K1);
         const T* row = phi + ((int64_t)ix * K1 + iy) * K2;
         T rz = 0, rdz = 0;
 #pragma unroll
@@ -435,6 +437,7 @@ struct InflParams {
   double phi_scale;  // multiplies 2 * eps * G
   double eps;
   int conv;          // ADMP_KCONV_*
+  int ortho;         // box diagonal: |k| of the Hermitian partner equals |k|
 };
 
 __device__ __forceinline__ int fft_m(int idx, int K) { return idx < (K + 1) / 2 ? idx : idx - K; }
@@ -445,15 +448,8 @@ __device__ __forceinline__ double theta1(int m, int K) {
   return (66.0 + 52.0 * c1 + 2.0 * c2) / 120.0;
 }
 
-// G(m) = 2 pi / V exp(-k^2 / 4 kappa^2) / (k^2 theta^2) and the factor
-// g' = dG/dk_j / k_j = -2 G (1/(4 kappa^2) + 1/k^2).
-__device__ __forceinline__ double influence(const InflParams& P, int mx, int my, int mz,
-                                            double kv[3], double* gp) {
-  if (mx == 0 && my == 0 && mz == 0) {
-    kv[0] = kv[1] = kv[2] = 0.0;
-    *gp = 0.0;
-    return 0.0;
-  }
+// k vector of integer index m: rows (k = 2 pi m A) or columns (k = 2 pi A m)
+__device__ __forceinline__ void kvec(const InflParams& P, int mx, int my, int mz, double kv[3]) {
   const double m[3] = {(double)mx, (double)my, (double)mz};
   const double* A = P.g.a;
 #pragma unroll
@@ -468,8 +464,17 @@ __device__ __forceinline__ double influence(const InflParams& P, int mx, int my,
     }
     kv[j] = 2.0 * kPi * s;
   }
+}
+
+// G = 2 pi / V exp(-k^2 / 4 kappa^2) / (k^2 theta^2) and g' = (dG/dk_j) / k_j
+// = -2 G (1/(4 kappa^2) + 1/k^2); zero at k = 0.
+__device__ __forceinline__ double gfun(const InflParams& P, const double kv[3], double th,
+                                       double* gp) {
   const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
-  const double th = theta1(mx, P.g.k[0]) * theta1(my, P.g.k[1]) * theta1(mz, P.g.k[2]);
+  if (k2 == 0.0) {
+    *gp = 0.0;
+    return 0.0;
+  }
   const double inv4k2 = 0.25 / (P.kappa * P.kappa);
   const double G = 2.0 * kPi / P.g.vol * exp(-k2 * inv4k2) / (k2 * th * th);
   *gp = -2.0 * G * (inv4k2 + 1.0 / k2);
@@ -517,20 +522,34 @@ __global__ void __launch_bounds__(kBlock) k_fft_x_convolve(cplx<T>* __restrict__
   __syncthreads();
   CT* res = stockham<T, -1>(a, b, L, C, rd, tw);
   CT* oth = (res == a) ? b : a;
+  // B-spline moduli theta(m) = prod_d theta_d(m_d) from per-dimension factors
+  double* thx = reinterpret_cast<double*>(oth);  // the idle Stockham buffer holds the x factors
+  for (int ix = threadIdx.x; ix < L; ix += blockDim.x) thx[ix] = theta1(fft_m(ix, K0), K0);
+  __syncthreads();
   double acc[10];
 #pragma unroll
   for (int i = 0; i < 10; ++i) acc[i] = 0.0;
   const int my = fft_m(iy, K1);
   const int myp = fft_m((K1 - iy) % K1, K1);
+  const double thy = theta1(my, K1);
   for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
     const int ix = idx / C, c = idx % C;
     if (c >= cmax) continue;
     const int iz = z0 + c;
     const int mx = fft_m(ix, K0), mz = fft_m(iz, K2);
     const int mxp = fft_m((K0 - ix) % K0, K0), mzp = fft_m((K2 - iz) % K2, K2);
+    const double th = thx[ix] * thy * theta1(mz, K2);
     double kv[3], kvp[3], gp, gpp;
-    const double G = influence(P, mx, my, mz, kv, &gp);
-    const double Gp = influence(P, mxp, myp, mzp, kvp, &gpp);
+    kvec(P, mx, my, mz, kv);
+    kvec(P, mxp, myp, mzp, kvp);
+    const double G = gfun(P, kv, th, &gp);
+    double Gp;
+    if (P.ortho) {
+      Gp = G;  // |k(m')| = |k(m)| for a diagonal box
+      gpp = gp;
+    } else {
+      Gp = gfun(P, kvp, th, &gpp);
+    }
     const double Gs = 0.5 * (G + Gp);
     const double w = (iz > 0 && 2 * iz != K2) ? 2.0 : 1.0;
     CT s = res[idx];
@@ -648,6 +667,8 @@ hipError_t launch_convolve(const T* mesh, const Geom& g, int conv, double kappa,
   P.phi_scale = phi_scale;
   P.eps = eps;
   P.conv = conv;
+  P.ortho = (g.h[1] == 0.0 && g.h[2] == 0.0 && g.h[3] == 0.0 && g.h[5] == 0.0 && g.h[6] == 0.0 &&
+             g.h[7] == 0.0) ? 1 : 0;
   const int C3 = pick_chunk(K0, (int64_t)K1 * Kzh, sizeof(CT));
   dim3 grid3((Kzh + C3 - 1) / C3, K1);
   {

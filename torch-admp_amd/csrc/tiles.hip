@@ -84,13 +84,14 @@ __global__ void __launch_bounds__(kBlock) k_spread_tiles(const double* __restric
     bspline6<T>((T)f[1], wy, dd);
     bspline6<T>((T)f[2], wz, dd);
     const T qa = (T)q[i] * wx[a];
-    const int lx = pmod((long long)m0[0] - 3 + a - lo0, tg.K[0]);
+    const int lx = wrapk(pmod32(m0[0] - 3 - lo0, tg.K[0]) + a, tg.K[0]);
+    const int by = pmod32(m0[1] - 3 - lo1, tg.K[1]), bz = pmod32(m0[2] - 3 - lo2, tg.K[2]);
     int ly[6], lz[6];
     bool inside = lx < W0;
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
-      ly[j] = pmod((long long)m0[1] - 3 + j - lo1, tg.K[1]);
-      lz[j] = pmod((long long)m0[2] - 3 + j - lo2, tg.K[2]);
+      ly[j] = wrapk(by + j, tg.K[1]);
+      lz[j] = wrapk(bz + j, tg.K[2]);
       inside = inside && ly[j] < W1 && lz[j] < W2;
     }
     if (inside) {
@@ -102,15 +103,15 @@ __global__ void __launch_bounds__(kBlock) k_spread_tiles(const double* __restric
         for (int cc = 0; cc < 6; ++cc) atomicAdd(row + lz[cc], (double)(qab * wz[cc]));
       }
     } else {
-      const int ix = pmod((long long)m0[0] - 3 + a, tg.K[0]);
+      const int ix = wrapk(pmod32(m0[0] - 3, tg.K[0]) + a, tg.K[0]);
+      const int gy = pmod32(m0[1] - 3, tg.K[1]), gz = pmod32(m0[2] - 3, tg.K[2]);
 #pragma unroll
       for (int bb = 0; bb < 6; ++bb) {
-        const int iy = pmod((long long)m0[1] - 3 + bb, tg.K[1]);
+        const int iy = wrapk(gy + bb, tg.K[1]);
         T* row = mesh + ((int64_t)ix * tg.K[1] + iy) * tg.K[2];
         const T qab = qa * wy[bb];
 #pragma unroll
-        for (int cc = 0; cc < 6; ++cc)
-          unsafeAtomicAdd(row + pmod((long long)m0[2] - 3 + cc, tg.K[2]), qab * wz[cc]);
+        for (int cc = 0; cc < 6; ++cc) unsafeAtomicAdd(row + wrapk(gz + cc, tg.K[2]), qab * wz[cc]);
       }
     }
   }
@@ -122,12 +123,17 @@ __global__ void __launch_bounds__(kBlock) k_spread_tiles(const double* __restric
 // (tile, offset) pairs covering mesh index x along one dimension
 __device__ __forceinline__ int cover(int x, int K, int nc, int W, int* t, int* o) {
   int n = 0;
-  for (int c = 0; c < nc; ++c)
-    for (int off = pmod((long long)x - tile_lo(K, nc, c), K); off < W && n < kMaxCover; off += K) {
+  for (int c = 0; c < nc; ++c) {
+    // x - lo(c) lies in (-K, K + 4]
+    int off = x - tile_lo(K, nc, c);
+    if (off < 0) off += K;
+    if (off >= K) off -= K;
+    for (; off < W && n < kMaxCover; off += K) {
       t[n] = c;
       o[n] = off;
       ++n;
     }
+  }
   return n;
 }
 
