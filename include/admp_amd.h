@@ -154,6 +154,57 @@ int admp_pme_fused(const double* pos, const double* q, int64_t n, const double* 
                    double* dEdbox, void* workspace, size_t ws_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Charge equilibration.  The QEq energy (src/torch_admp/qeq.py:33-209,408-458)
+ *   E(q) = chi.q + sum hardness q^2 + E_coulomb(q) + E_damping(q)
+ * is quadratic in q.  The problem below describes its Hessian H once per
+ * geometry; admp_qeq_matvec applies it (the matrix-free replacement of the
+ * autograd double backward of calc_hessian, qeq.py:460-515) and admp_qeq_solve
+ * minimises E on {A q = b} by projected, Jacobi-preconditioned conjugate
+ * gradients (replacing the solvers of qeq.py:609-1114 and the projection of
+ * src/torch_admp/utils.py:115-210) to |P grad E| / N < tol. */
+#define ADMP_QEQ_KSPACE 1   /* reciprocal, self and non-neutral Coulomb terms */
+#define ADMP_QEQ_RSPACE 2   /* real-space erfc pairs (PBC) or bare pairs (box NULL) */
+#define ADMP_QEQ_DAMPING 4  /* Gaussian-damping pairs (the per-atom term is in hdiag) */
+#define ADMP_QEQ_SLAB 8     /* slab correction along slab_axis */
+#define ADMP_QEQ_MAX_CONS 8
+
+typedef struct admp_qeq_problem {
+  const double* pos;        /* (n, 3) device */
+  int64_t n;
+  const double* box;        /* host (9) or NULL for open boundaries */
+  double cutoff;            /* real-space cutoff, sets the cell grid of the spread */
+  int32_t kmesh[3];
+  int32_t precision;        /* ADMP_F64 / ADMP_F32 for the reciprocal mesh */
+  int32_t convention;       /* ADMP_KCONV_* */
+  double kappa;
+  double dielectric;
+  int32_t flags;            /* ADMP_QEQ_* */
+  int32_t slab_axis;
+  const int64_t* pairs;     /* (n_pairs, 2) device */
+  const double* ds;         /* (n_pairs) device */
+  const double* bs;         /* (n_pairs) device, buffer scales */
+  int64_t n_pairs;
+  const double* eta;        /* (n) device, Gaussian widths (damping) */
+  const double* hdiag;      /* (n) device, diagonal terms: 2 hardness + self terms */
+  const double* chi;        /* (n) device, linear term (solve only) */
+  const double* precond;    /* (n) device Jacobi diagonal, or NULL (solve only) */
+  const double* cons;       /* (n_cons, n) device constraint matrix A (solve only) */
+  const double* cons_vals;  /* (n_cons) device right-hand side b (solve only) */
+  int32_t n_cons;
+} admp_qeq_problem;
+
+int admp_qeq_workspace_bytes(const admp_qeq_problem* problem, size_t* bytes);
+/* out = H v */
+int admp_qeq_matvec(const admp_qeq_problem* problem, const double* v, double* out,
+                    void* workspace, size_t ws_bytes, void* stream);
+/* q = argmin E on {A q = b} from the start q0 (projected onto the constraints);
+ * synchronises every check_every iterations to test the residual.  *iters is
+ * the iteration count at convergence or -1; *residual = |P grad E| / n. */
+int admp_qeq_solve(const admp_qeq_problem* problem, const double* q0, double tol,
+                   int32_t max_iter, int32_t check_every, double* q, int32_t* iters,
+                   double* residual, void* workspace, size_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Stage timing of admp_pme_fused (a profiling aid with no counterpart in the
  * reference; single-threaded use).  admp_profile_begin arms hipEvent pairs
  * around the stages of the next max_calls fused calls; admp_profile_collect

@@ -1,0 +1,86 @@
+"""GPU parity of the QEq Hessian product and the projected-CG solve against
+the float64 oracle (KKT solution).  Tolerances: matvec 1e-10 relative in
+float64; solve to |P grad E|/N < 1e-9 giving charges within 1e-7 and energy
+within 1e-9 relative (tighter than the reference's own matinv-vs-pgrad test,
+tests/test_qeq_numerical.py:209-329, which uses 1e-6 / 5e-6)."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import qeq_ref
+from test_oracle_qeq import qeq_system
+from torch_admp_amd.qeq import QEqForceModule, matinv_optimize, pgrad_optimize
+from torch_admp_amd.utils import calc_grads
+
+pytestmark = pytest.mark.gpu
+F64 = torch.float64
+
+
+def _to(dev, *xs):
+    return [None if x is None else x.to(dev) for x in xs]
+
+
+@pytest.mark.parametrize("box,slab", [(True, False), (False, False), (True, True)])
+def test_hessian_matvec(dev, box, slab):
+    pos, b, chi, hard, eta, pairs, ds, bs = qeq_system(n=60, box=box)
+    orc = qeq_ref.QEqOracle(4.5, kmesh=20, slab_corr=slab)
+    H = orc.hessian(pos, b, chi, hard, eta, pairs, ds, bs)
+    mod = QEqForceModule(4.5, kmesh=20, slab_corr=slab)
+    v = torch.tensor(np.random.default_rng(1).normal(size=pos.shape[0]))
+    hv = mod.hessian_matvec(*_to(dev, v, pos, b, chi, hard, eta, pairs, ds, bs)).cpu()
+    ref = H @ v
+    assert float((hv - ref).abs().max()) <= 1e-10 * float(ref.abs().max())
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_pgrad_solve_matches_kkt(dev, precision):
+    pos, b, chi, hard, eta, pairs, ds, bs = qeq_system(n=80, L=11.0, seed=2)
+    n = pos.shape[0]
+    orc = qeq_ref.QEqOracle(4.5, kmesh=24)
+    H = orc.hessian(pos, b, chi, hard, eta, pairs, ds, bs)
+    A = torch.ones(1, n, dtype=F64)
+    bv = torch.zeros(1, dtype=F64)
+    q_ref, _, _ = qeq_ref.kkt_solve(H, chi, A, bv)
+    e_ref = float(orc.func_energy(q_ref, pos, b, chi, hard, eta, pairs, ds, bs))
+    tol = 1e-9 if precision == "float64" else 1e-6
+    mod = QEqForceModule(4.5, kmesh=24, eps=tol, precision=precision)
+    p = pos.to(dev).requires_grad_(True)
+    args = _to(dev, b, chi, hard, eta, pairs, ds, bs, A, bv)
+    e, q = pgrad_optimize(mod, torch.zeros(n, dtype=F64, device=dev), p, *args, reinit_q=True)
+    assert mod.converge_iter >= 0 and mod.residual < tol
+    qtol, etol = (1e-7, 1e-9) if precision == "float64" else (2e-5, 1e-6)
+    assert float((q.detach().cpu() - q_ref).abs().max()) < qtol
+    assert abs(float(e.detach()) - e_ref) < etol * abs(e_ref)
+    f = -calc_grads(e, p)
+    assert torch.isfinite(f).all()
+
+
+def test_matrix_inversion_and_forces_match_oracle(dev):
+    pos, b, chi, hard, eta, pairs, ds, bs = qeq_system(n=50, seed=4)
+    n = pos.shape[0]
+    orc = qeq_ref.QEqOracle(4.5, kmesh=20)
+    H = orc.hessian(pos, b, chi, hard, eta, pairs, ds, bs)
+    A = torch.ones(1, n, dtype=F64)
+    bv = torch.zeros(1, dtype=F64)
+    q_ref, _, fermi_ref = qeq_ref.kkt_solve(H, chi, A, bv)
+    # oracle forces at fixed q*
+    pp = pos.clone().requires_grad_(True)
+    from oracle import nblist_ref
+    ds_p = nblist_ref.pairs_ds(pp, pairs, b)
+    e_ref = orc.func_energy(q_ref, pp, b, chi, hard, eta, pairs, ds_p, bs)
+    f_ref = -torch.autograd.grad(e_ref.sum(), pp)[0]
+
+    from torch_admp_amd.nblist import TorchNeighborList
+    p = pos.to(dev).requires_grad_(True)
+    nbl = TorchNeighborList(4.5)
+    pr = nbl(p, b.to(dev))
+    mod = QEqForceModule(4.5, kmesh=20)
+    e, q, hdiag, fermi = matinv_optimize(mod, p, *_to(dev, b, chi, hard, eta), pr, nbl.get_ds(),
+                                         nbl.get_buffer_scales(), *_to(dev, A, bv))
+    assert torch.allclose(hdiag.cpu(), torch.diag(H), rtol=1e-9, atol=1e-9)
+    assert float((q.detach().cpu() - q_ref).abs().max()) < 1e-9
+    assert abs(float(fermi) - float(fermi_ref)) < 1e-8
+    assert abs(float(e.detach()) - float(e_ref)) < 1e-10 * abs(float(e_ref))
+    f = -calc_grads(e, p)
+    assert float((f.cpu() - f_ref).abs().max()) < 1e-8

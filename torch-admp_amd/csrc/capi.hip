@@ -6,6 +6,7 @@
 
 #include "nblist_kernels.hpp"
 #include "pme_kernels.hpp"
+#include "qeq.hpp"
 #include "tiles.hpp"
 
 using namespace admp;
@@ -229,6 +230,52 @@ static hipError_t fused_impl(const double* pos, const double* q, int64_t n, cons
   return hipGetLastError();
 }
 
+bool qeq_problem(const admp_qeq_problem* pr, bool need_solve, QeqProblem* P) {
+  if (!pr || pr->n < 0 || pr->n_pairs < 0) return false;
+  if (pr->n > 0 && (!pr->pos || !pr->hdiag)) return false;
+  if (pr->n_pairs > 0 && (!pr->pairs || !pr->ds || !pr->bs)) return false;
+  if ((pr->flags & ADMP_QEQ_DAMPING) && pr->n_pairs > 0 && !pr->eta) return false;
+  if (pr->box && !box_ok(pr->box)) return false;
+  const bool pbc = pr->box != nullptr;
+  const bool kspace = pbc && (pr->flags & ADMP_QEQ_KSPACE);
+  if (kspace && (!kmesh_ok(pr->kmesh) || !(pr->kappa > 0.0) || !(pr->cutoff > 0.0))) return false;
+  if (pr->precision != ADMP_F32 && pr->precision != ADMP_F64) return false;
+  if (pr->convention != ADMP_KCONV_REFERENCE && pr->convention != ADMP_KCONV_COLUMNS) return false;
+  if (pr->n_cons < 0 || pr->n_cons > ADMP_QEQ_MAX_CONS) return false;
+  if (need_solve && (!pr->chi || (pr->n_cons > 0 && (!pr->cons || !pr->cons_vals)))) return false;
+  const bool slab = pbc && (pr->flags & ADMP_QEQ_SLAB);
+  if (slab && (pr->slab_axis < 0 || pr->slab_axis > 2)) return false;
+  P->pos = pr->pos;
+  P->n = pr->n;
+  P->box = pr->box;
+  P->cutoff = pr->cutoff > 0.0 ? pr->cutoff : 1.0;
+  P->kmesh = pr->kmesh;
+  P->precision = pr->precision;
+  P->convention = pr->convention;
+  P->kappa = pr->kappa;
+  P->eps = pr->dielectric;
+  P->kspace = kspace ? 1 : 0;
+  P->coul = (pr->flags & ADMP_QEQ_RSPACE) ? (pbc ? 1 : 2) : 0;
+  P->damping = (pr->flags & ADMP_QEQ_DAMPING) ? 1 : 0;
+  P->slab = slab ? 1 : 0;
+  P->slab_axis = slab ? pr->slab_axis : 0;
+  P->lz2 = 0.0;
+  if (slab)
+    for (int d = 0; d < 3; ++d) P->lz2 += pr->box[3 * pr->slab_axis + d] * pr->box[3 * pr->slab_axis + d];
+  P->pairs = pr->pairs;
+  P->ds = pr->ds;
+  P->bs = pr->bs;
+  P->n_pairs = pr->n_pairs;
+  P->eta = pr->eta;
+  P->hdiag = pr->hdiag;
+  P->chi = pr->chi;
+  P->precond = pr->precond;
+  P->cons = pr->cons;
+  P->cons_vals_dev = pr->cons_vals;
+  P->n_cons = pr->n_cons;
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -450,6 +497,45 @@ int admp_pme_gather(const double* pos, const double* q, int64_t n, const double*
                                            accumulate, partials, s));
   return hip_status(launch_gather<double>(pos, q, n, g, (const double*)phi, grad_pos, dEdq, dEdA,
                                           accumulate, partials, s));
+}
+
+// --------------------------------------------------------------------- QEq
+int admp_qeq_workspace_bytes(const admp_qeq_problem* problem, size_t* bytes) {
+  QeqProblem P;
+  if (!bytes || !qeq_problem(problem, false, &P)) return ADMP_ERR_INVALID;
+  *bytes = qeq_layout(P).bytes;
+  return ADMP_OK;
+}
+
+int admp_qeq_matvec(const admp_qeq_problem* problem, const double* v, double* out,
+                    void* workspace, size_t ws_bytes, void* stream) {
+  QeqProblem P;
+  if (!qeq_problem(problem, false, &P) || !workspace || (P.n > 0 && (!v || !out)))
+    return ADMP_ERR_INVALID;
+  if (ws_bytes < qeq_layout(P).bytes) return ADMP_ERR_INVALID;
+  if (P.n == 0) return ADMP_OK;
+  hipStream_t s = (hipStream_t)stream;
+  return P.precision == ADMP_F32 ? qeq_matvec_impl<float>(P, v, out, workspace, s)
+                                 : qeq_matvec_impl<double>(P, v, out, workspace, s);
+}
+
+int admp_qeq_solve(const admp_qeq_problem* problem, const double* q0, double tol,
+                   int32_t max_iter, int32_t check_every, double* q, int32_t* iters,
+                   double* residual, void* workspace, size_t ws_bytes, void* stream) {
+  QeqProblem P;
+  if (!qeq_problem(problem, true, &P) || !workspace || !iters || !residual || max_iter < 0 ||
+      check_every < 1 || !(tol >= 0.0) || (P.n > 0 && (!q0 || !q)))
+    return ADMP_ERR_INVALID;
+  if (ws_bytes < qeq_layout(P).bytes) return ADMP_ERR_INVALID;
+  *iters = 0;
+  *residual = 0.0;
+  if (P.n == 0) return ADMP_OK;
+  hipStream_t s = (hipStream_t)stream;
+  return P.precision == ADMP_F32
+             ? qeq_solve_impl<float>(P, q0, tol, max_iter, check_every, q, iters, residual,
+                                     workspace, s)
+             : qeq_solve_impl<double>(P, q0, tol, max_iter, check_every, q, iters, residual,
+                                      workspace, s);
 }
 
 // ------------------------------------------------------------------- fused

@@ -434,7 +434,8 @@ __global__ void __launch_bounds__(kBlock) k_pair_energy(int mode, int64_t n, int
       s = 1.0;
       ds_s = 0.0;
     } else {
-      const double eij = sqrt((etaf[i] * etaf[i] + etaf[j] * etaf[j]) * 0.5);
+      double eij = sqrt((etaf[i] * etaf[i] + etaf[j] * etaf[j]) * 0.5);
+      if (eij == 0.0) eij = 1e-10;  // src/torch_admp/qeq.py:111
       const double x = d / (2.0 * eij);
       s = -erfc(x);
       ds_s = 2.0 / kSqrtPi * exp(-x * x) / (2.0 * eij);

@@ -1,0 +1,58 @@
+// QEq Hessian-vector product and projected CG solve (see qeq.hip).
+#pragma once
+
+#include "admp_common.hpp"
+#include "nblist_kernels.hpp"
+#include "tiles.hpp"
+
+namespace admp {
+
+constexpr int kQeqMaxCons = 8;                   // constraint rows
+constexpr int kQeqMaxDots = 1 + kQeqMaxCons + 3;  // values of one k_dots pass
+
+struct QeqProblem {
+  const double* pos;
+  int64_t n;
+  const double* box;  // host, null for open boundaries
+  double cutoff;
+  const int32_t* kmesh;
+  int precision;
+  int convention;
+  double kappa;
+  double eps;
+  int kspace;
+  int coul;     // pair Coulomb: 0 none, 1 erfc (Ewald real space), 2 bare (open boundary)
+  int damping;  // Gaussian damping pairs
+  int slab;
+  int slab_axis;
+  double lz2;   // squared length of the slab-axis box vector
+  const int64_t* pairs;
+  const double* ds;
+  const double* bs;
+  int64_t n_pairs;
+  const double* eta;
+  const double* hdiag;
+  const double* chi;
+  const double* precond;
+  const double* cons;
+  const double* cons_vals_dev;
+  int n_cons;
+};
+
+struct QeqLayout {
+  size_t c, r, z, p, w, s, zc, dots, sc, status, nbws, tiles, mesh, spec, ppart, bytes;
+  CellGrid cg;
+  NbLayout nb;
+  TileGeom tg;
+};
+
+QeqLayout qeq_layout(const QeqProblem& P);
+template <typename T>
+int qeq_matvec_impl(const QeqProblem& P, const double* v, double* out, void* workspace,
+                    hipStream_t s);
+template <typename T>
+int qeq_solve_impl(const QeqProblem& P, const double* q0, double tol, int max_iter,
+                   int check_every, double* q, int* iters, double* resid, void* workspace,
+                   hipStream_t s);
+
+}  // namespace admp

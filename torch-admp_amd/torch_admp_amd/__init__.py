@@ -8,10 +8,11 @@ The compute path is the HIP library in ``_lib/libadmp_amd.so``; calls on CPU
 tensors or without the library raise.
 """
 
-from . import _lib, base_force, nblist, ops, pme, utils  # noqa: F401
+from . import _lib, base_force, nblist, ops, pme, qeq, utils  # noqa: F401
 from .base_force import BaseForceModule  # noqa: F401
 from .nblist import TorchNeighborList  # noqa: F401
 from .pme import CoulombForceModule, setup_ewald_parameters  # noqa: F401
+from .qeq import QEqForceModule, matinv_optimize, pgrad_optimize  # noqa: F401
 from .utils import calc_grads, safe_inverse  # noqa: F401
 
 __version__ = "0.1.0"
