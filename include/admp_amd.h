@@ -141,7 +141,12 @@ int admp_pme_gather(const double* pos, const double* q, int64_t n, const double*
  * positions: cell-list real space (no pair list materialised), spread, FFT,
  * influence function, gather, self and non-neutral terms.  Equivalent to
  * CoulombForceModule.forward on the TorchNeighborList pair list followed by
- * calc_grads (src/torch_admp/pme.py:261-310, src/torch_admp/utils.py:84-112).
+ * calc_grads (src/torch_admp/pme.py:261-310, src/torch_admp/utils.py:84-112)
+ * whenever every box height is at least 2 cutoff.  Below that a pair has
+ * several images inside the cutoff; this entry point sums each of them (the
+ * lattice sum, as the extended-coordinate lists of dp_nblist do,
+ * src/torch_admp/nblist.py:36-95,166-240), where the minimum-image pair
+ * distances of TorchNeighborList (nblist.py:517-551) repeat the nearest image.
  *   energy (5, device): real, reciprocal, self, non-neutral, total
  *   forces (n, 3, device): -dE/dr (overwritten)
  *   dEdbox (9, device, may be NULL): dE/dh at fixed Cartesian positions
@@ -152,6 +157,30 @@ int admp_pme_fused(const double* pos, const double* q, int64_t n, const double* 
                    double cutoff, double kappa, const int32_t* kmesh, int32_t precision,
                    int32_t convention, double dielectric, double* energy, double* forces,
                    double* dEdbox, void* workspace, size_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched form of admp_pme_fused: n_sys independent periodic systems of any
+ * mix of atom counts, boxes, cutoffs and meshes in one launch sequence (each
+ * stage is one launch over the whole batch).  Per system the outputs equal
+ * those of admp_pme_fused to float64 rounding, and they do not depend on the
+ * other systems of the batch.  The reference evaluates one frame per
+ * CoulombForceModule.forward call (src/torch_admp/pme.py:261-310); this entry
+ * point has no single counterpart there and serves callers that hold many
+ * systems at once.
+ *   host arrays: n_atoms[n_sys], boxes[n_sys][9], cutoffs[n_sys],
+ *                kappas[n_sys], kmesh[n_sys][3]
+ *   device: pos (sum n_atoms, 3) and q (sum n_atoms), systems back to back;
+ *           energy (n_sys, 5); forces (sum n_atoms, 3); dEdbox (n_sys, 9) or NULL
+ * Returns ADMP_ERR_UNSUPPORTED when a box is so small against its cutoff that
+ * the real-space stencil exceeds 128 periodic images. */
+int admp_pme_batch_workspace_bytes(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                                   const double* cutoffs, const int32_t* kmesh,
+                                   int32_t precision, size_t* bytes);
+int admp_pme_batch(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                   const double* cutoffs, const double* kappas, const int32_t* kmesh,
+                   int32_t precision, int32_t convention, double dielectric, const double* pos,
+                   const double* q, double* energy, double* forces, double* dEdbox,
+                   void* workspace, size_t ws_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Charge equilibration.  The QEq energy (src/torch_admp/qeq.py:33-209,408-458)
@@ -211,7 +240,8 @@ int admp_qeq_solve(const admp_qeq_problem* problem, const double* q0, double tol
  * synchronises, writes the summed GPU milliseconds per stage and the number of
  * calls recorded, and disarms.  Stages: 0 cell binning and packing, 1 real-space
  * pair kernel, 2 charge spreading, 3 FFT passes with the influence function,
- * 4 gather, 5 reductions and finalisation. */
+ * 4 gather, 5 reductions and finalisation.  admp_pme_batch calls are timed
+ * the same way. */
 #define ADMP_PROFILE_STAGES 6
 int admp_profile_begin(int32_t max_calls);
 int admp_profile_collect(double* stage_ms, int32_t* n_calls);

@@ -70,3 +70,29 @@ def test_cpu_tensors_fail_loudly():
     with pytest.raises(RuntimeError, match="ROCm GPU"):
         TorchNeighborList(3.0)(torch.rand(10, 3, dtype=torch.float64) * 10,
                                torch.eye(3, dtype=torch.float64) * 10)
+
+
+def test_batch_plan_on_host():
+    """Batch planning is host-only: sizes grow with the batch, invalid systems
+    are rejected and a box far below the cutoff is reported unsupported."""
+    import numpy as np
+    import torch
+
+    from torch_admp_amd import ops
+    from torch_admp_amd._lib import AdmpError
+
+    one = ops.PMEBatchPlan([300], [np.eye(3) * 20.0], 6.0, 0.4, [24, 24, 24])
+    two = ops.PMEBatchPlan([300, 500], [np.eye(3) * 20.0, np.eye(3) * 25.0], [6.0, 7.0],
+                           [0.4, 0.35], [[24, 24, 24], [30, 30, 30]], precision="float32")
+    assert two.workspace_bytes > one.workspace_bytes > 0
+    assert two.n_total == 800 and list(two.offsets) == [0, 300, 800]
+    with pytest.raises(AdmpError, match="status -1"):
+        ops.PMEBatchPlan([10], [np.zeros((3, 3))], 6.0, 0.4, [24, 24, 24])
+    with pytest.raises(AdmpError, match="status -1"):
+        ops.PMEBatchPlan([10], [np.eye(3) * 10.0], 6.0, 0.4, [0, 24, 24])
+    with pytest.raises(AdmpError, match="status -3"):
+        ops.PMEBatchPlan([10], [np.eye(3) * 3.0], 9.0, 0.4, [8, 8, 8])
+    with pytest.raises(ValueError):
+        ops.PMEBatchPlan([10, 20], [np.eye(3) * 10.0] * 2, [6.0], 0.4, [8, 8, 8])
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        one.run(torch.zeros(300, 3, dtype=torch.float64), torch.zeros(300, dtype=torch.float64))

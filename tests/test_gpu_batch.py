@@ -1,0 +1,160 @@
+"""Batched PME (admp_pme_batch) against the oracle, the single-system fused
+path and itself: each system's result must not depend on the rest of the batch.
+
+Tolerances: float64 energies within 1e-10 relative of the oracle and forces
+within 1e-8 RMSE; float32 1e-6 relative and 1e-4 eV/A RMSE.  A system alone
+and the same system inside a mixed batch agree to 1e-12 relative in float64
+(LDS float64 atomics leave only last-bit differences) and 1e-6 in float32
+(global float32 atomics of the untiled spread are order dependent).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import nblist_ref, pme_ref
+from torch_admp_amd import ops
+from torch_admp_amd.workloads import spc_water
+
+pytestmark = pytest.mark.gpu
+
+F64 = torch.float64
+
+TRICLINIC = [[12.0, 0.0, 0.0], [2.5, 13.0, 0.0], [-1.5, 1.0, 14.0]]
+BIG_TRICLINIC = [[24.0, 0.0, 0.0], [3.0, 23.0, 0.0], [-2.0, 1.5, 25.0]]
+
+
+def _system(n, box, seed):
+    rng = np.random.default_rng(seed)
+    box = np.asarray(box, dtype=np.float64)
+    pos = rng.random((n, 3)) @ box
+    q = rng.uniform(-1, 1, n)
+    if n:
+        q -= q.mean()
+    return pos, q, box
+
+
+# (n atoms, box, kmesh, kappa): one cell with images, the tiled cell grid,
+# triclinic boxes, an empty system and a box below 2 rc
+MIX = [
+    (400, np.eye(3) * 16.0, [32, 32, 32], 0.40),
+    (1200, np.eye(3) * 24.0, [32, 30, 27], 0.35),
+    (0, np.eye(3) * 12.0, [16, 16, 16], 0.40),
+    (400, TRICLINIC, [30, 32, 36], 0.45),
+    (1200, BIG_TRICLINIC, [25, 24, 40], 0.38),
+    (60, np.eye(3) * 9.0, [10, 12, 15], 0.50),
+]
+RC = 5.5
+
+
+def _mix(seed=3):
+    return [_system(n, box, seed + i) + (km, kap) for i, (n, box, km, kap) in enumerate(MIX)]
+
+
+def _run(systems, dev, precision, convention="reference"):
+    pos = [torch.tensor(s[0], device=dev) for s in systems]
+    q = [torch.tensor(s[1], device=dev) for s in systems]
+    boxes = [s[2] for s in systems]
+    return ops.pme_batch(pos, boxes, q, RC, [s[4] for s in systems], [s[3] for s in systems],
+                         precision=precision, convention=convention)
+
+
+def _oracle(pos, q, box, km, kap):
+    """Oracle energy, forces and dE/dbox.  Below 2 rc a pair has several
+    images inside the cutoff: the fused and batched paths sum all of them
+    (the lattice sum), so the oracle uses the all-image pair list there."""
+    pos, q, box = torch.tensor(pos), torch.tensor(q), torch.tensor(box)
+    orc = pme_ref.CoulombOracle(RC, 1e-5, kmesh=km, kappa=kap)
+    vol = float(torch.linalg.det(box).abs())
+    heights = [vol / float(torch.linalg.cross(box[a], box[b]).norm())
+               for a, b in ((1, 2), (2, 0), (0, 1))]
+    if min(heights) < 2 * RC:
+        pairs, _, bs = nblist_ref.multi_image_pairs(pos, box, RC)
+        ds_fn = lambda p, b: nblist_ref.multi_image_pairs(p, b, RC)[1]  # noqa: E731
+    else:
+        pairs = nblist_ref.nblist_pbc(pos, box, RC)
+        bs = nblist_ref.buffer_scales(pairs)
+        ds_fn = lambda p, b: nblist_ref.pairs_ds(p, pairs, b)  # noqa: E731
+    e, f, gb = pme_ref.energy_forces(pos, box, q, pairs, ds_fn, bs, orc)
+    return float(e), f.numpy(), gb.numpy()
+
+
+@pytest.mark.parametrize("precision,etol,ftol", [("float64", 1e-10, 1e-8),
+                                                 ("float32", 1e-6, 1e-4)])
+def test_batch_mixed_vs_oracle(dev, precision, etol, ftol):
+    systems = _mix()
+    energy, forces, dedbox = _run(systems, dev, precision)
+    assert energy.shape == (len(systems), 5)
+    for i, (pos, q, box, km, kap) in enumerate(systems):
+        e = energy[i].cpu().numpy()
+        if len(q) == 0:
+            assert np.all(e == 0.0) and forces[i].numel() == 0
+            assert float(dedbox[i].abs().max()) == 0.0
+            continue
+        e_ref, f_ref, gb_ref = _oracle(pos, q, box, km, kap)
+        assert abs(e[4] - e_ref) <= etol * abs(e_ref), (i, e[4], e_ref)
+        assert abs(e[0] + e[1] + e[2] + e[3] - e[4]) <= 1e-12 * abs(e_ref)
+        rmse = float(np.sqrt(((forces[i].cpu().numpy() - f_ref) ** 2).mean()))
+        assert rmse < ftol, (i, rmse)
+        gb = dedbox[i].cpu().numpy()
+        assert np.abs(gb - gb_ref).max() <= 100 * etol * np.abs(gb_ref).max(), i
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_batch_equals_fused(dev, precision):
+    systems = [s for s in _mix(11) if len(s[1])]
+    energy, forces, dedbox = _run(systems, dev, precision)
+    tol = 1e-12 if precision == "float64" else 2e-6
+    for i, (pos, q, box, km, kap) in enumerate(systems):
+        e1, f1, g1 = ops.pme_fused(torch.tensor(pos, device=dev), torch.tensor(box),
+                                   torch.tensor(q, device=dev), RC, kap, km, precision=precision)
+        assert torch.allclose(energy[i], e1, rtol=tol, atol=tol * float(e1.abs().max())), i
+        assert float((forces[i] - f1).abs().max()) <= tol * max(1.0, float(f1.abs().max())), i
+        assert float((dedbox[i] - g1).abs().max()) <= tol * max(1.0, float(g1.abs().max())) * 10
+
+
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-12), ("float32", 1e-6)])
+def test_batch_alone_equals_in_mix(dev, precision, tol):
+    systems = _mix(21)
+    energy, forces, dedbox = _run(systems, dev, precision)
+    for i, s in enumerate(systems):
+        e1, f1, g1 = _run([s], dev, precision)
+        scale = max(1.0, float(e1.abs().max()))
+        assert float((energy[i] - e1[0]).abs().max()) <= tol * scale, i
+        if len(s[1]):
+            assert float((forces[i] - f1[0]).abs().max()) <= tol * max(1.0, float(f1[0].abs().max()))
+        assert float((dedbox[i] - g1[0]).abs().max()) <= 10 * tol * max(1.0, float(g1[0].abs().max()))
+
+
+def test_batch_many_small_systems(dev):
+    """64 systems of 50-300 atoms and mixed meshes against the fused path."""
+    rng = np.random.default_rng(5)
+    systems = []
+    for i in range(64):
+        n = int(rng.integers(50, 300))
+        edge = float(rng.uniform(10.0, 16.0))
+        km = [int(k) for k in rng.choice([12, 15, 16, 18, 20, 24], 3)]
+        systems.append(_system(n, np.eye(3) * edge, 100 + i) + (km, float(rng.uniform(0.3, 0.5))))
+    energy, forces, dedbox = _run(systems, dev, "float64")
+    for i in (0, 17, 40, 63):
+        pos, q, box, km, kap = systems[i]
+        e1, f1, _ = ops.pme_fused(torch.tensor(pos, device=dev), torch.tensor(box),
+                                  torch.tensor(q, device=dev), RC, kap, km)
+        assert torch.allclose(energy[i], e1, rtol=1e-12, atol=1e-12 * float(e1.abs().max()))
+        assert float((forces[i] - f1).abs().max()) <= 1e-11
+
+
+def test_batch_water_copies(dev):
+    """Two copies of the 24k-atom water box in one batch give the same values
+    (a size-independent check at the full size), equal to the fused path."""
+    pos, q, box = spc_water(8000, seed=0)
+    kap, km, rc = 0.3474, [64, 64, 64], 8.0
+    pos_t = torch.tensor(pos, device=dev)
+    q_t = torch.tensor(q, device=dev)
+    energy, forces, dedbox = ops.pme_batch([pos_t, pos_t], [box, box], [q_t, q_t], rc, kap, km,
+                                           precision="float32")
+    assert float((energy[0] - energy[1]).abs().max()) <= 1e-9 * float(energy[0].abs().max())
+    assert float((forces[0] - forces[1]).abs().max()) <= 1e-5
+    e1, f1, _ = ops.pme_fused(pos_t, torch.tensor(box), q_t, rc, kap, km, precision="float32")
+    assert abs(float(energy[0, 4] - e1[4])) <= 1e-7 * abs(float(e1[4]))
+    assert float(((forces[0] - f1) ** 2).mean().sqrt()) <= 1e-5

@@ -165,4 +165,52 @@ __global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
     p[i] = v;
 }
 
+// Energies and dE/dh of one system from its reduced partial sums:
+//   rr[0..17]  real space: E, strain derivative D (6), sum r (x) g (9), sum q^2, sum q
+//   rk[0..9]   reciprocal: E, dE/dA through k (9)
+//   rg[0..8]   gather: dE/dA through s = r A (9)
+// energy[5] = real, reciprocal, self, non-neutral, total; dEdbox (may be null)
+// = A^T (D - R^T g) - A^T dA A^T - (E_recip + E_nn) A^T.
+__device__ inline void finalize_system(const double* rr, const double* rk, const double* rg,
+                                       const Geom& g, double kappa, double eps, double* energy,
+                                       double* dEdbox) {
+  const double e_real = rr[0];
+  const double e_recip = rk[0];
+  const double e_self = -eps * kappa / kSqrtPi * rr[16];
+  const double e_nn = -eps * kPi / (2.0 * g.vol * kappa * kappa) * rr[17] * rr[17];
+  energy[0] = e_real;
+  energy[1] = e_recip;
+  energy[2] = e_self;
+  energy[3] = e_nn;
+  energy[4] = e_real + e_recip + e_self + e_nn;
+  if (!dEdbox) return;
+  const double* A = g.a;
+  const double D[9] = {rr[1], rr[4], rr[5], rr[4], rr[2], rr[6], rr[5], rr[6], rr[3]};
+  double M[9], dA[9];
+  for (int i = 0; i < 9; ++i) {
+    M[i] = D[i] - rr[7 + i];
+    dA[i] = rk[1 + i] + rg[i];
+  }
+  for (int c = 0; c < 3; ++c)
+    for (int b = 0; b < 3; ++b) {
+      double gr = 0.0, gk = 0.0;
+      for (int a = 0; a < 3; ++a) gr += A[a * 3 + c] * M[a * 3 + b];
+      // -(A^T dA A^T)[c][b] = -sum_{a,e} A[a][c] dA[a][e] A[b][e]
+      for (int a = 0; a < 3; ++a)
+        for (int e = 0; e < 3; ++e) gk -= A[a * 3 + c] * dA[a * 3 + e] * A[b * 3 + e];
+      dEdbox[c * 3 + b] = gr + gk - (e_recip + e_nn) * A[b * 3 + c];
+    }
+}
+
+// out[i] = -(sum over k < nrows of g[k * m + i])
+__global__ inline void k_neg_sum_rows(const double* __restrict__ g, int nrows, int64_t m,
+                                      double* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int k = 0; k < nrows; ++k) s += g[k * m + i];
+    out[i] = -s;
+  }
+}
+
 }  // namespace admp

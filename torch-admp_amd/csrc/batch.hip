@@ -1,0 +1,518 @@
+// Batched PME energy + forces + dE/dbox for many independent periodic
+// systems of different atom counts, boxes and meshes in one launch sequence.
+//
+// Every kernel reads its system's parameters from a device table (SysDesc)
+// and finds the system of a global index (atom, cell, mesh point, FFT block,
+// gather block) by binary search over prefix offsets, so a batch of any mix
+// costs the same ~20 launches as a single system.  A single system is a
+// batch of one.  Reductions are per system and in a fixed order.
+//
+// Per system the result equals CoulombForceModule.forward on the
+// TorchNeighborList pairs followed by autograd (src/torch_admp/pme.py:261-310,
+// src/torch_admp/utils.py:84-112); boxes smaller than 2 rc use periodic images.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "batch.hpp"
+#include "fft_device.hpp"
+#include "mesh_device.hpp"
+#include "real_device.hpp"
+
+namespace admp {
+
+__device__ __forceinline__ int find_seg(const int64_t* __restrict__ off, int nseg, int64_t x) {
+  int lo = 0, hi = nseg;  // largest s with off[s] <= x
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ------------------------------------------------------------------ binning
+__global__ void __launch_bounds__(kBlock) k_b_bin(int64_t ntot, int nsys,
+                                                  const int64_t* __restrict__ atom_off,
+                                                  const SysDesc* __restrict__ tab,
+                                                  const double* __restrict__ pos,
+                                                  double* __restrict__ w, int* __restrict__ cell_of,
+                                                  int* __restrict__ sys_of,
+                                                  int* __restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntot) return;
+  const int s = find_seg(atom_off, nsys, i);
+  const CellGrid& cg = tab[s].cg;
+  const double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  double sf[3];
+  int c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double v = r[0] * cg.a[d] + r[1] * cg.a[3 + d] + r[2] * cg.a[6 + d];
+    v = v + 1e-7;
+    v = v - floor(v);
+    sf[d] = v - 1e-7;
+    const int nc = cg.nc[d];
+    int cc = (int)floor(sf[d] * nc);
+    c[d] = cc < 0 ? 0 : (cc >= nc ? nc - 1 : cc);
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    w[3 * i + j] = sf[0] * cg.h[j] + sf[1] * cg.h[3 + j] + sf[2] * cg.h[6 + j];
+  const int cid = (int)tab[s].cell0 + (c[0] * cg.nc[1] + c[1]) * cg.nc[2] + c[2];
+  cell_of[i] = cid;
+  sys_of[i] = s;
+  atomicAdd(&count[cid * kSub + (int)(i & (kSub - 1))], 1);
+}
+
+// ------------------------------------------------------------------ real space
+// One work-group per (global cell, candidate split); partial rows are
+// cell-major so that a system's rows are contiguous.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_b_real(int64_t ntot, int nsys,
+                                                   const int64_t* __restrict__ cell_off,
+                                                   const SysDesc* __restrict__ tab,
+                                                   const P4<double>* __restrict__ ps,
+                                                   const int* __restrict__ sorted,
+                                                   const int* __restrict__ start,
+                                                   const double* __restrict__ pos,
+                                                   double* __restrict__ gsplit,
+                                                   double* __restrict__ partials) {
+  const int64_t c = blockIdx.x;
+  const SysDesc* D = tab + find_seg(cell_off, nsys, c);
+  real_cell_body<T>(D->cg, D->cell0, (int)(c - D->cell0), ntot, (T)D->kappa, (T)D->eps, ps,
+                    sorted, start, pos, gsplit,
+                    partials + (c * gridDim.y + blockIdx.y) * kRealNV);
+}
+
+// ------------------------------------------------------------------ spread
+// systems whose tiles fit LDS: one work-group per cell (global cell index)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_b_spread_tiles(int nsys,
+                                                           const int64_t* __restrict__ cell_off,
+                                                           const SysDesc* __restrict__ tab,
+                                                           const double* __restrict__ pos,
+                                                           const double* __restrict__ qv,
+                                                           const int* __restrict__ sorted,
+                                                           const int* __restrict__ start,
+                                                           T* __restrict__ tiles, T* __restrict__ mesh) {
+  const int64_t c = blockIdx.x;
+  const SysDesc* D = tab + find_seg(cell_off, nsys, c);
+  if (!D->tg.ok) return;
+  const int lc = (int)(c - D->cell0);
+  spread_cell_tile<T>(D->g, D->tg, lc, start[c * kSub], start[(c + 1) * kSub], sorted, pos, qv,
+                      mesh + D->mesh0, tiles + D->tile0 + (int64_t)lc * D->tg.ntile);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_b_spread_reduce(int64_t mtot, int nsys,
+                                                            const int64_t* __restrict__ mesh_off,
+                                                            const SysDesc* __restrict__ tab,
+                                                            const T* __restrict__ tiles,
+                                                            T* __restrict__ mesh) {
+  for (int64_t gidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gidx < mtot;
+       gidx += (int64_t)gridDim.x * blockDim.x) {
+    const SysDesc* D = tab + find_seg(mesh_off, nsys, gidx);
+    if (D->tg.ok) mesh[gidx] += tile_sum<T>(D->tg, gidx - D->mesh0, tiles + D->tile0);
+  }
+}
+
+// atoms of systems whose tiles do not fit: global atomics into their mesh
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_b_spread_atoms(int64_t ntot,
+                                                           const int* __restrict__ sys_of,
+                                                           const SysDesc* __restrict__ tab,
+                                                           const double* __restrict__ pos,
+                                                           const double* __restrict__ qv,
+                                                           T* __restrict__ mesh) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t >> 3;
+  const int a = (int)(t & 7);
+  if (i >= ntot || a >= kOrder) return;
+  const SysDesc* D = tab + sys_of[i];
+  if (D->tg.ok) return;
+  spread_lane<T>(D->g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], qv[i], a, mesh + D->mesh0);
+}
+
+// ------------------------------------------------------------------ FFT passes
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_b_r2c(int nsys, const int64_t* __restrict__ off,
+                                                  const SysDesc* __restrict__ tab,
+                                                  const T* __restrict__ meshes,
+                                                  cplx<T>* __restrict__ specs) {
+  const int sid = find_seg(off, nsys, blockIdx.x);
+  const SysDesc* D = tab + sid;
+  const int C = D->fchunk[0];
+  r2c_rows<T>(meshes + D->mesh0, specs + D->spec0, (int64_t)D->g.k[0] * D->g.k[1],
+              (blockIdx.x - off[sid]) * 2 * C, D->g.k[2], C, D->rad[2]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_b_c2r(int nsys, const int64_t* __restrict__ off,
+                                                  const SysDesc* __restrict__ tab,
+                                                  const cplx<T>* __restrict__ specs,
+                                                  T* __restrict__ meshes) {
+  const int sid = find_seg(off, nsys, blockIdx.x);
+  const SysDesc* D = tab + sid;
+  const int C = D->fchunk[0];
+  c2r_rows<T>(specs + D->spec0, meshes + D->mesh0, (int64_t)D->g.k[0] * D->g.k[1],
+              (blockIdx.x - off[sid]) * 2 * C, D->g.k[2], C, D->rad[2]);
+}
+
+// lines along y: element (x, y, kz) at (x * K1 + y) * Kzh + kz
+template <typename T, int SIGN>
+__global__ void __launch_bounds__(kBlock) k_b_cols_y(int nsys, const int64_t* __restrict__ off,
+                                                     const SysDesc* __restrict__ tab,
+                                                     cplx<T>* __restrict__ specs) {
+  const int sid = find_seg(off, nsys, blockIdx.x);
+  const SysDesc* D = tab + sid;
+  const int K1 = D->g.k[1], Kzh = D->g.k[2] / 2 + 1, C = D->fchunk[1];
+  const int nchunk = (Kzh + C - 1) / C;
+  const int64_t lb = blockIdx.x - off[sid];
+  const int x = (int)(lb / nchunk), chunk = (int)(lb % nchunk);
+  strided_lines<T, SIGN>(specs + D->spec0 + (int64_t)x * K1 * Kzh + (int64_t)chunk * C, K1, Kzh,
+                         min(C, Kzh - chunk * C), C, D->rad[1]);
+}
+
+// lines along x with the influence function; per-block partials (10, times eps)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_b_cols_x(int nsys, const int64_t* __restrict__ off,
+                                                     const SysDesc* __restrict__ tab,
+                                                     cplx<T>* __restrict__ specs,
+                                                     double* __restrict__ partials) {
+  const int sid = find_seg(off, nsys, blockIdx.x);
+  const SysDesc* D = tab + sid;
+  InflParams P;
+  P.g = D->g;
+  P.kappa = D->kappa;
+  P.phi_scale = 1.0;
+  P.eps = D->eps;
+  P.conv = D->conv;
+  P.ortho = D->ortho;
+  const int Kzh = D->g.k[2] / 2 + 1, C = D->fchunk[2];
+  const int nchunk = (Kzh + C - 1) / C;
+  const int64_t lb = blockIdx.x - off[sid];
+  double* dst = partials + (int64_t)blockIdx.x * 10;
+  x_convolve_lines<T>(specs + D->spec0, (int)(lb / nchunk), (int)(lb % nchunk) * C, C, D->rad[0],
+                      P, dst);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < 10; ++i) dst[i] *= P.eps;
+}
+
+// ------------------------------------------------------------------ gather
+// kGatherAtomsPerBlock atoms of one system per work-group, 8 lanes per atom;
+// adds the reciprocal-space gradient to grad (row 0 of the split force rows)
+// and writes the block's dE/dA partials (9 values).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __restrict__ off,
+                                                     const SysDesc* __restrict__ tab,
+                                                     const double* __restrict__ pos,
+                                                     const double* __restrict__ qv,
+                                                     const T* __restrict__ meshes,
+                                                     double* __restrict__ grad,
+                                                     double* __restrict__ partials) {
+  __shared__ double red[9 * (kBlock / 64)];
+  const int sid = find_seg(off, nsys, blockIdx.x);
+  const SysDesc* D = tab + sid;
+  const Geom& g = D->g;
+  const int a = threadIdx.x & 7;
+  const int64_t atom =
+      D->atom0 + (blockIdx.x - off[sid]) * kGatherAtomsPerBlock + (threadIdx.x >> 3);
+  const bool valid = atom < D->atom0 + D->n;
+  double acc[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) acc[i] = 0.0;
+  T sx = 0, sy = 0, sz = 0, sp = 0;
+  double x = 0.0, y = 0.0, z = 0.0;
+  if (valid) {
+    x = pos[3 * atom];
+    y = pos[3 * atom + 1];
+    z = pos[3 * atom + 2];
+    gather_lane<T>(g, x, y, z, a, meshes + D->mesh0, &sx, &sy, &sz, &sp);
+  }
+#pragma unroll
+  for (int o = 1; o < 8; o <<= 1) {
+    sx += __shfl_xor(sx, o, 64);
+    sy += __shfl_xor(sy, o, 64);
+    sz += __shfl_xor(sz, o, 64);
+  }
+  if (valid && a == 0) {
+    const double qi = qv[atom];
+    const double dR[3] = {-qi * (double)sx, -qi * (double)sy, -qi * (double)sz};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double gj = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) gj += g.k[d] * g.a[j * 3 + d] * dR[d];
+      grad[3 * atom + j] += gj;
+    }
+    const double r[3] = {x, y, z};
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) acc[j * 3 + d] = dR[d] * g.k[d] * r[j];
+  }
+  block_sum<9>(acc, red);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < 9; ++i) partials[(int64_t)blockIdx.x * 9 + i] = acc[i];
+}
+
+// ------------------------------------------------------------------ reductions
+// out[s * out_stride + out_first + v] = sum over rows [off[s], off[s+1]) of p[row * nv + v]
+__global__ void __launch_bounds__(kBlock) k_b_reduce_seg(const double* __restrict__ p, int nv,
+                                                         const int64_t* __restrict__ off,
+                                                         int64_t row_mul, double* __restrict__ out,
+                                                         int out_stride, int out_first) {
+  const int s = blockIdx.x;
+  const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  const int64_t r0 = off[s] * row_mul, r1 = off[s + 1] * row_mul;
+  for (int v = threadIdx.x >> 6; v < nv; v += nw) {
+    double acc = 0.0;
+    for (int64_t r = r0 + lane; r < r1; r += 64) acc += p[r * nv + v];
+    acc = wave_sum(acc);
+    if (lane == 0) out[(int64_t)s * out_stride + out_first + v] = acc;
+  }
+}
+
+__global__ void k_b_finalize(int nsys, const SysDesc* __restrict__ tab,
+                             const double* __restrict__ red, double* __restrict__ energy,
+                             double* __restrict__ dEdbox) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  const SysDesc* D = tab + s;
+  const double* rr = red + (int64_t)s * kSysValues;
+  finalize_system(rr, rr + kRealNV, rr + kRealNV + 10, D->g, D->kappa, D->eps, energy + 5 * s,
+                  dEdbox ? dEdbox + 9 * s : nullptr);
+}
+
+// ------------------------------------------------------------------ host
+// Lines per FFT work-group: enough for about one point per thread, within the
+// LDS budget; a function of the system alone, so that a system's result does
+// not depend on the rest of the batch.
+static int pick_lines(int L, size_t elem) {
+  int C = 1;
+  while (C < 16 && L * C < kBlock) C <<= 1;
+  while (C > 1 && (size_t)(2 * L * C + L) * elem > 65536) C >>= 1;
+  return C;
+}
+
+static size_t al(size_t b) { return (b + 255) & ~size_t(255); }
+
+BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, const double* cutoffs,
+                     const double* kappas, const int32_t* kmesh, int precision, int convention,
+                     double eps) {
+  BatchPlan P;
+  P.nsys = nsys;
+  P.precision = precision;
+  P.status = ADMP_OK;
+  P.sys.resize(nsys);
+  for (int k = 0; k < kSegCount; ++k) P.off[k].assign(nsys + 1, 0);
+  const size_t es = precision == ADMP_F32 ? 4 : 8;
+  const size_t ces = 2 * es;
+  P.lds_fft[0] = P.lds_fft[1] = P.lds_fft[2] = 0;
+  P.lds_tile = 0;
+  P.any_untiled = 0;
+  for (int s = 0; s < nsys; ++s) {
+    SysDesc& D = P.sys[s];
+    std::memset(&D, 0, sizeof(D));
+    const double* box = boxes + 9 * s;
+    const int32_t* km = kmesh + 3 * s;
+    D.cg = make_grid(box, cutoffs[s], false);
+    int nst = 1;
+    for (int d = 0; d < 3; ++d) nst *= D.cg.nc[d] >= 3 ? 3 : 2 * D.cg.nimg[d] + 1;
+    if (nst > kMaxStencil) P.status = ADMP_ERR_UNSUPPORTED;
+    for (int i = 0; i < 9; ++i) {
+      D.g.h[i] = D.cg.h[i];
+      D.g.a[i] = D.cg.a[i];
+    }
+    D.g.vol = box[0] * (box[4] * box[8] - box[5] * box[7]) - box[1] * (box[3] * box[8] - box[5] * box[6]) +
+              box[2] * (box[3] * box[7] - box[4] * box[6]);
+    for (int d = 0; d < 3; ++d) {
+      D.g.k[d] = km[d];
+      D.rad[d] = factorize(km[d]);
+    }
+    D.tg = make_tile_geom(D.cg, km, es);
+    D.kappa = kappas ? kappas[s] : 0.0;
+    D.eps = eps;
+    D.conv = convention;
+    D.ortho = (box[1] == 0.0 && box[2] == 0.0 && box[3] == 0.0 && box[5] == 0.0 &&
+               box[6] == 0.0 && box[7] == 0.0) ? 1 : 0;
+    const int K0 = km[0], K1 = km[1], K2 = km[2], Kzh = K2 / 2 + 1;
+    D.fchunk[0] = pick_lines(K2, ces);
+    D.fchunk[1] = pick_lines(K1, ces);
+    D.fchunk[2] = pick_lines(K0, ces);
+    D.atom0 = P.off[kSegAtom][s];
+    D.n = n_atoms[s];
+    D.cell0 = P.off[kSegCell][s];
+    D.ncell = D.cg.ncell;
+    D.mesh0 = P.off[kSegMesh][s];
+    int64_t sp0 = s == 0 ? 0 : P.sys[s - 1].spec0 + (int64_t)P.sys[s - 1].g.k[0] * P.sys[s - 1].g.k[1] *
+                                                     (P.sys[s - 1].g.k[2] / 2 + 1);
+    D.spec0 = sp0;
+    int64_t tl0 = 0;
+    if (s > 0) {
+      const SysDesc& Pv = P.sys[s - 1];
+      tl0 = Pv.tile0 + (Pv.tg.ok ? (int64_t)Pv.ncell * Pv.tg.ntile : 0);
+    }
+    D.tile0 = tl0;
+    P.off[kSegAtom][s + 1] = D.atom0 + D.n;
+    P.off[kSegCell][s + 1] = D.cell0 + D.ncell;
+    P.off[kSegMesh][s + 1] = D.mesh0 + (int64_t)K0 * K1 * K2;
+    P.off[kSegFftZ][s + 1] = P.off[kSegFftZ][s] + ((int64_t)K0 * K1 + 2 * D.fchunk[0] - 1) / (2 * D.fchunk[0]);
+    P.off[kSegFftY][s + 1] = P.off[kSegFftY][s] + (int64_t)K0 * ((Kzh + D.fchunk[1] - 1) / D.fchunk[1]);
+    P.off[kSegFftX][s + 1] = P.off[kSegFftX][s] + (int64_t)K1 * ((Kzh + D.fchunk[2] - 1) / D.fchunk[2]);
+    P.off[kSegGather][s + 1] = P.off[kSegGather][s] + (D.n + kGatherAtomsPerBlock - 1) / kGatherAtomsPerBlock;
+    P.lds_fft[0] = std::max(P.lds_fft[0], (size_t)(K2 + 2 * K2 * D.fchunk[0]) * ces);
+    P.lds_fft[1] = std::max(P.lds_fft[1], (size_t)(K1 + 2 * K1 * D.fchunk[1]) * ces);
+    P.lds_fft[2] = std::max(P.lds_fft[2], (size_t)(K0 + 2 * K0 * D.fchunk[2]) * ces);
+    if (D.tg.ok) P.lds_tile = std::max(P.lds_tile, (size_t)D.tg.ntile * sizeof(double));
+    else P.any_untiled = 1;
+  }
+  P.ntot = P.off[kSegAtom][nsys];
+  P.ctot = P.off[kSegCell][nsys];
+  P.mtot = P.off[kSegMesh][nsys];
+  const SysDesc& L = P.sys[nsys - 1];
+  P.stot = L.spec0 + (int64_t)L.g.k[0] * L.g.k[1] * (L.g.k[2] / 2 + 1);
+  P.ttot = L.tile0 + (L.tg.ok ? (int64_t)L.ncell * L.tg.ntile : 0);
+  std::memset(&P.cg_all, 0, sizeof(P.cg_all));
+  P.cg_all.ncell = (int)P.ctot;
+  nb_layout(P.ntot, P.cg_all, &P.nb);
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    const size_t o = off;
+    off += al(b);
+    return o;
+  };
+  P.o_tab = take(sizeof(SysDesc) * nsys);
+  P.o_off = take(sizeof(int64_t) * (nsys + 1) * kSegCount);
+  P.o_nb = take(P.nb.bytes);
+  P.o_sys_of = take(sizeof(int) * std::max<int64_t>(P.ntot, 1));
+  P.o_ps = take(sizeof(P4<double>) * std::max<int64_t>(P.ntot, 1));
+  P.o_grad = take(sizeof(double) * 3 * kRealSplit * std::max<int64_t>(P.ntot, 1));
+  P.o_mesh = take(es * std::max<int64_t>(P.mtot, 1));
+  P.o_spec = take(ces * std::max<int64_t>(P.stot, 1));
+  P.o_tiles = take(es * std::max<int64_t>(P.ttot, 1));
+  P.o_rpart = take(sizeof(double) * kRealNV * kRealSplit * std::max<int64_t>(P.ctot, 1));
+  P.o_cpart = take(sizeof(double) * 10 * std::max<int64_t>(P.off[kSegFftX][nsys], 1));
+  P.o_gpart = take(sizeof(double) * 9 * std::max<int64_t>(P.off[kSegGather][nsys], 1));
+  P.o_red = take(sizeof(double) * kSysValues * nsys);
+  P.bytes = off;
+  return P;
+}
+
+static void free_host(void* p) { delete[] static_cast<char*>(p); }
+
+template <typename T>
+static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, double* energy,
+                       double* forces, double* dEdbox, char* ws, hipStream_t s, hipEvent_t* marks) {
+  using CT = cplx<T>;
+  const int nsys = P.nsys;
+  // upload the table and the offsets; the host copy lives until the stream passes it
+  const size_t tab_bytes = sizeof(SysDesc) * nsys;
+  const size_t off_bytes = sizeof(int64_t) * (nsys + 1) * kSegCount;
+  char* host = new char[tab_bytes + off_bytes];
+  std::memcpy(host, P.sys.data(), tab_bytes);
+  for (int k = 0; k < kSegCount; ++k)
+    std::memcpy(host + tab_bytes + sizeof(int64_t) * (nsys + 1) * k, P.off[k].data(),
+                sizeof(int64_t) * (nsys + 1));
+  if (hipMemcpyAsync(ws + P.o_tab, host, tab_bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(ws + P.o_off, host + tab_bytes, off_bytes, hipMemcpyHostToDevice, s) !=
+          hipSuccess) {
+    delete[] host;
+    return ADMP_ERR_HIP;
+  }
+  if (hipLaunchHostFunc(s, free_host, host) != hipSuccess) return ADMP_ERR_HIP;
+  const SysDesc* tab = (const SysDesc*)(ws + P.o_tab);
+  const int64_t* offs = (const int64_t*)(ws + P.o_off);
+  auto seg = [&](int k) { return offs + (int64_t)(nsys + 1) * k; };
+  auto mark = [&](int k) {
+    if (marks) (void)hipEventRecord(marks[k], s);
+  };
+  const int64_t ntot = P.ntot;
+  char* nbws = ws + P.o_nb;
+  const NbLayout& L = P.nb;
+  int* count = (int*)(nbws + L.count);
+  int* start = (int*)(nbws + L.start);
+  double* gsplit = (double*)(ws + P.o_grad);
+  double* red = (double*)(ws + P.o_red);
+  T* mesh = (T*)(ws + P.o_mesh);
+  CT* spec = (CT*)(ws + P.o_spec);
+  P4<double>* ps = (P4<double>*)(ws + P.o_ps);
+  const unsigned nb_atoms = (unsigned)std::max<int64_t>((ntot + kBlock - 1) / kBlock, 1);
+
+  mark(0);
+  // ---- binning
+  const int64_t nkeys = P.ctot * kSub + 1;
+  k_fill<int><<<(unsigned)std::min<int64_t>((2 * nkeys + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(
+      count, 2 * nkeys, 0);
+  if (ntot > 0)
+    k_b_bin<<<nb_atoms, kBlock, 0, s>>>(ntot, nsys, seg(kSegAtom), tab, pos, (double*)(nbws + L.w),
+                                        (int*)(nbws + L.cell_of), (int*)(ws + P.o_sys_of), count);
+  (void)launch_cell_sort(ntot, P.ctot, nbws, L, s);
+  if (ntot > 0) (void)launch_pack(q, ntot, nbws, L, ps, s);
+  mark(1);
+  // ---- real space (also for empty batches: its partial rows feed the reductions)
+  if (P.ctot > 0) {
+    dim3 grid((unsigned)P.ctot, kRealSplit);
+    k_b_real<T><<<grid, kBlock, 0, s>>>(ntot, nsys, seg(kSegCell), tab, ps,
+                                        (const int*)(nbws + L.sorted), start, pos, gsplit,
+                                        (double*)(ws + P.o_rpart));
+  }
+  mark(2);
+  // ---- spread
+  k_fill<T><<<(unsigned)std::min<int64_t>((P.mtot + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
+      mesh, P.mtot, T(0));
+  if (ntot > 0) {
+    if (P.lds_tile > 0)
+      k_b_spread_tiles<T><<<(unsigned)P.ctot, kBlock, P.lds_tile, s>>>(
+          nsys, seg(kSegCell), tab, pos, q, (const int*)(nbws + L.sorted), start,
+          (T*)(ws + P.o_tiles), mesh);
+    if (P.any_untiled)
+      k_b_spread_atoms<T><<<(unsigned)((ntot * 8 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+          ntot, (const int*)(ws + P.o_sys_of), tab, pos, q, mesh);
+    if (P.lds_tile > 0)
+      k_b_spread_reduce<T><<<(unsigned)std::min<int64_t>((P.mtot + kBlock - 1) / kBlock, 16384),
+                             kBlock, 0, s>>>(P.mtot, nsys, seg(kSegMesh), tab, (const T*)(ws + P.o_tiles),
+                                            mesh);
+  }
+  mark(3);
+  // ---- FFT, influence function, inverse FFT
+  const unsigned bz = (unsigned)P.off[kSegFftZ][nsys], by = (unsigned)P.off[kSegFftY][nsys],
+                 bx = (unsigned)P.off[kSegFftX][nsys];
+  k_b_r2c<T><<<bz, kBlock, P.lds_fft[0], s>>>(nsys, seg(kSegFftZ), tab, mesh, spec);
+  k_b_cols_y<T, -1><<<by, kBlock, P.lds_fft[1], s>>>(nsys, seg(kSegFftY), tab, spec);
+  k_b_cols_x<T><<<bx, kBlock, P.lds_fft[2], s>>>(nsys, seg(kSegFftX), tab, spec,
+                                                 (double*)(ws + P.o_cpart));
+  k_b_cols_y<T, 1><<<by, kBlock, P.lds_fft[1], s>>>(nsys, seg(kSegFftY), tab, spec);
+  k_b_c2r<T><<<bz, kBlock, P.lds_fft[0], s>>>(nsys, seg(kSegFftZ), tab, spec, mesh);
+  mark(4);
+  // ---- gather
+  const unsigned bg = (unsigned)P.off[kSegGather][nsys];
+  if (bg > 0)
+    k_b_gather<T><<<bg, kBlock, 0, s>>>(nsys, seg(kSegGather), tab, pos, q, mesh, gsplit,
+                                        (double*)(ws + P.o_gpart));
+  mark(5);
+  // ---- per-system reductions and outputs
+  k_b_reduce_seg<<<nsys, kBlock, 0, s>>>((double*)(ws + P.o_rpart), kRealNV, seg(kSegCell),
+                                          kRealSplit, red, kSysValues, 0);
+  k_b_reduce_seg<<<nsys, kBlock, 0, s>>>((double*)(ws + P.o_cpart), 10, seg(kSegFftX), 1, red,
+                                          kSysValues, kRealNV);
+  k_b_reduce_seg<<<nsys, kBlock, 0, s>>>((double*)(ws + P.o_gpart), 9, seg(kSegGather), 1, red,
+                                          kSysValues, kRealNV + 10);
+  k_b_finalize<<<(nsys + 63) / 64, 64, 0, s>>>(nsys, tab, red, energy, dEdbox);
+  if (ntot > 0)
+    k_neg_sum_rows<<<(unsigned)std::min<int64_t>((3 * ntot + kBlock - 1) / kBlock, 8192), kBlock,
+                     0, s>>>(gsplit, kRealSplit, 3 * ntot, forces);
+  mark(6);
+  return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
+}
+
+int run_batch(const BatchPlan& plan, const double* pos, const double* q, double* energy,
+              double* forces, double* dEdbox, char* ws, hipStream_t s, hipEvent_t* marks) {
+  if (plan.status != ADMP_OK) return plan.status;
+  return plan.precision == ADMP_F32
+             ? run_batch_t<float>(plan, pos, q, energy, forces, dEdbox, ws, s, marks)
+             : run_batch_t<double>(plan, pos, q, energy, forces, dEdbox, ws, s, marks);
+}
+
+}  // namespace admp

@@ -1,0 +1,61 @@
+// Batched PME engine: many independent periodic systems of different sizes,
+// boxes and meshes evaluated by one launch sequence (see batch.hip).
+#pragma once
+
+#include <vector>
+
+#include "admp_common.hpp"
+#include "nblist_kernels.hpp"
+#include "pme_kernels.hpp"
+#include "tiles.hpp"
+
+namespace admp {
+
+constexpr int kGatherAtomsPerBlock = kBlock / 8;
+constexpr int kSysValues = kRealNV + 10 + 9;  // reduced values per system
+
+// Per-system description, built on the host and read by every batched kernel.
+struct SysDesc {
+  Geom g;
+  CellGrid cg;
+  TileGeom tg;
+  Radix rad[3];
+  double kappa, eps;
+  int conv, ortho;
+  int fchunk[3];  // lines per block: z passes (row pairs), y passes, x pass
+  int64_t atom0, n;
+  int64_t cell0, ncell;
+  int64_t mesh0, spec0, tile0;
+};
+
+// Segment offsets (n_sys + 1 each) used to map a global index to its system.
+enum SegKind { kSegAtom = 0, kSegCell, kSegMesh, kSegFftZ, kSegFftY, kSegFftX, kSegGather, kSegCount };
+
+struct BatchPlan {
+  int nsys;
+  int precision;
+  std::vector<SysDesc> sys;
+  std::vector<int64_t> off[kSegCount];
+  int64_t ntot, ctot, mtot, stot, ttot;
+  size_t lds_fft[3];   // dynamic LDS of the z, y and x passes
+  size_t lds_tile;
+  int any_untiled;     // some system spreads with global atomics
+  NbLayout nb;         // binning workspace over all cells
+  CellGrid cg_all;     // only ncell is used by the shared binning helpers
+  // workspace byte offsets
+  size_t o_tab, o_off, o_nb, o_sys_of, o_ps, o_grad, o_mesh, o_spec, o_tiles, o_rpart, o_cpart,
+      o_gpart, o_red, bytes;
+  int status;          // ADMP_OK or an error code from planning
+};
+
+// Host planning of a batch; status != ADMP_OK when a system is unsupported.
+BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, const double* cutoffs,
+                     const double* kappas, const int32_t* kmesh, int precision, int convention,
+                     double eps);
+
+// Runs the whole launch sequence.  Outputs: energy (nsys x 5: real,
+// reciprocal, self, non-neutral, total), forces (ntot x 3), dEdbox (nsys x 9, may be null).
+int run_batch(const BatchPlan& plan, const double* pos, const double* q, double* energy,
+              double* forces, double* dEdbox, char* ws, hipStream_t s, hipEvent_t* marks);
+
+}  // namespace admp

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "batch.hpp"
 #include "nblist_kernels.hpp"
 #include "pme_kernels.hpp"
 #include "qeq.hpp"
@@ -101,55 +102,12 @@ FusedLayout fused_layout(int64_t n, const CellGrid& cg, const int32_t* k, int32_
   return F;
 }
 
-// Combine the reduced partial sums into energies and dE/dh on the device.
-//   red[0..17]  real space: E, D(6), sum r (x) g (9), sum q^2, sum q
-//   red[18..27] reciprocal: E, dE/dA through k (9)
-//   red[28..36] gather: dE/dA through s = r A (9)
+// Energies and dE/dh of one system from its reduced partial sums (layout in
+// finalize_system).
 __global__ void k_finalize(const double* __restrict__ red, Geom g, double kappa, double eps,
                            double* __restrict__ energy, double* __restrict__ dEdbox) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double* rr = red;
-  const double* rk = red + 18;
-  const double* rg = red + 28;
-  const double e_real = rr[0];
-  const double e_recip = rk[0];
-  const double e_self = -eps * kappa / kSqrtPi * rr[16];
-  const double e_nn = -eps * kPi / (2.0 * g.vol * kappa * kappa) * rr[17] * rr[17];
-  energy[0] = e_real;
-  energy[1] = e_recip;
-  energy[2] = e_self;
-  energy[3] = e_nn;
-  energy[4] = e_real + e_recip + e_self + e_nn;
-  if (!dEdbox) return;
-  const double* A = g.a;
-  // real: G = A^T (D - R^T g)
-  double D[9] = {rr[1], rr[4], rr[5], rr[4], rr[2], rr[6], rr[5], rr[6], rr[3]};
-  double M[9];
-  for (int i = 0; i < 9; ++i) M[i] = D[i] - rr[7 + i];
-  double dA[9];
-  for (int i = 0; i < 9; ++i) dA[i] = rk[1 + i] + rg[i];
-  for (int c = 0; c < 3; ++c)
-    for (int b = 0; b < 3; ++b) {
-      double gr = 0.0, gk = 0.0;
-      for (int a = 0; a < 3; ++a) gr += A[a * 3 + c] * M[a * 3 + b];
-      // reciprocal: -(A^T dA A^T)[c][b] = -sum_{a,e} A[a][c] dA[a][e] A[b][e]
-      for (int a = 0; a < 3; ++a)
-        for (int e = 0; e < 3; ++e) gk -= A[a * 3 + c] * dA[a * 3 + e] * A[b * 3 + e];
-      // volume terms: -E A^T for the reciprocal and non-neutral energies
-      const double at = A[b * 3 + c];
-      dEdbox[c * 3 + b] = gr + gk - (e_recip + e_nn) * at;
-    }
-}
-
-// forces = -(sum of the nsplit gradient rows)
-__global__ void k_forces(const double* __restrict__ g, int nsplit, int64_t m,
-                         double* __restrict__ out) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int k = 0; k < nsplit; ++k) s += g[k * m + i];
-    out[i] = -s;
-  }
+  finalize_system(red, red + kRealNV, red + kRealNV + 10, g, kappa, eps, energy, dEdbox);
 }
 
 // Stage timing (profiling aid): hipEvent pairs around the fused stages.
@@ -224,8 +182,8 @@ static hipError_t fused_impl(const double* pos, const double* q, int64_t n, cons
   k_reduce_partials<<<1, kReduceThreads, 0, s>>>((double*)(ws + F.rpart), nbr, kRealNV, 1.0, red, red + 1);
   k_finalize<<<1, 64, 0, s>>>(red, g, kappa, eps, energy, dEdbox);
   if (n > 0)
-    k_forces<<<(unsigned)std::min<int64_t>((3 * n + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(
-        grad, cells ? kRealSplit : 1, 3 * n, forces);
+    k_neg_sum_rows<<<(unsigned)std::min<int64_t>((3 * n + kBlock - 1) / kBlock, 4096), kBlock, 0,
+                     s>>>(grad, cells ? kRealSplit : 1, 3 * n, forces);
   pm.mark(6);
   return hipGetLastError();
 }
@@ -570,6 +528,70 @@ int admp_pme_fused(const double* pos, const double* q, int64_t n, const double* 
   return hip_status(fused_impl<double>(pos, q, n, box, cutoff, kappa, kmesh, convention,
                                        dielectric, energy, forces, dEdbox, (char*)workspace, F,
                                        cg, s));
+}
+
+// ------------------------------------------------------------------- batch
+static int batch_args_ok(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                         const double* cutoffs, const int32_t* kmesh, int32_t precision) {
+  if (n_sys < 1 || n_sys > (1 << 20) || !n_atoms || !boxes || !cutoffs || !kmesh)
+    return ADMP_ERR_INVALID;
+  if (precision != ADMP_F32 && precision != ADMP_F64) return ADMP_ERR_INVALID;
+  int64_t ntot = 0;
+  for (int s = 0; s < n_sys; ++s) {
+    if (n_atoms[s] < 0 || !box_ok(boxes + 9 * s) || !(cutoffs[s] > 0.0) ||
+        !kmesh_ok(kmesh + 3 * s))
+      return ADMP_ERR_INVALID;
+    ntot += n_atoms[s];
+  }
+  if (ntot > (int64_t)1 << 30) return ADMP_ERR_UNSUPPORTED;
+  return ADMP_OK;
+}
+
+static int batch_plan_ok(const BatchPlan& P) {
+  if (P.status != ADMP_OK) return P.status;
+  // cell ids and sorted slots are 32-bit
+  if (P.ctot * kSub + 1 >= ((int64_t)1 << 31)) return ADMP_ERR_UNSUPPORTED;
+  return ADMP_OK;
+}
+
+int admp_pme_batch_workspace_bytes(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                                   const double* cutoffs, const int32_t* kmesh,
+                                   int32_t precision, size_t* bytes) {
+  if (!bytes) return ADMP_ERR_INVALID;
+  const int st = batch_args_ok(n_sys, n_atoms, boxes, cutoffs, kmesh, precision);
+  if (st != ADMP_OK) return st;
+  const BatchPlan P = plan_batch(n_sys, n_atoms, boxes, cutoffs, nullptr, kmesh, precision,
+                                 ADMP_KCONV_REFERENCE, 1.0);
+  const int ps = batch_plan_ok(P);
+  if (ps != ADMP_OK) return ps;
+  *bytes = P.bytes;
+  return ADMP_OK;
+}
+
+int admp_pme_batch(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                   const double* cutoffs, const double* kappas, const int32_t* kmesh,
+                   int32_t precision, int32_t convention, double dielectric, const double* pos,
+                   const double* q, double* energy, double* forces, double* dEdbox,
+                   void* workspace, size_t ws_bytes, void* stream) {
+  const int st = batch_args_ok(n_sys, n_atoms, boxes, cutoffs, kmesh, precision);
+  if (st != ADMP_OK) return st;
+  if (!kappas || !energy || !workspace) return ADMP_ERR_INVALID;
+  if (convention != ADMP_KCONV_REFERENCE && convention != ADMP_KCONV_COLUMNS)
+    return ADMP_ERR_INVALID;
+  int64_t ntot = 0;
+  for (int s = 0; s < n_sys; ++s) {
+    if (!(kappas[s] > 0.0) || !std::isfinite(kappas[s])) return ADMP_ERR_INVALID;
+    ntot += n_atoms[s];
+  }
+  if (ntot > 0 && (!pos || !q || !forces)) return ADMP_ERR_INVALID;
+  const BatchPlan P = plan_batch(n_sys, n_atoms, boxes, cutoffs, kappas, kmesh, precision,
+                                 convention, dielectric);
+  const int ps = batch_plan_ok(P);
+  if (ps != ADMP_OK) return ps;
+  if (ws_bytes < P.bytes) return ADMP_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  StageMarks pm = stage_marks(s);
+  return run_batch(P, pos, q, energy, forces, dEdbox, (char*)workspace, s, pm.ev);
 }
 
 }  // extern "C"

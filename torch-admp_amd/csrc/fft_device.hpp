@@ -1,0 +1,450 @@
+// Device building blocks of the LDS Stockham FFT and the PME influence
+// function, shared by the single-system and batched kernels.
+#pragma once
+
+#include "admp_common.hpp"
+#include "pme_kernels.hpp"
+
+namespace admp {
+
+// ---------------------------------------------------------------------------
+// Stockham FFT on LDS.  C interleaved lines of length L, element e of line c
+// at buf[e * C + c].  One pass of radix r (p = product of the earlier radices,
+// n = L / r): for butterfly i, k = i mod p,
+//   u_j = x[i + j n] * w^(sign k j / (p r)),  U = DFT_r(u),  y[(i - k) r + k + j p] = U_j.
+// tw[t] = exp(-2 pi i t / L).
+
+template <typename C> __device__ __forceinline__ C mul_i(C a, int sign) {
+  // a * (sign * i)
+  C r;
+  if (sign > 0) { r.x = -a.y; r.y = a.x; }
+  else { r.x = a.y; r.y = -a.x; }
+  return r;
+}
+
+template <typename T, int SIGN, int R>
+__device__ __forceinline__ void dft_r(cplx<T>* u) {
+  using C = cplx<T>;
+  if constexpr (R == 2) {
+    const C a = u[0], b = u[1];
+    u[0] = cadd(a, b);
+    u[1] = csub(a, b);
+  } else if constexpr (R == 3) {
+    const T h = T(0.86602540378443864676);
+    const C s = cadd(u[1], u[2]);
+    const C d = csub(u[1], u[2]);
+    C m;
+    m.x = u[0].x - T(0.5) * s.x;
+    m.y = u[0].y - T(0.5) * s.y;
+    C t = mul_i(d, SIGN);
+    t.x *= h;
+    t.y *= h;
+    u[0] = cadd(u[0], s);
+    u[1] = cadd(m, t);
+    u[2] = csub(m, t);
+  } else if constexpr (R == 4) {
+    const C t0 = cadd(u[0], u[2]), t1 = csub(u[0], u[2]);
+    const C t2 = cadd(u[1], u[3]), t3 = mul_i(csub(u[1], u[3]), SIGN);
+    u[0] = cadd(t0, t2);
+    u[2] = csub(t0, t2);
+    u[1] = cadd(t1, t3);
+    u[3] = csub(t1, t3);
+  } else if constexpr (R == 5) {
+    const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+    const T s1 = T(0.95105651629515357212), s2 = T(0.58778525229247312917);
+    const C a1 = cadd(u[1], u[4]), b1 = csub(u[1], u[4]);
+    const C a2 = cadd(u[2], u[3]), b2 = csub(u[2], u[3]);
+    C p1, p2, v1, v2;
+    p1.x = u[0].x + c1 * a1.x + c2 * a2.x;
+    p1.y = u[0].y + c1 * a1.y + c2 * a2.y;
+    p2.x = u[0].x + c2 * a1.x + c1 * a2.x;
+    p2.y = u[0].y + c2 * a1.y + c1 * a2.y;
+    v1.x = s1 * b1.x + s2 * b2.x;
+    v1.y = s1 * b1.y + s2 * b2.y;
+    v2.x = s2 * b1.x - s1 * b2.x;
+    v2.y = s2 * b1.y - s1 * b2.y;
+    const C q1 = mul_i(v1, SIGN);
+    const C q2 = mul_i(v2, SIGN);
+    u[0] = cadd(u[0], cadd(a1, a2));
+    u[1] = cadd(p1, q1);
+    u[4] = csub(p1, q1);
+    u[2] = cadd(p2, q2);
+    u[3] = csub(p2, q2);
+  } else if constexpr (R == 8) {
+    C e[4] = {u[0], u[2], u[4], u[6]};
+    C o[4] = {u[1], u[3], u[5], u[7]};
+    dft_r<T, SIGN, 4>(e);
+    dft_r<T, SIGN, 4>(o);
+    const T h = T(0.70710678118654752440);
+    C w1, w3;  // exp(SIGN 2 pi i k / 8) for k = 1, 3
+    w1.x = h;
+    w1.y = SIGN * h;
+    w3.x = -h;
+    w3.y = SIGN * h;
+    const C o1 = cmul(o[1], w1);
+    const C o2 = mul_i(o[2], SIGN);
+    const C o3 = cmul(o[3], w3);
+    u[0] = cadd(e[0], o[0]);
+    u[4] = csub(e[0], o[0]);
+    u[1] = cadd(e[1], o1);
+    u[5] = csub(e[1], o1);
+    u[2] = cadd(e[2], o2);
+    u[6] = csub(e[2], o2);
+    u[3] = cadd(e[3], o3);
+    u[7] = csub(e[3], o3);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> tw_get(const cplx<T>* tw, int t, int sign) {
+  cplx<T> w = tw[t];
+  if (sign > 0) w.y = -w.y;
+  return w;
+}
+
+// One Stockham pass with a radix known at compile time (registers only).
+template <typename T, int SIGN, int R>
+__device__ __forceinline__ void pass_r(const cplx<T>* __restrict__ a, cplx<T>* __restrict__ b,
+                                       int L, int C, int p, const cplx<T>* tw) {
+  using CT = cplx<T>;
+  const int nb = L / R;
+  const int tstep = L / (p * R);
+  for (int idx = threadIdx.x; idx < nb * C; idx += blockDim.x) {
+    const int c = idx % C;
+    const int i = idx / C;
+    const int k = i % p;
+    CT u[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      CT v = a[(i + j * nb) * C + c];
+      if (j > 0 && k > 0) v = cmul(v, tw_get<T>(tw, (k * j * tstep) % L, SIGN));
+      u[j] = v;
+    }
+    dft_r<T, SIGN, R>(u);
+    const int base = (i - k) * R + k;
+#pragma unroll
+    for (int j = 0; j < R; ++j) b[(base + j * p) * C + c] = u[j];
+  }
+}
+
+// Generic prime radix: each output is a direct sum over the r inputs read
+// from LDS, O(r^2) per butterfly, no per-thread arrays.
+template <typename T, int SIGN>
+__device__ void pass_generic(const cplx<T>* __restrict__ a, cplx<T>* __restrict__ b, int L,
+                             int C, int p, int r, const cplx<T>* tw) {
+  using CT = cplx<T>;
+  const int nb = L / r;
+  const int tstep = L / (p * r);
+  const int rstep = L / r;
+  for (int idx = threadIdx.x; idx < nb * r * C; idx += blockDim.x) {
+    const int c = idx % C;
+    const int rest = idx / C;
+    const int i = rest % nb;
+    const int qq = rest / nb;
+    const int k = i % p;
+    CT s = cmk<CT>(0, 0);
+    for (int j = 0; j < r; ++j) {
+      CT v = a[(i + j * nb) * C + c];
+      // twiddle w_L^(k j tstep) times w_r^(j q) = w_L^(k j tstep + j q rstep)
+      const long long t = ((long long)k * j * tstep + (long long)j * qq * rstep) % L;
+      v = cmul(v, tw_get<T>(tw, (int)t, SIGN));
+      s = cadd(s, v);
+    }
+    b[((i - k) * r + k + qq * p) * C + c] = s;
+  }
+}
+
+template <typename T, int SIGN>
+__device__ cplx<T>* stockham(cplx<T>* a, cplx<T>* b, int L, int C, const Radix& rd,
+                             const cplx<T>* tw) {
+  int p = 1;
+  for (int s = 0; s < rd.n; ++s) {
+    const int r = rd.r[s];
+    switch (r) {
+      case 2: pass_r<T, SIGN, 2>(a, b, L, C, p, tw); break;
+      case 3: pass_r<T, SIGN, 3>(a, b, L, C, p, tw); break;
+      case 4: pass_r<T, SIGN, 4>(a, b, L, C, p, tw); break;
+      case 5: pass_r<T, SIGN, 5>(a, b, L, C, p, tw); break;
+      case 8: pass_r<T, SIGN, 8>(a, b, L, C, p, tw); break;
+      default: pass_generic<T, SIGN>(a, b, L, C, p, r, tw); break;
+    }
+    __syncthreads();
+    cplx<T>* t = a;
+    a = b;
+    b = t;
+    p *= r;
+  }
+  return a;
+}
+
+template <typename T>
+__device__ void build_twiddles(cplx<T>* tw, int L) {
+  for (int t = threadIdx.x; t < L; t += blockDim.x) {
+    double s, c;
+    sincospi(2.0 * t / L, &s, &c);
+    tw[t].x = (T)c;
+    tw[t].y = (T)(-s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 3: along x, forward FFT, influence function, energy and dE/dA
+// reduction, multiply, inverse FFT.  Block (chunk, y) holds K0 x C elements.
+struct InflParams {
+  Geom g;
+  double kappa;
+  double phi_scale;  // multiplies 2 * eps * G
+  double eps;
+  int conv;          // ADMP_KCONV_*
+  int ortho;         // box diagonal: |k| of the Hermitian partner equals |k|
+};
+
+__device__ __forceinline__ int fft_m(int idx, int K) { return idx < (K + 1) / 2 ? idx : idx - K; }
+
+__device__ __forceinline__ double theta1(int m, int K) {
+  // sum_{j=-2..2} M6(j + 3) cos(2 pi j m / K) with M6(1..5) = 1, 26, 66, 26, 1 (/120)
+  const double c1 = cospi(2.0 * m / K), c2 = cospi(4.0 * m / K);
+  return (66.0 + 52.0 * c1 + 2.0 * c2) / 120.0;
+}
+
+// k vector of integer index m: rows (k = 2 pi m A) or columns (k = 2 pi A m)
+__device__ __forceinline__ void kvec(const InflParams& P, int mx, int my, int mz, double kv[3]) {
+  const double m[3] = {(double)mx, (double)my, (double)mz};
+  const double* A = P.g.a;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    double s = 0.0;
+    if (P.conv == ADMP_KCONV_REFERENCE) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) s += m[i] * A[i * 3 + j];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) s += A[j * 3 + i] * m[i];
+    }
+    kv[j] = 2.0 * kPi * s;
+  }
+}
+
+// G = 2 pi / V exp(-k^2 / 4 kappa^2) / (k^2 theta^2) and g' = (dG/dk_j) / k_j
+// = -2 G (1/(4 kappa^2) + 1/k^2); zero at k = 0.
+__device__ __forceinline__ double gfun(const InflParams& P, const double kv[3], double th,
+                                       double* gp) {
+  const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+  if (k2 == 0.0) {
+    *gp = 0.0;
+    return 0.0;
+  }
+  const double inv4k2 = 0.25 / (P.kappa * P.kappa);
+  const double G = 2.0 * kPi / P.g.vol * exp(-k2 * inv4k2) / (k2 * th * th);
+  *gp = -2.0 * G * (inv4k2 + 1.0 / k2);
+  return G;
+}
+
+__device__ __forceinline__ void add_dEdA(double acc[10], const InflParams& P, int mx, int my,
+                                         int mz, const double kv[3], double gp, double wS2) {
+  // dE/dk_j = wS2 * gp * k_j ;  rows: dE/dA[i][j] += dE/dk_j 2 pi m_i
+  //                             columns: dE/dA[j][i] += dE/dk_j 2 pi m_i
+  const double m[3] = {(double)mx, (double)my, (double)mz};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double v = wS2 * gp * kv[j] * 2.0 * kPi * m[i];
+      if (P.conv == ADMP_KCONV_REFERENCE) acc[1 + i * 3 + j] += v;
+      else acc[1 + j * 3 + i] += v;
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Block bodies of the five FFT passes.  Each runs one work-group's lines with
+// the Stockham buffers in dynamic LDS: twiddles (L) then two L x C buffers.
+
+// Pass 1: real-to-complex along z for rows [row0, row0 + 2C).  Rows 2c and
+// 2c+1 are packed as the real and imaginary parts of one complex line; after
+// the FFT A_k = (Z_k + conj Z_{L-k}) / 2 and B_k = (Z_k - conj Z_{L-k}) / (2i).
+template <typename T>
+__device__ __forceinline__ void r2c_rows(const T* __restrict__ in, cplx<T>* __restrict__ out,
+                                         int64_t nrows, int64_t row0, int L, int C,
+                                         const Radix& rd) {
+  using CT = cplx<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  CT* tw = reinterpret_cast<CT*>(smem_raw);
+  CT* a = tw + L;
+  CT* b = a + (size_t)L * C;
+  build_twiddles<T>(tw, L);
+  for (int idx = threadIdx.x; idx < 2 * C * L; idx += blockDim.x) {
+    const int rl = idx / L, z = idx % L;
+    const int64_t row = row0 + rl;
+    const T v = row < nrows ? in[row * L + z] : T(0);
+    T* dst = reinterpret_cast<T*>(&a[z * C + (rl >> 1)]);
+    dst[rl & 1] = v;
+  }
+  __syncthreads();
+  CT* res = stockham<T, -1>(a, b, L, C, rd, tw);
+  const int Lh = L / 2 + 1;
+  for (int idx = threadIdx.x; idx < 2 * C * Lh; idx += blockDim.x) {
+    const int rl = idx / Lh, k = idx % Lh;
+    const int64_t row = row0 + rl;
+    if (row >= nrows) continue;
+    const CT zk = res[k * C + (rl >> 1)];
+    const CT zm = res[((L - k) % L) * C + (rl >> 1)];
+    CT o;
+    if ((rl & 1) == 0) {
+      o.x = T(0.5) * (zk.x + zm.x);
+      o.y = T(0.5) * (zk.y - zm.y);
+    } else {
+      o.x = T(0.5) * (zk.y + zm.y);
+      o.y = T(-0.5) * (zk.x - zm.x);
+    }
+    out[row * Lh + k] = o;
+  }
+}
+
+// Pass 5: complex-to-real along z (unnormalised inverse) for rows
+// [row0, row0 + 2C).  Z_k = A_k + i B_k with A, B extended by Hermitian
+// symmetry; the inverse transform gives z = a + i b.
+template <typename T>
+__device__ __forceinline__ void c2r_rows(const cplx<T>* __restrict__ in, T* __restrict__ out,
+                                         int64_t nrows, int64_t row0, int L, int C,
+                                         const Radix& rd) {
+  using CT = cplx<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  CT* tw = reinterpret_cast<CT*>(smem_raw);
+  CT* a = tw + L;
+  CT* b = a + (size_t)L * C;
+  build_twiddles<T>(tw, L);
+  const int Lh = L / 2 + 1;
+  for (int idx = threadIdx.x; idx < C * L; idx += blockDim.x) {
+    const int c = idx / L, k = idx % L;
+    const int64_t ra = row0 + 2 * c, rb = ra + 1;
+    const int kk = k < Lh ? k : L - k;
+    const T sg = k < Lh ? T(1) : T(-1);
+    CT A = cmk<CT>(0, 0), B = cmk<CT>(0, 0);
+    if (ra < nrows) A = in[ra * Lh + kk];
+    if (rb < nrows) B = in[rb * Lh + kk];
+    A.y *= sg;
+    B.y *= sg;
+    CT z;
+    z.x = A.x - B.y;
+    z.y = A.y + B.x;
+    a[k * C + c] = z;
+  }
+  __syncthreads();
+  CT* res = stockham<T, 1>(a, b, L, C, rd, tw);
+  for (int idx = threadIdx.x; idx < 2 * C * L; idx += blockDim.x) {
+    const int rl = idx / L, z = idx % L;
+    const int64_t row = row0 + rl;
+    if (row >= nrows) continue;
+    const T* src = reinterpret_cast<const T*>(&res[z * C + (rl >> 1)]);
+    out[row * L + z] = src[rl & 1];
+  }
+}
+
+// Passes 2 and 4: cmax (<= C) complex lines of length L along a strided axis,
+// starting at base; line elements are `stride` apart, lines are contiguous.
+template <typename T, int SIGN>
+__device__ __forceinline__ void strided_lines(cplx<T>* __restrict__ base, int L, int64_t stride,
+                                              int cmax, int C, const Radix& rd) {
+  using CT = cplx<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  CT* tw = reinterpret_cast<CT*>(smem_raw);
+  CT* a = tw + L;
+  CT* b = a + (size_t)L * C;
+  build_twiddles<T>(tw, L);
+  for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
+    const int e = idx / C, c = idx % C;
+    a[idx] = c < cmax ? base[e * stride + c] : cmk<CT>(0, 0);
+  }
+  __syncthreads();
+  CT* res = stockham<T, SIGN>(a, b, L, C, rd, tw);
+  for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
+    const int e = idx / C, c = idx % C;
+    if (c < cmax) base[e * stride + c] = res[idx];
+  }
+}
+
+// Pass 3: forward FFT along x of the lines (iy, z0 .. z0 + C), the
+// influence-function product and the inverse FFT, in LDS.  Accumulates
+// E = sum_m w |S(m)|^2 Gsym(m) and dE/dA (10 values, without the factor eps)
+// into partial_dst when it is not null.  Gsym = (G(m) + G(m')) / 2 with m'
+// the index of -m, which makes the result exact for either k convention; the
+// half-spectrum weight w is 2 for interior kz.
+template <typename T>
+__device__ __forceinline__ void x_convolve_lines(cplx<T>* __restrict__ data, int iy, int z0,
+                                                 int C, const Radix& rd, const InflParams& P,
+                                                 double* __restrict__ partial_dst) {
+  using CT = cplx<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ double red[10 * (kBlock / 64)];
+  const int K0 = P.g.k[0], K1 = P.g.k[1], K2 = P.g.k[2];
+  const int Kzh = K2 / 2 + 1;
+  const int L = K0;
+  const int64_t stride = (int64_t)K1 * Kzh;
+  CT* tw = reinterpret_cast<CT*>(smem_raw);
+  CT* a = tw + L;
+  CT* b = a + (size_t)L * C;
+  build_twiddles<T>(tw, L);
+  CT* base = data + (int64_t)iy * Kzh + z0;
+  const int cmax = min(C, Kzh - z0);
+  for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
+    const int e = idx / C, c = idx % C;
+    a[idx] = c < cmax ? base[e * stride + c] : cmk<CT>(0, 0);
+  }
+  __syncthreads();
+  CT* res = stockham<T, -1>(a, b, L, C, rd, tw);
+  CT* oth = (res == a) ? b : a;
+  // B-spline moduli theta(m) = prod_d theta_d(m_d) from per-dimension factors
+  double* thx = reinterpret_cast<double*>(oth);  // the idle Stockham buffer holds the x factors
+  for (int ix = threadIdx.x; ix < L; ix += blockDim.x) thx[ix] = theta1(fft_m(ix, K0), K0);
+  __syncthreads();
+  double acc[10];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) acc[i] = 0.0;
+  const int my = fft_m(iy, K1);
+  const int myp = fft_m((K1 - iy) % K1, K1);
+  const double thy = theta1(my, K1);
+  for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
+    const int ix = idx / C, c = idx % C;
+    if (c >= cmax) continue;
+    const int iz = z0 + c;
+    const int mx = fft_m(ix, K0), mz = fft_m(iz, K2);
+    const int mxp = fft_m((K0 - ix) % K0, K0), mzp = fft_m((K2 - iz) % K2, K2);
+    const double th = thx[ix] * thy * theta1(mz, K2);
+    double kv[3], kvp[3], gp, gpp;
+    kvec(P, mx, my, mz, kv);
+    kvec(P, mxp, myp, mzp, kvp);
+    const double G = gfun(P, kv, th, &gp);
+    double Gp;
+    if (P.ortho) {
+      Gp = G;  // |k(m')| = |k(m)| for a diagonal box
+      gpp = gp;
+    } else {
+      Gp = gfun(P, kvp, th, &gpp);
+    }
+    const double Gs = 0.5 * (G + Gp);
+    const double w = (iz > 0 && 2 * iz != K2) ? 2.0 : 1.0;
+    CT s = res[idx];
+    const double S2 = (double)s.x * (double)s.x + (double)s.y * (double)s.y;
+    acc[0] += w * Gs * S2;
+    add_dEdA(acc, P, mx, my, mz, kv, gp, 0.5 * w * S2);
+    add_dEdA(acc, P, mxp, myp, mzp, kvp, gpp, 0.5 * w * S2);
+    const T f = (T)(2.0 * P.eps * P.phi_scale * Gs);
+    s.x *= f;
+    s.y *= f;
+    res[idx] = s;
+  }
+  __syncthreads();
+  CT* out = stockham<T, 1>(res, oth, L, C, rd, tw);
+  for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
+    const int e = idx / C, c = idx % C;
+    if (c < cmax) base[e * stride + c] = out[idx];
+  }
+  if (partial_dst) {
+    block_sum<10>(acc, red);
+    if (threadIdx.x == 0)
+      for (int i = 0; i < 10; ++i) partial_dst[i] = acc[i];
+  }
+}
+
+}  // namespace admp

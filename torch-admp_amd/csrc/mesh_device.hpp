@@ -1,0 +1,197 @@
+// Device building blocks of charge spreading and force gathering, shared by
+// the single-system and batched kernels.
+#pragma once
+
+#include "admp_common.hpp"
+#include "tiles.hpp"
+
+namespace admp {
+
+__host__ __device__ inline int tile_lo(int K, int nc, int c) {
+  return (int)(((long long)K * c + nc - 1) / nc) - 4;
+}
+
+// One lane's share of spreading one atom with global atomics:
+// Q[(m0 - 3 + a, m0 - 3 + b, m0 - 3 + c) mod K] += q wx[a] wy[b] wz[c] for the
+// lane's x offset a < 6 (36 adds).
+template <typename T>
+__device__ __forceinline__ void spread_lane(const Geom& g, double x, double y, double z, double q,
+                                            int a, T* __restrict__ mesh) {
+  int m0[3];
+  double f[3];
+  mesh_ref(g, x, y, z, m0, f);
+  T wx[6], wy[6], wz[6], dd[6];
+  bspline6<T>((T)f[0], wx, dd);
+  bspline6<T>((T)f[1], wy, dd);
+  bspline6<T>((T)f[2], wz, dd);
+  const int K0 = g.k[0], K1 = g.k[1], K2 = g.k[2];
+  const T qa = (T)q * wx[a];
+  const int ix = wrapk(pmod32(m0[0] - 3, K0) + a, K0);
+  const int by = pmod32(m0[1] - 3, K1), bz = pmod32(m0[2] - 3, K2);
+  int iz[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) iz[c] = wrapk(bz + c, K2);
+#pragma unroll
+  for (int b = 0; b < 6; ++b) {
+    const int iy = wrapk(by + b, K1);
+    T* row = mesh + ((int64_t)ix * K1 + iy) * K2;
+    const T qab = qa * wy[b];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) unsafeAtomicAdd(row + iz[c], qab * wz[c]);
+  }
+}
+
+// One work-group spreads the atoms of cell lc (sorted slots [b, e)) into a
+// float64 LDS tile (dynamic LDS, tg.ntile doubles) and writes the tile to
+// out[tg.ntile].  Atoms whose stencil leaves the tile add to the mesh with
+// global atomics instead.
+template <typename T>
+__device__ __forceinline__ void spread_cell_tile(const Geom& g, const TileGeom& tg, int lc, int b,
+                                                 int e, const int* __restrict__ sorted,
+                                                 const double* __restrict__ pos,
+                                                 const double* __restrict__ q,
+                                                 T* __restrict__ mesh, T* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* tile = reinterpret_cast<double*>(smem_raw);  // float64 LDS atomics in both precisions
+  const int W0 = tg.W[0], W1 = tg.W[1], W2 = tg.W[2];
+  for (int k = threadIdx.x; k < tg.ntile; k += blockDim.x) tile[k] = 0.0;
+  __syncthreads();
+  const int cz = lc % tg.nc[2];
+  const int cy = (lc / tg.nc[2]) % tg.nc[1];
+  const int cx = lc / (tg.nc[1] * tg.nc[2]);
+  const int lo0 = tile_lo(tg.K[0], tg.nc[0], cx);
+  const int lo1 = tile_lo(tg.K[1], tg.nc[1], cy);
+  const int lo2 = tile_lo(tg.K[2], tg.nc[2], cz);
+  const int a = threadIdx.x & 7;
+  for (int slot = b + (threadIdx.x >> 3); slot < e; slot += blockDim.x >> 3) {
+    if (a >= kOrder) continue;
+    const int i = sorted[slot];
+    int m0[3];
+    double f[3];
+    mesh_ref(g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], m0, f);
+    T wx[6], wy[6], wz[6], dd[6];
+    bspline6<T>((T)f[0], wx, dd);
+    bspline6<T>((T)f[1], wy, dd);
+    bspline6<T>((T)f[2], wz, dd);
+    const T qa = (T)q[i] * wx[a];
+    const int lx = wrapk(pmod32(m0[0] - 3 - lo0, tg.K[0]) + a, tg.K[0]);
+    const int by = pmod32(m0[1] - 3 - lo1, tg.K[1]), bz = pmod32(m0[2] - 3 - lo2, tg.K[2]);
+    int ly[6], lz[6];
+    bool inside = lx < W0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      ly[j] = wrapk(by + j, tg.K[1]);
+      lz[j] = wrapk(bz + j, tg.K[2]);
+      inside = inside && ly[j] < W1 && lz[j] < W2;
+    }
+    if (inside) {
+#pragma unroll
+      for (int bb = 0; bb < 6; ++bb) {
+        double* row = tile + (lx * W1 + ly[bb]) * W2;
+        const T qab = qa * wy[bb];
+#pragma unroll
+        for (int cc = 0; cc < 6; ++cc) atomicAdd(row + lz[cc], (double)(qab * wz[cc]));
+      }
+    } else {
+      const int ix = wrapk(pmod32(m0[0] - 3, tg.K[0]) + a, tg.K[0]);
+      const int gy = pmod32(m0[1] - 3, tg.K[1]), gz = pmod32(m0[2] - 3, tg.K[2]);
+#pragma unroll
+      for (int bb = 0; bb < 6; ++bb) {
+        const int iy = wrapk(gy + bb, tg.K[1]);
+        T* row = mesh + ((int64_t)ix * tg.K[1] + iy) * tg.K[2];
+        const T qab = qa * wy[bb];
+#pragma unroll
+        for (int cc = 0; cc < 6; ++cc) unsafeAtomicAdd(row + wrapk(gz + cc, tg.K[2]), qab * wz[cc]);
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < tg.ntile; k += blockDim.x) out[k] = (T)tile[k];
+}
+
+// (tile, offset) pairs covering mesh index x along one dimension
+__device__ __forceinline__ int tile_cover(int x, int K, int nc, int W, int* t, int* o) {
+  int n = 0;
+  for (int c = 0; c < nc; ++c) {
+    // x - lo(c) lies in (-K, K + 4]
+    int off = x - tile_lo(K, nc, c);
+    if (off < 0) off += K;
+    if (off >= K) off -= K;
+    for (; off < W && n < kMaxCover; off += K) {
+      t[n] = c;
+      o[n] = off;
+      ++n;
+    }
+  }
+  return n;
+}
+
+// Sum of the tiles that cover mesh point lp (row-major index), in a fixed order.
+template <typename T>
+__device__ __forceinline__ T tile_sum(const TileGeom& tg, int64_t lp, const T* __restrict__ tiles) {
+  const int gz = (int)(lp % tg.K[2]);
+  const int gy = (int)((lp / tg.K[2]) % tg.K[1]);
+  const int gx = (int)(lp / ((int64_t)tg.K[1] * tg.K[2]));
+  int tx[kMaxCover], ox[kMaxCover], ty[kMaxCover], oy[kMaxCover], tz[kMaxCover], oz[kMaxCover];
+  const int nx = tile_cover(gx, tg.K[0], tg.nc[0], tg.W[0], tx, ox);
+  const int ny = tile_cover(gy, tg.K[1], tg.nc[1], tg.W[1], ty, oy);
+  const int nz = tile_cover(gz, tg.K[2], tg.nc[2], tg.W[2], tz, oz);
+  T s = T(0);
+  for (int a = 0; a < nx; ++a)
+    for (int b = 0; b < ny; ++b) {
+      const int64_t cab = ((int64_t)tx[a] * tg.nc[1] + ty[b]) * tg.nc[2];
+      const int oab = (ox[a] * tg.W[1] + oy[b]) * tg.W[2];
+      for (int c = 0; c < nz; ++c) s += tiles[(cab + tz[c]) * tg.ntile + oab + oz[c]];
+    }
+  return s;
+}
+
+// One lane's share of the gather for one atom (lane a < 6 owns x offset a);
+// the caller sums the outputs over the atom's 8 lanes.  With phi = dE/dQ:
+//   sx, sy, sz = sum_g phi(g) dtheta/du_d (u = f + j, du/dR = -1)
+//   sp        = sum_g phi(g) theta(g)
+template <typename T>
+__device__ __forceinline__ void gather_lane(const Geom& g, double x, double y, double z, int a,
+                                            const T* __restrict__ phi, T* sx, T* sy, T* sz, T* sp) {
+  int m0[3];
+  double f[3];
+  mesh_ref(g, x, y, z, m0, f);
+  T wx[6], wy[6], wz[6], dx[6], dy[6], dz[6];
+  bspline6<T>((T)f[0], wx, dx);
+  bspline6<T>((T)f[1], wy, dy);
+  bspline6<T>((T)f[2], wz, dz);
+  T ax = 0, ay = 0, az = 0, ap = 0;
+  if (a < kOrder) {
+    const int K0 = g.k[0], K1 = g.k[1], K2 = g.k[2];
+    const int ix = wrapk(pmod32(m0[0] - 3, K0) + a, K0);
+    const int by = pmod32(m0[1] - 3, K1), bz = pmod32(m0[2] - 3, K2);
+    int iz[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) iz[c] = wrapk(bz + c, K2);
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+      const int iy = wrapk(by + b, K1);
+      const T* row = phi + ((int64_t)ix * K1 + iy) * K2;
+      T rz = 0, rdz = 0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const T p = row[iz[c]];
+        rz += p * wz[c];
+        rdz += p * dz[c];
+      }
+      ap += wy[b] * rz;
+      ay += dy[b] * rz;
+      az += wy[b] * rdz;
+    }
+    ax = dx[a] * ap;
+    ay *= wx[a];
+    az *= wx[a];
+    ap *= wx[a];
+  }
+  *sx = ax;
+  *sy = ay;
+  *sz = az;
+  *sp = ap;
+}
+
+}  // namespace admp

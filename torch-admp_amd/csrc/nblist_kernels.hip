@@ -5,6 +5,7 @@
 // src/torch_admp/spatial.py:15-36) and the P-sized torch chains of
 // src/torch_admp/pme.py:312-349, 529-561 and src/torch_admp/qeq.py:106-121.
 #include "nblist_kernels.hpp"
+#include "real_device.hpp"
 
 namespace admp {
 
@@ -71,25 +72,6 @@ CellGrid make_grid(const double* box, double cutoff, bool obc) {
   }
   cg.ncell = (int)ncell;
   return cg;
-}
-
-// Stencil entries along one dimension: cell index and image shift.
-__device__ __forceinline__ int stencil_len(const CellGrid& cg, int d) {
-  return cg.nc[d] >= 3 ? 3 : 2 * cg.nimg[d] + 1;
-}
-__device__ __forceinline__ void stencil_at(const CellGrid& cg, int d, int ci, int o, int* cell,
-                                           int* shift) {
-  if (cg.nc[d] >= 3) {
-    int c = ci + o - 1;
-    int s = 0;
-    if (c < 0) { c += cg.nc[d]; s = -1; }
-    else if (c >= cg.nc[d]) { c -= cg.nc[d]; s = 1; }
-    *cell = c;
-    *shift = s;
-  } else {
-    *cell = 0;
-    *shift = o - cg.nimg[d];
-  }
 }
 
 // Bin: wrapped fractional s = (r A + 1e-7) mod 1 - 1e-7 and wrapped
@@ -613,15 +595,21 @@ void nb_layout(int64_t n, const CellGrid& cg, NbLayout* L) {
 hipError_t launch_bin(const double* pos, int64_t n, const CellGrid& cg, char* ws,
                       const NbLayout& L, hipStream_t s) {
   int* count = (int*)(ws + L.count);
-  int* start = (int*)(ws + L.start);
-  int* cursor = (int*)(ws + L.cursor);
   const int nfill = 2 * (cg.ncell * kSub + 1);
   k_fill<int><<<std::min((nfill + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(count, nfill, 0);
   const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
   if (n > 0)
     k_nb_bin<<<nb, kBlock, 0, s>>>(pos, n, cg, (double*)(ws + L.w), (int*)(ws + L.cell_of), count);
-  launch_scan<int>(count, (int64_t)cg.ncell * kSub, start, (int*)(ws + L.scan), s);
+  return launch_cell_sort(n, cg.ncell, ws, L, s);
+}
+
+hipError_t launch_cell_sort(int64_t n, int64_t ncell, char* ws, const NbLayout& L, hipStream_t s) {
+  int* count = (int*)(ws + L.count);
+  int* start = (int*)(ws + L.start);
+  int* cursor = (int*)(ws + L.cursor);
+  launch_scan<int>(count, ncell * kSub, start, (int*)(ws + L.scan), s);
   if (n > 0) {
+    const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
     k_nb_scatter<<<nb, kBlock, 0, s>>>(n, (int*)(ws + L.cell_of), start, cursor,
                                        (int*)(ws + L.unsorted));
     k_nb_rank<<<nb, kBlock, 0, s>>>(n, (int*)(ws + L.cell_of), start, (int*)(ws + L.unsorted),
@@ -694,64 +682,6 @@ hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const
 // arithmetic always runs on 64 useful lanes.  Forces of the home atoms gather
 // in LDS (atomics) and leave as one row per split: gsplit[s][i][3].
 // Per-block partials as in k_real_fused (kRealNV values).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <typename T>
-struct PairAcc {
-  double e;
-  T vxx, vyy, vzz, vxy, vxz, vyz;
-};
-
-// Staged atom: position relative to the home-cell origin as an unevaluated sum
-// hi + lo of the bulk type (exact to float64 for float), plus the charge.
-template <typename T>
-struct Staged {
-  T xh, yh, zh, q, xl, yl, zl, pad;
-};
-
-template <typename T>
-__device__ __forceinline__ Staged<T> stage(double x, double y, double z, double q) {
-  Staged<T> s;
-  s.xh = (T)x;
-  s.yh = (T)y;
-  s.zh = (T)z;
-  s.xl = (T)(x - (double)s.xh);
-  s.yl = (T)(y - (double)s.yh);
-  s.zl = (T)(z - (double)s.zh);
-  s.q = (T)q;
-  s.pad = T(0);
-  return s;
-}
-
-template <typename T>
-__device__ __forceinline__ void pair_eval(const Staged<T>& pi, const Staged<T>& pj, T kappa,
-                                          T eps, T c_exp, T k2, double* facc_i, PairAcc<T>& acc) {
-  // hi parts of nearby atoms subtract exactly; the lo parts restore float64 accuracy
-  const T dx = (pj.xh - pi.xh) + (pj.xl - pi.xl);
-  const T dy = (pj.yh - pi.yh) + (pj.yl - pi.yl);
-  const T dz = (pj.zh - pi.zh) + (pj.zl - pi.zl);
-  const T r2 = dx * dx + dy * dy + dz * dz;
-  const T r = sqrt(r2);
-  const T ir = T(1) / r;
-  const T qq = eps * pi.q * pj.q;
-  const T ec = erfc(kappa * r);
-  const T fs = -qq * (c_exp * exp(-k2 * r2) + ec * ir) * ir * ir;  // (de/dr) / r
-  acc.e += (double)(qq * ec * ir);
-  atomicAdd(facc_i + 0, (double)(-fs * dx));
-  atomicAdd(facc_i + 1, (double)(-fs * dy));
-  atomicAdd(facc_i + 2, (double)(-fs * dz));
-  acc.vxx += fs * dx * dx;
-  acc.vyy += fs * dy * dy;
-  acc.vzz += fs * dz * dz;
-  acc.vxy += fs * dx * dy;
-  acc.vxz += fs * dx * dz;
-  acc.vyz += fs * dy * dz;
-}
-
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_real_cells(int64_t n, CellGrid cg, T kappa, T eps,
                                                        const P4<double>* __restrict__ ps,
@@ -760,192 +690,8 @@ __global__ void __launch_bounds__(kBlock) k_real_cells(int64_t n, CellGrid cg, T
                                                        const double* __restrict__ pos,
                                                        double* __restrict__ gsplit,
                                                        double* __restrict__ partials) {
-  __shared__ Staged<T> cand[kCandCap];
-  __shared__ Staged<T> home[kHomeCap];
-  __shared__ double origin[3];
-  __shared__ double facc[kHomeCap * 3];  // float64 LDS atomics in both precisions
-  __shared__ int queue[(kBlock / 64) * kQueueCap];
-  __shared__ int st_pre[kMaxStencil + 1];
-  __shared__ int st_beg[kMaxStencil];
-  __shared__ double st_sh[kMaxStencil * 3];
-  __shared__ double red[kRealNV * (kBlock / 64)];
-
-  const int c = blockIdx.x;
-  const int split = blockIdx.y, S = gridDim.y;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ci[3] = {c / (cg.nc[1] * cg.nc[2]), (c / cg.nc[2]) % cg.nc[1], c % cg.nc[2]};
-  const int l0 = stencil_len(cg, 0), l1 = stencil_len(cg, 1), l2 = stencil_len(cg, 2);
-  const int nst = l0 * l1 * l2;
-  if (threadIdx.x < nst) {
-    const int st = threadIdx.x;
-    const int o2 = st % l2, o1 = (st / l2) % l1, o0 = st / (l1 * l2);
-    int c0, s0, c1, s1, c2, s2;
-    stencil_at(cg, 0, ci[0], o0, &c0, &s0);
-    stencil_at(cg, 1, ci[1], o1, &c1, &s1);
-    stencil_at(cg, 2, ci[2], o2, &c2, &s2);
-    const int cc = (c0 * cg.nc[1] + c1) * cg.nc[2] + c2;
-    st_beg[st] = start[cc * kSub];
-    st_pre[st + 1] = start[(cc + 1) * kSub] - start[cc * kSub];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) st_sh[st * 3 + d] = s0 * cg.h[d] + s1 * cg.h[3 + d] + s2 * cg.h[6 + d];
-  }
-  __syncthreads();
-  if (wave == 0) {
-    // inclusive prefix of the counts st_pre[1..nst] by wavefront scan, 64 at a time
-    int carry = 0;
-    for (int b0 = 1; b0 <= nst; b0 += 64) {
-      const int idx = b0 + lane;
-      int v = idx <= nst ? st_pre[idx] : 0;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int u = __shfl_up(v, off, 64);
-        if (lane >= off) v += u;
-      }
-      if (idx <= nst) st_pre[idx] = v + carry;
-      carry += __shfl(v, 63, 64);
-    }
-    if (lane == 0) st_pre[0] = 0;
-  }
-  __syncthreads();
-  if (threadIdx.x < 3) {
-    const int d = threadIdx.x;
-    double o = 0.0;
-#pragma unroll
-    for (int e = 0; e < 3; ++e) o += ((double)ci[e] / cg.nc[e]) * cg.h[e * 3 + d];
-    origin[d] = o;
-  }
-  __syncthreads();
-  const double ox = origin[0], oy = origin[1], oz = origin[2];
-  const int ctot = st_pre[nst];
-  const int cs = ctot > split ? (ctot - split + S - 1) / S : 0;
-  const int hb = start[c * kSub], he = start[(c + 1) * kSub];
-  const T rc2 = (T)cg.rc2, rmin2 = (T)1e-4;
-  const T c_exp = (T)(2.0 / kSqrtPi) * kappa, k2 = kappa * kappa;
-  PairAcc<T> pacc;
-  pacc.e = 0.0;
-  pacc.vxx = pacc.vyy = pacc.vzz = pacc.vxy = pacc.vxz = pacc.vyz = T(0);
-  double acc[kRealNV];
-#pragma unroll
-  for (int k = 0; k < kRealNV; ++k) acc[k] = 0.0;
-  int* q = queue + wave * kQueueCap;
-
-  for (int h0 = hb; h0 < he; h0 += kHomeCap) {
-    const int hc = min(kHomeCap, he - h0);
-    for (int t = threadIdx.x; t < hc; t += blockDim.x) {
-      const P4<double> p = ps[h0 + t];
-      home[t] = stage<T>(p.x - ox, p.y - oy, p.z - oz, p.q);
-      facc[3 * t] = facc[3 * t + 1] = facc[3 * t + 2] = 0.0;
-    }
-    for (int k0 = 0; k0 < cs; k0 += kCandCap) {
-      const int cc = min(kCandCap, cs - k0);
-      for (int kk = threadIdx.x; kk < cc; kk += blockDim.x) {
-        const int kg = split + S * (k0 + kk);
-        int lo = 0, hi = nst;  // last st with st_pre[st] <= kg
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (st_pre[mid] <= kg) lo = mid;
-          else hi = mid;
-        }
-        const P4<double> p = ps[st_beg[lo] + (kg - st_pre[lo])];
-        cand[kk] = stage<T>((p.x + st_sh[lo * 3]) - ox, (p.y + st_sh[lo * 3 + 1]) - oy,
-                            (p.z + st_sh[lo * 3 + 2]) - oz, p.q);
-      }
-      __syncthreads();
-      for (int g0 = 0; g0 < hc; g0 += 64) {
-        const int hg = min(64, hc - g0);
-        const int nph = 64 / hg;
-        const int hl = lane % hg, ph = lane / hg;
-        const bool active = ph < nph;
-        const Staged<T> pi = home[g0 + hl];
-        const int stride = nph * (kBlock / 64);
-        const int nit = (cc + stride - 1) / stride;
-        int qn = 0;
-        for (int j0 = 0; j0 < nit; j0 += kFilterUnroll) {
-          bool pred[kFilterUnroll];
-          int kks[kFilterUnroll];
-#pragma unroll
-          for (int u = 0; u < kFilterUnroll; ++u) {
-            const int kk = ph + nph * wave + stride * (j0 + u);
-            kks[u] = kk;
-            pred[u] = false;
-            if (active && kk < cc) {
-              // cutoff test on the hi parts only (a 1e-6 A band at rc in float32)
-              const Staged<T>& pj = cand[kk];
-              const T dx = pj.xh - pi.xh, dy = pj.yh - pi.yh, dz = pj.zh - pi.zh;
-              const T r2 = dx * dx + dy * dy + dz * dz;
-              pred[u] = r2 < rc2 && r2 > rmin2;
-            }
-          }
-#pragma unroll
-          for (int u = 0; u < kFilterUnroll; ++u) {
-            const unsigned long long m = __ballot(pred[u]);
-            if (pred[u]) {
-              const int at = qn + __popcll(m & ((1ull << lane) - 1ull));
-              q[at] = ((g0 + hl) << 16) | kks[u];
-            }
-            qn += __popcll(m);
-          }
-          while (qn >= 64) {
-            wave_sync();
-            const int ent = q[lane];
-            const int hx = ent >> 16, kx = ent & 0xffff;
-            pair_eval<T>(home[hx], cand[kx], kappa, eps, c_exp, k2, facc + 3 * hx, pacc);
-            const int rem = qn - 64;
-            // move the tail (at most kQueueCap - 64 entries) to the front
-            int tails[kFilterUnroll];
-#pragma unroll
-            for (int u = 0; u < kFilterUnroll; ++u)
-              tails[u] = (64 * u + lane < rem) ? q[64 + 64 * u + lane] : 0;
-            wave_sync();
-#pragma unroll
-            for (int u = 0; u < kFilterUnroll; ++u)
-              if (64 * u + lane < rem) q[64 * u + lane] = tails[u];
-            wave_sync();
-            qn = rem;
-          }
-        }
-        wave_sync();
-        if (lane < qn) {
-          const int ent = q[lane];
-          const int hx = ent >> 16, kx = ent & 0xffff;
-          pair_eval<T>(home[hx], cand[kx], kappa, eps, c_exp, k2, facc + 3 * hx, pacc);
-        }
-        wave_sync();
-      }
-      __syncthreads();
-    }
-    // home forces of this split
-    for (int t = threadIdx.x; t < hc; t += blockDim.x) {
-      const int i = sorted[h0 + t];
-      const double g[3] = {facc[3 * t], facc[3 * t + 1], facc[3 * t + 2]};
-      double* dst = gsplit + ((int64_t)split * n + i) * 3;
-      dst[0] = g[0];
-      dst[1] = g[1];
-      dst[2] = g[2];
-      const double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-#pragma unroll
-      for (int a2 = 0; a2 < 3; ++a2)
-#pragma unroll
-        for (int b2 = 0; b2 < 3; ++b2) acc[7 + a2 * 3 + b2] += r[a2] * g[b2];
-      if (split == 0) {
-        const double qi = ps[h0 + t].q;
-        acc[16] += qi * qi;
-        acc[17] += qi;
-      }
-    }
-    __syncthreads();
-  }
-  acc[0] = 0.5 * pacc.e;
-  acc[1] = 0.5 * (double)pacc.vxx;
-  acc[2] = 0.5 * (double)pacc.vyy;
-  acc[3] = 0.5 * (double)pacc.vzz;
-  acc[4] = 0.5 * (double)pacc.vxy;
-  acc[5] = 0.5 * (double)pacc.vxz;
-  acc[6] = 0.5 * (double)pacc.vyz;
-  block_sum<kRealNV>(acc, red);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < kRealNV; ++k)
-      partials[((int64_t)split * gridDim.x + blockIdx.x) * kRealNV + k] = acc[k];
+  real_cell_body<T>(cg, 0, blockIdx.x, n, kappa, eps, ps, sorted, start, pos, gsplit,
+                    partials + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * kRealNV);
 }
 
 bool real_cells_ok(const CellGrid& cg) {

@@ -9,8 +9,8 @@ namespace admp {
 
 constexpr int kRealNV = 18;               // fused real-space partial values per block
 constexpr int kPairBlocksPerFrame = 256;  // fixed grid of the pair-energy reduction
-constexpr int kSub = 8;
-constexpr int64_t kMaxCells = 1 << 21;    // cap on the cell-grid size                   // sub-counters per cell (atomic contention)
+constexpr int kSub = 8;                   // sub-counters per cell (atomic contention)
+constexpr int64_t kMaxCells = 1 << 21;    // cap on the cell-grid size
 constexpr int kRealAtomsPerBlock = kBlock / 32;
 constexpr int kCandCap = 512;     // staged candidates per pass of the cell kernel (16 KiB)
 constexpr int kHomeCap = 128;     // home atoms per pass of the cell kernel
@@ -49,6 +49,8 @@ void nb_layout(int64_t n, const CellGrid& cg, NbLayout* L);
 
 hipError_t launch_bin(const double* pos, int64_t n, const CellGrid& cg, char* ws,
                       const NbLayout& L, hipStream_t s);
+// counting sort of binned atoms (cell_of and counts filled): start, sorted
+hipError_t launch_cell_sort(int64_t n, int64_t ncell, char* ws, const NbLayout& L, hipStream_t s);
 // mode 0: count pairs and scan the per-atom counts (total at offsets[n]);
 // mode 1: write the pairs
 hipError_t launch_pairs(int mode, int64_t n, const CellGrid& cg, char* ws, const NbLayout& L,

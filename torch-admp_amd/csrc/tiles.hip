@@ -17,12 +17,9 @@
 #include "nblist_kernels.hpp"
 #include "pme_kernels.hpp"
 #include "tiles.hpp"
+#include "mesh_device.hpp"
 
 namespace admp {
-
-__host__ __device__ inline int tile_lo(int K, int nc, int c) {
-  return (int)(((long long)K * c + nc - 1) / nc) - 4;
-}
 
 TileGeom make_tile_geom(const CellGrid& cg, const int32_t* kmesh, size_t elem_bytes) {
   (void)elem_bytes;  // the LDS tile is float64 in both precisions
@@ -59,82 +56,9 @@ __global__ void __launch_bounds__(kBlock) k_spread_tiles(const double* __restric
                                                          const int* __restrict__ start,
                                                          T* __restrict__ tiles,
                                                          T* __restrict__ mesh) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  double* tile = reinterpret_cast<double*>(smem_raw);  // float64 LDS atomics in both precisions
   const int c = blockIdx.x;
-  const int W0 = tg.W[0], W1 = tg.W[1], W2 = tg.W[2];
-  for (int k = threadIdx.x; k < tg.ntile; k += blockDim.x) tile[k] = 0.0;
-  __syncthreads();
-  const int cz = c % tg.nc[2];
-  const int cy = (c / tg.nc[2]) % tg.nc[1];
-  const int cx = c / (tg.nc[1] * tg.nc[2]);
-  const int lo0 = tile_lo(tg.K[0], tg.nc[0], cx);
-  const int lo1 = tile_lo(tg.K[1], tg.nc[1], cy);
-  const int lo2 = tile_lo(tg.K[2], tg.nc[2], cz);
-  const int b = start[c * kSub], e = start[(c + 1) * kSub];
-  const int a = threadIdx.x & 7;
-  for (int slot = b + (threadIdx.x >> 3); slot < e; slot += blockDim.x >> 3) {
-    if (a >= kOrder) continue;
-    const int i = sorted[slot];
-    int m0[3];
-    double f[3];
-    mesh_ref(g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], m0, f);
-    T wx[6], wy[6], wz[6], dd[6];
-    bspline6<T>((T)f[0], wx, dd);
-    bspline6<T>((T)f[1], wy, dd);
-    bspline6<T>((T)f[2], wz, dd);
-    const T qa = (T)q[i] * wx[a];
-    const int lx = wrapk(pmod32(m0[0] - 3 - lo0, tg.K[0]) + a, tg.K[0]);
-    const int by = pmod32(m0[1] - 3 - lo1, tg.K[1]), bz = pmod32(m0[2] - 3 - lo2, tg.K[2]);
-    int ly[6], lz[6];
-    bool inside = lx < W0;
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      ly[j] = wrapk(by + j, tg.K[1]);
-      lz[j] = wrapk(bz + j, tg.K[2]);
-      inside = inside && ly[j] < W1 && lz[j] < W2;
-    }
-    if (inside) {
-#pragma unroll
-      for (int bb = 0; bb < 6; ++bb) {
-        double* row = tile + (lx * W1 + ly[bb]) * W2;
-        const T qab = qa * wy[bb];
-#pragma unroll
-        for (int cc = 0; cc < 6; ++cc) atomicAdd(row + lz[cc], (double)(qab * wz[cc]));
-      }
-    } else {
-      const int ix = wrapk(pmod32(m0[0] - 3, tg.K[0]) + a, tg.K[0]);
-      const int gy = pmod32(m0[1] - 3, tg.K[1]), gz = pmod32(m0[2] - 3, tg.K[2]);
-#pragma unroll
-      for (int bb = 0; bb < 6; ++bb) {
-        const int iy = wrapk(gy + bb, tg.K[1]);
-        T* row = mesh + ((int64_t)ix * tg.K[1] + iy) * tg.K[2];
-        const T qab = qa * wy[bb];
-#pragma unroll
-        for (int cc = 0; cc < 6; ++cc) unsafeAtomicAdd(row + wrapk(gz + cc, tg.K[2]), qab * wz[cc]);
-      }
-    }
-  }
-  __syncthreads();
-  T* out = tiles + (int64_t)c * tg.ntile;
-  for (int k = threadIdx.x; k < tg.ntile; k += blockDim.x) out[k] = (T)tile[k];
-}
-
-// (tile, offset) pairs covering mesh index x along one dimension
-__device__ __forceinline__ int cover(int x, int K, int nc, int W, int* t, int* o) {
-  int n = 0;
-  for (int c = 0; c < nc; ++c) {
-    // x - lo(c) lies in (-K, K + 4]
-    int off = x - tile_lo(K, nc, c);
-    if (off < 0) off += K;
-    if (off >= K) off -= K;
-    for (; off < W && n < kMaxCover; off += K) {
-      t[n] = c;
-      o[n] = off;
-      ++n;
-    }
-  }
-  return n;
+  spread_cell_tile<T>(g, tg, c, start[c * kSub], start[(c + 1) * kSub], sorted, pos, q, mesh,
+                      tiles + (int64_t)c * tg.ntile);
 }
 
 template <typename T>
@@ -142,23 +66,8 @@ __global__ void __launch_bounds__(kBlock) k_spread_reduce(const T* __restrict__ 
                                                           T* __restrict__ mesh) {
   const int64_t nm = (int64_t)tg.K[0] * tg.K[1] * tg.K[2];
   for (int64_t gidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gidx < nm;
-       gidx += (int64_t)gridDim.x * blockDim.x) {
-    const int gz = (int)(gidx % tg.K[2]);
-    const int gy = (int)((gidx / tg.K[2]) % tg.K[1]);
-    const int gx = (int)(gidx / ((int64_t)tg.K[1] * tg.K[2]));
-    int tx[kMaxCover], ox[kMaxCover], ty[kMaxCover], oy[kMaxCover], tz[kMaxCover], oz[kMaxCover];
-    const int nx = cover(gx, tg.K[0], tg.nc[0], tg.W[0], tx, ox);
-    const int ny = cover(gy, tg.K[1], tg.nc[1], tg.W[1], ty, oy);
-    const int nz = cover(gz, tg.K[2], tg.nc[2], tg.W[2], tz, oz);
-    T s = T(0);
-    for (int a = 0; a < nx; ++a)
-      for (int b = 0; b < ny; ++b) {
-        const int64_t cab = ((int64_t)tx[a] * tg.nc[1] + ty[b]) * tg.nc[2];
-        const int oab = (ox[a] * tg.W[1] + oy[b]) * tg.W[2];
-        for (int c = 0; c < nz; ++c) s += tiles[(cab + tz[c]) * tg.ntile + oab + oz[c]];
-      }
-    mesh[gidx] += s;
-  }
+       gidx += (int64_t)gridDim.x * blockDim.x)
+    mesh[gidx] += tile_sum<T>(tg, gidx, tiles);
 }
 
 template <typename T>

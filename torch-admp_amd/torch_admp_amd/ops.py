@@ -216,3 +216,103 @@ def pme_fused(positions: torch.Tensor, box: torch.Tensor, charges: torch.Tensor,
                                   _lib.ptr(forces), _lib.ptr(dedbox), _lib.ptr(workspace),
                                   nbytes.value, _lib.stream_of(pos)), "admp_pme_fused")
     return energy, forces, dedbox
+
+
+def _host_array(values, dtype, shape):
+    import numpy as np
+    arr = np.ascontiguousarray(np.asarray(values, dtype=dtype).reshape(shape))
+    return arr, ctypes.c_void_p(arr.ctypes.data)
+
+
+def _per_system(value, n_sys: int, what: str):
+    if isinstance(value, (int, float)):
+        return [float(value)] * n_sys
+    vals = [float(v) for v in value]
+    if len(vals) != n_sys:
+        raise ValueError(f"{what}: expected {n_sys} values, got {len(vals)}")
+    return vals
+
+
+class PMEBatchPlan:
+    """Host description of a batch of periodic systems for admp_pme_batch.
+
+    n_atoms, boxes ((n_sys, 3, 3) host values), cutoffs, kappas and kmesh
+    ((n_sys, 3)) fix the workspace size; the same plan serves any positions
+    and charges of that shape.
+    """
+
+    def __init__(self, n_atoms, boxes, cutoffs, kappas, kmesh, precision: str = "float64",
+                 convention: str = "reference", dielectric: float = DIELECTRIC):
+        import numpy as np
+        self.n_atoms = [int(n) for n in n_atoms]
+        self.n_sys = len(self.n_atoms)
+        if self.n_sys < 1:
+            raise ValueError("a batch needs at least one system")
+        boxes = np.asarray([b.detach().cpu().numpy() if isinstance(b, torch.Tensor) else b
+                            for b in boxes], dtype=np.float64).reshape(self.n_sys, 9)
+        self._n, self._pn = _host_array(self.n_atoms, np.int64, (self.n_sys,))
+        self._box, self._pbox = _host_array(boxes, np.float64, (self.n_sys, 9))
+        self._rc, self._prc = _host_array(_per_system(cutoffs, self.n_sys, "cutoffs"),
+                                          np.float64, (self.n_sys,))
+        self._kap, self._pkap = _host_array(_per_system(kappas, self.n_sys, "kappas"),
+                                            np.float64, (self.n_sys,))
+        km = np.asarray(kmesh, dtype=np.int32)
+        if km.ndim == 1:
+            km = np.tile(km, (self.n_sys, 1))
+        self._km, self._pkm = _host_array(km, np.int32, (self.n_sys, 3))
+        self.precision = precision
+        self.convention = convention
+        self.dielectric = float(dielectric)
+        self.offsets = np.concatenate([[0], np.cumsum(self.n_atoms)]).astype(np.int64)
+        self.n_total = int(self.offsets[-1])
+        lib = _lib.load()
+        nbytes = ctypes.c_size_t()
+        _lib.check(lib.admp_pme_batch_workspace_bytes(self.n_sys, self._pn, self._pbox, self._prc,
+                                                      self._pkm, _lib.PRECISION[precision],
+                                                      ctypes.byref(nbytes)),
+                   "admp_pme_batch_workspace_bytes")
+        self.workspace_bytes = int(nbytes.value)
+
+    def run(self, pos: torch.Tensor, q: torch.Tensor, workspace: Optional[torch.Tensor] = None,
+            want_box_grad: bool = True):
+        """Energies (n_sys, 5), forces (n_total, 3) and dE/dbox (n_sys, 3, 3)
+        for positions (n_total, 3) and charges (n_total,) stored back to back."""
+        _lib.require_gpu(pos, q)
+        pos = _f64(pos)
+        q = _f64(q).reshape(-1)
+        if pos.shape != (self.n_total, 3) or q.shape[0] != self.n_total:
+            raise ValueError(f"expected ({self.n_total}, 3) positions and {self.n_total} charges")
+        dev = pos.device
+        if workspace is None or workspace.numel() < self.workspace_bytes:
+            workspace = _lib.workspace(self.workspace_bytes, dev)
+        energy = torch.empty(self.n_sys, 5, dtype=torch.float64, device=dev)
+        forces = torch.empty(self.n_total, 3, dtype=torch.float64, device=dev)
+        dedbox = (torch.empty(self.n_sys, 3, 3, dtype=torch.float64, device=dev)
+                  if want_box_grad else None)
+        lib = _lib.load()
+        _lib.check(lib.admp_pme_batch(self.n_sys, self._pn, self._pbox, self._prc, self._pkap,
+                                      self._pkm, _lib.PRECISION[self.precision],
+                                      _lib.KCONV[self.convention], self.dielectric,
+                                      _lib.ptr(pos), _lib.ptr(q), _lib.ptr(energy),
+                                      _lib.ptr(forces), _lib.ptr(dedbox), _lib.ptr(workspace),
+                                      self.workspace_bytes, _lib.stream_of(pos)),
+                   "admp_pme_batch")
+        return energy, forces, dedbox
+
+
+def pme_batch(positions: Sequence[torch.Tensor], boxes, charges: Sequence[torch.Tensor], rcut,
+              kappa, kmesh, precision: str = "float64", convention: str = "reference",
+              dielectric: float = DIELECTRIC):
+    """pme_fused over a list of independent periodic systems in one launch sequence.
+
+    rcut and kappa are scalars or one value per system; kmesh is one (3,) mesh
+    or one per system.  Returns (energy (n_sys, 5), list of (n_i, 3) forces,
+    dE/dbox (n_sys, 3, 3)); each system's values equal pme_fused on it alone.
+    """
+    _lib.require_gpu(*positions, *charges)
+    plan = PMEBatchPlan([p.shape[0] for p in positions], boxes, rcut, kappa, kmesh, precision,
+                        convention, dielectric)
+    pos = torch.cat([_f64(p).reshape(-1, 3) for p in positions])
+    q = torch.cat([_f64(c).reshape(-1) for c in charges])
+    energy, forces, dedbox = plan.run(pos, q)
+    return energy, list(torch.split(forces, plan.n_atoms)), dedbox
