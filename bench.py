@@ -40,7 +40,128 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the float64 and drop-in module timings")
+    ap.add_argument("--serving-requests", type=int, default=4096,
+                    help="requests of the mixed-size serving stream (0 skips it)")
     return ap.parse_args()
+
+
+def serving_bench(n_requests, torch, dev):
+    """Mixed-size serving stream: PME E+F requests with atom counts log-uniform
+    in [64, 8192], orthorhombic and triclinic boxes at liquid density, rcut 6-8 A,
+    ethresh 1e-5, neutral +-1 charges.  For the float64 and the float32-bulk
+    engine: (1) capacity with every request queued at t = 0, (2) p50/p99
+    latency with Poisson arrivals at 70 % of that capacity.  Then (3) the plain
+    GPU path (drop-in TorchNeighborList + CoulombForceModule + autograd, float64,
+    one request at a time) and the single-system fused path on a subsample of
+    the requests the plain path accepts, against the batched engines' capacity
+    on those same requests, and (4) the float32 engine's error against the
+    float64 engine on 64 requests."""
+    import numpy as np
+
+    from torch_admp_amd import ops, serving
+    from torch_admp_amd.nblist import TorchNeighborList
+    from torch_admp_amd.pme import CoulombForceModule
+    from torch_admp_amd.utils import calc_grads
+
+    budget = 16 << 30
+
+    def server(prec, max_systems=256):
+        return serving.PMEServer(dev, hbm_budget=budget, max_systems=max_systems, streams=2,
+                                 precision=prec)
+
+    def clone(r, arrival=0.0):
+        return serving.PMERequest(r.positions, r.box, r.charges, r.rcut, r.ethresh,
+                                  arrival=arrival)
+
+    def min_height(b):
+        vol = abs(np.linalg.det(b))
+        return min(vol / np.linalg.norm(np.cross(b[i], b[j])) for i, j in ((1, 2), (2, 0), (0, 1)))
+
+    base = serving.synthetic_stream(n_requests, 1.0, seed=7)  # arrivals rescaled per rate
+    gaps = np.array([r.arrival for r in base])
+    # TorchNeighborList requires every box height >= 2 rcut (src/torch_admp/nblist.py:60-80)
+    eligible = [r for r in base if min_height(r.box) >= 2 * r.rcut]
+    out = {}
+    for prec in ("float64", "float32"):
+        server(prec).serve([clone(r) for r in base[:64]])  # warm-up
+        torch.cuda.synchronize()
+        cap = server(prec).serve([clone(r) for r in base])
+        rate = 0.7 * cap["throughput_req_s"]
+        online = server(prec).serve([clone(r, g / rate) for r, g in zip(base, gaps)])
+        elig = server(prec).serve([clone(r) for r in eligible])
+        tag = "fp64" if prec == "float64" else "fp32"
+        out[tag] = {"capacity_req_s": round(cap["throughput_req_s"], 2),
+                    "capacity_atoms_per_s": round(cap["atoms_per_s"], 0),
+                    "batches": cap["batches"],
+                    "poisson_rate_req_s": round(rate, 2),
+                    "p50_ms": round(online["p50_ms"], 3), "p99_ms": round(online["p99_ms"], 3),
+                    "poisson_throughput_req_s": round(online["throughput_req_s"], 2),
+                    "eligible_capacity_req_s": round(elig["throughput_req_s"], 2)}
+
+    sample = eligible[::max(1, len(eligible) // 96)]
+
+    def plain(r):
+        pos = torch.tensor(r.positions, device=dev, requires_grad=True)
+        box = torch.tensor(r.box, device=dev)
+        q = torch.tensor(r.charges, device=dev)
+        nbl = TorchNeighborList(r.rcut)
+        pairs = nbl(pos, box)
+        mod = CoulombForceModule(r.rcut, r.ethresh, kmesh=r.kmesh)
+        e = mod(pos, box, pairs, nbl.get_ds(), nbl.get_buffer_scales(), {"charge": q})
+        return e, -calc_grads(e, pos)
+
+    def fused(r):
+        pos = torch.tensor(r.positions, device=dev)
+        q = torch.tensor(r.charges, device=dev)
+        return ops.pme_fused(pos, torch.tensor(r.box), q, r.rcut, r.kappa, r.kmesh)
+
+    def per_request(fn):
+        fn(sample[0])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r in sample:
+            fn(r)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / len(sample)
+
+    t_plain = per_request(plain)
+    t_fused = per_request(fused)
+
+    # float32 engine against the float64 engine (which matches the oracle to 1e-10)
+    chk = base[:64]
+    res = {}
+    for prec in ("float64", "float32"):
+        srv = server(prec, max_systems=64)
+        reqs = [clone(r) for r in chk]
+        for r in reqs:
+            srv.submit(r)
+        srv.drain()
+        res[prec] = reqs
+    e_rel = max(abs(a.energy[4] - b.energy[4]) / abs(b.energy[4])
+                for a, b in zip(res["float32"], res["float64"]))
+    rec_rel = max(abs(a.energy[1] - b.energy[1]) / abs(b.energy[1])
+                  for a, b in zip(res["float32"], res["float64"]))
+    f_rmse = max(float(np.sqrt(((a.forces - b.forces) ** 2).mean()))
+                 for a, b in zip(res["float32"], res["float64"]))
+    atoms = np.array([r.n_atoms for r in base])
+    out.update({
+        "workload": f"{n_requests} PME E+F requests, atoms log-uniform [64, 8192] "
+                    f"(mean {atoms.mean():.0f}), ortho+triclinic boxes at 0.1 atoms/A^3, "
+                    "rcut 6-8 A, ethresh 1e-5, mesh from the module rule",
+        "engine": "admp_pme_batch, 2 streams, <= 256 systems per batch, "
+                  f"HBM budget {budget >> 30} GiB",
+        "plain_eligible_requests": len(eligible),
+        "plain_fp64_req_s": round(1.0 / t_plain, 3),
+        "fused_single_fp64_req_s": round(1.0 / t_fused, 3),
+        "plain_sample": len(sample),
+        "batched_fp64_vs_plain_fp64": round(out["fp64"]["eligible_capacity_req_s"] * t_plain, 2),
+        "batched_fp64_vs_fused_single_fp64": round(out["fp64"]["eligible_capacity_req_s"] * t_fused, 2),
+        "batched_fp32_vs_plain_fp64": round(out["fp32"]["eligible_capacity_req_s"] * t_plain, 2),
+        "fp32_vs_fp64_energy_rel_max": float(e_rel),
+        "fp32_vs_fp64_recip_energy_rel_max": float(rec_rel),
+        "fp32_vs_fp64_force_rmse_max": float(f_rmse),
+    })
+    return out
 
 
 def timed(fn, steps, warmup, torch):
@@ -134,6 +255,9 @@ def main():
         secd, (ed, fd) = timed(dropin, 3, 1, torch)
         extras["dropin_module_fp64_evals_per_s"] = round(1.0 / secd, 3)
         extras["dropin_vs_fused_fp64_energy_rel"] = float(abs(ed.detach()[0] - e64) / abs(e64))
+
+    if args.serving_requests > 0:
+        extras["serving"] = serving_bench(args.serving_requests, torch, dev)
 
     err32_vs64 = {"energy_rel": float(abs(e32 - e64) / abs(e64)),
                   "force_rmse_eV_per_A": float(((f32 - f64) ** 2).mean().sqrt())}

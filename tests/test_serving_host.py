@@ -1,0 +1,40 @@
+"""Host-side logic of the serving loop (no GPU needed)."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import pme_ref
+from torch_admp_amd import serving
+
+
+def test_ewald_parameters_follow_module_rule():
+    box = np.diag([31.0, 27.5, 40.2])
+    for rc, eth in ((6.0, 1e-5), (8.0, 1e-4), (7.3, 5e-6)):
+        kappa, kmesh = serving.ewald_parameters(box, rc, eth)
+        assert kappa == pytest.approx(pme_ref.kappa_from_ethresh(rc, eth), rel=1e-15)
+        assert kmesh == pme_ref.kmesh_rule(torch.tensor(box), kappa, eth)
+
+
+def test_synthetic_stream_shape():
+    reqs = serving.synthetic_stream(300, 500.0, seed=1)
+    n = np.array([r.n_atoms for r in reqs])
+    assert n.min() >= 64 and n.max() <= 8194 and np.all(n % 2 == 0)
+    # log-uniform: the median sits near the geometric mean of the range
+    assert 400 < np.median(n) < 1400
+    arr = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(arr) >= 0)
+    assert 0.3 < arr[-1] < 1.0  # 300 arrivals at 500 Hz
+    for r in reqs[:50]:
+        assert abs(r.charges.sum()) == 0.0
+        vol = abs(np.linalg.det(r.box))
+        assert vol == pytest.approx(r.n_atoms / 0.1, rel=1e-9)
+        assert 6.0 <= r.rcut <= 8.0
+        assert r.kappa == pytest.approx(math.sqrt(-math.log(2e-5)) / r.rcut)
+
+
+def test_server_requires_gpu():
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        serving.PMEServer("cpu", hbm_budget=1 << 20)

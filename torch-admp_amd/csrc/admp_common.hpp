@@ -90,6 +90,20 @@ __device__ __forceinline__ void bspline6(T f, T w[6], T dw[6]) {
   }
 }
 
+// Weights and derivatives evaluated in float64 and rounded once to T.  The
+// float32 recursion multiplies by rounded constants (1/3, 1/5), which biases
+// every weight in the same direction and the reciprocal energy by ~1e-7.
+template <typename T>
+__device__ __forceinline__ void bspline6_rounded(double f, T w[6], T dw[6]) {
+  double wd[6], dd[6];
+  bspline6<double>(f, wd, dd);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    w[j] = (T)wd[j];
+    dw[j] = (T)dd[j];
+  }
+}
+
 // Reference point of an atom on the mesh: R = K (r A); m0 = ceil(R);
 // f = m0 - R in [0, 1).  Grid point m0 - 3 + j carries weight M6(f + j).
 __device__ __forceinline__ void mesh_ref(const Geom& g, double x, double y, double z,

@@ -21,9 +21,9 @@ __device__ __forceinline__ void spread_lane(const Geom& g, double x, double y, d
   double f[3];
   mesh_ref(g, x, y, z, m0, f);
   T wx[6], wy[6], wz[6], dd[6];
-  bspline6<T>((T)f[0], wx, dd);
-  bspline6<T>((T)f[1], wy, dd);
-  bspline6<T>((T)f[2], wz, dd);
+  bspline6_rounded<T>(f[0], wx, dd);
+  bspline6_rounded<T>(f[1], wy, dd);
+  bspline6_rounded<T>(f[2], wz, dd);
   const int K0 = g.k[0], K1 = g.k[1], K2 = g.k[2];
   const T qa = (T)q * wx[a];
   const int ix = wrapk(pmod32(m0[0] - 3, K0) + a, K0);
@@ -70,9 +70,9 @@ __device__ __forceinline__ void spread_cell_tile(const Geom& g, const TileGeom& 
     double f[3];
     mesh_ref(g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], m0, f);
     T wx[6], wy[6], wz[6], dd[6];
-    bspline6<T>((T)f[0], wx, dd);
-    bspline6<T>((T)f[1], wy, dd);
-    bspline6<T>((T)f[2], wz, dd);
+    bspline6_rounded<T>(f[0], wx, dd);
+    bspline6_rounded<T>(f[1], wy, dd);
+    bspline6_rounded<T>(f[2], wz, dd);
     const T qa = (T)q[i] * wx[a];
     const int lx = wrapk(pmod32(m0[0] - 3 - lo0, tg.K[0]) + a, tg.K[0]);
     const int by = pmod32(m0[1] - 3 - lo1, tg.K[1]), bz = pmod32(m0[2] - 3 - lo2, tg.K[2]);
@@ -157,9 +157,9 @@ __device__ __forceinline__ void gather_lane(const Geom& g, double x, double y, d
   double f[3];
   mesh_ref(g, x, y, z, m0, f);
   T wx[6], wy[6], wz[6], dx[6], dy[6], dz[6];
-  bspline6<T>((T)f[0], wx, dx);
-  bspline6<T>((T)f[1], wy, dy);
-  bspline6<T>((T)f[2], wz, dz);
+  bspline6_rounded<T>(f[0], wx, dx);
+  bspline6_rounded<T>(f[1], wy, dy);
+  bspline6_rounded<T>(f[2], wz, dz);
   T ax = 0, ay = 0, az = 0, ap = 0;
   if (a < kOrder) {
     const int K0 = g.k[0], K1 = g.k[1], K2 = g.k[2];

@@ -1,0 +1,277 @@
+"""Continuous-batching request loop for PME energy + force evaluations.
+
+Requests of any atom count, box and mesh arrive over time.  The server keeps
+a FIFO queue and up to ``streams`` batches in flight, one per HIP stream.
+Whenever a stream is free it forms a batch from the head of the queue, as
+many requests as fit the HBM budget that in-flight batches leave over (and
+at most ``max_systems``), copies their positions and charges to the device
+once, and launches the whole batch as one admp_pme_batch launch sequence.
+Requests that arrive while other batches run are admitted as soon as a
+stream and memory free up, so their work overlaps on the device with the
+batches already there.  Each request's result is the same whether it runs
+alone or in a mix (the batched kernels treat every system independently).
+
+The reference evaluates one frame per CoulombForceModule call
+(src/torch_admp/pme.py:261-310); this module has no counterpart there.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Deque, List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .ops import PMEBatchPlan
+from .utils import DIELECTRIC
+
+
+def ewald_parameters(box: np.ndarray, rcut: float, ethresh: float):
+    """kappa and mesh of CoulombForceModule for a box (src/torch_admp/pme.py:113-115,383-392):
+    kappa = sqrt(-ln(2 ethresh)) / rcut, K_d = ceil(2 kappa h_dd / (3 ethresh^(1/5)))."""
+    kappa = math.sqrt(-math.log(2.0 * ethresh)) / rcut
+    diag = np.diagonal(np.asarray(box, dtype=np.float64))
+    kmesh = [int(math.ceil(2.0 * kappa * d / (3.0 * ethresh ** 0.2))) for d in diag]
+    return kappa, kmesh
+
+
+@dataclass
+class PMERequest:
+    """One energy + force evaluation: positions (n, 3), box (3, 3) rows,
+    charges (n,), cutoff; kappa and mesh follow CoulombForceModule's rule for
+    ``ethresh`` unless given.  ``arrival`` is in seconds from the stream start."""
+
+    positions: np.ndarray
+    box: np.ndarray
+    charges: np.ndarray
+    rcut: float
+    ethresh: float = 1e-5
+    kappa: Optional[float] = None
+    kmesh: Optional[List[int]] = None
+    arrival: float = 0.0
+    rid: int = -1
+    # results (host float64)
+    energy: Optional[np.ndarray] = None  # real, reciprocal, self, non-neutral, total
+    forces: Optional[np.ndarray] = None
+    dedbox: Optional[np.ndarray] = None
+    t_admit: float = math.nan
+    t_done: float = math.nan
+    cost: int = 0
+
+    def __post_init__(self):
+        self.positions = np.ascontiguousarray(self.positions, dtype=np.float64).reshape(-1, 3)
+        self.charges = np.ascontiguousarray(self.charges, dtype=np.float64).reshape(-1)
+        self.box = np.asarray(self.box, dtype=np.float64).reshape(3, 3)
+        if self.kappa is None or self.kmesh is None:
+            kappa, kmesh = ewald_parameters(self.box, self.rcut, self.ethresh)
+            self.kappa = kappa if self.kappa is None else self.kappa
+            self.kmesh = kmesh if self.kmesh is None else list(self.kmesh)
+
+    @property
+    def n_atoms(self) -> int:
+        return self.positions.shape[0]
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.arrival
+
+
+@dataclass
+class _Batch:
+    requests: List[PMERequest]
+    plan: PMEBatchPlan
+    stream: torch.cuda.Stream
+    done: torch.cuda.Event
+    host_energy: torch.Tensor
+    host_forces: torch.Tensor
+    host_dedbox: torch.Tensor
+    keep: list = field(default_factory=list)  # device buffers alive until the batch ends
+    cost: int = 0
+
+
+class PMEServer:
+    """Continuous batching of PME requests under an HBM budget (bytes)."""
+
+    def __init__(self, device="cuda", hbm_budget: Optional[int] = None, max_systems: int = 1024,
+                 streams: int = 2, precision: str = "float32", convention: str = "reference",
+                 dielectric: float = DIELECTRIC):
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise RuntimeError("PMEServer runs only on a ROCm GPU (no CPU fallback)")
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            hbm_budget = int(free * 0.5)
+        self.hbm_budget = int(hbm_budget)
+        self.max_systems = int(max_systems)
+        self.precision = precision
+        self.convention = convention
+        self.dielectric = float(dielectric)
+        self.streams = [torch.cuda.Stream(self.device) for _ in range(max(1, streams))]
+        self.queue: Deque[PMERequest] = deque()
+        self.inflight: List[Optional[_Batch]] = [None] * len(self.streams)
+        self.bytes_in_flight = 0
+        self.completed: List[PMERequest] = []
+        self.batches_launched = 0
+        self._next_id = 0
+        self._t0 = time.perf_counter()
+
+    # ------------------------------------------------------------------ admission
+    def request_cost(self, req: PMERequest) -> int:
+        """HBM bytes a request holds while in flight: its share of the batch
+        workspace plus positions, charges and results."""
+        plan = PMEBatchPlan([req.n_atoms], [req.box], req.rcut, req.kappa, [req.kmesh],
+                            self.precision, self.convention, self.dielectric)
+        return plan.workspace_bytes + 56 * req.n_atoms + 8 * 14 + 4096
+
+    def now(self) -> float:
+        return time.perf_counter() - self._t0
+
+    def submit(self, req: PMERequest) -> int:
+        req.cost = self.request_cost(req)
+        if req.cost > self.hbm_budget:
+            raise ValueError(f"request of {req.n_atoms} atoms needs {req.cost} bytes, more than "
+                             f"the HBM budget of {self.hbm_budget}")
+        req.rid = self._next_id
+        self._next_id += 1
+        self.queue.append(req)
+        return req.rid
+
+    # ------------------------------------------------------------------ batches
+    def _launch(self, slot: int) -> bool:
+        take: List[PMERequest] = []
+        cost = 0
+        room = self.hbm_budget - self.bytes_in_flight
+        while self.queue and len(take) < self.max_systems:
+            r = self.queue[0]
+            if cost + r.cost > room:
+                break
+            take.append(self.queue.popleft())
+            cost += r.cost
+        if not take:
+            return False
+        plan = PMEBatchPlan([r.n_atoms for r in take], [r.box for r in take],
+                            [r.rcut for r in take], [r.kappa for r in take],
+                            [r.kmesh for r in take], self.precision, self.convention,
+                            self.dielectric)
+        n = plan.n_total
+        stream = self.streams[slot]
+        t = self.now()
+        for r in take:
+            r.t_admit = t
+        host_in = torch.empty(n, 4, dtype=torch.float64, pin_memory=True)
+        if n:
+            host_in.numpy()[:, :3] = np.concatenate([r.positions for r in take])
+            host_in.numpy()[:, 3] = np.concatenate([r.charges for r in take])
+        with torch.cuda.stream(stream):
+            dev_in = host_in.to(self.device, non_blocking=True)
+            pos = dev_in[:, :3].contiguous()
+            q = dev_in[:, 3].contiguous()
+            ws = _lib.workspace(plan.workspace_bytes, self.device)
+            energy, forces, dedbox = plan.run(pos, q, workspace=ws)
+            h_e = torch.empty(energy.shape, dtype=torch.float64, pin_memory=True)
+            h_f = torch.empty(forces.shape, dtype=torch.float64, pin_memory=True)
+            h_g = torch.empty(dedbox.shape, dtype=torch.float64, pin_memory=True)
+            h_e.copy_(energy, non_blocking=True)
+            h_f.copy_(forces, non_blocking=True)
+            h_g.copy_(dedbox, non_blocking=True)
+            done = torch.cuda.Event()
+            done.record(stream)
+        self.inflight[slot] = _Batch(take, plan, stream, done, h_e, h_f, h_g,
+                                     [host_in, dev_in, pos, q, ws, energy, forces, dedbox], cost)
+        self.bytes_in_flight += cost
+        self.batches_launched += 1
+        return True
+
+    def _finish(self, slot: int) -> List[PMERequest]:
+        b = self.inflight[slot]
+        t = self.now()
+        e = b.host_energy.numpy()
+        f = b.host_forces.numpy()
+        g = b.host_dedbox.numpy()
+        off = b.plan.offsets
+        for k, r in enumerate(b.requests):
+            r.energy = e[k].copy()
+            r.forces = f[off[k]:off[k + 1]].copy()
+            r.dedbox = g[k].copy()
+            r.t_done = t
+        self.inflight[slot] = None
+        self.bytes_in_flight -= b.cost
+        self.completed.extend(b.requests)
+        return b.requests
+
+    def pump(self) -> List[PMERequest]:
+        """Collect finished batches, then fill free streams from the queue.
+        Returns the requests completed by this call."""
+        out: List[PMERequest] = []
+        for slot, b in enumerate(self.inflight):
+            if b is not None and b.done.query():
+                out.extend(self._finish(slot))
+        for slot, b in enumerate(self.inflight):
+            if b is None and self.queue:
+                if not self._launch(slot):
+                    break
+        return out
+
+    def idle(self) -> bool:
+        return not self.queue and all(b is None for b in self.inflight)
+
+    def drain(self) -> None:
+        while not self.idle():
+            self.pump()
+
+    # ------------------------------------------------------------------ streams
+    def serve(self, requests: List[PMERequest]) -> dict:
+        """Replay requests at their arrival times (seconds from now) and return
+        latency and throughput statistics."""
+        order = sorted(requests, key=lambda r: r.arrival)
+        self.completed = []
+        self._t0 = time.perf_counter()
+        i = 0
+        while i < len(order) or not self.idle():
+            now = self.now()
+            while i < len(order) and order[i].arrival <= now:
+                self.submit(order[i])
+                i += 1
+            self.pump()
+        wall = self.now()
+        lat = np.array([r.latency for r in self.completed]) * 1e3
+        atoms = sum(r.n_atoms for r in self.completed)
+        return {"requests": len(self.completed), "wall_s": wall,
+                "throughput_req_s": len(self.completed) / wall,
+                "atoms_per_s": atoms / wall,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_ms": float(lat.mean()), "batches": self.batches_launched}
+
+
+def synthetic_stream(n_requests: int, rate_hz: float, seed: int = 0, n_min: int = 64,
+                     n_max: int = 8192, density: float = 0.1, rcut_range=(6.0, 8.0),
+                     triclinic_fraction: float = 0.5, ethresh: float = 1e-5) -> List[PMERequest]:
+    """Poisson arrivals of PME requests: atom counts log-uniform in [n_min, n_max],
+    orthorhombic or triclinic boxes at ``density`` atoms per A^3 (liquid water is
+    0.1), uniform positions and random neutral +-1 charges."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate_hz, n_requests)) if rate_hz > 0 else np.zeros(n_requests)
+    reqs = []
+    for k in range(n_requests):
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        n += n % 2
+        vol = n / density
+        aspect = rng.uniform(0.8, 1.25, 3)
+        edges = aspect * (vol / aspect.prod()) ** (1.0 / 3.0)
+        box = np.diag(edges)
+        if rng.random() < triclinic_fraction:
+            # lower-triangular tilt keeps the volume (and the diagonal used by the mesh rule)
+            box[1, 0] = rng.uniform(-0.3, 0.3) * edges[0]
+            box[2, 0] = rng.uniform(-0.3, 0.3) * edges[0]
+            box[2, 1] = rng.uniform(-0.3, 0.3) * edges[1]
+        pos = rng.random((n, 3)) @ box
+        q = np.r_[np.ones(n // 2), -np.ones(n - n // 2)]
+        rng.shuffle(q)
+        reqs.append(PMERequest(pos, box, q, float(rng.uniform(*rcut_range)), ethresh,
+                               arrival=float(t[k])))
+    return reqs
