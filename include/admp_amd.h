@@ -171,8 +171,7 @@ int admp_pme_fused(const double* pos, const double* q, int64_t n, const double* 
  *                kappas[n_sys], kmesh[n_sys][3]
  *   device: pos (sum n_atoms, 3) and q (sum n_atoms), systems back to back;
  *           energy (n_sys, 5); forces (sum n_atoms, 3); dEdbox (n_sys, 9) or NULL
- * Returns ADMP_ERR_UNSUPPORTED when a box is so small against its cutoff that
- * the real-space stencil exceeds 128 periodic images. */
+ * Returns ADMP_ERR_UNSUPPORTED when the batch exceeds 2^30 atoms or 2^28 cells. */
 int admp_pme_batch_workspace_bytes(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
                                    const double* cutoffs, const int32_t* kmesh,
                                    int32_t precision, size_t* bytes);

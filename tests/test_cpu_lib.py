@@ -73,8 +73,8 @@ def test_cpu_tensors_fail_loudly():
 
 
 def test_batch_plan_on_host():
-    """Batch planning is host-only: sizes grow with the batch, invalid systems
-    are rejected and a box far below the cutoff is reported unsupported."""
+    """Batch planning is host-only: sizes grow with the batch and invalid
+    systems are rejected."""
     import numpy as np
     import torch
 
@@ -90,8 +90,8 @@ def test_batch_plan_on_host():
         ops.PMEBatchPlan([10], [np.zeros((3, 3))], 6.0, 0.4, [24, 24, 24])
     with pytest.raises(AdmpError, match="status -1"):
         ops.PMEBatchPlan([10], [np.eye(3) * 10.0], 6.0, 0.4, [0, 24, 24])
-    with pytest.raises(AdmpError, match="status -3"):
-        ops.PMEBatchPlan([10], [np.eye(3) * 3.0], 9.0, 0.4, [8, 8, 8])
+    # a box far below the cutoff (7^3 periodic images per dimension triple) is planned
+    assert ops.PMEBatchPlan([10], [np.eye(3) * 3.0], 9.0, 0.4, [8, 8, 8]).workspace_bytes > 0
     with pytest.raises(ValueError):
         ops.PMEBatchPlan([10, 20], [np.eye(3) * 10.0] * 2, [6.0], 0.4, [8, 8, 8])
     with pytest.raises(RuntimeError, match="ROCm GPU"):

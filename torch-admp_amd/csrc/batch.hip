@@ -4,12 +4,15 @@
 // Every kernel reads its system's parameters from a device table (SysDesc)
 // and finds the system of a global index (atom, cell, mesh point, FFT block,
 // gather block) by binary search over prefix offsets, so a batch of any mix
-// costs the same ~20 launches as a single system.  A single system is a
-// batch of one.  Reductions are per system and in a fixed order.
+// costs the same 14 launches as a single system; admp_pme_fused is a batch of
+// one whose table is written by the first kernel from its arguments (no copy,
+// so the sequence can be captured in a HIP graph).  Reductions are per system
+// and in a fixed order.
 //
 // Per system the result equals CoulombForceModule.forward on the
 // TorchNeighborList pairs followed by autograd (src/torch_admp/pme.py:261-310,
-// src/torch_admp/utils.py:84-112); boxes smaller than 2 rc use periodic images.
+// src/torch_admp/utils.py:84-112) when every box height is at least 2 rcut;
+// below that every periodic image inside the cutoff is summed.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -29,6 +32,34 @@ __device__ __forceinline__ int find_seg(const int64_t* __restrict__ off, int nse
     else hi = mid;
   }
   return lo;
+}
+
+// ------------------------------------------------------------------ prologue
+// Zero the bin counters and scatter cursors, the meshes and the per-system
+// reduction tickets; for a batch of one also write the system table and the
+// segment offsets from the kernel arguments.
+struct OneSystem {
+  SysDesc d;
+  int64_t tot[kSegCount];  // segment sizes (offsets are [0, tot])
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_b_prologue(int inline_table, OneSystem one,
+                                                       SysDesc* __restrict__ tab,
+                                                       int64_t* __restrict__ offs,
+                                                       int* __restrict__ count, int64_t ncount,
+                                                       T* __restrict__ mesh, int64_t nmesh,
+                                                       int* __restrict__ tickets, int ntickets) {
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  if (inline_table && t0 == 0) tab[0] = one.d;
+  if (inline_table && t0 < kSegCount) {
+    offs[2 * t0] = 0;
+    offs[2 * t0 + 1] = one.tot[t0];
+  }
+  for (int64_t i = t0; i < ncount; i += step) count[i] = 0;
+  for (int64_t i = t0; i < nmesh; i += step) mesh[i] = T(0);
+  for (int64_t i = t0; i < ntickets; i += step) tickets[i] = 0;
 }
 
 // ------------------------------------------------------------------ binning
@@ -65,9 +96,47 @@ __global__ void __launch_bounds__(kBlock) k_b_bin(int64_t ntot, int nsys,
   atomicAdd(&count[cid * kSub + (int)(i & (kSub - 1))], 1);
 }
 
+// Deterministic in-cell order and packing, one work-group per cell: the
+// rank of atom i among the atoms of its cell is the number of members with a
+// smaller index.  Members are staged in LDS (global reads for cells above
+// kSortCap atoms).  Writes sorted[] and the packed (w, q) records.
+constexpr int kSortCap = 4096;
+
+__global__ void __launch_bounds__(kBlock) k_b_sort_pack(const int* __restrict__ start,
+                                                        const int* __restrict__ unsorted,
+                                                        const double* __restrict__ w,
+                                                        const double* __restrict__ q,
+                                                        int* __restrict__ sorted,
+                                                        P4<double>* __restrict__ ps) {
+  __shared__ int ids[kSortCap];
+  const int64_t c = blockIdx.x;
+  const int b = start[c * kSub], e = start[(c + 1) * kSub];
+  const int m = e - b;
+  const bool in_lds = m <= kSortCap;
+  if (in_lds)
+    for (int t = threadIdx.x; t < m; t += blockDim.x) ids[t] = unsorted[b + t];
+  __syncthreads();
+  for (int t = threadIdx.x; t < m; t += blockDim.x) {
+    const int i = in_lds ? ids[t] : unsorted[b + t];
+    int rank = 0;
+    if (in_lds)
+      for (int u = 0; u < m; ++u) rank += ids[u] < i;
+    else
+      for (int u = 0; u < m; ++u) rank += unsorted[b + u] < i;
+    sorted[b + rank] = i;
+    P4<double> v;
+    v.x = w[3 * i];
+    v.y = w[3 * i + 1];
+    v.z = w[3 * i + 2];
+    v.q = q[i];
+    ps[b + rank] = v;
+  }
+}
+
 // ------------------------------------------------------------------ real space
-// One work-group per (global cell, candidate split); partial rows are
-// cell-major so that a system's rows are contiguous.
+// One work-group per (global cell, candidate split).  Partials are stored
+// value-major, rows cell-major (row = cell * splits + split), so that a
+// system's rows are contiguous and the reduction reads them coalesced.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_b_real(int64_t ntot, int nsys,
                                                    const int64_t* __restrict__ cell_off,
@@ -81,8 +150,8 @@ __global__ void __launch_bounds__(kBlock) k_b_real(int64_t ntot, int nsys,
   const int64_t c = blockIdx.x;
   const SysDesc* D = tab + find_seg(cell_off, nsys, c);
   real_cell_body<T>(D->cg, D->cell0, (int)(c - D->cell0), ntot, (T)D->kappa, (T)D->eps, ps,
-                    sorted, start, pos, gsplit,
-                    partials + (c * gridDim.y + blockIdx.y) * kRealNV);
+                    sorted, start, pos, gsplit, partials + c * gridDim.y + blockIdx.y,
+                    (int64_t)gridDim.x * gridDim.y);
 }
 
 // ------------------------------------------------------------------ spread
@@ -104,19 +173,6 @@ __global__ void __launch_bounds__(kBlock) k_b_spread_tiles(int nsys,
                       mesh + D->mesh0, tiles + D->tile0 + (int64_t)lc * D->tg.ntile);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_b_spread_reduce(int64_t mtot, int nsys,
-                                                            const int64_t* __restrict__ mesh_off,
-                                                            const SysDesc* __restrict__ tab,
-                                                            const T* __restrict__ tiles,
-                                                            T* __restrict__ mesh) {
-  for (int64_t gidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gidx < mtot;
-       gidx += (int64_t)gridDim.x * blockDim.x) {
-    const SysDesc* D = tab + find_seg(mesh_off, nsys, gidx);
-    if (D->tg.ok) mesh[gidx] += tile_sum<T>(D->tg, gidx - D->mesh0, tiles + D->tile0);
-  }
-}
-
 // atoms of systems whose tiles do not fit: global atomics into their mesh
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_b_spread_atoms(int64_t ntot,
@@ -135,16 +191,29 @@ __global__ void __launch_bounds__(kBlock) k_b_spread_atoms(int64_t ntot,
 }
 
 // ------------------------------------------------------------------ FFT passes
+// r2c along z of mesh + (for tiled systems) the sum of the cell tiles that
+// cover each point, so the tiles are reduced on the way into the FFT.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_b_r2c(int nsys, const int64_t* __restrict__ off,
                                                   const SysDesc* __restrict__ tab,
                                                   const T* __restrict__ meshes,
+                                                  const T* __restrict__ tiles,
                                                   cplx<T>* __restrict__ specs) {
   const int sid = find_seg(off, nsys, blockIdx.x);
   const SysDesc* D = tab + sid;
   const int C = D->fchunk[0];
-  r2c_rows<T>(meshes + D->mesh0, specs + D->spec0, (int64_t)D->g.k[0] * D->g.k[1],
-              (blockIdx.x - off[sid]) * 2 * C, D->g.k[2], C, D->rad[2]);
+  const int L = D->g.k[2];
+  const T* mesh = meshes + D->mesh0;
+  if (D->tg.ok) {
+    const TileGeom tg = D->tg;
+    const T* tl = tiles + D->tile0;
+    r2c_rows<T>([=](int64_t row, int z) { return mesh[row * L + z] + tile_sum<T>(tg, row * L + z, tl); },
+                specs + D->spec0, (int64_t)D->g.k[0] * D->g.k[1], (blockIdx.x - off[sid]) * 2 * C,
+                L, C, D->rad[2]);
+  } else {
+    r2c_rows<T>([=](int64_t row, int z) { return mesh[row * L + z]; }, specs + D->spec0,
+                (int64_t)D->g.k[0] * D->g.k[1], (blockIdx.x - off[sid]) * 2 * C, L, C, D->rad[2]);
+  }
 }
 
 template <typename T>
@@ -174,12 +243,13 @@ __global__ void __launch_bounds__(kBlock) k_b_cols_y(int nsys, const int64_t* __
                          min(C, Kzh - chunk * C), C, D->rad[1]);
 }
 
-// lines along x with the influence function; per-block partials (10, times eps)
+// lines along x with the influence function; per-block partials (10 values
+// times eps, value-major with nrows rows)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_b_cols_x(int nsys, const int64_t* __restrict__ off,
                                                      const SysDesc* __restrict__ tab,
                                                      cplx<T>* __restrict__ specs,
-                                                     double* __restrict__ partials) {
+                                                     double* __restrict__ partials, int64_t nrows) {
   const int sid = find_seg(off, nsys, blockIdx.x);
   const SysDesc* D = tab + sid;
   InflParams P;
@@ -192,25 +262,26 @@ __global__ void __launch_bounds__(kBlock) k_b_cols_x(int nsys, const int64_t* __
   const int Kzh = D->g.k[2] / 2 + 1, C = D->fchunk[2];
   const int nchunk = (Kzh + C - 1) / C;
   const int64_t lb = blockIdx.x - off[sid];
-  double* dst = partials + (int64_t)blockIdx.x * 10;
+  __shared__ double vals[10];
   x_convolve_lines<T>(specs + D->spec0, (int)(lb / nchunk), (int)(lb % nchunk) * C, C, D->rad[0],
-                      P, dst);
-  if (threadIdx.x == 0)
-    for (int i = 0; i < 10; ++i) dst[i] *= P.eps;
+                      P, vals);
+  __syncthreads();
+  if (threadIdx.x < 10) partials[(int64_t)threadIdx.x * nrows + blockIdx.x] = vals[threadIdx.x] * P.eps;
 }
 
 // ------------------------------------------------------------------ gather
-// kGatherAtomsPerBlock atoms of one system per work-group, 8 lanes per atom;
-// adds the reciprocal-space gradient to grad (row 0 of the split force rows)
-// and writes the block's dE/dA partials (9 values).
+// kGatherAtomsPerBlock atoms of one system per work-group, 8 lanes per atom.
+// Forces are final here: -(sum of the real-space split rows + reciprocal
+// gradient).  The block's dE/dA partials (9 values) are stored value-major.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __restrict__ off,
                                                      const SysDesc* __restrict__ tab,
                                                      const double* __restrict__ pos,
                                                      const double* __restrict__ qv,
                                                      const T* __restrict__ meshes,
-                                                     double* __restrict__ grad,
-                                                     double* __restrict__ partials) {
+                                                     const double* __restrict__ gsplit, int nsplit,
+                                                     int64_t ntot, double* __restrict__ forces,
+                                                     double* __restrict__ partials, int64_t nrows) {
   __shared__ double red[9 * (kBlock / 64)];
   const int sid = find_seg(off, nsys, blockIdx.x);
   const SysDesc* D = tab + sid;
@@ -236,53 +307,95 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __
     sy += __shfl_xor(sy, o, 64);
     sz += __shfl_xor(sz, o, 64);
   }
-  if (valid && a == 0) {
+  if (valid && a < 3) {
+    // lane a (< 3) owns force component a
     const double qi = qv[atom];
     const double dR[3] = {-qi * (double)sx, -qi * (double)sy, -qi * (double)sz};
+    double gj = 0.0;
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      double gj = 0.0;
+    for (int d = 0; d < 3; ++d) gj += g.k[d] * g.a[a * 3 + d] * dR[d];
+    for (int k = 0; k < nsplit; ++k) gj += gsplit[((int64_t)k * ntot + atom) * 3 + a];
+    forces[3 * atom + a] = -gj;
+    if (a == 0) {
+      const double r[3] = {x, y, z};
 #pragma unroll
-      for (int d = 0; d < 3; ++d) gj += g.k[d] * g.a[j * 3 + d] * dR[d];
-      grad[3 * atom + j] += gj;
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) acc[j * 3 + d] = dR[d] * g.k[d] * r[j];
     }
-    const double r[3] = {x, y, z};
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) acc[j * 3 + d] = dR[d] * g.k[d] * r[j];
   }
-  block_sum<9>(acc, red);
+  block_sum<9>(acc, red);  // the block total is in thread 0
   if (threadIdx.x == 0)
-    for (int i = 0; i < 9; ++i) partials[(int64_t)blockIdx.x * 9 + i] = acc[i];
+    for (int i = 0; i < 9; ++i) partials[(int64_t)i * nrows + blockIdx.x] = acc[i];
 }
 
 // ------------------------------------------------------------------ reductions
-// out[s * out_stride + out_first + v] = sum over rows [off[s], off[s+1]) of p[row * nv + v]
-__global__ void __launch_bounds__(kBlock) k_b_reduce_seg(const double* __restrict__ p, int nv,
-                                                         const int64_t* __restrict__ off,
-                                                         int64_t row_mul, double* __restrict__ out,
-                                                         int out_stride, int out_first) {
-  const int s = blockIdx.x;
-  const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-  const int64_t r0 = off[s] * row_mul, r1 = off[s + 1] * row_mul;
-  for (int v = threadIdx.x >> 6; v < nv; v += nw) {
-    double acc = 0.0;
-    for (int64_t r = r0 + lane; r < r1; r += 64) acc += p[r * nv + v];
-    acc = wave_sum(acc);
-    if (lane == 0) out[(int64_t)s * out_stride + out_first + v] = acc;
-  }
-}
+// Work-group (s, g) reduces values 4g .. 4g + 3 of system s (one wavefront per
+// value, fixed order) into red[s][kSysValues]; the last work-group of a system
+// to finish (a ticket counter) turns them into energies and dE/dbox.
+constexpr int kRedGroups = (kSysValues + 3) / 4;
 
-__global__ void k_b_finalize(int nsys, const SysDesc* __restrict__ tab,
-                             const double* __restrict__ red, double* __restrict__ energy,
-                             double* __restrict__ dEdbox) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsys) return;
-  const SysDesc* D = tab + s;
-  const double* rr = red + (int64_t)s * kSysValues;
-  finalize_system(rr, rr + kRealNV, rr + kRealNV + 10, D->g, D->kappa, D->eps, energy + 5 * s,
-                  dEdbox ? dEdbox + 9 * s : nullptr);
+struct PartialSets {
+  const double* real;  // kRealNV values x rows_real (rows = cells x splits)
+  const double* recip; // 10 values x rows_recip (x-pass blocks)
+  const double* gath;  // 9 values x rows_gath (gather blocks)
+  int64_t rows_real, rows_recip, rows_gath;
+  int nsplit;
+};
+
+__global__ void __launch_bounds__(kBlock) k_b_reduce_finalize(
+    int nsys, const int64_t* __restrict__ offs, const SysDesc* __restrict__ tab, PartialSets ps,
+    double* __restrict__ red, int* __restrict__ tickets, double* __restrict__ energy,
+    double* __restrict__ dEdbox) {
+  const int s = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (v < kSysValues) {
+    const double* p;
+    int64_t r0, r1;
+    const int64_t* cell = offs + (int64_t)(nsys + 1) * kSegCell;
+    const int64_t* xo = offs + (int64_t)(nsys + 1) * kSegFftX;
+    const int64_t* go = offs + (int64_t)(nsys + 1) * kSegGather;
+    if (v < kRealNV) {
+      p = ps.real + (int64_t)v * ps.rows_real;
+      r0 = cell[s] * ps.nsplit;
+      r1 = cell[s + 1] * ps.nsplit;
+    } else if (v < kRealNV + 10) {
+      p = ps.recip + (int64_t)(v - kRealNV) * ps.rows_recip;
+      r0 = xo[s];
+      r1 = xo[s + 1];
+    } else {
+      p = ps.gath + (int64_t)(v - kRealNV - 10) * ps.rows_gath;
+      r0 = go[s];
+      r1 = go[s + 1];
+    }
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int64_t r = r0 + lane;
+    for (; r + 192 < r1; r += 256) {
+      a0 += p[r];
+      a1 += p[r + 64];
+      a2 += p[r + 128];
+      a3 += p[r + 192];
+    }
+    for (; r < r1; r += 64) a0 += p[r];
+    const double acc = wave_sum((a0 + a1) + (a2 + a3));
+    if (lane == 0) red[(int64_t)s * kSysValues + v] = acc;
+  }
+  __threadfence();
+  __syncthreads();
+  __shared__ int last;
+  if (threadIdx.x == 0) last = atomicAdd(&tickets[s], 1) == (int)gridDim.y - 1;
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    const SysDesc* D = tab + s;
+    const double* rr = red + (int64_t)s * kSysValues;
+    double vals[kSysValues];
+    for (int k = 0; k < kSysValues; ++k)
+      vals[k] = __hip_atomic_load(rr + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    finalize_system(vals, vals + kRealNV, vals + kRealNV + 10, D->g, D->kappa, D->eps,
+                    energy + 5 * s, dEdbox ? dEdbox + 9 * s : nullptr);
+  }
 }
 
 // ------------------------------------------------------------------ host
@@ -312,15 +425,13 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
   P.lds_fft[0] = P.lds_fft[1] = P.lds_fft[2] = 0;
   P.lds_tile = 0;
   P.any_untiled = 0;
+  int64_t spec = 0, tile = 0;
   for (int s = 0; s < nsys; ++s) {
     SysDesc& D = P.sys[s];
     std::memset(&D, 0, sizeof(D));
     const double* box = boxes + 9 * s;
     const int32_t* km = kmesh + 3 * s;
     D.cg = make_grid(box, cutoffs[s], false);
-    int nst = 1;
-    for (int d = 0; d < 3; ++d) nst *= D.cg.nc[d] >= 3 ? 3 : 2 * D.cg.nimg[d] + 1;
-    if (nst > kMaxStencil) P.status = ADMP_ERR_UNSUPPORTED;
     for (int i = 0; i < 9; ++i) {
       D.g.h[i] = D.cg.h[i];
       D.g.a[i] = D.cg.a[i];
@@ -346,15 +457,10 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
     D.cell0 = P.off[kSegCell][s];
     D.ncell = D.cg.ncell;
     D.mesh0 = P.off[kSegMesh][s];
-    int64_t sp0 = s == 0 ? 0 : P.sys[s - 1].spec0 + (int64_t)P.sys[s - 1].g.k[0] * P.sys[s - 1].g.k[1] *
-                                                     (P.sys[s - 1].g.k[2] / 2 + 1);
-    D.spec0 = sp0;
-    int64_t tl0 = 0;
-    if (s > 0) {
-      const SysDesc& Pv = P.sys[s - 1];
-      tl0 = Pv.tile0 + (Pv.tg.ok ? (int64_t)Pv.ncell * Pv.tg.ntile : 0);
-    }
-    D.tile0 = tl0;
+    D.spec0 = spec;
+    D.tile0 = tile;
+    spec += (int64_t)K0 * K1 * Kzh;
+    if (D.tg.ok) tile += (int64_t)D.ncell * D.tg.ntile;
     P.off[kSegAtom][s + 1] = D.atom0 + D.n;
     P.off[kSegCell][s + 1] = D.cell0 + D.ncell;
     P.off[kSegMesh][s + 1] = D.mesh0 + (int64_t)K0 * K1 * K2;
@@ -371,11 +477,10 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
   P.ntot = P.off[kSegAtom][nsys];
   P.ctot = P.off[kSegCell][nsys];
   P.mtot = P.off[kSegMesh][nsys];
-  const SysDesc& L = P.sys[nsys - 1];
-  P.stot = L.spec0 + (int64_t)L.g.k[0] * L.g.k[1] * (L.g.k[2] / 2 + 1);
-  P.ttot = L.tile0 + (L.tg.ok ? (int64_t)L.ncell * L.tg.ntile : 0);
+  P.stot = spec;
+  P.ttot = tile;
   std::memset(&P.cg_all, 0, sizeof(P.cg_all));
-  P.cg_all.ncell = (int)P.ctot;
+  P.cg_all.ncell = (int)std::min<int64_t>(P.ctot, INT32_MAX);
   nb_layout(P.ntot, P.cg_all, &P.nb);
   size_t off = 0;
   auto take = [&](size_t b) {
@@ -383,6 +488,7 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
     off += al(b);
     return o;
   };
+  const int64_t rows_x = P.off[kSegFftX][nsys], rows_g = P.off[kSegGather][nsys];
   P.o_tab = take(sizeof(SysDesc) * nsys);
   P.o_off = take(sizeof(int64_t) * (nsys + 1) * kSegCount);
   P.o_nb = take(P.nb.bytes);
@@ -393,9 +499,10 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
   P.o_spec = take(ces * std::max<int64_t>(P.stot, 1));
   P.o_tiles = take(es * std::max<int64_t>(P.ttot, 1));
   P.o_rpart = take(sizeof(double) * kRealNV * kRealSplit * std::max<int64_t>(P.ctot, 1));
-  P.o_cpart = take(sizeof(double) * 10 * std::max<int64_t>(P.off[kSegFftX][nsys], 1));
-  P.o_gpart = take(sizeof(double) * 9 * std::max<int64_t>(P.off[kSegGather][nsys], 1));
+  P.o_cpart = take(sizeof(double) * 10 * std::max<int64_t>(rows_x, 1));
+  P.o_gpart = take(sizeof(double) * 9 * std::max<int64_t>(rows_g, 1));
   P.o_red = take(sizeof(double) * kSysValues * nsys);
+  P.o_tickets = take(sizeof(int) * nsys);
   P.bytes = off;
   return P;
 }
@@ -407,24 +514,32 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
                        double* forces, double* dEdbox, char* ws, hipStream_t s, hipEvent_t* marks) {
   using CT = cplx<T>;
   const int nsys = P.nsys;
-  // upload the table and the offsets; the host copy lives until the stream passes it
-  const size_t tab_bytes = sizeof(SysDesc) * nsys;
-  const size_t off_bytes = sizeof(int64_t) * (nsys + 1) * kSegCount;
-  char* host = new char[tab_bytes + off_bytes];
-  std::memcpy(host, P.sys.data(), tab_bytes);
-  for (int k = 0; k < kSegCount; ++k)
-    std::memcpy(host + tab_bytes + sizeof(int64_t) * (nsys + 1) * k, P.off[k].data(),
-                sizeof(int64_t) * (nsys + 1));
-  if (hipMemcpyAsync(ws + P.o_tab, host, tab_bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipMemcpyAsync(ws + P.o_off, host + tab_bytes, off_bytes, hipMemcpyHostToDevice, s) !=
-          hipSuccess) {
-    delete[] host;
-    return ADMP_ERR_HIP;
+  SysDesc* tab = (SysDesc*)(ws + P.o_tab);
+  int64_t* offs = (int64_t*)(ws + P.o_off);
+  OneSystem one;
+  std::memset(&one, 0, sizeof(one));
+  if (nsys == 1) {
+    // the prologue writes the table from its arguments
+    one.d = P.sys[0];
+    for (int k = 0; k < kSegCount; ++k) one.tot[k] = P.off[k][1];
+  } else {
+    // upload the table and the offsets; the host copy lives until the stream passes it
+    const size_t tab_bytes = sizeof(SysDesc) * nsys;
+    const size_t off_bytes = sizeof(int64_t) * (nsys + 1) * kSegCount;
+    char* host = new char[tab_bytes + off_bytes];
+    std::memcpy(host, P.sys.data(), tab_bytes);
+    for (int k = 0; k < kSegCount; ++k)
+      std::memcpy(host + tab_bytes + sizeof(int64_t) * (nsys + 1) * k, P.off[k].data(),
+                  sizeof(int64_t) * (nsys + 1));
+    if (hipMemcpyAsync(tab, host, tab_bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(offs, host + tab_bytes, off_bytes, hipMemcpyHostToDevice, s) !=
+            hipSuccess) {
+      delete[] host;
+      return ADMP_ERR_HIP;
+    }
+    if (hipLaunchHostFunc(s, free_host, host) != hipSuccess) return ADMP_ERR_HIP;
   }
-  if (hipLaunchHostFunc(s, free_host, host) != hipSuccess) return ADMP_ERR_HIP;
-  const SysDesc* tab = (const SysDesc*)(ws + P.o_tab);
-  const int64_t* offs = (const int64_t*)(ws + P.o_off);
-  auto seg = [&](int k) { return offs + (int64_t)(nsys + 1) * k; };
+  auto seg = [&](int k) { return (const int64_t*)offs + (int64_t)(nsys + 1) * k; };
   auto mark = [&](int k) {
     if (marks) (void)hipEventRecord(marks[k], s);
   };
@@ -434,75 +549,76 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   int* count = (int*)(nbws + L.count);
   int* start = (int*)(nbws + L.start);
   double* gsplit = (double*)(ws + P.o_grad);
-  double* red = (double*)(ws + P.o_red);
   T* mesh = (T*)(ws + P.o_mesh);
+  T* tiles = (T*)(ws + P.o_tiles);
   CT* spec = (CT*)(ws + P.o_spec);
   P4<double>* ps = (P4<double>*)(ws + P.o_ps);
+  int* tickets = (int*)(ws + P.o_tickets);
   const unsigned nb_atoms = (unsigned)std::max<int64_t>((ntot + kBlock - 1) / kBlock, 1);
+  const int64_t rows_real = P.ctot * kRealSplit;
+  const int64_t rows_x = P.off[kSegFftX][nsys], rows_g = P.off[kSegGather][nsys];
 
   mark(0);
-  // ---- binning
-  const int64_t nkeys = P.ctot * kSub + 1;
-  k_fill<int><<<(unsigned)std::min<int64_t>((2 * nkeys + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(
-      count, 2 * nkeys, 0);
+  // ---- prologue, binning, in-cell order
+  const int64_t ncount = 2 * (P.ctot * kSub + 1);  // counters and scatter cursors
+  const int64_t nfill = std::max(ncount, P.mtot);
+  k_b_prologue<T><<<(unsigned)std::min<int64_t>((nfill + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
+      nsys == 1, one, tab, offs, count, ncount, mesh, P.mtot, tickets, nsys);
   if (ntot > 0)
     k_b_bin<<<nb_atoms, kBlock, 0, s>>>(ntot, nsys, seg(kSegAtom), tab, pos, (double*)(nbws + L.w),
                                         (int*)(nbws + L.cell_of), (int*)(ws + P.o_sys_of), count);
-  (void)launch_cell_sort(ntot, P.ctot, nbws, L, s);
-  if (ntot > 0) (void)launch_pack(q, ntot, nbws, L, ps, s);
+  (void)launch_cell_scatter(ntot, P.ctot, nbws, L, s);
+  if (ntot > 0)
+    k_b_sort_pack<<<(unsigned)P.ctot, kBlock, 0, s>>>(start, (const int*)(nbws + L.unsorted),
+                                                      (const double*)(nbws + L.w), q,
+                                                      (int*)(nbws + L.sorted), ps);
   mark(1);
   // ---- real space (also for empty batches: its partial rows feed the reductions)
+  double* rpart = (double*)(ws + P.o_rpart);
   if (P.ctot > 0) {
     dim3 grid((unsigned)P.ctot, kRealSplit);
     k_b_real<T><<<grid, kBlock, 0, s>>>(ntot, nsys, seg(kSegCell), tab, ps,
-                                        (const int*)(nbws + L.sorted), start, pos, gsplit,
-                                        (double*)(ws + P.o_rpart));
+                                        (const int*)(nbws + L.sorted), start, pos, gsplit, rpart);
   }
   mark(2);
-  // ---- spread
-  k_fill<T><<<(unsigned)std::min<int64_t>((P.mtot + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
-      mesh, P.mtot, T(0));
+  // ---- spread into per-cell tiles (tiled systems) or the mesh (the others)
   if (ntot > 0) {
     if (P.lds_tile > 0)
       k_b_spread_tiles<T><<<(unsigned)P.ctot, kBlock, P.lds_tile, s>>>(
-          nsys, seg(kSegCell), tab, pos, q, (const int*)(nbws + L.sorted), start,
-          (T*)(ws + P.o_tiles), mesh);
+          nsys, seg(kSegCell), tab, pos, q, (const int*)(nbws + L.sorted), start, tiles, mesh);
     if (P.any_untiled)
       k_b_spread_atoms<T><<<(unsigned)((ntot * 8 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
           ntot, (const int*)(ws + P.o_sys_of), tab, pos, q, mesh);
-    if (P.lds_tile > 0)
-      k_b_spread_reduce<T><<<(unsigned)std::min<int64_t>((P.mtot + kBlock - 1) / kBlock, 16384),
-                             kBlock, 0, s>>>(P.mtot, nsys, seg(kSegMesh), tab, (const T*)(ws + P.o_tiles),
-                                            mesh);
   }
   mark(3);
-  // ---- FFT, influence function, inverse FFT
+  // ---- tile reduction + FFT, influence function, inverse FFT
   const unsigned bz = (unsigned)P.off[kSegFftZ][nsys], by = (unsigned)P.off[kSegFftY][nsys],
-                 bx = (unsigned)P.off[kSegFftX][nsys];
-  k_b_r2c<T><<<bz, kBlock, P.lds_fft[0], s>>>(nsys, seg(kSegFftZ), tab, mesh, spec);
+                 bx = (unsigned)rows_x;
+  double* cpart = (double*)(ws + P.o_cpart);
+  k_b_r2c<T><<<bz, kBlock, P.lds_fft[0], s>>>(nsys, seg(kSegFftZ), tab, mesh, tiles, spec);
   k_b_cols_y<T, -1><<<by, kBlock, P.lds_fft[1], s>>>(nsys, seg(kSegFftY), tab, spec);
-  k_b_cols_x<T><<<bx, kBlock, P.lds_fft[2], s>>>(nsys, seg(kSegFftX), tab, spec,
-                                                 (double*)(ws + P.o_cpart));
+  k_b_cols_x<T><<<bx, kBlock, P.lds_fft[2], s>>>(nsys, seg(kSegFftX), tab, spec, cpart, rows_x);
   k_b_cols_y<T, 1><<<by, kBlock, P.lds_fft[1], s>>>(nsys, seg(kSegFftY), tab, spec);
   k_b_c2r<T><<<bz, kBlock, P.lds_fft[0], s>>>(nsys, seg(kSegFftZ), tab, spec, mesh);
   mark(4);
-  // ---- gather
-  const unsigned bg = (unsigned)P.off[kSegGather][nsys];
-  if (bg > 0)
-    k_b_gather<T><<<bg, kBlock, 0, s>>>(nsys, seg(kSegGather), tab, pos, q, mesh, gsplit,
-                                        (double*)(ws + P.o_gpart));
+  // ---- gather with the final forces
+  double* gpart = (double*)(ws + P.o_gpart);
+  if (rows_g > 0)
+    k_b_gather<T><<<(unsigned)rows_g, kBlock, 0, s>>>(nsys, seg(kSegGather), tab, pos, q, mesh,
+                                                      gsplit, kRealSplit, ntot, forces, gpart,
+                                                      rows_g);
   mark(5);
   // ---- per-system reductions and outputs
-  k_b_reduce_seg<<<nsys, kBlock, 0, s>>>((double*)(ws + P.o_rpart), kRealNV, seg(kSegCell),
-                                          kRealSplit, red, kSysValues, 0);
-  k_b_reduce_seg<<<nsys, kBlock, 0, s>>>((double*)(ws + P.o_cpart), 10, seg(kSegFftX), 1, red,
-                                          kSysValues, kRealNV);
-  k_b_reduce_seg<<<nsys, kBlock, 0, s>>>((double*)(ws + P.o_gpart), 9, seg(kSegGather), 1, red,
-                                          kSysValues, kRealNV + 10);
-  k_b_finalize<<<(nsys + 63) / 64, 64, 0, s>>>(nsys, tab, red, energy, dEdbox);
-  if (ntot > 0)
-    k_neg_sum_rows<<<(unsigned)std::min<int64_t>((3 * ntot + kBlock - 1) / kBlock, 8192), kBlock,
-                     0, s>>>(gsplit, kRealSplit, 3 * ntot, forces);
+  PartialSets sets;
+  sets.real = rpart;
+  sets.recip = cpart;
+  sets.gath = gpart;
+  sets.rows_real = rows_real;
+  sets.rows_recip = rows_x;
+  sets.rows_gath = rows_g;
+  sets.nsplit = kRealSplit;
+  k_b_reduce_finalize<<<dim3(nsys, kRedGroups), kBlock, 0, s>>>(
+      nsys, offs, tab, sets, (double*)(ws + P.o_red), tickets, energy, dEdbox);
   mark(6);
   return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
 }

@@ -44,7 +44,7 @@ struct BatchPlan {
   CellGrid cg_all;     // only ncell is used by the shared binning helpers
   // workspace byte offsets
   size_t o_tab, o_off, o_nb, o_sys_of, o_ps, o_grad, o_mesh, o_spec, o_tiles, o_rpart, o_cpart,
-      o_gpart, o_red, bytes;
+      o_gpart, o_red, o_tickets, bytes;
   int status;          // ADMP_OK or an error code from planning
 };
 

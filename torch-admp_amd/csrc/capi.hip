@@ -68,48 +68,6 @@ PmeLayout pme_layout(const int32_t* k, int32_t precision) {
   return L;
 }
 
-// Fused workspace: neighbour list + sorted (x, y, z, q) + gradient + mesh +
-// PME workspace + real-space partials + reduced scalars.
-struct FusedLayout {
-  NbLayout nb;
-  size_t ps, grad, mesh, pme, rpart, red, tiles, bytes;
-  PmeLayout pl;
-  TileGeom tg;
-};
-
-FusedLayout fused_layout(int64_t n, const CellGrid& cg, const int32_t* k, int32_t precision) {
-  FusedLayout F;
-  nb_layout(n, cg, &F.nb);
-  size_t off = align256(F.nb.bytes);
-  F.ps = off;
-  off += align256((size_t)std::max<int64_t>(n, 1) * 4 * sizeof(double));
-  F.grad = off;
-  off += align256((size_t)kRealSplit * std::max<int64_t>(n, 1) * 3 * sizeof(double));
-  F.mesh = off;
-  off += align256((size_t)k[0] * k[1] * k[2] * elem_size(precision));
-  F.pl = pme_layout(k, precision);
-  F.pme = off;
-  off += align256(F.pl.bytes);
-  F.rpart = off;
-  off += align256((size_t)(std::max<int64_t>(real_fused_blocks(n), (int64_t)kRealSplit * cg.ncell) + 1) *
-                  kRealNV * sizeof(double));
-  F.red = off;
-  off += align256(64 * sizeof(double));
-  F.tg = make_tile_geom(cg, k, elem_size(precision));
-  F.tiles = off;
-  if (F.tg.ok) off += align256((size_t)cg.ncell * F.tg.ntile * elem_size(precision));
-  F.bytes = off;
-  return F;
-}
-
-// Energies and dE/dh of one system from its reduced partial sums (layout in
-// finalize_system).
-__global__ void k_finalize(const double* __restrict__ red, Geom g, double kappa, double eps,
-                           double* __restrict__ energy, double* __restrict__ dEdbox) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  finalize_system(red, red + kRealNV, red + kRealNV + 10, g, kappa, eps, energy, dEdbox);
-}
-
 // Stage timing (profiling aid): hipEvent pairs around the fused stages.
 struct Profiler {
   bool on = false;
@@ -135,57 +93,6 @@ StageMarks stage_marks(hipStream_t s) {
     ++g_prof.calls;
   }
   return m;
-}
-
-template <typename T>
-static hipError_t fused_impl(const double* pos, const double* q, int64_t n, const double* box,
-                             double cutoff, double kappa, const int32_t* kmesh, int32_t conv,
-                             double eps, double* energy, double* forces, double* dEdbox,
-                             char* ws, const FusedLayout& F, const CellGrid& cg, hipStream_t s) {
-  (void)cutoff;
-  const Geom g = make_geom(box, kmesh);
-  double* grad = (double*)(ws + F.grad);
-  double* red = (double*)(ws + F.red);
-  T* mesh = (T*)(ws + F.mesh);
-  char* pws = ws + F.pme;
-  double* ppart = (double*)(pws + F.pl.partials);
-  P4<double>* ps = (P4<double>*)(ws + F.ps);
-  StageMarks pm = stage_marks(s);
-  pm.mark(0);
-  hipError_t e = launch_bin(pos, n, cg, ws, F.nb, s);
-  if (e == hipSuccess) e = launch_pack(q, n, ws, F.nb, ps, s);
-  pm.mark(1);
-  const bool cells = real_cells_ok(cg);
-  if (e == hipSuccess) {
-    if (cells)
-      e = launch_real_cells<T>(pos, n, cg, kappa, eps, ws, F.nb, ps, grad, (double*)(ws + F.rpart), s);
-    else
-      e = launch_real_fused<T>(pos, n, cg, kappa, eps, ws, F.nb, ps, grad, (double*)(ws + F.rpart), s);
-  }
-  pm.mark(2);
-  if (e == hipSuccess) {
-    if (F.tg.ok)
-      e = launch_spread_tiled<T>(pos, q, n, g, F.tg, cg.ncell, (const int*)(ws + F.nb.sorted),
-                                 (const int*)(ws + F.nb.start), (T*)(ws + F.tiles), mesh, s);
-    else
-      e = launch_spread<T>(pos, q, n, g, mesh, s);
-  }
-  pm.mark(3);
-  if (e == hipSuccess)
-    e = launch_convolve<T>(mesh, g, conv, kappa, eps, 1.0, mesh, red + 18, red + 19,
-                           (cplx<T>*)(pws + F.pl.spec), ppart, s);
-  pm.mark(4);
-  if (e == hipSuccess) e = launch_gather<T>(pos, q, n, g, mesh, grad, nullptr, red + 28, 1, ppart, s);
-  pm.mark(5);
-  if (e != hipSuccess) return e;
-  const int nbr = n > 0 ? (cells ? kRealSplit * cg.ncell : (int)real_fused_blocks(n)) : 0;
-  k_reduce_partials<<<1, kReduceThreads, 0, s>>>((double*)(ws + F.rpart), nbr, kRealNV, 1.0, red, red + 1);
-  k_finalize<<<1, 64, 0, s>>>(red, g, kappa, eps, energy, dEdbox);
-  if (n > 0)
-    k_neg_sum_rows<<<(unsigned)std::min<int64_t>((3 * n + kBlock - 1) / kBlock, 4096), kBlock, 0,
-                     s>>>(grad, cells ? kRealSplit : 1, 3 * n, forces);
-  pm.mark(6);
-  return hipGetLastError();
 }
 
 bool qeq_problem(const admp_qeq_problem* pr, bool need_solve, QeqProblem* P) {
@@ -497,13 +404,12 @@ int admp_qeq_solve(const admp_qeq_problem* problem, const double* q0, double tol
 }
 
 // ------------------------------------------------------------------- fused
+// A batch of one: the first kernel writes the system table from its arguments.
 int admp_pme_fused_workspace_bytes(int64_t n, const double* box, double cutoff,
                                    const int32_t* kmesh, int32_t precision, size_t* bytes) {
   if (n < 0 || !box_ok(box) || !(cutoff > 0.0) || !kmesh_ok(kmesh) || !bytes)
     return ADMP_ERR_INVALID;
-  if (precision != ADMP_F32 && precision != ADMP_F64) return ADMP_ERR_INVALID;
-  *bytes = fused_layout(n, make_grid(box, cutoff, false), kmesh, precision).bytes;
-  return ADMP_OK;
+  return admp_pme_batch_workspace_bytes(1, &n, box, &cutoff, kmesh, precision, bytes);
 }
 
 int admp_pme_fused(const double* pos, const double* q, int64_t n, const double* box,
@@ -513,21 +419,8 @@ int admp_pme_fused(const double* pos, const double* q, int64_t n, const double* 
   if (n < 0 || !box_ok(box) || !(cutoff > 0.0) || !(kappa > 0.0) || !kmesh_ok(kmesh) ||
       !energy || !workspace || (n > 0 && (!pos || !q || !forces)))
     return ADMP_ERR_INVALID;
-  if (convention != ADMP_KCONV_REFERENCE && convention != ADMP_KCONV_COLUMNS)
-    return ADMP_ERR_INVALID;
-  if (n > (int64_t)1 << 30) return ADMP_ERR_UNSUPPORTED;
-  const CellGrid cg = make_grid(box, cutoff, false);
-  if (precision != ADMP_F32 && precision != ADMP_F64) return ADMP_ERR_INVALID;
-  const FusedLayout F = fused_layout(n, cg, kmesh, precision);
-  if (ws_bytes < F.bytes) return ADMP_ERR_INVALID;
-  hipStream_t s = (hipStream_t)stream;
-  if (precision == ADMP_F32)
-    return hip_status(fused_impl<float>(pos, q, n, box, cutoff, kappa, kmesh, convention,
-                                        dielectric, energy, forces, dEdbox, (char*)workspace, F,
-                                        cg, s));
-  return hip_status(fused_impl<double>(pos, q, n, box, cutoff, kappa, kmesh, convention,
-                                       dielectric, energy, forces, dEdbox, (char*)workspace, F,
-                                       cg, s));
+  return admp_pme_batch(1, &n, box, &cutoff, &kappa, kmesh, precision, convention, dielectric,
+                        pos, q, energy, forces, dEdbox, workspace, ws_bytes, stream);
 }
 
 // ------------------------------------------------------------------- batch

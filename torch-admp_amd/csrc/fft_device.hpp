@@ -263,10 +263,10 @@ __device__ __forceinline__ void add_dEdA(double acc[10], const InflParams& P, in
 // Pass 1: real-to-complex along z for rows [row0, row0 + 2C).  Rows 2c and
 // 2c+1 are packed as the real and imaginary parts of one complex line; after
 // the FFT A_k = (Z_k + conj Z_{L-k}) / 2 and B_k = (Z_k - conj Z_{L-k}) / (2i).
-template <typename T>
-__device__ __forceinline__ void r2c_rows(const T* __restrict__ in, cplx<T>* __restrict__ out,
-                                         int64_t nrows, int64_t row0, int L, int C,
-                                         const Radix& rd) {
+// load(row, z) supplies the real input of row `row`, element z.
+template <typename T, typename Load>
+__device__ __forceinline__ void r2c_rows(Load load, cplx<T>* __restrict__ out, int64_t nrows,
+                                         int64_t row0, int L, int C, const Radix& rd) {
   using CT = cplx<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   CT* tw = reinterpret_cast<CT*>(smem_raw);
@@ -276,7 +276,7 @@ __device__ __forceinline__ void r2c_rows(const T* __restrict__ in, cplx<T>* __re
   for (int idx = threadIdx.x; idx < 2 * C * L; idx += blockDim.x) {
     const int rl = idx / L, z = idx % L;
     const int64_t row = row0 + rl;
-    const T v = row < nrows ? in[row * L + z] : T(0);
+    const T v = row < nrows ? load(row, z) : T(0);
     T* dst = reinterpret_cast<T*>(&a[z * C + (rl >> 1)]);
     dst[rl & 1] = v;
   }

@@ -469,102 +469,6 @@ __global__ void __launch_bounds__(kBlock) k_pack_sorted(int64_t n, const int* __
   ps[t] = v;
 }
 
-// 32 lanes per atom (atoms taken in cell order, 8 per block); lane s owns the
-// stencil entries s, s + 32, ...  Per-lane sums are combined with shuffles
-// inside the 32-lane half-wave, then per block.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_real_fused(int64_t n, CellGrid cg, T kappa, T eps,
-                                                       const P4<double>* __restrict__ ps,
-                                                       const int* __restrict__ sorted,
-                                                       const int* __restrict__ cell_of,
-                                                       const int* __restrict__ start,
-                                                       const double* __restrict__ pos,
-                                                       double* __restrict__ grad,
-                                                       double* __restrict__ partials) {
-  __shared__ double red[kRealNV * (kBlock / 64)];
-  const int l32 = threadIdx.x & 31;
-  const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 5) + (threadIdx.x >> 5);
-  const bool valid = t < n;
-  T gx = 0, gy = 0, gz = 0, e = 0;
-  T vxx = 0, vyy = 0, vzz = 0, vxy = 0, vxz = 0, vyz = 0;
-  int i = 0;
-  P4<double> pi;
-  pi.x = pi.y = pi.z = pi.q = 0.0;
-  if (valid) {
-    i = sorted[t];
-    pi = ps[t];
-    const int cid = cell_of[i];
-    const int ci[3] = {cid / (cg.nc[1] * cg.nc[2]), (cid / cg.nc[2]) % cg.nc[1], cid % cg.nc[2]};
-    const T rc2 = (T)cg.rc2;
-    const T rmin2 = (T)1e-4;
-    const T c_exp = (T)(2.0 / kSqrtPi) * kappa;
-    const T k2 = kappa * kappa;
-    const int l0 = stencil_len(cg, 0), l1 = stencil_len(cg, 1), l2 = stencil_len(cg, 2);
-    const int nst = l0 * l1 * l2;
-    for (int st = l32; st < nst; st += 32) {
-      const int o2 = st % l2, o1 = (st / l2) % l1, o0 = st / (l1 * l2);
-      int c0, s0, c1, s1, c2, s2;
-      stencil_at(cg, 0, ci[0], o0, &c0, &s0);
-      stencil_at(cg, 1, ci[1], o1, &c1, &s1);
-      stencil_at(cg, 2, ci[2], o2, &c2, &s2);
-      const double shx = (s0 * cg.h[0] + s1 * cg.h[3] + s2 * cg.h[6]) - pi.x;
-      const double shy = (s0 * cg.h[1] + s1 * cg.h[4] + s2 * cg.h[7]) - pi.y;
-      const double shz = (s0 * cg.h[2] + s1 * cg.h[5] + s2 * cg.h[8]) - pi.z;
-      const T qi = (T)pi.q;
-      const int cc = (c0 * cg.nc[1] + c1) * cg.nc[2] + c2;
-      const int xe = start[(cc + 1) * kSub];
-      for (int x = start[cc * kSub]; x < xe; ++x) {
-        const P4<double> pj = ps[x];
-        const T dx = (T)(pj.x + shx), dy = (T)(pj.y + shy), dz = (T)(pj.z + shz);
-        const T r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < rc2 && r2 > rmin2) {
-          const T r = sqrt(r2);
-          const T ir = T(1) / r;
-          const T qq = eps * qi * (T)pj.q;
-          const T ec = erfc(kappa * r);
-          // (de/dr) / r with e = qq erfc(kappa r) / r
-          const T fs = -qq * (c_exp * exp(-k2 * r2) + ec * ir) * ir * ir;
-          e += qq * ec * ir;
-          gx -= fs * dx;
-          gy -= fs * dy;
-          gz -= fs * dz;
-          vxx += fs * dx * dx;
-          vyy += fs * dy * dy;
-          vzz += fs * dz * dz;
-          vxy += fs * dx * dy;
-          vxz += fs * dx * dz;
-          vyz += fs * dy * dz;
-        }
-      }
-    }
-  }
-  double v[10] = {(double)e, (double)vxx, (double)vyy, (double)vzz, (double)vxy,
-                  (double)vxz, (double)vyz, (double)gx, (double)gy, (double)gz};
-#pragma unroll
-  for (int off = 16; off > 0; off >>= 1)
-#pragma unroll
-    for (int k = 0; k < 10; ++k) v[k] += __shfl_xor(v[k], off, 64);
-  double acc[kRealNV];
-#pragma unroll
-  for (int k = 0; k < kRealNV; ++k) acc[k] = 0.0;
-  if (valid && l32 == 0) {
-    grad[3 * i] = v[7];
-    grad[3 * i + 1] = v[8];
-    grad[3 * i + 2] = v[9];
-#pragma unroll
-    for (int k = 0; k < 7; ++k) acc[k] = 0.5 * v[k];
-    const double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b) acc[7 + a * 3 + b] = r[a] * v[7 + b];
-    acc[16] = (double)pi.q * (double)pi.q;
-    acc[17] = (double)pi.q;
-  }
-  block_sum<kRealNV>(acc, red);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < kRealNV; ++k) partials[(int64_t)blockIdx.x * kRealNV + k] = acc[k];
-}
 
 
 // ---------------------------------------------------------------------------
@@ -603,17 +507,24 @@ hipError_t launch_bin(const double* pos, int64_t n, const CellGrid& cg, char* ws
   return launch_cell_sort(n, cg.ncell, ws, L, s);
 }
 
-hipError_t launch_cell_sort(int64_t n, int64_t ncell, char* ws, const NbLayout& L, hipStream_t s) {
+hipError_t launch_cell_scatter(int64_t n, int64_t ncell, char* ws, const NbLayout& L,
+                               hipStream_t s) {
   int* count = (int*)(ws + L.count);
   int* start = (int*)(ws + L.start);
   int* cursor = (int*)(ws + L.cursor);
   launch_scan<int>(count, ncell * kSub, start, (int*)(ws + L.scan), s);
-  if (n > 0) {
-    const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
-    k_nb_scatter<<<nb, kBlock, 0, s>>>(n, (int*)(ws + L.cell_of), start, cursor,
-                                       (int*)(ws + L.unsorted));
-    k_nb_rank<<<nb, kBlock, 0, s>>>(n, (int*)(ws + L.cell_of), start, (int*)(ws + L.unsorted),
-                                    (int*)(ws + L.sorted));
+  if (n > 0)
+    k_nb_scatter<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        n, (int*)(ws + L.cell_of), start, cursor, (int*)(ws + L.unsorted));
+  return hipGetLastError();
+}
+
+hipError_t launch_cell_sort(int64_t n, int64_t ncell, char* ws, const NbLayout& L, hipStream_t s) {
+  hipError_t e = launch_cell_scatter(n, ncell, ws, L, s);
+  if (e == hipSuccess && n > 0) {
+    const int* start = (const int*)(ws + L.start);
+    k_nb_rank<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        n, (int*)(ws + L.cell_of), start, (int*)(ws + L.unsorted), (int*)(ws + L.sorted));
   }
   return hipGetLastError();
 }
@@ -673,54 +584,6 @@ hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const
   return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------
-// Fused real space, cell-staged.  Work-group (c, s) takes home cell c and every
-// S-th candidate (split s) of its stencil.  The candidates, shifted to the
-// right periodic image, are staged once in LDS in float64; home atoms map onto
-// lanes 2-D (Hg atoms x 64/Hg candidate phases), and each wavefront compacts
-// the in-cutoff (home, candidate) pairs into an LDS queue so the erfc / exp
-// arithmetic always runs on 64 useful lanes.  Forces of the home atoms gather
-// in LDS (atomics) and leave as one row per split: gsplit[s][i][3].
-// Per-block partials as in k_real_fused (kRealNV values).
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_real_cells(int64_t n, CellGrid cg, T kappa, T eps,
-                                                       const P4<double>* __restrict__ ps,
-                                                       const int* __restrict__ sorted,
-                                                       const int* __restrict__ start,
-                                                       const double* __restrict__ pos,
-                                                       double* __restrict__ gsplit,
-                                                       double* __restrict__ partials) {
-  real_cell_body<T>(cg, 0, blockIdx.x, n, kappa, eps, ps, sorted, start, pos, gsplit,
-                    partials + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * kRealNV);
-}
-
-bool real_cells_ok(const CellGrid& cg) {
-  int nst = 1;
-  for (int d = 0; d < 3; ++d) nst *= cg.nc[d] >= 3 ? 3 : 2 * cg.nimg[d] + 1;
-  return !cg.obc && nst <= kMaxStencil;
-}
-
-template <typename T>
-hipError_t launch_real_cells(const double* pos, int64_t n, const CellGrid& cg, double kappa,
-                             double eps, const char* ws, const NbLayout& L,
-                             const P4<double>* ps, double* gsplit, double* partials,
-                             hipStream_t s) {
-  if (n > 0) {
-    dim3 grid(cg.ncell, kRealSplit);
-    k_real_cells<T><<<grid, kBlock, 0, s>>>(n, cg, (T)kappa, (T)eps, ps,
-                                            (const int*)(ws + L.sorted),
-                                            (const int*)(ws + L.start), pos, gsplit, partials);
-  }
-  return hipGetLastError();
-}
-
-template hipError_t launch_real_cells<float>(const double*, int64_t, const CellGrid&, double,
-                                             double, const char*, const NbLayout&,
-                                             const P4<double>*, double*, double*, hipStream_t);
-template hipError_t launch_real_cells<double>(const double*, int64_t, const CellGrid&, double,
-                                              double, const char*, const NbLayout&,
-                                              const P4<double>*, double*, double*, hipStream_t);
-
 hipError_t launch_pack(const double* q, int64_t n, char* ws, const NbLayout& L, P4<double>* ps,
                        hipStream_t s) {
   if (n > 0)
@@ -728,23 +591,5 @@ hipError_t launch_pack(const double* q, int64_t n, char* ws, const NbLayout& L, 
         n, (const int*)(ws + L.sorted), (const double*)(ws + L.w), q, ps);
   return hipGetLastError();
 }
-
-template <typename T>
-hipError_t launch_real_fused(const double* pos, int64_t n, const CellGrid& cg, double kappa,
-                             double eps, char* ws, const NbLayout& L, const P4<double>* ps,
-                             double* grad, double* partials, hipStream_t s) {
-  if (n > 0)
-    k_real_fused<T><<<(unsigned)real_fused_blocks(n), kBlock, 0, s>>>(
-        n, cg, (T)kappa, (T)eps, ps, (const int*)(ws + L.sorted), (const int*)(ws + L.cell_of),
-        (const int*)(ws + L.start), pos, grad, partials);
-  return hipGetLastError();
-}
-
-template hipError_t launch_real_fused<float>(const double*, int64_t, const CellGrid&, double,
-                                             double, char*, const NbLayout&, const P4<double>*,
-                                             double*, double*, hipStream_t);
-template hipError_t launch_real_fused<double>(const double*, int64_t, const CellGrid&, double,
-                                              double, char*, const NbLayout&, const P4<double>*,
-                                              double*, double*, hipStream_t);
 
 }  // namespace admp

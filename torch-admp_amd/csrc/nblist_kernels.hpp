@@ -11,7 +11,6 @@ constexpr int kRealNV = 18;               // fused real-space partial values per
 constexpr int kPairBlocksPerFrame = 256;  // fixed grid of the pair-energy reduction
 constexpr int kSub = 8;                   // sub-counters per cell (atomic contention)
 constexpr int64_t kMaxCells = 1 << 21;    // cap on the cell-grid size
-constexpr int kRealAtomsPerBlock = kBlock / 32;
 constexpr int kCandCap = 512;     // staged candidates per pass of the cell kernel (16 KiB)
 constexpr int kHomeCap = 128;     // home atoms per pass of the cell kernel
 constexpr int kMaxStencil = 128;  // stencil entries the cell kernel supports
@@ -20,10 +19,6 @@ constexpr int kFilterUnroll = 4;  // candidates tested per lane between queue ch
 constexpr int kQueueCap = 64 + 64 * kFilterUnroll;  // per-wavefront pair queue entries
 
 // cell c holds the sorted slots [start[c * kSub], start[(c + 1) * kSub])
-inline int64_t real_fused_blocks(int64_t n) {
-  return (n + kRealAtomsPerBlock - 1) / kRealAtomsPerBlock;
-}
-
 struct CellGrid {
   double h[9];
   double a[9];
@@ -51,6 +46,9 @@ hipError_t launch_bin(const double* pos, int64_t n, const CellGrid& cg, char* ws
                       const NbLayout& L, hipStream_t s);
 // counting sort of binned atoms (cell_of and counts filled): start, sorted
 hipError_t launch_cell_sort(int64_t n, int64_t ncell, char* ws, const NbLayout& L, hipStream_t s);
+// its first half: start and the unordered cell lists in `unsorted`
+hipError_t launch_cell_scatter(int64_t n, int64_t ncell, char* ws, const NbLayout& L,
+                               hipStream_t s);
 // mode 0: count pairs and scan the per-atom counts (total at offsets[n]);
 // mode 1: write the pairs
 hipError_t launch_pairs(int mode, int64_t n, const CellGrid& cg, char* ws, const NbLayout& L,
@@ -66,19 +64,4 @@ hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const
                               double* gds, double* gq, double* partials, hipStream_t s);
 hipError_t launch_pack(const double* q, int64_t n, char* ws, const NbLayout& L, P4<double>* ps,
                        hipStream_t s);
-// per-block partials of the fused real-space kernel: real_fused_blocks(n) rows of kRealNV
-template <typename T>
-hipError_t launch_real_fused(const double* pos, int64_t n, const CellGrid& cg, double kappa,
-                             double eps, char* ws, const NbLayout& L, const P4<double>* ps,
-                             double* grad, double* partials, hipStream_t s);
-
-// cell-staged real space (grid ncell x kRealSplit): force rows gsplit[kRealSplit][n][3],
-// partials kRealSplit * ncell rows of kRealNV
-bool real_cells_ok(const CellGrid& cg);
-template <typename T>
-hipError_t launch_real_cells(const double* pos, int64_t n, const CellGrid& cg, double kappa,
-                             double eps, const char* ws, const NbLayout& L,
-                             const P4<double>* ps, double* gsplit, double* partials,
-                             hipStream_t s);
-
 }  // namespace admp

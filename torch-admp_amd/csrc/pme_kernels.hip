@@ -93,7 +93,8 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_fft_r2c_z(const T* __restrict__ in,
                                                       cplx<T>* __restrict__ out, int64_t nrows,
                                                       int L, int C, Radix rd) {
-  r2c_rows<T>(in, out, nrows, (int64_t)blockIdx.x * 2 * C, L, C, rd);
+  r2c_rows<T>([=](int64_t row, int z) { return in[row * L + z]; }, out, nrows,
+              (int64_t)blockIdx.x * 2 * C, L, C, rd);
 }
 
 template <typename T>

@@ -83,14 +83,60 @@ __device__ __forceinline__ void pair_eval(const Staged<T>& pi, const Staged<T>& 
   acc.vyz += fs * dy * dz;
 }
 
+// Stencil entries [st0, st0 + kMaxStencil) of home cell ci: first slot, atom
+// count (prefix-summed into st_pre) and image shift of every entry.
+// Returns the number of entries of this chunk.
+__device__ __forceinline__ int load_stencil_chunk(const CellGrid& cg, int64_t cell0,
+                                                  const int ci[3], int st0, int nst,
+                                                  const int* __restrict__ start, int* st_pre,
+                                                  int* st_beg, double* st_sh) {
+  const int l1 = stencil_len(cg, 1), l2 = stencil_len(cg, 2);
+  const int m = min(kMaxStencil, nst - st0);
+  const int lane = threadIdx.x & 63;
+  for (int e = threadIdx.x; e < m; e += blockDim.x) {
+    const int st = st0 + e;
+    const int o2 = st % l2, o1 = (st / l2) % l1, o0 = st / (l1 * l2);
+    int c0, s0, c1, s1, c2, s2;
+    stencil_at(cg, 0, ci[0], o0, &c0, &s0);
+    stencil_at(cg, 1, ci[1], o1, &c1, &s1);
+    stencil_at(cg, 2, ci[2], o2, &c2, &s2);
+    const int64_t cc = cell0 + (c0 * cg.nc[1] + c1) * cg.nc[2] + c2;
+    st_beg[e] = start[cc * kSub];
+    st_pre[e + 1] = start[(cc + 1) * kSub] - start[cc * kSub];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) st_sh[e * 3 + d] = s0 * cg.h[d] + s1 * cg.h[3 + d] + s2 * cg.h[6 + d];
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    // inclusive prefix of the counts st_pre[1..m] by wavefront scan, 64 at a time
+    int carry = 0;
+    for (int b0 = 1; b0 <= m; b0 += 64) {
+      const int idx = b0 + lane;
+      int v = idx <= m ? st_pre[idx] : 0;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+      }
+      if (idx <= m) st_pre[idx] = v + carry;
+      carry += __shfl(v, 63, 64);
+    }
+    if (lane == 0) st_pre[0] = 0;
+  }
+  __syncthreads();
+  return m;
+}
+
 // One work-group's share of the cell-staged real-space sum: home cell lc of a
 // grid whose cells start at global index cell0 (0 for a single system), the
 // candidates of stencil split blockIdx.y out of gridDim.y.  Candidates are
 // staged in LDS as hi/lo pairs relative to the home-cell origin; lanes of a
 // wavefront filter (home, candidate) pairs against the cutoff and compact the
 // survivors into a per-wavefront queue that is evaluated 64 pairs at a time.
-// Home forces go to gsplit[split][i][3] (rows of ntot atoms); the block's
-// energy, virial, r x force, sum q^2 and sum q go to partial_row[kRealNV].
+// Stencils longer than kMaxStencil entries (boxes far below the cutoff) are
+// walked in chunks.  Home forces go to gsplit[split][i][3] (rows of n atoms);
+// the block's energy, virial, r x force, sum q^2 and sum q (kRealNV values)
+// go to partial[k * pstride].
 template <typename T>
 __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0, int lc,
                                                int64_t n, T kappa, T eps,
@@ -99,10 +145,9 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
                                                const int* __restrict__ start,
                                                const double* __restrict__ pos,
                                                double* __restrict__ gsplit,
-                                               double* __restrict__ partial_row) {
+                                               double* __restrict__ partial, int64_t pstride) {
   __shared__ Staged<T> cand[kCandCap];
   __shared__ Staged<T> home[kHomeCap];
-  __shared__ double origin[3];
   __shared__ double facc[kHomeCap * 3];  // float64 LDS atomics in both precisions
   __shared__ int queue[(kBlock / 64) * kQueueCap];
   __shared__ int st_pre[kMaxStencil + 1];
@@ -113,50 +158,16 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
   const int split = blockIdx.y, S = gridDim.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ci[3] = {lc / (cg.nc[1] * cg.nc[2]), (lc / cg.nc[2]) % cg.nc[1], lc % cg.nc[2]};
-  const int l0 = stencil_len(cg, 0), l1 = stencil_len(cg, 1), l2 = stencil_len(cg, 2);
-  const int nst = l0 * l1 * l2;
-  if (threadIdx.x < nst) {
-    const int st = threadIdx.x;
-    const int o2 = st % l2, o1 = (st / l2) % l1, o0 = st / (l1 * l2);
-    int c0, s0, c1, s1, c2, s2;
-    stencil_at(cg, 0, ci[0], o0, &c0, &s0);
-    stencil_at(cg, 1, ci[1], o1, &c1, &s1);
-    stencil_at(cg, 2, ci[2], o2, &c2, &s2);
-    const int64_t cc = cell0 + (c0 * cg.nc[1] + c1) * cg.nc[2] + c2;
-    st_beg[st] = start[cc * kSub];
-    st_pre[st + 1] = start[(cc + 1) * kSub] - start[cc * kSub];
+  const int nst = stencil_len(cg, 0) * stencil_len(cg, 1) * stencil_len(cg, 2);
+  double o3[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) st_sh[st * 3 + d] = s0 * cg.h[d] + s1 * cg.h[3 + d] + s2 * cg.h[6 + d];
-  }
-  __syncthreads();
-  if (wave == 0) {
-    // inclusive prefix of the counts st_pre[1..nst] by wavefront scan, 64 at a time
-    int carry = 0;
-    for (int b0 = 1; b0 <= nst; b0 += 64) {
-      const int idx = b0 + lane;
-      int v = idx <= nst ? st_pre[idx] : 0;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int u = __shfl_up(v, off, 64);
-        if (lane >= off) v += u;
-      }
-      if (idx <= nst) st_pre[idx] = v + carry;
-      carry += __shfl(v, 63, 64);
-    }
-    if (lane == 0) st_pre[0] = 0;
-  }
-  __syncthreads();
-  if (threadIdx.x < 3) {
-    const int d = threadIdx.x;
+  for (int d = 0; d < 3; ++d) {
     double o = 0.0;
 #pragma unroll
     for (int e = 0; e < 3; ++e) o += ((double)ci[e] / cg.nc[e]) * cg.h[e * 3 + d];
-    origin[d] = o;
+    o3[d] = o;
   }
-  __syncthreads();
-  const double ox = origin[0], oy = origin[1], oz = origin[2];
-  const int ctot = st_pre[nst];
-  const int cs = ctot > split ? (ctot - split + S - 1) / S : 0;
+  const double ox = o3[0], oy = o3[1], oz = o3[2];
   const int64_t c = cell0 + lc;
   const int hb = start[c * kSub], he = start[(c + 1) * kSub];
   const T rc2 = (T)cg.rc2, rmin2 = (T)1e-4;
@@ -168,6 +179,9 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
 #pragma unroll
   for (int k = 0; k < kRealNV; ++k) acc[k] = 0.0;
   int* q = queue + wave * kQueueCap;
+  const bool one_chunk = nst <= kMaxStencil;
+  int m = 0;
+  if (one_chunk) m = load_stencil_chunk(cg, cell0, ci, 0, nst, start, st_pre, st_beg, st_sh);
 
   for (int h0 = hb; h0 < he; h0 += kHomeCap) {
     const int hc = min(kHomeCap, he - h0);
@@ -176,83 +190,91 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
       home[t] = stage<T>(p.x - ox, p.y - oy, p.z - oz, p.q);
       facc[3 * t] = facc[3 * t + 1] = facc[3 * t + 2] = 0.0;
     }
-    for (int k0 = 0; k0 < cs; k0 += kCandCap) {
-      const int cc = min(kCandCap, cs - k0);
-      for (int kk = threadIdx.x; kk < cc; kk += blockDim.x) {
-        const int kg = split + S * (k0 + kk);
-        int lo = 0, hi = nst;  // last st with st_pre[st] <= kg
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (st_pre[mid] <= kg) lo = mid;
-          else hi = mid;
-        }
-        const P4<double> p = ps[st_beg[lo] + (kg - st_pre[lo])];
-        cand[kk] = stage<T>((p.x + st_sh[lo * 3]) - ox, (p.y + st_sh[lo * 3 + 1]) - oy,
-                            (p.z + st_sh[lo * 3 + 2]) - oz, p.q);
+    for (int st0 = 0; st0 < nst; st0 += kMaxStencil) {
+      if (!one_chunk) {
+        __syncthreads();
+        m = load_stencil_chunk(cg, cell0, ci, st0, nst, start, st_pre, st_beg, st_sh);
       }
-      __syncthreads();
-      for (int g0 = 0; g0 < hc; g0 += 64) {
-        const int hg = min(64, hc - g0);
-        const int nph = 64 / hg;
-        const int hl = lane % hg, ph = lane / hg;
-        const bool active = ph < nph;
-        const Staged<T> pi = home[g0 + hl];
-        const int stride = nph * (kBlock / 64);
-        const int nit = (cc + stride - 1) / stride;
-        int qn = 0;
-        for (int j0 = 0; j0 < nit; j0 += kFilterUnroll) {
-          bool pred[kFilterUnroll];
-          int kks[kFilterUnroll];
+      const int ctot = st_pre[m];
+      const int cs = ctot > split ? (ctot - split + S - 1) / S : 0;
+      for (int k0 = 0; k0 < cs; k0 += kCandCap) {
+        const int cc = min(kCandCap, cs - k0);
+        for (int kk = threadIdx.x; kk < cc; kk += blockDim.x) {
+          const int kg = split + S * (k0 + kk);
+          int lo = 0, hi = m;  // last entry with st_pre[entry] <= kg
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (st_pre[mid] <= kg) lo = mid;
+            else hi = mid;
+          }
+          const P4<double> p = ps[st_beg[lo] + (kg - st_pre[lo])];
+          cand[kk] = stage<T>((p.x + st_sh[lo * 3]) - ox, (p.y + st_sh[lo * 3 + 1]) - oy,
+                              (p.z + st_sh[lo * 3 + 2]) - oz, p.q);
+        }
+        __syncthreads();
+        for (int g0 = 0; g0 < hc; g0 += 64) {
+          const int hg = min(64, hc - g0);
+          const int nph = 64 / hg;
+          const int hl = lane % hg, ph = lane / hg;
+          const bool active = ph < nph;
+          const Staged<T> pi = home[g0 + hl];
+          const int stride = nph * (kBlock / 64);
+          const int nit = (cc + stride - 1) / stride;
+          int qn = 0;
+          for (int j0 = 0; j0 < nit; j0 += kFilterUnroll) {
+            bool pred[kFilterUnroll];
+            int kks[kFilterUnroll];
 #pragma unroll
-          for (int u = 0; u < kFilterUnroll; ++u) {
-            const int kk = ph + nph * wave + stride * (j0 + u);
-            kks[u] = kk;
-            pred[u] = false;
-            if (active && kk < cc) {
-              // cutoff test on the hi parts only (a 1e-6 A band at rc in float32)
-              const Staged<T>& pj = cand[kk];
-              const T dx = pj.xh - pi.xh, dy = pj.yh - pi.yh, dz = pj.zh - pi.zh;
-              const T r2 = dx * dx + dy * dy + dz * dz;
-              pred[u] = r2 < rc2 && r2 > rmin2;
+            for (int u = 0; u < kFilterUnroll; ++u) {
+              const int kk = ph + nph * wave + stride * (j0 + u);
+              kks[u] = kk;
+              pred[u] = false;
+              if (active && kk < cc) {
+                // cutoff test on the hi parts only (a 1e-6 A band at rc in float32)
+                const Staged<T>& pj = cand[kk];
+                const T dx = pj.xh - pi.xh, dy = pj.yh - pi.yh, dz = pj.zh - pi.zh;
+                const T r2 = dx * dx + dy * dy + dz * dz;
+                pred[u] = r2 < rc2 && r2 > rmin2;
+              }
+            }
+#pragma unroll
+            for (int u = 0; u < kFilterUnroll; ++u) {
+              const unsigned long long msk = __ballot(pred[u]);
+              if (pred[u]) {
+                const int at = qn + __popcll(msk & ((1ull << lane) - 1ull));
+                q[at] = ((g0 + hl) << 16) | kks[u];
+              }
+              qn += __popcll(msk);
+            }
+            while (qn >= 64) {
+              wave_sync();
+              const int ent = q[lane];
+              const int hx = ent >> 16, kx = ent & 0xffff;
+              pair_eval<T>(home[hx], cand[kx], kappa, eps, c_exp, k2, facc + 3 * hx, pacc);
+              const int rem = qn - 64;
+              // move the tail (at most kQueueCap - 64 entries) to the front
+              int tails[kFilterUnroll];
+#pragma unroll
+              for (int u = 0; u < kFilterUnroll; ++u)
+                tails[u] = (64 * u + lane < rem) ? q[64 + 64 * u + lane] : 0;
+              wave_sync();
+#pragma unroll
+              for (int u = 0; u < kFilterUnroll; ++u)
+                if (64 * u + lane < rem) q[64 * u + lane] = tails[u];
+              wave_sync();
+              qn = rem;
             }
           }
-#pragma unroll
-          for (int u = 0; u < kFilterUnroll; ++u) {
-            const unsigned long long m = __ballot(pred[u]);
-            if (pred[u]) {
-              const int at = qn + __popcll(m & ((1ull << lane) - 1ull));
-              q[at] = ((g0 + hl) << 16) | kks[u];
-            }
-            qn += __popcll(m);
-          }
-          while (qn >= 64) {
-            wave_sync();
+          wave_sync();
+          if (lane < qn) {
             const int ent = q[lane];
             const int hx = ent >> 16, kx = ent & 0xffff;
             pair_eval<T>(home[hx], cand[kx], kappa, eps, c_exp, k2, facc + 3 * hx, pacc);
-            const int rem = qn - 64;
-            // move the tail (at most kQueueCap - 64 entries) to the front
-            int tails[kFilterUnroll];
-#pragma unroll
-            for (int u = 0; u < kFilterUnroll; ++u)
-              tails[u] = (64 * u + lane < rem) ? q[64 + 64 * u + lane] : 0;
-            wave_sync();
-#pragma unroll
-            for (int u = 0; u < kFilterUnroll; ++u)
-              if (64 * u + lane < rem) q[64 * u + lane] = tails[u];
-            wave_sync();
-            qn = rem;
           }
+          wave_sync();
         }
-        wave_sync();
-        if (lane < qn) {
-          const int ent = q[lane];
-          const int hx = ent >> 16, kx = ent & 0xffff;
-          pair_eval<T>(home[hx], cand[kx], kappa, eps, c_exp, k2, facc + 3 * hx, pacc);
-        }
-        wave_sync();
+        __syncthreads();
       }
-      __syncthreads();
     }
     // home forces of this split
     for (int t = threadIdx.x; t < hc; t += blockDim.x) {
@@ -284,7 +306,7 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
   acc[6] = 0.5 * (double)pacc.vyz;
   block_sum<kRealNV>(acc, red);
   if (threadIdx.x == 0)
-    for (int k = 0; k < kRealNV; ++k) partial_row[k] = acc[k];
+    for (int k = 0; k < kRealNV; ++k) partial[k * pstride] = acc[k];
 }
 
 }  // namespace admp
