@@ -94,13 +94,14 @@ int admp_pair_ds_backward(const double* grad_ds, const int64_t* pairs, int64_t n
  * mode 2: Gaussian damping -eps*erfc(d / (2 eta_ij)) qi qj safe_inv(d, 1e-4) bs,
  *         eta_ij = sqrt((eta_i^2 + eta_j^2)/2) (src/torch_admp/qeq.py:106-121)
  * q is (nf, n); eta (nf, n) for mode 2 else NULL; pairs (nf, P, 2); ds, bs (nf, P).
- * Writes energy (nf) and, when non-NULL, dE/dds (nf, P) and dE/dq (nf, n)
- * (dE/dq is overwritten).  bs is float64. */
+ * Writes energy (nf) and, when non-NULL, dE/dds (nf, P), dE/dq (nf, n) and,
+ * for mode 2 only, dE/deta (nf, n) (dE/dq and dE/deta are overwritten).
+ * bs is float64. */
 int admp_pair_energy(int32_t mode, int64_t nf, int64_t n, int64_t n_pairs, const double* q,
                      const double* eta, const int64_t* pairs, const double* ds,
                      const double* bs, double kappa, double dielectric, double* energy,
-                     double* grad_ds, double* grad_q, void* workspace, size_t ws_bytes,
-                     void* stream);
+                     double* grad_ds, double* grad_q, double* grad_eta, void* workspace,
+                     size_t ws_bytes, void* stream);
 int admp_pair_energy_workspace_bytes(int64_t nf, size_t* bytes);
 
 /* ------------------------------------------------------------------------ */

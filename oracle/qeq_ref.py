@@ -14,6 +14,8 @@ calc_hessian           src/torch_admp/qeq.py:460-515, 1620-1670     symmetry; fi
 kkt_solve              src/torch_admp/qeq.py:517-607, 1530-1617     KKT residual; constrained
                                                                     optimality; invariant
                                                                     q.Jq/2 = E_coul + E_damp
+hessian_graph          src/torch_admp/qeq.py:460-607                central finite differences of
+                                                                    q*(chi, positions)
 pgrad_norm             src/torch_admp/qeq.py:714-765,               projection P A^T = 0
                        src/torch_admp/utils.py:115-210
 =====================  ===========================================  ==============================
@@ -85,6 +87,18 @@ class QEqOracle:
         (g,) = torch.autograd.grad(y.sum(), q, create_graph=True)
         rows = [torch.autograd.grad(g[i], q, retain_graph=True)[0] for i in range(n)]
         return torch.stack(rows)
+
+
+def hessian_graph(oracle: "QEqOracle", pos, box, chi, hardness, eta, pairs, ds, bs):
+    """d2E/dq2 kept differentiable in pos, box, hardness, eta and ds (double
+    backward with create_graph), for gradients through the dense KKT solve as
+    the reference takes them (src/torch_admp/qeq.py:460-607)."""
+    n = pos.shape[0]
+    q = torch.zeros(n, dtype=pos.dtype, requires_grad=True)
+    y = oracle.func_energy(q, pos, box, chi, hardness, eta, pairs, ds, bs)
+    (g,) = torch.autograd.grad(y.sum(), q, create_graph=True)
+    rows = [torch.autograd.grad(g[i], q, create_graph=True)[0] for i in range(n)]
+    return torch.stack(rows)
 
 
 def kkt_solve(hessian, chi, constraint_matrix=None, constraint_vals=None):

@@ -84,3 +84,23 @@ def test_matrix_inversion_and_forces_match_oracle(dev):
     assert abs(float(e.detach()) - float(e_ref)) < 1e-10 * abs(float(e_ref))
     f = -calc_grads(e, p)
     assert float((f.cpu() - f_ref).abs().max()) < 1e-8
+
+
+def test_damping_gradients_match_oracle(dev):
+    """dE_damp/d(q, ds, eta) from the pair kernel's backward against autograd of
+    the oracle damping energy (1e-10 relative)."""
+    from torch_admp_amd.qeq import GaussianDampingForceModule
+
+    pos, b, chi, hard, eta, pairs, ds, bs = qeq_system(n=50, seed=7)
+    q = torch.tensor(np.random.default_rng(5).normal(size=pos.shape[0]))
+    qo, dso, etao = (t.clone().requires_grad_(True) for t in (q, ds, eta))
+    e_ref = qeq_ref.damping_energy(qo[None], etao[None], pairs[None], dso[None],
+                                   bs.to(F64)[None]).sum()
+    g_ref = torch.autograd.grad(e_ref, (qo, dso, etao))
+    qg, dsg, etag = (t.to(dev).requires_grad_(True) for t in (q, ds, eta))
+    mod = GaussianDampingForceModule()
+    e = mod(pos.to(dev), b.to(dev), pairs.to(dev), dsg, bs.to(dev), {"charge": qg, "eta": etag})
+    g = torch.autograd.grad(e.sum(), (qg, dsg, etag))
+    assert abs(float(e.sum()) - float(e_ref)) <= 1e-12 * abs(float(e_ref))
+    for a, r in zip(g, g_ref):
+        assert float((a.cpu() - r).abs().max()) <= 1e-10 * float(r.abs().max())

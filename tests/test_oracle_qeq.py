@@ -102,3 +102,34 @@ def test_damping_closed_form_two_charges():
                                 + 0.7 ** 2 / (2 * math.sqrt(math.pi) * 0.9)
                                 + 0.4 ** 2 / (2 * math.sqrt(math.pi) * 1.3))
     assert abs(e - ref) < 1e-13 * abs(ref)
+
+
+def test_kkt_gradients_through_differentiable_hessian_match_finite_differences():
+    """d(w.q*)/d(chi, positions) through hessian_graph + kkt_solve against central
+    differences of re-solved charges (pins the oracle for the implicit backward)."""
+    pos, b, chi, hard, eta, pairs, _, bs = qeq_system(n=16, L=8.0, seed=5)
+    orc = qeq_ref.QEqOracle(4.0, kmesh=16)
+    A = torch.ones(1, pos.shape[0], dtype=F64)
+    bval = torch.zeros(1, dtype=F64)
+    w = torch.tensor(np.random.default_rng(8).normal(size=pos.shape[0]))
+
+    def loss(p, c):
+        ds = nblist_ref.pairs_ds(p, pairs, b)
+        H = qeq_ref.hessian_graph(orc, p, b, c, hard, eta, pairs, ds, bs)
+        q, _, _ = qeq_ref.kkt_solve(H, c, A, bval)
+        return (w * q).sum()
+
+    p0 = pos.clone().requires_grad_(True)
+    c0 = chi.clone().requires_grad_(True)
+    gp, gc = torch.autograd.grad(loss(p0, c0), (p0, c0))
+    h = 1e-5
+    for i, d in ((0, 0), (5, 2), (11, 1)):
+        dp = torch.zeros_like(pos)
+        dp[i, d] = h
+        fd = (loss(pos + dp, chi) - loss(pos - dp, chi)) / (2 * h)
+        assert abs(float(fd) - float(gp[i, d])) < 1e-6 * max(1.0, abs(float(fd)))
+    for i in (1, 9):
+        dc = torch.zeros_like(chi)
+        dc[i] = h
+        fd = (loss(pos, chi + dc) - loss(pos, chi - dc)) / (2 * h)
+        assert abs(float(fd) - float(gc[i])) < 1e-7 * max(1.0, abs(float(fd)))

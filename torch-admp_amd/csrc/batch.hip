@@ -330,11 +330,10 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __
 }
 
 // ------------------------------------------------------------------ reductions
-// Work-group (s, g) reduces values 4g .. 4g + 3 of system s (one wavefront per
-// value, fixed order) into red[s][kSysValues]; the last work-group of a system
-// to finish (a ticket counter) turns them into energies and dE/dbox.
-constexpr int kRedGroups = (kSysValues + 3) / 4;
-
+// Work-group (s, v) reduces value v of system s over its rows (a fixed
+// order: thread t sums rows t, t + 256, ..., then a block tree) into
+// red[s][kSysValues]; the last work-group of a system to finish (a ticket
+// counter) turns the values into energies and dE/dbox.
 struct PartialSets {
   const double* real;  // kRealNV values x rows_real (rows = cells x splits)
   const double* recip; // 10 values x rows_recip (x-pass blocks)
@@ -347,10 +346,11 @@ __global__ void __launch_bounds__(kBlock) k_b_reduce_finalize(
     int nsys, const int64_t* __restrict__ offs, const SysDesc* __restrict__ tab, PartialSets ps,
     double* __restrict__ red, int* __restrict__ tickets, double* __restrict__ energy,
     double* __restrict__ dEdbox) {
+  __shared__ double wsum[kBlock / 64];
   const int s = blockIdx.x;
-  const int lane = threadIdx.x & 63;
-  const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (v < kSysValues) {
+  const int v = blockIdx.y;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  {
     const double* p;
     int64_t r0, r1;
     const int64_t* cell = offs + (int64_t)(nsys + 1) * kSegCell;
@@ -369,17 +369,24 @@ __global__ void __launch_bounds__(kBlock) k_b_reduce_finalize(
       r0 = go[s];
       r1 = go[s + 1];
     }
+    const int nt = blockDim.x;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int64_t r = r0 + lane;
-    for (; r + 192 < r1; r += 256) {
+    int64_t r = r0 + threadIdx.x;
+    for (; r + 3 * nt < r1; r += 4 * nt) {
       a0 += p[r];
-      a1 += p[r + 64];
-      a2 += p[r + 128];
-      a3 += p[r + 192];
+      a1 += p[r + nt];
+      a2 += p[r + 2 * nt];
+      a3 += p[r + 3 * nt];
     }
-    for (; r < r1; r += 64) a0 += p[r];
+    for (; r < r1; r += nt) a0 += p[r];
     const double acc = wave_sum((a0 + a1) + (a2 + a3));
-    if (lane == 0) red[(int64_t)s * kSysValues + v] = acc;
+    if (lane == 0) wsum[wid] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += wsum[w];
+      red[(int64_t)s * kSysValues + v] = t;
+    }
   }
   __threadfence();
   __syncthreads();
@@ -617,7 +624,7 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   sets.rows_recip = rows_x;
   sets.rows_gath = rows_g;
   sets.nsplit = kRealSplit;
-  k_b_reduce_finalize<<<dim3(nsys, kRedGroups), kBlock, 0, s>>>(
+  k_b_reduce_finalize<<<dim3(nsys, kSysValues), kBlock, 0, s>>>(
       nsys, offs, tab, sets, (double*)(ws + P.o_red), tickets, energy, dEdbox);
   mark(6);
   return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;

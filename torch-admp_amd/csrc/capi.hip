@@ -283,18 +283,20 @@ int admp_pair_energy_workspace_bytes(int64_t nf, size_t* bytes) {
 int admp_pair_energy(int32_t mode, int64_t nf, int64_t n, int64_t n_pairs, const double* q,
                      const double* eta, const int64_t* pairs, const double* ds,
                      const double* bs, double kappa, double dielectric, double* energy,
-                     double* grad_ds, double* grad_q, void* workspace, size_t ws_bytes,
-                     void* stream) {
+                     double* grad_ds, double* grad_q, double* grad_eta, void* workspace,
+                     size_t ws_bytes, void* stream) {
   if (mode < 0 || mode > 2 || nf < 1 || nf > 65535 || n < 0 || n_pairs < 0 || !energy ||
       !workspace)
     return ADMP_ERR_INVALID;
   if (n_pairs > 0 && (!q || !pairs || !ds || !bs)) return ADMP_ERR_INVALID;
   if (mode == 2 && n_pairs > 0 && !eta) return ADMP_ERR_INVALID;
+  if (grad_eta && (mode != 2 || !eta)) return ADMP_ERR_INVALID;
   size_t need;
   admp_pair_energy_workspace_bytes(nf, &need);
   if (ws_bytes < need) return ADMP_ERR_INVALID;
   return hip_status(launch_pair_energy(mode, nf, n, n_pairs, q, eta, pairs, ds, bs, kappa,
-                                       dielectric, energy, grad_ds, grad_q, (double*)workspace,
+                                       dielectric, energy, grad_ds, grad_q, grad_eta,
+                                       (double*)workspace,
                                        (hipStream_t)stream));
 }
 

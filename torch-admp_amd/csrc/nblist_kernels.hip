@@ -391,7 +391,8 @@ __global__ void __launch_bounds__(kBlock) k_pair_energy(int mode, int64_t n, int
                                                         double kappa, double eps,
                                                         double* __restrict__ partials,
                                                         double* __restrict__ gds,
-                                                        double* __restrict__ gq) {
+                                                        double* __restrict__ gq,
+                                                        double* __restrict__ geta) {
   __shared__ double red[kBlock / 64];
   const int64_t f = blockIdx.y;
   const double* qf = q + f * n;
@@ -417,10 +418,18 @@ __global__ void __launch_bounds__(kBlock) k_pair_energy(int mode, int64_t n, int
       ds_s = 0.0;
     } else {
       double eij = sqrt((etaf[i] * etaf[i] + etaf[j] * etaf[j]) * 0.5);
-      if (eij == 0.0) eij = 1e-10;  // src/torch_admp/qeq.py:111
+      const bool guard = eij == 0.0;
+      if (guard) eij = 1e-10;  // src/torch_admp/qeq.py:111
       const double x = d / (2.0 * eij);
+      const double gx = exp(-x * x);
       s = -erfc(x);
-      ds_s = 2.0 / kSqrtPi * exp(-x * x) / (2.0 * eij);
+      ds_s = 2.0 / kSqrtPi * gx / (2.0 * eij);
+      if (geta && !guard && b != 0.0) {
+        // d s / d eij = -(2 / sqrt(pi)) exp(-x^2) d / (2 eij^2); d eij / d eta_i = eta_i / (2 eij)
+        const double de = eps * b * qi * qj * inv * (-gx * d / (kSqrtPi * eij * eij));
+        unsafeAtomicAdd(&geta[f * n + i], de * etaf[i] / (2.0 * eij));
+        unsafeAtomicAdd(&geta[f * n + j], de * etaf[j] / (2.0 * eij));
+      }
     }
     const double pre = eps * b;
     const double e = pre * s * qi * qj * inv;
@@ -447,10 +456,6 @@ __global__ void __launch_bounds__(kBlock) k_reduce_frames(const double* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Fused real space: one thread per atom in cell order, the neighbour cells'
-// atoms read from a cell-sorted (x, y, z, q) copy.  Per block partials (17):
-//   0 E_real, 1..6 strain derivative D (xx yy zz xy xz yz), 7..15 sum r_i (x) g_i,
-//   16 sum q^2 ; the total charge is kept in slot 17.
 // Cell-sorted copy of the wrapped positions and charges.  Positions stay in
 // float64 so that pair displacements are formed without cancellation; the
 // pair arithmetic itself runs in the bulk type.
@@ -569,17 +574,18 @@ hipError_t launch_pair_ds_bwd(const double* gds, const int64_t* pairs, int64_t n
 hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const double* q,
                               const double* eta, const int64_t* pairs, const double* ds,
                               const double* bs, double kappa, double eps, double* energy,
-                              double* gds, double* gq, double* partials, hipStream_t s) {
+                              double* gds, double* gq, double* geta, double* partials,
+                              hipStream_t s) {
   const int bpf = kPairBlocksPerFrame;
-  if (gq) {
-    const int64_t m = nf * n;
-    if (m > 0)
-      k_fill<double><<<(unsigned)std::min<int64_t>((m + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(
-          gq, m, 0.0);
+  const int64_t m = nf * n;
+  if (m > 0) {
+    const unsigned nb = (unsigned)std::min<int64_t>((m + kBlock - 1) / kBlock, 4096);
+    if (gq) k_fill<double><<<nb, kBlock, 0, s>>>(gq, m, 0.0);
+    if (geta) k_fill<double><<<nb, kBlock, 0, s>>>(geta, m, 0.0);
   }
   dim3 grid(bpf, (unsigned)nf);
   k_pair_energy<<<grid, kBlock, 0, s>>>(mode, n, np, q, eta, pairs, ds, bs, kappa, eps, partials,
-                                        gds, gq);
+                                        gds, gq, geta);
   k_reduce_frames<<<(unsigned)nf, kBlock, 0, s>>>(partials, bpf, energy);
   return hipGetLastError();
 }

@@ -61,7 +61,8 @@ hipError_t launch_pair_ds_bwd(const double* gds, const int64_t* pairs, int64_t n
 hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const double* q,
                               const double* eta, const int64_t* pairs, const double* ds,
                               const double* bs, double kappa, double eps, double* energy,
-                              double* gds, double* gq, double* partials, hipStream_t s);
+                              double* gds, double* gq, double* geta, double* partials,
+                              hipStream_t s);
 hipError_t launch_pack(const double* q, int64_t n, char* ws, const NbLayout& L, P4<double>* ps,
                        hipStream_t s);
 }  // namespace admp

@@ -47,7 +47,7 @@ _SIGNATURES = {
     "admp_pair_energy_workspace_bytes": (ctypes.c_int, [_c_i64, _P(_c_sz)]),
     "admp_pair_energy": (ctypes.c_int, [_c_i32, _c_i64, _c_i64, _c_i64, _c_vp, _c_vp, _c_vp,
                                         _c_vp, _c_vp, _c_dbl, _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp,
-                                        _c_sz, _c_vp]),
+                                        _c_vp, _c_sz, _c_vp]),
     "admp_pme_workspace_bytes": (ctypes.c_int, [_c_vp, _c_i32, _c_i64, _P(_c_sz)]),
     "admp_pme_spread": (ctypes.c_int, [_c_vp, _c_vp, _c_i64, _c_vp, _c_vp, _c_i32, _c_vp, _c_vp,
                                        _c_sz, _c_vp]),

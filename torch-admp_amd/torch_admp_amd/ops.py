@@ -88,25 +88,29 @@ class PairEnergy(torch.autograd.Function):
         need = ctx.needs_input_grad
         gds = torch.empty(nf, npairs, dtype=torch.float64, device=dev) if need[1] else None
         gq = torch.empty(nf, n, dtype=torch.float64, device=dev) if need[0] else None
+        geta = (torch.empty(nf, n, dtype=torch.float64, device=dev)
+                if need[4] and mode == PAIR_DAMPED and eta is not None else None)
         nbytes = ctypes.c_size_t()
         _lib.check(lib.admp_pair_energy_workspace_bytes(nf, ctypes.byref(nbytes)), "workspace")
         ws = _lib.workspace(nbytes.value, dev)
         _lib.check(lib.admp_pair_energy(mode, nf, n, npairs, _lib.ptr(q64), _lib.ptr(eta64),
                                         _lib.ptr(prs), _lib.ptr(ds64), _lib.ptr(bs64), kappa, eps,
                                         _lib.ptr(energy), _lib.ptr(gds), _lib.ptr(gq),
-                                        _lib.ptr(ws), nbytes.value, _lib.stream_of(q64)),
+                                        _lib.ptr(geta), _lib.ptr(ws), nbytes.value,
+                                        _lib.stream_of(q64)),
                    "admp_pair_energy")
-        ctx.save_for_backward(gds, gq)
-        ctx.dtypes = (q.dtype, ds.dtype)
+        ctx.save_for_backward(gds, gq, geta)
+        ctx.dtypes = (q.dtype, ds.dtype, None if eta is None else eta.dtype)
         return energy.to(q.dtype)
 
     @staticmethod
     def backward(ctx, grad_e):
-        gds, gq = ctx.saved_tensors
+        gds, gq, geta = ctx.saved_tensors
         ge = grad_e.detach().to(torch.float64).reshape(-1, 1)
         out_q = None if gq is None else (gq * ge).to(ctx.dtypes[0])
         out_ds = None if gds is None else (gds * ge).to(ctx.dtypes[1])
-        return out_q, out_ds, None, None, None, None, None, None
+        out_eta = None if geta is None else (geta * ge).to(ctx.dtypes[2])
+        return out_q, out_ds, None, None, out_eta, None, None, None
 
 
 def _inverse_transpose(boxh) -> torch.Tensor:

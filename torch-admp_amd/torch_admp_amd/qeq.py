@@ -298,11 +298,48 @@ class QEqForceModule(BaseForceModule):
                                   buffer_scales)
         return energy, q_opt
 
+    def solve_implicit(self, positions, box, chi, hardness, eta, pairs, ds, buffer_scales,
+                       constraint_matrix=None, constraint_vals=None, tol: float = 1e-10):
+        """Equilibrated charges q*(positions, box, chi, hardness, eta, ds, b) as a
+        differentiable function of its inputs.
+
+        Forward: the projected CG solve to |P grad E| / N < tol.  Backward (the
+        implicit-function theorem on the KKT system [H A^T; A 0][q; mu] = [-chi; b]):
+        one adjoint solve H u + A^T nu = g, A u = 0 for the upstream gradient g,
+        then dL/dchi = -u, dL/dhardness = -2 u q*, dL/db = nu and, for the
+        geometry (positions, box, ds, eta), dL/dtheta = -d/dtheta (u^T J q*) with
+        u^T J q* = (E_c(u + q*) - E_c(u - q*)) / 2 over the Coulomb and damping
+        terms, whose gradients come from the library's backward kernels.  The
+        reference differentiates through its dense solve instead
+        (src/torch_admp/qeq.py:517-607).
+        """
+        n = positions.reshape(-1, 3).shape[0]
+        if constraint_matrix is not None:
+            constraint_matrix = constraint_matrix.detach().to(torch.float64).reshape(-1, n)
+            if constraint_vals is None:
+                constraint_vals = torch.zeros(constraint_matrix.shape[0], dtype=torch.float64,
+                                              device=positions.device)
+        else:
+            constraint_vals = None
+        return _QEqImplicit.apply(self, positions, box, chi, hardness, eta, ds, constraint_vals,
+                                  pairs, buffer_scales, constraint_matrix, float(tol))
+
+    def _coulomb_damping_energy(self, charges, positions, box, eta, pairs, ds, buffer_scales):
+        """E_c(q): the Coulomb and damping terms only (quadratic in q)."""
+        params = {"charge": charges, "eta": eta}
+        energy = torch.zeros(1, device=positions.device, dtype=positions.dtype)
+        for name, model in self.submodels.items():
+            if name != "site":
+                energy = energy + model(positions, box, pairs, ds, buffer_scales, params)
+        return energy.sum()
+
     def _pcg(self, q0, positions, box, chi, hardness, eta, pairs, ds, buffer_scales,
              constraint_matrix, constraint_vals, tol, max_iter=None, check_every=4):
         pr, keep, ws, nb = self._problem(positions, box, chi, hardness, eta, pairs, ds,
                                          buffer_scales, constraint_matrix, constraint_vals)
         lib = _lib.load()
+        if q0 is None:
+            q0 = torch.zeros(pr.n, dtype=torch.float64, device=positions.device)
         start = q0.detach().reshape(-1).to(torch.float64).contiguous()
         q = torch.empty_like(start)
         iters = ctypes.c_int32()
@@ -351,3 +388,81 @@ def calc_hessian(func_energy, positions, box, chi, hardness, eta, pairs, ds, buf
 __all__ = ["SiteForceModule", "GaussianDampingForceModule", "QEqForceModule", "pgrad_optimize",
            "matinv_optimize", "calc_hessian", "vector_projection", "vector_projection_coeff_matrix",
            "calc_grads"]
+
+
+class _QEqImplicit(torch.autograd.Function):
+    """q* of the QEq solve with the implicit-function backward (see
+    QEqForceModule.solve_implicit)."""
+
+    @staticmethod
+    def forward(ctx, module, positions, box, chi, hardness, eta, ds, constraint_vals, pairs,
+                buffer_scales, constraint_matrix, tol):
+        q, iters, resid = module._pcg(None, positions, box, chi, hardness, eta, pairs, ds,
+                                      buffer_scales, constraint_matrix, constraint_vals, tol)
+        if iters < 0:
+            raise RuntimeError(f"QEq solve did not converge (|P grad E| / N = {resid:.3e})")
+        module.converge_iter = iters
+        module.residual = resid
+        q = q.to(positions.dtype)
+        ctx.module = module
+        ctx.pairs = pairs
+        ctx.buffer_scales = buffer_scales
+        ctx.constraint_matrix = constraint_matrix
+        ctx.tol = tol
+        ctx.has_box = box is not None
+        ctx.has_cons = constraint_vals is not None
+        ctx.save_for_backward(positions, box if box is not None else torch.empty(0), chi,
+                              hardness, eta, ds, q)
+        return q
+
+    @staticmethod
+    def backward(ctx, g):
+        module = ctx.module
+        positions, box, chi, hardness, eta, ds, q = ctx.saved_tensors
+        box = box if ctx.has_box else None
+        A = ctx.constraint_matrix
+        n = q.shape[0]
+        g = g.detach().reshape(-1).to(torch.float64)
+        zeros_b = None if A is None else torch.zeros(A.shape[0], dtype=torch.float64,
+                                                     device=g.device)
+        # adjoint solve: min 1/2 u^T H u - g^T u on {A u = 0}
+        scale = float(g.abs().max()) if g.numel() else 0.0
+        u = torch.zeros(n, dtype=torch.float64, device=g.device)
+        if scale > 0.0:
+            u, iters, resid = module._pcg(None, positions, box, -g, hardness, eta, ctx.pairs, ds,
+                                          ctx.buffer_scales, A, zeros_b,
+                                          max(ctx.tol * scale, 1e-300))
+            if iters < 0:
+                raise RuntimeError("QEq adjoint solve did not converge "
+                                   f"(|P (H u - g)| / N = {resid:.3e})")
+        grads = [None] * 12
+        uq = u.to(q.dtype)
+        if ctx.needs_input_grad[3]:
+            grads[3] = (-uq).reshape(chi.shape)
+        if ctx.needs_input_grad[4]:
+            grads[4] = (-2.0 * uq * q).reshape(hardness.shape)
+        if ctx.needs_input_grad[7] and A is not None:
+            # nu = (A A^T)^-1 A (g - H u)
+            hu = module.hessian_matvec(u, positions, box, None, hardness, eta, ctx.pairs, ds,
+                                       ctx.buffer_scales).to(torch.float64)
+            r = g - hu
+            nu = torch.linalg.solve(A @ A.T, A @ r)
+            grads[7] = nu.to(q.dtype)
+        geo = [(1, positions), (2, box), (5, eta), (6, ds)]
+        want = [(i, t) for i, t in geo if t is not None and ctx.needs_input_grad[i]]
+        if want:
+            with torch.enable_grad():
+                inputs = {i: t.detach().requires_grad_(True) for i, t in want}
+                pos_ = inputs.get(1, positions.detach())
+                box_ = inputs.get(2, None if box is None else box.detach())
+                eta_ = inputs.get(5, eta.detach())
+                ds_ = inputs.get(6, ds.detach())
+                e_p = module._coulomb_damping_energy(uq + q, pos_, box_, eta_, ctx.pairs, ds_,
+                                                     ctx.buffer_scales)
+                e_m = module._coulomb_damping_energy(uq - q, pos_, box_, eta_, ctx.pairs, ds_,
+                                                     ctx.buffer_scales)
+                mixed = 0.5 * (e_p - e_m)
+                out = torch.autograd.grad(mixed, [inputs[i] for i, _ in want], allow_unused=True)
+            for (i, t), gr in zip(want, out):
+                grads[i] = None if gr is None else -gr
+        return tuple(grads)
