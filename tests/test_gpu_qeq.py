@@ -76,11 +76,16 @@ def test_matrix_inversion_and_forces_match_oracle(dev):
     nbl = TorchNeighborList(4.5)
     pr = nbl(p, b.to(dev))
     mod = QEqForceModule(4.5, kmesh=20)
-    e, q, hdiag, fermi = matinv_optimize(mod, p, *_to(dev, b, chi, hard, eta), pr, nbl.get_ds(),
-                                         nbl.get_buffer_scales(), *_to(dev, A, bv))
+    args = (p, *_to(dev, b, chi, hard, eta), pr, nbl.get_ds(), nbl.get_buffer_scales(),
+            *_to(dev, A, bv))
+    _, q4, hdiag, fermi = mod.solve_matrix_inversion(*args)
     assert torch.allclose(hdiag.cpu(), torch.diag(H), rtol=1e-9, atol=1e-9)
-    assert float((q.detach().cpu() - q_ref).abs().max()) < 1e-9
+    assert float((q4.detach().cpu() - q_ref).abs().max()) < 1e-9
     assert abs(float(fermi) - float(fermi_ref)) < 1e-8
+    # the module-level form returns (energy, q_opt) like the reference's
+    e, q = matinv_optimize(mod, *args)
+    assert q.requires_grad and q.is_leaf
+    assert float((q.detach().cpu() - q_ref).abs().max()) < 1e-9
     assert abs(float(e.detach()) - float(e_ref)) < 1e-10 * abs(float(e_ref))
     f = -calc_grads(e, p)
     assert float((f.cpu() - f_ref).abs().max()) < 1e-8

@@ -371,9 +371,16 @@ def pgrad_optimize(module: QEqForceModule, q0, positions, box, chi, hardness, et
 
 def matinv_optimize(module: QEqForceModule, positions, box, chi, hardness, eta, pairs, ds,
                     buffer_scales, constraint_matrix, constraint_vals, **kwargs):
-    """Module-level dense solve (src/torch_admp/qeq.py:1530-1617)."""
-    return module.solve_matrix_inversion(positions, box, chi, hardness, eta, pairs, ds,
-                                         buffer_scales, constraint_matrix, constraint_vals)
+    """Module-level dense solve (src/torch_admp/qeq.py:1530-1617): returns
+    (energy, q_opt), q_opt a detached leaf that requires grad and the energy
+    evaluated at it."""
+    _, q_full, _, _ = module.solve_matrix_inversion(positions, box, chi, hardness, eta, pairs,
+                                                    ds, buffer_scales, constraint_matrix,
+                                                    constraint_vals)
+    q_opt = q_full.detach().clone().requires_grad_(True)
+    energy = module.func_energy(q_opt, positions, box, chi, hardness, eta, pairs, ds,
+                                buffer_scales)
+    return energy, q_opt
 
 
 def calc_hessian(func_energy, positions, box, chi, hardness, eta, pairs, ds, buffer_scales):
