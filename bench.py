@@ -42,7 +42,136 @@ def parse():
                     help="skip the float64 and drop-in module timings")
     ap.add_argument("--serving-requests", type=int, default=4096,
                     help="requests of the mixed-size serving stream (0 skips it)")
+    ap.add_argument("--no-qeq", action="store_true", help="skip the QEq and electrode timings")
+    ap.add_argument("--batch-copies", type=int, default=64,
+                    help="copies of the water box in the batched roofline run (0 skips it)")
     return ap.parse_args()
+
+
+def compulsory_bytes(n, kmesh, scalar=4):
+    """Compulsory HBM bytes of one E+F evaluation: spread (xyz+q in, mesh out),
+    r2c + c2r (mesh and half spectrum in and out), gather (mesh and xyz+q in,
+    F out) and cell-list real space (xyz+q in, F out)."""
+    k0, k1, k2 = kmesh
+    k3 = k0 * k1 * k2
+    half = k0 * k1 * (k2 // 2 + 1)
+    s = scalar
+    return (16 * n + s * k3) + 2 * (s * k3 + 2 * s * half) + (s * k3 + 28 * n) + 28 * n
+
+
+def batched_roofline(copies, pos_np, q_np, box_np, rc, kappa, kmesh, torch, dev):
+    """Aggregate throughput of `copies` independent water boxes in one batch
+    (working set well above the 256 MB Infinity Cache) and the compulsory-byte
+    bandwidth it implies."""
+    import numpy as np
+
+    from torch_admp_amd import ops
+
+    rng = np.random.default_rng(11)
+    n = pos_np.shape[0]
+    plan = ops.PMEBatchPlan([n] * copies, [box_np] * copies, rc, kappa, [kmesh] * copies,
+                            "float32")
+    pos = torch.tensor(np.concatenate([pos_np + 1e-3 * rng.standard_normal(pos_np.shape)
+                                       for _ in range(copies)]), device=dev)
+    q = torch.tensor(np.tile(q_np, copies), device=dev)
+    ws = torch.empty(plan.workspace_bytes, dtype=torch.uint8, device=dev)
+    sec, _ = timed(lambda: plan.run(pos, q, workspace=ws), 5, 2, torch)
+    evals = copies / sec
+    b = compulsory_bytes(n, kmesh)
+    gbs = b * evals / 1e9
+    return {"copies": copies, "workspace_GB": round(plan.workspace_bytes / 1e9, 3),
+            "ms_per_batch": round(sec * 1e3, 3), "evals_per_s": round(evals, 1),
+            "compulsory_bytes_per_eval": b, "achieved_GBps": round(gbs, 1),
+            "peak_GBps": HBM_PEAK_GBS, "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def qeq_bench(torch, dev):
+    """QEq of 3000 atoms of 3 species in a 30 A cube (rc 6, ethresh 1e-5, explicit
+    smooth mesh 120^3, total charge 0): projected CG to |P grad E| <= 1e-8, then the
+    implicit-function backward of L = w . q*."""
+    import numpy as np
+
+    from torch_admp_amd.nblist import TorchNeighborList
+    from torch_admp_amd.qeq import QEqForceModule
+    from torch_admp_amd.workloads import qeq_species_box
+
+    pos_np, box_np, chi_np, hard_np, eta_np = qeq_species_box(3000, 30.0, seed=3)
+    n, rc, km = pos_np.shape[0], 6.0, [120, 120, 120]
+    t = lambda a: torch.tensor(a, device=dev)  # noqa: E731
+    pos, box = t(pos_np).requires_grad_(True), t(box_np)
+    chi, hard, eta = t(chi_np).requires_grad_(True), t(hard_np), t(eta_np)
+    A = torch.ones(1, n, dtype=torch.float64, device=dev)
+    b = torch.zeros(1, dtype=torch.float64, device=dev)
+    w = t(np.random.default_rng(1).normal(size=n))
+    out = {"workload": "QEq, 3000 atoms of 3 species in a 30 A cube, rc 6 A, mesh 120^3, "
+                       "sum q = 0, projected CG to |P grad E| <= 1e-8, hardness = J/2"}
+    for prec in ("float64", "float32"):
+        mod = QEqForceModule(rc, 1e-5, kmesh=km, precision=prec)
+        nbl = TorchNeighborList(rc)
+        pairs = nbl(pos, box)
+        ds, bs = nbl.get_ds(), nbl.get_buffer_scales()
+        tol = 1e-8 / n if prec == "float64" else 1e-6 / n
+        mod.solve_implicit(pos, box, chi, hard, eta, pairs, ds, bs, A, b, tol=tol)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        q = mod.solve_implicit(pos, box, chi, hard, eta, pairs, ds, bs, A, b, tol=tol)
+        torch.cuda.synchronize()
+        t_solve = time.perf_counter() - t0
+        iters = mod.converge_iter
+        t0 = time.perf_counter()
+        gp, gc = torch.autograd.grad((w * q).sum(), (pos, chi))
+        torch.cuda.synchronize()
+        t_back = time.perf_counter() - t0
+        out[prec] = {"solve_ms": round(t_solve * 1e3, 3), "cg_iterations": iters,
+                     "residual_per_atom": mod.residual,
+                     "implicit_backward_ms": round(t_back * 1e3, 3),
+                     "q_abs_max": float(q.detach().abs().max())}
+    return out
+
+
+def electrode_bench(torch, dev):
+    """Constant-potential step of two 800-atom fcc(111) gold-like electrodes and
+    2000 ions in a 40 x 40 x 120 A cell (rc 6, ethresh 1e-5, explicit smooth mesh
+    150 x 150 x 450, slab correction, 1 V between the electrodes with symm,
+    Gaussian width 0.5 A): neighbour list, electrolyte potential, electrode
+    charge solve and energy + forces at the solved charges."""
+    from torch_admp_amd.electrode import (LAMMPSElectrodeConstraint, PolarizableElectrode,
+                                          infer, setup_from_lammps)
+    from torch_admp_amd.nblist import TorchNeighborList
+    from torch_admp_amd.workloads import electrode_cell
+
+    pos_np, q_np, box_np = electrode_cell(seed=0)
+    n = pos_np.shape[0]
+    eta_lmp = 2.0 ** 0.5 / (2.0 * 0.5)  # width 0.5 A in the LAMMPS convention
+    groups = [LAMMPSElectrodeConstraint(range(800), "conp", 1.0, eta_lmp),
+              LAMMPSElectrodeConstraint(range(800, 1600), "conp", 0.0, eta_lmp)]
+    inputs = setup_from_lammps(n, groups, True, device=dev)
+    pos = torch.tensor(pos_np, device=dev).requires_grad_(True)
+    box = torch.tensor(box_np, device=dev)
+    q = torch.tensor(q_np, device=dev)
+    out = {"workload": "CONP step: 2 x 800 fcc(111) electrode atoms + 2000 ions, 40x40x120 A, "
+                       "rc 6 A, mesh 150x150x450, slab correction, 1 V, symm"}
+    for prec in ("float64", "float32"):
+        calc = PolarizableElectrode(rcut=6.0, ethresh=1e-5, kmesh=[150, 150, 450],
+                                    slab_corr=True, eps=1e-10 if prec == "float64" else 1e-6,
+                                    precision=prec)
+
+        def step():
+            nbl = TorchNeighborList(6.0)
+            pairs = nbl(pos, box)
+            return infer(calc, pos, box, q, pairs, nbl.get_ds(), nbl.get_buffer_scales(),
+                         *inputs, method="lbfgs")
+
+        sec, (e, f, q_opt) = timed(step, 3, 1, torch)
+        qe = q_opt[:1600].detach()
+        out[prec] = {"ms_per_step": round(sec * 1e3, 3), "cg_iterations": calc.converge_iter,
+                     "energy_eV": float(e.detach()),
+                     "electrode_charge_left": float(qe[:800].sum()),
+                     "electrode_charge_right": float(qe[800:].sum())}
+        out[prec]["_q"] = qe
+    dq = float((out["float32"].pop("_q") - out["float64"].pop("_q")).abs().max())
+    out["fp32_vs_fp64_electrode_charge_maxdiff"] = dq
+    return out
 
 
 def serving_bench(n_requests, torch, dev):
@@ -208,6 +337,17 @@ def main():
                                   args.warmup, torch)
     value = 1.0 / sec
 
+    # ---- the same evaluation replayed from a captured HIP graph ---------------
+    from torch_admp_amd import ops
+
+    plan1 = ops.PMEBatchPlan([n], [box_np], rc, mod32.kappa, [[kmesh] * 3], "float32")
+    graph = ops.PMEGraph(plan1, dev)
+    graph.positions.copy_(pos)
+    graph.charges.copy_(q)
+    sec_graph, _ = timed(graph.replay, args.steps, args.warmup, torch)
+    graph_check = float((graph.energy[0, 4] - e32).abs() / e32.abs())
+    del graph
+
     # ---- stage timing with HIP events on the launch stream ---------------------
     _lib.check(lib.admp_profile_begin(args.steps), "admp_profile_begin")
     for _ in range(args.steps):
@@ -258,6 +398,12 @@ def main():
 
     if args.serving_requests > 0:
         extras["serving"] = serving_bench(args.serving_requests, torch, dev)
+    if args.batch_copies > 0:
+        extras["roofline_batched"] = batched_roofline(args.batch_copies, pos_np, q_np, box_np, rc,
+                                                      mod32.kappa, [kmesh] * 3, torch, dev)
+    if not args.no_qeq:
+        extras["qeq_3000"] = qeq_bench(torch, dev)
+        extras["electrode_3600"] = electrode_bench(torch, dev)
 
     err32_vs64 = {"energy_rel": float(abs(e32 - e64) / abs(e64)),
                   "force_rmse_eV_per_A": float(((f32 - f64) ** 2).mean().sqrt())}
@@ -275,17 +421,31 @@ def main():
         b_c = torch.tensor(box_np, dtype=torch.float64)
         q_c = torch.tensor(q_np, dtype=torch.float64)
         orc = pme_ref.CoulombOracle(rc, ethresh, kmesh=kmesh)
+
+        def oracle_eval():
+            pairs = nblist_ref.nblist_pbc_cells(p_c, b_c, rc)
+            bs = nblist_ref.buffer_scales(pairs)
+            return pme_ref.energy_forces(
+                p_c, b_c, q_c, pairs, lambda p, b: nblist_ref.pairs_ds(p, pairs, b), bs, orc)
+
+        e_ref, f_ref, gb_ref = oracle_eval()  # warm-up
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            oracle_eval()
+            times.append(time.perf_counter() - t0)
+        t_cpu = float(np.median(times))
+        torch.set_num_threads(1)
         t0 = time.perf_counter()
-        pairs = nblist_ref.nblist_pbc_cells(p_c, b_c, rc)
-        bs = nblist_ref.buffer_scales(pairs)
-        e_ref, f_ref, gb_ref = pme_ref.energy_forces(
-            p_c, b_c, q_c, pairs, lambda p, b: nblist_ref.pairs_ds(p, pairs, b), bs, orc)
-        t_cpu = time.perf_counter() - t0
+        oracle_eval()
+        t_cpu1 = time.perf_counter() - t0
+        torch.set_num_threads(cores)
         cpu = {"value": round(1.0 / t_cpu, 5), "unit": "evals/s", "cores": cores,
                "kind": "port",
-               "sample": "1 full E+F+dE/dbox evaluation of the same 24k-atom box "
-                         f"({t_cpu:.1f} s): reference cell-table neighbour list, "
-                         "CoulombForceModule restatement, autograd forces, float64"}
+               "sample": "median of 5 full E+F+dE/dbox evaluations of the same 24k-atom box "
+                         f"after 1 warm-up ({t_cpu:.2f} s each): reference cell-table neighbour "
+                         "list, CoulombForceModule restatement, autograd forces, float64",
+               "one_thread_value": round(1.0 / t_cpu1, 5)}
         e_ref = float(e_ref)
         oracle_err = {
             "fp32_energy_rel": float(abs(float(e32) - e_ref) / abs(e_ref)),
@@ -321,6 +481,9 @@ def main():
         "error_fp32_vs_fp64_gpu": err32_vs64,
         "error_vs_fp64_oracle": oracle_err,
         "energy_eV": float(e32),
+        "graph_replay_evals_per_s": round(1.0 / sec_graph, 3),
+        "graph_replay_ms_per_step": round(sec_graph * 1e3, 4),
+        "graph_vs_eager_energy_rel": graph_check,
         **extras,
     }
     print(json.dumps(line))

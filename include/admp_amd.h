@@ -182,6 +182,24 @@ int admp_pme_batch(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
                    const double* q, double* energy, double* forces, double* dEdbox,
                    void* workspace, size_t ws_bytes, void* stream);
 
+/* Graph capture of a batch (no counterpart in the reference; for callers that
+ * replay one batch composition many times under a HIP graph).  admp_pme_batch
+ * copies the system table from host memory on every call, which a captured
+ * graph cannot repeat.  Instead: admp_pme_batch_upload writes the table into
+ * the workspace once (synchronously), and admp_pme_batch_run (same arguments
+ * as admp_pme_batch) launches only kernels, reading that table; it may be
+ * captured and replayed with new positions and charges in the same buffers.
+ * A batch of one needs no upload: its first kernel writes the table. */
+int admp_pme_batch_upload(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                          const double* cutoffs, const double* kappas, const int32_t* kmesh,
+                          int32_t precision, int32_t convention, double dielectric,
+                          void* workspace, size_t ws_bytes, void* stream);
+int admp_pme_batch_run(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                       const double* cutoffs, const double* kappas, const int32_t* kmesh,
+                       int32_t precision, int32_t convention, double dielectric,
+                       const double* pos, const double* q, double* energy, double* forces,
+                       double* dEdbox, void* workspace, size_t ws_bytes, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Charge equilibration.  The QEq energy (src/torch_admp/qeq.py:33-209,408-458)
  *   E(q) = chi.q + sum hardness q^2 + E_coulomb(q) + E_damping(q)

@@ -516,9 +516,36 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
 
 static void free_host(void* p) { delete[] static_cast<char*>(p); }
 
+// Copy the system table and segment offsets into the workspace.  With
+// `sync` the copy completes before returning (the host buffer is freed at
+// once); otherwise a host function frees it when the stream reaches it.
+int upload_table(const BatchPlan& P, char* ws, hipStream_t s, bool sync) {
+  const int nsys = P.nsys;
+  const size_t tab_bytes = sizeof(SysDesc) * nsys;
+  const size_t off_bytes = sizeof(int64_t) * (nsys + 1) * kSegCount;
+  char* host = new char[tab_bytes + off_bytes];
+  std::memcpy(host, P.sys.data(), tab_bytes);
+  for (int k = 0; k < kSegCount; ++k)
+    std::memcpy(host + tab_bytes + sizeof(int64_t) * (nsys + 1) * k, P.off[k].data(),
+                sizeof(int64_t) * (nsys + 1));
+  if (hipMemcpyAsync(ws + P.o_tab, host, tab_bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(ws + P.o_off, host + tab_bytes, off_bytes, hipMemcpyHostToDevice, s) !=
+          hipSuccess) {
+    delete[] host;
+    return ADMP_ERR_HIP;
+  }
+  if (sync) {
+    const hipError_t e = hipStreamSynchronize(s);
+    delete[] host;
+    return e == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
+  }
+  return hipLaunchHostFunc(s, free_host, host) == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
+}
+
 template <typename T>
 static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, double* energy,
-                       double* forces, double* dEdbox, char* ws, hipStream_t s, hipEvent_t* marks) {
+                       double* forces, double* dEdbox, char* ws, hipStream_t s, hipEvent_t* marks,
+                       bool upload) {
   using CT = cplx<T>;
   const int nsys = P.nsys;
   SysDesc* tab = (SysDesc*)(ws + P.o_tab);
@@ -529,22 +556,9 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
     // the prologue writes the table from its arguments
     one.d = P.sys[0];
     for (int k = 0; k < kSegCount; ++k) one.tot[k] = P.off[k][1];
-  } else {
-    // upload the table and the offsets; the host copy lives until the stream passes it
-    const size_t tab_bytes = sizeof(SysDesc) * nsys;
-    const size_t off_bytes = sizeof(int64_t) * (nsys + 1) * kSegCount;
-    char* host = new char[tab_bytes + off_bytes];
-    std::memcpy(host, P.sys.data(), tab_bytes);
-    for (int k = 0; k < kSegCount; ++k)
-      std::memcpy(host + tab_bytes + sizeof(int64_t) * (nsys + 1) * k, P.off[k].data(),
-                  sizeof(int64_t) * (nsys + 1));
-    if (hipMemcpyAsync(tab, host, tab_bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(offs, host + tab_bytes, off_bytes, hipMemcpyHostToDevice, s) !=
-            hipSuccess) {
-      delete[] host;
-      return ADMP_ERR_HIP;
-    }
-    if (hipLaunchHostFunc(s, free_host, host) != hipSuccess) return ADMP_ERR_HIP;
+  } else if (upload) {
+    const int st = upload_table(P, ws, s, false);
+    if (st != ADMP_OK) return st;
   }
   auto seg = [&](int k) { return (const int64_t*)offs + (int64_t)(nsys + 1) * k; };
   auto mark = [&](int k) {
@@ -631,11 +645,12 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
 }
 
 int run_batch(const BatchPlan& plan, const double* pos, const double* q, double* energy,
-              double* forces, double* dEdbox, char* ws, hipStream_t s, hipEvent_t* marks) {
+              double* forces, double* dEdbox, char* ws, hipStream_t s, hipEvent_t* marks,
+              bool upload) {
   if (plan.status != ADMP_OK) return plan.status;
   return plan.precision == ADMP_F32
-             ? run_batch_t<float>(plan, pos, q, energy, forces, dEdbox, ws, s, marks)
-             : run_batch_t<double>(plan, pos, q, energy, forces, dEdbox, ws, s, marks);
+             ? run_batch_t<float>(plan, pos, q, energy, forces, dEdbox, ws, s, marks, upload)
+             : run_batch_t<double>(plan, pos, q, energy, forces, dEdbox, ws, s, marks, upload);
 }
 
 }  // namespace admp

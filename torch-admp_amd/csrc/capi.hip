@@ -463,14 +463,15 @@ int admp_pme_batch_workspace_bytes(int32_t n_sys, const int64_t* n_atoms, const 
   return ADMP_OK;
 }
 
-int admp_pme_batch(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
-                   const double* cutoffs, const double* kappas, const int32_t* kmesh,
-                   int32_t precision, int32_t convention, double dielectric, const double* pos,
-                   const double* q, double* energy, double* forces, double* dEdbox,
-                   void* workspace, size_t ws_bytes, void* stream) {
+static int batch_call(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                      const double* cutoffs, const double* kappas, const int32_t* kmesh,
+                      int32_t precision, int32_t convention, double dielectric,
+                      const double* pos, const double* q, double* energy, double* forces,
+                      double* dEdbox, void* workspace, size_t ws_bytes, void* stream, int mode) {
+  // mode 0: upload and run; 1: upload only (synchronous); 2: run on the uploaded table
   const int st = batch_args_ok(n_sys, n_atoms, boxes, cutoffs, kmesh, precision);
   if (st != ADMP_OK) return st;
-  if (!kappas || !energy || !workspace) return ADMP_ERR_INVALID;
+  if (!kappas || !workspace || (mode != 1 && !energy)) return ADMP_ERR_INVALID;
   if (convention != ADMP_KCONV_REFERENCE && convention != ADMP_KCONV_COLUMNS)
     return ADMP_ERR_INVALID;
   int64_t ntot = 0;
@@ -478,15 +479,43 @@ int admp_pme_batch(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
     if (!(kappas[s] > 0.0) || !std::isfinite(kappas[s])) return ADMP_ERR_INVALID;
     ntot += n_atoms[s];
   }
-  if (ntot > 0 && (!pos || !q || !forces)) return ADMP_ERR_INVALID;
+  if (mode != 1 && ntot > 0 && (!pos || !q || !forces)) return ADMP_ERR_INVALID;
   const BatchPlan P = plan_batch(n_sys, n_atoms, boxes, cutoffs, kappas, kmesh, precision,
                                  convention, dielectric);
   const int ps = batch_plan_ok(P);
   if (ps != ADMP_OK) return ps;
   if (ws_bytes < P.bytes) return ADMP_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
+  if (mode == 1) return n_sys > 1 ? upload_table(P, (char*)workspace, s, true) : ADMP_OK;
   StageMarks pm = stage_marks(s);
-  return run_batch(P, pos, q, energy, forces, dEdbox, (char*)workspace, s, pm.ev);
+  return run_batch(P, pos, q, energy, forces, dEdbox, (char*)workspace, s, pm.ev, mode == 0);
+}
+
+int admp_pme_batch(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                   const double* cutoffs, const double* kappas, const int32_t* kmesh,
+                   int32_t precision, int32_t convention, double dielectric, const double* pos,
+                   const double* q, double* energy, double* forces, double* dEdbox,
+                   void* workspace, size_t ws_bytes, void* stream) {
+  return batch_call(n_sys, n_atoms, boxes, cutoffs, kappas, kmesh, precision, convention,
+                    dielectric, pos, q, energy, forces, dEdbox, workspace, ws_bytes, stream, 0);
+}
+
+int admp_pme_batch_upload(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                          const double* cutoffs, const double* kappas, const int32_t* kmesh,
+                          int32_t precision, int32_t convention, double dielectric,
+                          void* workspace, size_t ws_bytes, void* stream) {
+  return batch_call(n_sys, n_atoms, boxes, cutoffs, kappas, kmesh, precision, convention,
+                    dielectric, nullptr, nullptr, nullptr, nullptr, nullptr, workspace, ws_bytes,
+                    stream, 1);
+}
+
+int admp_pme_batch_run(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                       const double* cutoffs, const double* kappas, const int32_t* kmesh,
+                       int32_t precision, int32_t convention, double dielectric,
+                       const double* pos, const double* q, double* energy, double* forces,
+                       double* dEdbox, void* workspace, size_t ws_bytes, void* stream) {
+  return batch_call(n_sys, n_atoms, boxes, cutoffs, kappas, kmesh, precision, convention,
+                    dielectric, pos, q, energy, forces, dEdbox, workspace, ws_bytes, stream, 2);
 }
 
 }  // extern "C"

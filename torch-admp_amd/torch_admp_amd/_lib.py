@@ -64,6 +64,11 @@ _SIGNATURES = {
     "admp_pme_batch": (ctypes.c_int, [_c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i32, _c_i32,
                                       _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_sz,
                                       _c_vp]),
+    "admp_pme_batch_upload": (ctypes.c_int, [_c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i32,
+                                             _c_i32, _c_dbl, _c_vp, _c_sz, _c_vp]),
+    "admp_pme_batch_run": (ctypes.c_int, [_c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i32,
+                                          _c_i32, _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
+                                          _c_vp, _c_sz, _c_vp]),
     "admp_qeq_workspace_bytes": (ctypes.c_int, [_c_vp, _P(_c_sz)]),
     "admp_qeq_matvec": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_sz, _c_vp]),
     "admp_qeq_solve": (ctypes.c_int, [_c_vp, _c_vp, _c_dbl, _c_i32, _c_i32, _c_vp, _P(_c_i32),

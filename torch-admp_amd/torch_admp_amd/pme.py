@@ -183,8 +183,10 @@ class CoulombForceModule(BaseForceModule):
                       ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
         """Energy, forces and dE/dbox of one periodic frame through the fused
         cell-list + PME launch sequence.  Same result as forward() on the
-        TorchNeighborList(rcut) pair list followed by autograd.  Real and
-        reciprocal space only (kspace and rspace on, no slab term)."""
+        TorchNeighborList(rcut) pair list followed by autograd when every box
+        height is at least 2 rcut; below that every periodic image inside the
+        cutoff is summed (the lattice sum).  Real and reciprocal space only
+        (kspace and rspace on, no slab term)."""
         if not (self.kspace_flag and self.rspace_flag) or self.slab_corr_flag:
             raise ValueError("energy_forces requires kspace=True, rspace=True, slab_corr=False")
         lc = self.const_lib.length_coeff
