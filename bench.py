@@ -45,6 +45,10 @@ def parse():
     ap.add_argument("--no-qeq", action="store_true", help="skip the QEq and electrode timings")
     ap.add_argument("--batch-copies", type=int, default=64,
                     help="copies of the water box in the batched roofline run (0 skips it)")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay timing")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one device copy of this many MiB (a known-size kernel for "
+                         "calibrating PMC byte counters) and exit before timing")
     return ap.parse_args()
 
 
@@ -328,6 +332,14 @@ def main():
     box = torch.tensor(box_np, dtype=torch.float64, device=dev)
     box_h = torch.tensor(box_np, dtype=torch.float64)  # launch parameter, kept on the host
 
+    if args.calib_copy_mib > 0:
+        src = torch.empty(args.calib_copy_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     mod32 = CoulombForceModule(rc, ethresh, kmesh=kmesh, precision="float32")
     mod64 = CoulombForceModule(rc, ethresh, kmesh=kmesh, precision="float64")
     lib = _lib.load()
@@ -340,13 +352,15 @@ def main():
     # ---- the same evaluation replayed from a captured HIP graph ---------------
     from torch_admp_amd import ops
 
-    plan1 = ops.PMEBatchPlan([n], [box_np], rc, mod32.kappa, [[kmesh] * 3], "float32")
-    graph = ops.PMEGraph(plan1, dev)
-    graph.positions.copy_(pos)
-    graph.charges.copy_(q)
-    sec_graph, _ = timed(graph.replay, args.steps, args.warmup, torch)
-    graph_check = float((graph.energy[0, 4] - e32).abs() / e32.abs())
-    del graph
+    sec_graph, graph_check = float("nan"), float("nan")
+    if not args.no_graph:
+        plan1 = ops.PMEBatchPlan([n], [box_np], rc, mod32.kappa, [[kmesh] * 3], "float32")
+        graph = ops.PMEGraph(plan1, dev)
+        graph.positions.copy_(pos)
+        graph.charges.copy_(q)
+        sec_graph, _ = timed(graph.replay, args.steps, args.warmup, torch)
+        graph_check = float((graph.energy[0, 4] - e32).abs() / e32.abs())
+        del graph
 
     # ---- stage timing with HIP events on the launch stream ---------------------
     _lib.check(lib.admp_profile_begin(args.steps), "admp_profile_begin")
@@ -370,9 +384,18 @@ def main():
     dominant = max(("real_space", "spread", "fft_convolve", "gather"), key=lambda k: stage_ms[k])
     t_dom = stage_ms[dominant] * 1e-3
     achieved = alg_bytes[dominant] / t_dom / 1e9
+    # HBM bytes per launch measured by rocprofv3 PMC (FETCH_SIZE, WRITE_SIZE, each
+    # calibrated on a copy kernel of known size), committed beside the kernel stats
+    traffic = None
+    tfile = ROOT / "profiles" / "pme_water24k_pmc_traffic.txt"
+    if tfile.exists():
+        for ln in tfile.read_text().splitlines():
+            tok = ln.split()
+            if len(tok) == 3 and tok[0] == dominant:
+                traffic = int(tok[2])
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": None, "alg_bytes_per_launch": alg_bytes[dominant],
+                "traffic": traffic, "alg_bytes_per_launch": alg_bytes[dominant],
                 "avg_launch_us": round(stage_ms[dominant] * 1e3, 2)}
     stage_gbs = {k: round(alg_bytes[k] / (stage_ms[k] * 1e-3) / 1e9, 1)
                  for k in alg_bytes if stage_ms.get(k, 0) > 0}
