@@ -158,3 +158,22 @@ def test_batch_water_copies(dev):
     e1, f1, _ = ops.pme_fused(pos_t, torch.tensor(box), q_t, rc, kap, km, precision="float32")
     assert abs(float(energy[0, 4] - e1[4])) <= 1e-7 * abs(float(e1[4]))
     assert float(((forces[0] - f1) ** 2).mean().sqrt()) <= 1e-5
+
+
+@pytest.mark.parametrize("precision,etol", [("float64", 1e-10), ("float32", 1e-6)])
+def test_batch_box_far_below_cutoff(dev, precision, etol):
+    """A 3.6 A box with rc 8 A: 7^3 periodic images per cell, so the half
+    stencil (172 entries) is walked in two chunks; against the all-image
+    oracle pair list."""
+    rc = 8.0
+    pos, q, box = _system(20, np.eye(3) * 3.6, 31)
+    km, kap = [8, 8, 8], 0.45
+    e, f, gb = ops.pme_batch([torch.tensor(pos, device=dev)], [box],
+                             [torch.tensor(q, device=dev)], rc, kap, [km], precision=precision)
+    p, b, c = torch.tensor(pos), torch.tensor(box), torch.tensor(q)
+    pairs, _, bs = nblist_ref.multi_image_pairs(p, b, rc)
+    orc = pme_ref.CoulombOracle(rc, 1e-5, kmesh=km, kappa=kap)
+    e_ref, f_ref, gb_ref = pme_ref.energy_forces(
+        p, b, c, pairs, lambda pp, bb: nblist_ref.multi_image_pairs(pp, bb, rc)[1], bs, orc)
+    assert abs(float(e[0, 4]) - float(e_ref)) <= etol * abs(float(e_ref))
+    assert float(((f[0].cpu() - f_ref) ** 2).mean().sqrt()) < (1e-8 if precision == "float64" else 1e-4)

@@ -35,8 +35,8 @@ __device__ __forceinline__ int find_seg(const int64_t* __restrict__ off, int nse
 }
 
 // ------------------------------------------------------------------ prologue
-// Zero the bin counters and scatter cursors, the meshes and the per-system
-// reduction tickets; for a batch of one also write the system table and the
+// Zero the bin counters and scatter cursors, the meshes, the real-space
+// gradient accumulators and the per-system reduction tickets; for a batch of one also write the system table and the
 // segment offsets from the kernel arguments.
 struct OneSystem {
   SysDesc d;
@@ -49,6 +49,7 @@ __global__ void __launch_bounds__(kBlock) k_b_prologue(int inline_table, OneSyst
                                                        int64_t* __restrict__ offs,
                                                        int* __restrict__ count, int64_t ncount,
                                                        T* __restrict__ mesh, int64_t nmesh,
+                                                       double* __restrict__ grad, int64_t ngrad,
                                                        int* __restrict__ tickets, int ntickets) {
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t step = (int64_t)gridDim.x * blockDim.x;
@@ -59,6 +60,7 @@ __global__ void __launch_bounds__(kBlock) k_b_prologue(int inline_table, OneSyst
   }
   for (int64_t i = t0; i < ncount; i += step) count[i] = 0;
   for (int64_t i = t0; i < nmesh; i += step) mesh[i] = T(0);
+  for (int64_t i = t0; i < ngrad; i += step) grad[i] = 0.0;
   for (int64_t i = t0; i < ntickets; i += step) tickets[i] = 0;
 }
 
@@ -138,19 +140,19 @@ __global__ void __launch_bounds__(kBlock) k_b_sort_pack(const int* __restrict__ 
 // value-major, rows cell-major (row = cell * splits + split), so that a
 // system's rows are contiguous and the reduction reads them coalesced.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_b_real(int64_t ntot, int nsys,
+__global__ void __launch_bounds__(kBlock) k_b_real(int nsys,
                                                    const int64_t* __restrict__ cell_off,
                                                    const SysDesc* __restrict__ tab,
                                                    const P4<double>* __restrict__ ps,
                                                    const int* __restrict__ sorted,
                                                    const int* __restrict__ start,
                                                    const double* __restrict__ pos,
-                                                   double* __restrict__ gsplit,
+                                                   double* __restrict__ grad,
                                                    double* __restrict__ partials) {
   const int64_t c = blockIdx.x;
   const SysDesc* D = tab + find_seg(cell_off, nsys, c);
-  real_cell_body<T>(D->cg, D->cell0, (int)(c - D->cell0), ntot, (T)D->kappa, (T)D->eps, ps,
-                    sorted, start, pos, gsplit, partials + c * gridDim.y + blockIdx.y,
+  real_cell_body<T>(D->cg, D->cell0, (int)(c - D->cell0), (T)D->kappa, (T)D->eps, ps, sorted,
+                    start, pos, grad, partials + c * gridDim.y + blockIdx.y,
                     (int64_t)gridDim.x * gridDim.y);
 }
 
@@ -271,16 +273,15 @@ __global__ void __launch_bounds__(kBlock) k_b_cols_x(int nsys, const int64_t* __
 
 // ------------------------------------------------------------------ gather
 // kGatherAtomsPerBlock atoms of one system per work-group, 8 lanes per atom.
-// Forces are final here: -(sum of the real-space split rows + reciprocal
-// gradient).  The block's dE/dA partials (9 values) are stored value-major.
+// Forces are final here: -(real-space gradient rows + reciprocal gradient).  The block's dE/dA partials (9 values) are stored value-major.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __restrict__ off,
                                                      const SysDesc* __restrict__ tab,
                                                      const double* __restrict__ pos,
                                                      const double* __restrict__ qv,
                                                      const T* __restrict__ meshes,
-                                                     const double* __restrict__ gsplit, int nsplit,
-                                                     int64_t ntot, double* __restrict__ forces,
+                                                     const double* __restrict__ grad,
+                                                     double* __restrict__ forces,
                                                      double* __restrict__ partials, int64_t nrows) {
   __shared__ double red[9 * (kBlock / 64)];
   const int sid = find_seg(off, nsys, blockIdx.x);
@@ -314,7 +315,7 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __
     double gj = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) gj += g.k[d] * g.a[a * 3 + d] * dR[d];
-    for (int k = 0; k < nsplit; ++k) gj += gsplit[((int64_t)k * ntot + atom) * 3 + a];
+    gj += grad[3 * atom + a];
     forces[3 * atom + a] = -gj;
     if (a == 0) {
       const double r[3] = {x, y, z};
@@ -418,6 +419,14 @@ static int pick_lines(int L, size_t elem) {
 
 static size_t al(size_t b) { return (b + 255) & ~size_t(255); }
 
+// Candidate splits per home cell: enough work-groups to fill the chip for
+// small batches, one per cell once the batch has that many cells.
+static int real_splits(int64_t ctot) {
+  const int64_t c = std::max<int64_t>(ctot, 1);
+  const int64_t s = (kRealTargetBlocks + c - 1) / c;
+  return (int)std::min<int64_t>(std::max<int64_t>(s, 1), kRealSplit);
+}
+
 BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, const double* cutoffs,
                      const double* kappas, const int32_t* kmesh, int precision, int convention,
                      double eps) {
@@ -501,11 +510,12 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
   P.o_nb = take(P.nb.bytes);
   P.o_sys_of = take(sizeof(int) * std::max<int64_t>(P.ntot, 1));
   P.o_ps = take(sizeof(P4<double>) * std::max<int64_t>(P.ntot, 1));
-  P.o_grad = take(sizeof(double) * 3 * kRealSplit * std::max<int64_t>(P.ntot, 1));
+  P.o_grad = take(sizeof(double) * 3 * std::max<int64_t>(P.ntot, 1));
   P.o_mesh = take(es * std::max<int64_t>(P.mtot, 1));
   P.o_spec = take(ces * std::max<int64_t>(P.stot, 1));
   P.o_tiles = take(es * std::max<int64_t>(P.ttot, 1));
-  P.o_rpart = take(sizeof(double) * kRealNV * kRealSplit * std::max<int64_t>(P.ctot, 1));
+  P.nsplit = real_splits(P.ctot);
+  P.o_rpart = take(sizeof(double) * kRealNV * P.nsplit * std::max<int64_t>(P.ctot, 1));
   P.o_cpart = take(sizeof(double) * 10 * std::max<int64_t>(rows_x, 1));
   P.o_gpart = take(sizeof(double) * 9 * std::max<int64_t>(rows_g, 1));
   P.o_red = take(sizeof(double) * kSysValues * nsys);
@@ -569,22 +579,22 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   const NbLayout& L = P.nb;
   int* count = (int*)(nbws + L.count);
   int* start = (int*)(nbws + L.start);
-  double* gsplit = (double*)(ws + P.o_grad);
+  double* grad = (double*)(ws + P.o_grad);
   T* mesh = (T*)(ws + P.o_mesh);
   T* tiles = (T*)(ws + P.o_tiles);
   CT* spec = (CT*)(ws + P.o_spec);
   P4<double>* ps = (P4<double>*)(ws + P.o_ps);
   int* tickets = (int*)(ws + P.o_tickets);
   const unsigned nb_atoms = (unsigned)std::max<int64_t>((ntot + kBlock - 1) / kBlock, 1);
-  const int64_t rows_real = P.ctot * kRealSplit;
+  const int64_t rows_real = P.ctot * P.nsplit;
   const int64_t rows_x = P.off[kSegFftX][nsys], rows_g = P.off[kSegGather][nsys];
 
   mark(0);
   // ---- prologue, binning, in-cell order
   const int64_t ncount = 2 * (P.ctot * kSub + 1);  // counters and scatter cursors
-  const int64_t nfill = std::max(ncount, P.mtot);
+  const int64_t nfill = std::max(std::max(ncount, P.mtot), 3 * ntot);
   k_b_prologue<T><<<(unsigned)std::min<int64_t>((nfill + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
-      nsys == 1, one, tab, offs, count, ncount, mesh, P.mtot, tickets, nsys);
+      nsys == 1, one, tab, offs, count, ncount, mesh, P.mtot, grad, 3 * ntot, tickets, nsys);
   if (ntot > 0)
     k_b_bin<<<nb_atoms, kBlock, 0, s>>>(ntot, nsys, seg(kSegAtom), tab, pos, (double*)(nbws + L.w),
                                         (int*)(nbws + L.cell_of), (int*)(ws + P.o_sys_of), count);
@@ -597,9 +607,9 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   // ---- real space (also for empty batches: its partial rows feed the reductions)
   double* rpart = (double*)(ws + P.o_rpart);
   if (P.ctot > 0) {
-    dim3 grid((unsigned)P.ctot, kRealSplit);
-    k_b_real<T><<<grid, kBlock, 0, s>>>(ntot, nsys, seg(kSegCell), tab, ps,
-                                        (const int*)(nbws + L.sorted), start, pos, gsplit, rpart);
+    dim3 grid((unsigned)P.ctot, (unsigned)P.nsplit);
+    k_b_real<T><<<grid, kBlock, 0, s>>>(nsys, seg(kSegCell), tab, ps,
+                                        (const int*)(nbws + L.sorted), start, pos, grad, rpart);
   }
   mark(2);
   // ---- spread into per-cell tiles (tiled systems) or the mesh (the others)
@@ -626,8 +636,7 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   double* gpart = (double*)(ws + P.o_gpart);
   if (rows_g > 0)
     k_b_gather<T><<<(unsigned)rows_g, kBlock, 0, s>>>(nsys, seg(kSegGather), tab, pos, q, mesh,
-                                                      gsplit, kRealSplit, ntot, forces, gpart,
-                                                      rows_g);
+                                                      grad, forces, gpart, rows_g);
   mark(5);
   // ---- per-system reductions and outputs
   PartialSets sets;
@@ -637,7 +646,7 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   sets.rows_real = rows_real;
   sets.rows_recip = rows_x;
   sets.rows_gath = rows_g;
-  sets.nsplit = kRealSplit;
+  sets.nsplit = P.nsplit;
   k_b_reduce_finalize<<<dim3(nsys, kSysValues), kBlock, 0, s>>>(
       nsys, offs, tab, sets, (double*)(ws + P.o_red), tickets, energy, dEdbox);
   mark(6);

@@ -40,6 +40,7 @@ struct BatchPlan {
   size_t lds_fft[3];   // dynamic LDS of the z, y and x passes
   size_t lds_tile;
   int any_untiled;     // some system spreads with global atomics
+  int nsplit;          // real-space candidate splits per cell
   NbLayout nb;         // binning workspace over all cells
   CellGrid cg_all;     // only ncell is used by the shared binning helpers
   // workspace byte offsets

@@ -11,10 +11,12 @@ constexpr int kRealNV = 18;               // fused real-space partial values per
 constexpr int kPairBlocksPerFrame = 256;  // fixed grid of the pair-energy reduction
 constexpr int kSub = 8;                   // sub-counters per cell (atomic contention)
 constexpr int64_t kMaxCells = 1 << 21;    // cap on the cell-grid size
-constexpr int kCandCap = 512;     // staged candidates per pass of the cell kernel (16 KiB)
+constexpr int kCandCap = 256;     // staged candidates per pass of the cell kernel
 constexpr int kHomeCap = 128;     // home atoms per pass of the cell kernel
 constexpr int kMaxStencil = 128;  // stencil entries the cell kernel supports
-constexpr int kRealSplit = 8;     // candidate splits (work-groups) per home cell
+constexpr int kRealSplit = 8;     // most candidate splits (work-groups) per home cell
+constexpr int64_t kRealTargetBlocks = 512;  // real-space work-groups aimed for (measured best
+                                            // at 24k atoms: 2 splits of 343 cells)
 constexpr int kFilterUnroll = 4;  // candidates tested per lane between queue checks
 constexpr int kQueueCap = 64 + 64 * kFilterUnroll;  // per-wavefront pair queue entries
 

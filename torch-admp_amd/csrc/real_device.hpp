@@ -2,6 +2,8 @@
 // single-system and batched paths.
 #pragma once
 
+#include <climits>
+
 #include "nblist_kernels.hpp"
 
 namespace admp {
@@ -58,29 +60,62 @@ __device__ __forceinline__ Staged<T> stage(double x, double y, double z, double 
   return s;
 }
 
+// One unordered pair per lane of a wavefront (`valid` lanes only): adds dE/dr_i
+// to the home accumulator, the energy and the pair virial, and dE/dr_j to the
+// candidate accumulator.  Lanes holding the same candidate form contiguous
+// runs (queue order), so their candidate terms are summed by a segmented
+// wavefront reduction and added by one LDS atomic per run instead of
+// colliding on one address.
 template <typename T>
-__device__ __forceinline__ void pair_eval(const Staged<T>& pi, const Staged<T>& pj, T kappa,
-                                          T eps, T c_exp, T k2, double* facc_i, PairAcc<T>& acc) {
-  // hi parts of nearby atoms subtract exactly; the lo parts restore float64 accuracy
-  const T dx = (pj.xh - pi.xh) + (pj.xl - pi.xl);
-  const T dy = (pj.yh - pi.yh) + (pj.yl - pi.yl);
-  const T dz = (pj.zh - pi.zh) + (pj.zl - pi.zl);
-  const T r2 = dx * dx + dy * dy + dz * dz;
-  const T r = sqrt(r2);
-  const T ir = T(1) / r;
-  const T qq = eps * pi.q * pj.q;
-  const T ec = erfc(kappa * r);
-  const T fs = -qq * (c_exp * exp(-k2 * r2) + ec * ir) * ir * ir;  // (de/dr) / r
-  acc.e += (double)(qq * ec * ir);
-  atomicAdd(facc_i + 0, (double)(-fs * dx));
-  atomicAdd(facc_i + 1, (double)(-fs * dy));
-  atomicAdd(facc_i + 2, (double)(-fs * dz));
-  acc.vxx += fs * dx * dx;
-  acc.vyy += fs * dy * dy;
-  acc.vzz += fs * dz * dz;
-  acc.vxy += fs * dx * dy;
-  acc.vxz += fs * dx * dz;
-  acc.vyz += fs * dy * dz;
+__device__ __forceinline__ void pair_eval(const Staged<T>& pi, const Staged<T>& pj, bool valid,
+                                          int kx, T kappa, T eps, T c_exp, T k2, double* facc_i,
+                                          double* cacc, PairAcc<T>& acc) {
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  if (valid) {
+    // hi parts of nearby atoms subtract exactly; the lo parts restore float64 accuracy
+    const T dx = (pj.xh - pi.xh) + (pj.xl - pi.xl);
+    const T dy = (pj.yh - pi.yh) + (pj.yl - pi.yl);
+    const T dz = (pj.zh - pi.zh) + (pj.zl - pi.zl);
+    const T r2 = dx * dx + dy * dy + dz * dz;
+    const T r = sqrt(r2);
+    const T ir = T(1) / r;
+    const T qq = eps * pi.q * pj.q;
+    const T ec = erfc(kappa * r);
+    const T fs = -qq * (c_exp * exp(-k2 * r2) + ec * ir) * ir * ir;  // (de/dr) / r
+    acc.e += (double)(qq * ec * ir);
+    gx = (double)(fs * dx);
+    gy = (double)(fs * dy);
+    gz = (double)(fs * dz);
+    atomicAdd(facc_i + 0, -gx);
+    atomicAdd(facc_i + 1, -gy);
+    atomicAdd(facc_i + 2, -gz);
+    acc.vxx += fs * dx * dx;
+    acc.vyy += fs * dy * dy;
+    acc.vzz += fs * dz * dz;
+    acc.vxy += fs * dx * dy;
+    acc.vxz += fs * dx * dz;
+    acc.vyz += fs * dy * dz;
+  }
+  const int lane = threadIdx.x & 63;
+  const int key = valid ? kx : -1 - lane;  // invalid lanes form runs of their own
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double tx = __shfl_down(gx, off, 64);
+    const double ty = __shfl_down(gy, off, 64);
+    const double tz = __shfl_down(gz, off, 64);
+    const int tk = __shfl_down(key, off, 64);
+    if (lane + off < 64 && tk == key) {
+      gx += tx;
+      gy += ty;
+      gz += tz;
+    }
+  }
+  const int prev = __shfl_up(key, 1, 64);
+  if (valid && (lane == 0 || prev != key)) {
+    atomicAdd(cacc + 3 * kx, gx);
+    atomicAdd(cacc + 3 * kx + 1, gy);
+    atomicAdd(cacc + 3 * kx + 2, gz);
+  }
 }
 
 // Stencil entries [st0, st0 + kMaxStencil) of home cell ci: first slot, atom
@@ -127,26 +162,30 @@ __device__ __forceinline__ int load_stencil_chunk(const CellGrid& cg, int64_t ce
   return m;
 }
 
-// One work-group's share of the cell-staged real-space sum: home cell lc of a
-// grid whose cells start at global index cell0 (0 for a single system), the
-// candidates of stencil split blockIdx.y out of gridDim.y.  Candidates are
-// staged in LDS as hi/lo pairs relative to the home-cell origin; lanes of a
-// wavefront filter (home, candidate) pairs against the cutoff and compact the
-// survivors into a per-wavefront queue that is evaluated 64 pairs at a time.
-// Stencils longer than kMaxStencil entries (boxes far below the cutoff) are
-// walked in chunks.  Home forces go to gsplit[split][i][3] (rows of n atoms);
-// the block's energy, virial, r x force, sum q^2 and sum q (kRealNV values)
-// go to partial[k * pstride].
+// One work-group's share of the cell-staged real-space sum, each unordered
+// pair once (Newton's third law): home cell lc of a grid whose cells start at
+// global index cell0, against the half stencil -- the entries after the centre
+// in lexicographic offset order, plus the centre itself with slot j > slot i --
+// split over gridDim.y work-groups (candidate k goes to split k mod S).
+// Candidates are staged in LDS as hi/lo pairs relative to the home-cell
+// origin; lanes of a wavefront filter (home, candidate) pairs against the
+// cutoff and compact the survivors into a per-wavefront queue evaluated 64
+// pairs at a time.  dE/dr of home and candidate atoms gather in float64 LDS
+// accumulators and are added to grad[i][3] with float64 global atomics
+// (grad zeroed beforehand).  The block's energy, pair virial, sum r (x) dE/dr,
+// sum q^2 and sum q (kRealNV values) go to partial[k * pstride].
 template <typename T>
 __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0, int lc,
-                                               int64_t n, T kappa, T eps,
+                                               T kappa, T eps,
                                                const P4<double>* __restrict__ ps,
                                                const int* __restrict__ sorted,
                                                const int* __restrict__ start,
                                                const double* __restrict__ pos,
-                                               double* __restrict__ gsplit,
+                                               double* __restrict__ grad,
                                                double* __restrict__ partial, int64_t pstride) {
   __shared__ Staged<T> cand[kCandCap];
+  __shared__ int cslot[kCandCap];
+  __shared__ double cacc[kCandCap * 3];
   __shared__ Staged<T> home[kHomeCap];
   __shared__ double facc[kHomeCap * 3];  // float64 LDS atomics in both precisions
   __shared__ int queue[(kBlock / 64) * kQueueCap];
@@ -158,7 +197,9 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
   const int split = blockIdx.y, S = gridDim.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ci[3] = {lc / (cg.nc[1] * cg.nc[2]), (lc / cg.nc[2]) % cg.nc[1], lc % cg.nc[2]};
-  const int nst = stencil_len(cg, 0) * stencil_len(cg, 1) * stencil_len(cg, 2);
+  const int l0 = stencil_len(cg, 0), l1 = stencil_len(cg, 1), l2 = stencil_len(cg, 2);
+  const int nst = l0 * l1 * l2;
+  const int st_c = ((l0 / 2) * l1 + l1 / 2) * l2 + l2 / 2;  // the zero offset
   double o3[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -179,9 +220,10 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
 #pragma unroll
   for (int k = 0; k < kRealNV; ++k) acc[k] = 0.0;
   int* q = queue + wave * kQueueCap;
-  const bool one_chunk = nst <= kMaxStencil;
+  const int nhalf = nst - st_c;
+  const bool one_chunk = nhalf <= kMaxStencil;
   int m = 0;
-  if (one_chunk) m = load_stencil_chunk(cg, cell0, ci, 0, nst, start, st_pre, st_beg, st_sh);
+  if (one_chunk) m = load_stencil_chunk(cg, cell0, ci, st_c, nst, start, st_pre, st_beg, st_sh);
 
   for (int h0 = hb; h0 < he; h0 += kHomeCap) {
     const int hc = min(kHomeCap, he - h0);
@@ -190,11 +232,13 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
       home[t] = stage<T>(p.x - ox, p.y - oy, p.z - oz, p.q);
       facc[3 * t] = facc[3 * t + 1] = facc[3 * t + 2] = 0.0;
     }
-    for (int st0 = 0; st0 < nst; st0 += kMaxStencil) {
+    for (int st0 = st_c; st0 < nst; st0 += kMaxStencil) {
       if (!one_chunk) {
         __syncthreads();
         m = load_stencil_chunk(cg, cell0, ci, st0, nst, start, st_pre, st_beg, st_sh);
       }
+      // the centre entry (home cell, no shift) leads the first chunk
+      const int ncentre = st0 == st_c ? st_pre[1] : 0;
       const int ctot = st_pre[m];
       const int cs = ctot > split ? (ctot - split + S - 1) / S : 0;
       for (int k0 = 0; k0 < cs; k0 += kCandCap) {
@@ -207,9 +251,13 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
             if (st_pre[mid] <= kg) lo = mid;
             else hi = mid;
           }
-          const P4<double> p = ps[st_beg[lo] + (kg - st_pre[lo])];
+          const int slot = st_beg[lo] + (kg - st_pre[lo]);
+          const P4<double> p = ps[slot];
           cand[kk] = stage<T>((p.x + st_sh[lo * 3]) - ox, (p.y + st_sh[lo * 3 + 1]) - oy,
                               (p.z + st_sh[lo * 3 + 2]) - oz, p.q);
+          // centre candidates pair only with home atoms of lower slot
+          cslot[kk] = kg < ncentre ? slot : INT_MAX;
+          cacc[3 * kk] = cacc[3 * kk + 1] = cacc[3 * kk + 2] = 0.0;
         }
         __syncthreads();
         for (int g0 = 0; g0 < hc; g0 += 64) {
@@ -218,6 +266,7 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
           const int hl = lane % hg, ph = lane / hg;
           const bool active = ph < nph;
           const Staged<T> pi = home[g0 + hl];
+          const int my_slot = h0 + g0 + hl;
           const int stride = nph * (kBlock / 64);
           const int nit = (cc + stride - 1) / stride;
           int qn = 0;
@@ -234,7 +283,7 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
                 const Staged<T>& pj = cand[kk];
                 const T dx = pj.xh - pi.xh, dy = pj.yh - pi.yh, dz = pj.zh - pi.zh;
                 const T r2 = dx * dx + dy * dy + dz * dz;
-                pred[u] = r2 < rc2 && r2 > rmin2;
+                pred[u] = r2 < rc2 && r2 > rmin2 && cslot[kk] > my_slot;
               }
             }
 #pragma unroll
@@ -250,7 +299,8 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
               wave_sync();
               const int ent = q[lane];
               const int hx = ent >> 16, kx = ent & 0xffff;
-              pair_eval<T>(home[hx], cand[kx], kappa, eps, c_exp, k2, facc + 3 * hx, pacc);
+              pair_eval<T>(home[hx], cand[kx], true, kx, kappa, eps, c_exp, k2, facc + 3 * hx,
+                           cacc, pacc);
               const int rem = qn - 64;
               // move the tail (at most kQueueCap - 64 entries) to the front
               int tails[kFilterUnroll];
@@ -266,29 +316,54 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
             }
           }
           wave_sync();
-          if (lane < qn) {
-            const int ent = q[lane];
+          if (qn > 0) {
+            const bool valid = lane < qn;
+            const int ent = valid ? q[lane] : 0;
             const int hx = ent >> 16, kx = ent & 0xffff;
-            pair_eval<T>(home[hx], cand[kx], kappa, eps, c_exp, k2, facc + 3 * hx, pacc);
+            pair_eval<T>(home[hx], cand[kx], valid, kx, kappa, eps, c_exp, k2, facc + 3 * hx,
+                         cacc, pacc);
           }
           wave_sync();
         }
         __syncthreads();
+        // candidate gradients of this chunk
+        for (int kk = threadIdx.x; kk < cc; kk += blockDim.x) {
+          const double g[3] = {cacc[3 * kk], cacc[3 * kk + 1], cacc[3 * kk + 2]};
+          if (g[0] == 0.0 && g[1] == 0.0 && g[2] == 0.0) continue;
+          const int kg = split + S * (k0 + kk);
+          int lo = 0, hi = m;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (st_pre[mid] <= kg) lo = mid;
+            else hi = mid;
+          }
+          const int j = sorted[st_beg[lo] + (kg - st_pre[lo])];
+          unsafeAtomicAdd(grad + 3 * j, g[0]);
+          unsafeAtomicAdd(grad + 3 * j + 1, g[1]);
+          unsafeAtomicAdd(grad + 3 * j + 2, g[2]);
+          const double r[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
+#pragma unroll
+          for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+            for (int b2 = 0; b2 < 3; ++b2) acc[7 + a2 * 3 + b2] += r[a2] * g[b2];
+        }
       }
     }
-    // home forces of this split
+    __syncthreads();
+    // home gradients of this split
     for (int t = threadIdx.x; t < hc; t += blockDim.x) {
       const int i = sorted[h0 + t];
       const double g[3] = {facc[3 * t], facc[3 * t + 1], facc[3 * t + 2]};
-      double* dst = gsplit + ((int64_t)split * n + i) * 3;
-      dst[0] = g[0];
-      dst[1] = g[1];
-      dst[2] = g[2];
-      const double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+      if (g[0] != 0.0 || g[1] != 0.0 || g[2] != 0.0) {
+        unsafeAtomicAdd(grad + 3 * i, g[0]);
+        unsafeAtomicAdd(grad + 3 * i + 1, g[1]);
+        unsafeAtomicAdd(grad + 3 * i + 2, g[2]);
+        const double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
 #pragma unroll
-      for (int a2 = 0; a2 < 3; ++a2)
+        for (int a2 = 0; a2 < 3; ++a2)
 #pragma unroll
-        for (int b2 = 0; b2 < 3; ++b2) acc[7 + a2 * 3 + b2] += r[a2] * g[b2];
+          for (int b2 = 0; b2 < 3; ++b2) acc[7 + a2 * 3 + b2] += r[a2] * g[b2];
+      }
       if (split == 0) {
         const double qi = ps[h0 + t].q;
         acc[16] += qi * qi;
@@ -297,13 +372,13 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
     }
     __syncthreads();
   }
-  acc[0] = 0.5 * pacc.e;
-  acc[1] = 0.5 * (double)pacc.vxx;
-  acc[2] = 0.5 * (double)pacc.vyy;
-  acc[3] = 0.5 * (double)pacc.vzz;
-  acc[4] = 0.5 * (double)pacc.vxy;
-  acc[5] = 0.5 * (double)pacc.vxz;
-  acc[6] = 0.5 * (double)pacc.vyz;
+  acc[0] = pacc.e;
+  acc[1] = (double)pacc.vxx;
+  acc[2] = (double)pacc.vyy;
+  acc[3] = (double)pacc.vzz;
+  acc[4] = (double)pacc.vxy;
+  acc[5] = (double)pacc.vxz;
+  acc[6] = (double)pacc.vyz;
   block_sum<kRealNV>(acc, red);
   if (threadIdx.x == 0)
     for (int k = 0; k < kRealNV; ++k) partial[k * pstride] = acc[k];
