@@ -13,6 +13,10 @@ charge_optimization    src/torch_admp/electrode.py:456-560          CONP: equal 
                                                                     atom; CONQ: group charges and
                                                                     equal potential per group;
                                                                     the full-system KKT residual
+conq + ffield          LAMMPS fix electrode/conq ffield (the        the self-consistent drop:
+                       reference raises, electrode.py:235-236)      (psi + dV r) constant per
+                                                                    group, equal to V_g, with
+                                                                    dV = V_0 - V_1; LAMMPS deck
 infer                  src/torch_admp/electrode.py:339-453          forces = -dE/dr at q*, field
                                                                     work; LAMMPS fix electrode
                                                                     forces through the GPU path
@@ -32,9 +36,11 @@ import torch
 from . import nblist_ref, qeq_ref
 
 
-def setup_from_lammps(n_atoms, groups, symm=False):
+def setup_from_lammps(n_atoms, groups, symm=False, conq_ffield=False):
     """groups: list of dicts {indices, mode ('conp'|'conq'), value, eta, chi=0,
-    hardness=0, ffield=False}.  Same outputs as the reference function on the CPU."""
+    hardness=0, ffield=False}.  Same outputs as the reference function on the CPU;
+    conq_ffield=True admits two constant-charge ffield groups (ffield potential
+    None: the drop is solved)."""
     mask = np.zeros(n_atoms, dtype=bool)
     eta = np.zeros(n_atoms)
     chi = np.zeros(n_atoms)
@@ -50,7 +56,7 @@ def setup_from_lammps(n_atoms, groups, symm=False):
             if symm:
                 raise AttributeError("symm should be False for conq, user can implement symm "
                                      "by conq")
-            if g.get("ffield", False):
+            if g.get("ffield", False) and not conq_ffield:
                 raise AttributeError("ffield with conq has not been implemented yet")
             r = np.zeros(n_atoms)
             r[idx] = 1.0
@@ -72,7 +78,8 @@ def setup_from_lammps(n_atoms, groups, symm=False):
                                                                        dtype=torch.float64)
     b = torch.tensor(vals, dtype=torch.float64)
     ffm = torch.tensor(np.stack(ff_rows)) if ff_rows else None
-    ffp = torch.tensor(ff_pot, dtype=torch.float64) if ff_rows else None
+    conq_ff = any(g["mode"] == "conq" and g.get("ffield", False) for g in groups)
+    ffp = torch.tensor(ff_pot, dtype=torch.float64) if ff_rows and not conq_ff else None
     return (torch.tensor(mask), torch.tensor(eta), torch.tensor(chi), torch.tensor(hard), A, b,
             ffm, ffp)
 
@@ -110,14 +117,36 @@ def charge_optimization(orc: qeq_ref.QEqOracle, pos, box, q, mask, eta, chi, har
     phi = full_potential(orc, q0, pos, box, eta, pairs, ds, bs)
     chi_e = (chi + phi)[mask]
     field = None
+    ramp = None
     if ff_mask is not None:
-        add, field = finite_field_add_chi(pos, box, ff_mask, ff_pot, axis)
-        chi_e = chi_e + add
+        if ff_pot is not None:
+            add, field = finite_field_add_chi(pos, box, ff_mask, ff_pot, axis)
+            chi_e = chi_e + add
+        else:
+            ramp, unit = finite_field_add_chi(pos, box, ff_mask,
+                                              torch.tensor([1.0, 0.0], dtype=pos.dtype), axis)
     pe = pos[mask]
     sub_pairs = nblist_ref.nblist_pbc(pe, box, orc.coulomb.rcut)
     sub_ds = nblist_ref.pairs_ds(pe, sub_pairs, box)
     sub_bs = nblist_ref.buffer_scales(sub_pairs)
     H = orc.hessian(pe, box, chi_e, hard[mask], eta[mask], sub_pairs, sub_ds, sub_bs)
+    if ramp is not None:
+        # unknowns [q, mu_0, mu_1, dV]: H q + A^T mu + dV r = -chi, A q = b,
+        # dV = V_0 - V_1 with the electrode potentials V_g = -mu_g
+        n, nc = H.shape[0], A.shape[0]
+        rows = [int(torch.argmax((A.bool() == g[mask][None]).all(1).to(torch.int)))
+                for g in ff_mask]
+        M = torch.zeros(n + nc + 1, n + nc + 1, dtype=H.dtype)
+        M[:n, :n] = H
+        M[:n, n:n + nc] = A.T
+        M[:n, -1] = ramp
+        M[n:n + nc, :n] = A
+        M[-1, n + rows[0]] = 1.0
+        M[-1, n + rows[1]] = -1.0
+        M[-1, -1] = 1.0
+        rhs = torch.cat([-chi_e, b, torch.zeros(1, dtype=H.dtype)])
+        sol = torch.linalg.solve(M, rhs)
+        return sol[:n], sol[-1] * unit
     if A.shape[0] == 0:
         q_e, _, _ = qeq_ref.kkt_solve(H, chi_e)
     else:

@@ -21,7 +21,7 @@ OUT = Path(__file__).resolve().parent
 DECKS = ["lmp_conp_slab_2d", "lmp_conp_slab_3d", "lmp_conp_interface_2d_pzc",
          "lmp_conp_interface_3d_pzc", "lmp_conp_interface_2d_bias", "lmp_conp_interface_3d_bias",
          "lmp_conq_interface_2d_pzc", "lmp_conq_interface_3d_pzc", "lmp_conq_interface_2d_edl",
-         "lmp_conq_interface_3d_edl", "lmp_conq_interface_2d_bias"]
+         "lmp_conq_interface_3d_edl", "lmp_conq_interface_2d_bias", "lmp_conq_interface_3d_bias"]
 
 
 def read_dump(path: Path):

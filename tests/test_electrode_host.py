@@ -42,6 +42,15 @@ def test_reference_errors():
                                          ffield=True)]
     with pytest.raises(AttributeError, match="ffield with conq has not been implemented yet"):
         setup_from_lammps(6, conq_ff, device="cpu")
+    # opt-in: two constant-charge ffield groups, the drop left to the solve
+    two = conq_ff + [LAMMPSElectrodeConstraint(indices=np.arange(3, 6), mode="conq",
+                                               value=-1.0, eta=1.6, ffield=True)]
+    out = setup_from_lammps(8, two, device="cpu", conq_ffield=True)
+    assert out[6].shape == (2, 8) and out[7] is None and out[4].shape == (2, 6)
+    mixed = [two[0], LAMMPSElectrodeConstraint(indices=np.arange(3, 6), mode="conp", value=0.0,
+                                               eta=1.6, ffield=True)]
+    with pytest.raises(AttributeError, match="both be conp or both be conq"):
+        setup_from_lammps(8, mixed, device="cpu", conq_ffield=True)
     conq = [LAMMPSElectrodeConstraint(indices=np.arange(3), mode="conq", value=1.0, eta=1.6)]
     with pytest.raises(AttributeError, match="symm should be False for conq"):
         setup_from_lammps(6, conq, True, device="cpu")

@@ -4,8 +4,9 @@ LAMMPS decks: the reference's own parity cases and tolerances
 (tests/test_electrode_conp.py:35-323, tests/test_electrode_conq.py:29-223):
 rcut 5, kappa 0.5, ethresh 1e-6, solver eps 1e-6, slab factor 3 for p p f
 cells; forces within atol = rtol = 5e-5 (slabs) or 5e-4 (interfaces) of the
-LAMMPS dump, for both solve methods.  The CONQ 3D-bias deck is not asserted by
-the reference (ffield with conq raises) and is checked for that error only.
+LAMMPS dump, for both solve methods.  The CONQ 3D-bias deck (finite field
+with constant charges), which the reference does not run (ffield with conq
+raises), is asserted here at the same 5e-4 with the self-consistent drop.
 Oracle: the small electrode system of test_oracle_electrode, charges and forces
 within 1e-8.
 """
@@ -53,6 +54,9 @@ DECKS = {
                                   [dict(indices=np.arange(216), value=-10.0, mode="conq",
                                         ffield=False)], False),
     "lmp_conq_interface_2d_bias": (True, 5e-4, _pair(-10.0, 10.0, "conq", False), False),
+    # finite field with constant charges: the reference raises here
+    # (tests/test_electrode_conq.py:193-223); the drop is solved self-consistently
+    "lmp_conq_interface_3d_bias": (False, 5e-4, _pair(-10.0, 10.0, "conq", True), False),
 }
 
 
@@ -70,7 +74,8 @@ def test_lammps_fix_electrode(dev, deck):
     box = torch.tensor(cell, device=dev)
     charges = torch.tensor(d["q"], device=dev).requires_grad_(True)
     cons = [LAMMPSElectrodeConstraint(eta=1.6, **g) for g in groups]
-    inputs = setup_from_lammps(len(d["q"]), cons, symm, device=dev)
+    inputs = setup_from_lammps(len(d["q"]), cons, symm, device=dev,
+                               conq_ffield=deck == "lmp_conq_interface_3d_bias")
     nbl = TorchNeighborList(cutoff=rcut)
     pairs = nbl(pos, box)
     for method in ("lbfgs", "matinv"):
@@ -81,13 +86,14 @@ def test_lammps_fix_electrode(dev, deck):
         assert torch.isfinite(energy)
 
 
-@pytest.mark.parametrize("case", ["conp_slab", "conq_slab", "conp_ffield"])
+@pytest.mark.parametrize("case", ["conp_slab", "conq_slab", "conp_ffield", "conq_ffield"])
 def test_electrode_matches_oracle(dev, case):
     pos, box, q = electrode_system(seed=5)
-    if case == "conq_slab":
-        groups = [dict(indices=range(18), mode="conq", value=-1.0, eta=1.6),
-                  dict(indices=range(18, 36), mode="conq", value=1.0, eta=1.6)]
-        symm, slab = False, True
+    if case.startswith("conq"):
+        ff = case == "conq_ffield"
+        groups = [dict(indices=range(18), mode="conq", value=-1.0, eta=1.6, ffield=ff),
+                  dict(indices=range(18, 36), mode="conq", value=1.0, eta=1.6, ffield=ff)]
+        symm, slab = False, not ff
     else:
         ff = case == "conp_ffield"
         groups = [dict(indices=range(18), mode="conp", value=2.0, eta=1.6, ffield=ff),
@@ -95,13 +101,13 @@ def test_electrode_matches_oracle(dev, case):
         symm, slab = True, not ff
     rc, km = 4.0, [12, 12, 40]
     orc = qeq_ref.QEqOracle(rc, kappa=0.5, kmesh=km, slab_corr=slab)
-    ref = electrode_ref.setup_from_lammps(len(q), groups, symm)
+    ref = electrode_ref.setup_from_lammps(len(q), groups, symm, conq_ffield=True)
     e_ref, f_ref, q_ref = electrode_ref.infer(orc, pos, box, q, *ref)
 
     calc = PolarizableElectrode(rcut=rc, kappa=0.5, kmesh=km, slab_corr=slab, eps=1e-10)
     cons = [LAMMPSElectrodeConstraint(**{k: (np.array(list(v)) if k == "indices" else v)
                                          for k, v in g.items()}) for g in groups]
-    inputs = setup_from_lammps(len(q), cons, symm, device=dev)
+    inputs = setup_from_lammps(len(q), cons, symm, device=dev, conq_ffield=True)
     p = pos.to(dev).requires_grad_(True)
     b = box.to(dev)
     nbl = TorchNeighborList(cutoff=rc)

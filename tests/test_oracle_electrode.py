@@ -118,3 +118,31 @@ def test_conp_finite_field_equal_potential():
     ramp, _ = electrode_ref.finite_field_add_chi(pos, box, ffm, ffp)
     shifted = (psi + chi)[mask] + ramp
     assert float(shifted.max() - shifted.min()) < 1e-8 * float(psi[mask].abs().max())
+
+
+def test_conq_finite_field_self_consistent_drop():
+    """CONQ with a finite field (LAMMPS ffield on): the drop dV is solved with the
+    charges; on each group psi + dV r is one constant V_g and dV = V_0 - V_1."""
+    pos, box, q = electrode_system(seed=6)
+    groups = [dict(indices=range(18), mode="conq", value=-1.0, eta=1.6, ffield=True),
+              dict(indices=range(18, 36), mode="conq", value=1.0, eta=1.6, ffield=True)]
+    mask, eta, chi, hard, A, b, ffm, ffp = electrode_ref.setup_from_lammps(len(q), groups,
+                                                                          conq_ffield=True)
+    assert ffp is None
+    orc = qeq_ref.QEqOracle(4.0, kappa=0.5, kmesh=[12, 12, 40])
+    q_e, field = electrode_ref.charge_optimization(orc, pos, box, q, mask, eta, chi, hard, A, b,
+                                                   ffm, ffp)
+    assert abs(float(q_e[:18].sum()) + 1.0) < 1e-10 and abs(float(q_e[18:].sum()) - 1.0) < 1e-10
+    q_full = q.clone()
+    q_full[mask] = q_e
+    psi = _potential(orc, q_full, pos, box, eta)
+    ramp, unit = electrode_ref.finite_field_add_chi(pos, box, ffm,
+                                                    torch.tensor([1.0, 0.0], dtype=F64))
+    dv = float(field / unit)
+    v = psi[mask] + dv * ramp
+    scale = float(psi[mask].abs().max())
+    for g in (v[:18], v[18:]):
+        assert float(g.max() - g.min()) < 1e-8 * scale
+    assert abs(float(v[:18].mean() - v[18:].mean()) - dv) < 1e-8 * scale
+    with pytest.raises(AttributeError, match="ffield with conq"):
+        electrode_ref.setup_from_lammps(len(q), groups)

@@ -90,12 +90,18 @@ class LAMMPSElectrodeConstraint:
 
 
 def setup_from_lammps(n_atoms: int, constraint_list: List[LAMMPSElectrodeConstraint],
-                      symm: bool = False, device=None, dtype=torch.float64):
+                      symm: bool = False, device=None, dtype=torch.float64,
+                      conq_ffield: bool = False):
     """Per-atom electrode arrays from LAMMPS-style groups (src/torch_admp/electrode.py:203-285).
 
     Returns (mask, eta, chi, hardness, constraint_matrix over electrode atoms,
     constraint_vals, ffield_electrode_mask or None, ffield_potential or None).
     The Gaussian width is eta_i = sqrt(2) / (2 eta_lammps).
+
+    ``conq_ffield=True`` enables the finite field with two constant-charge
+    groups (LAMMPS `fix electrode/conq ... ffield on`), which the reference
+    rejects: the potential drop is then an unknown, returned ffield_potential
+    is None and charge_optimization solves it self-consistently.
     """
     if device is None:
         device = torch.device("cuda") if torch.cuda.is_available() else torch.device("cpu")
@@ -113,7 +119,7 @@ def setup_from_lammps(n_atoms: int, constraint_list: List[LAMMPSElectrodeConstra
             if symm:
                 raise AttributeError("symm should be False for conq, user can implement symm "
                                      "by conq")
-            if c.ffield:
+            if c.ffield and not conq_ffield:
                 raise AttributeError("ffield with conq has not been implemented yet")
             row = np.zeros(n_atoms)
             row[c.indices] = 1.0
@@ -130,7 +136,13 @@ def setup_from_lammps(n_atoms: int, constraint_list: List[LAMMPSElectrodeConstra
         ff_mask, ff_potential = None, None
     elif len(ff_rows) == 2:
         ff_mask = torch.tensor(np.stack(ff_rows), dtype=torch.bool, device=device)
-        ff_potential = torch.tensor(ff_pot, dtype=dtype, device=device)
+        modes = [c.mode for c in constraint_list if c.ffield]
+        if "conq" in modes:
+            if modes != ["conq", "conq"]:
+                raise AttributeError("ffield groups must both be conp or both be conq")
+            ff_potential = None  # the drop is solved with the charges
+        else:
+            ff_potential = torch.tensor(ff_pot, dtype=dtype, device=device)
     else:
         raise AttributeError("number of ffield group should be 0 or 2")
     if symm:
@@ -188,12 +200,18 @@ def charge_optimization(calculator: PolarizableElectrode, positions, box, charge
                                                pairs, ds, buffer_scales)
     chi_e = (chi + phi)[electrode_mask]
     efield = None
+    ramp = None
     if ffield_electrode_mask is not None:
-        chi_ff, ef = finite_field_add_chi(positions, box, ffield_electrode_mask, ffield_potential,
+        drop = ffield_potential if ffield_potential is not None else torch.tensor(
+            [1.0, 0.0], dtype=dtype, device=device)
+        chi_ff, ef = finite_field_add_chi(positions, box, ffield_electrode_mask, drop,
                                           calculator.slab_axis)
-        chi_e = chi_e + chi_ff
-        efield = torch.zeros(3, dtype=dtype, device=device)
-        efield[calculator.slab_axis] = ef
+        if ffield_potential is not None:
+            chi_e = chi_e + chi_ff
+            efield = torch.zeros(3, dtype=dtype, device=device)
+            efield[calculator.slab_axis] = ef
+        else:
+            ramp, unit_field = chi_ff, ef  # per unit drop; the drop is solved below
     # the electrode-electrode pairs, renumbered within the electrode subsystem
     pair_mask = electrode_mask[pairs[:, 0]] & electrode_mask[pairs[:, 1]]
     remap = torch.full((electrode_mask.size(0),), -1, dtype=torch.long, device=device)
@@ -206,12 +224,51 @@ def charge_optimization(calculator: PolarizableElectrode, positions, box, charge
     if constraint_matrix is not None and constraint_matrix.shape[0] == 0:
         kwargs["constraint_matrix"] = None
         kwargs["constraint_vals"] = None
+    if ramp is not None:
+        q_e, drop = _conq_ffield_solve(calculator, kwargs, ffield_electrode_mask[:, electrode_mask],
+                                       ramp, method)
+        efield = torch.zeros(3, dtype=dtype, device=device)
+        efield[calculator.slab_axis] = drop * unit_field
+        return q_e, efield
     if method == "matinv":
         _, q_e = matinv_optimize(**kwargs)
     else:
         _, q_e = pgrad_optimize(q0=charges[electrode_mask].reshape(-1, 1), method=method,
                                 reinit_q=True, **kwargs)
     return q_e, efield
+
+
+def _solve(kwargs, chi, vals, method):
+    kw = dict(kwargs, chi=chi, constraint_vals=vals)
+    if method == "matinv":
+        return matinv_optimize(**kw)[1].detach()
+    return pgrad_optimize(q0=None, method=method, reinit_q=True, **kw)[1].detach()
+
+
+def _conq_ffield_solve(calculator, kwargs, ff_groups, ramp, method):
+    """Constant group charges under the finite field whose potential drop dV is
+    itself unknown.  Stationarity H q + c + dV r + A^T mu = 0 with A q = b and
+    the electrode potentials -mu_g giving dV = mu_1 - mu_0.  Being linear,
+    q = q_a + dV q_b with q_a the solve at dV = 0 and q_b the solve of the
+    unit ramp with zero group charges; their multipliers fix dV."""
+    A, b = kwargs["constraint_matrix"], kwargs["constraint_vals"]
+    chi = kwargs["chi"]
+    q_a = _solve(kwargs, chi, b, method)
+    q_b = _solve(kwargs, ramp, torch.zeros_like(b), method)
+    args = [kwargs[k] for k in ("positions", "box", "chi", "hardness", "eta", "pairs", "ds",
+                                "buffer_scales")]
+    module = kwargs["module"]
+
+    def multipliers(q, c):
+        hq = module.hessian_matvec(q, *args).to(q.dtype)
+        return -torch.linalg.solve(A @ A.T, A @ (hq + c))
+
+    # the two ffield groups as rows of the constraint matrix
+    rows = [int(torch.argmax((A.bool() == g[None]).all(1).to(torch.int))) for g in ff_groups]
+    mu_a = multipliers(q_a, chi)[rows]
+    mu_b = multipliers(q_b, ramp)[rows]
+    drop = (mu_a[1] - mu_a[0]) / (1.0 - (mu_b[1] - mu_b[0]))
+    return q_a + drop * q_b, drop
 
 
 def infer(calculator: PolarizableElectrode, positions: torch.Tensor, box: torch.Tensor,
