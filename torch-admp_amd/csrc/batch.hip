@@ -194,27 +194,85 @@ __global__ void __launch_bounds__(kBlock) k_b_spread_atoms(int64_t ntot,
 
 // ------------------------------------------------------------------ FFT passes
 // r2c along z of mesh + (for tiled systems) the sum of the cell tiles that
-// cover each point, so the tiles are reduced on the way into the FFT.
+// cover each point, so the tiles are reduced on the way into the FFT.  The
+// (tile, offset) covers of the block's z indices and of its rows' x and y
+// indices are tabulated in LDS once, so each point costs only its few tile
+// reads.
+constexpr int kCovMaxL = 256;   // longest z line with tabulated covers
+constexpr int kCovMaxRows = 32;  // rows of one r2c block (2 x 16 lines)
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_b_r2c(int nsys, const int64_t* __restrict__ off,
                                                   const SysDesc* __restrict__ tab,
                                                   const T* __restrict__ meshes,
                                                   const T* __restrict__ tiles,
                                                   cplx<T>* __restrict__ specs) {
+  __shared__ short zt[kCovMaxL * kMaxCover], zo[kCovMaxL * kMaxCover];
+  __shared__ unsigned char zn[kCovMaxL];
+  __shared__ short xt[kCovMaxRows * kMaxCover], xo[kCovMaxRows * kMaxCover];
+  __shared__ short yt[kCovMaxRows * kMaxCover], yo[kCovMaxRows * kMaxCover];
+  __shared__ unsigned char xn[kCovMaxRows], yn[kCovMaxRows];
   const int sid = find_seg(off, nsys, blockIdx.x);
   const SysDesc* D = tab + sid;
   const int C = D->fchunk[0];
   const int L = D->g.k[2];
+  const int K1 = D->g.k[1];
   const T* mesh = meshes + D->mesh0;
-  if (D->tg.ok) {
+  const int64_t nrows = (int64_t)D->g.k[0] * K1;
+  const int64_t row0 = (blockIdx.x - off[sid]) * 2 * C;
+  if (D->tg.ok && L <= kCovMaxL && 2 * C <= kCovMaxRows) {
+    const TileGeom tg = D->tg;
+    for (int z = threadIdx.x; z < L; z += blockDim.x) {
+      int t[kMaxCover], o[kMaxCover];
+      const int n = tile_cover(z, tg.K[2], tg.nc[2], tg.W[2], t, o);
+      zn[z] = (unsigned char)n;
+      for (int k = 0; k < n; ++k) {
+        zt[z * kMaxCover + k] = (short)t[k];
+        zo[z * kMaxCover + k] = (short)o[k];
+      }
+    }
+    for (int r = threadIdx.x; r < 2 * C; r += blockDim.x) {
+      const int64_t row = min(row0 + r, nrows - 1);
+      int t[kMaxCover], o[kMaxCover];
+      int n = tile_cover((int)(row / K1), tg.K[0], tg.nc[0], tg.W[0], t, o);
+      xn[r] = (unsigned char)n;
+      for (int k = 0; k < n; ++k) {
+        xt[r * kMaxCover + k] = (short)t[k];
+        xo[r * kMaxCover + k] = (short)o[k];
+      }
+      n = tile_cover((int)(row % K1), tg.K[1], tg.nc[1], tg.W[1], t, o);
+      yn[r] = (unsigned char)n;
+      for (int k = 0; k < n; ++k) {
+        yt[r * kMaxCover + k] = (short)t[k];
+        yo[r * kMaxCover + k] = (short)o[k];
+      }
+    }
+    __syncthreads();
+    const T* tl = tiles + D->tile0;
+    r2c_rows<T>(
+        [=](int64_t row, int z) {
+          const int r = (int)(row - row0);
+          T s = mesh[row * L + z];
+          const int nz = zn[z];
+          for (int a = 0; a < xn[r]; ++a)
+            for (int b = 0; b < yn[r]; ++b) {
+              const int64_t cab = ((int64_t)xt[r * kMaxCover + a] * tg.nc[1] +
+                                   yt[r * kMaxCover + b]) * tg.nc[2];
+              const int oab = (xo[r * kMaxCover + a] * tg.W[1] + yo[r * kMaxCover + b]) * tg.W[2];
+              for (int c = 0; c < nz; ++c)
+                s += tl[(cab + zt[z * kMaxCover + c]) * tg.ntile + oab + zo[z * kMaxCover + c]];
+            }
+          return s;
+        },
+        specs + D->spec0, nrows, row0, L, C, D->rad[2]);
+  } else if (D->tg.ok) {
     const TileGeom tg = D->tg;
     const T* tl = tiles + D->tile0;
     r2c_rows<T>([=](int64_t row, int z) { return mesh[row * L + z] + tile_sum<T>(tg, row * L + z, tl); },
-                specs + D->spec0, (int64_t)D->g.k[0] * D->g.k[1], (blockIdx.x - off[sid]) * 2 * C,
-                L, C, D->rad[2]);
+                specs + D->spec0, nrows, row0, L, C, D->rad[2]);
   } else {
-    r2c_rows<T>([=](int64_t row, int z) { return mesh[row * L + z]; }, specs + D->spec0,
-                (int64_t)D->g.k[0] * D->g.k[1], (blockIdx.x - off[sid]) * 2 * C, L, C, D->rad[2]);
+    r2c_rows<T>([=](int64_t row, int z) { return mesh[row * L + z]; }, specs + D->spec0, nrows,
+                row0, L, C, D->rad[2]);
   }
 }
 
