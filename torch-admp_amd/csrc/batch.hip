@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(kBlock) k_b_real(int nsys,
                                                    double* __restrict__ partials) {
   const int64_t c = blockIdx.x;
   const SysDesc* D = tab + find_seg(cell_off, nsys, c);
-  real_cell_body<T>(D->cg, D->cell0, (int)(c - D->cell0), (T)D->kappa, (T)D->eps, ps, sorted,
+  real_cell_body<T>(D->cg, D->cell0, (int)(c - D->cell0), D->kappa, (T)D->eps, ps, sorted,
                     start, pos, grad, partials + c * gridDim.y + blockIdx.y,
                     (int64_t)gridDim.x * gridDim.y);
 }

@@ -18,7 +18,10 @@ constexpr int kRealSplit = 8;     // most candidate splits (work-groups) per hom
 constexpr int64_t kRealTargetBlocks = 512;  // real-space work-groups aimed for (measured best
                                             // at 24k atoms: 2 splits of 343 cells)
 constexpr int kFilterUnroll = 4;  // candidates tested per lane between queue checks
-constexpr int kQueueCap = 64 + 64 * kFilterUnroll;  // per-wavefront pair queue entries
+// per-wavefront pair ring: holds < 64 + 64 * kFilterUnroll entries, and new
+// entries never reach the slots of the 64 being evaluated
+constexpr int kQueueRing = 512;
+static_assert(2 * 64 + 64 * kFilterUnroll <= kQueueRing, "pair ring too small");
 
 // cell c holds the sorted slots [start[c * kSub], start[(c + 1) * kSub])
 struct CellGrid {

@@ -33,88 +33,171 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Staged atom: position relative to the home-cell origin as an unevaluated sum
+// hi + lo of the bulk type (exact to float64 for float), the charge, and a tag
+// in the bits of the fourth hi word (for candidates: the sorted slot of a
+// centre-cell atom, INT_MAX otherwise), so the cutoff filter reads one
+// 16-byte word (32 in float64) per candidate.
+template <typename T>
+struct Staged {
+  typedef T V4 __attribute__((ext_vector_type(4)));
+  V4 hi;  // x, y, z hi parts, tag
+  V4 lo;  // x, y, z lo parts, charge
+};
+
+__device__ __forceinline__ float tag_bits(int v, float) { return __int_as_float(v); }
+__device__ __forceinline__ double tag_bits(int v, double) {
+  return __longlong_as_double((long long)v);
+}
+__device__ __forceinline__ int tag_of(float v) { return __float_as_int(v); }
+__device__ __forceinline__ int tag_of(double v) { return (int)__double_as_longlong(v); }
+
+template <typename T>
+__device__ __forceinline__ Staged<T> stage(double x, double y, double z, double q, int tag) {
+  Staged<T> s;
+  s.hi.x = (T)x;
+  s.hi.y = (T)y;
+  s.hi.z = (T)z;
+  s.hi.w = tag_bits(tag, T(0));
+  s.lo.x = (T)(x - (double)s.hi.x);
+  s.lo.y = (T)(y - (double)s.hi.y);
+  s.lo.z = (T)(z - (double)s.hi.z);
+  s.lo.w = (T)q;
+  return s;
+}
+
+// Screened Coulomb pair term at squared distance r2: e = qq erfc(kappa r) / r and
+// fs = (de/dr) / r.  Constants: kappa, kappa^2 as hi + lo, 2 kappa / sqrt(pi).
+template <typename T>
+struct CoulConst {
+  T kappa, k2h, k2l, c_exp;
+};
+
+__device__ __forceinline__ void coul_pair(double r2, double qq, const CoulConst<double>& C,
+                                          double* e, double* fs) {
+  const double r = sqrt(r2);
+  const double ir = 1.0 / r;
+  const double ec = erfc(C.kappa * r);
+  *e = qq * ec * ir;
+  *fs = -qq * (C.c_exp * exp(-C.k2h * r2) + ec * ir) * ir * ir;
+}
+
+// Float32: erfc(x) = exp(-x^2) t S(t) with t = 1 / (1 + p x), the form of
+// Abramowitz & Stegun 7.1.26, S of degree 10 fitted to erfc(x) exp(x^2) / t for
+// all x >= 0 by weighted least squares (relative error 1e-9 in exact
+// arithmetic).  exp(-kappa^2 r^2) is one hardware exp2 with the rounding of
+// kappa^2 r^2 and of its log2(e) scaling carried as a first-order correction,
+// and serves both the energy and the force.  Relative error of erfc evaluated
+// this way in float32: below 4e-7, mean 7e-8, no bias above 2e-9.
+__device__ __forceinline__ void coul_pair(float r2, float qq, const CoulConst<float>& C, float* e,
+                                          float* fs) {
+  constexpr float kLog2eH = 1.44269502163f, kLog2eL = 1.925963033e-8f, kLn2 = 0.693147182f;
+  const float ir = __builtin_amdgcn_rsqf(r2);
+  const float r = r2 * ir;
+  const float y = C.k2h * r2;
+  const float yl = fmaf(C.k2h, r2, -y) + C.k2l * r2;
+  const float a = y * kLog2eH;
+  const float al = fmaf(y, kLog2eH, -a) + fmaf(y, kLog2eL, yl * kLog2eH);
+  float ex = __builtin_amdgcn_exp2f(-a);
+  ex = fmaf(-ex, al * kLn2, ex);
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.47f, C.kappa * r, 1.0f));
+  float s = 3.685952723e-02f;
+  s = fmaf(s, t, -2.302833945e-01f);
+  s = fmaf(s, t, 5.820118189e-01f);
+  s = fmaf(s, t, -7.192687988e-01f);
+  s = fmaf(s, t, 4.181051552e-01f);
+  s = fmaf(s, t, -1.793078780e-01f);
+  s = fmaf(s, t, 1.588312238e-01f);
+  s = fmaf(s, t, 1.654326916e-01f);
+  s = fmaf(s, t, 2.373880148e-01f);
+  s = fmaf(s, t, 2.650590837e-01f);
+  s = fmaf(s, t, 2.651726007e-01f);
+  const float ec_ir = ex * t * s * ir;  // erfc(kappa r) / r
+  *e = qq * ec_ir;
+  *fs = -qq * fmaf(C.c_exp, ex, ec_ir) * ir * ir;
+}
+
+// Lane i receives lane i - K of its row of 16 (rows do not mix); lanes with no
+// such source get `old`.
+template <int K>
+__device__ __forceinline__ int row_shr(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, 0x110 + K, 0xf, 0xf, false);
+}
+template <int K>
+__device__ __forceinline__ float row_shr(float v) {
+  return __int_as_float(row_shr<K>(0, __float_as_int(v)));
+}
+template <int K>
+__device__ __forceinline__ double row_shr(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = row_shr<K>(0, (int)b), hi = row_shr<K>(0, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// Lane i receives lane i + 1 of its row; the last lane of a row gets `old`.
+__device__ __forceinline__ int row_shl1(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, 0x101, 0xf, 0xf, false);
+}
+
 template <typename T>
 struct PairAcc {
   double e;
   T vxx, vyy, vzz, vxy, vxz, vyz;
 };
 
-// Staged atom: position relative to the home-cell origin as an unevaluated sum
-// hi + lo of the bulk type (exact to float64 for float), plus the charge.
-template <typename T>
-struct Staged {
-  T xh, yh, zh, q, xl, yl, zl, pad;
-};
-
-template <typename T>
-__device__ __forceinline__ Staged<T> stage(double x, double y, double z, double q) {
-  Staged<T> s;
-  s.xh = (T)x;
-  s.yh = (T)y;
-  s.zh = (T)z;
-  s.xl = (T)(x - (double)s.xh);
-  s.yl = (T)(y - (double)s.yh);
-  s.zl = (T)(z - (double)s.zh);
-  s.q = (T)q;
-  s.pad = T(0);
-  return s;
-}
-
 // One unordered pair per lane of a wavefront (`valid` lanes only): adds dE/dr_i
 // to the home accumulator, the energy and the pair virial, and dE/dr_j to the
 // candidate accumulator.  Lanes holding the same candidate form contiguous
-// runs (queue order), so their candidate terms are summed by a segmented
-// wavefront reduction and added by one LDS atomic per run instead of
-// colliding on one address.
+// runs (queue order); within each row of 16 lanes the candidate terms of a run
+// are summed by a segmented inclusive scan over DPP row shifts, and the last
+// lane of each run piece adds the sum with one LDS atomic.
 template <typename T>
 __device__ __forceinline__ void pair_eval(const Staged<T>& pi, const Staged<T>& pj, bool valid,
-                                          int kx, T kappa, T eps, T c_exp, T k2, double* facc_i,
+                                          int kx, T eps, const CoulConst<T>& C, double* facc_i,
                                           double* cacc, PairAcc<T>& acc) {
-  double gx = 0.0, gy = 0.0, gz = 0.0;
+  T gx = T(0), gy = T(0), gz = T(0);
   if (valid) {
     // hi parts of nearby atoms subtract exactly; the lo parts restore float64 accuracy
-    const T dx = (pj.xh - pi.xh) + (pj.xl - pi.xl);
-    const T dy = (pj.yh - pi.yh) + (pj.yl - pi.yl);
-    const T dz = (pj.zh - pi.zh) + (pj.zl - pi.zl);
+    const T dx = (pj.hi.x - pi.hi.x) + (pj.lo.x - pi.lo.x);
+    const T dy = (pj.hi.y - pi.hi.y) + (pj.lo.y - pi.lo.y);
+    const T dz = (pj.hi.z - pi.hi.z) + (pj.lo.z - pi.lo.z);
     const T r2 = dx * dx + dy * dy + dz * dz;
-    const T r = sqrt(r2);
-    const T ir = T(1) / r;
-    const T qq = eps * pi.q * pj.q;
-    const T ec = erfc(kappa * r);
-    const T fs = -qq * (c_exp * exp(-k2 * r2) + ec * ir) * ir * ir;  // (de/dr) / r
-    acc.e += (double)(qq * ec * ir);
-    gx = (double)(fs * dx);
-    gy = (double)(fs * dy);
-    gz = (double)(fs * dz);
-    atomicAdd(facc_i + 0, -gx);
-    atomicAdd(facc_i + 1, -gy);
-    atomicAdd(facc_i + 2, -gz);
-    acc.vxx += fs * dx * dx;
-    acc.vyy += fs * dy * dy;
-    acc.vzz += fs * dz * dz;
-    acc.vxy += fs * dx * dy;
-    acc.vxz += fs * dx * dz;
-    acc.vyz += fs * dy * dz;
+    T e, fs;
+    coul_pair(r2, eps * pi.lo.w * pj.lo.w, C, &e, &fs);
+    acc.e += (double)e;
+    gx = fs * dx;
+    gy = fs * dy;
+    gz = fs * dz;
+    atomicAdd(facc_i + 0, -(double)gx);
+    atomicAdd(facc_i + 1, -(double)gy);
+    atomicAdd(facc_i + 2, -(double)gz);
+    acc.vxx += gx * dx;
+    acc.vyy += gy * dy;
+    acc.vzz += gz * dz;
+    acc.vxy += gx * dy;
+    acc.vxz += gx * dz;
+    acc.vyz += gy * dz;
   }
   const int lane = threadIdx.x & 63;
-  const int key = valid ? kx : -1 - lane;  // invalid lanes form runs of their own
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const double tx = __shfl_down(gx, off, 64);
-    const double ty = __shfl_down(gy, off, 64);
-    const double tz = __shfl_down(gz, off, 64);
-    const int tk = __shfl_down(key, off, 64);
-    if (lane + off < 64 && tk == key) {
-      gx += tx;
-      gy += ty;
-      gz += tz;
-    }
+  const int key = valid ? kx + 1 : -1 - lane;  // invalid lanes form runs of their own; 0 = none
+#define ADMP_SEG_STEP(K)                            \
+  {                                                 \
+    const int tk = row_shr<K>(0, key);              \
+    const T tx = row_shr<K>(gx), ty = row_shr<K>(gy), tz = row_shr<K>(gz); \
+    const bool same = tk == key;                    \
+    gx += same ? tx : T(0);                         \
+    gy += same ? ty : T(0);                         \
+    gz += same ? tz : T(0);                         \
   }
-  const int prev = __shfl_up(key, 1, 64);
-  if (valid && (lane == 0 || prev != key)) {
-    atomicAdd(cacc + 3 * kx, gx);
-    atomicAdd(cacc + 3 * kx + 1, gy);
-    atomicAdd(cacc + 3 * kx + 2, gz);
+  ADMP_SEG_STEP(1)
+  ADMP_SEG_STEP(2)
+  ADMP_SEG_STEP(4)
+  ADMP_SEG_STEP(8)
+#undef ADMP_SEG_STEP
+  const int next = row_shl1(0, key);
+  if (valid && next != key) {
+    atomicAdd(cacc + 3 * kx, (double)gx);
+    atomicAdd(cacc + 3 * kx + 1, (double)gy);
+    atomicAdd(cacc + 3 * kx + 2, (double)gz);
   }
 }
 
@@ -176,7 +259,7 @@ __device__ __forceinline__ int load_stencil_chunk(const CellGrid& cg, int64_t ce
 // sum q^2 and sum q (kRealNV values) go to partial[k * pstride].
 template <typename T>
 __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0, int lc,
-                                               T kappa, T eps,
+                                               double kappa, T eps,
                                                const P4<double>* __restrict__ ps,
                                                const int* __restrict__ sorted,
                                                const int* __restrict__ start,
@@ -184,11 +267,10 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
                                                double* __restrict__ grad,
                                                double* __restrict__ partial, int64_t pstride) {
   __shared__ Staged<T> cand[kCandCap];
-  __shared__ int cslot[kCandCap];
   __shared__ double cacc[kCandCap * 3];
   __shared__ Staged<T> home[kHomeCap];
   __shared__ double facc[kHomeCap * 3];  // float64 LDS atomics in both precisions
-  __shared__ int queue[(kBlock / 64) * kQueueCap];
+  __shared__ int queue[(kBlock / 64) * kQueueRing];
   __shared__ int st_pre[kMaxStencil + 1];
   __shared__ int st_beg[kMaxStencil];
   __shared__ double st_sh[kMaxStencil * 3];
@@ -212,14 +294,18 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
   const int64_t c = cell0 + lc;
   const int hb = start[c * kSub], he = start[(c + 1) * kSub];
   const T rc2 = (T)cg.rc2, rmin2 = (T)1e-4;
-  const T c_exp = (T)(2.0 / kSqrtPi) * kappa, k2 = kappa * kappa;
+  CoulConst<T> C;
+  C.kappa = (T)kappa;
+  C.k2h = (T)(kappa * kappa);
+  C.k2l = (T)(kappa * kappa - (double)C.k2h);
+  C.c_exp = (T)(2.0 / kSqrtPi * kappa);
   PairAcc<T> pacc;
   pacc.e = 0.0;
   pacc.vxx = pacc.vyy = pacc.vzz = pacc.vxy = pacc.vxz = pacc.vyz = T(0);
   double acc[kRealNV];
 #pragma unroll
   for (int k = 0; k < kRealNV; ++k) acc[k] = 0.0;
-  int* q = queue + wave * kQueueCap;
+  int* q = queue + wave * kQueueRing;
   const int nhalf = nst - st_c;
   const bool one_chunk = nhalf <= kMaxStencil;
   int m = 0;
@@ -229,7 +315,7 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
     const int hc = min(kHomeCap, he - h0);
     for (int t = threadIdx.x; t < hc; t += blockDim.x) {
       const P4<double> p = ps[h0 + t];
-      home[t] = stage<T>(p.x - ox, p.y - oy, p.z - oz, p.q);
+      home[t] = stage<T>(p.x - ox, p.y - oy, p.z - oz, p.q, 0);
       facc[3 * t] = facc[3 * t + 1] = facc[3 * t + 2] = 0.0;
     }
     for (int st0 = st_c; st0 < nst; st0 += kMaxStencil) {
@@ -253,10 +339,10 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
           }
           const int slot = st_beg[lo] + (kg - st_pre[lo]);
           const P4<double> p = ps[slot];
-          cand[kk] = stage<T>((p.x + st_sh[lo * 3]) - ox, (p.y + st_sh[lo * 3 + 1]) - oy,
-                              (p.z + st_sh[lo * 3 + 2]) - oz, p.q);
           // centre candidates pair only with home atoms of lower slot
-          cslot[kk] = kg < ncentre ? slot : INT_MAX;
+          cand[kk] = stage<T>((p.x + st_sh[lo * 3]) - ox, (p.y + st_sh[lo * 3 + 1]) - oy,
+                              (p.z + st_sh[lo * 3 + 2]) - oz, p.q,
+                              kg < ncentre ? slot : INT_MAX);
           cacc[3 * kk] = cacc[3 * kk + 1] = cacc[3 * kk + 2] = 0.0;
         }
         __syncthreads();
@@ -269,59 +355,48 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
           const int my_slot = h0 + g0 + hl;
           const int stride = nph * (kBlock / 64);
           const int nit = (cc + stride - 1) / stride;
-          int qn = 0;
+          // per-wavefront ring of (home, candidate) pairs: entries [qh, qt)
+          int qh = 0, qt = 0;
           for (int j0 = 0; j0 < nit; j0 += kFilterUnroll) {
-            bool pred[kFilterUnroll];
+            typename Staged<T>::V4 cw[kFilterUnroll];
             int kks[kFilterUnroll];
 #pragma unroll
             for (int u = 0; u < kFilterUnroll; ++u) {
-              const int kk = ph + nph * wave + stride * (j0 + u);
-              kks[u] = kk;
-              pred[u] = false;
-              if (active && kk < cc) {
-                // cutoff test on the hi parts only (a 1e-6 A band at rc in float32)
-                const Staged<T>& pj = cand[kk];
-                const T dx = pj.xh - pi.xh, dy = pj.yh - pi.yh, dz = pj.zh - pi.zh;
-                const T r2 = dx * dx + dy * dy + dz * dz;
-                pred[u] = r2 < rc2 && r2 > rmin2 && cslot[kk] > my_slot;
-              }
+              kks[u] = ph + nph * wave + stride * (j0 + u);
+              cw[u] = cand[min(kks[u], cc - 1)].hi;
+            }
+            bool pred[kFilterUnroll];
+#pragma unroll
+            for (int u = 0; u < kFilterUnroll; ++u) {
+              // cutoff test on the hi parts only (a 1e-6 A band at rc in float32)
+              const T dx = cw[u].x - pi.hi.x, dy = cw[u].y - pi.hi.y, dz = cw[u].z - pi.hi.z;
+              const T r2 = dx * dx + dy * dy + dz * dz;
+              pred[u] = active & (kks[u] < cc) & (r2 < rc2) & (r2 > rmin2) &
+                        (tag_of(cw[u].w) > my_slot);
             }
 #pragma unroll
             for (int u = 0; u < kFilterUnroll; ++u) {
               const unsigned long long msk = __ballot(pred[u]);
               if (pred[u]) {
-                const int at = qn + __popcll(msk & ((1ull << lane) - 1ull));
-                q[at] = ((g0 + hl) << 16) | kks[u];
+                const int at = qt + __popcll(msk & ((1ull << lane) - 1ull));
+                q[at & (kQueueRing - 1)] = ((g0 + hl) << 16) | kks[u];
               }
-              qn += __popcll(msk);
+              qt += __popcll(msk);
             }
-            while (qn >= 64) {
+            while (qt - qh >= 64) {
               wave_sync();
-              const int ent = q[lane];
+              const int ent = q[(qh + lane) & (kQueueRing - 1)];
               const int hx = ent >> 16, kx = ent & 0xffff;
-              pair_eval<T>(home[hx], cand[kx], true, kx, kappa, eps, c_exp, k2, facc + 3 * hx,
-                           cacc, pacc);
-              const int rem = qn - 64;
-              // move the tail (at most kQueueCap - 64 entries) to the front
-              int tails[kFilterUnroll];
-#pragma unroll
-              for (int u = 0; u < kFilterUnroll; ++u)
-                tails[u] = (64 * u + lane < rem) ? q[64 + 64 * u + lane] : 0;
-              wave_sync();
-#pragma unroll
-              for (int u = 0; u < kFilterUnroll; ++u)
-                if (64 * u + lane < rem) q[64 * u + lane] = tails[u];
-              wave_sync();
-              qn = rem;
+              pair_eval<T>(home[hx], cand[kx], true, kx, eps, C, facc + 3 * hx, cacc, pacc);
+              qh += 64;
             }
           }
           wave_sync();
-          if (qn > 0) {
-            const bool valid = lane < qn;
-            const int ent = valid ? q[lane] : 0;
+          if (qt > qh) {
+            const bool valid = lane < qt - qh;
+            const int ent = valid ? q[(qh + lane) & (kQueueRing - 1)] : 0;
             const int hx = ent >> 16, kx = ent & 0xffff;
-            pair_eval<T>(home[hx], cand[kx], valid, kx, kappa, eps, c_exp, k2, facc + 3 * hx,
-                         cacc, pacc);
+            pair_eval<T>(home[hx], cand[kx], valid, kx, eps, C, facc + 3 * hx, cacc, pacc);
           }
           wave_sync();
         }
