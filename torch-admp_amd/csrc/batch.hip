@@ -13,8 +13,10 @@
 // TorchNeighborList pairs followed by autograd (src/torch_admp/pme.py:261-310,
 // src/torch_admp/utils.py:84-112) when every box height is at least 2 rcut;
 // below that every periodic image inside the cutoff is summed.
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "batch.hpp"
@@ -140,7 +142,7 @@ __global__ void __launch_bounds__(kBlock) k_b_sort_pack(const int* __restrict__ 
 // value-major, rows cell-major (row = cell * splits + split), so that a
 // system's rows are contiguous and the reduction reads them coalesced.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_b_real(int nsys,
+__global__ void __launch_bounds__(kRealBlock) k_b_real(int nsys,
                                                    const int64_t* __restrict__ cell_off,
                                                    const SysDesc* __restrict__ tab,
                                                    const P4<double>* __restrict__ ps,
@@ -151,9 +153,9 @@ __global__ void __launch_bounds__(kBlock) k_b_real(int nsys,
                                                    double* __restrict__ partials) {
   const int64_t c = blockIdx.x;
   const SysDesc* D = tab + find_seg(cell_off, nsys, c);
-  real_cell_body<T>(D->cg, D->cell0, (int)(c - D->cell0), D->kappa, (T)D->eps, ps, sorted,
-                    start, pos, grad, partials + c * gridDim.y + blockIdx.y,
-                    (int64_t)gridDim.x * gridDim.y);
+  real_cell_body<T>(D->cg, D->cell0, (int)(c - D->cell0), (int)blockIdx.y, (int)gridDim.y,
+                    D->kappa, (T)D->eps, ps, sorted, start, pos, grad,
+                    partials + c * gridDim.y + blockIdx.y, (int64_t)gridDim.x * gridDim.y);
 }
 
 // ------------------------------------------------------------------ spread
@@ -584,6 +586,44 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
 
 static void free_host(void* p) { delete[] static_cast<char*>(p); }
 
+// Per device: a side stream (default priority) on which the real-space kernel
+// runs concurrently with the reciprocal-space chain, and a ring of fork/join
+// event pairs, one pair per call (a wait takes the event's latest record at
+// the time it is enqueued, so reusing a pair 64 calls later is safe).  Made at
+// the first call on the device that is not under stream capture; until then,
+// and if making it fails, calls run every kernel on the caller's stream.
+namespace {
+constexpr int kForkRing = 64;
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork[kForkRing] = {}, join[kForkRing] = {};
+  std::atomic<unsigned> next{0};
+  int state = 0;  // 0 not made, 1 ready, -1 failed
+};
+SideStream g_side[64];
+std::mutex g_side_mu;
+
+SideStream* side_stream(hipStream_t caller) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  SideStream& x = g_side[dev];
+  std::lock_guard<std::mutex> g(g_side_mu);
+  if (x.state == 0) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(caller, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+      return nullptr;
+    x.state = -1;
+    if (hipStreamCreateWithFlags(&x.s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    for (int k = 0; k < kForkRing; ++k)
+      if (hipEventCreateWithFlags(&x.fork[k], hipEventDisableTiming) != hipSuccess ||
+          hipEventCreateWithFlags(&x.join[k], hipEventDisableTiming) != hipSuccess)
+        return nullptr;
+    x.state = 1;
+  }
+  return x.state == 1 ? &x : nullptr;
+}
+}  // namespace
+
 // Copy the system table and segment offsets into the workspace.  With
 // `sync` the copy completes before returning (the host buffer is freed at
 // once); otherwise a host function frees it when the stream reaches it.
@@ -662,13 +702,27 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
                                                       (const double*)(nbws + L.w), q,
                                                       (int*)(nbws + L.sorted), ps);
   mark(1);
-  // ---- real space (also for empty batches: its partial rows feed the reductions)
+  // ---- real space (also for empty batches: its partial rows feed the reductions),
+  // forked onto the side stream unless stages are being timed
   double* rpart = (double*)(ws + P.o_rpart);
+  SideStream* side = (P.ctot > 0 && !marks) ? side_stream(s) : nullptr;
+  hipStream_t rs = s;
+  hipEvent_t join = nullptr;
+  if (side) {
+    const unsigned k = side->next.fetch_add(1) % kForkRing;
+    if (hipEventRecord(side->fork[k], s) == hipSuccess &&
+        hipStreamWaitEvent(side->s, side->fork[k], 0) == hipSuccess) {
+      rs = side->s;
+      join = side->join[k];
+    }
+  }
   if (P.ctot > 0) {
     dim3 grid((unsigned)P.ctot, (unsigned)P.nsplit);
-    k_b_real<T><<<grid, kBlock, 0, s>>>(nsys, seg(kSegCell), tab, ps,
-                                        (const int*)(nbws + L.sorted), start, pos, grad, rpart);
+    k_b_real<T><<<grid, kRealBlock, 0, rs>>>(nsys, seg(kSegCell), tab, ps,
+                                             (const int*)(nbws + L.sorted), start, pos, grad,
+                                             rpart);
   }
+  if (join && hipEventRecord(join, rs) != hipSuccess) return ADMP_ERR_HIP;
   mark(2);
   // ---- spread into per-cell tiles (tiled systems) or the mesh (the others)
   if (ntot > 0) {
@@ -690,6 +744,7 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   k_b_cols_y<T, 1><<<by, kBlock, P.lds_fft[1], s>>>(nsys, seg(kSegFftY), tab, spec);
   k_b_c2r<T><<<bz, kBlock, P.lds_fft[0], s>>>(nsys, seg(kSegFftZ), tab, spec, mesh);
   mark(4);
+  if (join && hipStreamWaitEvent(s, join, 0) != hipSuccess) return ADMP_ERR_HIP;
   // ---- gather with the final forces
   double* gpart = (double*)(ws + P.o_gpart);
   if (rows_g > 0)

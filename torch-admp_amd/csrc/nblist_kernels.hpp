@@ -11,12 +11,19 @@ constexpr int kRealNV = 18;               // fused real-space partial values per
 constexpr int kPairBlocksPerFrame = 256;  // fixed grid of the pair-energy reduction
 constexpr int kSub = 8;                   // sub-counters per cell (atomic contention)
 constexpr int64_t kMaxCells = 1 << 21;    // cap on the cell-grid size
-constexpr int kCandCap = 256;     // staged candidates per pass of the cell kernel
+// staged candidates per pass of the cell kernel (32 and 64 bytes each; 512
+// float32 candidates measured no faster)
+template <typename T>
+constexpr int cand_cap() { return 256; }
 constexpr int kHomeCap = 128;     // home atoms per pass of the cell kernel
 constexpr int kMaxStencil = 128;  // stencil entries the cell kernel supports
+constexpr int kRealBlock = 256;  // threads of a real-space work-group (whole wavefronts)
 constexpr int kRealSplit = 8;     // most candidate splits (work-groups) per home cell
-constexpr int64_t kRealTargetBlocks = 512;  // real-space work-groups aimed for (measured best
-                                            // at 24k atoms: 2 splits of 343 cells)
+// real-space work-groups aimed for: at 24k atoms 2 splits of 343 cells fill
+// two thirds of the resident slots and measured fastest; 3 splits (1029
+// groups) spill past the 1024 resident slots, and a persistent grid pulling
+// (cell, split) items from a counter measured 10-50% slower at every item count
+constexpr int64_t kRealTargetBlocks = 512;
 constexpr int kFilterUnroll = 4;  // candidates tested per lane between queue checks
 // per-wavefront pair ring: holds < 64 + 64 * kFilterUnroll entries, and new
 // entries never reach the slots of the 64 being evaluated

@@ -145,14 +145,17 @@ struct PairAcc {
 };
 
 // One unordered pair per lane of a wavefront (`valid` lanes only): adds dE/dr_i
-// to the home accumulator, the energy and the pair virial, and dE/dr_j to the
-// candidate accumulator.  Lanes holding the same candidate form contiguous
-// runs (queue order); within each row of 16 lanes the candidate terms of a run
-// are summed by a segmented inclusive scan over DPP row shifts, and the last
-// lane of each run piece adds the sum with one LDS atomic.
+// to the home accumulator, the energy and the pair virial, and dE/dr_j to
+// candidate kx's accumulator.  `run` identifies the filter ballot the pair came
+// from (home group and candidate); lanes of one run are contiguous in queue
+// order and no run id recurs elsewhere in the wavefront, so within each row of
+// 16 lanes the candidate terms of a run are summed by a segmented inclusive
+// scan over DPP row shifts, and the last lane of each run piece adds the sum
+// with one LDS atomic.
 template <typename T>
 __device__ __forceinline__ void pair_eval(const Staged<T>& pi, const Staged<T>& pj, bool valid,
-                                          int kx, T eps, const CoulConst<T>& C, double* facc_i,
+                                          int kx, int run, T eps, const CoulConst<T>& C,
+                                          double* facc_i,
                                           double* cacc, PairAcc<T>& acc) {
   T gx = T(0), gy = T(0), gz = T(0);
   if (valid) {
@@ -178,7 +181,7 @@ __device__ __forceinline__ void pair_eval(const Staged<T>& pi, const Staged<T>& 
     acc.vyz += gy * dz;
   }
   const int lane = threadIdx.x & 63;
-  const int key = valid ? kx + 1 : -1 - lane;  // invalid lanes form runs of their own; 0 = none
+  const int key = valid ? run + 1 : -1 - lane;  // invalid lanes form runs of their own; 0 = none
 #define ADMP_SEG_STEP(K)                            \
   {                                                 \
     const int tk = row_shr<K>(0, key);              \
@@ -249,7 +252,7 @@ __device__ __forceinline__ int load_stencil_chunk(const CellGrid& cg, int64_t ce
 // pair once (Newton's third law): home cell lc of a grid whose cells start at
 // global index cell0, against the half stencil -- the entries after the centre
 // in lexicographic offset order, plus the centre itself with slot j > slot i --
-// split over gridDim.y work-groups (candidate k goes to split k mod S).
+// split S ways (candidate k goes to split k mod S).
 // Candidates are staged in LDS as hi/lo pairs relative to the home-cell
 // origin; lanes of a wavefront filter (home, candidate) pairs against the
 // cutoff and compact the survivors into a per-wavefront queue evaluated 64
@@ -259,24 +262,24 @@ __device__ __forceinline__ int load_stencil_chunk(const CellGrid& cg, int64_t ce
 // sum q^2 and sum q (kRealNV values) go to partial[k * pstride].
 template <typename T>
 __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0, int lc,
-                                               double kappa, T eps,
+                                               int split, int S, double kappa, T eps,
                                                const P4<double>* __restrict__ ps,
                                                const int* __restrict__ sorted,
                                                const int* __restrict__ start,
                                                const double* __restrict__ pos,
                                                double* __restrict__ grad,
                                                double* __restrict__ partial, int64_t pstride) {
+  constexpr int kCandCap = cand_cap<T>();
   __shared__ Staged<T> cand[kCandCap];
   __shared__ double cacc[kCandCap * 3];
   __shared__ Staged<T> home[kHomeCap];
   __shared__ double facc[kHomeCap * 3];  // float64 LDS atomics in both precisions
-  __shared__ int queue[(kBlock / 64) * kQueueRing];
+  __shared__ int queue[(kRealBlock / 64) * kQueueRing];
   __shared__ int st_pre[kMaxStencil + 1];
   __shared__ int st_beg[kMaxStencil];
   __shared__ double st_sh[kMaxStencil * 3];
-  __shared__ double red[kRealNV * (kBlock / 64)];
+  __shared__ double red[kRealNV * (kRealBlock / 64)];
 
-  const int split = blockIdx.y, S = gridDim.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ci[3] = {lc / (cg.nc[1] * cg.nc[2]), (lc / cg.nc[2]) % cg.nc[1], lc % cg.nc[2]};
   const int l0 = stencil_len(cg, 0), l1 = stencil_len(cg, 1), l2 = stencil_len(cg, 2);
@@ -346,6 +349,9 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
           cacc[3 * kk] = cacc[3 * kk + 1] = cacc[3 * kk + 2] = 0.0;
         }
         __syncthreads();
+        // per-wavefront ring of (home, candidate) pairs, entries [qh, qt); it runs
+        // on across home groups and is drained once per candidate chunk
+        int qh = 0, qt = 0;
         for (int g0 = 0; g0 < hc; g0 += 64) {
           const int hg = min(64, hc - g0);
           const int nph = 64 / hg;
@@ -353,10 +359,8 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
           const bool active = ph < nph;
           const Staged<T> pi = home[g0 + hl];
           const int my_slot = h0 + g0 + hl;
-          const int stride = nph * (kBlock / 64);
+          const int stride = nph * (kRealBlock / 64);
           const int nit = (cc + stride - 1) / stride;
-          // per-wavefront ring of (home, candidate) pairs: entries [qh, qt)
-          int qh = 0, qt = 0;
           for (int j0 = 0; j0 < nit; j0 += kFilterUnroll) {
             typename Staged<T>::V4 cw[kFilterUnroll];
             int kks[kFilterUnroll];
@@ -387,19 +391,21 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
               wave_sync();
               const int ent = q[(qh + lane) & (kQueueRing - 1)];
               const int hx = ent >> 16, kx = ent & 0xffff;
-              pair_eval<T>(home[hx], cand[kx], true, kx, eps, C, facc + 3 * hx, cacc, pacc);
+              pair_eval<T>(home[hx], cand[kx], true, kx, ((hx >> 6) << 16) | kx, eps, C,
+                           facc + 3 * hx, cacc, pacc);
               qh += 64;
             }
           }
-          wave_sync();
-          if (qt > qh) {
-            const bool valid = lane < qt - qh;
-            const int ent = valid ? q[(qh + lane) & (kQueueRing - 1)] : 0;
-            const int hx = ent >> 16, kx = ent & 0xffff;
-            pair_eval<T>(home[hx], cand[kx], valid, kx, eps, C, facc + 3 * hx, cacc, pacc);
-          }
-          wave_sync();
         }
+        wave_sync();
+        if (qt > qh) {
+          const bool valid = lane < qt - qh;
+          const int ent = valid ? q[(qh + lane) & (kQueueRing - 1)] : 0;
+          const int hx = ent >> 16, kx = ent & 0xffff;
+          pair_eval<T>(home[hx], cand[kx], valid, kx, ((hx >> 6) << 16) | kx, eps, C,
+                       facc + 3 * hx, cacc, pacc);
+        }
+        wave_sync();
         __syncthreads();
         // candidate gradients of this chunk
         for (int kk = threadIdx.x; kk < cc; kk += blockDim.x) {
