@@ -216,15 +216,4 @@ __device__ inline void finalize_system(const double* rr, const double* rk, const
     }
 }
 
-// out[i] = -(sum over k < nrows of g[k * m + i])
-__global__ inline void k_neg_sum_rows(const double* __restrict__ g, int nrows, int64_t m,
-                                      double* __restrict__ out) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int k = 0; k < nrows; ++k) s += g[k * m + i];
-    out[i] = -s;
-  }
-}
-
 }  // namespace admp

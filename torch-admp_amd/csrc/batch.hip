@@ -168,13 +168,13 @@ __global__ void __launch_bounds__(kBlock) k_b_spread_tiles(int nsys,
                                                            const double* __restrict__ qv,
                                                            const int* __restrict__ sorted,
                                                            const int* __restrict__ start,
-                                                           T* __restrict__ tiles, T* __restrict__ mesh) {
+                                                           T* __restrict__ mesh) {
   const int64_t c = blockIdx.x;
   const SysDesc* D = tab + find_seg(cell_off, nsys, c);
   if (!D->tg.ok) return;
   const int lc = (int)(c - D->cell0);
   spread_cell_tile<T>(D->g, D->tg, lc, start[c * kSub], start[(c + 1) * kSub], sorted, pos, qv,
-                      mesh + D->mesh0, tiles + D->tile0 + (int64_t)lc * D->tg.ntile);
+                      mesh + D->mesh0);
 }
 
 // atoms of systems whose tiles do not fit: global atomics into their mesh
@@ -195,87 +195,19 @@ __global__ void __launch_bounds__(kBlock) k_b_spread_atoms(int64_t ntot,
 }
 
 // ------------------------------------------------------------------ FFT passes
-// r2c along z of mesh + (for tiled systems) the sum of the cell tiles that
-// cover each point, so the tiles are reduced on the way into the FFT.  The
-// (tile, offset) covers of the block's z indices and of its rows' x and y
-// indices are tabulated in LDS once, so each point costs only its few tile
-// reads.
-constexpr int kCovMaxL = 256;   // longest z line with tabulated covers
-constexpr int kCovMaxRows = 32;  // rows of one r2c block (2 x 16 lines)
-
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_b_r2c(int nsys, const int64_t* __restrict__ off,
                                                   const SysDesc* __restrict__ tab,
                                                   const T* __restrict__ meshes,
-                                                  const T* __restrict__ tiles,
                                                   cplx<T>* __restrict__ specs) {
-  __shared__ short zt[kCovMaxL * kMaxCover], zo[kCovMaxL * kMaxCover];
-  __shared__ unsigned char zn[kCovMaxL];
-  __shared__ short xt[kCovMaxRows * kMaxCover], xo[kCovMaxRows * kMaxCover];
-  __shared__ short yt[kCovMaxRows * kMaxCover], yo[kCovMaxRows * kMaxCover];
-  __shared__ unsigned char xn[kCovMaxRows], yn[kCovMaxRows];
   const int sid = find_seg(off, nsys, blockIdx.x);
   const SysDesc* D = tab + sid;
   const int C = D->fchunk[0];
   const int L = D->g.k[2];
-  const int K1 = D->g.k[1];
   const T* mesh = meshes + D->mesh0;
-  const int64_t nrows = (int64_t)D->g.k[0] * K1;
-  const int64_t row0 = (blockIdx.x - off[sid]) * 2 * C;
-  if (D->tg.ok && L <= kCovMaxL && 2 * C <= kCovMaxRows) {
-    const TileGeom tg = D->tg;
-    for (int z = threadIdx.x; z < L; z += blockDim.x) {
-      int t[kMaxCover], o[kMaxCover];
-      const int n = tile_cover(z, tg.K[2], tg.nc[2], tg.W[2], t, o);
-      zn[z] = (unsigned char)n;
-      for (int k = 0; k < n; ++k) {
-        zt[z * kMaxCover + k] = (short)t[k];
-        zo[z * kMaxCover + k] = (short)o[k];
-      }
-    }
-    for (int r = threadIdx.x; r < 2 * C; r += blockDim.x) {
-      const int64_t row = min(row0 + r, nrows - 1);
-      int t[kMaxCover], o[kMaxCover];
-      int n = tile_cover((int)(row / K1), tg.K[0], tg.nc[0], tg.W[0], t, o);
-      xn[r] = (unsigned char)n;
-      for (int k = 0; k < n; ++k) {
-        xt[r * kMaxCover + k] = (short)t[k];
-        xo[r * kMaxCover + k] = (short)o[k];
-      }
-      n = tile_cover((int)(row % K1), tg.K[1], tg.nc[1], tg.W[1], t, o);
-      yn[r] = (unsigned char)n;
-      for (int k = 0; k < n; ++k) {
-        yt[r * kMaxCover + k] = (short)t[k];
-        yo[r * kMaxCover + k] = (short)o[k];
-      }
-    }
-    __syncthreads();
-    const T* tl = tiles + D->tile0;
-    r2c_rows<T>(
-        [=](int64_t row, int z) {
-          const int r = (int)(row - row0);
-          T s = mesh[row * L + z];
-          const int nz = zn[z];
-          for (int a = 0; a < xn[r]; ++a)
-            for (int b = 0; b < yn[r]; ++b) {
-              const int64_t cab = ((int64_t)xt[r * kMaxCover + a] * tg.nc[1] +
-                                   yt[r * kMaxCover + b]) * tg.nc[2];
-              const int oab = (xo[r * kMaxCover + a] * tg.W[1] + yo[r * kMaxCover + b]) * tg.W[2];
-              for (int c = 0; c < nz; ++c)
-                s += tl[(cab + zt[z * kMaxCover + c]) * tg.ntile + oab + zo[z * kMaxCover + c]];
-            }
-          return s;
-        },
-        specs + D->spec0, nrows, row0, L, C, D->rad[2]);
-  } else if (D->tg.ok) {
-    const TileGeom tg = D->tg;
-    const T* tl = tiles + D->tile0;
-    r2c_rows<T>([=](int64_t row, int z) { return mesh[row * L + z] + tile_sum<T>(tg, row * L + z, tl); },
-                specs + D->spec0, nrows, row0, L, C, D->rad[2]);
-  } else {
-    r2c_rows<T>([=](int64_t row, int z) { return mesh[row * L + z]; }, specs + D->spec0, nrows,
-                row0, L, C, D->rad[2]);
-  }
+  const int64_t nrows = (int64_t)D->g.k[0] * D->g.k[1];
+  r2c_rows<T>([=](int64_t row, int z) { return mesh[row * L + z]; }, specs + D->spec0, nrows,
+              (blockIdx.x - off[sid]) * 2 * C, L, C, D->rad[2], D->plane != 0);
 }
 
 template <typename T>
@@ -287,7 +219,7 @@ __global__ void __launch_bounds__(kBlock) k_b_c2r(int nsys, const int64_t* __res
   const SysDesc* D = tab + sid;
   const int C = D->fchunk[0];
   c2r_rows<T>(specs + D->spec0, meshes + D->mesh0, (int64_t)D->g.k[0] * D->g.k[1],
-              (blockIdx.x - off[sid]) * 2 * C, D->g.k[2], C, D->rad[2]);
+              (blockIdx.x - off[sid]) * 2 * C, D->g.k[2], C, D->rad[2], D->plane != 0);
 }
 
 // lines along y: element (x, y, kz) at (x * K1 + y) * Kzh + kz
@@ -314,6 +246,7 @@ __global__ void __launch_bounds__(kBlock) k_b_cols_x(int nsys, const int64_t* __
                                                      double* __restrict__ partials, int64_t nrows) {
   const int sid = find_seg(off, nsys, blockIdx.x);
   const SysDesc* D = tab + sid;
+  if (D->plane) return;
   InflParams P;
   P.g = D->g;
   P.kappa = D->kappa;
@@ -327,6 +260,32 @@ __global__ void __launch_bounds__(kBlock) k_b_cols_x(int nsys, const int64_t* __
   __shared__ double vals[10];
   x_convolve_lines<T>(specs + D->spec0, (int)(lb / nchunk), (int)(lb % nchunk) * C, C, D->rad[0],
                       P, vals);
+  __syncthreads();
+  if (threadIdx.x < 10) partials[(int64_t)threadIdx.x * nrows + blockIdx.x] = vals[threadIdx.x] * P.eps;
+}
+
+// one kz plane per work-group (systems with plane set; same partial rows as
+// k_b_cols_x, whose segment the planes share)
+template <typename T>
+__global__ void __launch_bounds__(kPlaneThreads) k_b_plane(int nsys,
+                                                         const int64_t* __restrict__ off,
+                                                         const SysDesc* __restrict__ tab,
+                                                         cplx<T>* __restrict__ specs,
+                                                         double* __restrict__ partials,
+                                                         int64_t nrows) {
+  const int sid = find_seg(off, nsys, blockIdx.x);
+  const SysDesc* D = tab + sid;
+  if (!D->plane) return;
+  InflParams P;
+  P.g = D->g;
+  P.kappa = D->kappa;
+  P.phi_scale = 1.0;
+  P.eps = D->eps;
+  P.conv = D->conv;
+  P.ortho = D->ortho;
+  __shared__ double vals[10];
+  plane_convolve<T>(specs + D->spec0, (int)(blockIdx.x - off[sid]), D->rad[0], D->rad[1], P,
+                    vals);
   __syncthreads();
   if (threadIdx.x < 10) partials[(int64_t)threadIdx.x * nrows + blockIdx.x] = vals[threadIdx.x] * P.eps;
 }
@@ -487,6 +446,24 @@ static int real_splits(int64_t ctot) {
   return (int)std::min<int64_t>(std::max<int64_t>(s, 1), kRealSplit);
 }
 
+// The y and x passes of a K0 x K1 plane fit one work-group: the plane within
+// 64 KiB of LDS and kPlaneLoad elements per thread, radices the in-place
+// passes have, and every pass within kPlaneB butterflies per thread.
+static bool plane_fits(int K0, int K1, const Radix& rx, const Radix& ry, size_t lds) {
+  if (lds > 65536 || !radix_ok(rx) || !radix_ok(ry)) return false;
+  if ((int64_t)K0 * K1 > (int64_t)kPlaneLoad * kPlaneThreads || 2 * (K0 + K1) > kPlaneThreads)
+    return false;
+  auto ok = [](const Radix& rd, int L, int C) {
+    for (int k = 0; k < rd.n; ++k) {
+      const int r = rd.r[k];
+      if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8) return false;
+      if ((int64_t)(L / r) * C > (int64_t)kPlaneB * kPlaneThreads) return false;
+    }
+    return true;
+  };
+  return ok(rx, K0, K1) && ok(ry, K1, K0);
+}
+
 BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, const double* cutoffs,
                      const double* kappas, const int32_t* kmesh, int precision, int convention,
                      double eps) {
@@ -500,8 +477,10 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
   const size_t ces = 2 * es;
   P.lds_fft[0] = P.lds_fft[1] = P.lds_fft[2] = 0;
   P.lds_tile = 0;
+  P.lds_plane = 0;
+  P.any_plane = P.any_cols = 0;
   P.any_untiled = 0;
-  int64_t spec = 0, tile = 0;
+  int64_t spec = 0;
   for (int s = 0; s < nsys; ++s) {
     SysDesc& D = P.sys[s];
     std::memset(&D, 0, sizeof(D));
@@ -534,19 +513,27 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
     D.ncell = D.cg.ncell;
     D.mesh0 = P.off[kSegMesh][s];
     D.spec0 = spec;
-    D.tile0 = tile;
     spec += (int64_t)K0 * K1 * Kzh;
-    if (D.tg.ok) tile += (int64_t)D.ncell * D.tg.ntile;
     P.off[kSegAtom][s + 1] = D.atom0 + D.n;
     P.off[kSegCell][s + 1] = D.cell0 + D.ncell;
     P.off[kSegMesh][s + 1] = D.mesh0 + (int64_t)K0 * K1 * K2;
     P.off[kSegFftZ][s + 1] = P.off[kSegFftZ][s] + ((int64_t)K0 * K1 + 2 * D.fchunk[0] - 1) / (2 * D.fchunk[0]);
-    P.off[kSegFftY][s + 1] = P.off[kSegFftY][s] + (int64_t)K0 * ((Kzh + D.fchunk[1] - 1) / D.fchunk[1]);
-    P.off[kSegFftX][s + 1] = P.off[kSegFftX][s] + (int64_t)K1 * ((Kzh + D.fchunk[2] - 1) / D.fchunk[2]);
+    const size_t lds_plane = plane_lds_bytes(K0, K1, ces);
+    D.plane = plane_fits(K0, K1, D.rad[0], D.rad[1], lds_plane) ? 1 : 0;
+    P.off[kSegFftY][s + 1] =
+        P.off[kSegFftY][s] + (D.plane ? 0 : (int64_t)K0 * ((Kzh + D.fchunk[1] - 1) / D.fchunk[1]));
+    P.off[kSegFftX][s + 1] =
+        P.off[kSegFftX][s] + (D.plane ? Kzh : (int64_t)K1 * ((Kzh + D.fchunk[2] - 1) / D.fchunk[2]));
     P.off[kSegGather][s + 1] = P.off[kSegGather][s] + (D.n + kGatherAtomsPerBlock - 1) / kGatherAtomsPerBlock;
     P.lds_fft[0] = std::max(P.lds_fft[0], (size_t)(K2 + 2 * K2 * D.fchunk[0]) * ces);
-    P.lds_fft[1] = std::max(P.lds_fft[1], (size_t)(K1 + 2 * K1 * D.fchunk[1]) * ces);
-    P.lds_fft[2] = std::max(P.lds_fft[2], (size_t)(K0 + 2 * K0 * D.fchunk[2]) * ces);
+    if (D.plane) {
+      P.lds_plane = std::max(P.lds_plane, lds_plane);
+      P.any_plane = 1;
+    } else {
+      P.lds_fft[1] = std::max(P.lds_fft[1], (size_t)(K1 + 2 * K1 * D.fchunk[1]) * ces);
+      P.lds_fft[2] = std::max(P.lds_fft[2], (size_t)(K0 + 2 * K0 * D.fchunk[2]) * ces);
+      P.any_cols = 1;
+    }
     if (D.tg.ok) P.lds_tile = std::max(P.lds_tile, (size_t)D.tg.ntile * sizeof(double));
     else P.any_untiled = 1;
   }
@@ -554,7 +541,6 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
   P.ctot = P.off[kSegCell][nsys];
   P.mtot = P.off[kSegMesh][nsys];
   P.stot = spec;
-  P.ttot = tile;
   std::memset(&P.cg_all, 0, sizeof(P.cg_all));
   P.cg_all.ncell = (int)std::min<int64_t>(P.ctot, INT32_MAX);
   nb_layout(P.ntot, P.cg_all, &P.nb);
@@ -573,7 +559,6 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
   P.o_grad = take(sizeof(double) * 3 * std::max<int64_t>(P.ntot, 1));
   P.o_mesh = take(es * std::max<int64_t>(P.mtot, 1));
   P.o_spec = take(ces * std::max<int64_t>(P.stot, 1));
-  P.o_tiles = take(es * std::max<int64_t>(P.ttot, 1));
   P.nsplit = real_splits(P.ctot);
   P.o_rpart = take(sizeof(double) * kRealNV * P.nsplit * std::max<int64_t>(P.ctot, 1));
   P.o_cpart = take(sizeof(double) * 10 * std::max<int64_t>(rows_x, 1));
@@ -679,7 +664,6 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   int* start = (int*)(nbws + L.start);
   double* grad = (double*)(ws + P.o_grad);
   T* mesh = (T*)(ws + P.o_mesh);
-  T* tiles = (T*)(ws + P.o_tiles);
   CT* spec = (CT*)(ws + P.o_spec);
   P4<double>* ps = (P4<double>*)(ws + P.o_ps);
   int* tickets = (int*)(ws + P.o_tickets);
@@ -724,24 +708,30 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   }
   if (join && hipEventRecord(join, rs) != hipSuccess) return ADMP_ERR_HIP;
   mark(2);
-  // ---- spread into per-cell tiles (tiled systems) or the mesh (the others)
+  // ---- spread: LDS tiles added to the mesh (tiled systems) or per-atom atomics (the others)
   if (ntot > 0) {
     if (P.lds_tile > 0)
       k_b_spread_tiles<T><<<(unsigned)P.ctot, kBlock, P.lds_tile, s>>>(
-          nsys, seg(kSegCell), tab, pos, q, (const int*)(nbws + L.sorted), start, tiles, mesh);
+          nsys, seg(kSegCell), tab, pos, q, (const int*)(nbws + L.sorted), start, mesh);
     if (P.any_untiled)
       k_b_spread_atoms<T><<<(unsigned)((ntot * 8 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
           ntot, (const int*)(ws + P.o_sys_of), tab, pos, q, mesh);
   }
   mark(3);
-  // ---- tile reduction + FFT, influence function, inverse FFT
+  // ---- FFT, influence function, inverse FFT
   const unsigned bz = (unsigned)P.off[kSegFftZ][nsys], by = (unsigned)P.off[kSegFftY][nsys],
                  bx = (unsigned)rows_x;
   double* cpart = (double*)(ws + P.o_cpart);
-  k_b_r2c<T><<<bz, kBlock, P.lds_fft[0], s>>>(nsys, seg(kSegFftZ), tab, mesh, tiles, spec);
-  k_b_cols_y<T, -1><<<by, kBlock, P.lds_fft[1], s>>>(nsys, seg(kSegFftY), tab, spec);
-  k_b_cols_x<T><<<bx, kBlock, P.lds_fft[2], s>>>(nsys, seg(kSegFftX), tab, spec, cpart, rows_x);
-  k_b_cols_y<T, 1><<<by, kBlock, P.lds_fft[1], s>>>(nsys, seg(kSegFftY), tab, spec);
+  k_b_r2c<T><<<bz, kBlock, P.lds_fft[0], s>>>(nsys, seg(kSegFftZ), tab, mesh, spec);
+  if (by > 0)
+    k_b_cols_y<T, -1><<<by, kBlock, P.lds_fft[1], s>>>(nsys, seg(kSegFftY), tab, spec);
+  if (P.any_cols)
+    k_b_cols_x<T><<<bx, kBlock, P.lds_fft[2], s>>>(nsys, seg(kSegFftX), tab, spec, cpart, rows_x);
+  if (P.any_plane)
+    k_b_plane<T><<<bx, kPlaneThreads, P.lds_plane, s>>>(nsys, seg(kSegFftX), tab, spec, cpart,
+                                                        rows_x);
+  if (by > 0)
+    k_b_cols_y<T, 1><<<by, kBlock, P.lds_fft[1], s>>>(nsys, seg(kSegFftY), tab, spec);
   k_b_c2r<T><<<bz, kBlock, P.lds_fft[0], s>>>(nsys, seg(kSegFftZ), tab, spec, mesh);
   mark(4);
   if (join && hipStreamWaitEvent(s, join, 0) != hipSuccess) return ADMP_ERR_HIP;

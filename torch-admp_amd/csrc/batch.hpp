@@ -23,9 +23,11 @@ struct SysDesc {
   double kappa, eps;
   int conv, ortho;
   int fchunk[3];  // lines per block: z passes (row pairs), y passes, x pass
+  int plane;      // the y and x passes run as one LDS-resident plane per kz
+                  // (the half spectrum is then stored kz-major)
   int64_t atom0, n;
   int64_t cell0, ncell;
-  int64_t mesh0, spec0, tile0;
+  int64_t mesh0, spec0;
 };
 
 // Segment offsets (n_sys + 1 each) used to map a global index to its system.
@@ -36,15 +38,17 @@ struct BatchPlan {
   int precision;
   std::vector<SysDesc> sys;
   std::vector<int64_t> off[kSegCount];
-  int64_t ntot, ctot, mtot, stot, ttot;
+  int64_t ntot, ctot, mtot, stot;
   size_t lds_fft[3];   // dynamic LDS of the z, y and x passes
   size_t lds_tile;
+  size_t lds_plane;    // dynamic LDS of the plane pass
+  int any_plane, any_cols;  // some system takes the plane pass / the line passes
   int any_untiled;     // some system spreads with global atomics
   int nsplit;          // real-space candidate splits per cell
   NbLayout nb;         // binning workspace over all cells
   CellGrid cg_all;     // only ncell is used by the shared binning helpers
   // workspace byte offsets
-  size_t o_tab, o_off, o_nb, o_sys_of, o_ps, o_grad, o_mesh, o_spec, o_tiles, o_rpart, o_cpart,
+  size_t o_tab, o_off, o_nb, o_sys_of, o_ps, o_grad, o_mesh, o_spec, o_rpart, o_cpart,
       o_gpart, o_red, o_tickets, bytes;
   int status;          // ADMP_OK or an error code from planning
 };

@@ -81,7 +81,7 @@ struct StageMarks {
   hipStream_t s;
   hipEvent_t* ev = nullptr;
   void mark(int k) {
-    if (ev) hipEventRecord(ev[k], s);
+    if (ev) (void)hipEventRecord(ev[k], s);  // a failed mark shows as a failed elapsed time
   }
 };
 
@@ -151,7 +151,7 @@ int admp_profile_begin(int32_t max_calls) {
   if (max_calls < 1 || max_calls > 100000) return ADMP_ERR_INVALID;
   if (g_prof.ev) {
     for (int i = 0; i < g_prof.max_calls * (ADMP_PROFILE_STAGES + 1); ++i)
-      hipEventDestroy(g_prof.ev[i]);
+      (void)hipEventDestroy(g_prof.ev[i]);
     delete[] g_prof.ev;
   }
   const int nev = max_calls * (ADMP_PROFILE_STAGES + 1);
@@ -182,7 +182,7 @@ int admp_profile_collect(double* stage_ms, int32_t* n_calls) {
   }
   *n_calls = g_prof.calls;
   for (int i = 0; i < g_prof.max_calls * (ADMP_PROFILE_STAGES + 1); ++i)
-    hipEventDestroy(g_prof.ev[i]);
+    (void)hipEventDestroy(g_prof.ev[i]);
   delete[] g_prof.ev;
   g_prof.ev = nullptr;
   g_prof.calls = g_prof.max_calls = 0;

@@ -102,6 +102,19 @@ __device__ __forceinline__ cplx<T> tw_get(const cplx<T>* tw, int t, int sign) {
   return w;
 }
 
+// q = x / d, r = x mod d (x >= 0, d > 0 and the same in every lane): shifts
+// when d is a power of two (the usual mesh sizes), else a division.
+__device__ __forceinline__ void divmod_u(int x, int d, int& q, int& r) {
+  if ((d & (d - 1)) == 0) {
+    const int sh = __ffs(d) - 1;
+    q = x >> sh;
+    r = x & (d - 1);
+  } else {
+    q = x / d;
+    r = x - q * d;
+  }
+}
+
 // One Stockham pass with a radix known at compile time (registers only).
 template <typename T, int SIGN, int R>
 __device__ __forceinline__ void pass_r(const cplx<T>* __restrict__ a, cplx<T>* __restrict__ b,
@@ -110,14 +123,15 @@ __device__ __forceinline__ void pass_r(const cplx<T>* __restrict__ a, cplx<T>* _
   const int nb = L / R;
   const int tstep = L / (p * R);
   for (int idx = threadIdx.x; idx < nb * C; idx += blockDim.x) {
-    const int c = idx % C;
-    const int i = idx / C;
-    const int k = i % p;
+    int c, i, k, ip;
+    divmod_u(idx, C, i, c);
+    divmod_u(i, p, ip, k);
     CT u[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       CT v = a[(i + j * nb) * C + c];
-      if (j > 0 && k > 0) v = cmul(v, tw_get<T>(tw, (k * j * tstep) % L, SIGN));
+      // k j tstep <= (p - 1)(R - 1) L / (p R) < L
+      if (j > 0 && k > 0) v = cmul(v, tw_get<T>(tw, k * j * tstep, SIGN));
       u[j] = v;
     }
     dft_r<T, SIGN, R>(u);
@@ -264,9 +278,12 @@ __device__ __forceinline__ void add_dEdA(double acc[10], const InflParams& P, in
 // 2c+1 are packed as the real and imaginary parts of one complex line; after
 // the FFT A_k = (Z_k + conj Z_{L-k}) / 2 and B_k = (Z_k - conj Z_{L-k}) / (2i).
 // load(row, z) supplies the real input of row `row`, element z.
+// kz_major: store element (row, k) at out[k * nrows + row] (planes of one k
+// contiguous) instead of out[row * (L / 2 + 1) + k].
 template <typename T, typename Load>
 __device__ __forceinline__ void r2c_rows(Load load, cplx<T>* __restrict__ out, int64_t nrows,
-                                         int64_t row0, int L, int C, const Radix& rd) {
+                                         int64_t row0, int L, int C, const Radix& rd,
+                                         bool kz_major = false) {
   using CT = cplx<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   CT* tw = reinterpret_cast<CT*>(smem_raw);
@@ -274,7 +291,8 @@ __device__ __forceinline__ void r2c_rows(Load load, cplx<T>* __restrict__ out, i
   CT* b = a + (size_t)L * C;
   build_twiddles<T>(tw, L);
   for (int idx = threadIdx.x; idx < 2 * C * L; idx += blockDim.x) {
-    const int rl = idx / L, z = idx % L;
+    int rl, z;
+    divmod_u(idx, L, rl, z);
     const int64_t row = row0 + rl;
     const T v = row < nrows ? load(row, z) : T(0);
     T* dst = reinterpret_cast<T*>(&a[z * C + (rl >> 1)]);
@@ -284,11 +302,14 @@ __device__ __forceinline__ void r2c_rows(Load load, cplx<T>* __restrict__ out, i
   CT* res = stockham<T, -1>(a, b, L, C, rd, tw);
   const int Lh = L / 2 + 1;
   for (int idx = threadIdx.x; idx < 2 * C * Lh; idx += blockDim.x) {
-    const int rl = idx / Lh, k = idx % Lh;
+    // consecutive threads take consecutive k (row-major store) or rows (kz-major)
+    int rl, k;
+    if (kz_major) divmod_u(idx, 2 * C, k, rl);
+    else divmod_u(idx, Lh, rl, k);
     const int64_t row = row0 + rl;
     if (row >= nrows) continue;
     const CT zk = res[k * C + (rl >> 1)];
-    const CT zm = res[((L - k) % L) * C + (rl >> 1)];
+    const CT zm = res[(k ? L - k : 0) * C + (rl >> 1)];
     CT o;
     if ((rl & 1) == 0) {
       o.x = T(0.5) * (zk.x + zm.x);
@@ -297,7 +318,7 @@ __device__ __forceinline__ void r2c_rows(Load load, cplx<T>* __restrict__ out, i
       o.x = T(0.5) * (zk.y + zm.y);
       o.y = T(-0.5) * (zk.x - zm.x);
     }
-    out[row * Lh + k] = o;
+    out[kz_major ? (int64_t)k * nrows + row : row * Lh + k] = o;
   }
 }
 
@@ -307,7 +328,7 @@ __device__ __forceinline__ void r2c_rows(Load load, cplx<T>* __restrict__ out, i
 template <typename T>
 __device__ __forceinline__ void c2r_rows(const cplx<T>* __restrict__ in, T* __restrict__ out,
                                          int64_t nrows, int64_t row0, int L, int C,
-                                         const Radix& rd) {
+                                         const Radix& rd, bool kz_major = false) {
   using CT = cplx<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   CT* tw = reinterpret_cast<CT*>(smem_raw);
@@ -316,13 +337,15 @@ __device__ __forceinline__ void c2r_rows(const cplx<T>* __restrict__ in, T* __re
   build_twiddles<T>(tw, L);
   const int Lh = L / 2 + 1;
   for (int idx = threadIdx.x; idx < C * L; idx += blockDim.x) {
-    const int c = idx / L, k = idx % L;
+    int c, k;
+    if (kz_major) divmod_u(idx, C, k, c);
+    else divmod_u(idx, L, c, k);
     const int64_t ra = row0 + 2 * c, rb = ra + 1;
     const int kk = k < Lh ? k : L - k;
     const T sg = k < Lh ? T(1) : T(-1);
     CT A = cmk<CT>(0, 0), B = cmk<CT>(0, 0);
-    if (ra < nrows) A = in[ra * Lh + kk];
-    if (rb < nrows) B = in[rb * Lh + kk];
+    if (ra < nrows) A = in[kz_major ? (int64_t)kk * nrows + ra : ra * Lh + kk];
+    if (rb < nrows) B = in[kz_major ? (int64_t)kk * nrows + rb : rb * Lh + kk];
     A.y *= sg;
     B.y *= sg;
     CT z;
@@ -333,7 +356,8 @@ __device__ __forceinline__ void c2r_rows(const cplx<T>* __restrict__ in, T* __re
   __syncthreads();
   CT* res = stockham<T, 1>(a, b, L, C, rd, tw);
   for (int idx = threadIdx.x; idx < 2 * C * L; idx += blockDim.x) {
-    const int rl = idx / L, z = idx % L;
+    int rl, z;
+    divmod_u(idx, L, rl, z);
     const int64_t row = row0 + rl;
     if (row >= nrows) continue;
     const T* src = reinterpret_cast<const T*>(&res[z * C + (rl >> 1)]);
@@ -353,13 +377,15 @@ __device__ __forceinline__ void strided_lines(cplx<T>* __restrict__ base, int L,
   CT* b = a + (size_t)L * C;
   build_twiddles<T>(tw, L);
   for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
-    const int e = idx / C, c = idx % C;
+    int e, c;
+    divmod_u(idx, C, e, c);
     a[idx] = c < cmax ? base[e * stride + c] : cmk<CT>(0, 0);
   }
   __syncthreads();
   CT* res = stockham<T, SIGN>(a, b, L, C, rd, tw);
   for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
-    const int e = idx / C, c = idx % C;
+    int e, c;
+    divmod_u(idx, C, e, c);
     if (c < cmax) base[e * stride + c] = res[idx];
   }
 }
@@ -388,7 +414,8 @@ __device__ __forceinline__ void x_convolve_lines(cplx<T>* __restrict__ data, int
   CT* base = data + (int64_t)iy * Kzh + z0;
   const int cmax = min(C, Kzh - z0);
   for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
-    const int e = idx / C, c = idx % C;
+    int e, c;
+    divmod_u(idx, C, e, c);
     a[idx] = c < cmax ? base[e * stride + c] : cmk<CT>(0, 0);
   }
   __syncthreads();
@@ -405,7 +432,8 @@ __device__ __forceinline__ void x_convolve_lines(cplx<T>* __restrict__ data, int
   const int myp = fft_m((K1 - iy) % K1, K1);
   const double thy = theta1(my, K1);
   for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
-    const int ix = idx / C, c = idx % C;
+    int ix, c;
+    divmod_u(idx, C, ix, c);
     if (c >= cmax) continue;
     const int iz = z0 + c;
     const int mx = fft_m(ix, K0), mz = fft_m(iz, K2);
@@ -437,7 +465,8 @@ __device__ __forceinline__ void x_convolve_lines(cplx<T>* __restrict__ data, int
   __syncthreads();
   CT* out = stockham<T, 1>(res, oth, L, C, rd, tw);
   for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
-    const int e = idx / C, c = idx % C;
+    int e, c;
+    divmod_u(idx, C, e, c);
     if (c < cmax) base[e * stride + c] = out[idx];
   }
   if (partial_dst) {
@@ -445,6 +474,215 @@ __device__ __forceinline__ void x_convolve_lines(cplx<T>* __restrict__ data, int
     if (threadIdx.x == 0)
       for (int i = 0; i < 10; ++i) partial_dst[i] = acc[i];
   }
+}
+
+
+// ---------------------------------------------------------------------------
+// One kz plane of the half spectrum held whole in LDS (element (x, y) at
+// buf[y * pitch + x], pitch = K0 + 1 so that lines along y and along x both
+// fall on distinct banks).  Forward FFT along y and x, influence function,
+// inverse along x and y, in place: each Stockham pass first reads the inputs
+// of a thread's butterflies into registers, the block synchronises, then the
+// outputs are written.
+constexpr int kPlaneThreads = 1024;
+constexpr int kPlaneB = 1;  // butterflies per thread and pass
+constexpr int kPlaneLoad = 4;  // plane elements per thread (K0 K1 <= kPlaneLoad kPlaneThreads)
+
+template <typename T, int SIGN, int R>
+__device__ __forceinline__ void pass_inplace(cplx<T>* buf, int L, int C, int es, int cs, int p,
+                                             const cplx<T>* tw) {
+  using CT = cplx<T>;
+  const int nb = L / R;
+  const int tstep = L / (p * R);
+  const int nbf = nb * C;
+  CT u[kPlaneB][R];
+#pragma unroll
+  for (int b = 0; b < kPlaneB; ++b) {
+    const int idx = threadIdx.x + b * blockDim.x;
+    if (idx < nbf) {
+      int c, i, k, ip;
+      divmod_u(idx, C, i, c);
+      divmod_u(i, p, ip, k);
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        CT v = buf[(i + j * nb) * es + c * cs];
+        if (j > 0 && k > 0) v = cmul(v, tw_get<T>(tw, k * j * tstep, SIGN));  // < L
+        u[b][j] = v;
+      }
+      dft_r<T, SIGN, R>(u[b]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < kPlaneB; ++b) {
+    const int idx = threadIdx.x + b * blockDim.x;
+    if (idx < nbf) {
+      int c, i, k, ip;
+      divmod_u(idx, C, i, c);
+      divmod_u(i, p, ip, k);
+      const int base = (i - k) * R + k;
+#pragma unroll
+      for (int j = 0; j < R; ++j) buf[(base + j * p) * es + c * cs] = u[b][j];
+    }
+  }
+  __syncthreads();
+}
+
+// C lines of length L, element stride es, line stride cs; radices 2, 3, 4, 5,
+// 8 with (L / r) * C <= kPlaneB * blockDim.x (checked when planning).
+template <typename T, int SIGN>
+__device__ void stockham_inplace(cplx<T>* buf, int L, int C, int es, int cs, const Radix& rd,
+                                 const cplx<T>* tw) {
+  int p = 1;
+  for (int s = 0; s < rd.n; ++s) {
+    const int r = rd.r[s];
+    switch (r) {
+      case 2: pass_inplace<T, SIGN, 2>(buf, L, C, es, cs, p, tw); break;
+      case 3: pass_inplace<T, SIGN, 3>(buf, L, C, es, cs, p, tw); break;
+      case 4: pass_inplace<T, SIGN, 4>(buf, L, C, es, cs, p, tw); break;
+      case 5: pass_inplace<T, SIGN, 5>(buf, L, C, es, cs, p, tw); break;
+      default: pass_inplace<T, SIGN, 8>(buf, L, C, es, cs, p, tw); break;
+    }
+    p *= r;
+  }
+}
+
+// Bytes of dynamic LDS the plane body needs.
+__host__ __device__ inline size_t plane_lds_bytes(int K0, int K1, size_t celem) {
+  return (size_t)K1 * (K0 + 1) * celem + (size_t)(K0 + K1) * celem +
+         (size_t)3 * (K0 + K1) * sizeof(double);
+}
+
+// The plane kz of the kz-major half spectrum `spec` (element (x, y, kz) at
+// spec[(kz * K0 + x) * K1 + y]): forward FFT along y and x, the same
+// influence-function product, energy and dE/dA sums as x_convolve_lines,
+// inverse FFT along x and y.  For a diagonal box exp(-k^2 / 4 kappa^2) / theta^2
+// factors over the three axes and is tabulated per axis.
+template <typename T>
+__device__ __forceinline__ void plane_convolve(cplx<T>* __restrict__ spec, int kz,
+                                               const Radix& rx, const Radix& ry,
+                                               const InflParams& P,
+                                               double* __restrict__ partial_dst) {
+  using CT = cplx<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ double red[10 * (kPlaneThreads / 64)];
+  const int K0 = P.g.k[0], K1 = P.g.k[1], K2 = P.g.k[2];
+  const int pitch = K0 + 1;
+  CT* buf = reinterpret_cast<CT*>(smem_raw);
+  CT* twy = buf + (size_t)K1 * pitch;
+  CT* twx = twy + K1;
+  double* thx = reinterpret_cast<double*>(twx + K0);
+  double* thy = thx + K0;
+  double* exx = thy + K1;  // diagonal box: exp(-k_x^2 / 4 kappa^2) / theta_x^2
+  double* exy = exx + K0;
+  double* kvx = exy + K1;  // diagonal box: k_x
+  double* kvy = kvx + K0;
+  CT* plane = spec + (int64_t)kz * K0 * K1;
+  const int ne = K0 * K1;
+  // global loads first (into registers), the tables while they are in flight
+  CT ld[kPlaneLoad];
+#pragma unroll
+  for (int k = 0; k < kPlaneLoad; ++k) {
+    const int e = threadIdx.x + k * blockDim.x;
+    if (e < ne) ld[k] = plane[e];
+  }
+  const double inv4k2 = 0.25 / (P.kappa * P.kappa);
+  // tasks: twiddles along y, along x, x tables, y tables (2 (K0 + K1) <= blockDim.x)
+  {
+    const int t = threadIdx.x;
+    if (t < K1 + K0) {
+      const int L = t < K1 ? K1 : K0, i = t < K1 ? t : t - K1;
+      double sn, cs;
+      sincospi(2.0 * i / L, &sn, &cs);
+      CT w;
+      w.x = (T)cs;
+      w.y = (T)(-sn);
+      (t < K1 ? twy : twx)[i] = w;
+    } else if (t < 2 * K1 + 2 * K0) {
+      const bool dx = t < K1 + 2 * K0;  // x tables at [K1 + K0, K1 + 2 K0), then y
+      const int i = dx ? t - (K1 + K0) : t - (K1 + 2 * K0);
+      const int K = dx ? K0 : K1;
+      const double th = theta1(fft_m(i, K), K);
+      const double k = 2.0 * kPi * fft_m(i, K) * P.g.a[dx ? 0 : 4];
+      (dx ? thx : thy)[i] = th;
+      (dx ? kvx : kvy)[i] = k;
+      (dx ? exx : exy)[i] = exp(-k * k * inv4k2) / (th * th);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kPlaneLoad; ++k) {
+    const int e = threadIdx.x + k * blockDim.x;
+    if (e < ne) {
+      int x, y;
+      divmod_u(e, K1, x, y);
+      buf[y * pitch + x] = ld[k];
+    }
+  }
+  __syncthreads();
+  stockham_inplace<T, -1>(buf, K1, K0, pitch, 1, ry, twy);  // along y: lines x
+  stockham_inplace<T, -1>(buf, K0, K1, 1, pitch, rx, twx);  // along x: lines y
+  double acc[10];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) acc[i] = 0.0;
+  const int mz = fft_m(kz, K2), mzp = fft_m((K2 - kz) % K2, K2);
+  const double thz = theta1(mz, K2);
+  const double kz_v = 2.0 * kPi * mz * P.g.a[8];
+  const double exz = exp(-kz_v * kz_v * inv4k2) / (thz * thz);
+  const double pref = 2.0 * kPi / P.g.vol;
+  const double w = (kz > 0 && 2 * kz != K2) ? 2.0 : 1.0;
+  for (int e = threadIdx.x; e < K0 * K1; e += blockDim.x) {
+    int ix, iy;
+    divmod_u(e, K0, iy, ix);
+    const int ixp = ix ? K0 - ix : 0, iyp = iy ? K1 - iy : 0;  // index of -m
+    const int mx = fft_m(ix, K0), my = fft_m(iy, K1);
+    const int mxp = fft_m(ixp, K0), myp = fft_m(iyp, K1);
+    double kv[3], kvp[3], G, gp, Gp, gpp;
+    if (P.ortho) {
+      kv[0] = kvx[ix];
+      kv[1] = kvy[iy];
+      kv[2] = kz_v;
+      kvp[0] = kvx[ixp];
+      kvp[1] = kvy[iyp];
+      kvp[2] = 2.0 * kPi * mzp * P.g.a[8];
+      const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+      if (k2 == 0.0) {
+        G = gp = 0.0;
+      } else {
+        const double ik2 = 1.0 / k2;
+        G = pref * exx[ix] * exy[iy] * exz * ik2;
+        gp = -2.0 * G * (inv4k2 + ik2);
+      }
+      Gp = G;  // |k(m')| = |k(m)| for a diagonal box
+      gpp = gp;
+    } else {
+      const double th = thx[ix] * thy[iy] * thz;
+      kvec(P, mx, my, mz, kv);
+      kvec(P, mxp, myp, mzp, kvp);
+      G = gfun(P, kv, th, &gp);
+      Gp = gfun(P, kvp, th, &gpp);
+    }
+    const double Gs = 0.5 * (G + Gp);
+    CT s = buf[iy * pitch + ix];
+    const double S2 = (double)s.x * (double)s.x + (double)s.y * (double)s.y;
+    acc[0] += w * Gs * S2;
+    add_dEdA(acc, P, mx, my, mz, kv, gp, 0.5 * w * S2);
+    add_dEdA(acc, P, mxp, myp, mzp, kvp, gpp, 0.5 * w * S2);
+    const T f = (T)(2.0 * P.eps * P.phi_scale * Gs);
+    s.x *= f;
+    s.y *= f;
+    buf[iy * pitch + ix] = s;
+  }
+  __syncthreads();
+  stockham_inplace<T, 1>(buf, K0, K1, 1, pitch, rx, twx);
+  stockham_inplace<T, 1>(buf, K1, K0, pitch, 1, ry, twy);
+  for (int e = threadIdx.x; e < K0 * K1; e += blockDim.x) {
+    int x, y;
+    divmod_u(e, K1, x, y);
+    plane[e] = buf[y * pitch + x];
+  }
+  block_sum<10>(acc, red);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < 10; ++i) partial_dst[i] = acc[i];
 }
 
 }  // namespace admp

@@ -42,15 +42,16 @@ __device__ __forceinline__ void spread_lane(const Geom& g, double x, double y, d
 }
 
 // One work-group spreads the atoms of cell lc (sorted slots [b, e)) into a
-// float64 LDS tile (dynamic LDS, tg.ntile doubles) and writes the tile to
-// out[tg.ntile].  Atoms whose stencil leaves the tile add to the mesh with
-// global atomics instead.
+// float64 LDS tile (dynamic LDS, tg.ntile doubles), then adds the tile's
+// nonzero points to the mesh with global atomics (one per point and cell
+// instead of one per point and atom).  Atoms whose stencil leaves the tile add
+// to the mesh directly.
 template <typename T>
 __device__ __forceinline__ void spread_cell_tile(const Geom& g, const TileGeom& tg, int lc, int b,
                                                  int e, const int* __restrict__ sorted,
                                                  const double* __restrict__ pos,
                                                  const double* __restrict__ q,
-                                                 T* __restrict__ mesh, T* __restrict__ out) {
+                                                 T* __restrict__ mesh) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* tile = reinterpret_cast<double*>(smem_raw);  // float64 LDS atomics in both precisions
   const int W0 = tg.W[0], W1 = tg.W[1], W2 = tg.W[2];
@@ -106,44 +107,16 @@ __device__ __forceinline__ void spread_cell_tile(const Geom& g, const TileGeom& 
     }
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < tg.ntile; k += blockDim.x) out[k] = (T)tile[k];
-}
-
-// (tile, offset) pairs covering mesh index x along one dimension
-__device__ __forceinline__ int tile_cover(int x, int K, int nc, int W, int* t, int* o) {
-  int n = 0;
-  for (int c = 0; c < nc; ++c) {
-    // x - lo(c) lies in (-K, K + 4]
-    int off = x - tile_lo(K, nc, c);
-    if (off < 0) off += K;
-    if (off >= K) off -= K;
-    for (; off < W && n < kMaxCover; off += K) {
-      t[n] = c;
-      o[n] = off;
-      ++n;
-    }
+  const int b0 = pmod32(lo0, tg.K[0]), b1 = pmod32(lo1, tg.K[1]), b2 = pmod32(lo2, tg.K[2]);
+  for (int k = threadIdx.x; k < tg.ntile; k += blockDim.x) {
+    const double v = tile[k];
+    if (v == 0.0) continue;
+    const int lz = k % W2, ly = (k / W2) % W1, lx = k / (W1 * W2);
+    // tile offsets are mesh indices from lo, modulo the mesh
+    const int ix = wrapk(b0 + lx, tg.K[0]), iy = wrapk(b1 + ly, tg.K[1]),
+              iz = wrapk(b2 + lz, tg.K[2]);
+    unsafeAtomicAdd(mesh + ((int64_t)ix * tg.K[1] + iy) * tg.K[2] + iz, (T)v);
   }
-  return n;
-}
-
-// Sum of the tiles that cover mesh point lp (row-major index), in a fixed order.
-template <typename T>
-__device__ __forceinline__ T tile_sum(const TileGeom& tg, int64_t lp, const T* __restrict__ tiles) {
-  const int gz = (int)(lp % tg.K[2]);
-  const int gy = (int)((lp / tg.K[2]) % tg.K[1]);
-  const int gx = (int)(lp / ((int64_t)tg.K[1] * tg.K[2]));
-  int tx[kMaxCover], ox[kMaxCover], ty[kMaxCover], oy[kMaxCover], tz[kMaxCover], oz[kMaxCover];
-  const int nx = tile_cover(gx, tg.K[0], tg.nc[0], tg.W[0], tx, ox);
-  const int ny = tile_cover(gy, tg.K[1], tg.nc[1], tg.W[1], ty, oy);
-  const int nz = tile_cover(gz, tg.K[2], tg.nc[2], tg.W[2], tz, oz);
-  T s = T(0);
-  for (int a = 0; a < nx; ++a)
-    for (int b = 0; b < ny; ++b) {
-      const int64_t cab = ((int64_t)tx[a] * tg.nc[1] + ty[b]) * tg.nc[2];
-      const int oab = (ox[a] * tg.W[1] + oy[b]) * tg.W[2];
-      for (int c = 0; c < nz; ++c) s += tiles[(cab + tz[c]) * tg.ntile + oab + oz[c]];
-    }
-  return s;
 }
 
 // One lane's share of the gather for one atom (lane a < 6 owns x offset a);

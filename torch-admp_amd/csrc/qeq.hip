@@ -280,13 +280,12 @@ QeqLayout qeq_layout(const QeqProblem& P) {
     nb_layout(P.n, L.cg, &L.nb);
     L.nbws = take(L.nb.bytes);
     L.tg = make_tile_geom(L.cg, P.kmesh, es);
-    L.tiles = take(L.tg.ok ? (size_t)L.cg.ncell * L.tg.ntile * es : 0);
     L.mesh = take((size_t)P.kmesh[0] * P.kmesh[1] * P.kmesh[2] * es);
     L.spec = take((size_t)P.kmesh[0] * P.kmesh[1] * (P.kmesh[2] / 2 + 1) * 2 * es);
     L.ppart = take(sizeof(double) * (std::max(convolve_partials_count(P.kmesh), (size_t)kRedBlocks * 9) + 16));
   } else {
     L.tg.ok = 0;
-    L.nbws = L.tiles = L.mesh = L.spec = L.ppart = 0;
+    L.nbws = L.mesh = L.spec = L.ppart = 0;
   }
   L.bytes = off;
   return L;
@@ -312,8 +311,8 @@ __global__ void __launch_bounds__(kReduceThreads) k_reduce_cols(const double* __
 }
 
 template <typename T>
-static void matvec(const QeqProblem& P, const QeqLayout& L, char* ws, const Geom& g,
-                   const double* v, double* out, hipStream_t s) {
+static hipError_t matvec(const QeqProblem& P, const QeqLayout& L, char* ws, const Geom& g,
+                         const double* v, double* out, hipStream_t s) {
   const int64_t n = P.n;
   k_diag_mul<<<grid_for(n), kBlock, 0, s>>>(n, P.hdiag, v, out);
   if (P.n_pairs > 0)
@@ -321,14 +320,18 @@ static void matvec(const QeqProblem& P, const QeqLayout& L, char* ws, const Geom
         P.n_pairs, P.pairs, (const double*)(ws + L.c), v, out);
   if (P.kspace) {
     T* mesh = (T*)(ws + L.mesh);
-    if (L.tg.ok)
-      launch_spread_tiled<T>(P.pos, v, n, g, L.tg, L.cg.ncell, (const int*)(ws + L.nbws + L.nb.sorted),
-                             (const int*)(ws + L.nbws + L.nb.start), (T*)(ws + L.tiles), mesh, s);
-    else
-      launch_spread<T>(P.pos, v, n, g, mesh, s);
-    launch_convolve<T>(mesh, g, P.convention, P.kappa, P.eps, 1.0, mesh, nullptr, nullptr,
-                       (cplx<T>*)(ws + L.spec), (double*)(ws + L.ppart), s);
-    launch_gather<T>(P.pos, v, n, g, mesh, nullptr, out, nullptr, 1, (double*)(ws + L.ppart), s);
+    hipError_t e =
+        L.tg.ok ? launch_spread_tiled<T>(P.pos, v, n, g, L.tg, L.cg.ncell,
+                                         (const int*)(ws + L.nbws + L.nb.sorted),
+                                         (const int*)(ws + L.nbws + L.nb.start), mesh, s)
+                : launch_spread<T>(P.pos, v, n, g, mesh, s);
+    if (e != hipSuccess) return e;
+    e = launch_convolve<T>(mesh, g, P.convention, P.kappa, P.eps, 1.0, mesh, nullptr, nullptr,
+                           (cplx<T>*)(ws + L.spec), (double*)(ws + L.ppart), s);
+    if (e != hipSuccess) return e;
+    e = launch_gather<T>(P.pos, v, n, g, mesh, nullptr, out, nullptr, 1,
+                         (double*)(ws + L.ppart), s);
+    if (e != hipSuccess) return e;
   }
   if (P.kspace || P.slab) {
     double* sc = (double*)(ws + L.sc);
@@ -343,6 +346,7 @@ static void matvec(const QeqProblem& P, const QeqLayout& L, char* ws, const Geom
     k_global_terms<<<grid_for(n), kBlock, 0, s>>>(n, (const double*)(ws + L.zc), sc + kScSums, c_nn,
                                                   pre_slab, P.lz2, P.slab, out);
   }
+  return hipGetLastError();
 }
 
 // dots of (x, y) and A x into sc[dst..]: dst = x.y, dst+1.. = A x
@@ -373,7 +377,8 @@ static int setup(const QeqProblem& P, const QeqLayout& L, char* ws, Geom* g, hip
     k_pair_coeff<<<(unsigned)((P.n_pairs + kBlock - 1) / kBlock), kBlock, 0, s>>>(
         P.n_pairs, P.pairs, P.ds, P.bs, P.eta, P.coul, P.damping, P.kappa, P.eps,
         (double*)(ws + L.c));
-  if (P.kspace) launch_bin(P.pos, P.n, L.cg, ws + L.nbws, L.nb, s);
+  if (P.kspace && launch_bin(P.pos, P.n, L.cg, ws + L.nbws, L.nb, s) != hipSuccess)
+    return ADMP_ERR_HIP;
   if (P.slab || P.kspace)
     k_extract_z<<<grid_for(P.n), kBlock, 0, s>>>(P.n, P.pos, P.slab ? P.slab_axis : 0,
                                                   (double*)(ws + L.zc));
@@ -388,7 +393,7 @@ int qeq_matvec_impl(const QeqProblem& P, const double* v, double* out, void* wor
   Geom g;
   int st = setup<T>(P, L, ws, &g, s);
   if (st != ADMP_OK) return st;
-  matvec<T>(P, L, ws, g, v, out, s);
+  if (matvec<T>(P, L, ws, g, v, out, s) != hipSuccess) return ADMP_ERR_HIP;
   return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
 }
 
@@ -426,7 +431,7 @@ int qeq_solve_impl(const QeqProblem& P, const double* q0, double tol, int max_it
     k_project<<<gb, kBlock, 0, s>>>(n, q, P.cons, nc, G, sc + kScAx, P.cons_vals_dev);
   }
   // r = -P (H q + chi)
-  matvec<T>(P, L, ws, g, q, w, s);
+  if (matvec<T>(P, L, ws, g, q, w, s) != hipSuccess) return ADMP_ERR_HIP;
   k_init_residual<<<gb, kBlock, 0, s>>>(n, w, P.chi, r);
   if (nc > 0) {
     dots_into(P, L, ws, r, nullptr, kScAx - 1, s);
@@ -457,7 +462,7 @@ int qeq_solve_impl(const QeqProblem& P, const double* q0, double tol, int max_it
     ++it;
     const int rz_old = (it & 1) ? kScRz0 : kScRz1;
     const int rz_new = (it & 1) ? kScRz1 : kScRz0;
-    matvec<T>(P, L, ws, g, p, w, s);
+    if (matvec<T>(P, L, ws, g, p, w, s) != hipSuccess) return ADMP_ERR_HIP;
     dots_into(P, L, ws, w, p, kScDots, s);  // w.p and A w (A acts on the first argument)
     k_cg_step<<<gb, kBlock, 0, s>>>(n, q, r, p, w, P.cons, nc, G, sc + rz_old, sc + kScDots);
     if (it % check_every == 0 || it == max_iter) {

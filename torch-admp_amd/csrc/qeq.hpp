@@ -40,7 +40,7 @@ struct QeqProblem {
 };
 
 struct QeqLayout {
-  size_t c, r, z, p, w, s, zc, dots, sc, status, nbws, tiles, mesh, spec, ppart, bytes;
+  size_t c, r, z, p, w, s, zc, dots, sc, status, nbws, mesh, spec, ppart, bytes;
   CellGrid cg;
   NbLayout nb;
   TileGeom tg;

@@ -6,7 +6,6 @@
 
 namespace admp {
 
-constexpr int kMaxCover = 8;             // (tile, offset) pairs per mesh index and dimension
 constexpr size_t kMaxTileBytes = 65536;  // LDS budget of one tile
 
 struct TileGeom {
@@ -14,7 +13,7 @@ struct TileGeom {
   int K[3];
   int W[3];
   int ntile;  // W0 * W1 * W2
-  int ok;     // tiles fit LDS and the cover bound
+  int ok;     // the tile fits LDS
 };
 
 // Tile geometry for the cell grid; ok == 0 when the tiled path does not apply.
@@ -23,6 +22,6 @@ TileGeom make_tile_geom(const CellGrid& cg, const int32_t* kmesh, size_t elem_by
 template <typename T>
 hipError_t launch_spread_tiled(const double* pos, const double* q, int64_t n, const Geom& g,
                                const TileGeom& tg, int ncell, const int* sorted, const int* start,
-                               T* tiles, T* mesh, hipStream_t s);
+                               T* mesh, hipStream_t s);
 
 }  // namespace admp
