@@ -429,7 +429,8 @@ def main():
         extras["electrode_3600"] = electrode_bench(torch, dev)
 
     err32_vs64 = {"energy_rel": float(abs(e32 - e64) / abs(e64)),
-                  "force_rmse_eV_per_A": float(((f32 - f64) ** 2).mean().sqrt())}
+                  "force_rmse_eV_per_A": float(((f32 - f64) ** 2).mean().sqrt()),
+                  "dedbox_max_rel": float((gb32 - gb64).abs().max() / gb64.abs().max())}
 
     # ---- CPU baseline: the float64 oracle (reference algorithm) ---------------
     cpu = None

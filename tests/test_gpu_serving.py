@@ -1,9 +1,13 @@
 """Serving loop: continuous batching under an HBM budget.
 
 Each served result equals the same request evaluated alone (a batch of one)
-to 1e-6 relative in the float32 engine (global float32 atomics of the untiled
-spread make the last bits order dependent) and 1e-12 in float64, and matches
-the oracle within the float32 bars (energy 1e-6 relative, forces 1e-4 eV/A).
+to 1e-6 relative in the float32 engine and 1e-12 in float64, dE/dbox to 10x
+that in float64 and to the float32 engine's dE/dbox accuracy bar (1e-4
+relative, as against the oracle in the batch tests) in float32: float32
+atomics and the in-cell atom order, both fixed by timing, change float32
+partial sums, and dE/dbox amplifies those differences through the cancelling
+pair and atomic virials.  Results also match the oracle within the float32
+bars (energy 1e-6 relative, forces 1e-4 eV/A).
 """
 
 import numpy as np
@@ -37,7 +41,8 @@ def test_served_results_equal_alone(dev, precision, tol):
         e1, f1, g1 = _alone(r, dev, precision)
         assert np.abs(r.energy - e1).max() <= tol * max(1.0, np.abs(e1).max()), r.rid
         assert np.abs(r.forces - f1).max() <= tol * max(1.0, np.abs(f1).max()), r.rid
-        assert np.abs(r.dedbox - g1).max() <= 10 * tol * max(1.0, np.abs(g1).max()), r.rid
+        gtol = 1e-4 if precision == "float32" else 10 * tol
+        assert np.abs(r.dedbox - g1).max() <= gtol * max(1.0, np.abs(g1).max()), r.rid
         assert r.t_admit >= r.arrival and r.t_done >= r.t_admit
 
 

@@ -100,41 +100,64 @@ __global__ void __launch_bounds__(kBlock) k_b_bin(int64_t ntot, int nsys,
   atomicAdd(&count[cid * kSub + (int)(i & (kSub - 1))], 1);
 }
 
-// Deterministic in-cell order and packing, one work-group per cell: the
-// rank of atom i among the atoms of its cell is the number of members with a
-// smaller index.  Members are staged in LDS (global reads for cells above
-// kSortCap atoms).  Writes sorted[] and the packed (w, q) records.
-constexpr int kSortCap = 4096;
+// Cell order and packing, one thread per atom: slot = start[key] + a cursor
+// atomic (key = cell * kSub + i mod kSub), then sorted[slot] = i and
+// ps[slot] = (wrapped position, charge).  The order within a cell is that of
+// the atomics; every consumer (centre-cell pair test, spread, gather) works
+// with any order.  With start == nullptr each work-group forms the exclusive
+// scan of the ncount counters in LDS itself (ncount <= kScatterScanMax) and
+// work-group 0 stores it as start; otherwise start was scanned beforehand.
+constexpr int kScatterScanMax = 8192;
 
-__global__ void __launch_bounds__(kBlock) k_b_sort_pack(const int* __restrict__ start,
-                                                        const int* __restrict__ unsorted,
-                                                        const double* __restrict__ w,
-                                                        const double* __restrict__ q,
-                                                        int* __restrict__ sorted,
-                                                        P4<double>* __restrict__ ps) {
-  __shared__ int ids[kSortCap];
-  const int64_t c = blockIdx.x;
-  const int b = start[c * kSub], e = start[(c + 1) * kSub];
-  const int m = e - b;
-  const bool in_lds = m <= kSortCap;
-  if (in_lds)
-    for (int t = threadIdx.x; t < m; t += blockDim.x) ids[t] = unsorted[b + t];
-  __syncthreads();
-  for (int t = threadIdx.x; t < m; t += blockDim.x) {
-    const int i = in_lds ? ids[t] : unsorted[b + t];
-    int rank = 0;
-    if (in_lds)
-      for (int u = 0; u < m; ++u) rank += ids[u] < i;
-    else
-      for (int u = 0; u < m; ++u) rank += unsorted[b + u] < i;
-    sorted[b + rank] = i;
-    P4<double> v;
-    v.x = w[3 * i];
-    v.y = w[3 * i + 1];
-    v.z = w[3 * i + 2];
-    v.q = q[i];
-    ps[b + rank] = v;
+__global__ void __launch_bounds__(kBlock) k_b_scatter_pack(int64_t ntot, int ncount,
+                                                           const int* __restrict__ count,
+                                                           int* __restrict__ start_out,
+                                                           const int* __restrict__ start,
+                                                           int* __restrict__ cursor,
+                                                           const int* __restrict__ cell_of,
+                                                           const double* __restrict__ w,
+                                                           const double* __restrict__ q,
+                                                           int* __restrict__ sorted,
+                                                           P4<double>* __restrict__ ps) {
+  __shared__ int st[kScatterScanMax + 1];
+  __shared__ int part[kBlock];
+  if (!start) {
+    // thread t sums counts [t * per, (t + 1) * per); scan of the sums; write back
+    const int per = (ncount + kBlock - 1) / kBlock;
+    const int base = threadIdx.x * per;
+    int sum = 0;
+    for (int k = 0; k < per; ++k)
+      if (base + k < ncount) sum += count[base + k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+      const int add = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+      __syncthreads();
+      part[threadIdx.x] += add;
+      __syncthreads();
+    }
+    int run = part[threadIdx.x] - sum;
+    for (int k = 0; k < per; ++k)
+      if (base + k < ncount) {
+        st[base + k] = run;
+        run += count[base + k];
+      }
+    if (threadIdx.x == kBlock - 1) st[ncount] = part[kBlock - 1];
+    __syncthreads();
+    if (blockIdx.x == 0)
+      for (int k = threadIdx.x; k <= ncount; k += blockDim.x) start_out[k] = st[k];
   }
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntot) return;
+  const int key = cell_of[i] * kSub + (int)(i & (kSub - 1));
+  const int slot = (start ? start[key] : st[key]) + atomicAdd(&cursor[key], 1);
+  sorted[slot] = (int)i;
+  P4<double> v;
+  v.x = w[3 * i];
+  v.y = w[3 * i + 1];
+  v.z = w[3 * i + 2];
+  v.q = q[i];
+  ps[slot] = v;
 }
 
 // ------------------------------------------------------------------ real space
@@ -680,11 +703,17 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   if (ntot > 0)
     k_b_bin<<<nb_atoms, kBlock, 0, s>>>(ntot, nsys, seg(kSegAtom), tab, pos, (double*)(nbws + L.w),
                                         (int*)(nbws + L.cell_of), (int*)(ws + P.o_sys_of), count);
-  (void)launch_cell_scatter(ntot, P.ctot, nbws, L, s);
-  if (ntot > 0)
-    k_b_sort_pack<<<(unsigned)P.ctot, kBlock, 0, s>>>(start, (const int*)(nbws + L.unsorted),
-                                                      (const double*)(nbws + L.w), q,
-                                                      (int*)(nbws + L.sorted), ps);
+  {
+    // cell starts (scanned in the scatter kernel when the counters fit LDS), scatter + pack
+    const int64_t nkeys = P.ctot * kSub;
+    const bool scan_inline = nkeys <= kScatterScanMax;
+    if (!scan_inline) launch_scan<int>(count, nkeys, start, (int*)(nbws + L.scan), s);
+    k_b_scatter_pack<<<(unsigned)std::max<int64_t>((ntot + kBlock - 1) / kBlock, 1), kBlock, 0,
+                       s>>>(ntot, (int)std::min<int64_t>(nkeys, INT32_MAX), count, start,
+                            scan_inline ? nullptr : start, (int*)(nbws + L.cursor),
+                            (const int*)(nbws + L.cell_of), (const double*)(nbws + L.w), q,
+                            (int*)(nbws + L.sorted), ps);
+  }
   mark(1);
   // ---- real space (also for empty batches: its partial rows feed the reductions),
   // forked onto the side stream unless stages are being timed

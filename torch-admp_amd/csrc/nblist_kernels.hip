@@ -319,6 +319,9 @@ void launch_scan(const I* in, int64_t n, I* out, I* scratch, hipStream_t s) {
   k_scan_apply<I><<<nb, kBlock, 0, s>>>(in, n, scratch, out);
 }
 
+template void launch_scan<int>(const int*, int64_t, int*, int*, hipStream_t);
+template void launch_scan<int64_t>(const int64_t*, int64_t, int64_t*, int64_t*, hipStream_t);
+
 // ---------------------------------------------------------------------------
 // Pair distances: dr = pbc_shift(r_i - r_j) (src/torch_admp/spatial.py:15-36)
 __global__ void __launch_bounds__(kBlock) k_pair_ds(const double* __restrict__ pos,

@@ -56,6 +56,9 @@ void nb_layout(int64_t n, const CellGrid& cg, NbLayout* L);
 
 hipError_t launch_bin(const double* pos, int64_t n, const CellGrid& cg, char* ws,
                       const NbLayout& L, hipStream_t s);
+// out[0..n] = exclusive scan of in[0..n-1] with out[n] = total (I = int or int64_t)
+template <typename I>
+void launch_scan(const I* in, int64_t n, I* out, I* scratch, hipStream_t s);
 // counting sort of binned atoms (cell_of and counts filled): start, sorted
 hipError_t launch_cell_sort(int64_t n, int64_t ncell, char* ws, const NbLayout& L, hipStream_t s);
 // its first half: start and the unordered cell lists in `unsorted`

@@ -138,10 +138,9 @@ __device__ __forceinline__ int row_shl1(int old, int v) {
   return __builtin_amdgcn_update_dpp(old, v, 0x101, 0xf, 0xf, false);
 }
 
-template <typename T>
+// Per-lane energy and pair virial, float64 in both precisions.
 struct PairAcc {
-  double e;
-  T vxx, vyy, vzz, vxy, vxz, vyz;
+  double e, vxx, vyy, vzz, vxy, vxz, vyz;
 };
 
 // One unordered pair per lane of a wavefront (`valid` lanes only): adds dE/dr_i
@@ -156,7 +155,7 @@ template <typename T>
 __device__ __forceinline__ void pair_eval(const Staged<T>& pi, const Staged<T>& pj, bool valid,
                                           int kx, int run, T eps, const CoulConst<T>& C,
                                           double* facc_i,
-                                          double* cacc, PairAcc<T>& acc) {
+                                          double* cacc, PairAcc& acc) {
   T gx = T(0), gy = T(0), gz = T(0);
   if (valid) {
     // hi parts of nearby atoms subtract exactly; the lo parts restore float64 accuracy
@@ -173,12 +172,12 @@ __device__ __forceinline__ void pair_eval(const Staged<T>& pi, const Staged<T>& 
     atomicAdd(facc_i + 0, -(double)gx);
     atomicAdd(facc_i + 1, -(double)gy);
     atomicAdd(facc_i + 2, -(double)gz);
-    acc.vxx += gx * dx;
-    acc.vyy += gy * dy;
-    acc.vzz += gz * dz;
-    acc.vxy += gx * dy;
-    acc.vxz += gx * dz;
-    acc.vyz += gy * dz;
+    acc.vxx += (double)(gx * dx);
+    acc.vyy += (double)(gy * dy);
+    acc.vzz += (double)(gz * dz);
+    acc.vxy += (double)(gx * dy);
+    acc.vxz += (double)(gx * dz);
+    acc.vyz += (double)(gy * dz);
   }
   const int lane = threadIdx.x & 63;
   const int key = valid ? run + 1 : -1 - lane;  // invalid lanes form runs of their own; 0 = none
@@ -302,9 +301,8 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
   C.k2h = (T)(kappa * kappa);
   C.k2l = (T)(kappa * kappa - (double)C.k2h);
   C.c_exp = (T)(2.0 / kSqrtPi * kappa);
-  PairAcc<T> pacc;
-  pacc.e = 0.0;
-  pacc.vxx = pacc.vyy = pacc.vzz = pacc.vxy = pacc.vxz = pacc.vyz = T(0);
+  PairAcc pacc;
+  pacc.e = pacc.vxx = pacc.vyy = pacc.vzz = pacc.vxy = pacc.vxz = pacc.vyz = 0.0;
   double acc[kRealNV];
 #pragma unroll
   for (int k = 0; k < kRealNV; ++k) acc[k] = 0.0;
@@ -454,12 +452,12 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
     __syncthreads();
   }
   acc[0] = pacc.e;
-  acc[1] = (double)pacc.vxx;
-  acc[2] = (double)pacc.vyy;
-  acc[3] = (double)pacc.vzz;
-  acc[4] = (double)pacc.vxy;
-  acc[5] = (double)pacc.vxz;
-  acc[6] = (double)pacc.vyz;
+  acc[1] = pacc.vxx;
+  acc[2] = pacc.vyy;
+  acc[3] = pacc.vzz;
+  acc[4] = pacc.vxy;
+  acc[5] = pacc.vxz;
+  acc[6] = pacc.vyz;
   block_sum<kRealNV>(acc, red);
   if (threadIdx.x == 0)
     for (int k = 0; k < kRealNV; ++k) partial[k * pstride] = acc[k];
