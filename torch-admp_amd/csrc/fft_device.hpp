@@ -484,9 +484,9 @@ __device__ __forceinline__ void x_convolve_lines(cplx<T>* __restrict__ data, int
 // inverse along x and y, in place: each Stockham pass first reads the inputs
 // of a thread's butterflies into registers, the block synchronises, then the
 // outputs are written.
-constexpr int kPlaneThreads = 1024;
+constexpr int kPlaneThreads = 512;  // measured faster than 1024 and 256 at 64^2
 constexpr int kPlaneB = 1;  // butterflies per thread and pass
-constexpr int kPlaneLoad = 4;  // plane elements per thread (K0 K1 <= kPlaneLoad kPlaneThreads)
+constexpr int kPlaneLoad = 8;  // plane elements per thread (K0 K1 <= kPlaneLoad kPlaneThreads)
 
 template <typename T, int SIGN, int R>
 __device__ __forceinline__ void pass_inplace(cplx<T>* buf, int L, int C, int es, int cs, int p,
@@ -630,43 +630,56 @@ __device__ __forceinline__ void plane_convolve(cplx<T>* __restrict__ spec, int k
   const double exz = exp(-kz_v * kz_v * inv4k2) / (thz * thz);
   const double pref = 2.0 * kPi / P.g.vol;
   const double w = (kz > 0 && 2 * kz != K2) ? 2.0 : 1.0;
+  // diagonal box: the partner m' of m is m with every component negated except
+  // a Nyquist one (index K/2), so m' (x) k(m') = s_i s_j m (x) k(m) with
+  // s_d = -1 (+1 at Nyquist), and |k(m')| = |k(m)|: the two dE/dA terms of
+  // the general case merge into one with factor 1 + s_i s_j
+  const bool nyq_z = 2 * kz == K2;
   for (int e = threadIdx.x; e < K0 * K1; e += blockDim.x) {
     int ix, iy;
     divmod_u(e, K0, iy, ix);
-    const int ixp = ix ? K0 - ix : 0, iyp = iy ? K1 - iy : 0;  // index of -m
-    const int mx = fft_m(ix, K0), my = fft_m(iy, K1);
-    const int mxp = fft_m(ixp, K0), myp = fft_m(iyp, K1);
-    double kv[3], kvp[3], G, gp, Gp, gpp;
+    CT s = buf[iy * pitch + ix];
+    const double S2 = (double)s.x * (double)s.x + (double)s.y * (double)s.y;
+    double Gs;
     if (P.ortho) {
-      kv[0] = kvx[ix];
-      kv[1] = kvy[iy];
-      kv[2] = kz_v;
-      kvp[0] = kvx[ixp];
-      kvp[1] = kvy[iyp];
-      kvp[2] = 2.0 * kPi * mzp * P.g.a[8];
-      const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
-      if (k2 == 0.0) {
-        G = gp = 0.0;
-      } else {
+      const double kx = kvx[ix], ky = kvy[iy];
+      const double k2 = kx * kx + ky * ky + kz_v * kz_v;
+      double G = 0.0, gp = 0.0;
+      if (k2 != 0.0) {
         const double ik2 = 1.0 / k2;
         G = pref * exx[ix] * exy[iy] * exz * ik2;
         gp = -2.0 * G * (inv4k2 + ik2);
       }
-      Gp = G;  // |k(m')| = |k(m)| for a diagonal box
-      gpp = gp;
+      Gs = G;
+      acc[0] += w * G * S2;
+      const double c = w * S2 * gp * 2.0 * kPi;  // 2 x (0.5 w S2 gp 2 pi)
+      const double m[3] = {(double)fft_m(ix, K0), (double)fft_m(iy, K1), (double)mz};
+      const double kk[3] = {kx, ky, kz_v};
+      const bool ny[3] = {2 * ix == K0, 2 * iy == K1, nyq_z};
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          if (i != j && ny[i] != ny[j]) continue;  // 1 + s_i s_j = 0
+          const double v = c * kk[j] * m[i];
+          if (P.conv == ADMP_KCONV_REFERENCE) acc[1 + i * 3 + j] += v;
+          else acc[1 + j * 3 + i] += v;
+        }
     } else {
+      const int ixp = ix ? K0 - ix : 0, iyp = iy ? K1 - iy : 0;  // index of -m
+      const int mx = fft_m(ix, K0), my = fft_m(iy, K1);
+      const int mxp = fft_m(ixp, K0), myp = fft_m(iyp, K1);
+      double kv[3], kvp[3], gp, gpp;
       const double th = thx[ix] * thy[iy] * thz;
       kvec(P, mx, my, mz, kv);
       kvec(P, mxp, myp, mzp, kvp);
-      G = gfun(P, kv, th, &gp);
-      Gp = gfun(P, kvp, th, &gpp);
+      const double G = gfun(P, kv, th, &gp);
+      const double Gp = gfun(P, kvp, th, &gpp);
+      Gs = 0.5 * (G + Gp);
+      acc[0] += w * Gs * S2;
+      add_dEdA(acc, P, mx, my, mz, kv, gp, 0.5 * w * S2);
+      add_dEdA(acc, P, mxp, myp, mzp, kvp, gpp, 0.5 * w * S2);
     }
-    const double Gs = 0.5 * (G + Gp);
-    CT s = buf[iy * pitch + ix];
-    const double S2 = (double)s.x * (double)s.x + (double)s.y * (double)s.y;
-    acc[0] += w * Gs * S2;
-    add_dEdA(acc, P, mx, my, mz, kv, gp, 0.5 * w * S2);
-    add_dEdA(acc, P, mxp, myp, mzp, kvp, gpp, 0.5 * w * S2);
     const T f = (T)(2.0 * P.eps * P.phi_scale * Gs);
     s.x *= f;
     s.y *= f;
