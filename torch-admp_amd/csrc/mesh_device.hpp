@@ -107,15 +107,21 @@ __device__ __forceinline__ void spread_cell_tile(const Geom& g, const TileGeom& 
     }
   }
   __syncthreads();
+  // tile offsets are mesh indices from lo, modulo the mesh; consecutive lanes
+  // take consecutive z of one tile row (rows_per_pass rows at a time), so a
+  // wavefront's atomics fall on few mesh lines
   const int b0 = pmod32(lo0, tg.K[0]), b1 = pmod32(lo1, tg.K[1]), b2 = pmod32(lo2, tg.K[2]);
-  for (int k = threadIdx.x; k < tg.ntile; k += blockDim.x) {
-    const double v = tile[k];
-    if (v == 0.0) continue;
-    const int lz = k % W2, ly = (k / W2) % W1, lx = k / (W1 * W2);
-    // tile offsets are mesh indices from lo, modulo the mesh
-    const int ix = wrapk(b0 + lx, tg.K[0]), iy = wrapk(b1 + ly, tg.K[1]),
-              iz = wrapk(b2 + lz, tg.K[2]);
-    unsafeAtomicAdd(mesh + ((int64_t)ix * tg.K[1] + iy) * tg.K[2] + iz, (T)v);
+  const int rows_per_pass = blockDim.x / W2;
+  const int lz = threadIdx.x % W2, r0 = threadIdx.x / W2;
+  if (r0 < rows_per_pass) {
+    const int iz = wrapk(b2 + lz, tg.K[2]);
+    for (int row = r0; row < W0 * W1; row += rows_per_pass) {
+      const double v = tile[row * W2 + lz];
+      if (v == 0.0) continue;
+      const int lx = row / W1, ly = row - lx * W1;
+      const int ix = wrapk(b0 + lx, tg.K[0]), iy = wrapk(b1 + ly, tg.K[1]);
+      unsafeAtomicAdd(mesh + ((int64_t)ix * tg.K[1] + iy) * tg.K[2] + iz, (T)v);
+    }
   }
 }
 

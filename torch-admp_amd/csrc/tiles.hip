@@ -32,6 +32,7 @@ TileGeom make_tile_geom(const CellGrid& cg, const int32_t* kmesh, size_t elem_by
   }
   t.ntile = (int)std::min<size_t>(ntile, 1u << 30);
   if (ntile * sizeof(double) > kMaxTileBytes) t.ok = 0;
+  if (t.W[2] > kBlock) t.ok = 0;  // the tile flush puts one tile row's z points on lanes
   if (cg.obc) t.ok = 0;
   return t;
 }
