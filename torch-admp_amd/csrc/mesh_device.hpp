@@ -125,6 +125,29 @@ __device__ __forceinline__ void spread_cell_tile(const Geom& g, const TileGeom& 
   }
 }
 
+// Six consecutive mesh values (element-aligned only) in two or three vector loads.
+__device__ __forceinline__ void load6(const float* r, float p[6]) {
+  typedef float f4 __attribute__((ext_vector_type(4), aligned(4)));
+  typedef float f2 __attribute__((ext_vector_type(2), aligned(4)));
+  const f4 u = *reinterpret_cast<const f4*>(r);
+  const f2 v = *reinterpret_cast<const f2*>(r + 4);
+  p[0] = u.x;
+  p[1] = u.y;
+  p[2] = u.z;
+  p[3] = u.w;
+  p[4] = v.x;
+  p[5] = v.y;
+}
+__device__ __forceinline__ void load6(const double* r, double p[6]) {
+  typedef double d2 __attribute__((ext_vector_type(2), aligned(8)));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const d2 u = *reinterpret_cast<const d2*>(r + 2 * k);
+    p[2 * k] = u.x;
+    p[2 * k + 1] = u.y;
+  }
+}
+
 // One lane's share of the gather for one atom (lane a < 6 owns x offset a);
 // the caller sums the outputs over the atom's 8 lanes.  With phi = dE/dQ:
 //   sx, sy, sz = sum_g phi(g) dtheta/du_d (u = f + j, du/dR = -1)
@@ -147,16 +170,22 @@ __device__ __forceinline__ void gather_lane(const Geom& g, double x, double y, d
     int iz[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) iz[c] = wrapk(bz + c, K2);
+    const bool run = bz + 5 < K2;  // the 6 z points are consecutive (no wrap)
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
       const int iy = wrapk(by + b, K1);
       const T* row = phi + ((int64_t)ix * K1 + iy) * K2;
+      T p[6];
+      if (run) load6(row + bz, p);
+      else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) p[c] = row[iz[c]];
+      }
       T rz = 0, rdz = 0;
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
-        const T p = row[iz[c]];
-        rz += p * wz[c];
-        rdz += p * dz[c];
+        rz += p[c] * wz[c];
+        rdz += p[c] * dz[c];
       }
       ap += wy[b] * rz;
       ay += dy[b] * rz;
