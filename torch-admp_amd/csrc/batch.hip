@@ -182,9 +182,13 @@ __global__ void __launch_bounds__(kRealBlock) k_b_real(int nsys,
 }
 
 // ------------------------------------------------------------------ spread
-// systems whose tiles fit LDS: one work-group per cell (global cell index)
+// systems whose tiles fit LDS: one work-group per cell (global cell index),
+// kSpreadThreads threads (64 atoms a pass: at ~70 atoms a cell, 512 measured
+// faster than 256 and 768)
+constexpr int kSpreadThreads = 512;
+
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_b_spread_tiles(int nsys,
+__global__ void __launch_bounds__(kSpreadThreads) k_b_spread_tiles(int nsys,
                                                            const int64_t* __restrict__ cell_off,
                                                            const SysDesc* __restrict__ tab,
                                                            const double* __restrict__ pos,
@@ -740,7 +744,7 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   // ---- spread: LDS tiles added to the mesh (tiled systems) or per-atom atomics (the others)
   if (ntot > 0) {
     if (P.lds_tile > 0)
-      k_b_spread_tiles<T><<<(unsigned)P.ctot, kBlock, P.lds_tile, s>>>(
+      k_b_spread_tiles<T><<<(unsigned)P.ctot, kSpreadThreads, P.lds_tile, s>>>(
           nsys, seg(kSegCell), tab, pos, q, (const int*)(nbws + L.sorted), start, mesh);
     if (P.any_untiled)
       k_b_spread_atoms<T><<<(unsigned)((ntot * 8 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
