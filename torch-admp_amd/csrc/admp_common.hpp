@@ -118,12 +118,38 @@ __device__ __forceinline__ void mesh_ref(const Geom& g, double x, double y, doub
   }
 }
 
-// ---- wavefront / block reductions (64-wide wavefronts) ---------------------
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+// ---- wavefront / block reductions (64-wide wavefronts, every lane active) ----
+// DPP moves: lane i receives the lane that CTRL names; lanes of the rows (16
+// lanes each) outside ROWS receive 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_move(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_move<CTRL, ROWS>((int)b), hi = dpp_move<CTRL, ROWS>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double read_lane(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes, returned in every lane, in a fixed order: pairs
+// (quad_perm 1,0,3,2), quads (2,3,0,1), half rows (row_half_mirror), rows
+// (row_mirror), then row_bcast:15 adds row 0 into row 1 and row 2 into row 3,
+// and row_bcast:31 adds rows 0-1 into row 3, whose lane 63 holds the total.
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_move<0xb1, 0xf>(v);
+  v += dpp_move<0x4e, 0xf>(v);
+  v += dpp_move<0x141, 0xf>(v);
+  v += dpp_move<0x140, 0xf>(v);
+  v += dpp_move<0x142, 0xa>(v);
+  v += dpp_move<0x143, 0xc>(v);
+  return read_lane(v, 63);
 }
 
 // Sum NV doubles over the block; result valid in thread 0.  `smem` holds

@@ -213,6 +213,9 @@ struct InflParams {
   int ortho;         // box diagonal: |k| of the Hermitian partner equals |k|
 };
 
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }  // 1 ulp
+
 __device__ __forceinline__ int fft_m(int idx, int K) { return idx < (K + 1) / 2 ? idx : idx - K; }
 
 __device__ __forceinline__ double theta1(int m, int K) {
@@ -531,7 +534,7 @@ __device__ __forceinline__ void pass_inplace(cplx<T>* buf, int L, int C, int es,
 // C lines of length L, element stride es, line stride cs; radices 2, 3, 4, 5,
 // 8 with (L / r) * C <= kPlaneB * blockDim.x (checked when planning).
 template <typename T, int SIGN>
-__device__ void stockham_inplace(cplx<T>* buf, int L, int C, int es, int cs, const Radix& rd,
+__device__ __forceinline__ void stockham_inplace(cplx<T>* buf, int L, int C, int es, int cs, const Radix& rd,
                                  const cplx<T>* tw) {
   int p = 1;
   for (int s = 0; s < rd.n; ++s) {
@@ -573,10 +576,10 @@ __device__ __forceinline__ void plane_convolve(cplx<T>* __restrict__ spec, int k
   CT* twx = twy + K1;
   double* thx = reinterpret_cast<double*>(twx + K0);
   double* thy = thx + K0;
-  double* exx = thy + K1;  // diagonal box: exp(-k_x^2 / 4 kappa^2) / theta_x^2
-  double* exy = exx + K0;
-  double* kvx = exy + K1;  // diagonal box: k_x
-  double* kvy = kvx + K0;
+  T* exx = reinterpret_cast<T*>(thy + K1);  // diagonal box: exp(-k_x^2 / 4 kappa^2) / theta_x^2
+  T* exy = exx + K0;
+  T* kvx = exy + K1;  // diagonal box: k_x
+  T* kvy = kvx + K0;
   CT* plane = spec + (int64_t)kz * K0 * K1;
   const int ne = K0 * K1;
   // global loads first (into registers), the tables while they are in flight
@@ -605,10 +608,15 @@ __device__ __forceinline__ void plane_convolve(cplx<T>* __restrict__ spec, int k
       const double th = theta1(fft_m(i, K), K);
       const double k = 2.0 * kPi * fft_m(i, K) * P.g.a[dx ? 0 : 4];
       (dx ? thx : thy)[i] = th;
-      (dx ? kvx : kvy)[i] = k;
-      (dx ? exx : exy)[i] = exp(-k * k * inv4k2) / (th * th);
+      (dx ? kvx : kvy)[i] = (T)k;
+      (dx ? exx : exy)[i] = (T)(exp(-k * k * inv4k2) / (th * th));
     }
   }
+  // the plane's own factors along z, also while the loads are in flight
+  const int mz = fft_m(kz, K2), mzp = fft_m((K2 - kz) % K2, K2);
+  const double thz = theta1(mz, K2);
+  const double kz_v = 2.0 * kPi * mz * P.g.a[8];
+  const double exz = exp(-kz_v * kz_v * inv4k2) / (thz * thz);
 #pragma unroll
   for (int k = 0; k < kPlaneLoad; ++k) {
     const int e = threadIdx.x + k * blockDim.x;
@@ -624,10 +632,6 @@ __device__ __forceinline__ void plane_convolve(cplx<T>* __restrict__ spec, int k
   double acc[10];
 #pragma unroll
   for (int i = 0; i < 10; ++i) acc[i] = 0.0;
-  const int mz = fft_m(kz, K2), mzp = fft_m((K2 - kz) % K2, K2);
-  const double thz = theta1(mz, K2);
-  const double kz_v = 2.0 * kPi * mz * P.g.a[8];
-  const double exz = exp(-kz_v * kz_v * inv4k2) / (thz * thz);
   const double pref = 2.0 * kPi / P.g.vol;
   const double w = (kz > 0 && 2 * kz != K2) ? 2.0 : 1.0;
   // diagonal box: the partner m' of m is m with every component negated except
@@ -635,37 +639,58 @@ __device__ __forceinline__ void plane_convolve(cplx<T>* __restrict__ spec, int k
   // s_d = -1 (+1 at Nyquist), and |k(m')| = |k(m)|: the two dE/dA terms of
   // the general case merge into one with factor 1 + s_i s_j
   const bool nyq_z = 2 * kz == K2;
-  for (int e = threadIdx.x; e < K0 * K1; e += blockDim.x) {
-    int ix, iy;
-    divmod_u(e, K0, iy, ix);
-    CT s = buf[iy * pitch + ix];
-    const double S2 = (double)s.x * (double)s.x + (double)s.y * (double)s.y;
-    double Gs;
-    if (P.ortho) {
-      const double kx = kvx[ix], ky = kvy[iy];
-      const double k2 = kx * kx + ky * ky + kz_v * kz_v;
-      double G = 0.0, gp = 0.0;
-      if (k2 != 0.0) {
-        const double ik2 = 1.0 / k2;
-        G = pref * exx[ix] * exy[iy] * exz * ik2;
-        gp = -2.0 * G * (inv4k2 + ik2);
+  if (P.ortho) {
+    // per point in T; a thread's partials over its <= kPlaneLoad points in T,
+    // then float64 across threads and planes
+    T pa[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) pa[i] = T(0);
+    const T kzt = (T)kz_v, kz2 = (T)(kz_v * kz_v), i4k2 = (T)inv4k2, pz = (T)(pref * exz);
+    const T wt = (T)w, c2pi = (T)(2.0 * kPi), fscale = (T)(2.0 * P.eps * P.phi_scale);
+    const T mzt = (T)mz;
+#pragma unroll
+    for (int k = 0; k < kPlaneLoad; ++k) {
+      const int e = threadIdx.x + k * blockDim.x;
+      if (e >= ne) break;
+      int ix, iy;
+      divmod_u(e, K0, iy, ix);
+      CT s = buf[iy * pitch + ix];
+      const T S2 = s.x * s.x + s.y * s.y;
+      const T kx = kvx[ix], ky = kvy[iy];
+      const T k2 = kx * kx + ky * ky + kz2;
+      T G = T(0), gp = T(0);
+      if (k2 != T(0)) {
+        const T ik2 = recip(k2);
+        G = pz * exx[ix] * exy[iy] * ik2;
+        gp = T(-2) * G * (i4k2 + ik2);
       }
-      Gs = G;
-      acc[0] += w * G * S2;
-      const double c = w * S2 * gp * 2.0 * kPi;  // 2 x (0.5 w S2 gp 2 pi)
-      const double m[3] = {(double)fft_m(ix, K0), (double)fft_m(iy, K1), (double)mz};
-      const double kk[3] = {kx, ky, kz_v};
+      pa[0] += wt * G * S2;
+      const T c = wt * S2 * gp * c2pi;  // 2 x (0.5 w S2 gp 2 pi)
+      const T m[3] = {(T)fft_m(ix, K0), (T)fft_m(iy, K1), mzt};
+      const T kk[3] = {kx, ky, kzt};
       const bool ny[3] = {2 * ix == K0, 2 * iy == K1, nyq_z};
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
           if (i != j && ny[i] != ny[j]) continue;  // 1 + s_i s_j = 0
-          const double v = c * kk[j] * m[i];
-          if (P.conv == ADMP_KCONV_REFERENCE) acc[1 + i * 3 + j] += v;
-          else acc[1 + j * 3 + i] += v;
+          const T v = c * kk[j] * m[i];
+          if (P.conv == ADMP_KCONV_REFERENCE) pa[1 + i * 3 + j] += v;
+          else pa[1 + j * 3 + i] += v;
         }
-    } else {
+      const T f = fscale * G;
+      s.x *= f;
+      s.y *= f;
+      buf[iy * pitch + ix] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < 10; ++i) acc[i] = (double)pa[i];
+  } else {
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+      int ix, iy;
+      divmod_u(e, K0, iy, ix);
+      CT s = buf[iy * pitch + ix];
+      const double S2 = (double)s.x * (double)s.x + (double)s.y * (double)s.y;
       const int ixp = ix ? K0 - ix : 0, iyp = iy ? K1 - iy : 0;  // index of -m
       const int mx = fft_m(ix, K0), my = fft_m(iy, K1);
       const int mxp = fft_m(ixp, K0), myp = fft_m(iyp, K1);
@@ -675,15 +700,15 @@ __device__ __forceinline__ void plane_convolve(cplx<T>* __restrict__ spec, int k
       kvec(P, mxp, myp, mzp, kvp);
       const double G = gfun(P, kv, th, &gp);
       const double Gp = gfun(P, kvp, th, &gpp);
-      Gs = 0.5 * (G + Gp);
+      const double Gs = 0.5 * (G + Gp);
       acc[0] += w * Gs * S2;
       add_dEdA(acc, P, mx, my, mz, kv, gp, 0.5 * w * S2);
       add_dEdA(acc, P, mxp, myp, mzp, kvp, gpp, 0.5 * w * S2);
+      const T f = (T)(2.0 * P.eps * P.phi_scale * Gs);
+      s.x *= f;
+      s.y *= f;
+      buf[iy * pitch + ix] = s;
     }
-    const T f = (T)(2.0 * P.eps * P.phi_scale * Gs);
-    s.x *= f;
-    s.y *= f;
-    buf[iy * pitch + ix] = s;
   }
   __syncthreads();
   stockham_inplace<T, 1>(buf, K0, K1, 1, pitch, rx, twx);

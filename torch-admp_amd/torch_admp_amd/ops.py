@@ -187,6 +187,43 @@ class PMEReciprocal(torch.autograd.Function):
         return ((gpos * ge).to(pdtype), grad_box, (gq * ge), None, None, None, None, None)
 
 
+# Host state reused across fused calls: per problem shape the workspace size and
+# the ctypes box and mesh arrays, and one workspace per (device, stream).  Calls
+# on one stream run in order, so reusing that stream's workspace is safe; the
+# caching allocator keeps a workspace made on one stream for that stream.
+_FUSED_SHAPES: dict = {}
+_FUSED_WS: dict = {}
+_FUSED_MAX_ENTRIES = 256
+
+
+def _as_f64(t: torch.Tensor) -> torch.Tensor:
+    if t.dtype == torch.float64 and t.is_contiguous():
+        return t
+    return _f64(t)
+
+
+def _fused_shape(lib, n: int, box: torch.Tensor, rcut: float, kmesh, prec: int):
+    """(workspace bytes, ctypes box, ctypes mesh) of one problem shape, cached."""
+    if box.device.type == "cpu" and box.dtype == torch.float64:
+        boxv = tuple(box.detach().reshape(9).tolist())
+    else:
+        boxv = tuple(box.detach().reshape(9).to("cpu", torch.float64).tolist())
+    km = tuple(int(k) for k in kmesh)
+    key = (n, boxv, float(rcut), km, prec)
+    hit = _FUSED_SHAPES.get(key)
+    if hit is None:
+        boxh = (ctypes.c_double * 9)(*boxv)
+        kmh = (ctypes.c_int32 * 3)(*km)
+        nbytes = ctypes.c_size_t()
+        _lib.check(lib.admp_pme_fused_workspace_bytes(n, boxh, rcut, kmh, prec,
+                                                      ctypes.byref(nbytes)),
+                   "admp_pme_fused_workspace_bytes")
+        if len(_FUSED_SHAPES) >= _FUSED_MAX_ENTRIES:
+            _FUSED_SHAPES.clear()
+        hit = _FUSED_SHAPES[key] = (nbytes.value, boxh, kmh)
+    return hit
+
+
 def pme_fused(positions: torch.Tensor, box: torch.Tensor, charges: torch.Tensor, rcut: float,
               kappa: float, kmesh: Sequence[int], precision: str = "float64",
               convention: str = "reference", dielectric: float = DIELECTRIC,
@@ -196,29 +233,34 @@ def pme_fused(positions: torch.Tensor, box: torch.Tensor, charges: torch.Tensor,
     Returns (energy[5] = real, reciprocal, self, non-neutral, total; forces
     (N, 3); dE/dbox (3, 3)), all float64 on the device.  Matches
     CoulombForceModule on the TorchNeighborList pair list followed by
-    calc_grads (src/torch_admp/pme.py:261-310).
+    calc_grads (src/torch_admp/pme.py:261-310).  Without `workspace` the call
+    reuses a workspace kept for the current stream.
     """
     _lib.require_gpu(positions, charges)
     lib = _lib.load()
-    pos = _f64(positions)
-    q = _f64(charges).reshape(-1)
+    pos = _as_f64(positions)
+    q = _as_f64(charges)
+    if q.dim() != 1:
+        q = q.reshape(-1)
     n = pos.shape[0]
     dev = pos.device
-    boxh = _lib.box_array(box)
-    km = _lib.kmesh_array(kmesh)
     prec = _lib.PRECISION[precision]
-    nbytes = ctypes.c_size_t()
-    _lib.check(lib.admp_pme_fused_workspace_bytes(n, boxh, rcut, km, prec, ctypes.byref(nbytes)),
-               "admp_pme_fused_workspace_bytes")
-    if workspace is None or workspace.numel() < nbytes.value:
-        workspace = _lib.workspace(nbytes.value, dev)
-    energy = torch.empty(5, dtype=torch.float64, device=dev)
-    forces = torch.empty(n, 3, dtype=torch.float64, device=dev)
-    dedbox = torch.empty(3, 3, dtype=torch.float64, device=dev)
-    _lib.check(lib.admp_pme_fused(_lib.ptr(pos), _lib.ptr(q), n, boxh, rcut, kappa, km, prec,
-                                  _lib.KCONV[convention], dielectric, _lib.ptr(energy),
-                                  _lib.ptr(forces), _lib.ptr(dedbox), _lib.ptr(workspace),
-                                  nbytes.value, _lib.stream_of(pos)), "admp_pme_fused")
+    nbytes, boxh, kmh = _fused_shape(lib, n, box, rcut, kmesh, prec)
+    stream = torch._C._cuda_getCurrentRawStream(dev.index)
+    if workspace is None or workspace.numel() < nbytes:
+        wkey = (dev.index, stream)
+        ws = _FUSED_WS.get(wkey)
+        if ws is None or ws.numel() < nbytes:
+            if ws is None and len(_FUSED_WS) >= _FUSED_MAX_ENTRIES:
+                _FUSED_WS.clear()
+            ws = _FUSED_WS[wkey] = _lib.workspace(nbytes, dev)
+        workspace = ws
+    out = torch.empty(14 + 3 * n, dtype=torch.float64, device=dev)
+    energy, dedbox, forces = out[:5], out[5:14].view(3, 3), out[14:].view(n, 3)
+    _lib.check(lib.admp_pme_fused(pos.data_ptr(), q.data_ptr(), n, boxh, rcut, kappa, kmh, prec,
+                                  _lib.KCONV[convention], dielectric, energy.data_ptr(),
+                                  forces.data_ptr(), dedbox.data_ptr(), workspace.data_ptr(),
+                                  nbytes, stream), "admp_pme_fused")
     return energy, forces, dedbox
 
 

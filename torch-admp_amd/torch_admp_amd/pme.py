@@ -65,6 +65,7 @@ class CoulombForceModule(BaseForceModule):
         else:
             self.kmesh = None
         self._kmesh = torch.zeros(3, dtype=torch.long)
+        self._kmesh_list: Optional[List[int]] = None
         if spacing is not None:
             if isinstance(spacing, float):
                 spacing = [spacing, spacing, spacing]
@@ -98,7 +99,7 @@ class CoulombForceModule(BaseForceModule):
     # -- mesh size rule (src/torch_admp/pme.py:380-392) -------------------------
     def mesh_for_box(self, box_frame: torch.Tensor) -> List[int]:
         if self.kmesh is not None:
-            return [int(k) for k in self.kmesh.tolist()]
+            return self.kmesh.tolist()
         diag = torch.diagonal(box_frame.detach().reshape(3, 3)).to("cpu", torch.float64)
         if self.spacing is not None:
             k = torch.ceil(diag / self.spacing)
@@ -145,6 +146,7 @@ class CoulombForceModule(BaseForceModule):
                                            _lib.PRECISION[self.precision],
                                            _lib.KCONV[self.convention]))
             self._kmesh = torch.tensor(km, dtype=torch.long)
+            self._kmesh_list = None
         return torch.stack(out)
 
     def _forward_pbc_self(self, charges):
@@ -194,15 +196,20 @@ class CoulombForceModule(BaseForceModule):
         if lc != 1.0:
             positions, box = positions * lc, box * lc
         km = self.mesh_for_box(box)
-        self._kmesh = torch.tensor(km, dtype=torch.long)
+        if km != self._kmesh_list:
+            self._kmesh = torch.tensor(km, dtype=torch.long)
+            self.__dict__["_kmesh_list"] = km
         e, f, gbox = pme_fused(positions, box, charges, self.rcut * lc, float(self.kappa), km,
                                self.precision, self.convention, self.const_lib.dielectric,
                                workspace)
         if ec != 1.0 or lc != 1.0:
             e, f, gbox = e / ec, f * (lc / ec), gbox * (lc / ec)
-        self.real_energy, self.reciprocal_energy = e[0:1], e[1:2]
-        self.self_energy, self.non_neutral_energy = e[2:3], e[3:4]
-        return e[4], f, gbox
+        # plain attributes: written into __dict__ directly, skipping nn.Module's
+        # parameter/buffer checks on this per-step path
+        d = self.__dict__
+        (d["real_energy"], d["reciprocal_energy"], d["self_energy"], d["non_neutral_energy"],
+         total) = e.split(1)
+        return total.view(()), f, gbox
 
 
 def setup_ewald_parameters(rcut: float, box: Union[torch.Tensor, np.ndarray, None] = None,

@@ -215,6 +215,7 @@ class QEqForceModule(BaseForceModule):
             pr.n_cons = A.shape[0]
         if box_int is not None and (flags & QEQ_KSPACE):
             coul._kmesh = torch.tensor(km, dtype=torch.long)
+            coul._kmesh_list = None
         lib = _lib.load()
         nbytes = ctypes.c_size_t()
         _lib.check(lib.admp_qeq_workspace_bytes(ctypes.byref(pr), ctypes.byref(nbytes)),
