@@ -131,6 +131,22 @@ __device__ __forceinline__ double dpp_move(double v) {
   const int lo = dpp_move<CTRL, ROWS>((int)b), hi = dpp_move<CTRL, ROWS>((int)(b >> 32));
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(dpp_move<CTRL, ROWS>(__float_as_int(v)));
+}
+
+// Sum over each aligned group of 8 lanes, in every lane of the group: pairs
+// (quad_perm 1,0,3,2), quads (2,3,0,1), then lane i adds lane 7 - i of its half
+// row (row_half_mirror).
+template <typename T>
+__device__ __forceinline__ T group8_sum(T v) {
+  v += dpp_move<0xb1, 0xf>(v);
+  v += dpp_move<0x4e, 0xf>(v);
+  v += dpp_move<0x141, 0xf>(v);
+  return v;
+}
+
 __device__ __forceinline__ double read_lane(double v, int lane) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)b, lane);

@@ -195,13 +195,15 @@ __global__ void __launch_bounds__(kSpreadThreads) k_b_spread_tiles(int nsys,
                                                            const double* __restrict__ qv,
                                                            const int* __restrict__ sorted,
                                                            const int* __restrict__ start,
+                                                           int4* __restrict__ stm,
+                                                           T* __restrict__ stw,
                                                            T* __restrict__ mesh) {
   const int64_t c = blockIdx.x;
   const SysDesc* D = tab + find_seg(cell_off, nsys, c);
   if (!D->tg.ok) return;
   const int lc = (int)(c - D->cell0);
   spread_cell_tile<T>(D->g, D->tg, lc, start[c * kSub], start[(c + 1) * kSub], sorted, pos, qv,
-                      mesh + D->mesh0);
+                      mesh + D->mesh0, stm, stw);
 }
 
 // atoms of systems whose tiles do not fit: global atomics into their mesh
@@ -211,14 +213,20 @@ __global__ void __launch_bounds__(kBlock) k_b_spread_atoms(int64_t ntot,
                                                            const SysDesc* __restrict__ tab,
                                                            const double* __restrict__ pos,
                                                            const double* __restrict__ qv,
+                                                           int4* __restrict__ stm,
+                                                           T* __restrict__ stw,
                                                            T* __restrict__ mesh) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t i = t >> 3;
   const int a = (int)(t & 7);
-  if (i >= ntot || a >= kOrder) return;
+  if (i >= ntot || a == 7) return;
   const SysDesc* D = tab + sys_of[i];
   if (D->tg.ok) return;
-  spread_lane<T>(D->g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], qv[i], a, mesh + D->mesh0);
+  if (a == 6)  // the stencil record for the gather
+    stencil_record<T>(D->g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], stm + i,
+                      stw + (int64_t)kStencilVals * i);
+  else
+    spread_lane<T>(D->g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], qv[i], a, mesh + D->mesh0);
 }
 
 // ------------------------------------------------------------------ FFT passes
@@ -327,6 +335,8 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __
                                                      const double* __restrict__ qv,
                                                      const T* __restrict__ meshes,
                                                      const double* __restrict__ grad,
+                                                     const int4* __restrict__ stm,
+                                                     const T* __restrict__ stw,
                                                      double* __restrict__ forces,
                                                      double* __restrict__ partials, int64_t nrows) {
   __shared__ double red[9 * (kBlock / 64)];
@@ -346,14 +356,12 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __
     x = pos[3 * atom];
     y = pos[3 * atom + 1];
     z = pos[3 * atom + 2];
-    gather_lane<T>(g, x, y, z, a, meshes + D->mesh0, &sx, &sy, &sz, &sp);
+    gather_lane_rec<T>(g.k[0], g.k[1], g.k[2], stm[atom], stw + (int64_t)kStencilVals * atom, a,
+                       meshes + D->mesh0, &sx, &sy, &sz, &sp);
   }
-#pragma unroll
-  for (int o = 1; o < 8; o <<= 1) {
-    sx += __shfl_xor(sx, o, 64);
-    sy += __shfl_xor(sy, o, 64);
-    sz += __shfl_xor(sz, o, 64);
-  }
+  sx = group8_sum(sx);
+  sy = group8_sum(sy);
+  sz = group8_sum(sz);
   if (valid && a < 3) {
     // lane a (< 3) owns force component a
     const double qi = qv[atom];
@@ -583,6 +591,8 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
   P.o_nb = take(P.nb.bytes);
   P.o_sys_of = take(sizeof(int) * std::max<int64_t>(P.ntot, 1));
   P.o_ps = take(sizeof(P4<double>) * std::max<int64_t>(P.ntot, 1));
+  P.o_stm = take(sizeof(int4) * std::max<int64_t>(P.ntot, 1));
+  P.o_stw = take(es * kStencilVals * std::max<int64_t>(P.ntot, 1));
   P.o_grad = take(sizeof(double) * 3 * std::max<int64_t>(P.ntot, 1));
   P.o_mesh = take(es * std::max<int64_t>(P.mtot, 1));
   P.o_spec = take(ces * std::max<int64_t>(P.stot, 1));
@@ -697,6 +707,8 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   T* mesh = (T*)(ws + P.o_mesh);
   CT* spec = (CT*)(ws + P.o_spec);
   P4<double>* ps = (P4<double>*)(ws + P.o_ps);
+  int4* stm = (int4*)(ws + P.o_stm);
+  T* stw = (T*)(ws + P.o_stw);
   int* tickets = (int*)(ws + P.o_tickets);
   const unsigned nb_atoms = (unsigned)std::max<int64_t>((ntot + kBlock - 1) / kBlock, 1);
   const int64_t rows_real = P.ctot * P.nsplit;
@@ -755,10 +767,10 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   if (ntot > 0) {
     if (P.lds_tile > 0)
       k_b_spread_tiles<T><<<(unsigned)P.ctot, kSpreadThreads, P.lds_tile, s>>>(
-          nsys, seg(kSegCell), tab, pos, q, (const int*)(nbws + L.sorted), start, mesh);
+          nsys, seg(kSegCell), tab, pos, q, (const int*)(nbws + L.sorted), start, stm, stw, mesh);
     if (P.any_untiled)
       k_b_spread_atoms<T><<<(unsigned)((ntot * 8 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-          ntot, (const int*)(ws + P.o_sys_of), tab, pos, q, mesh);
+          ntot, (const int*)(ws + P.o_sys_of), tab, pos, q, stm, stw, mesh);
   }
   if (join && !launch_real()) return ADMP_ERR_HIP;
   mark(3);
@@ -783,7 +795,7 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   double* gpart = (double*)(ws + P.o_gpart);
   if (rows_g > 0)
     k_b_gather<T><<<(unsigned)rows_g, kBlock, 0, s>>>(nsys, seg(kSegGather), tab, pos, q, mesh,
-                                                      grad, forces, gpart, rows_g);
+                                                      grad, stm, stw, forces, gpart, rows_g);
   mark(5);
   // ---- per-system reductions and outputs
   PartialSets sets;

@@ -48,8 +48,8 @@ struct BatchPlan {
   NbLayout nb;         // binning workspace over all cells
   CellGrid cg_all;     // only ncell is used by the shared binning helpers
   // workspace byte offsets
-  size_t o_tab, o_off, o_nb, o_sys_of, o_ps, o_grad, o_mesh, o_spec, o_rpart, o_cpart,
-      o_gpart, o_red, o_tickets, bytes;
+  size_t o_tab, o_off, o_nb, o_sys_of, o_ps, o_stm, o_stw, o_grad, o_mesh, o_spec, o_rpart,
+      o_cpart, o_gpart, o_red, o_tickets, bytes;
   int status;          // ADMP_OK or an error code from planning
 };
 
