@@ -37,6 +37,15 @@ __device__ __forceinline__ int find_seg(const int64_t* __restrict__ off, int nse
 }
 
 // ------------------------------------------------------------------ prologue
+// Zero `bytes` (a multiple of 4) at p (16-byte aligned) with 16-byte stores.
+__device__ __forceinline__ void zero_words(void* p, int64_t bytes, int64_t t0, int64_t step) {
+  int4* p4 = reinterpret_cast<int4*>(p);
+  const int64_t n4 = bytes >> 4;
+  for (int64_t i = t0; i < n4; i += step) p4[i] = make_int4(0, 0, 0, 0);
+  int* p1 = reinterpret_cast<int*>(p);
+  for (int64_t i = (n4 << 2) + t0; i < (bytes >> 2); i += step) p1[i] = 0;
+}
+
 // Zero the bin counters and scatter cursors, the meshes, the real-space
 // gradient accumulators and the per-system reduction tickets; for a batch of one also write the system table and the
 // segment offsets from the kernel arguments.
@@ -55,15 +64,21 @@ __global__ void __launch_bounds__(kBlock) k_b_prologue(int inline_table, OneSyst
                                                        int* __restrict__ tickets, int ntickets) {
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t step = (int64_t)gridDim.x * blockDim.x;
-  if (inline_table && t0 == 0) tab[0] = one.d;
-  if (inline_table && t0 < kSegCount) {
-    offs[2 * t0] = 0;
-    offs[2 * t0 + 1] = one.tot[t0];
+  if (inline_table) {
+    // the table word by word from the kernel arguments
+    static_assert(sizeof(SysDesc) % 4 == 0, "SysDesc is a whole number of words");
+    const int* src = reinterpret_cast<const int*>(&one.d);
+    int* dst = reinterpret_cast<int*>(tab);
+    for (int64_t i = t0; i < (int64_t)(sizeof(SysDesc) / 4); i += step) dst[i] = src[i];
+    if (t0 < kSegCount) {
+      offs[2 * t0] = 0;
+      offs[2 * t0 + 1] = one.tot[t0];
+    }
   }
-  for (int64_t i = t0; i < ncount; i += step) count[i] = 0;
-  for (int64_t i = t0; i < nmesh; i += step) mesh[i] = T(0);
-  for (int64_t i = t0; i < ngrad; i += step) grad[i] = 0.0;
-  for (int64_t i = t0; i < ntickets; i += step) tickets[i] = 0;
+  zero_words(count, (int64_t)sizeof(int) * ncount, t0, step);
+  zero_words(mesh, (int64_t)sizeof(T) * nmesh, t0, step);
+  zero_words(grad, (int64_t)sizeof(double) * ngrad, t0, step);
+  zero_words(tickets, (int64_t)sizeof(int) * ntickets, t0, step);
 }
 
 // ------------------------------------------------------------------ binning
@@ -128,21 +143,28 @@ __global__ void __launch_bounds__(kBlock) k_b_scatter_pack(int64_t ntot, int nco
     int sum = 0;
     for (int k = 0; k < per; ++k)
       if (base + k < ncount) sum += count[base + k];
-    part[threadIdx.x] = sum;
+    // inclusive scan of the per-thread sums: DPP within each wavefront, then
+    // the wavefront totals
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int inc = sum;
+    inc += dpp_move<0x111, 0xf>(inc);  // row_shr:1
+    inc += dpp_move<0x112, 0xf>(inc);  // row_shr:2
+    inc += dpp_move<0x114, 0xf>(inc);  // row_shr:4
+    inc += dpp_move<0x118, 0xf>(inc);  // row_shr:8
+    inc += dpp_move<0x142, 0xa>(inc);  // row_bcast:15 into rows 1 and 3
+    inc += dpp_move<0x143, 0xc>(inc);  // row_bcast:31 into rows 2 and 3
+    if (lane == 63) part[wid] = inc;
     __syncthreads();
-    for (int off = 1; off < kBlock; off <<= 1) {
-      const int add = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
-      __syncthreads();
-      part[threadIdx.x] += add;
-      __syncthreads();
-    }
-    int run = part[threadIdx.x] - sum;
+    for (int w = 0; w < wid; ++w) inc += part[w];
+    if (threadIdx.x == kBlock - 1) part[kBlock / 64] = inc;
+    int run = inc - sum;
     for (int k = 0; k < per; ++k)
       if (base + k < ncount) {
         st[base + k] = run;
         run += count[base + k];
       }
-    if (threadIdx.x == kBlock - 1) st[ncount] = part[kBlock - 1];
+    __syncthreads();
+    if (threadIdx.x == 0) st[ncount] = part[kBlock / 64];
     __syncthreads();
     if (blockIdx.x == 0)
       for (int k = threadIdx.x; k <= ncount; k += blockDim.x) start_out[k] = st[k];
@@ -718,7 +740,7 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   // ---- prologue, binning, in-cell order
   const int64_t ncount = 2 * (P.ctot * kSub + 1);  // counters and scatter cursors
   const int64_t nfill = std::max(std::max(ncount, P.mtot), 3 * ntot);
-  k_b_prologue<T><<<(unsigned)std::min<int64_t>((nfill + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
+  k_b_prologue<T><<<(unsigned)std::min<int64_t>((nfill / 4 + kBlock) / kBlock, 8192), kBlock, 0, s>>>(
       nsys == 1, one, tab, offs, count, ncount, mesh, P.mtot, grad, 3 * ntot, tickets, nsys);
   if (ntot > 0)
     k_b_bin<<<nb_atoms, kBlock, 0, s>>>(ntot, nsys, seg(kSegAtom), tab, pos, (double*)(nbws + L.w),
