@@ -204,6 +204,31 @@ def test_fused_energy_forces(dev, precision, etol, ftol, box, kmesh, n):
     assert float((gb.cpu() - gb_ref).abs().max()) <= 100 * etol * float(gb_ref.abs().max())
 
 
+def test_fused_two_streams_in_flight(dev):
+    """Without a workspace argument the fused call keeps one workspace per
+    stream: calls in flight on two streams at once each give their own result
+    (float64; 1e-12, the order of the gradient atomics aside)."""
+    pos_a, q_a, box = _system(1200, np.eye(3) * 24.0, 21)
+    pos_b, q_b, _ = _system(1200, np.eye(3) * 24.0, 22)
+    args = (5.5, 0.35, [32, 30, 27], "float64")
+    pa, qa, pb, qb = (t.to(dev) for t in (pos_a, q_a, pos_b, q_b))
+    ref_a = ops.pme_fused(pa, box, qa, *args)
+    ref_b = ops.pme_fused(pb, box, qb, *args)
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(dev), torch.cuda.Stream(dev)
+    outs = []
+    for _ in range(3):
+        with torch.cuda.stream(s1):
+            r1 = ops.pme_fused(pa, box, qa, *args)
+        with torch.cuda.stream(s2):
+            r2 = ops.pme_fused(pb, box, qb, *args)
+        outs.append((r1, r2))
+    torch.cuda.synchronize()
+    for r1, r2 in outs:
+        for got, want in list(zip(r1, ref_a)) + list(zip(r2, ref_b)):
+            assert torch.allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
 def test_module_batched_frames(dev):
     frames = [_system(120, np.eye(3) * 12.0, s) for s in (11, 12, 13)]
     pos = torch.stack([f[0] for f in frames])
