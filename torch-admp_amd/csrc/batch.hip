@@ -373,8 +373,11 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __
 #pragma unroll
   for (int i = 0; i < 9; ++i) acc[i] = 0.0;
   T sx = 0, sy = 0, sz = 0, sp = 0;
-  double x = 0.0, y = 0.0, z = 0.0;
+  double x = 0.0, y = 0.0, z = 0.0, qi = 0.0, g_own = 0.0;
   if (valid) {
+    // loads that do not depend on the mesh sums, issued first
+    qi = qv[atom];
+    if (a < 3) g_own = grad[3 * atom + a];
     x = pos[3 * atom];
     y = pos[3 * atom + 1];
     z = pos[3 * atom + 2];
@@ -386,12 +389,11 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __
   sz = group8_sum(sz);
   if (valid && a < 3) {
     // lane a (< 3) owns force component a
-    const double qi = qv[atom];
     const double dR[3] = {-qi * (double)sx, -qi * (double)sy, -qi * (double)sz};
     double gj = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) gj += g.k[d] * g.a[a * 3 + d] * dR[d];
-    gj += grad[3 * atom + a];
+    gj += g_own;
     forces[3 * atom + a] = -gj;
     if (a == 0) {
       const double r[3] = {x, y, z};
