@@ -96,3 +96,20 @@ def test_batch_plan_on_host():
         ops.PMEBatchPlan([10, 20], [np.eye(3) * 10.0] * 2, [6.0], 0.4, [8, 8, 8])
     with pytest.raises(RuntimeError, match="ROCm GPU"):
         one.run(torch.zeros(300, 3, dtype=torch.float64), torch.zeros(300, dtype=torch.float64))
+
+
+def test_host_extension_binds_and_rejects_host_tensors():
+    """The native binding of the fused call loads, resolves its C-ABI entry
+    points in the HIP library and refuses host tensors before any GPU call."""
+    import torch
+
+    from torch_admp_amd import ops
+
+    ext = ops._host_ext()
+    if ext is None:
+        pytest.skip("host extension not built")
+    pos = torch.zeros(4, 3, dtype=torch.float64)
+    q = torch.zeros(4, dtype=torch.float64)
+    box = torch.eye(3, dtype=torch.float64) * 10.0
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        ext.pme_fused(pos, q, box, 4.0, 0.3, 8, 8, 8, 0, 0, 138.935, 0)

@@ -196,6 +196,27 @@ _FUSED_WS: dict = {}
 _FUSED_MAX_ENTRIES = 256
 
 
+_EXT = None
+_EXT_TRIED = False
+
+
+def _host_ext():
+    """The native binding of the fused call (host_ext.cpp), bound to the HIP
+    library in use, or None when it has not been built."""
+    global _EXT, _EXT_TRIED
+    if not _EXT_TRIED:
+        _EXT_TRIED = True
+        try:
+            from . import _host_ext as ext
+        except ImportError:
+            ext = None
+        if ext is not None:
+            _lib.load()
+            ext.bind(str(_lib.library_path()))
+        _EXT = ext
+    return _EXT
+
+
 def _as_f64(t: torch.Tensor) -> torch.Tensor:
     if t.dtype == torch.float64 and t.is_contiguous():
         return t
@@ -234,14 +255,24 @@ def pme_fused(positions: torch.Tensor, box: torch.Tensor, charges: torch.Tensor,
     (N, 3); dE/dbox (3, 3)), all float64 on the device.  Matches
     CoulombForceModule on the TorchNeighborList pair list followed by
     calc_grads (src/torch_admp/pme.py:261-310).  Without `workspace` the call
-    reuses a workspace kept for the current stream.
+    goes through the native binding (host_ext.cpp) when it is built, and reuses
+    a workspace kept for the current stream in either case.
     """
     _lib.require_gpu(positions, charges)
-    lib = _lib.load()
     pos = _as_f64(positions)
     q = _as_f64(charges)
     if q.dim() != 1:
         q = q.reshape(-1)
+    ext = _host_ext()
+    if ext is not None and workspace is None:
+        if box.device.type != "cpu" or box.dtype != torch.float64 or not box.is_contiguous():
+            box = box.detach().to("cpu", torch.float64).contiguous()
+        k0, k1, k2 = (int(k) for k in kmesh)
+        e, f, g = ext.pme_fused(pos, q, box, float(rcut), float(kappa), k0, k1, k2,
+                                _lib.PRECISION[precision], _lib.KCONV[convention],
+                                float(dielectric), torch._C._cuda_getCurrentRawStream(pos.device.index))
+        return e, f, g
+    lib = _lib.load()
     n = pos.shape[0]
     dev = pos.device
     prec = _lib.PRECISION[precision]
