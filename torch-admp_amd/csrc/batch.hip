@@ -186,8 +186,11 @@ __global__ void __launch_bounds__(kBlock) k_b_scatter_pack(int64_t ntot, int nco
 // One work-group per (global cell, candidate split).  Partials are stored
 // value-major, rows cell-major (row = cell * splits + split), so that a
 // system's rows are contiguous and the reduction reads them coalesced.
+// The register cap keeps room on each SIMD for the r2c and FFT wavefronts
+// that run beside this kernel on the other stream: at 117 registers r2c ran
+// 28 us beside it instead of 14.
 template <typename T>
-__global__ void __launch_bounds__(kRealBlock) k_b_real(int nsys,
+__global__ void __launch_bounds__(kRealBlock) __attribute__((amdgpu_num_vgpr(104))) k_b_real(int nsys,
                                                    const int64_t* __restrict__ cell_off,
                                                    const SysDesc* __restrict__ tab,
                                                    const P4<double>* __restrict__ ps,

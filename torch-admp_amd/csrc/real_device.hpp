@@ -278,6 +278,9 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
   __shared__ int st_beg[kMaxStencil];
   __shared__ double st_sh[kMaxStencil * 3];
   __shared__ double red[kRealNV * (kRealBlock / 64)];
+  __shared__ int cslot[kCandCap];            // sorted slot of each staged candidate
+  __shared__ int wcnt[kRealBlock / 64];      // staged candidates per wavefront
+  static_assert(kCandCap == kRealBlock, "one candidate of a chunk per thread");
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ci[3] = {lc / (cg.nc[1] * cg.nc[2]), (lc / cg.nc[2]) % cg.nc[1], lc % cg.nc[2]};
@@ -293,6 +296,13 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
     o3[d] = o;
   }
   const double ox = o3[0], oy = o3[1], oz = o3[2];
+  // rectangular cells: candidates farther than rc from the home cell's box
+  // pair with no home atom and are not staged (home atoms lie in the box up to
+  // the binning's rounding; 1e-4 A of slack)
+  const bool prune = cg.h[1] == 0.0 && cg.h[2] == 0.0 && cg.h[3] == 0.0 && cg.h[5] == 0.0 &&
+                     cg.h[6] == 0.0 && cg.h[7] == 0.0;
+  const double ex = cg.h[0] / cg.nc[0], ey = cg.h[4] / cg.nc[1], ez = cg.h[8] / cg.nc[2];
+  const double prune_r = sqrt(cg.rc2) + 1e-4, prune_r2 = prune_r * prune_r;
   const int64_t c = cell0 + lc;
   const int hb = start[c * kSub], he = start[(c + 1) * kSub];
   const T rc2 = (T)cg.rc2, rmin2 = (T)1e-4;
@@ -329,22 +339,48 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
       const int ctot = st_pre[m];
       const int cs = ctot > split ? (ctot - split + S - 1) / S : 0;
       for (int k0 = 0; k0 < cs; k0 += kCandCap) {
-        const int cc = min(kCandCap, cs - k0);
-        for (int kk = threadIdx.x; kk < cc; kk += blockDim.x) {
-          const int kg = split + S * (k0 + kk);
-          int lo = 0, hi = m;  // last entry with st_pre[entry] <= kg
+        // stage the chunk's candidates (thread kk takes candidate kk), keeping
+        // those that can reach the home cell, compacted in candidate order
+        const int kk = threadIdx.x;
+        bool keep = false;
+        int slot = 0, kg = 0;
+        double sx = 0.0, sy = 0.0, sz = 0.0, sq = 0.0;
+        if (kk < min(kCandCap, cs - k0)) {
+          kg = split + S * (k0 + kk);
+          int lo = 0, hi = m;
           while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if (st_pre[mid] <= kg) lo = mid;
             else hi = mid;
           }
-          const int slot = st_beg[lo] + (kg - st_pre[lo]);
+          slot = st_beg[lo] + (kg - st_pre[lo]);
           const P4<double> p = ps[slot];
+          sx = (p.x + st_sh[lo * 3]) - ox;
+          sy = (p.y + st_sh[lo * 3 + 1]) - oy;
+          sz = (p.z + st_sh[lo * 3 + 2]) - oz;
+          sq = p.q;
+          keep = true;
+          if (prune) {
+            const double dx = fmax(0.0, fmax(-sx, sx - ex));
+            const double dy = fmax(0.0, fmax(-sy, sy - ey));
+            const double dz = fmax(0.0, fmax(-sz, sz - ez));
+            keep = dx * dx + dy * dy + dz * dz < prune_r2;
+          }
+        }
+        const unsigned long long kmask = __ballot(keep);
+        if (lane == 0) wcnt[wave] = __popcll(kmask);
+        __syncthreads();
+        int cc = 0, at = __popcll(kmask & ((1ull << lane) - 1ull));
+#pragma unroll
+        for (int w = 0; w < kRealBlock / 64; ++w) {
+          at += w < wave ? wcnt[w] : 0;
+          cc += wcnt[w];
+        }
+        if (keep) {
           // centre candidates pair only with home atoms of lower slot
-          cand[kk] = stage<T>((p.x + st_sh[lo * 3]) - ox, (p.y + st_sh[lo * 3 + 1]) - oy,
-                              (p.z + st_sh[lo * 3 + 2]) - oz, p.q,
-                              kg < ncentre ? slot : INT_MAX);
-          cacc[3 * kk] = cacc[3 * kk + 1] = cacc[3 * kk + 2] = 0.0;
+          cand[at] = stage<T>(sx, sy, sz, sq, kg < ncentre ? slot : INT_MAX);
+          cslot[at] = slot;
+          cacc[3 * at] = cacc[3 * at + 1] = cacc[3 * at + 2] = 0.0;
         }
         __syncthreads();
         // per-wavefront ring of (home, candidate) pairs, entries [qh, qt); it runs
@@ -365,7 +401,7 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
 #pragma unroll
             for (int u = 0; u < kFilterUnroll; ++u) {
               kks[u] = ph + nph * wave + stride * (j0 + u);
-              cw[u] = cand[min(kks[u], cc - 1)].hi;
+              cw[u] = cand[min(kks[u], max(cc, 1) - 1)].hi;
             }
             bool pred[kFilterUnroll];
 #pragma unroll
@@ -406,17 +442,10 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
         wave_sync();
         __syncthreads();
         // candidate gradients of this chunk
-        for (int kk = threadIdx.x; kk < cc; kk += blockDim.x) {
-          const double g[3] = {cacc[3 * kk], cacc[3 * kk + 1], cacc[3 * kk + 2]};
+        for (int kc = threadIdx.x; kc < cc; kc += blockDim.x) {
+          const double g[3] = {cacc[3 * kc], cacc[3 * kc + 1], cacc[3 * kc + 2]};
           if (g[0] == 0.0 && g[1] == 0.0 && g[2] == 0.0) continue;
-          const int kg = split + S * (k0 + kk);
-          int lo = 0, hi = m;
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (st_pre[mid] <= kg) lo = mid;
-            else hi = mid;
-          }
-          const int j = sorted[st_beg[lo] + (kg - st_pre[lo])];
+          const int j = sorted[cslot[kc]];
           unsafeAtomicAdd(grad + 3 * j, g[0]);
           unsafeAtomicAdd(grad + 3 * j + 1, g[1]);
           unsafeAtomicAdd(grad + 3 * j + 2, g[2]);
