@@ -169,7 +169,7 @@ __device__ void pass_generic(const cplx<T>* __restrict__ a, cplx<T>* __restrict_
 }
 
 template <typename T, int SIGN>
-__device__ cplx<T>* stockham(cplx<T>* a, cplx<T>* b, int L, int C, const Radix& rd,
+__device__ __forceinline__ cplx<T>* stockham(cplx<T>* a, cplx<T>* b, int L, int C, const Radix& rd,
                              const cplx<T>* tw) {
   int p = 1;
   for (int s = 0; s < rd.n; ++s) {
