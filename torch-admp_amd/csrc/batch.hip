@@ -196,13 +196,12 @@ __global__ void __launch_bounds__(kRealBlock) __attribute__((amdgpu_num_vgpr(104
                                                    const P4<double>* __restrict__ ps,
                                                    const int* __restrict__ sorted,
                                                    const int* __restrict__ start,
-                                                   const double* __restrict__ pos,
                                                    double* __restrict__ grad,
                                                    double* __restrict__ partials) {
   const int64_t c = blockIdx.x;
   const SysDesc* D = tab + find_seg(cell_off, nsys, c);
   real_cell_body<T>(D->cg, D->cell0, (int)(c - D->cell0), (int)blockIdx.y, (int)gridDim.y,
-                    D->kappa, (T)D->eps, ps, sorted, start, pos, grad,
+                    D->kappa, (T)D->eps, ps, sorted, start, grad,
                     partials + c * gridDim.y + blockIdx.y, (int64_t)gridDim.x * gridDim.y);
 }
 
@@ -364,7 +363,7 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __
                                                      const T* __restrict__ stw,
                                                      double* __restrict__ forces,
                                                      double* __restrict__ partials, int64_t nrows) {
-  __shared__ double red[9 * (kBlock / 64)];
+  __shared__ double red[kGatherNV * (kBlock / 64)];
   const int sid = find_seg(off, nsys, blockIdx.x);
   const SysDesc* D = tab + sid;
   const Geom& g = D->g;
@@ -372,9 +371,9 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __
   const int64_t atom =
       D->atom0 + (blockIdx.x - off[sid]) * kGatherAtomsPerBlock + (threadIdx.x >> 3);
   const bool valid = atom < D->atom0 + D->n;
-  double acc[9];
+  double acc[kGatherNV];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) acc[i] = 0.0;
+  for (int i = 0; i < kGatherNV; ++i) acc[i] = 0.0;
   T sx = 0, sy = 0, sz = 0, sp = 0;
   double x = 0.0, y = 0.0, z = 0.0, qi = 0.0, g_own = 0.0;
   if (valid) {
@@ -390,6 +389,10 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __
   sx = group8_sum(sx);
   sy = group8_sum(sy);
   sz = group8_sum(sz);
+  // the atom's real-space gradient row in lane 0 (lanes 0-2 hold one
+  // component each; quad_perm broadcasts within the quad)
+  const double gr0 = dpp_move<0x00, 0xf>(g_own), gr1 = dpp_move<0x55, 0xf>(g_own),
+               gr2 = dpp_move<0xaa, 0xf>(g_own);
   if (valid && a < 3) {
     // lane a (< 3) owns force component a
     const double dR[3] = {-qi * (double)sx, -qi * (double)sy, -qi * (double)sz};
@@ -399,16 +402,24 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __
     gj += g_own;
     forces[3 * atom + a] = -gj;
     if (a == 0) {
+      // dE/dA from the reciprocal sum, r (x) (k dR), and from the real-space
+      // gradient row g, r (x) (h g): the finishing step applies A = h^-1 on
+      // both sides, so h g enters as -A (r (x) g) does in the box derivative
       const double r[3] = {x, y, z};
+      const double gv[3] = {gr0, gr1, gr2};
+      double u[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        u[d] = dR[d] * g.k[d] + (g.h[d * 3] * gv[0] + g.h[d * 3 + 1] * gv[1] + g.h[d * 3 + 2] * gv[2]);
 #pragma unroll
       for (int j = 0; j < 3; ++j)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) acc[j * 3 + d] = dR[d] * g.k[d] * r[j];
+        for (int d = 0; d < 3; ++d) acc[j * 3 + d] = r[j] * u[d];
     }
   }
-  block_sum<9>(acc, red);  // the block total is in thread 0
+  block_sum<kGatherNV>(acc, red);  // the block total is in thread 0
   if (threadIdx.x == 0)
-    for (int i = 0; i < 9; ++i) partials[(int64_t)i * nrows + blockIdx.x] = acc[i];
+    for (int i = 0; i < kGatherNV; ++i) partials[(int64_t)i * nrows + blockIdx.x] = acc[i];
 }
 
 // ------------------------------------------------------------------ reductions
@@ -419,7 +430,7 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __
 struct PartialSets {
   const double* real;  // kRealNV values x rows_real (rows = cells x splits)
   const double* recip; // 10 values x rows_recip (x-pass blocks)
-  const double* gath;  // 9 values x rows_gath (gather blocks)
+  const double* gath;  // kGatherNV values x rows_gath (gather blocks)
   int64_t rows_real, rows_recip, rows_gath;
   int nsplit;
 };
@@ -482,7 +493,15 @@ __global__ void __launch_bounds__(kBlock) k_b_reduce_finalize(
     double vals[kSysValues];
     for (int k = 0; k < kSysValues; ++k)
       vals[k] = __hip_atomic_load(rr + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    finalize_system(vals, vals + kRealNV, vals + kRealNV + 10, D->g, D->kappa, D->eps,
+    // real-space values in finalize_system's order: energy and pair virial,
+    // sum r (x) dE/dr (zero here: the gather folded it into its dE/dA
+    // terms), sum q^2, sum q
+    double real[18];
+    for (int k = 0; k < 7; ++k) real[k] = vals[k];
+    for (int k = 7; k < 16; ++k) real[k] = 0.0;
+    real[16] = vals[7];
+    real[17] = vals[8];
+    finalize_system(real, vals + kRealNV, vals + kRealNV + 10, D->g, D->kappa, D->eps,
                     energy + 5 * s, dEdbox ? dEdbox + 9 * s : nullptr);
   }
 }
@@ -626,7 +645,7 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
   P.nsplit = real_splits(P.ctot);
   P.o_rpart = take(sizeof(double) * kRealNV * P.nsplit * std::max<int64_t>(P.ctot, 1));
   P.o_cpart = take(sizeof(double) * 10 * std::max<int64_t>(rows_x, 1));
-  P.o_gpart = take(sizeof(double) * 9 * std::max<int64_t>(rows_g, 1));
+  P.o_gpart = take(sizeof(double) * kGatherNV * std::max<int64_t>(rows_g, 1));
   P.o_red = take(sizeof(double) * kSysValues * nsys);
   P.o_tickets = take(sizeof(int) * nsys);
   P.bytes = off;
@@ -783,8 +802,7 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
     if (P.ctot > 0) {
       dim3 grid((unsigned)P.ctot, (unsigned)P.nsplit);
       k_b_real<T><<<grid, kRealBlock, 0, rs>>>(nsys, seg(kSegCell), tab, ps,
-                                               (const int*)(nbws + L.sorted), start, pos, grad,
-                                               rpart);
+                                               (const int*)(nbws + L.sorted), start, grad, rpart);
     }
     return !join || hipEventRecord(join, rs) == hipSuccess;
   };

@@ -12,7 +12,10 @@
 namespace admp {
 
 constexpr int kGatherAtomsPerBlock = kBlock / 8;
-constexpr int kSysValues = kRealNV + 10 + 9;  // reduced values per system
+// gather partial values per block: the dE/dA terms of the reciprocal sum and
+// of sum r (x) dE/dr over the real-space gradient rows (in one 3 x 3 form)
+constexpr int kGatherNV = 9;
+constexpr int kSysValues = kRealNV + 10 + kGatherNV;  // reduced values per system
 
 // Per-system description, built on the host and read by every batched kernel.
 struct SysDesc {

@@ -257,15 +257,15 @@ __device__ __forceinline__ int load_stencil_chunk(const CellGrid& cg, int64_t ce
 // cutoff and compact the survivors into a per-wavefront queue evaluated 64
 // pairs at a time.  dE/dr of home and candidate atoms gather in float64 LDS
 // accumulators and are added to grad[i][3] with float64 global atomics
-// (grad zeroed beforehand).  The block's energy, pair virial, sum r (x) dE/dr,
-// sum q^2 and sum q (kRealNV values) go to partial[k * pstride].
+// (grad zeroed beforehand).  The block's energy, pair virial, sum q^2 and
+// sum q (kRealNV values) go to partial[k * pstride]; the term sum r (x) dE/dr
+// of the virial is formed by the gather from the finished gradient rows.
 template <typename T>
 __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0, int lc,
                                                int split, int S, double kappa, T eps,
                                                const P4<double>* __restrict__ ps,
                                                const int* __restrict__ sorted,
                                                const int* __restrict__ start,
-                                               const double* __restrict__ pos,
                                                double* __restrict__ grad,
                                                double* __restrict__ partial, int64_t pstride) {
   constexpr int kCandCap = cand_cap<T>();
@@ -313,9 +313,7 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
   C.c_exp = (T)(2.0 / kSqrtPi * kappa);
   PairAcc pacc;
   pacc.e = pacc.vxx = pacc.vyy = pacc.vzz = pacc.vxy = pacc.vxz = pacc.vyz = 0.0;
-  double acc[kRealNV];
-#pragma unroll
-  for (int k = 0; k < kRealNV; ++k) acc[k] = 0.0;
+  double qsq = 0.0, qsum = 0.0;
   int* q = queue + wave * kQueueRing;
   const int nhalf = nst - st_c;
   const bool one_chunk = nhalf <= kMaxStencil;
@@ -449,11 +447,6 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
           unsafeAtomicAdd(grad + 3 * j, g[0]);
           unsafeAtomicAdd(grad + 3 * j + 1, g[1]);
           unsafeAtomicAdd(grad + 3 * j + 2, g[2]);
-          const double r[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
-#pragma unroll
-          for (int a2 = 0; a2 < 3; ++a2)
-#pragma unroll
-            for (int b2 = 0; b2 < 3; ++b2) acc[7 + a2 * 3 + b2] += r[a2] * g[b2];
         }
       }
     }
@@ -466,27 +459,17 @@ __device__ __forceinline__ void real_cell_body(const CellGrid& cg, int64_t cell0
         unsafeAtomicAdd(grad + 3 * i, g[0]);
         unsafeAtomicAdd(grad + 3 * i + 1, g[1]);
         unsafeAtomicAdd(grad + 3 * i + 2, g[2]);
-        const double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-#pragma unroll
-        for (int a2 = 0; a2 < 3; ++a2)
-#pragma unroll
-          for (int b2 = 0; b2 < 3; ++b2) acc[7 + a2 * 3 + b2] += r[a2] * g[b2];
       }
       if (split == 0) {
         const double qi = ps[h0 + t].q;
-        acc[16] += qi * qi;
-        acc[17] += qi;
+        qsq += qi * qi;
+        qsum += qi;
       }
     }
     __syncthreads();
   }
-  acc[0] = pacc.e;
-  acc[1] = pacc.vxx;
-  acc[2] = pacc.vyy;
-  acc[3] = pacc.vzz;
-  acc[4] = pacc.vxy;
-  acc[5] = pacc.vxz;
-  acc[6] = pacc.vyz;
+  double acc[kRealNV] = {pacc.e,   pacc.vxx, pacc.vyy, pacc.vzz, pacc.vxy,
+                         pacc.vxz, pacc.vyz, qsq,      qsum};
   block_sum<kRealNV>(acc, red);
   if (threadIdx.x == 0)
     for (int k = 0; k < kRealNV; ++k) partial[k * pstride] = acc[k];
