@@ -198,7 +198,9 @@ def serving_bench(n_requests, torch, dev):
     one request at a time) and the single-system fused path on a subsample of
     the requests the plain path accepts, against the batched engines' capacity
     on those same requests, and (4) the float32 engine's error against the
-    float64 engine on 64 requests."""
+    float64 engine on 64 requests, (5) the float64 Poisson stream again with 16
+    QEq solves (300-2000 atoms of 3 species, rc 6 A, to |P grad E| / n <= 1e-10)
+    arriving uniformly over the same interval, served beside the PME batches."""
     import numpy as np
 
     from torch_admp_amd import ops, serving
@@ -289,6 +291,33 @@ def serving_bench(n_requests, torch, dev):
                   for a, b in zip(res["float32"], res["float64"]))
     f_rmse = max(float(np.sqrt(((a.forces - b.forces) ** 2).mean()))
                  for a, b in zip(res["float32"], res["float64"]))
+    # (5) PME and QEq requests in one stream
+    from torch_admp_amd.workloads import qeq_species_box
+
+    span = gaps[-1] / rate
+    rng = np.random.default_rng(5)
+    qeq = []
+    for k in range(16):
+        nq = int(rng.integers(300, 2001))
+        pq, bq, cq, hq, eq = qeq_species_box(nq, (nq / 0.1) ** (1.0 / 3.0), seed=100 + k)
+        qeq.append(serving.QEqRequest(pq, bq, cq, hq, eq, rcut=6.0, tol=1e-10,
+                                      arrival=span * (k + 0.5) / 16))
+    warm = server("float64")
+    warm.serve([clone(base[0])] + [serving.QEqRequest(
+        qeq[0].positions, qeq[0].box, qeq[0].chi, qeq[0].hardness, qeq[0].eta, 6.0, tol=1e-10)])
+    warm.close()
+    srv = server("float64")
+    try:
+        mix = srv.serve([clone(r, g / rate) for r, g in zip(base, gaps)] + qeq)
+    finally:
+        srv.close()
+    out["mixed_pme_qeq_fp64"] = {
+        "pme_p50_ms": round(mix["pme"]["p50_ms"], 3), "pme_p99_ms": round(mix["pme"]["p99_ms"], 3),
+        "qeq_requests": len(qeq), "qeq_p50_ms": round(mix["qeq"]["p50_ms"], 3),
+        "qeq_p99_ms": round(mix["qeq"]["p99_ms"], 3),
+        "qeq_mean_cg_iterations": round(float(np.mean([r.iterations for r in qeq])), 1),
+        "wall_s": round(mix["wall_s"], 3)}
+
     atoms = np.array([r.n_atoms for r in base])
     out.update({
         "workload": f"{n_requests} PME E+F requests, atoms log-uniform [64, 8192] "

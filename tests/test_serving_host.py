@@ -38,3 +38,17 @@ def test_synthetic_stream_shape():
 def test_server_requires_gpu():
     with pytest.raises(RuntimeError, match="ROCm GPU"):
         serving.PMEServer("cpu", hbm_budget=1 << 20)
+
+
+def test_qeq_request_checks_and_cost():
+    from torch_admp_amd.workloads import qeq_species_box
+
+    pos, box, chi, hard, eta = qeq_species_box(300, 15.0, seed=2)
+    r = serving.QEqRequest(pos, box, chi, hard, eta, rcut=6.0)
+    assert r.n_atoms == 300 and r.kmesh == serving.ewald_parameters(box, 6.0, 1e-5)[1]
+    # pair list (half sphere of rc at this density), per-atom arrays and mesh
+    pairs = 300 * (4.0 / 3.0 * math.pi * 6.0 ** 3 * 300 / 15.0 ** 3) / 2.0
+    mesh = int(np.prod(r.kmesh))
+    assert serving.PMEServer.qeq_cost(r) == int(64 * pairs + 512 * 300 + 48 * mesh + (1 << 20))
+    with pytest.raises(ValueError, match="eta"):
+        serving.QEqRequest(pos, box, chi, hard, eta[:-1], rcut=6.0)

@@ -1,4 +1,5 @@
-"""Continuous-batching request loop for PME energy + force evaluations.
+"""Continuous-batching request loop for PME energy + force evaluations and
+QEq charge solves.
 
 Requests of any atom count, box and mesh arrive over time.  The server keeps
 a FIFO queue and up to ``streams`` batches in flight, one per HIP stream.
@@ -11,6 +12,13 @@ stream and memory free up, so their work overlaps on the device with the
 batches already there.  Each request's result is the same whether it runs
 alone or in a mix (the batched kernels treat every system independently).
 
+QEq requests (charge equilibration) share the HBM budget.  A QEq solve checks
+its residual on the host every few iterations, so it is not folded into a
+PME launch: a worker thread admits QEq requests in arrival order as the
+budget allows and solves each on its own stream with the library's projected
+CG (QEqForceModule.solve_implicit, forward only), concurrently with the PME
+batches.
+
 The reference evaluates one frame per CoulombForceModule call
 (src/torch_admp/pme.py:261-310); this module has no counterpart there.
 """
@@ -18,10 +26,12 @@ The reference evaluates one frame per CoulombForceModule call
 from __future__ import annotations
 
 import math
+import sys
+import threading
 import time
 from collections import deque
 from dataclasses import dataclass, field
-from typing import Deque, List, Optional
+from typing import Deque, List, Optional, Union
 
 import numpy as np
 import torch
@@ -82,6 +92,63 @@ class PMERequest:
 
 
 @dataclass
+class QEqRequest:
+    """One charge equilibration: q* = argmin E(q) on sum q = total_charge for
+    positions (n, 3), box (3, 3) rows, chi, hardness (the q^2 coefficient) and
+    Gaussian widths eta (n,), to |P grad E| / n <= tol (float64 solve).  Mesh
+    and kappa follow CoulombForceModule's rule for ``ethresh``."""
+
+    positions: np.ndarray
+    box: np.ndarray
+    chi: np.ndarray
+    hardness: np.ndarray
+    eta: np.ndarray
+    rcut: float
+    ethresh: float = 1e-5
+    total_charge: float = 0.0
+    tol: float = 1e-10
+    arrival: float = 0.0
+    rid: int = -1
+    # results
+    charges: Optional[np.ndarray] = None
+    iterations: int = -1
+    residual: float = math.nan
+    error: Optional[str] = None
+    t_admit: float = math.nan
+    t_done: float = math.nan
+    cost: int = 0
+
+    def __post_init__(self):
+        self.positions = np.ascontiguousarray(self.positions, dtype=np.float64).reshape(-1, 3)
+        n = self.positions.shape[0]
+        for name in ("chi", "hardness", "eta"):
+            v = np.ascontiguousarray(getattr(self, name), dtype=np.float64).reshape(-1)
+            if v.shape[0] != n:
+                raise ValueError(f"{name}: expected {n} values, got {v.shape[0]}")
+            setattr(self, name, v)
+        self.box = np.asarray(self.box, dtype=np.float64).reshape(3, 3)
+        self.kappa, self.kmesh = ewald_parameters(self.box, self.rcut, self.ethresh)
+
+    @property
+    def n_atoms(self) -> int:
+        return self.positions.shape[0]
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.arrival
+
+
+def _stats(reqs: List, wall: float) -> dict:
+    lat = np.array([r.latency for r in reqs]) * 1e3
+    if len(lat) == 0:
+        return {"requests": 0}
+    return {"requests": len(reqs), "throughput_req_s": len(reqs) / wall,
+            "atoms_per_s": sum(r.n_atoms for r in reqs) / wall,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean())}
+
+
+@dataclass
 class _Batch:
     requests: List[PMERequest]
     plan: PMEBatchPlan
@@ -95,7 +162,8 @@ class _Batch:
 
 
 class PMEServer:
-    """Continuous batching of PME requests under an HBM budget (bytes)."""
+    """Continuous batching of PME requests, and QEq solves beside them, under an
+    HBM budget (bytes)."""
 
     def __init__(self, device="cuda", hbm_budget: Optional[int] = None, max_systems: int = 1024,
                  streams: int = 2, precision: str = "float32", convention: str = "reference",
@@ -119,6 +187,14 @@ class PMEServer:
         self.batches_launched = 0
         self._next_id = 0
         self._t0 = time.perf_counter()
+        # QEq: queue, worker thread and the lock that also guards bytes_in_flight
+        self.qeq_queue: Deque[QEqRequest] = deque()
+        self._cv = threading.Condition()
+        self._qeq_done: List[QEqRequest] = []
+        self._qeq_busy = False
+        self._stop = False
+        self._worker: Optional[threading.Thread] = None
+        self._switch: Optional[float] = None
 
     # ------------------------------------------------------------------ admission
     def request_cost(self, req: PMERequest) -> int:
@@ -131,21 +207,108 @@ class PMEServer:
     def now(self) -> float:
         return time.perf_counter() - self._t0
 
-    def submit(self, req: PMERequest) -> int:
-        req.cost = self.request_cost(req)
+    @staticmethod
+    def qeq_cost(req: QEqRequest) -> int:
+        """HBM bytes a QEq solve holds: its pair list (cutoff sphere at the
+        request's density), per-atom arrays and the reciprocal mesh buffers."""
+        n = req.n_atoms
+        vol = abs(float(np.linalg.det(req.box)))
+        pairs = n * (4.0 / 3.0 * math.pi * req.rcut ** 3 * n / max(vol, 1e-30)) / 2.0
+        mesh = int(np.prod(req.kmesh))
+        return int(64 * pairs + 512 * n + 48 * mesh + (1 << 20))
+
+    def submit(self, req: Union[PMERequest, QEqRequest]) -> int:
+        req.cost = self.qeq_cost(req) if isinstance(req, QEqRequest) else self.request_cost(req)
         if req.cost > self.hbm_budget:
             raise ValueError(f"request of {req.n_atoms} atoms needs {req.cost} bytes, more than "
                              f"the HBM budget of {self.hbm_budget}")
         req.rid = self._next_id
         self._next_id += 1
-        self.queue.append(req)
+        if isinstance(req, QEqRequest):
+            with self._cv:
+                if self._worker is None:
+                    # The serving loop polls and the worker returns from a library
+                    # call every few CG iterations: with Python's default 5 ms
+                    # thread switch interval each side waits up to 5 ms for the
+                    # other at every hand-over.  0.1 ms while the worker lives.
+                    self._switch = sys.getswitchinterval()
+                    sys.setswitchinterval(min(self._switch, 1e-4))
+                    self._worker = threading.Thread(target=self._qeq_worker, daemon=True)
+                    self._worker.start()
+                self.qeq_queue.append(req)
+                self._cv.notify_all()
+        else:
+            self.queue.append(req)
         return req.rid
+
+    # ------------------------------------------------------------------ QEq
+    def _qeq_worker(self) -> None:
+        torch.cuda.set_device(self.device)
+        stream = torch.cuda.Stream(self.device)
+        while True:
+            with self._cv:
+                while not self._stop and not (
+                        self.qeq_queue and
+                        self.bytes_in_flight + self.qeq_queue[0].cost <= self.hbm_budget):
+                    self._cv.wait(0.01)
+                if self._stop:
+                    return
+                req = self.qeq_queue.popleft()
+                self.bytes_in_flight += req.cost
+                self._qeq_busy = True
+            req.t_admit = self.now()
+            try:
+                with torch.cuda.stream(stream):
+                    self._solve_qeq(req)
+            except Exception as exc:  # reported by pump() on the serving thread
+                req.error = repr(exc)
+            req.t_done = self.now()
+            with self._cv:
+                self.bytes_in_flight -= req.cost
+                self._qeq_busy = False
+                self._qeq_done.append(req)
+                self._cv.notify_all()
+
+    def _solve_qeq(self, req: QEqRequest) -> None:
+        from .nblist import TorchNeighborList
+        from .qeq import QEqForceModule
+
+        def dev(a):
+            return torch.tensor(a, dtype=torch.float64, device=self.device)
+
+        n = req.n_atoms
+        pos, box = dev(req.positions), dev(req.box)
+        chi, hard, eta = dev(req.chi), dev(req.hardness), dev(req.eta)
+        mod = QEqForceModule(req.rcut, req.ethresh, kappa=req.kappa, kmesh=req.kmesh,
+                             precision="float64", convention=self.convention)
+        nbl = TorchNeighborList(req.rcut)
+        with torch.no_grad():
+            pairs = nbl(pos, box)
+            ds, bs = nbl.get_ds(), nbl.get_buffer_scales()
+            A = torch.ones(1, n, dtype=torch.float64, device=self.device)
+            b = torch.full((1,), float(req.total_charge), dtype=torch.float64, device=self.device)
+            q = mod.solve_implicit(pos, box, chi, hard, eta, pairs, ds, bs, A, b, tol=req.tol)
+        req.charges = q.detach().cpu().numpy()  # waits for this stream only
+        req.iterations = int(mod.converge_iter)
+        req.residual = float(mod.residual)
+
+    def close(self) -> None:
+        """Stop the QEq worker (queued QEq requests are dropped) and restore the
+        thread switch interval it set."""
+        with self._cv:
+            self._stop = True
+            self._cv.notify_all()
+        if self._worker is not None:
+            self._worker.join()
+            self._worker = None
+            sys.setswitchinterval(self._switch)
 
     # ------------------------------------------------------------------ batches
     def _launch(self, slot: int) -> bool:
         take: List[PMERequest] = []
         cost = 0
-        room = self.hbm_budget - self.bytes_in_flight
+        with self._cv:
+            room = self.hbm_budget - self.bytes_in_flight
         while self.queue and len(take) < self.max_systems:
             r = self.queue[0]
             if cost + r.cost > room:
@@ -183,7 +346,8 @@ class PMEServer:
             done.record(stream)
         self.inflight[slot] = _Batch(take, plan, stream, done, h_e, h_f, h_g,
                                      [host_in, dev_in, pos, q, ws, energy, forces, dedbox], cost)
-        self.bytes_in_flight += cost
+        with self._cv:
+            self.bytes_in_flight += cost
         self.batches_launched += 1
         return True
 
@@ -200,7 +364,9 @@ class PMEServer:
             r.dedbox = g[k].copy()
             r.t_done = t
         self.inflight[slot] = None
-        self.bytes_in_flight -= b.cost
+        with self._cv:
+            self.bytes_in_flight -= b.cost
+            self._cv.notify_all()
         self.completed.extend(b.requests)
         return b.requests
 
@@ -215,10 +381,19 @@ class PMEServer:
             if b is None and self.queue:
                 if not self._launch(slot):
                     break
+        with self._cv:
+            done, self._qeq_done = self._qeq_done, []
+        for r in done:
+            if r.error is not None:
+                raise RuntimeError(f"QEq request {r.rid} failed: {r.error}")
+        self.completed.extend(done)
+        out.extend(done)
         return out
 
     def idle(self) -> bool:
-        return not self.queue and all(b is None for b in self.inflight)
+        with self._cv:
+            qeq_idle = not self.qeq_queue and not self._qeq_busy and not self._qeq_done
+        return qeq_idle and not self.queue and all(b is None for b in self.inflight)
 
     def drain(self) -> None:
         while not self.idle():
@@ -234,18 +409,19 @@ class PMEServer:
         i = 0
         while i < len(order) or not self.idle():
             now = self.now()
+            i0 = i
             while i < len(order) and order[i].arrival <= now:
                 self.submit(order[i])
                 i += 1
-            self.pump()
+            if not self.pump() and i == i0 and self._worker is not None:
+                time.sleep(5e-5)  # nothing moved: let the QEq worker have the interpreter
         wall = self.now()
-        lat = np.array([r.latency for r in self.completed]) * 1e3
-        atoms = sum(r.n_atoms for r in self.completed)
-        return {"requests": len(self.completed), "wall_s": wall,
-                "throughput_req_s": len(self.completed) / wall,
-                "atoms_per_s": atoms / wall,
-                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                "mean_ms": float(lat.mean()), "batches": self.batches_launched}
+        out = dict(_stats(self.completed, wall), wall_s=wall, batches=self.batches_launched)
+        qeq = [r for r in self.completed if isinstance(r, QEqRequest)]
+        if qeq:
+            out["pme"] = _stats([r for r in self.completed if isinstance(r, PMERequest)], wall)
+            out["qeq"] = _stats(qeq, wall)
+        return out
 
 
 def synthetic_stream(n_requests: int, rate_hz: float, seed: int = 0, n_min: int = 64,
