@@ -147,6 +147,15 @@ __device__ __forceinline__ T group8_sum(T v) {
   return v;
 }
 
+// Sum over each aligned group of 16 lanes (one DPP row), in every lane of the
+// row: the 8-lane sums, then lane i adds lane 15 - i (row_mirror).
+template <typename T>
+__device__ __forceinline__ T group16_sum(T v) {
+  v = group8_sum(v);
+  v += dpp_move<0x140, 0xf>(v);
+  return v;
+}
+
 __device__ __forceinline__ double read_lane(double v, int lane) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)b, lane);

@@ -454,12 +454,14 @@ __device__ __forceinline__ void x_convolve_lines(cplx<T>* __restrict__ data, int
       Gp = gfun(P, kvp, th, &gpp);
     }
     const double Gs = 0.5 * (G + Gp);
-    const double w = (iz > 0 && 2 * iz != K2) ? 2.0 : 1.0;
     CT s = res[idx];
-    const double S2 = (double)s.x * (double)s.x + (double)s.y * (double)s.y;
-    acc[0] += w * Gs * S2;
-    add_dEdA(acc, P, mx, my, mz, kv, gp, 0.5 * w * S2);
-    add_dEdA(acc, P, mxp, myp, mzp, kvp, gpp, 0.5 * w * S2);
+    if (partial_dst) {  // energy and dE/dA sums only when a caller reads them
+      const double w = (iz > 0 && 2 * iz != K2) ? 2.0 : 1.0;
+      const double S2 = (double)s.x * (double)s.x + (double)s.y * (double)s.y;
+      acc[0] += w * Gs * S2;
+      add_dEdA(acc, P, mx, my, mz, kv, gp, 0.5 * w * S2);
+      add_dEdA(acc, P, mxp, myp, mzp, kvp, gpp, 0.5 * w * S2);
+    }
     const T f = (T)(2.0 * P.eps * P.phi_scale * Gs);
     s.x *= f;
     s.y *= f;

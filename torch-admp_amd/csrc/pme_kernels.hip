@@ -156,12 +156,26 @@ static int pick_chunk(int L, int64_t lines, size_t elem) {
   return C;
 }
 
+// Zero n 16-byte words of a 16-byte aligned buffer.
+__global__ void __launch_bounds__(kBlock) k_zero16(int4* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = make_int4(0, 0, 0, 0);
+}
+
 template <typename T>
 hipError_t launch_spread(const double* pos, const double* q, int64_t n, const Geom& g, T* mesh,
                          hipStream_t s) {
   const int64_t nm = (int64_t)g.k[0] * g.k[1] * g.k[2];
-  k_fill<T><<<(int)std::min<int64_t>((nm + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(mesh, nm,
-                                                                                     T(0));
+  // 16-byte stores over the whole words of an aligned mesh, element stores for the rest
+  const int64_t n16 = (reinterpret_cast<uintptr_t>(mesh) & 15) ? 0 : nm * (int64_t)sizeof(T) / 16;
+  if (n16 > 0)
+    k_zero16<<<(int)std::min<int64_t>((n16 + kBlock - 1) / kBlock, 8192), kBlock, 0, s>>>(
+        reinterpret_cast<int4*>(mesh), n16);
+  const int64_t done = n16 * 16 / (int64_t)sizeof(T);
+  if (done < nm)
+    k_fill<T><<<(int)std::min<int64_t>((nm - done + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(
+        mesh + done, nm - done, T(0));
   if (n > 0) {
     const int64_t threads = n * 8;
     k_spread<T><<<(unsigned)((threads + kBlock - 1) / kBlock), kBlock, 0, s>>>(pos, q, n, g, mesh);
@@ -217,10 +231,12 @@ hipError_t launch_convolve(const T* mesh, const Geom& g, int conv, double kappa,
   dim3 grid3((Kzh + C3 - 1) / C3, K1);
   {
     const size_t sh = (size_t)(K0 + 2 * K0 * C3) * sizeof(CT);
-    k_fft_x_convolve<T><<<grid3, kBlock, sh, s>>>(spec, C3, r0, P, partials);
+    k_fft_x_convolve<T><<<grid3, kBlock, sh, s>>>(spec, C3, r0, P,
+                                                  (energy || dEdA) ? partials : nullptr);
   }
   const int nb3 = grid3.x * grid3.y;
-  k_reduce_partials<<<1, kReduceThreads, 0, s>>>(partials, nb3, 10, eps, energy, dEdA);
+  if (energy || dEdA)
+    k_reduce_partials<<<1, kReduceThreads, 0, s>>>(partials, nb3, 10, eps, energy, dEdA);
   if (phi) {
     // pass 4: inverse along y
     {

@@ -60,6 +60,80 @@ __global__ void __launch_bounds__(kBlock) k_pair_matvec(int64_t np, const int64_
   unsafeAtomicAdd(&out[j], cp * v[i]);
 }
 
+// Row lists of the nonzero pair coefficients, built once per geometry: every
+// pair (i, j, c) enters row i as (j, c) and row j as (i, c).  The order inside
+// a row follows the atomics of the fill.
+__global__ void __launch_bounds__(kBlock) k_csr_count(int64_t np, const int64_t* __restrict__ pairs,
+                                                      const double* __restrict__ c,
+                                                      int* __restrict__ cnt) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np || c[p] == 0.0) return;
+  atomicAdd(&cnt[pairs[2 * p]], 1);
+  atomicAdd(&cnt[pairs[2 * p + 1]], 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_csr_fill(int64_t np, const int64_t* __restrict__ pairs,
+                                                     const double* __restrict__ c,
+                                                     const int* __restrict__ rowptr,
+                                                     int* __restrict__ cur, int* __restrict__ col,
+                                                     double* __restrict__ val) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const double cp = c[p];
+  if (cp == 0.0) return;
+  const int64_t i = pairs[2 * p], j = pairs[2 * p + 1];
+  const int ki = rowptr[i] + atomicAdd(&cur[i], 1);
+  col[ki] = (int)j;
+  val[ki] = cp;
+  const int kj = rowptr[j] + atomicAdd(&cur[j], 1);
+  col[kj] = (int)i;
+  val[kj] = cp;
+}
+
+// Terms of the Hessian product that depend on sums over v (sums[0..2] =
+// sum v, sum v z, sum v z^2): the neutralising-background term of the
+// reciprocal sum and the slab correction.
+struct GlobalTerms {
+  const double* z;
+  const double* sums;
+  double c_nn, pre_slab, lz2;
+  int slab, on;
+};
+
+__device__ __forceinline__ double global_term(const GlobalTerms& gt, int64_t i) {
+  const double Q = gt.sums[0];
+  double t = gt.c_nn * Q;
+  if (gt.slab) {
+    const double M = gt.sums[1], S = gt.sums[2], zi = gt.z[i];
+    t += gt.pre_slab * (2.0 * M * zi - S - Q * zi * zi - Q * gt.lz2 / 6.0);
+  }
+  return t;
+}
+
+// out[i] = hdiag[i] v[i] + sum over row i of c v[col] (+ the global terms)
+// (overwrite): 16 lanes per row, DPP row sum, no atomics.
+__global__ void __launch_bounds__(kBlock) k_csr_matvec(int64_t n, const int* __restrict__ rowptr,
+                                                       const int* __restrict__ col,
+                                                       const double* __restrict__ val,
+                                                       const double* __restrict__ d,
+                                                       const double* __restrict__ v,
+                                                       GlobalTerms gt,
+                                                       double* __restrict__ out) {
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  const int sub = threadIdx.x & 15;
+  double acc = 0.0;
+  if (row < n) {
+    const int e = rowptr[row + 1];
+    for (int k = rowptr[row] + sub; k < e; k += 16) acc += val[k] * v[col[k]];
+  }
+  acc = group16_sum(acc);  // every lane of the wavefront takes part
+  if (row < n && sub == 0) {
+    double o = d[row] * v[row] + acc;
+    if (gt.on) o += global_term(gt, row);
+    out[row] = o;
+  }
+}
+
 // out = hdiag .* v (overwrite)
 __global__ void k_diag_mul(int64_t n, const double* __restrict__ d, const double* __restrict__ v,
                            double* __restrict__ out) {
@@ -77,7 +151,9 @@ __global__ void __launch_bounds__(kBlock) k_dots(int64_t n, const double* __rest
                                                  const double* __restrict__ y,
                                                  const double* __restrict__ A, int nc,
                                                  const double* __restrict__ z,
-                                                 double* __restrict__ partials) {
+                                                 double* __restrict__ partials, int nv,
+                                                 double* __restrict__ dst,
+                                                 int* __restrict__ ticket) {
   __shared__ double red[kQeqMaxDots * (kBlock / 64)];
   double acc[kQeqMaxDots];
 #pragma unroll
@@ -97,6 +173,25 @@ __global__ void __launch_bounds__(kBlock) k_dots(int64_t n, const double* __rest
   block_sum<kQeqMaxDots>(acc, red);
   if (threadIdx.x == 0)
     for (int k = 0; k < kQeqMaxDots; ++k) partials[(int64_t)blockIdx.x * kQeqMaxDots + k] = acc[k];
+  // the last work-group to finish (ticket) sums values 0..nv-1 over the blocks
+  // in block order into dst[0..nv-1] and re-arms the ticket
+  __threadfence();
+  __syncthreads();
+  __shared__ int last;
+  if (threadIdx.x == 0) last = atomicAdd(ticket, 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const int lane = threadIdx.x & 63;
+  for (int v = threadIdx.x >> 6; v < nv; v += blockDim.x >> 6) {
+    double t = 0.0;
+    for (int bi = lane; bi < (int)gridDim.x; bi += 64)
+      t += __hip_atomic_load(partials + (int64_t)bi * kQeqMaxDots + v, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    t = wave_sum(t);
+    if (lane == 0) dst[v] = t;
+  }
+  if (threadIdx.x == 0) *ticket = 0;
 }
 
 // Gram matrix A A^T (nc x nc), per-block partials
@@ -180,16 +275,10 @@ __global__ void k_project(int64_t n, double* __restrict__ x, const double* __res
 __global__ void k_global_terms(int64_t n, const double* __restrict__ z,
                                const double* __restrict__ sums, double c_nn, double pre_slab,
                                double lz2, int slab, double* __restrict__ out) {
-  const double Q = sums[0], M = sums[1], S = sums[2];
+  const GlobalTerms gt{z, sums, c_nn, pre_slab, lz2, slab, 1};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    double v = c_nn * Q;
-    if (slab) {
-      const double zi = z[i];
-      v += pre_slab * (2.0 * M * zi - S - Q * zi * zi - Q * lz2 / 6.0);
-    }
-    out[i] += v;
-  }
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] += global_term(gt, i);
 }
 
 // g <- g + chi ; r <- -g (projection applied by the caller)
@@ -249,6 +338,166 @@ __global__ void k_extract_z(int64_t n, const double* __restrict__ pos, int axis,
     z[i] = pos[3 * i + axis];
 }
 
+// ------------------------------------------------------------------ point spread
+// The Hessian product spreads a new charge vector on fixed positions every
+// iteration.  Once per geometry each atom's stencil is stored: base b_d =
+// (m0_d - 3) mod K_d (mesh_ref) and weights w_d[a] = M6(f_d + a), so that
+// point (b + a) mod K receives q wx[a] wy[b] wz[c]; atoms are binned by base
+// in 4-point bins.  The spread then runs over mesh points: a work-group of
+// 4 x 4 x 16 points lists the bins within reach (bases p - 5 .. p), loads
+// their extents in parallel (one bin per thread), stages the candidates
+// (atom, base) in LDS, and each point sums the weight at offset (p - b) mod K
+// of every candidate with all three offsets < 6 (unique for K >= 6).  Each
+// point is written once: no atomics and no zero fill.
+constexpr int kPtBin = 4;                         // mesh points per bin and axis (a shift)
+constexpr int kPtB0 = 16, kPtB1 = 8, kPtB2 = 16;  // points per work-group and axis
+constexpr int kPtThreads = 256;                   // 8 points along x per thread
+constexpr int kPtPer = kPtB0 * kPtB1 * kPtB2 / kPtThreads;
+constexpr int kPtL0 = 7, kPtL1 = 5, kPtL2 = 7;    // bins per axis: ceil((B + 5) / 4) + 1
+constexpr int kPtChunk = 128;                     // candidates staged in LDS at a time
+static_assert(kPtL0 * kPtL1 * kPtL2 <= kPtThreads, "one bin per thread");
+static_assert(kPtPer == 8 && kPtB2 == 16 && kPtB1 == 8, "thread to point map below");
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_point_rec(const double* __restrict__ pos, int64_t n,
+                                                      Geom g, int nb1, int nb2,
+                                                      int4* __restrict__ base,
+                                                      T* __restrict__ w,
+                                                      int* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int m0[3];
+  double f[3];
+  mesh_ref(g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], m0, f);
+  int b[3];
+  T* wi = w + 18 * i;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    b[d] = pmod32(m0[d] - 3, g.k[d]);
+    T wd[6], dd[6];
+    bspline6_rounded<T>(f[d], wd, dd);
+#pragma unroll
+    for (int a = 0; a < 6; ++a) wi[6 * d + a] = wd[a];
+  }
+  const int bin = ((b[0] / kPtBin) * nb1 + b[1] / kPtBin) * nb2 + b[2] / kPtBin;
+  base[i] = make_int4(b[0], b[1], b[2], bin);
+  atomicAdd(&cnt[bin], 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_point_fill(int64_t n, const int4* __restrict__ base,
+                                                       const int* __restrict__ start,
+                                                       int* __restrict__ cur,
+                                                       int* __restrict__ atoms) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int bin = base[i].w;
+  atoms[start[bin] + atomicAdd(&cur[bin], 1)] = (int)i;
+}
+
+// bins (each once) holding bases o - 5 .. o + B - 1 (mod K); returns the count
+__device__ __forceinline__ int point_bins(int o, int B, int K, int* lst) {
+  int m = 0;
+  for (int j = 0; j < B + 5; ++j) {
+    int v = o - 5 + j;
+    v += v < 0 ? K : 0;
+    v = wrapk(v, K);
+    const int bin = v / kPtBin;
+    bool seen = false;
+    for (int u = 0; u < m; ++u) seen |= lst[u] == bin;
+    if (!seen) lst[m++] = bin;
+  }
+  return m;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kPtThreads) k_point_spread(
+    const int4* __restrict__ base, const T* __restrict__ w, const int* __restrict__ start,
+    const int* __restrict__ atoms, const double* __restrict__ q, int K0, int K1, int K2, int nb1,
+    int nb2, int g1, int g2, T* __restrict__ mesh) {
+  __shared__ int l0[kPtL0 + 4], l1[kPtL1 + 4], l2[kPtL2 + 4], nl[3];
+  __shared__ int pre[kPtThreads], bst[kPtThreads];  // inclusive count prefix, bin start
+  __shared__ int wtot[kPtThreads / 64];
+  __shared__ int4 cb[kPtChunk];       // staged candidates: base,
+  __shared__ double cq[kPtChunk];     // charge
+  __shared__ T cw[kPtChunk][18];      // and weights
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int blk = blockIdx.x;
+  const int c2 = blk % g2, c1 = (blk / g2) % g1, c0 = blk / (g1 * g2);
+  const int o0 = c0 * kPtB0, o1 = c1 * kPtB1, o2 = c2 * kPtB2;
+  if (tid == 0) nl[0] = point_bins(o0, kPtB0, K0, l0);
+  if (tid == 64) nl[1] = point_bins(o1, kPtB1, K1, l1);
+  if (tid == 128) nl[2] = point_bins(o2, kPtB2, K2, l2);
+  __syncthreads();
+  const int n0 = nl[0], n1 = nl[1], n2 = nl[2], nbins = n0 * n1 * n2;
+  // bin extents, one per thread, and the inclusive prefix of their counts
+  int cnt = 0, st = 0;
+  if (tid < nbins) {
+    const int u2 = tid % n2, u1 = (tid / n2) % n1, u0 = tid / (n1 * n2);
+    const int bin = (l0[u0] * nb1 + l1[u1]) * nb2 + l2[u2];
+    st = start[bin];
+    cnt = start[bin + 1] - st;
+  }
+  int inc = cnt;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int up = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += up;
+  }
+  if (lane == 63) wtot[wid] = inc;
+  __syncthreads();
+  for (int u = 0; u < wid; ++u) inc += wtot[u];
+  pre[tid] = inc;
+  bst[tid] = st;
+  __syncthreads();
+  const int total = pre[kPtThreads - 1];
+  // thread -> z = tid % 16, y = (tid / 16) % 8, x = 8 (tid / 128) + k
+  const int p2 = o2 + (tid & 15), p1 = o1 + ((tid >> 4) & 7), px = o0 + kPtPer * (tid >> 7);
+  double acc[kPtPer];
+#pragma unroll
+  for (int k = 0; k < kPtPer; ++k) acc[k] = 0.0;
+  for (int c0k = 0; c0k < total; c0k += kPtChunk) {
+    const int c = c0k + tid;
+    if (tid < kPtChunk && c < total) {
+      // the bin slot holding candidate c: first slot with pre > c
+      int lo = 0, hi = nbins - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] > c) hi = mid; else lo = mid + 1;
+      }
+      const int first = lo ? pre[lo - 1] : 0;  // candidates before this slot
+      const int a = atoms[bst[lo] + (c - first)];
+      cb[tid] = base[a];
+      cq[tid] = q[a];
+      const T* wa = w + 18 * (int64_t)a;
+#pragma unroll
+      for (int e = 0; e < 18; ++e) cw[tid][e] = wa[e];
+    }
+    __syncthreads();
+    const int m = min(kPtChunk, total - c0k);
+    for (int j = 0; j < m; ++j) {
+      const int4 b = cb[j];
+      int d1 = p1 - b.y, d2 = p2 - b.z;
+      d1 += d1 < 0 ? K1 : 0;
+      d2 += d2 < 0 ? K2 : 0;
+      if (d1 < 6 && d2 < 6) {
+        const double qyz = cq[j] * ((double)cw[j][6 + d1] * (double)cw[j][12 + d2]);
+#pragma unroll
+        for (int k = 0; k < kPtPer; ++k) {
+          int d0 = px + k - b.x;
+          d0 += d0 < 0 ? K0 : 0;
+          if (d0 < 6) acc[k] += qyz * (double)cw[j][d0];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (p1 < K1 && p2 < K2) {
+#pragma unroll
+    for (int k = 0; k < kPtPer; ++k)
+      if (px + k < K0) mesh[((int64_t)(px + k) * K1 + p1) * K2 + p2] = (T)acc[k];
+  }
+}
+
 // ------------------------------------------------------------------ host side
 static unsigned grid_for(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, kRedBlocks));
@@ -265,6 +514,18 @@ QeqLayout qeq_layout(const QeqProblem& P) {
   const int64_t n = std::max<int64_t>(P.n, 1);
   const size_t vec = sizeof(double) * n;
   L.c = take(sizeof(double) * std::max<int64_t>(P.n_pairs, 1));
+  const int64_t ent = 2 * std::max<int64_t>(P.n_pairs, 1);
+  L.csr = ent < (int64_t)INT32_MAX && n < (int64_t)INT32_MAX;
+  if (L.csr) {
+    L.rowcnt = take(sizeof(int) * n);
+    L.rowcur = take(sizeof(int) * n);
+    L.rowptr = take(sizeof(int) * (n + 1));
+    L.col = take(sizeof(int) * ent);
+    L.val = take(sizeof(double) * ent);
+    L.scan = take(sizeof(int) * (n / kBlock + 2));
+  } else {
+    L.rowcnt = L.rowcur = L.rowptr = L.col = L.val = L.scan = 0;
+  }
   L.r = take(vec);
   L.z = take(vec);
   L.p = take(vec);
@@ -274,17 +535,35 @@ QeqLayout qeq_layout(const QeqProblem& P) {
   L.dots = take(sizeof(double) * kRedBlocks * std::max(kQeqMaxDots, kQeqMaxCons * kQeqMaxCons));
   L.sc = take(sizeof(double) * 256);  // device scalars, see kSc* below
   L.status = take(sizeof(int) * 8);
+  L.ticket = take(sizeof(int) * 4);
   const size_t es = P.precision == ADMP_F32 ? 4 : 8;
   if (P.kspace) {
     L.cg = make_grid(P.box, P.cutoff, false);
     nb_layout(P.n, L.cg, &L.nb);
     L.nbws = take(L.nb.bytes);
     L.tg = make_tile_geom(L.cg, P.kmesh, es);
+    int64_t nbins = 1;
+    for (int d = 0; d < 3; ++d) {
+      L.nbin[d] = (P.kmesh[d] + kPtBin - 1) / kPtBin;
+      nbins *= L.nbin[d];
+    }
+    L.pt = P.kmesh[0] >= 6 && P.kmesh[1] >= 6 && P.kmesh[2] >= 6 && nbins < (int64_t)INT32_MAX &&
+           n < (int64_t)INT32_MAX;
+    if (L.pt) {
+      L.prec_base = take(sizeof(int4) * n);
+      L.prec_w = take(sizeof(double) * 18 * n);
+      L.pbcnt = take(sizeof(int) * nbins);
+      L.pbcur = take(sizeof(int) * nbins);
+      L.pbstart = take(sizeof(int) * (nbins + 1));
+      L.pbatoms = take(sizeof(int) * n);
+      L.pbscan = take(sizeof(int) * (nbins / kBlock + 2));
+    }
     L.mesh = take((size_t)P.kmesh[0] * P.kmesh[1] * P.kmesh[2] * es);
     L.spec = take((size_t)P.kmesh[0] * P.kmesh[1] * (P.kmesh[2] / 2 + 1) * 2 * es);
     L.ppart = take(sizeof(double) * (std::max(convolve_partials_count(P.kmesh), (size_t)kRedBlocks * 9) + 16));
   } else {
     L.tg.ok = 0;
+    L.pt = 0;
     L.nbws = L.mesh = L.spec = L.ppart = 0;
   }
   L.bytes = off;
@@ -314,17 +593,48 @@ template <typename T>
 static hipError_t matvec(const QeqProblem& P, const QeqLayout& L, char* ws, const Geom& g,
                          const double* v, double* out, hipStream_t s) {
   const int64_t n = P.n;
-  k_diag_mul<<<grid_for(n), kBlock, 0, s>>>(n, P.hdiag, v, out);
-  if (P.n_pairs > 0)
-    k_pair_matvec<<<(unsigned)((P.n_pairs + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-        P.n_pairs, P.pairs, (const double*)(ws + L.c), v, out);
+  double* sc = (double*)(ws + L.sc);
+  const bool global = P.kspace || P.slab;
+  GlobalTerms gt{(const double*)(ws + L.zc), sc + kScSums,
+                 (P.kspace && P.kappa > 0.0) ? -P.eps * kPi / (g.vol * P.kappa * P.kappa) : 0.0,
+                 2.0 * kPi / g.vol * P.eps, P.lz2, P.slab, 0};
+  if (global) {
+    // slots 1..3 hold (sum v, sum v z, sum v z^2) -> sc[kScSums..]
+    k_dots<<<grid_for(n), kBlock, 0, s>>>(n, v, nullptr, nullptr, 0, gt.z,
+                                          (double*)(ws + L.dots), 4, sc + kScSums - 1,
+                                          (int*)(ws + L.ticket));
+  }
+  if (L.csr) {
+    gt.on = global ? 1 : 0;  // the row kernel adds the global terms as it writes
+    k_csr_matvec<<<(unsigned)((16 * n + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        n, (const int*)(ws + L.rowptr), (const int*)(ws + L.col), (const double*)(ws + L.val),
+        P.hdiag, v, gt, out);
+  } else {
+    k_diag_mul<<<grid_for(n), kBlock, 0, s>>>(n, P.hdiag, v, out);
+    if (P.n_pairs > 0)
+      k_pair_matvec<<<(unsigned)((P.n_pairs + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+          P.n_pairs, P.pairs, (const double*)(ws + L.c), v, out);
+    if (global)
+      k_global_terms<<<grid_for(n), kBlock, 0, s>>>(n, gt.z, gt.sums, gt.c_nn, gt.pre_slab,
+                                                    gt.lz2, gt.slab, out);
+  }
   if (P.kspace) {
     T* mesh = (T*)(ws + L.mesh);
-    hipError_t e =
-        L.tg.ok ? launch_spread_tiled<T>(P.pos, v, n, g, L.tg, L.cg.ncell,
-                                         (const int*)(ws + L.nbws + L.nb.sorted),
-                                         (const int*)(ws + L.nbws + L.nb.start), mesh, s)
-                : launch_spread<T>(P.pos, v, n, g, mesh, s);
+    hipError_t e = hipSuccess;
+    if (L.pt) {
+      const int g0 = (g.k[0] + kPtB0 - 1) / kPtB0, g1 = (g.k[1] + kPtB1 - 1) / kPtB1,
+                g2 = (g.k[2] + kPtB2 - 1) / kPtB2;
+      k_point_spread<T><<<(unsigned)((int64_t)g0 * g1 * g2), kPtThreads, 0, s>>>(
+          (const int4*)(ws + L.prec_base), (const T*)(ws + L.prec_w),
+          (const int*)(ws + L.pbstart), (const int*)(ws + L.pbatoms), v, g.k[0], g.k[1], g.k[2],
+          L.nbin[1], L.nbin[2], g1, g2, mesh);
+    } else if (L.tg.ok) {
+      e = launch_spread_tiled<T>(P.pos, v, n, g, L.tg, L.cg.ncell,
+                                 (const int*)(ws + L.nbws + L.nb.sorted),
+                                 (const int*)(ws + L.nbws + L.nb.start), mesh, s);
+    } else {
+      e = launch_spread<T>(P.pos, v, n, g, mesh, s);
+    }
     if (e != hipSuccess) return e;
     e = launch_convolve<T>(mesh, g, P.convention, P.kappa, P.eps, 1.0, mesh, nullptr, nullptr,
                            (cplx<T>*)(ws + L.spec), (double*)(ws + L.ppart), s);
@@ -332,19 +642,6 @@ static hipError_t matvec(const QeqProblem& P, const QeqLayout& L, char* ws, cons
     e = launch_gather<T>(P.pos, v, n, g, mesh, nullptr, out, nullptr, 1,
                          (double*)(ws + L.ppart), s);
     if (e != hipSuccess) return e;
-  }
-  if (P.kspace || P.slab) {
-    double* sc = (double*)(ws + L.sc);
-    const unsigned gb = grid_for(n);
-    k_dots<<<gb, kBlock, 0, s>>>(n, v, nullptr, nullptr, 0, (const double*)(ws + L.zc),
-                                 (double*)(ws + L.dots));
-    // slots 1..3 hold (sum v, sum v z, sum v z^2) -> sc[kScSums..]
-    k_reduce_cols<<<1, kReduceThreads, 0, s>>>((double*)(ws + L.dots), gb, kQeqMaxDots, 4,
-                                               sc + kScSums - 1);
-    const double c_nn = (P.kspace && P.kappa > 0.0) ? -P.eps * kPi / (g.vol * P.kappa * P.kappa) : 0.0;
-    const double pre_slab = 2.0 * kPi / g.vol * P.eps;
-    k_global_terms<<<grid_for(n), kBlock, 0, s>>>(n, (const double*)(ws + L.zc), sc + kScSums, c_nn,
-                                                  pre_slab, P.lz2, P.slab, out);
   }
   return hipGetLastError();
 }
@@ -354,9 +651,8 @@ static void dots_into(const QeqProblem& P, const QeqLayout& L, char* ws, const d
                       const double* y, int dst, hipStream_t s) {
   const unsigned gb = grid_for(P.n);
   double* sc = (double*)(ws + L.sc);
-  k_dots<<<gb, kBlock, 0, s>>>(P.n, x, y, P.cons, P.n_cons, nullptr, (double*)(ws + L.dots));
-  k_reduce_cols<<<1, kReduceThreads, 0, s>>>((double*)(ws + L.dots), gb, kQeqMaxDots,
-                                             1 + P.n_cons, sc + dst);
+  k_dots<<<gb, kBlock, 0, s>>>(P.n, x, y, P.cons, P.n_cons, nullptr, (double*)(ws + L.dots),
+                               1 + P.n_cons, sc + dst, (int*)(ws + L.ticket));
 }
 
 template <typename T>
@@ -377,8 +673,39 @@ static int setup(const QeqProblem& P, const QeqLayout& L, char* ws, Geom* g, hip
     k_pair_coeff<<<(unsigned)((P.n_pairs + kBlock - 1) / kBlock), kBlock, 0, s>>>(
         P.n_pairs, P.pairs, P.ds, P.bs, P.eta, P.coul, P.damping, P.kappa, P.eps,
         (double*)(ws + L.c));
-  if (P.kspace && launch_bin(P.pos, P.n, L.cg, ws + L.nbws, L.nb, s) != hipSuccess)
+  k_fill<int><<<1, 64, 0, s>>>((int*)(ws + L.ticket), 4, 0);  // re-armed by each k_dots
+  if (L.csr && P.n > 0) {
+    int* cnt = (int*)(ws + L.rowcnt);
+    int* cur = (int*)(ws + L.rowcur);
+    int* rowptr = (int*)(ws + L.rowptr);
+    k_fill<int><<<grid_for(P.n), kBlock, 0, s>>>(cnt, P.n, 0);
+    k_fill<int><<<grid_for(P.n), kBlock, 0, s>>>(cur, P.n, 0);
+    const unsigned gp = (unsigned)((P.n_pairs + kBlock - 1) / kBlock);
+    if (P.n_pairs > 0) k_csr_count<<<gp, kBlock, 0, s>>>(P.n_pairs, P.pairs, (const double*)(ws + L.c), cnt);
+    launch_scan<int>(cnt, P.n, rowptr, (int*)(ws + L.scan), s);
+    if (P.n_pairs > 0)
+      k_csr_fill<<<gp, kBlock, 0, s>>>(P.n_pairs, P.pairs, (const double*)(ws + L.c), rowptr, cur,
+                                       (int*)(ws + L.col), (double*)(ws + L.val));
+  }
+  if (L.pt) {
+    const int64_t nbins = (int64_t)L.nbin[0] * L.nbin[1] * L.nbin[2];
+    int* cnt = (int*)(ws + L.pbcnt);
+    int* cur = (int*)(ws + L.pbcur);
+    int* start = (int*)(ws + L.pbstart);
+    const unsigned gbin = (unsigned)std::min<int64_t>((nbins + kBlock - 1) / kBlock, 4096);
+    k_fill<int><<<gbin, kBlock, 0, s>>>(cnt, nbins, 0);
+    k_fill<int><<<gbin, kBlock, 0, s>>>(cur, nbins, 0);
+    if (P.n > 0)
+      k_point_rec<T><<<(unsigned)((P.n + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+          P.pos, P.n, *g, L.nbin[1], L.nbin[2], (int4*)(ws + L.prec_base), (T*)(ws + L.prec_w),
+          cnt);
+    launch_scan<int>(cnt, nbins, start, (int*)(ws + L.pbscan), s);
+    if (P.n > 0)
+      k_point_fill<<<(unsigned)((P.n + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+          P.n, (const int4*)(ws + L.prec_base), start, cur, (int*)(ws + L.pbatoms));
+  } else if (P.kspace && launch_bin(P.pos, P.n, L.cg, ws + L.nbws, L.nb, s) != hipSuccess) {
     return ADMP_ERR_HIP;
+  }
   if (P.slab || P.kspace)
     k_extract_z<<<grid_for(P.n), kBlock, 0, s>>>(P.n, P.pos, P.slab ? P.slab_axis : 0,
                                                   (double*)(ws + L.zc));
@@ -388,7 +715,9 @@ static int setup(const QeqProblem& P, const QeqLayout& L, char* ws, Geom* g, hip
 template <typename T>
 int qeq_matvec_impl(const QeqProblem& P, const double* v, double* out, void* workspace,
                     hipStream_t s) {
-  const QeqLayout L = qeq_layout(P);
+  QeqLayout L = qeq_layout(P);
+  L.csr = 0;  // one product: atomics cost less than building row lists and stencils
+  L.pt = 0;
   char* ws = (char*)workspace;
   Geom g;
   int st = setup<T>(P, L, ws, &g, s);

@@ -40,7 +40,17 @@ struct QeqProblem {
 };
 
 struct QeqLayout {
-  size_t c, r, z, p, w, s, zc, dots, sc, status, nbws, mesh, spec, ppart, bytes;
+  size_t c, r, z, p, w, s, zc, dots, sc, status, ticket, nbws, mesh, spec, ppart, bytes;
+  // row lists of the pair coefficients (both orientations of every pair):
+  // rowcnt/rowcur n ints, rowptr n + 1 ints, col/val 2 n_pairs entries, and the
+  // scan's block sums; csr = 0 when 2 n_pairs does not fit an int32
+  size_t rowcnt, rowcur, rowptr, col, val, scan;
+  int csr;
+  // point spread (each mesh point sums the atoms whose stencil covers it):
+  // per atom the wrapped stencil base and bin (int4) and 18 B-spline weights,
+  // atoms binned by base in 4-point bins (nbin per axis); pt = 0 when off
+  size_t prec_base, prec_w, pbcnt, pbcur, pbstart, pbatoms, pbscan;
+  int pt, nbin[3];
   CellGrid cg;
   NbLayout nb;
   TileGeom tg;
