@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--serving-requests", type=int, default=4096,
                     help="requests of the mixed-size serving stream (0 skips it)")
     ap.add_argument("--no-qeq", action="store_true", help="skip the QEq and electrode timings")
+    ap.add_argument("--qeq-only", action="store_true",
+                    help="run only the 3000-atom QEq timing and print its JSON (for profiling)")
     ap.add_argument("--batch-copies", type=int, default=64,
                     help="copies of the water box in the batched roofline run (0 skips it)")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay timing")
@@ -124,7 +126,9 @@ def qeq_bench(torch, dev):
         pairs = nbl(pos, box)
         ds, bs = nbl.get_ds(), nbl.get_buffer_scales()
         tol = 1e-8 / n if prec == "float64" else 1e-6 / n
-        mod.solve_implicit(pos, box, chi, hard, eta, pairs, ds, bs, A, b, tol=tol)  # warm-up
+        q = mod.solve_implicit(pos, box, chi, hard, eta, pairs, ds, bs, A, b, tol=tol)  # warm-up
+        # and its backward, untimed (the pair distances' graph is kept for the timed one)
+        torch.autograd.grad((w * q).sum(), (pos, chi), retain_graph=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         q = mod.solve_implicit(pos, box, chi, hard, eta, pairs, ds, bs, A, b, tol=tol)
@@ -354,7 +358,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(systems are independent, see DESIGN.md for how the path shards)", file=sys.stderr)
+              "(systems are independent: sharding assigns whole systems to GPUs)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
@@ -366,6 +370,9 @@ def main():
     from torch_admp_amd.workloads import spc_water
 
     dev = torch.device("cuda:0")
+    if args.qeq_only:
+        print(json.dumps({"qeq_3000": qeq_bench(torch, dev)}), flush=True)
+        return
     n_mol, rc, ethresh, kmesh = 8000, 8.0, 1e-5, 64
     pos_np, q_np, box_np = spc_water(n_mol, seed=2024)
     n = pos_np.shape[0]
