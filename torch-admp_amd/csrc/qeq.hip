@@ -338,6 +338,104 @@ __global__ void k_extract_z(int64_t n, const double* __restrict__ pos, int axis,
     z[i] = pos[3 * i + axis];
 }
 
+// ------------------------------------------------------------------ small systems
+// One work-group carries the vector part of a PCG iteration for n <= kPcgSmallN
+// (the Hessian product w = H p has been formed): w.p and A w, the step
+// (alpha, projected w, q and r), |r|^2 for the host's residual check, the
+// projected preconditioned residual z, r.z and the new direction p, with
+// work-group barriers between the reductions instead of kernel boundaries.
+// Fixed-order reductions (deterministic); same arithmetic as k_cg_step,
+// k_precond, k_project and k_cg_dir.
+constexpr int kPcgThreads = 1024;
+constexpr int64_t kPcgSmallN = 16384;
+
+// Sums over the work-group in every thread: wavefront sums, then thread i < NV
+// adds value i over the wavefronts in order.  smem holds NV (waves + 1) doubles.
+template <int NV>
+__device__ __forceinline__ void block_allsum(double (&v)[NV], double* smem) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) smem[wid * NV + i] = v[i];
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double t = 0.0;
+    for (int u = 0; u < nw; ++u) t += smem[u * NV + threadIdx.x];
+    smem[nw * NV + threadIdx.x] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = smem[nw * NV + i];
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kPcgThreads) k_pcg_vector(
+    int64_t n, double* __restrict__ q, double* __restrict__ r, double* __restrict__ p,
+    const double* __restrict__ w, double* __restrict__ z, const double* __restrict__ A, int nc,
+    const double* __restrict__ G, const double* __restrict__ m, double* __restrict__ sc,
+    int rz_old, int rz_new, int rr_slot) {
+  __shared__ double red[(kQeqMaxCons + 1) * (kPcgThreads / 64 + 1)];
+  const int64_t t0 = threadIdx.x, st = blockDim.x;
+  // w.p and A w
+  double d[kQeqMaxCons + 1];
+#pragma unroll
+  for (int c = 0; c <= kQeqMaxCons; ++c) d[c] = 0.0;
+  for (int64_t i = t0; i < n; i += st) {
+    const double wi = w[i];
+    d[0] += wi * p[i];
+    for (int c = 0; c < nc; ++c) d[1 + c] += A[(int64_t)c * n + i] * wi;
+  }
+  block_allsum<kQeqMaxCons + 1>(d, red);
+  const double pw = d[0];
+  const double alpha = pw != 0.0 ? sc[rz_old] / pw : 0.0;
+  double y[kQeqMaxCons];
+  proj_coeff(G, d + 1, nullptr, nc, y);
+  // step; |r|^2
+  double e[1] = {0.0};
+  for (int64_t i = t0; i < n; i += st) {
+    double pw_i = w[i];
+    for (int c = 0; c < nc; ++c) pw_i -= A[(int64_t)c * n + i] * y[c];
+    q[i] += alpha * p[i];
+    const double ri = r[i] - alpha * pw_i;
+    r[i] = ri;
+    e[0] += ri * ri;
+  }
+  block_allsum<1>(e, red);
+  if (threadIdx.x == 0) sc[rr_slot] = e[0];
+  // z = M^-1 r, projected
+#pragma unroll
+  for (int c = 0; c <= kQeqMaxCons; ++c) d[c] = 0.0;
+  for (int64_t i = t0; i < n; i += st) {
+    const double zi = m ? r[i] / m[i] : r[i];
+    z[i] = zi;
+    for (int c = 0; c < nc; ++c) d[c] += A[(int64_t)c * n + i] * zi;
+  }
+  if (nc > 0) {
+    block_allsum<kQeqMaxCons + 1>(d, red);
+    proj_coeff(G, d, nullptr, nc, y);
+  }
+  // r.z of the projected z
+  e[0] = 0.0;
+  for (int64_t i = t0; i < n; i += st) {
+    double zi = z[i];
+    if (nc > 0) {
+      double s = 0.0;
+      for (int c = 0; c < nc; ++c) s += A[(int64_t)c * n + i] * y[c];
+      zi -= s;
+      z[i] = zi;
+    }
+    e[0] += r[i] * zi;
+  }
+  block_allsum<1>(e, red);
+  const double rzn = e[0], rzo = sc[rz_old];
+  const double beta = rzo != 0.0 ? rzn / rzo : 0.0;
+  for (int64_t i = t0; i < n; i += st) p[i] = z[i] + beta * p[i];
+  __syncthreads();  // every thread has read sc[rz_old] before it can be overwritten
+  if (threadIdx.x == 0) sc[rz_new] = rzn;
+}
+
 // ------------------------------------------------------------------ point spread
 // The Hessian product spreads a new charge vector on fixed positions every
 // iteration.  Once per geometry each atom's stencil is stored: base b_d =
@@ -792,6 +890,23 @@ int qeq_solve_impl(const QeqProblem& P, const double* q0, double tol, int max_it
     const int rz_old = (it & 1) ? kScRz0 : kScRz1;
     const int rz_new = (it & 1) ? kScRz1 : kScRz0;
     if (matvec<T>(P, L, ws, g, p, w, s) != hipSuccess) return ADMP_ERR_HIP;
+    if (n <= kPcgSmallN) {
+      // the whole vector part in one work-group; |r|^2 lands in sc[kScRr]
+      k_pcg_vector<<<1, kPcgThreads, 0, s>>>(n, q, r, p, w, z, P.cons, nc, G, P.precond, sc,
+                                              rz_old, rz_new, kScRr);
+      if (it % check_every == 0 || it == max_iter) {
+        if (hipMemcpyAsync(&res, sc + kScRr, sizeof(double), hipMemcpyDeviceToHost, s) !=
+                hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+          return ADMP_ERR_HIP;
+        res = std::sqrt(std::max(res, 0.0)) / (double)std::max<int64_t>(n, 1);
+        if (res < tol) {
+          converged = true;
+          break;
+        }
+      }
+      continue;
+    }
     dots_into(P, L, ws, w, p, kScDots, s);  // w.p and A w (A acts on the first argument)
     k_cg_step<<<gb, kBlock, 0, s>>>(n, q, r, p, w, P.cons, nc, G, sc + rz_old, sc + kScDots);
     if (it % check_every == 0 || it == max_iter) {
