@@ -375,7 +375,7 @@ __global__ void __launch_bounds__(kPcgThreads) k_pcg_vector(
     int64_t n, double* __restrict__ q, double* __restrict__ r, double* __restrict__ p,
     const double* __restrict__ w, double* __restrict__ z, const double* __restrict__ A, int nc,
     const double* __restrict__ G, const double* __restrict__ m, double* __restrict__ sc,
-    int rz_old, int rz_new, int rr_slot) {
+    int rz_old, int rz_new, int rr_slot, const double* __restrict__ zc, int sums_slot) {
   __shared__ double red[(kQeqMaxCons + 1) * (kPcgThreads / 64 + 1)];
   const int64_t t0 = threadIdx.x, st = blockDim.x;
   // w.p and A w
@@ -431,7 +431,24 @@ __global__ void __launch_bounds__(kPcgThreads) k_pcg_vector(
   block_allsum<1>(e, red);
   const double rzn = e[0], rzo = sc[rz_old];
   const double beta = rzo != 0.0 ? rzn / rzo : 0.0;
-  for (int64_t i = t0; i < n; i += st) p[i] = z[i] + beta * p[i];
+  // new direction; with zc, also (sum p, sum p z, sum p z^2) for the next
+  // Hessian product's global terms (sc[sums_slot..])
+  double g3[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = t0; i < n; i += st) {
+    const double pi = z[i] + beta * p[i];
+    p[i] = pi;
+    if (zc) {
+      const double zi = zc[i];
+      g3[0] += pi;
+      g3[1] += pi * zi;
+      g3[2] += pi * zi * zi;
+    }
+  }
+  if (zc) {
+    block_allsum<3>(g3, red);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 3; ++k) sc[sums_slot + k] = g3[k];
+  }
   __syncthreads();  // every thread has read sc[rz_old] before it can be overwritten
   if (threadIdx.x == 0) sc[rz_new] = rzn;
 }
@@ -687,16 +704,17 @@ __global__ void __launch_bounds__(kReduceThreads) k_reduce_cols(const double* __
   }
 }
 
+// sums_ready: sc[kScSums..] already holds the sums over v of the global terms
 template <typename T>
 static hipError_t matvec(const QeqProblem& P, const QeqLayout& L, char* ws, const Geom& g,
-                         const double* v, double* out, hipStream_t s) {
+                         const double* v, double* out, hipStream_t s, bool sums_ready = false) {
   const int64_t n = P.n;
   double* sc = (double*)(ws + L.sc);
   const bool global = P.kspace || P.slab;
   GlobalTerms gt{(const double*)(ws + L.zc), sc + kScSums,
                  (P.kspace && P.kappa > 0.0) ? -P.eps * kPi / (g.vol * P.kappa * P.kappa) : 0.0,
                  2.0 * kPi / g.vol * P.eps, P.lz2, P.slab, 0};
-  if (global) {
+  if (global && !sums_ready) {
     // slots 1..3 hold (sum v, sum v z, sum v z^2) -> sc[kScSums..]
     k_dots<<<grid_for(n), kBlock, 0, s>>>(n, v, nullptr, nullptr, 0, gt.z,
                                           (double*)(ws + L.dots), 4, sc + kScSums - 1,
@@ -889,11 +907,17 @@ int qeq_solve_impl(const QeqProblem& P, const double* q0, double tol, int max_it
     ++it;
     const int rz_old = (it & 1) ? kScRz0 : kScRz1;
     const int rz_new = (it & 1) ? kScRz1 : kScRz0;
-    if (matvec<T>(P, L, ws, g, p, w, s) != hipSuccess) return ADMP_ERR_HIP;
-    if (n <= kPcgSmallN) {
+    const bool small = n <= kPcgSmallN;
+    const bool global = P.kspace || P.slab;
+    // after the first iteration the vector step has summed p for the global terms
+    if (matvec<T>(P, L, ws, g, p, w, s, small && global && it > 1) != hipSuccess)
+      return ADMP_ERR_HIP;
+    if (small) {
       // the whole vector part in one work-group; |r|^2 lands in sc[kScRr]
       k_pcg_vector<<<1, kPcgThreads, 0, s>>>(n, q, r, p, w, z, P.cons, nc, G, P.precond, sc,
-                                              rz_old, rz_new, kScRr);
+                                              rz_old, rz_new, kScRr,
+                                              global ? (const double*)(ws + L.zc) : nullptr,
+                                              kScSums);
       if (it % check_every == 0 || it == max_iter) {
         if (hipMemcpyAsync(&res, sc + kScRr, sizeof(double), hipMemcpyDeviceToHost, s) !=
                 hipSuccess ||
