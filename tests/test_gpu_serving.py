@@ -10,6 +10,8 @@ pair and atomic virials.  Results also match the oracle within the float32
 bars (energy 1e-6 relative, forces 1e-4 eV/A).
 """
 
+import sys
+
 import numpy as np
 import pytest
 import torch
@@ -100,12 +102,11 @@ def test_qeq_requests_served_beside_pme(dev):
         pos, box, chi, hard, eta = qeq_species_box(n, length, seed=20 + k)
         qeq.append(serving.QEqRequest(pos, box, chi, hard, eta, rcut=6.0, total_charge=qtot,
                                       tol=1e-10, arrival=0.0))
-    server = serving.PMEServer(dev, hbm_budget=256 << 20, max_systems=8, streams=2,
-                               precision="float64")
-    try:
+    switch = sys.getswitchinterval()
+    with serving.PMEServer(dev, hbm_budget=256 << 20, max_systems=8, streams=2,
+                           precision="float64") as server:
         stats = server.serve(pme + qeq)
-    finally:
-        server.close()
+    assert sys.getswitchinterval() == switch  # restored when the worker stops
     assert stats["requests"] == len(pme) + len(qeq)
     assert stats["qeq"]["requests"] == len(qeq) and stats["pme"]["requests"] == len(pme)
     assert server.bytes_in_flight == 0

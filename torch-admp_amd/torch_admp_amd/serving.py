@@ -292,6 +292,12 @@ class PMEServer:
         req.iterations = int(mod.converge_iter)
         req.residual = float(mod.residual)
 
+    def __enter__(self) -> "PMEServer":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
     def close(self) -> None:
         """Stop the QEq worker (queued QEq requests are dropped) and restore the
         thread switch interval it set."""
