@@ -56,13 +56,15 @@ def test_pgrad_solve_matches_kkt(dev, precision):
     assert torch.isfinite(f).all()
 
 
-@pytest.mark.parametrize("kmesh", [[7, 11, 13], [6, 9, 21], [17, 18, 33]])
+@pytest.mark.parametrize("kmesh", [[7, 11, 13], [6, 9, 21], [17, 18, 33], [12, 18, 30]])
 def test_pcg_solve_odd_meshes(dev, kmesh):
     """The solver spreads each iteration by mesh point over atoms binned by
     stencil base (4-point bins, work-groups of 16 x 8 x 16 points): meshes
     with partial bins and blocks, wrap-around within one block and the
     smallest mesh it accepts (6) against the oracle's KKT solution, with a
-    nonzero total charge."""
+    nonzero total charge.  Even K1 and K2 ([12, 18, 30]: radices 4 3, 2 3 3,
+    2 3 5) take the one-work-group-per-x-slab z and y transforms; odd ones
+    the line passes."""
     pos, b, chi, hard, eta, pairs, ds, bs = qeq_system(n=70, L=10.5, seed=4)
     n = pos.shape[0]
     orc = qeq_ref.QEqOracle(4.5, kmesh=kmesh)

@@ -533,6 +533,85 @@ __device__ __forceinline__ void pass_inplace(cplx<T>* buf, int L, int C, int es,
   __syncthreads();
 }
 
+// In-place pass with up to B butterflies per thread: all reads into
+// registers, a barrier, then all writes.
+template <typename T, int SIGN, int R, int B>
+__device__ __forceinline__ void pass_ip(cplx<T>* buf, int L, int C, int es, int cs, int p,
+                                        const cplx<T>* tw) {
+  using CT = cplx<T>;
+  const int nb = L / R;
+  const int tstep = L / (p * R);
+  const int nbf = nb * C;
+  CT u[B][R];
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int idx = threadIdx.x + b * blockDim.x;
+    if (idx < nbf) {
+      int c, i, k, ip;
+      divmod_u(idx, C, i, c);
+      divmod_u(i, p, ip, k);
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        CT v = buf[(i + j * nb) * es + c * cs];
+        if (j > 0 && k > 0) v = cmul(v, tw_get<T>(tw, k * j * tstep, SIGN));  // < L
+        u[b][j] = v;
+      }
+      dft_r<T, SIGN, R>(u[b]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int idx = threadIdx.x + b * blockDim.x;
+    if (idx < nbf) {
+      int c, i, k, ip;
+      divmod_u(idx, C, i, c);
+      divmod_u(i, p, ip, k);
+      const int base = (i - k) * R + k;
+#pragma unroll
+      for (int j = 0; j < R; ++j) buf[(base + j * p) * es + c * cs] = u[b][j];
+    }
+  }
+  __syncthreads();
+}
+
+// Butterflies per thread and pass that pass_ip_any handles for radix R (at
+// most 16 complex values in registers); the plan checks (L / R) C against it.
+__host__ __device__ constexpr int ip_max_b(int R) {
+  return R == 8 ? 2 : (R == 5 || R == 4) ? 3 : R == 3 ? 4 : 5;
+}
+
+template <typename T, int SIGN, int R>
+__device__ __forceinline__ void pass_ip_any(cplx<T>* buf, int L, int C, int es, int cs, int p,
+                                            const cplx<T>* tw) {
+  constexpr int kMax = ip_max_b(R);
+  const int bpt = ((L / R) * C + (int)blockDim.x - 1) / (int)blockDim.x;
+  if (bpt <= 1) pass_ip<T, SIGN, R, 1>(buf, L, C, es, cs, p, tw);
+  else if (bpt == 2 || kMax == 2) pass_ip<T, SIGN, R, 2>(buf, L, C, es, cs, p, tw);
+  else if (bpt == 3 || kMax == 3) pass_ip<T, SIGN, R, (kMax >= 3 ? 3 : 2)>(buf, L, C, es, cs, p, tw);
+  else if (bpt == 4 || kMax == 4) pass_ip<T, SIGN, R, (kMax >= 4 ? 4 : 2)>(buf, L, C, es, cs, p, tw);
+  else pass_ip<T, SIGN, R, kMax>(buf, L, C, es, cs, p, tw);
+}
+
+// C lines of length L in place (element stride es, line stride cs); radices
+// 2, 3, 4, 5, 8 and at most ip_max_b(r) butterflies per thread in every pass.
+template <typename T, int SIGN>
+__device__ __forceinline__ void stockham_ip(cplx<T>* buf, int L, int C, int es, int cs,
+                                            const Radix& rd, const cplx<T>* tw) {
+  int p = 1;
+  for (int s = 0; s < rd.n; ++s) {
+    const int r = rd.r[s];
+    switch (r) {
+      case 2: pass_ip_any<T, SIGN, 2>(buf, L, C, es, cs, p, tw); break;
+      case 3: pass_ip_any<T, SIGN, 3>(buf, L, C, es, cs, p, tw); break;
+      case 4: pass_ip_any<T, SIGN, 4>(buf, L, C, es, cs, p, tw); break;
+      case 5: pass_ip_any<T, SIGN, 5>(buf, L, C, es, cs, p, tw); break;
+      default: pass_ip_any<T, SIGN, 8>(buf, L, C, es, cs, p, tw); break;
+    }
+    p *= r;
+  }
+}
+
 // C lines of length L, element stride es, line stride cs; radices 2, 3, 4, 5,
 // 8 with (L / r) * C <= kPlaneB * blockDim.x (checked when planning).
 template <typename T, int SIGN>

@@ -5,6 +5,7 @@
 // Replaces the torch-op chains of src/torch_admp/recip.py:13-252 (spread),
 // torch.fft.fftn at src/torch_admp/pme.py:434, the k-space sum of
 // pme.py:410-443 and 661-683, and the autograd transpose of the spread.
+#include <atomic>
 #include "fft_device.hpp"
 #include "mesh_device.hpp"
 #include "pme_kernels.hpp"
@@ -122,6 +123,130 @@ __global__ void __launch_bounds__(kBlock) k_fft_x_convolve(cplx<T>* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Passes 1-2 (and 4-5) for one x slab whose half spectrum K1 x (K2/2 + 1)
+// fits in LDS: one work-group per slab packs real rows 2p and 2p + 1 into
+// complex line p, transforms along z, untangles both rows' half spectra in
+// place (line p holds row 2p at [0, Kzh) and row 2p + 1 at [Kzh, 2 Kzh), so
+// the buffer becomes the [y][kz] plane with pitch Kzh), transforms along y
+// and stores the slab.  The inverse runs the same steps backwards, with
+// Z_k = A_k + i B_k and A, B extended by Hermitian symmetry (c2r_rows).
+// Same arithmetic as r2c_rows / strided_lines / c2r_rows.
+constexpr int kYzThreads = 1024;
+constexpr int kYzPack = 8;   // packed elements per thread in the inverse (checked when planning)
+constexpr int kYzSplit = 4;  // untangled element pairs per thread in the forward
+
+template <typename T>
+__global__ void __launch_bounds__(kYzThreads) k_fft_yz_fwd(const T* __restrict__ mesh,
+                                                            cplx<T>* __restrict__ spec, int K1,
+                                                            int K2, Radix ry, Radix rz) {
+  using CT = cplx<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int Kzh = K2 / 2 + 1, np = K1 / 2, pitch2 = 2 * Kzh;
+  CT* buf = reinterpret_cast<CT*>(smem_raw);
+  CT* twz = buf + (size_t)K1 * Kzh;
+  CT* twy = twz + K2;
+  build_twiddles<T>(twz, K2);
+  build_twiddles<T>(twy, K1);
+  const T* src = mesh + (int64_t)blockIdx.x * K1 * K2;
+  for (int idx = threadIdx.x; idx < np * K2; idx += blockDim.x) {
+    int pr, z;
+    divmod_u(idx, K2, pr, z);
+    CT v;
+    v.x = src[(int64_t)(2 * pr) * K2 + z];
+    v.y = src[(int64_t)(2 * pr + 1) * K2 + z];
+    buf[pr * pitch2 + z] = v;
+  }
+  __syncthreads();
+  stockham_ip<T, -1>(buf, K2, np, 1, pitch2, rz, twz);
+  // untangle: row 2p <- (Z_k + conj Z_-k) / 2, row 2p + 1 <- (Z_k - conj Z_-k) / 2i
+  CT ea[kYzSplit], eb[kYzSplit];
+#pragma unroll
+  for (int u = 0; u < kYzSplit; ++u) {
+    const int idx = threadIdx.x + u * blockDim.x;
+    if (idx < np * Kzh) {
+      int pr, k;
+      divmod_u(idx, Kzh, pr, k);
+      const CT zk = buf[pr * pitch2 + k];
+      const CT zm = buf[pr * pitch2 + (k ? K2 - k : 0)];
+      ea[u].x = T(0.5) * (zk.x + zm.x);
+      ea[u].y = T(0.5) * (zk.y - zm.y);
+      eb[u].x = T(0.5) * (zk.y + zm.y);
+      eb[u].y = T(-0.5) * (zk.x - zm.x);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kYzSplit; ++u) {
+    const int idx = threadIdx.x + u * blockDim.x;
+    if (idx < np * Kzh) {
+      int pr, k;
+      divmod_u(idx, Kzh, pr, k);
+      buf[pr * pitch2 + k] = ea[u];
+      buf[pr * pitch2 + Kzh + k] = eb[u];
+    }
+  }
+  __syncthreads();
+  stockham_ip<T, -1>(buf, K1, Kzh, Kzh, 1, ry, twy);  // along y: lines kz
+  CT* dst = spec + (int64_t)blockIdx.x * K1 * Kzh;
+  for (int idx = threadIdx.x; idx < K1 * Kzh; idx += blockDim.x) dst[idx] = buf[idx];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kYzThreads) k_fft_yz_inv(const cplx<T>* __restrict__ spec,
+                                                            T* __restrict__ mesh, int K1, int K2,
+                                                            Radix ry, Radix rz) {
+  using CT = cplx<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int Kzh = K2 / 2 + 1, np = K1 / 2, pitch2 = 2 * Kzh;
+  CT* buf = reinterpret_cast<CT*>(smem_raw);
+  CT* twz = buf + (size_t)K1 * Kzh;
+  CT* twy = twz + K2;
+  build_twiddles<T>(twz, K2);
+  build_twiddles<T>(twy, K1);
+  const CT* src = spec + (int64_t)blockIdx.x * K1 * Kzh;
+  for (int idx = threadIdx.x; idx < K1 * Kzh; idx += blockDim.x) buf[idx] = src[idx];
+  __syncthreads();
+  stockham_ip<T, 1>(buf, K1, Kzh, Kzh, 1, ry, twy);  // inverse along y
+  // pack: Z_k = A_k + i B_k for k < K2 (A, B of rows 2p, 2p + 1, Hermitian-extended)
+  CT ez[kYzPack];
+#pragma unroll
+  for (int u = 0; u < kYzPack; ++u) {
+    const int idx = threadIdx.x + u * blockDim.x;
+    if (idx < np * K2) {
+      int pr, k;
+      divmod_u(idx, K2, pr, k);
+      const int kk = k < Kzh ? k : K2 - k;
+      const T sg = k < Kzh ? T(1) : T(-1);
+      CT A = buf[pr * pitch2 + kk], B = buf[pr * pitch2 + Kzh + kk];
+      A.y *= sg;
+      B.y *= sg;
+      ez[u].x = A.x - B.y;
+      ez[u].y = A.y + B.x;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kYzPack; ++u) {
+    const int idx = threadIdx.x + u * blockDim.x;
+    if (idx < np * K2) {
+      int pr, k;
+      divmod_u(idx, K2, pr, k);
+      buf[pr * pitch2 + k] = ez[u];
+    }
+  }
+  __syncthreads();
+  stockham_ip<T, 1>(buf, K2, np, 1, pitch2, rz, twz);  // inverse along z
+  T* dst = mesh + (int64_t)blockIdx.x * K1 * K2;
+  for (int idx = threadIdx.x; idx < np * K2; idx += blockDim.x) {
+    int pr, z;
+    divmod_u(idx, K2, pr, z);
+    const CT v = buf[pr * pitch2 + z];
+    dst[(int64_t)(2 * pr) * K2 + z] = v.x;
+    dst[(int64_t)(2 * pr + 1) * K2 + z] = v.y;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host-side launchers
 
 Radix factorize(int L) {
@@ -195,6 +320,40 @@ hipError_t launch_gather(const double* pos, const double* q, int64_t n, const Ge
   return hipGetLastError();
 }
 
+// Dynamic LDS bytes of the slab kernels for this mesh, or 0 when they do not
+// apply: even K1 and K2, radices 2/3/4/5/8, at most ip_max_b(r) butterflies
+// and kYzPack / kYzSplit elements per thread, and the device's per-work-group
+// LDS.
+static size_t yz_lds(int K1, int K2, size_t elem) {
+  if ((K1 & 1) || (K2 & 1)) return 0;
+  const Radix ry = factorize(K1), rz = factorize(K2);
+  if (!radix_ok(ry) || !radix_ok(rz)) return 0;
+  const int Kzh = K2 / 2 + 1, np = K1 / 2;
+  auto fits = [](const Radix& rd, int L, int C) {
+    for (int i = 0; i < rd.n; ++i) {
+      const int r = rd.r[i];
+      if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8) return false;
+      if ((int64_t)(L / r) * C > (int64_t)ip_max_b(r) * kYzThreads) return false;
+    }
+    return true;
+  };
+  if (!fits(rz, K2, np) || !fits(ry, K1, Kzh)) return 0;
+  if ((int64_t)np * K2 > (int64_t)kYzPack * kYzThreads) return 0;
+  if ((int64_t)np * Kzh > (int64_t)kYzSplit * kYzThreads) return 0;
+  const size_t lds = ((size_t)K1 * Kzh + K1 + K2) * elem;
+  static std::atomic<int> max_lds[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  int lim = max_lds[dev].load(std::memory_order_relaxed);
+  if (lim == 0) {
+    if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+            hipSuccess || lim <= 0)
+      return 0;
+    max_lds[dev].store(lim, std::memory_order_relaxed);
+  }
+  return lds <= (size_t)lim ? lds : 0;
+}
+
 template <typename T>
 hipError_t launch_convolve(const T* mesh, const Geom& g, int conv, double kappa, double eps,
                            double phi_scale, T* phi, double* energy, double* dEdA,
@@ -204,6 +363,11 @@ hipError_t launch_convolve(const T* mesh, const Geom& g, int conv, double kappa,
   const int Kzh = K2 / 2 + 1;
   const Radix r0 = factorize(K0), r1 = factorize(K1), r2 = factorize(K2);
   const int64_t nrows = (int64_t)K0 * K1;
+  const size_t yz = yz_lds(K1, K2, sizeof(CT));
+  if (yz) {
+    // passes 1-2 in one work-group per x slab
+    k_fft_yz_fwd<T><<<K0, kYzThreads, yz, s>>>(mesh, spec, K1, K2, r1, r2);
+  } else {
   // pass 1: r2c along z
   {
     const int C = pick_chunk(K2, (nrows + 1) / 2, sizeof(CT));
@@ -217,6 +381,7 @@ hipError_t launch_convolve(const T* mesh, const Geom& g, int conv, double kappa,
     dim3 grid((Kzh + C - 1) / C, K0);
     const size_t sh = (size_t)(K1 + 2 * K1 * C) * sizeof(CT);
     k_fft_cols<T, -1><<<grid, kBlock, sh, s>>>(spec, K1, Kzh, Kzh, (int64_t)K1 * Kzh, C, r1);
+  }
   }
   // pass 3: along x with the influence function
   InflParams P;
@@ -237,7 +402,10 @@ hipError_t launch_convolve(const T* mesh, const Geom& g, int conv, double kappa,
   const int nb3 = grid3.x * grid3.y;
   if (energy || dEdA)
     k_reduce_partials<<<1, kReduceThreads, 0, s>>>(partials, nb3, 10, eps, energy, dEdA);
-  if (phi) {
+  if (phi && yz) {
+    // passes 4-5 in one work-group per x slab
+    k_fft_yz_inv<T><<<K0, kYzThreads, yz, s>>>(spec, phi, K1, K2, r1, r2);
+  } else if (phi) {
     // pass 4: inverse along y
     {
       const int C = pick_chunk(K1, (int64_t)K0 * Kzh, sizeof(CT));
