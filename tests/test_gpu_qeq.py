@@ -84,6 +84,35 @@ def test_pcg_solve_odd_meshes(dev, kmesh):
     assert float((hv - ref).abs().max()) <= 1e-10 * float(ref.abs().max())
 
 
+def test_full_size_solve_is_optimal_under_the_other_product(dev):
+    """3000 atoms of 3 species, 30 A cube, rc 6 A, mesh 120^3 (the timed QEq
+    workload): the solver's charges (row-list pair product, point spread,
+    x-slab transforms, one-work-group vector step) satisfy the optimality
+    condition evaluated by the single Hessian product, which takes the other
+    code paths (pair atomics, atomic spread): |P (H q + chi)| / n within 10x
+    the solve tolerance, and sum q = 0."""
+    from torch_admp_amd.nblist import TorchNeighborList
+    from torch_admp_amd.workloads import qeq_species_box
+
+    pos, box, chi, hard, eta = [torch.tensor(a, device=dev)
+                                for a in qeq_species_box(3000, 30.0, seed=3)]
+    n = pos.shape[0]
+    nbl = TorchNeighborList(6.0)
+    mod = QEqForceModule(6.0, 1e-5, kmesh=[120, 120, 120])
+    A = torch.ones(1, n, dtype=F64, device=dev)
+    b = torch.zeros(1, dtype=F64, device=dev)
+    tol = 1e-10 / n
+    with torch.no_grad():
+        pairs = nbl(pos, box)
+        ds, bs = nbl.get_ds(), nbl.get_buffer_scales()
+        q = mod.solve_implicit(pos, box, chi, hard, eta, pairs, ds, bs, A, b, tol=tol)
+        g = mod.hessian_matvec(q, pos, box, chi, hard, eta, pairs, ds, bs) + chi
+    assert mod.converge_iter > 0
+    pg = g - g.mean()
+    assert float(pg.norm()) / n < 10 * tol
+    assert abs(float(q.sum())) < 1e-10
+
+
 def test_matrix_inversion_and_forces_match_oracle(dev):
     pos, b, chi, hard, eta, pairs, ds, bs = qeq_system(n=50, seed=4)
     n = pos.shape[0]
