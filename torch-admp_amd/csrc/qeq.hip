@@ -842,6 +842,28 @@ int qeq_matvec_impl(const QeqProblem& P, const double* v, double* out, void* wor
   return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
 }
 
+// A pinned host word and an event for reading the residual without draining
+// the stream: one pair per thread and device, created on first use and kept
+// for the life of the process.  Returns false when either cannot be made.
+static bool residual_reader(double** host, hipEvent_t* ev) {
+  constexpr int kDev = 16;
+  static thread_local double* t_host = nullptr;
+  static thread_local hipEvent_t t_ev[kDev] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDev) return false;
+  if (!t_host && hipHostMalloc((void**)&t_host, 64, hipHostMallocDefault) != hipSuccess) {
+    t_host = nullptr;
+    return false;
+  }
+  if (!t_ev[dev] && hipEventCreateWithFlags(&t_ev[dev], hipEventDisableTiming) != hipSuccess) {
+    t_ev[dev] = nullptr;
+    return false;
+  }
+  *host = t_host;
+  *ev = t_ev[dev];
+  return true;
+}
+
 template <typename T>
 int qeq_solve_impl(const QeqProblem& P, const double* q0, double tol, int max_iter,
                    int check_every, double* q, int* iters, double* resid, void* workspace,
@@ -903,6 +925,10 @@ int qeq_solve_impl(const QeqProblem& P, const double* q0, double tol, int max_it
     return ADMP_ERR_HIP;
   res = std::sqrt(std::max(res, 0.0)) / (double)std::max<int64_t>(n, 1);
   converged = res < tol;
+  double* rr_host = nullptr;
+  hipEvent_t rr_ev = nullptr;
+  const bool ahead = n <= kPcgSmallN && residual_reader(&rr_host, &rr_ev);
+  bool product_queued = false;  // this iteration's H p was enqueued ahead of a residual read
   while (!converged && it < max_iter) {
     ++it;
     const int rz_old = (it & 1) ? kScRz0 : kScRz1;
@@ -910,8 +936,34 @@ int qeq_solve_impl(const QeqProblem& P, const double* q0, double tol, int max_it
     const bool small = n <= kPcgSmallN;
     const bool global = P.kspace || P.slab;
     // after the first iteration the vector step has summed p for the global terms
-    if (matvec<T>(P, L, ws, g, p, w, s, small && global && it > 1) != hipSuccess)
+    if (!product_queued && matvec<T>(P, L, ws, g, p, w, s, small && global && it > 1) != hipSuccess)
       return ADMP_ERR_HIP;
+    product_queued = false;
+    if (small && ahead) {
+      k_pcg_vector<<<1, kPcgThreads, 0, s>>>(n, q, r, p, w, z, P.cons, nc, G, P.precond, sc,
+                                              rz_old, rz_new, kScRr,
+                                              global ? (const double*)(ws + L.zc) : nullptr,
+                                              kScSums);
+      if (it % check_every == 0 || it == max_iter) {
+        // copy |r|^2 to pinned memory, queue the next product (it only writes w),
+        // then wait for the copy alone: the GPU stays busy while the host decides
+        if (hipMemcpyAsync(rr_host, sc + kScRr, sizeof(double), hipMemcpyDeviceToHost, s) !=
+                hipSuccess ||
+            hipEventRecord(rr_ev, s) != hipSuccess)
+          return ADMP_ERR_HIP;
+        if (it < max_iter) {
+          if (matvec<T>(P, L, ws, g, p, w, s, global) != hipSuccess) return ADMP_ERR_HIP;
+          product_queued = true;
+        }
+        if (hipEventSynchronize(rr_ev) != hipSuccess) return ADMP_ERR_HIP;
+        res = std::sqrt(std::max(*rr_host, 0.0)) / (double)std::max<int64_t>(n, 1);
+        if (res < tol) {
+          converged = true;
+          break;
+        }
+      }
+      continue;
+    }
     if (small) {
       // the whole vector part in one work-group; |r|^2 lands in sc[kScRr]
       k_pcg_vector<<<1, kPcgThreads, 0, s>>>(n, q, r, p, w, z, P.cons, nc, G, P.precond, sc,
