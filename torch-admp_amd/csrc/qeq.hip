@@ -349,10 +349,12 @@ __global__ void k_extract_z(int64_t n, const double* __restrict__ pos, int axis,
 constexpr int kPcgThreads = 1024;
 constexpr int64_t kPcgSmallN = 16384;
 
-// Sums over the work-group in every thread: wavefront sums, then thread i < NV
-// adds value i over the wavefronts in order.  smem holds NV (waves + 1) doubles.
+// Sums over the work-group in every thread: wavefront sums into smem, then
+// thread i < NV adds value i over the wavefronts in order into res[i].  res
+// lies outside every call's partial slots, so the next call (whose first
+// barrier every thread reaches only after reading res) needs no barrier here.
 template <int NV>
-__device__ __forceinline__ void block_allsum(double (&v)[NV], double* smem) {
+__device__ __forceinline__ void block_allsum(double (&v)[NV], double* smem, double* res) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
   for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
@@ -363,12 +365,11 @@ __device__ __forceinline__ void block_allsum(double (&v)[NV], double* smem) {
   if (threadIdx.x < NV) {
     double t = 0.0;
     for (int u = 0; u < nw; ++u) t += smem[u * NV + threadIdx.x];
-    smem[nw * NV + threadIdx.x] = t;
+    res[threadIdx.x] = t;
   }
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < NV; ++i) v[i] = smem[nw * NV + i];
-  __syncthreads();
+  for (int i = 0; i < NV; ++i) v[i] = res[i];
 }
 
 __global__ void __launch_bounds__(kPcgThreads) k_pcg_vector(
@@ -376,7 +377,8 @@ __global__ void __launch_bounds__(kPcgThreads) k_pcg_vector(
     const double* __restrict__ w, double* __restrict__ z, const double* __restrict__ A, int nc,
     const double* __restrict__ G, const double* __restrict__ m, double* __restrict__ sc,
     int rz_old, int rz_new, int rr_slot, const double* __restrict__ zc, int sums_slot) {
-  __shared__ double red[(kQeqMaxCons + 1) * (kPcgThreads / 64 + 1)];
+  __shared__ double red[(kQeqMaxCons + 1) * (kPcgThreads / 64)];  // wavefront partials
+  __shared__ double res[kQeqMaxCons + 1];                          // the sums
   const int64_t t0 = threadIdx.x, st = blockDim.x;
   // w.p and A w
   double d[kQeqMaxCons + 1];
@@ -387,7 +389,7 @@ __global__ void __launch_bounds__(kPcgThreads) k_pcg_vector(
     d[0] += wi * p[i];
     for (int c = 0; c < nc; ++c) d[1 + c] += A[(int64_t)c * n + i] * wi;
   }
-  block_allsum<kQeqMaxCons + 1>(d, red);
+  block_allsum<kQeqMaxCons + 1>(d, red, res);
   const double pw = d[0];
   const double alpha = pw != 0.0 ? sc[rz_old] / pw : 0.0;
   double y[kQeqMaxCons];
@@ -402,7 +404,7 @@ __global__ void __launch_bounds__(kPcgThreads) k_pcg_vector(
     r[i] = ri;
     e[0] += ri * ri;
   }
-  block_allsum<1>(e, red);
+  block_allsum<1>(e, red, res);
   if (threadIdx.x == 0) sc[rr_slot] = e[0];
   // z = M^-1 r, projected
 #pragma unroll
@@ -413,7 +415,7 @@ __global__ void __launch_bounds__(kPcgThreads) k_pcg_vector(
     for (int c = 0; c < nc; ++c) d[c] += A[(int64_t)c * n + i] * zi;
   }
   if (nc > 0) {
-    block_allsum<kQeqMaxCons + 1>(d, red);
+    block_allsum<kQeqMaxCons + 1>(d, red, res);
     proj_coeff(G, d, nullptr, nc, y);
   }
   // r.z of the projected z
@@ -428,7 +430,7 @@ __global__ void __launch_bounds__(kPcgThreads) k_pcg_vector(
     }
     e[0] += r[i] * zi;
   }
-  block_allsum<1>(e, red);
+  block_allsum<1>(e, red, res);
   const double rzn = e[0], rzo = sc[rz_old];
   const double beta = rzo != 0.0 ? rzn / rzo : 0.0;
   // new direction; with zc, also (sum p, sum p z, sum p z^2) for the next
@@ -445,7 +447,7 @@ __global__ void __launch_bounds__(kPcgThreads) k_pcg_vector(
     }
   }
   if (zc) {
-    block_allsum<3>(g3, red);
+    block_allsum<3>(g3, red, res);
     if (threadIdx.x == 0)
       for (int k = 0; k < 3; ++k) sc[sums_slot + k] = g3[k];
   }
