@@ -246,7 +246,10 @@ int admp_qeq_matvec(const admp_qeq_problem* problem, const double* v, double* ou
                     void* workspace, size_t ws_bytes, void* stream);
 /* q = argmin E on {A q = b} from the start q0 (projected onto the constraints);
  * synchronises every check_every iterations to test the residual.  *iters is
- * the iteration count at convergence or -1; *residual = |P grad E| / n. */
+ * the iteration count at convergence or -1; *residual = |P grad E| / n.
+ * Returns ADMP_ERR_INVALID when the rows of A are linearly dependent (a pivot
+ * of A A^T below 1e-12 of its largest entry), where the reference's
+ * torch.inverse of A A^T raises (src/torch_admp/utils.py:151-171). */
 int admp_qeq_solve(const admp_qeq_problem* problem, const double* q0, double tol,
                    int32_t max_iter, int32_t check_every, double* q, int32_t* iters,
                    double* residual, void* workspace, size_t ws_bytes, void* stream);

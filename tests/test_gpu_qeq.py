@@ -166,3 +166,57 @@ def test_damping_gradients_match_oracle(dev):
     assert abs(float(e.sum()) - float(e_ref)) <= 1e-12 * abs(float(e_ref))
     for a, r in zip(g, g_ref):
         assert float((a.cpu() - r).abs().max()) <= 1e-10 * float(r.abs().max())
+
+
+def test_dependent_constraint_rows_raise(dev):
+    """A duplicated total-charge row makes A A^T singular; the reference's
+    vector_projection_coeff_matrix (torch.inverse, src/torch_admp/utils.py:151-171)
+    raises there, and so does the solve (ADMP_ERR_INVALID -> AdmpError)."""
+    from torch_admp_amd._lib import AdmpError
+
+    pos, b, chi, hard, eta, pairs, ds, bs = qeq_system(n=40, seed=3)
+    n = pos.shape[0]
+    A = torch.ones(2, n, dtype=F64)
+    bv = torch.zeros(2, dtype=F64)
+    mod = QEqForceModule(4.5, kmesh=16)
+    with pytest.raises(AdmpError, match="invalid"):
+        with torch.no_grad():
+            mod.solve_implicit(*_to(dev, pos, b, chi, hard, eta, pairs, ds, bs, A, bv), tol=1e-9)
+    # independent rows still solve: a species-group row beside the total charge
+    A2 = torch.ones(2, n, dtype=F64)
+    A2[1, n // 2:] = 0.0
+    bv2 = torch.tensor([0.0, 0.5], dtype=F64)
+    with torch.no_grad():
+        q = mod.solve_implicit(*_to(dev, pos, b, chi, hard, eta, pairs, ds, bs, A2, bv2),
+                               tol=1e-11)
+    assert float((A2 @ q.cpu() - bv2).abs().max()) < 1e-10
+
+
+def test_large_solve_multi_kernel_path_is_optimal(dev):
+    """20000 atoms (above the 16384-atom limit of the one-work-group vector
+    step): the solve takes the multi-kernel iteration (dot products with
+    last-block sums, separate step, preconditioner, projection and direction
+    kernels, blocking residual reads).  Its charges satisfy the optimality
+    condition under the single Hessian product, |P (H q + chi)| / n within 10x
+    the tolerance, and sum q = 0."""
+    from torch_admp_amd.nblist import TorchNeighborList
+    from torch_admp_amd.workloads import qeq_species_box
+
+    n = 20000
+    length = (n / 0.1) ** (1.0 / 3.0)
+    pos, box, chi, hard, eta = [torch.tensor(a, device=dev)
+                                for a in qeq_species_box(n, length, seed=6)]
+    nbl = TorchNeighborList(6.0)
+    mod = QEqForceModule(6.0, 1e-5, kmesh=[60, 60, 60])
+    A = torch.ones(1, n, dtype=F64, device=dev)
+    b = torch.zeros(1, dtype=F64, device=dev)
+    tol = 1e-13
+    with torch.no_grad():
+        pairs = nbl(pos, box)
+        ds, bs = nbl.get_ds(), nbl.get_buffer_scales()
+        q = mod.solve_implicit(pos, box, chi, hard, eta, pairs, ds, bs, A, b, tol=tol)
+        g = mod.hessian_matvec(q, pos, box, chi, hard, eta, pairs, ds, bs) + chi
+    assert mod.converge_iter > 0
+    pg = g - g.mean()
+    assert float(pg.norm()) / n < 10 * tol
+    assert abs(float(q.sum())) < 1e-9

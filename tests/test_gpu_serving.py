@@ -79,7 +79,7 @@ def test_admission_respects_budget(dev):
     while not server.idle():
         server.pump()
         peak = max(peak, server.bytes_in_flight)
-    assert 0 < peak <= server.hbm_budget
+    assert 0 < peak <= server.peak_bytes_in_flight <= server.hbm_budget
     assert len(server.completed) == len(reqs)
     assert server.batches_launched >= 2
     big = serving.synthetic_stream(1, 0.0, seed=5, n_min=8000, n_max=8192)[0]
@@ -110,6 +110,8 @@ def test_qeq_requests_served_beside_pme(dev):
     assert stats["requests"] == len(pme) + len(qeq)
     assert stats["qeq"]["requests"] == len(qeq) and stats["pme"]["requests"] == len(pme)
     assert server.bytes_in_flight == 0
+    # PME batches and QEq solves reserve from one budget under one lock
+    assert 0 < server.peak_bytes_in_flight <= server.hbm_budget
     for r in qeq:
         t = lambda a: torch.tensor(a, device=dev)  # noqa: E731
         pos, box = t(r.positions), t(r.box)
@@ -130,3 +132,29 @@ def test_qeq_requests_served_beside_pme(dev):
         e1, f1, _ = _alone(r, dev, "float64")
         assert np.abs(r.energy - e1).max() <= 1e-12 * max(1.0, np.abs(e1).max()), r.rid
         assert np.abs(r.forces - f1).max() <= 1e-12 * max(1.0, np.abs(f1).max()), r.rid
+
+
+def test_close_cancels_queued_qeq_and_refuses_new(dev):
+    """close() with QEq requests still queued: they move to ``cancelled`` with
+    an error text, idle() becomes true, a later QEq submission raises, and PME
+    requests are still served."""
+    from torch_admp_amd.workloads import qeq_species_box
+
+    pos, box, chi, hard, eta = qeq_species_box(300, 15.0, seed=2)
+    # a budget that admits no QEq solve keeps the request queued
+    server = serving.PMEServer(dev, hbm_budget=64 << 20, precision="float64")
+    req = serving.QEqRequest(pos, box, chi, hard, eta, rcut=6.0)
+    with server._cv:
+        server.bytes_in_flight += server.hbm_budget  # hold the budget: nothing is admitted
+    server.submit(req)
+    assert not server.idle()
+    server.close()
+    with server._cv:
+        server.bytes_in_flight -= server.hbm_budget
+    assert server.cancelled == [req] and req.error and req.charges is None
+    assert server.idle()
+    with pytest.raises(RuntimeError, match="closed"):
+        server.submit(serving.QEqRequest(pos, box, chi, hard, eta, rcut=6.0))
+    pme = serving.synthetic_stream(3, 0.0, seed=2, n_min=100, n_max=300)
+    stats = server.serve(pme)
+    assert stats["requests"] == 3 and all(r.energy is not None for r in pme)

@@ -146,3 +146,29 @@ def test_conq_finite_field_self_consistent_drop():
     assert abs(float(v[:18].mean() - v[18:].mean()) - dv) < 1e-8 * scale
     with pytest.raises(AttributeError, match="ffield with conq"):
         electrode_ref.setup_from_lammps(len(q), groups)
+
+
+@pytest.mark.parametrize("deck,qtol,ftol", [("lmp_conp_slab_3d", 5e-4, 5e-5),
+                                            ("lmp_conq_interface_3d_bias", 2e-5, 5e-4)])
+def test_oracle_matches_lammps_charges_and_forces(deck, qtol, ftol):
+    """The oracle's solved electrode charges and forces against LAMMPS `fix
+    electrode` (the deck settings of the reference's tests,
+    tests/test_electrode_conp.py:108-323, tests/test_electrode_conq.py:29-223).
+    The per-atom charges of the dump are a pin the reference does not assert."""
+    from pathlib import Path
+
+    from test_gpu_electrode import DECKS
+
+    slab, _, groups, symm = DECKS[deck]
+    d = np.load(Path(__file__).resolve().parent / "golden" / f"{deck}.npz")
+    cell = np.diag(d["edges"]).astype(np.float64)
+    if slab:
+        cell[2, 2] *= 3.0
+    orc = qeq_ref.QEqOracle(5.0, ethresh=1e-6, kappa=0.5, slab_corr=slab)
+    ref = electrode_ref.setup_from_lammps(len(d["q"]), [dict(g, eta=1.6) for g in groups], symm,
+                                          conq_ffield=True)
+    _, f, q = electrode_ref.infer(orc, torch.tensor(d["positions"]), torch.tensor(cell),
+                                  torch.tensor(d["q"]), *ref)
+    mask = ref[0].numpy()
+    np.testing.assert_allclose(q.numpy()[mask], d["q"][mask], rtol=0, atol=qtol)
+    np.testing.assert_allclose(f.detach().numpy(), d["forces"], rtol=ftol, atol=ftol)

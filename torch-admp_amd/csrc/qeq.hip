@@ -219,12 +219,19 @@ __global__ void k_invert_small(const double* __restrict__ gram, int nc, double* 
   for (int r = 0; r < nc; ++r)
     for (int c = 0; c < 2 * nc; ++c)
       a[r][c] = c < nc ? gram[r * kQeqMaxCons + c] : (c - nc == r ? 1.0 : 0.0);
+  // Pivots below 1e-12 of the largest Gram entry mark dependent constraint rows
+  // (a duplicated row eliminates to an exact zero): the projection is then
+  // undefined and the host reports the problem as invalid.
+  double gmax = 0.0;
+  for (int r = 0; r < nc; ++r)
+    for (int c = 0; c < nc; ++c) gmax = fmax(gmax, fabs(a[r][c]));
+  const double pivot_floor = fmax(1e-12 * gmax, 1e-300);
   int ok = 1;
   for (int col = 0; col < nc; ++col) {
     int piv = col;
     for (int r = col + 1; r < nc; ++r)
       if (fabs(a[r][col]) > fabs(a[piv][col])) piv = r;
-    if (fabs(a[piv][col]) < 1e-300) {
+    if (!(fabs(a[piv][col]) > pivot_floor)) {
       ok = 0;
       break;
     }
@@ -687,8 +694,9 @@ QeqLayout qeq_layout(const QeqProblem& P) {
   return L;
 }
 
-// sc layout (doubles): 0 rz (even iterations), 1 rz (odd), 16.. reduced dots,
-// 32.. G (kQeqMaxCons^2), 112.. A q / A g, 128.. global sums, 160 |r|^2
+// sc layout (doubles): 0 rz (even iterations), 10 rz (odd), 20.. reduced dots,
+// 32.. G (kQeqMaxCons^2), 112.. A q / A g, 128.. global sums, 160 |r|^2,
+// 176.. Gram matrix A A^T
 constexpr int kScRz0 = 0, kScRz1 = 10, kScDots = 20, kScG = 32, kScAx = 112, kScSums = 128,
               kScRr = 160, kScGram = 176;
 
@@ -844,25 +852,35 @@ int qeq_matvec_impl(const QeqProblem& P, const double* v, double* out, void* wor
   return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
 }
 
-// A pinned host word and an event for reading the residual without draining
-// the stream: one pair per thread and device, created on first use and kept
-// for the life of the process.  Returns false when either cannot be made.
+// A pinned host word and an event per device for reading the residual without
+// draining the stream: one holder per thread, created on first use and freed
+// when the thread exits (a serving worker thread ends with its holder).
+struct ResidualReader {
+  static constexpr int kDev = 16;
+  double* host = nullptr;
+  hipEvent_t ev[kDev] = {};
+  ~ResidualReader() {
+    for (int d = 0; d < kDev; ++d)
+      if (ev[d]) (void)hipEventDestroy(ev[d]);
+    if (host) (void)hipHostFree(host);
+  }
+};
+
+// Returns false when the word or the event cannot be made.
 static bool residual_reader(double** host, hipEvent_t* ev) {
-  constexpr int kDev = 16;
-  static thread_local double* t_host = nullptr;
-  static thread_local hipEvent_t t_ev[kDev] = {};
+  static thread_local ResidualReader rr;
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDev) return false;
-  if (!t_host && hipHostMalloc((void**)&t_host, 64, hipHostMallocDefault) != hipSuccess) {
-    t_host = nullptr;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= ResidualReader::kDev) return false;
+  if (!rr.host && hipHostMalloc((void**)&rr.host, 64, hipHostMallocDefault) != hipSuccess) {
+    rr.host = nullptr;
     return false;
   }
-  if (!t_ev[dev] && hipEventCreateWithFlags(&t_ev[dev], hipEventDisableTiming) != hipSuccess) {
-    t_ev[dev] = nullptr;
+  if (!rr.ev[dev] && hipEventCreateWithFlags(&rr.ev[dev], hipEventDisableTiming) != hipSuccess) {
+    rr.ev[dev] = nullptr;
     return false;
   }
-  *host = t_host;
-  *ev = t_ev[dev];
+  *host = rr.host;
+  *ev = rr.ev[dev];
   return true;
 }
 
@@ -920,11 +938,15 @@ int qeq_solve_impl(const QeqProblem& P, const double* q0, double tol, int max_it
   int it = 0;
   double res = 0.0;
   bool converged = false;
-  // residual of the starting point
+  // residual of the starting point, and the status of the Gram inversion
   dots_into(P, L, ws, r, r, kScRr, s);
+  int gram_ok = 1;
   if (hipMemcpyAsync(&res, sc + kScRr, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      (nc > 0 && hipMemcpyAsync(&gram_ok, ws + L.status, sizeof(int), hipMemcpyDeviceToHost, s) !=
+                     hipSuccess) ||
       hipStreamSynchronize(s) != hipSuccess)
     return ADMP_ERR_HIP;
+  if (!gram_ok) return ADMP_ERR_INVALID;  // linearly dependent constraint rows
   res = std::sqrt(std::max(res, 0.0)) / (double)std::max<int64_t>(n, 1);
   converged = res < tol;
   double* rr_host = nullptr;

@@ -183,6 +183,7 @@ class PMEServer:
         self.queue: Deque[PMERequest] = deque()
         self.inflight: List[Optional[_Batch]] = [None] * len(self.streams)
         self.bytes_in_flight = 0
+        self.peak_bytes_in_flight = 0  # highest reservation seen, PME and QEq together
         self.completed: List[PMERequest] = []
         self.batches_launched = 0
         self._next_id = 0
@@ -193,6 +194,8 @@ class PMEServer:
         self._qeq_done: List[QEqRequest] = []
         self._qeq_busy = False
         self._stop = False
+        self._closed = False
+        self.cancelled: List[QEqRequest] = []  # QEq requests still queued at close()
         self._worker: Optional[threading.Thread] = None
         self._switch: Optional[float] = None
 
@@ -226,6 +229,8 @@ class PMEServer:
         self._next_id += 1
         if isinstance(req, QEqRequest):
             with self._cv:
+                if self._closed:
+                    raise RuntimeError("PMEServer is closed: QEq requests are no longer served")
                 if self._worker is None:
                     # The serving loop polls and the worker returns from a library
                     # call every few CG iterations: with Python's default 5 ms
@@ -254,7 +259,7 @@ class PMEServer:
                 if self._stop:
                     return
                 req = self.qeq_queue.popleft()
-                self.bytes_in_flight += req.cost
+                self._reserve(req.cost)
                 self._qeq_busy = True
             req.t_admit = self.now()
             try:
@@ -299,30 +304,58 @@ class PMEServer:
         self.close()
 
     def close(self) -> None:
-        """Stop the QEq worker (queued QEq requests are dropped) and restore the
-        thread switch interval it set."""
+        """Stop the QEq worker and restore the thread switch interval it set.
+        QEq requests still queued are not served: they get an error text and
+        move to ``cancelled``, so that idle() and drain() see an empty queue.
+        Later QEq submissions raise; PME requests are still served."""
         with self._cv:
             self._stop = True
+            self._closed = True
             self._cv.notify_all()
         if self._worker is not None:
             self._worker.join()
             self._worker = None
             sys.setswitchinterval(self._switch)
+        with self._cv:
+            while self.qeq_queue:
+                r = self.qeq_queue.popleft()
+                r.error = "PMEServer closed before the request was served"
+                self.cancelled.append(r)
+
+    def _reserve(self, cost: int) -> None:
+        """Add `cost` to the bytes in flight; the caller holds self._cv."""
+        self.bytes_in_flight += cost
+        self.peak_bytes_in_flight = max(self.peak_bytes_in_flight, self.bytes_in_flight)
 
     # ------------------------------------------------------------------ batches
     def _launch(self, slot: int) -> bool:
         take: List[PMERequest] = []
         cost = 0
+        # pick the batch and reserve its bytes in one hold of the lock, so that
+        # the QEq worker's admission sees them
         with self._cv:
             room = self.hbm_budget - self.bytes_in_flight
-        while self.queue and len(take) < self.max_systems:
-            r = self.queue[0]
-            if cost + r.cost > room:
-                break
-            take.append(self.queue.popleft())
-            cost += r.cost
+            while self.queue and len(take) < self.max_systems:
+                r = self.queue[0]
+                if cost + r.cost > room:
+                    break
+                take.append(self.queue.popleft())
+                cost += r.cost
+            self._reserve(cost)
         if not take:
             return False
+        try:
+            self._start_batch(slot, take, cost)
+        except BaseException:
+            with self._cv:
+                self.bytes_in_flight -= cost
+                self._cv.notify_all()
+            self.queue.extendleft(reversed(take))
+            raise
+        self.batches_launched += 1
+        return True
+
+    def _start_batch(self, slot: int, take: List[PMERequest], cost: int) -> None:
         plan = PMEBatchPlan([r.n_atoms for r in take], [r.box for r in take],
                             [r.rcut for r in take], [r.kappa for r in take],
                             [r.kmesh for r in take], self.precision, self.convention,
@@ -352,10 +385,6 @@ class PMEServer:
             done.record(stream)
         self.inflight[slot] = _Batch(take, plan, stream, done, h_e, h_f, h_g,
                                      [host_in, dev_in, pos, q, ws, energy, forces, dedbox], cost)
-        with self._cv:
-            self.bytes_in_flight += cost
-        self.batches_launched += 1
-        return True
 
     def _finish(self, slot: int) -> List[PMERequest]:
         b = self.inflight[slot]
