@@ -35,6 +35,7 @@ extern "C" {
 #define ADMP_ERR_INVALID (-1)
 #define ADMP_ERR_HIP (-2)
 #define ADMP_ERR_UNSUPPORTED (-3)
+#define ADMP_ERR_SCRATCH (-4)  /* more than ADMP_NOALLOC_SCRATCH_BYTES needed */
 
 #define ADMP_F64 0
 #define ADMP_F32 1
@@ -158,6 +159,22 @@ int admp_pme_fused(const double* pos, const double* q, int64_t n, const double* 
                    double cutoff, double kappa, const int32_t* kmesh, int32_t precision,
                    int32_t convention, double dielectric, double* energy, double* forces,
                    double* dEdbox, void* workspace, size_t ws_bytes, void* stream);
+
+/* admp_pme_fused without a workspace argument.  No counterpart in the
+ * reference: it serves callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as admp_pme_fused; the scratch is a static
+ * device array of ADMP_NOALLOC_SCRATCH_BYTES per GPU, held from library load to
+ * process end.  A problem whose admp_pme_fused_workspace_bytes exceeds it gets
+ * ADMP_ERR_SCRATCH (the 24,000-atom water box at 64^3 needs 8.4 MiB in float32
+ * and 13.8 MiB in float64).  It calls no allocating routine and launches every
+ * kernel on `stream` (no side stream).  Calls that overlap on one GPU share
+ * that GPU's scratch and race: serialise them. */
+#define ADMP_NOALLOC_SCRATCH_BYTES (64u << 20)
+int admp_pme_fused_noalloc(const double* pos, const double* q, int64_t n, const double* box,
+                           double cutoff, double kappa, const int32_t* kmesh, int32_t precision,
+                           int32_t convention, double dielectric, double* energy, double* forces,
+                           double* dEdbox, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Batched form of admp_pme_fused: n_sys independent periodic systems of any

@@ -725,7 +725,7 @@ int upload_table(const BatchPlan& P, char* ws, hipStream_t s, bool sync) {
 template <typename T>
 static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, double* energy,
                        double* forces, double* dEdbox, char* ws, hipStream_t s, hipEvent_t* marks,
-                       bool upload) {
+                       bool upload, bool fork) {
   using CT = cplx<T>;
   const int nsys = P.nsys;
   SysDesc* tab = (SysDesc*)(ws + P.o_tab);
@@ -782,9 +782,9 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   }
   mark(1);
   // ---- real space (also for empty batches: its partial rows feed the reductions),
-  // forked onto the side stream unless stages are being timed
+  // forked onto the side stream unless stages are being timed or the caller asks not to
   double* rpart = (double*)(ws + P.o_rpart);
-  SideStream* side = (P.ctot > 0 && !marks) ? side_stream(s) : nullptr;
+  SideStream* side = (P.ctot > 0 && !marks && fork) ? side_stream(s) : nullptr;
   hipStream_t rs = s;
   hipEvent_t join = nullptr;
   if (side) {
@@ -859,11 +859,12 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
 
 int run_batch(const BatchPlan& plan, const double* pos, const double* q, double* energy,
               double* forces, double* dEdbox, char* ws, hipStream_t s, hipEvent_t* marks,
-              bool upload) {
+              bool upload, bool fork) {
   if (plan.status != ADMP_OK) return plan.status;
   return plan.precision == ADMP_F32
-             ? run_batch_t<float>(plan, pos, q, energy, forces, dEdbox, ws, s, marks, upload)
-             : run_batch_t<double>(plan, pos, q, energy, forces, dEdbox, ws, s, marks, upload);
+             ? run_batch_t<float>(plan, pos, q, energy, forces, dEdbox, ws, s, marks, upload, fork)
+             : run_batch_t<double>(plan, pos, q, energy, forces, dEdbox, ws, s, marks, upload,
+                                   fork);
 }
 
 }  // namespace admp

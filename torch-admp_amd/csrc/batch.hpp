@@ -65,9 +65,10 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
 // reciprocal, self, non-neutral, total), forces (ntot x 3), dEdbox (nsys x 9, may be null).
 // `upload` = false assumes upload_table already put this plan's table in ws
 // (single systems never need it: the first kernel writes their table).
+// `fork` = false keeps every kernel on `s` (no side stream is created or used).
 int run_batch(const BatchPlan& plan, const double* pos, const double* q, double* energy,
               double* forces, double* dEdbox, char* ws, hipStream_t s, hipEvent_t* marks,
-              bool upload = true);
+              bool upload = true, bool fork = true);
 
 // Copy the system table and segment offsets into the workspace; `sync`
 // waits for the copy.

@@ -3,6 +3,7 @@
 // admp_nblist_count synchronises its stream.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "batch.hpp"
 #include "nblist_kernels.hpp"
@@ -143,6 +144,33 @@ bool qeq_problem(const admp_qeq_problem* pr, bool need_solve, QeqProblem* P) {
 
 }  // namespace
 
+// Scratch of admp_pme_fused_noalloc: one array per GPU, held from the library's
+// load until the process ends.  Host code never passes `scratch` itself (its
+// host-side name is not a device address): each device's address comes from
+// hipGetSymbolAddress once, with that device current, and only that pointer
+// reaches kernels.
+static __device__ __align__(256) unsigned char scratch[ADMP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kScratchDevices = 64;
+void* g_scratch_ptr[kScratchDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of the current device's scratch, looked up on the first call
+// on that device; nullptr when the runtime refuses.
+void* scratch_of_current_device() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
 extern "C" {
 
 int admp_version(void) { return 100; }
@@ -196,6 +224,9 @@ const char* admp_status_string(int status) {
     case ADMP_ERR_INVALID: return "invalid argument";
     case ADMP_ERR_HIP: return "HIP runtime error";
     case ADMP_ERR_UNSUPPORTED: return "unsupported configuration";
+    case ADMP_ERR_SCRATCH:
+      return "the problem needs more than the static scratch of admp_pme_fused_noalloc "
+             "(ADMP_NOALLOC_SCRATCH_BYTES); use admp_pme_fused with a workspace";
     default: return "unknown status";
   }
 }
@@ -423,6 +454,32 @@ int admp_pme_fused(const double* pos, const double* q, int64_t n, const double* 
     return ADMP_ERR_INVALID;
   return admp_pme_batch(1, &n, box, &cutoff, &kappa, kmesh, precision, convention, dielectric,
                         pos, q, energy, forces, dEdbox, workspace, ws_bytes, stream);
+}
+
+// The fused evaluation in the library's static scratch: no workspace argument,
+// no device allocation, every kernel on the caller's stream.  The launch
+// sequence clears what it accumulates into (counters, cursors, mesh, gradient
+// rows, tickets) before use, so the previous call's contents do not matter;
+// calls that overlap on one GPU share its scratch and race.
+int admp_pme_fused_noalloc(const double* pos, const double* q, int64_t n, const double* box,
+                           double cutoff, double kappa, const int32_t* kmesh, int32_t precision,
+                           int32_t convention, double dielectric, double* energy, double* forces,
+                           double* dEdbox, void* stream) {
+  if (n < 0 || !box_ok(box) || !(cutoff > 0.0) || !(kappa > 0.0) || !std::isfinite(kappa) ||
+      !kmesh_ok(kmesh) || !energy || (n > 0 && (!pos || !q || !forces)) ||
+      (precision != ADMP_F32 && precision != ADMP_F64) ||
+      (convention != ADMP_KCONV_REFERENCE && convention != ADMP_KCONV_COLUMNS))
+    return ADMP_ERR_INVALID;
+  if (n > (int64_t)1 << 30) return ADMP_ERR_UNSUPPORTED;
+  const BatchPlan P = plan_batch(1, &n, box, &cutoff, &kappa, kmesh, precision, convention,
+                                 dielectric);
+  if (P.status != ADMP_OK) return P.status;
+  if (P.ctot * kSub + 1 >= ((int64_t)1 << 31)) return ADMP_ERR_UNSUPPORTED;
+  if (P.bytes > (size_t)ADMP_NOALLOC_SCRATCH_BYTES) return ADMP_ERR_SCRATCH;
+  char* ws = (char*)scratch_of_current_device();
+  if (!ws) return ADMP_ERR_HIP;
+  return run_batch(P, pos, q, energy, forces, dEdbox, ws, (hipStream_t)stream, nullptr, true,
+                   false);
 }
 
 // ------------------------------------------------------------------- batch

@@ -20,6 +20,8 @@ _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libadmp_amd.so"
 
 ADMP_OK = 0
 ADMP_OVERFLOW = 1
+ADMP_ERR_SCRATCH = -4
+ADMP_NOALLOC_SCRATCH_BYTES = 64 << 20
 ADMP_F64 = 0
 ADMP_F32 = 1
 KCONV = {"reference": 0, "columns": 1}
@@ -59,6 +61,8 @@ _SIGNATURES = {
                                                       _P(_c_sz)]),
     "admp_pme_fused": (ctypes.c_int, [_c_vp, _c_vp, _c_i64, _c_vp, _c_dbl, _c_dbl, _c_vp, _c_i32,
                                       _c_i32, _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp, _c_sz, _c_vp]),
+    "admp_pme_fused_noalloc": (ctypes.c_int, [_c_vp, _c_vp, _c_i64, _c_vp, _c_dbl, _c_dbl, _c_vp,
+                                              _c_i32, _c_i32, _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp]),
     "admp_pme_batch_workspace_bytes": (ctypes.c_int, [_c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_i32,
                                                       _P(_c_sz)]),
     "admp_pme_batch": (ctypes.c_int, [_c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i32, _c_i32,
