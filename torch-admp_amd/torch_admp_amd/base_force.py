@@ -4,12 +4,11 @@
 ``forward(positions, box, pairs, ds, buffer_scales, params) -> energy (nf,)``
 normalises shapes to a leading frame axis, applies the unit coefficients and
 calls ``_forward_impl``.  The ValueError messages are the reference's own,
-word for word, because callers and tests match on them.
+word for word, because callers and tests match on them.  The class is
+TorchScript-compatible: subclasses script and save with torch.jit.
 """
 
-from __future__ import annotations
-
-from typing import Dict, Optional
+from typing import Dict, Optional, Tuple
 
 import torch
 
@@ -29,19 +28,27 @@ class BaseForceModule(torch.nn.Module):
         _pos, _box, _pairs, _ds, _bs = self.standardize_input_tensor(
             positions, box, pairs, ds, buffer_scales)
         nf = _pos.size(0)
-        _params = {k: v.reshape(nf, -1) for k, v in params.items()}
+        _params: Dict[str, torch.Tensor] = {}
+        for k, v in params.items():
+            _params[k] = v.reshape(nf, -1)
         lc = self.const_lib.length_coeff
         ec = self.const_lib.energy_coeff
-        energy = self._forward_impl(_pos * lc, _box * lc if _box is not None else None, _pairs,
-                                    _ds * lc, _bs, _params)
+        box_in: Optional[torch.Tensor] = None
+        if _box is not None:
+            box_in = _box * lc
+        energy = self._forward_impl(_pos * lc, box_in, _pairs, _ds * lc, _bs, _params)
         return energy / ec
 
-    def _forward_impl(self, positions, box, pairs, ds, buffer_scales, params) -> torch.Tensor:
-        raise NotImplementedError
+    def _forward_impl(self, positions: torch.Tensor, box: Optional[torch.Tensor],
+                      pairs: torch.Tensor, ds: torch.Tensor, buffer_scales: torch.Tensor,
+                      params: Dict[str, torch.Tensor]) -> torch.Tensor:
+        raise NotImplementedError("_forward_impl")
 
-    def standardize_input_tensor(self, positions: torch.Tensor, box: Optional[torch.Tensor],
-                                 pairs: torch.Tensor, ds: torch.Tensor,
-                                 buffer_scales: torch.Tensor):
+    @torch.jit.export
+    def standardize_input_tensor(
+            self, positions: torch.Tensor, box: Optional[torch.Tensor], pairs: torch.Tensor,
+            ds: torch.Tensor, buffer_scales: torch.Tensor
+    ) -> Tuple[torch.Tensor, Optional[torch.Tensor], torch.Tensor, torch.Tensor, torch.Tensor]:
         """Shape checks of src/torch_admp/base_force.py:175-339 (same messages)."""
         nframes = 1
         if positions.dim() == 3:
@@ -58,13 +65,13 @@ class BaseForceModule(torch.nn.Module):
 
         if box is not None:
             if box.dim() == 3:
-                if box.shape[1:] != (3, 3):
+                if box.size(1) != 3 or box.size(2) != 3:
                     raise ValueError(f"box must have shape (nframes, 3, 3), got {box.shape}")
                 if box.size(0) != nframes:
                     raise ValueError(
                         f"box is expected to have {nframes} frame(s), got {box.size(0)}")
             elif box.dim() == 2:
-                if box.shape != (3, 3):
+                if box.size(0) != 3 or box.size(1) != 3:
                     raise ValueError(f"box must have shape (3, 3), got {box.shape}")
                 if nframes != 1:
                     raise ValueError(
@@ -91,28 +98,24 @@ class BaseForceModule(torch.nn.Module):
             raise ValueError(f"pairs must be 2D or 3D tensor, got {pairs.dim()}D")
         n_pairs = pairs.size(-2)
 
-        for name in ("ds", "buffer_scales"):
-            t = ds if name == "ds" else buffer_scales
-            if t.dim() == 2:
-                if t.size(0) != nframes:
-                    raise ValueError(
-                        f"{name} is expected to have {nframes} frame(s), got {t.size(0)}")
-                if t.size(1) != n_pairs:
-                    raise ValueError(
-                        f"{name} is expected to have {n_pairs} pairs(s), got {t.size(1)}")
-            elif t.dim() == 1:
-                if t.size(0) != n_pairs:
-                    raise ValueError(
-                        f"{name} is expected to have {n_pairs} pairs(s), got {t.size(0)}")
-                if nframes != 1:
-                    raise ValueError(
-                        f"{name} must include a frame dimension when positions has {nframes} "
-                        "frames")
-                t = t.unsqueeze(0)
-            else:
-                raise ValueError(f"{name} must be 1D or 2D tensor, got {t.dim()}D")
-            if name == "ds":
-                ds = t
-            else:
-                buffer_scales = t
+        ds = self._per_pair(ds, "ds", nframes, n_pairs)
+        buffer_scales = self._per_pair(buffer_scales, "buffer_scales", nframes, n_pairs)
         return positions, box, pairs, ds, buffer_scales
+
+    def _per_pair(self, t: torch.Tensor, name: str, nframes: int, n_pairs: int) -> torch.Tensor:
+        """ds / buffer_scales to (nframes, n_pairs) with the reference's checks."""
+        if t.dim() == 2:
+            if t.size(0) != nframes:
+                raise ValueError(f"{name} is expected to have {nframes} frame(s), got {t.size(0)}")
+            if t.size(1) != n_pairs:
+                raise ValueError(f"{name} is expected to have {n_pairs} pairs(s), got {t.size(1)}")
+        elif t.dim() == 1:
+            if t.size(0) != n_pairs:
+                raise ValueError(f"{name} is expected to have {n_pairs} pairs(s), got {t.size(0)}")
+            if nframes != 1:
+                raise ValueError(
+                    f"{name} must include a frame dimension when positions has {nframes} frames")
+            t = t.unsqueeze(0)
+        else:
+            raise ValueError(f"{name} must be 1D or 2D tensor, got {t.dim()}D")
+        return t

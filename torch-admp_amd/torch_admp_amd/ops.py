@@ -1,13 +1,11 @@
-"""torch.autograd.Function wrappers around the HIP kernels.
+"""Python entry points over the HIP library.
 
-Each forward launches the library kernels on torch's current stream; each
-backward is a hand-written adjoint kernel (no autograd through torch ops):
-
-* PairDistance     -- pair distances and their transpose (scatter of dL/dds)
-* PairEnergy       -- erfc / bare / Gaussian-damped pair sums, dE/dds and dE/dq
-* PMEReciprocal    -- spread -> r2c FFT -> influence function -> c2r FFT; the
-                      backward is the gather (the transpose of the spread) plus
-                      the k-vector and volume terms of dE/dbox.
+* pair_distance, pair_energy, pme_reciprocal -- the differentiable pieces of
+  the drop-in modules, as calls of the torch.ops.admp operators (_ops.py),
+  whose backwards are the library's adjoint kernels;
+* pme_fused -- energy, forces and dE/dbox of one frame in one native call;
+* PMEBatchPlan / pme_batch -- many independent systems in one launch sequence;
+* PMEGraph -- a batch captured in a HIP graph.
 """
 
 from __future__ import annotations
@@ -18,6 +16,7 @@ from typing import Optional, Sequence
 import torch
 
 from . import _lib
+from ._ops import PAIR_BARE, PAIR_DAMPED, PAIR_EWALD  # noqa: F401
 from .utils import DIELECTRIC
 
 
@@ -25,166 +24,32 @@ def _f64(t: torch.Tensor) -> torch.Tensor:
     return t.detach().to(torch.float64).contiguous()
 
 
-class PairDistance(torch.autograd.Function):
-    """ds = |pbc_shift(r_i - r_j)| (src/torch_admp/nblist.py:517-551); open
-    boundaries when box is None."""
-
-    @staticmethod
-    def forward(ctx, positions: torch.Tensor, box: Optional[torch.Tensor], pairs: torch.Tensor):
-        _lib.require_gpu(positions, box, pairs)
-        lib = _lib.load()
-        pos = _f64(positions)
-        prs = pairs.detach().to(torch.int64).contiguous()
-        npairs = prs.shape[0]
-        ds = torch.empty(npairs, dtype=torch.float64, device=pos.device)
-        dr = torch.empty(npairs, 3, dtype=torch.float64, device=pos.device)
-        boxh = _lib.box_array(box)
-        _lib.check(lib.admp_pair_ds(_lib.ptr(pos), _lib.ptr(prs), npairs, boxh, _lib.ptr(ds),
-                                    _lib.ptr(dr), _lib.stream_of(pos)), "admp_pair_ds")
-        ctx.save_for_backward(pos, prs, dr, ds)
-        ctx.boxh = boxh
-        ctx.dtype = positions.dtype
-        ctx.box_dtype = None if box is None else box.dtype
-        return ds.to(positions.dtype)
-
-    @staticmethod
-    def backward(ctx, grad_ds):
-        pos, prs, dr, ds = ctx.saved_tensors
-        lib = _lib.load()
-        g = _f64(grad_ds)
-        gpos = torch.zeros_like(pos)
-        need_box = ctx.boxh is not None and ctx.needs_input_grad[1]
-        gbox = torch.zeros(3, 3, dtype=torch.float64, device=pos.device) if need_box else None
-        _lib.check(lib.admp_pair_ds_backward(_lib.ptr(g), _lib.ptr(prs), prs.shape[0],
-                                             _lib.ptr(pos), ctx.boxh, _lib.ptr(dr), _lib.ptr(ds),
-                                             _lib.ptr(gpos), _lib.ptr(gbox),
-                                             _lib.stream_of(pos)), "admp_pair_ds_backward")
-        return (gpos.to(ctx.dtype), None if gbox is None else gbox.to(ctx.box_dtype), None)
+def pair_distance(positions: torch.Tensor, box: Optional[torch.Tensor],
+                  pairs: torch.Tensor) -> torch.Tensor:
+    """ds = |pbc_shift(r_j - r_i)| (src/torch_admp/nblist.py:517-551); open
+    boundaries when box is None.  Differentiable in positions and box."""
+    return torch.ops.admp.pair_distance(positions, box, pairs)[0]
 
 
-PAIR_EWALD, PAIR_BARE, PAIR_DAMPED = 0, 1, 2
+def pair_energy(q: torch.Tensor, ds: torch.Tensor, pairs: torch.Tensor, bs: torch.Tensor,
+                eta: Optional[torch.Tensor], mode: int, kappa: float, eps: float) -> torch.Tensor:
+    """Per-frame pair energy over (nf, P) pair rows: mode 0
+    (src/torch_admp/pme.py:312-349), 1 (pme.py:529-561), 2
+    (src/torch_admp/qeq.py:106-121).  Differentiable in q, ds and eta, twice in q."""
+    return torch.ops.admp.pair_energy(q, ds, pairs, bs, eta, int(mode), float(kappa),
+                                      float(eps))
 
 
-class PairEnergy(torch.autograd.Function):
-    """Per-frame pair energy over (nf, P) pair rows with float64 accumulation.
-
-    mode 0 (src/torch_admp/pme.py:312-349), 1 (pme.py:529-561),
-    2 (src/torch_admp/qeq.py:106-121).
-    """
-
-    @staticmethod
-    def forward(ctx, q, ds, pairs, bs, eta, mode: int, kappa: float, eps: float):
-        _lib.require_gpu(q, ds, pairs, bs, eta)
-        lib = _lib.load()
-        nf, n = q.shape
-        npairs = pairs.shape[1]
-        q64 = _f64(q)
-        ds64 = _f64(ds)
-        bs64 = _f64(bs)
-        prs = pairs.detach().to(torch.int64).contiguous()
-        eta64 = None if eta is None else _f64(eta)
-        dev = q64.device
-        energy = torch.empty(nf, dtype=torch.float64, device=dev)
-        need = ctx.needs_input_grad
-        gds = torch.empty(nf, npairs, dtype=torch.float64, device=dev) if need[1] else None
-        gq = torch.empty(nf, n, dtype=torch.float64, device=dev) if need[0] else None
-        geta = (torch.empty(nf, n, dtype=torch.float64, device=dev)
-                if need[4] and mode == PAIR_DAMPED and eta is not None else None)
-        nbytes = ctypes.c_size_t()
-        _lib.check(lib.admp_pair_energy_workspace_bytes(nf, ctypes.byref(nbytes)), "workspace")
-        ws = _lib.workspace(nbytes.value, dev)
-        _lib.check(lib.admp_pair_energy(mode, nf, n, npairs, _lib.ptr(q64), _lib.ptr(eta64),
-                                        _lib.ptr(prs), _lib.ptr(ds64), _lib.ptr(bs64), kappa, eps,
-                                        _lib.ptr(energy), _lib.ptr(gds), _lib.ptr(gq),
-                                        _lib.ptr(geta), _lib.ptr(ws), nbytes.value,
-                                        _lib.stream_of(q64)),
-                   "admp_pair_energy")
-        ctx.save_for_backward(gds, gq, geta)
-        ctx.dtypes = (q.dtype, ds.dtype, None if eta is None else eta.dtype)
-        return energy.to(q.dtype)
-
-    @staticmethod
-    def backward(ctx, grad_e):
-        gds, gq, geta = ctx.saved_tensors
-        ge = grad_e.detach().to(torch.float64).reshape(-1, 1)
-        out_q = None if gq is None else (gq * ge).to(ctx.dtypes[0])
-        out_ds = None if gds is None else (gds * ge).to(ctx.dtypes[1])
-        out_eta = None if geta is None else (geta * ge).to(ctx.dtypes[2])
-        return out_q, out_ds, None, None, out_eta, None, None, None
-
-
-def _inverse_transpose(boxh) -> torch.Tensor:
-    b = torch.tensor(list(boxh), dtype=torch.float64).reshape(3, 3)
-    return torch.linalg.inv(b).transpose(0, 1)
-
-
-class PMEReciprocal(torch.autograd.Function):
-    """One frame of the SPME reciprocal energy (src/torch_admp/pme.py:378-446).
-
-    Differentiable in positions, box and charges; the backward launches the
-    gather kernel on the potential mesh kept from the forward.
-    """
-
-    @staticmethod
-    def forward(ctx, positions, box, q, kmesh: Sequence[int], kappa: float, eps: float,
-                precision: int, convention: int):
-        _lib.require_gpu(positions, box, q)
-        lib = _lib.load()
-        pos = _f64(positions)
-        q64 = _f64(q).reshape(-1)
-        n = pos.shape[0]
-        dev = pos.device
-        boxh = _lib.box_array(box)
-        km = _lib.kmesh_array(kmesh)
-        mesh_dtype = torch.float32 if precision == _lib.ADMP_F32 else torch.float64
-        mesh = torch.empty(int(kmesh[0]) * int(kmesh[1]) * int(kmesh[2]), dtype=mesh_dtype,
-                           device=dev)
-        nbytes = ctypes.c_size_t()
-        _lib.check(lib.admp_pme_workspace_bytes(km, precision, n, ctypes.byref(nbytes)),
-                   "admp_pme_workspace_bytes")
-        ws = _lib.workspace(nbytes.value, dev)
-        stream = _lib.stream_of(pos)
-        _lib.check(lib.admp_pme_spread(_lib.ptr(pos), _lib.ptr(q64), n, boxh, km, precision,
-                                       _lib.ptr(mesh), _lib.ptr(ws), nbytes.value, stream),
-                   "admp_pme_spread")
-        out = torch.empty(10, dtype=torch.float64, device=dev)
-        need_grad = any(ctx.needs_input_grad[:3])
-        _lib.check(lib.admp_pme_convolve(_lib.ptr(mesh), boxh, km, precision, convention, kappa,
-                                         eps, 1.0, _lib.ptr(mesh) if need_grad else None,
-                                         _lib.ptr(out), _lib.ptr(out[1:]), _lib.ptr(ws),
-                                         nbytes.value, stream), "admp_pme_convolve")
-        energy = out[0].clone()
-        if need_grad:
-            ctx.save_for_backward(pos, q64, mesh, out)
-        ctx.meta = (boxh, km, precision, positions.dtype, None if box is None else box.dtype)
-        return energy.to(positions.dtype)
-
-    @staticmethod
-    def backward(ctx, grad_e):
-        pos, q64, phi, out = ctx.saved_tensors
-        boxh, km, precision, pdtype, bdtype = ctx.meta
-        lib = _lib.load()
-        n = pos.shape[0]
-        dev = pos.device
-        gpos = torch.empty(n, 3, dtype=torch.float64, device=dev)
-        gq = torch.empty(n, dtype=torch.float64, device=dev)
-        dEdA = torch.empty(9, dtype=torch.float64, device=dev)
-        nbytes = ctypes.c_size_t()
-        _lib.check(lib.admp_pme_workspace_bytes(km, precision, n, ctypes.byref(nbytes)),
-                   "admp_pme_workspace_bytes")
-        ws = _lib.workspace(nbytes.value, dev)
-        _lib.check(lib.admp_pme_gather(_lib.ptr(pos), _lib.ptr(q64), n, boxh, km, precision,
-                                       _lib.ptr(phi), _lib.ptr(gpos), _lib.ptr(gq),
-                                       _lib.ptr(dEdA), 0, _lib.ptr(ws), nbytes.value,
-                                       _lib.stream_of(pos)), "admp_pme_gather")
-        ge = grad_e.detach().to(torch.float64)
-        grad_box = None
-        if ctx.needs_input_grad[1]:
-            at = _inverse_transpose(boxh).to(dev)                     # A^T
-            dA = (out[1:] + dEdA).reshape(3, 3)
-            grad_box = (-(at @ dA @ at) - out[0] * at) * ge
-            grad_box = grad_box.to(bdtype)
-        return ((gpos * ge).to(pdtype), grad_box, (gq * ge), None, None, None, None, None)
+def pme_reciprocal(positions: torch.Tensor, box: torch.Tensor, q: torch.Tensor,
+                   kmesh: Sequence[int], kappa: float, eps: float, precision: int,
+                   convention: int) -> torch.Tensor:
+    """One frame of the SPME reciprocal energy (src/torch_admp/pme.py:351-446).
+    Differentiable in positions, box and charges, twice in the charges."""
+    need = torch.is_grad_enabled() and (positions.requires_grad or box.requires_grad
+                                        or q.requires_grad)
+    return torch.ops.admp.pme_reciprocal(positions, box, q, [int(k) for k in kmesh],
+                                         float(kappa), float(eps), int(precision),
+                                         int(convention), need)[0]
 
 
 # Host state reused across fused calls: per problem shape the workspace size and

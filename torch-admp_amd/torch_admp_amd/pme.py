@@ -10,8 +10,6 @@ Extra keyword arguments beyond the reference:
               triclinic boxes.
 """
 
-from __future__ import annotations
-
 import math
 from typing import Dict, List, Optional, Tuple, Union
 
@@ -20,12 +18,22 @@ import torch
 
 from . import _lib
 from .base_force import BaseForceModule
-from .ops import PAIR_BARE, PAIR_EWALD, PMEReciprocal, PairEnergy, pme_fused
+from . import _ops  # noqa: F401  (registers torch.ops.admp)
+from .ops import pme_fused
 from .utils import safe_inverse  # noqa: F401  (re-export, reference API)
 
 
 class CoulombForceModule(BaseForceModule):
-    """Coulomb energy with PME (src/torch_admp/pme.py:24-561)."""
+    """Coulomb energy with PME (src/torch_admp/pme.py:24-561).
+
+    TorchScript-compatible: forward, func-style helpers and mesh_for_box
+    script (torch.jit.script) and save; the pair and reciprocal terms are the
+    torch.ops.admp operators.  energy_forces (the fused single-call path) is
+    Python only."""
+
+    kmesh: Optional[torch.Tensor]
+    spacing: Optional[torch.Tensor]
+    sel: Optional[List[int]]
 
     def __init__(self, rcut: float, ethresh: float = 1e-5, kspace: bool = True,
                  rspace: bool = True, slab_corr: bool = False, slab_axis: int = 2,
@@ -49,12 +57,12 @@ class CoulombForceModule(BaseForceModule):
         if kappa is not None:
             if kappa <= 0.0:
                 raise ValueError(f"kappa must be positive, got {kappa}")
-            self.kappa = kappa
+            self.kappa = float(kappa)
         elif kspace:
             self.kappa = math.sqrt(-math.log(2 * ethresh)) / rcut / self.const_lib.length_coeff
         else:
             self.kappa = 0.0
-        self.ethresh = ethresh
+        self.ethresh = float(ethresh)
         if kmesh is not None:
             if isinstance(kmesh, int):
                 kmesh = [kmesh, kmesh, kmesh]
@@ -65,13 +73,12 @@ class CoulombForceModule(BaseForceModule):
         else:
             self.kmesh = None
         self._kmesh = torch.zeros(3, dtype=torch.long)
-        self._kmesh_list: Optional[List[int]] = None
         if spacing is not None:
             if isinstance(spacing, float):
                 spacing = [spacing, spacing, spacing]
-            for i, s in enumerate(spacing):
-                if s <= 0:
-                    raise ValueError(f"spacing values must be positive, got spacing[{i}] = {s}")
+            for i, sp in enumerate(spacing):
+                if sp <= 0:
+                    raise ValueError(f"spacing values must be positive, got spacing[{i}] = {sp}")
             self.spacing = torch.tensor(list(spacing), dtype=torch.float64)
         else:
             self.spacing = None
@@ -80,6 +87,8 @@ class CoulombForceModule(BaseForceModule):
         self.slab_axis = slab_axis
         self.precision = precision
         self.convention = convention
+        self._precision_code: int = _lib.PRECISION[precision]
+        self._convention_code: int = _lib.KCONV[convention]
         zero = torch.zeros(1, dtype=torch.float64)
         self.real_energy = zero
         self.reciprocal_energy = zero
@@ -87,7 +96,7 @@ class CoulombForceModule(BaseForceModule):
         self.non_neutral_energy = zero
         self.slab_corr_energy = zero
         self.pme_order: int = 6
-        self.rcut = rcut
+        self.rcut = float(rcut)
         self.sel = sel
 
     def get_rcut(self) -> float:
@@ -96,18 +105,26 @@ class CoulombForceModule(BaseForceModule):
     def get_sel(self) -> Optional[List[int]]:
         return self.sel
 
-    # -- mesh size rule (src/torch_admp/pme.py:380-392) -------------------------
+    @torch.jit.export
     def mesh_for_box(self, box_frame: torch.Tensor) -> List[int]:
-        if self.kmesh is not None:
-            return self.kmesh.tolist()
-        diag = torch.diagonal(box_frame.detach().reshape(3, 3)).to("cpu", torch.float64)
-        if self.spacing is not None:
-            k = torch.ceil(diag / self.spacing)
+        """Mesh size: explicit kmesh, else ceil(diag / spacing), else the
+        accuracy rule ceil(2 kappa L / (3 ethresh^(1/5))) on the box diagonal
+        (src/torch_admp/pme.py:380-392)."""
+        kmesh = self.kmesh
+        if kmesh is not None:
+            return torch.jit.annotate(List[int], kmesh.tolist())
+        diag = torch.diagonal(box_frame.detach().reshape(3, 3)).to(torch.device("cpu"),
+                                                                  torch.float64)
+        spacing = self.spacing
+        if spacing is not None:
+            k = torch.ceil(diag / spacing)
         else:
             k = torch.ceil(2 * self.kappa * diag / (3.0 * self.ethresh ** (1.0 / 5.0)))
-        return [int(v) for v in k.tolist()]
+        return torch.jit.annotate(List[int], k.to(torch.long).tolist())
 
-    def _forward_impl(self, positions, box, pairs, ds, buffer_scales, params):
+    def _forward_impl(self, positions: torch.Tensor, box: Optional[torch.Tensor],
+                      pairs: torch.Tensor, ds: torch.Tensor, buffer_scales: torch.Tensor,
+                      params: Dict[str, torch.Tensor]) -> torch.Tensor:
         nf = positions.size(0)
         na = positions.size(1)
         charges = params["charge"].reshape(nf, na)
@@ -115,12 +132,12 @@ class CoulombForceModule(BaseForceModule):
             return self._forward_pbc(charges, positions, box, pairs, ds, buffer_scales)
         return self._forward_obc(charges, pairs, ds, buffer_scales)
 
-    def _forward_pbc(self, charges, positions, box, pairs, ds, buffer_scales):
+    def _forward_pbc(self, charges: torch.Tensor, positions: torch.Tensor, box: torch.Tensor,
+                     pairs: torch.Tensor, ds: torch.Tensor,
+                     buffer_scales: torch.Tensor) -> torch.Tensor:
         """Sum of the five terms (src/torch_admp/pme.py:261-310)."""
-        eps = self.const_lib.dielectric
         if self.rspace_flag:
-            self.real_energy = PairEnergy.apply(charges, ds, pairs, buffer_scales, None,
-                                                PAIR_EWALD, float(self.kappa), eps)
+            self.real_energy = self._forward_pbc_real(charges, pairs, ds, buffer_scales)
         if self.kspace_flag:
             self.reciprocal_energy = self._forward_pbc_reciprocal(charges, positions, box)
             self.self_energy = self._forward_pbc_self(charges)
@@ -132,38 +149,45 @@ class CoulombForceModule(BaseForceModule):
                 + self.self_energy.to(dev) + self.non_neutral_energy.to(dev)
                 + self.slab_corr_energy.to(dev))
 
-    def _forward_pbc_real(self, charges, pairs, ds, buffer_scales):
-        return PairEnergy.apply(charges, ds, pairs, buffer_scales, None, PAIR_EWALD,
-                                float(self.kappa), self.const_lib.dielectric)
+    def _forward_pbc_real(self, charges: torch.Tensor, pairs: torch.Tensor, ds: torch.Tensor,
+                          buffer_scales: torch.Tensor) -> torch.Tensor:
+        """eps erfc(kappa d) qi qj / d over the pair rows (src/torch_admp/pme.py:312-349)."""
+        # pair mode 0: erfc-screened pairs
+        return torch.ops.admp.pair_energy(charges, ds, pairs, buffer_scales, None, 0,
+                                          self.kappa, self.const_lib.dielectric)
 
-    def _forward_pbc_reciprocal(self, charges, positions, box):
-        """One PMEReciprocal launch sequence per frame (src/torch_admp/pme.py:351-446)."""
-        out = []
+    def _forward_pbc_reciprocal(self, charges: torch.Tensor, positions: torch.Tensor,
+                                box: torch.Tensor) -> torch.Tensor:
+        """One reciprocal launch sequence per frame (src/torch_admp/pme.py:351-446)."""
+        out: List[torch.Tensor] = []
         for ii in range(positions.size(0)):
             km = self.mesh_for_box(box[ii])
-            out.append(PMEReciprocal.apply(positions[ii], box[ii], charges[ii], km,
-                                           float(self.kappa), self.const_lib.dielectric,
-                                           _lib.PRECISION[self.precision],
-                                           _lib.KCONV[self.convention]))
+            need = torch.is_grad_enabled() and (positions.requires_grad or box.requires_grad
+                                                or charges.requires_grad)
+            e, _, _ = torch.ops.admp.pme_reciprocal(positions[ii], box[ii], charges[ii], km,
+                                                    self.kappa, self.const_lib.dielectric,
+                                                    self._precision_code, self._convention_code,
+                                                    need)
+            out.append(e)
             self._kmesh = torch.tensor(km, dtype=torch.long)
-            self._kmesh_list = None
         return torch.stack(out)
 
-    def _forward_pbc_self(self, charges):
+    def _forward_pbc_self(self, charges: torch.Tensor) -> torch.Tensor:
         """-eps kappa / sqrt(pi) sum q^2 (src/torch_admp/pme.py:448-461)."""
         if self.kspace_flag:
             coeff = self.kappa / self.const_lib.sqrt_pi
             return -torch.sum(coeff * charges ** 2, dim=-1) * self.const_lib.dielectric
-        return torch.zeros(charges.size(0), device=charges.device)
+        return torch.zeros(charges.size(0), device=charges.device, dtype=charges.dtype)
 
-    def _forward_pbc_non_neutral(self, charges, box):
+    def _forward_pbc_non_neutral(self, charges: torch.Tensor, box: torch.Tensor) -> torch.Tensor:
         """-eps pi Q^2 / (2 V kappa^2) (src/torch_admp/pme.py:463-493)."""
         volume = torch.det(box)
         q_tot = torch.sum(charges, dim=-1)
         coeff = -self.const_lib.pi / (2 * volume * self.kappa ** 2) * self.const_lib.dielectric
         return coeff * q_tot ** 2
 
-    def _forward_slab_corr(self, charges, positions, box, ds):
+    def _forward_slab_corr(self, charges: torch.Tensor, positions: torch.Tensor,
+                           box: torch.Tensor, ds: torch.Tensor) -> torch.Tensor:
         """Yeh-Berkowitz slab term (src/torch_admp/pme.py:495-527)."""
         volume = torch.det(box)
         pre = 2 * self.const_lib.pi / volume * self.const_lib.dielectric
@@ -174,12 +198,15 @@ class CoulombForceModule(BaseForceModule):
         return pre * (mz ** 2 - q_tot * torch.sum(charges * z ** 2, dim=-1)
                       - q_tot ** 2 * lz ** 2 / 12)
 
-    def _forward_obc(self, charges, pairs, ds, buffer_scales):
+    def _forward_obc(self, charges: torch.Tensor, pairs: torch.Tensor, ds: torch.Tensor,
+                     buffer_scales: torch.Tensor) -> torch.Tensor:
         """Bare Coulomb over the pair list (src/torch_admp/pme.py:529-561)."""
-        return PairEnergy.apply(charges, ds, pairs, buffer_scales, None, PAIR_BARE, 0.0,
-                                self.const_lib.dielectric)
+        # pair mode 1: bare pairs
+        return torch.ops.admp.pair_energy(charges, ds, pairs, buffer_scales, None, 1, 0.0,
+                                          self.const_lib.dielectric)
 
     # -- fused single-call path (no pair list crosses the boundary) -------------
+    @torch.jit.unused
     def energy_forces(self, positions: torch.Tensor, box: torch.Tensor, charges: torch.Tensor,
                       workspace: Optional[torch.Tensor] = None
                       ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
@@ -196,9 +223,10 @@ class CoulombForceModule(BaseForceModule):
         if lc != 1.0:
             positions, box = positions * lc, box * lc
         km = self.mesh_for_box(box)
-        if km != self._kmesh_list:
+        d = self.__dict__
+        if km != d.get("_kmesh_list"):
             self._kmesh = torch.tensor(km, dtype=torch.long)
-            self.__dict__["_kmesh_list"] = km
+            d["_kmesh_list"] = km
         e, f, gbox = pme_fused(positions, box, charges, self.rcut * lc, float(self.kappa), km,
                                self.precision, self.convention, self.const_lib.dielectric,
                                workspace)
@@ -206,7 +234,6 @@ class CoulombForceModule(BaseForceModule):
             e, f, gbox = e / ec, f * (lc / ec), gbox * (lc / ec)
         # plain attributes: written into __dict__ directly, skipping nn.Module's
         # parameter/buffer checks on this per-step path
-        d = self.__dict__
         (d["real_energy"], d["reciprocal_energy"], d["self_energy"], d["non_neutral_energy"],
          total) = e.split(1)
         return total.view(()), f, gbox

@@ -19,8 +19,6 @@ LBFGS and Polak-Ribiere solvers reach, with the same stopping metric
 the matrix-free product and solve the KKT system densely.
 """
 
-from __future__ import annotations
-
 import ctypes
 import math
 from typing import Dict, List, Optional, Union
@@ -29,7 +27,7 @@ import torch
 
 from . import _lib
 from .base_force import BaseForceModule
-from .ops import PAIR_DAMPED, PairEnergy
+from . import _ops  # noqa: F401  (registers torch.ops.admp)
 from .pme import CoulombForceModule
 from .utils import (calc_grads, calc_pgrads, safe_inverse, vector_projection,
                     vector_projection_coeff_matrix)
@@ -56,7 +54,9 @@ class _QeqProblem(ctypes.Structure):
 class SiteForceModule(BaseForceModule):
     """sum(chi q + hardness q^2) (src/torch_admp/qeq.py:132-209)."""
 
-    def _forward_impl(self, positions, box, pairs, ds, buffer_scales, params):
+    def _forward_impl(self, positions: torch.Tensor, box: Optional[torch.Tensor],
+                      pairs: torch.Tensor, ds: torch.Tensor, buffer_scales: torch.Tensor,
+                      params: Dict[str, torch.Tensor]) -> torch.Tensor:
         nf, na = positions.size(0), positions.size(1)
         ec = self.const_lib.energy_coeff
         chi = params["chi"].reshape(nf, na) * ec
@@ -69,18 +69,26 @@ class GaussianDampingForceModule(BaseForceModule):
     """Gaussian short-range damping (src/torch_admp/qeq.py:33-129): the pair
     sum runs in the HIP pair kernel, the per-atom term is elementwise."""
 
-    def _forward_impl(self, positions, box, pairs, ds, buffer_scales, params):
+    def _forward_impl(self, positions: torch.Tensor, box: Optional[torch.Tensor],
+                      pairs: torch.Tensor, ds: torch.Tensor, buffer_scales: torch.Tensor,
+                      params: Dict[str, torch.Tensor]) -> torch.Tensor:
         nf, na = positions.size(0), positions.size(1)
         q = params["charge"].reshape(nf, na)
         eta = params["eta"].reshape(nf, na) * self.const_lib.length_coeff
         eps = self.const_lib.dielectric
-        e_pair = PairEnergy.apply(q, ds, pairs, buffer_scales, eta, PAIR_DAMPED, 0.0, eps)
+        # pair mode 2: Gaussian-damped pairs
+        e_pair = torch.ops.admp.pair_energy(q, ds, pairs, buffer_scales, eta, 2, 0.0, eps)
         pre_self = safe_inverse(eta, threshold=1e-4) / (2 * self.const_lib.sqrt_pi)
         return e_pair + torch.sum(pre_self * q * q, dim=-1) * eps
 
 
 class QEqForceModule(BaseForceModule):
-    """Charge equilibration (src/torch_admp/qeq.py:212-765)."""
+    """Charge equilibration (src/torch_admp/qeq.py:212-765).
+
+    forward and func_energy script (torch.jit.script) and save; the solvers
+    and the Hessian are Python methods around the library's QEq entry points."""
+
+    sel: Optional[List[int]]
 
     def __init__(self, rcut: float, ethresh: float = 1e-5, kspace: bool = True,
                  rspace: bool = True, slab_corr: bool = False, slab_axis: int = 2,
@@ -120,7 +128,9 @@ class QEqForceModule(BaseForceModule):
     def get_sel(self) -> Optional[List[int]]:
         return self.sel
 
-    def _forward_impl(self, positions, box, pairs, ds, buffer_scales, params):
+    def _forward_impl(self, positions: torch.Tensor, box: Optional[torch.Tensor],
+                      pairs: torch.Tensor, ds: torch.Tensor, buffer_scales: torch.Tensor,
+                      params: Dict[str, torch.Tensor]) -> torch.Tensor:
         """Sum of the submodels (src/torch_admp/qeq.py:351-406)."""
         energy = torch.zeros(positions.size(0), device=positions.device, dtype=positions.dtype)
         for model in self.submodels.values():
@@ -128,8 +138,11 @@ class QEqForceModule(BaseForceModule):
                                                   params)
         return energy
 
-    def func_energy(self, charges, positions, box, chi, hardness, eta, pairs, ds,
-                    buffer_scales) -> torch.Tensor:
+    @torch.jit.export
+    def func_energy(self, charges: torch.Tensor, positions: torch.Tensor,
+                    box: Optional[torch.Tensor], chi: torch.Tensor, hardness: torch.Tensor,
+                    eta: torch.Tensor, pairs: torch.Tensor, ds: torch.Tensor,
+                    buffer_scales: torch.Tensor) -> torch.Tensor:
         """Total energy at the given charges (src/torch_admp/qeq.py:408-458)."""
         params = {"charge": charges, "chi": chi, "hardness": hardness, "eta": eta}
         energy = torch.zeros(1, device=positions.device, dtype=positions.dtype)
@@ -215,7 +228,7 @@ class QEqForceModule(BaseForceModule):
             pr.n_cons = A.shape[0]
         if box_int is not None and (flags & QEQ_KSPACE):
             coul._kmesh = torch.tensor(km, dtype=torch.long)
-            coul._kmesh_list = None
+            coul.__dict__.pop("_kmesh_list", None)
         lib = _lib.load()
         nbytes = ctypes.c_size_t()
         _lib.check(lib.admp_qeq_workspace_bytes(ctypes.byref(pr), ctypes.byref(nbytes)),

@@ -1,8 +1,6 @@
 """Constants, unit handling and the small tensor utilities of the reference
 interface (src/torch_admp/utils.py).  Host-side only: plain torch ops."""
 
-from __future__ import annotations
-
 import math
 from typing import Dict, List, Optional
 
