@@ -75,17 +75,23 @@ int admp_nblist_fill(const double* pos, int64_t n, const double* box, double cut
                      void* stream);
 
 /* Pair distances |pbc_shift(r_i - r_j)| (src/torch_admp/nblist.py:517-551,
- * src/torch_admp/spatial.py:15-36); box == NULL gives |r_j - r_i|.  Also writes
- * the minimum-image displacement dr (P, 3) used by the backward. */
+ * src/torch_admp/spatial.py:15-36); box == NULL gives |r_j - r_i|.  dr (may
+ * be NULL) receives the minimum-image displacement (P, 3). */
 int admp_pair_ds(const double* pos, const int64_t* pairs, int64_t n_pairs, const double* box,
                  double* ds, double* dr, void* stream);
 
 /* Backward of admp_pair_ds: grad_pos (n, 3) += dL/dds * d ds / d r;
  * grad_box (9) += dL/dds * d ds / d box (may be NULL).  grad_pos and grad_box
- * must be zeroed by the caller; accumulation uses float64 atomics. */
+ * must be zeroed by the caller.  dr may be NULL (the displacement is then
+ * recomputed from pos).  Runs of equal first atoms (lists grouped by i) are
+ * summed in registers before the float64 atomics; the box term goes through
+ * per-work-group partials in the workspace (admp_pair_ds_backward_workspace_bytes,
+ * needed when grad_box and box are given) and a fixed-order reduction. */
+int admp_pair_ds_backward_workspace_bytes(int64_t n_pairs, size_t* bytes);
 int admp_pair_ds_backward(const double* grad_ds, const int64_t* pairs, int64_t n_pairs,
                           const double* pos, const double* box, const double* dr,
-                          const double* ds, double* grad_pos, double* grad_box, void* stream);
+                          const double* ds, double* grad_pos, double* grad_box,
+                          void* workspace, size_t ws_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Pair energy over a caller-supplied pair list, nf frames of P pairs each.
@@ -113,7 +119,10 @@ int admp_pme_workspace_bytes(const int32_t* kmesh, int32_t precision, int64_t n,
                              size_t* bytes);
 
 /* Charge spreading Q = sum_i q_i theta_i (src/torch_admp/recip.py:216-252).
- * Overwrites mesh. */
+ * Overwrites mesh.  With the workspace of admp_pme_workspace_bytes for these n
+ * atoms, the atoms are binned into cells of about 8 mesh points per axis and
+ * each cell's stencils are summed in an LDS tile added once to the mesh;
+ * without it (workspace NULL) every atom adds its 216 weights to the mesh. */
 int admp_pme_spread(const double* pos, const double* q, int64_t n, const double* box,
                     const int32_t* kmesh, int32_t precision, void* mesh, void* workspace,
                     size_t ws_bytes, void* stream);

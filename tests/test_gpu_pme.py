@@ -34,6 +34,7 @@ def _system(n, box, seed, neutral=True):
 
 
 TRICLINIC = [[12.0, 0.0, 0.0], [2.5, 13.0, 0.0], [-1.5, 1.0, 14.0]]
+BIG_TRICLINIC_NB = [[36.0, 0.0, 0.0], [5.0, 34.0, 0.0], [-3.0, 4.0, 38.0]]
 SYMMETRIC = [[12.0, 0.4, 0.3], [0.4, 13.0, 0.2], [0.3, 0.2, 14.0]]
 
 
@@ -60,9 +61,29 @@ def test_nblist_pair_set_exact(dev, box):
     assert not amb
     assert got == ref == brute
     assert len(got) == pairs.shape[0]  # no duplicates
-    # grouped by i, buffer scales i < j
-    assert torch.all(pairs[1:, 0] >= pairs[:-1, 0])
+    _assert_grouped(pairs)
     assert torch.equal(nbl.get_buffer_scales().cpu(), (pairs[:, 0] < pairs[:, 1]).long())
+
+
+def _assert_grouped(pairs):
+    """Each first atom's pairs are contiguous."""
+    first = pairs[:, 0]
+    starts = torch.cat([torch.ones(1, dtype=torch.bool), first[1:] != first[:-1]])
+    assert int(starts.sum()) == len(torch.unique(first))
+
+
+@pytest.mark.parametrize("box,rc", [(np.eye(3) * 20.0, 6.0), (np.diag([33.0, 41.0, 29.0]), 7.5),
+                                    (BIG_TRICLINIC_NB, 6.0)])
+def test_nblist_many_cells(dev, box, rc):
+    """Boxes of 27-200 cells (the dense 20 A box stages more candidates than
+    one LDS chunk holds per cell), against brute force; grouped by first atom."""
+    pos, _, box = _system(3000, box, 21)
+    pairs = TorchNeighborList(rc)(pos.to(dev), box.to(dev)).cpu()
+    got = set(map(tuple, pairs.tolist()))
+    brute, amb = nblist_ref.brute_force_pairs(pos.numpy(), box.numpy(), rc)
+    assert not amb
+    assert got == brute and len(got) == pairs.shape[0]
+    _assert_grouped(pairs)
 
 
 def test_nblist_obc(dev):
@@ -73,14 +94,21 @@ def test_nblist_obc(dev):
     assert set(map(tuple, pairs.tolist())) == ref
 
 
+@pytest.mark.parametrize("order", ["grouped", "shuffled"])
 @pytest.mark.parametrize("box", [np.eye(3) * 13.0, TRICLINIC])
-def test_pair_ds_and_backward(dev, box):
+def test_pair_ds_and_backward(dev, box, order):
+    """ds and its transpose, for pair lists grouped by first atom and in
+    random order, with a third of the upstream gradients zero (as the i > j
+    half of a reference list is, through its zero buffer scales)."""
     pos, _, box = _system(150, box, 3)
     pairs = nblist_ref.nblist_pbc(pos, box, 5.0)
+    rng = np.random.default_rng(4)
+    if order == "shuffled":
+        pairs = pairs[torch.tensor(rng.permutation(pairs.shape[0]))]
     p = pos.clone().requires_grad_(True)
     b = box.clone().requires_grad_(True)
     ds_ref = nblist_ref.pairs_ds(p, pairs, b)
-    w = torch.tensor(np.random.default_rng(4).normal(size=ds_ref.shape[0]))
+    w = torch.tensor(rng.normal(size=ds_ref.shape[0]) * (rng.random(ds_ref.shape[0]) < 0.67))
     gp_ref, gb_ref = torch.autograd.grad((ds_ref * w).sum(), [p, b])
     pg = pos.to(dev).requires_grad_(True)
     bg = box.to(dev).requires_grad_(True)
@@ -111,9 +139,12 @@ def test_real_space_pair_energy(dev):
     assert torch.allclose(gd.cpu(), gd_ref, atol=1e-11, rtol=1e-10)
 
 
+@pytest.mark.parametrize("tiled", [False, True])
 @pytest.mark.parametrize("precision,tol", [("float64", 1e-12), ("float32", 2e-6)])
-@pytest.mark.parametrize("kmesh", [[16, 18, 20], [25, 12, 27], [74, 30, 22]])
-def test_spread_mesh(dev, precision, tol, kmesh):
+@pytest.mark.parametrize("kmesh", [[16, 18, 20], [25, 12, 27], [74, 30, 22], [7, 9, 64]])
+def test_spread_mesh(dev, precision, tol, kmesh, tiled):
+    """admp_pme_spread without a workspace (per-atom stencils) and with one
+    (atoms binned into cells of about 8 mesh points, one LDS tile per cell)."""
     import ctypes
 
     from torch_admp_amd import _lib
@@ -125,9 +156,15 @@ def test_spread_mesh(dev, precision, tol, kmesh):
     dt = torch.float32 if precision == "float32" else F64
     mesh = torch.empty(int(np.prod(kmesh)), dtype=dt, device=dev)
     pd, qd = pos.to(dev).contiguous(), q.to(dev).contiguous()
+    ws, nb = None, 0
+    if tiled:
+        nbytes = ctypes.c_size_t()
+        _lib.check(lib.admp_pme_workspace_bytes(_lib.kmesh_array(kmesh), _lib.PRECISION[precision],
+                                                pos.shape[0], ctypes.byref(nbytes)), "ws")
+        ws, nb = _lib.workspace(nbytes.value, dev), nbytes.value
     _lib.check(lib.admp_pme_spread(_lib.ptr(pd), _lib.ptr(qd), pos.shape[0], _lib.box_array(box),
                                    _lib.kmesh_array(kmesh), _lib.PRECISION[precision],
-                                   _lib.ptr(mesh), None, 0, _lib.stream_of(pd)), "spread")
+                                   _lib.ptr(mesh), _lib.ptr(ws), nb, _lib.stream_of(pd)), "spread")
     got = mesh.double().cpu().reshape(kmesh)
     assert (got - ref).abs().max() <= tol * ref.abs().max()
     assert abs(float(got.sum()) - float(q.sum())) < 1e-9 + tol * len(q)

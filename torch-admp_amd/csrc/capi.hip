@@ -52,12 +52,31 @@ Geom make_geom(const double* box, const int32_t* kmesh) {
 
 size_t elem_size(int32_t precision) { return precision == ADMP_F32 ? 4 : 8; }
 
-// PME workspace: spectrum (K1*K2*(K3/2+1) complex) + partials.
+// Cell grid of the tiled spread of admp_pme_spread: about kSpreadCellPts mesh
+// points per cell along each axis, from the mesh alone (the box only sets the
+// fractional coordinates).
+constexpr int kSpreadCellPts = 8;
+
+CellGrid spread_grid(const double* box, const int32_t* k) {
+  CellGrid cg = make_grid(box, 1.0, box == nullptr);
+  int64_t ncell = 1;
+  for (int d = 0; d < 3; ++d) {
+    cg.nc[d] = std::max(1, std::min(1024, k[d] / kSpreadCellPts));
+    cg.nimg[d] = 1;
+    ncell *= cg.nc[d];
+  }
+  cg.ncell = (int)ncell;
+  return cg;
+}
+
+// PME workspace: spectrum (K1*K2*(K3/2+1) complex) + partials, then the cell
+// binning of n atoms used by the tiled spread.
 struct PmeLayout {
-  size_t spec, partials, bytes;
+  size_t spec, partials, nb, bytes;
+  NbLayout nbl;
 };
 
-PmeLayout pme_layout(const int32_t* k, int32_t precision) {
+PmeLayout pme_layout(const int32_t* k, int32_t precision, int64_t n = 0) {
   PmeLayout L;
   const size_t nspec = (size_t)k[0] * k[1] * (k[2] / 2 + 1);
   L.spec = 0;
@@ -65,6 +84,12 @@ PmeLayout pme_layout(const int32_t* k, int32_t precision) {
   L.partials = off;
   const size_t np = std::max(convolve_partials_count(k), (size_t)kRedBlocks * 9 + 16);
   off += align256(np * sizeof(double));
+  L.nb = off;
+  CellGrid cg;
+  cg.ncell = 1;
+  for (int d = 0; d < 3; ++d) cg.ncell *= std::max(1, std::min(1024, k[d] / kSpreadCellPts));
+  nb_layout(n, cg, &L.nbl);
+  off += align256(L.nbl.bytes);
   L.bytes = off;
   return L;
 }
@@ -293,15 +318,25 @@ int admp_pair_ds(const double* pos, const int64_t* pairs, int64_t n_pairs, const
   return hip_status(launch_pair_ds(pos, pairs, n_pairs, cg, ds, dr, (hipStream_t)stream));
 }
 
+int admp_pair_ds_backward_workspace_bytes(int64_t n_pairs, size_t* bytes) {
+  if (n_pairs < 0 || !bytes) return ADMP_ERR_INVALID;
+  *bytes = align256(pair_ds_bwd_scratch_bytes(n_pairs));
+  return ADMP_OK;
+}
+
 int admp_pair_ds_backward(const double* grad_ds, const int64_t* pairs, int64_t n_pairs,
                           const double* pos, const double* box, const double* dr,
-                          const double* ds, double* grad_pos, double* grad_box, void* stream) {
-  if (n_pairs < 0 || (n_pairs > 0 && (!grad_ds || !pairs || !pos || !dr || !ds || !grad_pos)))
+                          const double* ds, double* grad_pos, double* grad_box,
+                          void* workspace, size_t ws_bytes, void* stream) {
+  if (n_pairs < 0 || (n_pairs > 0 && (!grad_ds || !pairs || !pos || !ds || !grad_pos)))
     return ADMP_ERR_INVALID;
   if (box && !box_ok(box)) return ADMP_ERR_INVALID;
+  if (grad_box && box && n_pairs > 0 &&
+      (!workspace || ws_bytes < pair_ds_bwd_scratch_bytes(n_pairs)))
+    return ADMP_ERR_INVALID;
   const CellGrid cg = make_grid(box, 1.0, box == nullptr);
   return hip_status(launch_pair_ds_bwd(grad_ds, pairs, n_pairs, pos, cg, dr, ds, grad_pos,
-                                       grad_box, (hipStream_t)stream));
+                                       grad_box, (double*)workspace, (hipStream_t)stream));
 }
 
 // ------------------------------------------------------------- pair energy
@@ -335,22 +370,36 @@ int admp_pair_energy(int32_t mode, int64_t nf, int64_t n, int64_t n_pairs, const
 int admp_pme_workspace_bytes(const int32_t* kmesh, int32_t precision, int64_t n, size_t* bytes) {
   if (!kmesh_ok(kmesh) || (precision != ADMP_F32 && precision != ADMP_F64) || n < 0 || !bytes)
     return ADMP_ERR_INVALID;
-  *bytes = pme_layout(kmesh, precision).bytes;
+  *bytes = pme_layout(kmesh, precision, n).bytes;
   return ADMP_OK;
 }
 
 int admp_pme_spread(const double* pos, const double* q, int64_t n, const double* box,
                     const int32_t* kmesh, int32_t precision, void* mesh, void* workspace,
                     size_t ws_bytes, void* stream) {
-  (void)workspace;
-  (void)ws_bytes;
   if (!kmesh_ok(kmesh) || !box_ok(box) || n < 0 || !mesh || (n > 0 && (!pos || !q)))
     return ADMP_ERR_INVALID;
+  if (precision != ADMP_F32 && precision != ADMP_F64) return ADMP_ERR_INVALID;
   const Geom g = make_geom(box, kmesh);
   hipStream_t s = (hipStream_t)stream;
+  const size_t es = elem_size(precision);
+  const PmeLayout L = pme_layout(kmesh, precision, n);
+  const CellGrid cg = spread_grid(box, kmesh);
+  const TileGeom tg = make_tile_geom(cg, kmesh, es);
+  if (workspace && ws_bytes >= L.bytes && tg.ok && n > 0) {
+    // bin the atoms into the spread cells, then one LDS tile per cell
+    char* nbws = (char*)workspace + L.nb;
+    if (launch_bin(pos, n, cg, nbws, L.nbl, s) != hipSuccess) return ADMP_ERR_HIP;
+    const int* sorted = (const int*)(nbws + L.nbl.sorted);
+    const int* start = (const int*)(nbws + L.nbl.start);
+    if (precision == ADMP_F32)
+      return hip_status(launch_spread_tiled<float>(pos, q, n, g, tg, cg.ncell, sorted, start,
+                                                   (float*)mesh, s));
+    return hip_status(launch_spread_tiled<double>(pos, q, n, g, tg, cg.ncell, sorted, start,
+                                                  (double*)mesh, s));
+  }
   if (precision == ADMP_F32) return hip_status(launch_spread<float>(pos, q, n, g, (float*)mesh, s));
-  if (precision == ADMP_F64) return hip_status(launch_spread<double>(pos, q, n, g, (double*)mesh, s));
-  return ADMP_ERR_INVALID;
+  return hip_status(launch_spread<double>(pos, q, n, g, (double*)mesh, s));
 }
 
 int admp_pme_convolve(const void* mesh, const double* box, const int32_t* kmesh,

@@ -132,65 +132,160 @@ __global__ void __launch_bounds__(kBlock) k_nb_rank(int64_t n, const int* __rest
   sorted[b + rank] = i;
 }
 
-// Count (MODE 0) or write (MODE 1) the ordered pairs of atom i:
-// PBC keeps 0.01 < |w_j + n h - w_i| < rc (src/torch_admp/nblist.py:433-445);
-// OBC keeps |r_j - r_i| < rc, j != i (nblist.py:472-490).
+// Count (MODE 0) or write (MODE 1) the ordered pairs of the home atoms of one
+// cell: PBC keeps 0.01 < |w_j + n h - w_i| < rc (src/torch_admp/nblist.py:433-445),
+// OBC keeps |r_j - r_i| < rc, j != i (nblist.py:472-490).  Work-group
+// (cell, split) owns the split-th contiguous share of the cell's sorted home
+// slots.  Candidates of the full stencil (both orderings are listed) are
+// staged in LDS in float64, kNbCand at a time; one wavefront takes one home
+// atom, its lanes test 64 candidates per step and the survivors of a step are
+// compacted by ballot.  The distance test compares squared distances and takes
+// the square root only within 1e-12 (relative) of rc^2 or 1e-4, where it
+// decides exactly as sqrt(d2) < rc and d > 0.01 do.  Both passes enumerate the
+// pairs in the same order (stencil entry, candidate chunk, lane), so pass 1
+// writes each home atom's pairs at the offset pass 0 counted, grouped by home
+// atom in cell order.
+constexpr int kNbThreads = 512;
+constexpr int kNbCand = 2048;     // staged candidates per chunk
+constexpr int kNbHomeBlk = 256;   // home atoms whose counters sit in LDS at once
+
+__device__ __forceinline__ bool pair_kept(double d2, double rc, double rc2_lo, double rc2_hi,
+                                          bool obc, bool same) {
+  constexpr double kMin2 = 0.01 * 0.01;
+  if (obc) {
+    if (same || d2 >= rc2_hi) return false;
+    if (d2 < rc2_lo) return true;
+    return sqrt(d2) < rc;
+  }
+  if (d2 >= rc2_hi) return false;
+  if (d2 < rc2_lo && d2 > kMin2 * (1.0 + 1e-12)) return true;
+  const double d = sqrt(d2);
+  return d < rc && d * d > kMin2 && d > 0.01;
+}
+
 template <int MODE>
-__global__ void __launch_bounds__(kBlock) k_nb_pairs(int64_t n, CellGrid cg,
-                                                     const double* __restrict__ w,
-                                                     const int* __restrict__ cell_of,
-                                                     const int* __restrict__ start,
-                                                     const int* __restrict__ sorted,
-                                                     int64_t* __restrict__ counts,
-                                                     const int64_t* __restrict__ offsets,
-                                                     int64_t* __restrict__ pairs,
-                                                     int64_t capacity) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int i = sorted[t];
-  const double wi[3] = {w[3 * i], w[3 * i + 1], w[3 * i + 2]};
-  const int cid = cell_of[i];
-  const int ci[3] = {cid / (cg.nc[1] * cg.nc[2]), (cid / cg.nc[2]) % cg.nc[1], cid % cg.nc[2]};
-  const double rmin2 = 0.01 * 0.01;
+__global__ void __launch_bounds__(kNbThreads) k_nb_cell_pairs(
+    CellGrid cg, int S, const double* __restrict__ w, const int* __restrict__ start,
+    const int* __restrict__ sorted, int64_t* __restrict__ pcount,
+    const int64_t* __restrict__ offsets, int64_t* __restrict__ pairs, int64_t capacity) {
+  __shared__ double cx[kNbCand], cy[kNbCand], cz[kNbCand];
+  __shared__ int cj[kNbCand];
+  __shared__ int64_t hcnt[kNbHomeBlk];
+  __shared__ int st_pre[kMaxStencil + 1];
+  __shared__ int st_beg[kMaxStencil];
+  __shared__ double st_sh[kMaxStencil * 3];
+  const int c = blockIdx.x, split = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int nw = kNbThreads / 64;
+  const int ci[3] = {c / (cg.nc[1] * cg.nc[2]), (c / cg.nc[2]) % cg.nc[1], c % cg.nc[2]};
+  const int cb = start[c * kSub], ce = start[(c + 1) * kSub];
+  const int share = (ce - cb + S - 1) / S;
+  const int hb = cb + split * share, he = min(ce, hb + share);
+  if (hb >= he) return;
   const double rc = sqrt(cg.rc2);
-  int64_t cnt = 0;
-  int64_t out = MODE == 1 ? offsets[i] : 0;
-  const int l0 = stencil_len(cg, 0), l1 = stencil_len(cg, 1), l2 = stencil_len(cg, 2);
-  for (int o0 = 0; o0 < l0; ++o0) {
-    int c0, s0;
-    stencil_at(cg, 0, ci[0], o0, &c0, &s0);
-    for (int o1 = 0; o1 < l1; ++o1) {
-      int c1, s1;
-      stencil_at(cg, 1, ci[1], o1, &c1, &s1);
-      for (int o2 = 0; o2 < l2; ++o2) {
-        int c2, s2;
-        stencil_at(cg, 2, ci[2], o2, &c2, &s2);
-        double sh[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) sh[j] = s0 * cg.h[j] + s1 * cg.h[3 + j] + s2 * cg.h[6 + j];
-        const int cc = (c0 * cg.nc[1] + c1) * cg.nc[2] + c2;
-        const int xe = start[(cc + 1) * kSub];
-        for (int x = start[cc * kSub]; x < xe; ++x) {
-          const int j = sorted[x];
-          const double dx = (w[3 * j] + sh[0]) - wi[0];
-          const double dy = (w[3 * j + 1] + sh[1]) - wi[1];
-          const double dz = (w[3 * j + 2] + sh[2]) - wi[2];
-          const double d = sqrt(dx * dx + dy * dy + dz * dz);
-          bool keep;
-          if (cg.obc) keep = (j != i) && d < rc;
-          else keep = d < rc && d * d > rmin2 && d > 0.01;
-          if (keep) {
-            if (MODE == 1 && out + cnt < capacity) {
-              pairs[2 * (out + cnt)] = i;
-              pairs[2 * (out + cnt) + 1] = j;
+  const double rc2_lo = cg.rc2 * (1.0 - 1e-12), rc2_hi = cg.rc2 * (1.0 + 1e-12);
+  const bool obc = cg.obc != 0;
+  // rectangular cells: candidates farther than rc from the home cell's box
+  // pair with no home atom (home atoms lie in the box up to the binning's
+  // rounding; 1e-4 A of slack)
+  const bool prune = !obc && cg.h[1] == 0.0 && cg.h[2] == 0.0 && cg.h[3] == 0.0 &&
+                     cg.h[5] == 0.0 && cg.h[6] == 0.0 && cg.h[7] == 0.0;
+  double blo[3], bhi[3];
+  blo[0] = cg.h[0] * ci[0] / cg.nc[0];
+  blo[1] = cg.h[4] * ci[1] / cg.nc[1];
+  blo[2] = cg.h[8] * ci[2] / cg.nc[2];
+  bhi[0] = cg.h[0] * (ci[0] + 1) / cg.nc[0];
+  bhi[1] = cg.h[4] * (ci[1] + 1) / cg.nc[1];
+  bhi[2] = cg.h[8] * (ci[2] + 1) / cg.nc[2];
+  const double prune_r = rc + 1e-4, prune_r2 = prune_r * prune_r;
+  const int nst = stencil_len(cg, 0) * stencil_len(cg, 1) * stencil_len(cg, 2);
+
+  for (int h0 = hb; h0 < he; h0 += kNbHomeBlk) {
+    const int nh = min(kNbHomeBlk, he - h0);
+    for (int t = threadIdx.x; t < nh; t += kNbThreads) hcnt[t] = MODE == 1 ? offsets[h0 + t] : 0;
+    for (int st0 = 0; st0 < nst; st0 += kMaxStencil) {
+      __syncthreads();
+      const int m = load_stencil_chunk(cg, 0, ci, st0, nst, start, st_pre, st_beg, st_sh);
+      const int ctot = st_pre[m];
+      for (int k0 = 0; k0 < ctot; k0 += kNbCand) {
+        // stage this chunk's candidates that can reach the home cell, compacted
+        // in candidate order (one ballot round of kNbThreads per pass)
+        int cc = 0;
+        for (int r0 = k0; r0 < min(ctot, k0 + kNbCand); r0 += kNbThreads) {
+          const int kg = r0 + threadIdx.x;
+          bool keep = false;
+          double px = 0.0, py = 0.0, pz = 0.0;
+          int j = 0;
+          if (kg < min(ctot, k0 + kNbCand)) {
+            int lo = 0, hi = m;
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (st_pre[mid] <= kg) lo = mid;
+              else hi = mid;
             }
-            ++cnt;
+            j = sorted[st_beg[lo] + (kg - st_pre[lo])];
+            px = w[3 * j] + st_sh[lo * 3];
+            py = w[3 * j + 1] + st_sh[lo * 3 + 1];
+            pz = w[3 * j + 2] + st_sh[lo * 3 + 2];
+            keep = true;
+            if (prune) {
+              const double dx = fmax(0.0, fmax(blo[0] - px, px - bhi[0]));
+              const double dy = fmax(0.0, fmax(blo[1] - py, py - bhi[1]));
+              const double dz = fmax(0.0, fmax(blo[2] - pz, pz - bhi[2]));
+              keep = dx * dx + dy * dy + dz * dz < prune_r2;
+            }
           }
+          __shared__ int wcnt[nw];
+          const unsigned long long msk = __ballot(keep);
+          if (lane == 0) wcnt[wave] = __popcll(msk);
+          __syncthreads();
+          int at = cc + __popcll(msk & ((1ull << lane) - 1ull)), tot = 0;
+#pragma unroll
+          for (int v = 0; v < nw; ++v) {
+            at += v < wave ? wcnt[v] : 0;
+            tot += wcnt[v];
+          }
+          if (keep) {
+            cx[at] = px;
+            cy[at] = py;
+            cz[at] = pz;
+            cj[at] = j;
+          }
+          cc += tot;
+          __syncthreads();
         }
+        // one wavefront per home atom, lanes over the staged candidates
+        for (int t = wave; t < nh; t += nw) {
+          const int i = sorted[h0 + t];
+          const double wx = w[3 * i], wy = w[3 * i + 1], wz = w[3 * i + 2];
+          int64_t cnt = hcnt[t];
+          for (int k = lane; k - lane < cc; k += 64) {
+            bool keep = false;
+            int j = 0;
+            if (k < cc) {
+              j = cj[k];
+              const double dx = cx[k] - wx, dy = cy[k] - wy, dz = cz[k] - wz;
+              keep = pair_kept(dx * dx + dy * dy + dz * dz, rc, rc2_lo, rc2_hi, obc, j == i);
+            }
+            const unsigned long long msk = __ballot(keep);
+            if (MODE == 1 && keep) {
+              const int64_t o = cnt + __popcll(msk & ((1ull << lane) - 1ull));
+              if (o < capacity) {
+                pairs[2 * o] = i;
+                pairs[2 * o + 1] = j;
+              }
+            }
+            cnt += __popcll(msk);
+          }
+          if (lane == 0) hcnt[t] = cnt;
+        }
+        __syncthreads();
       }
     }
+    __syncthreads();
+    if (MODE == 0)
+      for (int t = threadIdx.x; t < nh; t += kNbThreads) pcount[h0 + t] = hcnt[t];
   }
-  if (MODE == 0) counts[i] = cnt;
 }
 
 // ---------------------------------------------------------------------------
@@ -324,6 +419,26 @@ template void launch_scan<int64_t>(const int64_t*, int64_t, int64_t*, int64_t*, 
 
 // ---------------------------------------------------------------------------
 // Pair distances: dr = pbc_shift(r_i - r_j) (src/torch_admp/spatial.py:15-36)
+__device__ __forceinline__ void pair_disp(const double* __restrict__ pos, int64_t i, int64_t j,
+                                          const CellGrid& cg, double v[3], double nimg[3]) {
+  if (cg.obc) {
+    for (int d = 0; d < 3; ++d) {
+      v[d] = pos[3 * j + d] - pos[3 * i + d];
+      nimg[d] = 0.0;
+    }
+    return;
+  }
+  double x[3];
+  for (int d = 0; d < 3; ++d) x[d] = pos[3 * i + d] - pos[3 * j + d];
+  double s[3];
+  for (int d = 0; d < 3; ++d) {
+    s[d] = x[0] * cg.a[d] + x[1] * cg.a[3 + d] + x[2] * cg.a[6 + d];
+    nimg[d] = floor(s[d] + 0.5);
+    s[d] -= nimg[d];
+  }
+  for (int d = 0; d < 3; ++d) v[d] = s[0] * cg.h[d] + s[1] * cg.h[3 + d] + s[2] * cg.h[6 + d];
+}
+
 __global__ void __launch_bounds__(kBlock) k_pair_ds(const double* __restrict__ pos,
                                                     const int64_t* __restrict__ pairs,
                                                     int64_t np, CellGrid cg,
@@ -331,24 +446,21 @@ __global__ void __launch_bounds__(kBlock) k_pair_ds(const double* __restrict__ p
                                                     double* __restrict__ dr) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const int64_t i = pairs[2 * p], j = pairs[2 * p + 1];
-  double v[3];
-  if (cg.obc) {
-    for (int d = 0; d < 3; ++d) v[d] = pos[3 * j + d] - pos[3 * i + d];
-  } else {
-    double x[3];
-    for (int d = 0; d < 3; ++d) x[d] = pos[3 * i + d] - pos[3 * j + d];
-    double s[3];
-    for (int d = 0; d < 3; ++d) {
-      s[d] = x[0] * cg.a[d] + x[1] * cg.a[3 + d] + x[2] * cg.a[6 + d];
-      s[d] -= floor(s[d] + 0.5);
-    }
-    for (int d = 0; d < 3; ++d) v[d] = s[0] * cg.h[d] + s[1] * cg.h[3 + d] + s[2] * cg.h[6 + d];
-  }
+  double v[3], nimg[3];
+  pair_disp(pos, pairs[2 * p], pairs[2 * p + 1], cg, v, nimg);
   ds[p] = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
   if (dr)
     for (int d = 0; d < 3; ++d) dr[3 * p + d] = v[d];
 }
+
+// dL/dr and dL/dbox of L(ds).  The displacement is recomputed from the
+// positions (or read from dr when given).  Pair lists grouped by their first
+// atom put runs of equal i in consecutive lanes: a segmented inclusive scan
+// over DPP row shifts sums each run piece within a row of 16 lanes and its
+// last lane adds once; the second atom adds per pair.  The box term (image
+// pairs only) is summed per work-group into partial[block][9], reduced in a
+// fixed order by k_reduce_box.
+constexpr int kDsBwdBlocks = 1024;
 
 __global__ void __launch_bounds__(kBlock) k_pair_ds_bwd(const double* __restrict__ gds,
                                                         const int64_t* __restrict__ pairs,
@@ -356,30 +468,104 @@ __global__ void __launch_bounds__(kBlock) k_pair_ds_bwd(const double* __restrict
                                                         CellGrid cg, const double* __restrict__ dr,
                                                         const double* __restrict__ ds,
                                                         double* __restrict__ gpos,
-                                                        double* __restrict__ gbox) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  const double d = ds[p];
-  const double g = gds[p];
-  if (d <= 0.0 || g == 0.0) return;
-  const int64_t i = pairs[2 * p], j = pairs[2 * p + 1];
-  const double u[3] = {dr[3 * p] / d, dr[3 * p + 1] / d, dr[3 * p + 2] / d};
-  // PBC: dr = r_i - r_j - n h; OBC: dr = r_j - r_i
-  const double si = cg.obc ? -1.0 : 1.0;
-  for (int k = 0; k < 3; ++k) {
-    unsafeAtomicAdd(&gpos[3 * i + k], si * g * u[k]);
-    unsafeAtomicAdd(&gpos[3 * j + k], -si * g * u[k]);
-  }
-  if (gbox && !cg.obc) {
-    double x[3], nimg[3];
-    for (int k = 0; k < 3; ++k) x[k] = pos[3 * i + k] - pos[3 * j + k];
-    for (int k = 0; k < 3; ++k) {
-      const double s = x[0] * cg.a[k] + x[1] * cg.a[3 + k] + x[2] * cg.a[6 + k];
-      nimg[k] = floor(s + 0.5);
+                                                        double* __restrict__ partial) {
+  __shared__ double red[9 * (kBlock / 64)];
+  double gb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // every lane runs the same number of iterations (DPP needs the whole row)
+  const int64_t iters = (np + stride - 1) / stride;
+  for (int64_t it = 0; it < iters; ++it) {
+    const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    int key = -1 - lane;  // lanes past the end form runs of their own
+    int64_t j = 0;
+    bool act = false;
+    if (p < np) {
+      const int64_t i = pairs[2 * p];
+      key = (int)i;  // rows with a zero gradient keep their run contiguous
+      const double d = ds[p];
+      const double g = gds[p];
+      if (d > 0.0 && g != 0.0) {
+        j = pairs[2 * p + 1];
+        double v[3], nimg[3];
+        if (dr) {
+          for (int k = 0; k < 3; ++k) v[k] = dr[3 * p + k];
+          if (!cg.obc) {
+            double x[3];
+            for (int k = 0; k < 3; ++k) x[k] = pos[3 * i + k] - pos[3 * j + k];
+            for (int k = 0; k < 3; ++k)
+              nimg[k] = floor(x[0] * cg.a[k] + x[1] * cg.a[3 + k] + x[2] * cg.a[6 + k] + 0.5);
+          }
+        } else {
+          pair_disp(pos, i, j, cg, v, nimg);
+        }
+        // PBC: v = r_i - r_j - n h; OBC: v = r_j - r_i
+        const double si = cg.obc ? -1.0 : 1.0;
+        const double f = si * g / d;
+        gx = f * v[0];
+        gy = f * v[1];
+        gz = f * v[2];
+        act = true;
+        if (!cg.obc)
+          for (int c = 0; c < 3; ++c)
+            if (nimg[c] != 0.0)
+              for (int b = 0; b < 3; ++b) gb[c * 3 + b] -= g * nimg[c] * v[b] / d;
+      }
     }
-    for (int c = 0; c < 3; ++c)
-      if (nimg[c] != 0.0)
-        for (int b = 0; b < 3; ++b) unsafeAtomicAdd(&gbox[c * 3 + b], -g * nimg[c] * u[b]);
+    if (act) {
+      unsafeAtomicAdd(&gpos[3 * j], -gx);
+      unsafeAtomicAdd(&gpos[3 * j + 1], -gy);
+      unsafeAtomicAdd(&gpos[3 * j + 2], -gz);
+    }
+    // segmented inclusive sum of the i terms within each row of 16 lanes: rs is
+    // the row lane where this lane's run of equal keys starts (a max-scan of
+    // the run heads), and a lane adds the partial sum K lanes back only when
+    // that lane lies in the same contiguous run.  Keys that recur further on
+    // in the row start runs of their own, so any pair order is summed exactly.
+    const int rl = lane & 15;
+    const int prev = row_shr<1>(~key, key);
+    int rs = (rl == 0 || prev != key) ? rl : 0;
+    rs = max(rs, row_shr<1>(0, rs));
+    rs = max(rs, row_shr<2>(0, rs));
+    rs = max(rs, row_shr<4>(0, rs));
+    rs = max(rs, row_shr<8>(0, rs));
+#define ADMP_SEG_STEP(K)                                                            \
+  {                                                                                 \
+    const double tx = row_shr<K>(gx), ty = row_shr<K>(gy), tz = row_shr<K>(gz);     \
+    const bool same = rl - K >= rs;                                                 \
+    gx += same ? tx : 0.0;                                                          \
+    gy += same ? ty : 0.0;                                                          \
+    gz += same ? tz : 0.0;                                                          \
+  }
+    ADMP_SEG_STEP(1)
+    ADMP_SEG_STEP(2)
+    ADMP_SEG_STEP(4)
+    ADMP_SEG_STEP(8)
+#undef ADMP_SEG_STEP
+    const int next = row_shl1(~key, key);
+    if (key >= 0 && (rl == 15 || next != key) && (gx != 0.0 || gy != 0.0 || gz != 0.0)) {
+      unsafeAtomicAdd(&gpos[3 * (int64_t)key], gx);
+      unsafeAtomicAdd(&gpos[3 * (int64_t)key + 1], gy);
+      unsafeAtomicAdd(&gpos[3 * (int64_t)key + 2], gz);
+    }
+  }
+  if (partial) {
+    block_sum<9>(gb, red);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 9; ++k) partial[(int64_t)blockIdx.x * 9 + k] = gb[k];
+  }
+}
+
+// gbox[k] += sum over blocks of partial[b][k], in block order
+__global__ void __launch_bounds__(kBlock) k_reduce_box(const double* __restrict__ partial,
+                                                       int nblocks, double* __restrict__ gbox) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int k = wv; k < 9; k += kBlock / 64) {
+    double s = 0.0;
+    for (int b = lane; b < nblocks; b += 64) s += partial[(int64_t)b * 9 + k];
+    s = wave_sum(s);
+    if (lane == 0) gbox[k] += s;
   }
 }
 
@@ -547,13 +733,21 @@ hipError_t launch_pairs(int mode, int64_t n, const CellGrid& cg, char* ws, const
   const int* sorted = (const int*)(ws + L.sorted);
   int64_t* pc = (int64_t*)(ws + L.pcount);
   int64_t* off = (int64_t*)(ws + L.offsets);
+  (void)nb;
+  (void)cell_of;
+  // home splits per cell: at least ~512 work-groups, at most one per 64 home atoms
+  const int64_t ncell = cg.ncell;
+  const int max_occ = (int)((n + ncell - 1) / ncell);
+  int S = (int)std::max<int64_t>(1, (512 + ncell - 1) / ncell);
+  S = std::max(1, std::min(S, (max_occ + 63) / 64));
+  const dim3 grid((unsigned)ncell, (unsigned)S);
   if (mode == 0) {
-    k_nb_pairs<0><<<nb, kBlock, 0, s>>>(n, cg, w, cell_of, start, sorted, pc, nullptr, nullptr,
-                                        0);
+    k_nb_cell_pairs<0><<<grid, kNbThreads, 0, s>>>(cg, S, w, start, sorted, pc, nullptr, nullptr,
+                                                   0);
     launch_scan<int64_t>(pc, n, off, (int64_t*)(ws + L.scan), s);
   } else {
-    k_nb_pairs<1><<<nb, kBlock, 0, s>>>(n, cg, w, cell_of, start, sorted, nullptr, off, pairs,
-                                        capacity);
+    k_nb_cell_pairs<1><<<grid, kNbThreads, 0, s>>>(cg, S, w, start, sorted, nullptr, off, pairs,
+                                                   capacity);
   }
   return hipGetLastError();
 }
@@ -565,12 +759,21 @@ hipError_t launch_pair_ds(const double* pos, const int64_t* pairs, int64_t np, c
   return hipGetLastError();
 }
 
+size_t pair_ds_bwd_scratch_bytes(int64_t np) {
+  (void)np;
+  return sizeof(double) * 9 * kDsBwdBlocks;
+}
+
 hipError_t launch_pair_ds_bwd(const double* gds, const int64_t* pairs, int64_t np,
                               const double* pos, const CellGrid& cg, const double* dr,
-                              const double* ds, double* gpos, double* gbox, hipStream_t s) {
-  if (np > 0)
-    k_pair_ds_bwd<<<(unsigned)((np + kBlock - 1) / kBlock), kBlock, 0, s>>>(gds, pairs, np, pos, cg,
-                                                                          dr, ds, gpos, gbox);
+                              const double* ds, double* gpos, double* gbox, double* partial,
+                              hipStream_t s) {
+  if (np <= 0) return hipGetLastError();
+  const int nb = (int)std::min<int64_t>((np + kBlock - 1) / kBlock, kDsBwdBlocks);
+  const bool box_term = gbox && !cg.obc;
+  k_pair_ds_bwd<<<nb, kBlock, 0, s>>>(gds, pairs, np, pos, cg, dr, ds, gpos,
+                                      box_term ? partial : nullptr);
+  if (box_term) k_reduce_box<<<1, kBlock, 0, s>>>(partial, nb, gbox);
   return hipGetLastError();
 }
 

@@ -70,9 +70,12 @@ hipError_t launch_pairs(int mode, int64_t n, const CellGrid& cg, char* ws, const
                         int64_t* pairs, int64_t capacity, hipStream_t s);
 hipError_t launch_pair_ds(const double* pos, const int64_t* pairs, int64_t np,
                           const CellGrid& cg, double* ds, double* dr, hipStream_t s);
+// workspace of launch_pair_ds_bwd (per-block partial sums of dL/dbox)
+size_t pair_ds_bwd_scratch_bytes(int64_t np);
 hipError_t launch_pair_ds_bwd(const double* gds, const int64_t* pairs, int64_t np,
                               const double* pos, const CellGrid& cg, const double* dr,
-                              const double* ds, double* gpos, double* gbox, hipStream_t s);
+                              const double* ds, double* gpos, double* gbox, double* partial,
+                              hipStream_t s);
 hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const double* q,
                               const double* eta, const int64_t* pairs, const double* ds,
                               const double* bs, double kappa, double eps, double* energy,

@@ -27,9 +27,9 @@ positions, box, ds and eta.  Differentiating forces, dE/dds or dE/deta again
 (force-matching losses) raises NotImplementedError rather than returning a
 partial gradient.
 
-Auxiliary outputs (the displacement of ``pair_distance``, the potential mesh
-and k-space box derivative of ``pme_reciprocal``) feed the matching backward
-operator only; gradients sent into them are ignored.
+Auxiliary outputs (the potential mesh and k-space box derivative of
+``pme_reciprocal``) feed the matching backward operator only; gradients sent
+into them are ignored.
 """
 
 from __future__ import annotations
@@ -84,26 +84,26 @@ def neighbor_pairs(positions: Tensor, box: Optional[Tensor], cutoff: float) -> T
 
 # ------------------------------------------------------------------ pair distances
 @torch.library.custom_op("admp::pair_distance", mutates_args=())
-def pair_distance(positions: Tensor, box: Optional[Tensor], pairs: Tensor) -> Tuple[Tensor, Tensor]:
+def pair_distance(positions: Tensor, box: Optional[Tensor], pairs: Tensor) -> Tensor:
     """ds = |pbc_shift(r_j - r_i)| (src/torch_admp/nblist.py:517-551,
-    src/torch_admp/spatial.py:15-36), |r_j - r_i| when box is None; returns
-    (ds (P,) in the positions' dtype, displacement (P, 3) float64)."""
+    src/torch_admp/spatial.py:15-36), |r_j - r_i| when box is None; (P,) in the
+    positions' dtype."""
     _lib.require_gpu(positions, pairs)
     lib = _lib.load()
     pos = _f64(positions)
     prs = pairs.detach().to(torch.int64).contiguous()
     npairs = prs.shape[0]
     ds = torch.empty(npairs, dtype=_F64, device=pos.device)
-    dr = torch.empty(npairs, 3, dtype=_F64, device=pos.device)
     _lib.check(lib.admp_pair_ds(_lib.ptr(pos), _lib.ptr(prs), npairs, _lib.box_array(box),
-                                _lib.ptr(ds), _lib.ptr(dr), _lib.stream_of(pos)), "admp_pair_ds")
-    return ds.to(positions.dtype), dr
+                                _lib.ptr(ds), None, _lib.stream_of(pos)), "admp_pair_ds")
+    return ds.to(positions.dtype)
 
 
 @torch.library.custom_op("admp::pair_distance_backward", mutates_args=())
 def pair_distance_backward(grad_ds: Tensor, positions: Tensor, box: Optional[Tensor],
-                           pairs: Tensor, ds: Tensor, dr: Tensor) -> Tuple[Tensor, Tensor]:
-    """(dL/dpositions, dL/dbox) of L(ds); dL/dbox is zero when box is None."""
+                           pairs: Tensor, ds: Tensor) -> Tuple[Tensor, Tensor]:
+    """(dL/dpositions, dL/dbox) of L(ds); dL/dbox is zero when box is None.
+    The displacement is recomputed from the positions."""
     _lib.require_gpu(grad_ds, positions, pairs)
     lib = _lib.load()
     pos = _f64(positions)
@@ -112,10 +112,15 @@ def pair_distance_backward(grad_ds: Tensor, positions: Tensor, box: Optional[Ten
     gpos = torch.zeros_like(pos)
     gbox = torch.zeros(3, 3, dtype=_F64, device=pos.device)
     boxh = _lib.box_array(box)
+    nbytes = ctypes.c_size_t()
+    _lib.check(lib.admp_pair_ds_backward_workspace_bytes(prs.shape[0], ctypes.byref(nbytes)),
+               "admp_pair_ds_backward_workspace_bytes")
+    ws = _ws(nbytes.value, pos.device)
     _lib.check(lib.admp_pair_ds_backward(_lib.ptr(g), _lib.ptr(prs), prs.shape[0], _lib.ptr(pos),
-                                         boxh, _lib.ptr(_f64(dr)), _lib.ptr(_f64(ds)),
-                                         _lib.ptr(gpos), _lib.ptr(gbox) if boxh is not None else None,
-                                         _lib.stream_of(pos)), "admp_pair_ds_backward")
+                                         boxh, None, _lib.ptr(_f64(ds)), _lib.ptr(gpos),
+                                         _lib.ptr(gbox) if boxh is not None else None,
+                                         _lib.ptr(ws), nbytes.value, _lib.stream_of(pos)),
+               "admp_pair_ds_backward")
     if box is not None:
         gbox = gbox.to(device=box.device, dtype=box.dtype)
     return gpos.to(positions.dtype), gbox
@@ -123,15 +128,15 @@ def pair_distance_backward(grad_ds: Tensor, positions: Tensor, box: Optional[Ten
 
 def _pd_setup(ctx, inputs, output):
     positions, box, pairs = inputs
-    ctx.save_for_backward(positions, box, pairs, output[0], output[1])
+    ctx.save_for_backward(positions, box, pairs, output)
     ctx.set_materialize_grads(False)
 
 
-def _pd_backward(ctx, g_ds, g_dr):
-    positions, box, pairs, ds, dr = ctx.saved_tensors
+def _pd_backward(ctx, g_ds):
+    positions, box, pairs, ds = ctx.saved_tensors
     if g_ds is None:
         return None, None, None
-    gpos, gbox = torch.ops.admp.pair_distance_backward(g_ds, positions, box, pairs, ds, dr)
+    gpos, gbox = torch.ops.admp.pair_distance_backward(g_ds, positions, box, pairs, ds)
     return (gpos if ctx.needs_input_grad[0] else None,
             gbox if box is not None and ctx.needs_input_grad[1] else None, None)
 
@@ -143,7 +148,7 @@ def _pdb_setup(ctx, inputs, output):
 def _pdb_backward(ctx, g_gpos, g_gbox):
     if g_gpos is not None or g_gbox is not None:
         raise NotImplementedError(_SECOND_ORDER.format("the pair distances"))
-    return (None,) * 6
+    return (None,) * 5
 
 
 torch.library.register_autograd("admp::pair_distance", _pd_backward, setup_context=_pd_setup)

@@ -76,8 +76,8 @@ class TorchNeighborList(torch.nn.Module):
         """Differentiable pair distances (src/torch_admp/nblist.py:517-551)."""
         if pbc_flag:
             assert box is not None, "Box should be provided for periodic system."
-            return torch.ops.admp.pair_distance(positions, box, pairs)[0]
-        return torch.ops.admp.pair_distance(positions, None, pairs)[0]
+            return torch.ops.admp.pair_distance(positions, box, pairs)
+        return torch.ops.admp.pair_distance(positions, None, pairs)
 
     def set_pairs(self, pairs: torch.Tensor) -> None:
         self.pairs = pairs

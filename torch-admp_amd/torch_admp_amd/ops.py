@@ -28,7 +28,7 @@ def pair_distance(positions: torch.Tensor, box: Optional[torch.Tensor],
                   pairs: torch.Tensor) -> torch.Tensor:
     """ds = |pbc_shift(r_j - r_i)| (src/torch_admp/nblist.py:517-551); open
     boundaries when box is None.  Differentiable in positions and box."""
-    return torch.ops.admp.pair_distance(positions, box, pairs)[0]
+    return torch.ops.admp.pair_distance(positions, box, pairs)
 
 
 def pair_energy(q: torch.Tensor, ds: torch.Tensor, pairs: torch.Tensor, bs: torch.Tensor,
