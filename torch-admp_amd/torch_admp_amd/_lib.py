@@ -135,11 +135,14 @@ def require_gpu(*tensors: Optional[torch.Tensor]) -> None:
                 f"{t.device}")
 
 
-def box_array(box: Optional[torch.Tensor]):
-    """Host copy of a (3, 3) box as a ctypes double[9] (None for open boundaries)."""
+def box_array(box: Optional[torch.Tensor], box_host: Optional[torch.Tensor] = None):
+    """Host copy of a (3, 3) box as a ctypes double[9] (None for open
+    boundaries).  ``box_host``, a CPU copy the caller already holds, saves the
+    device-to-host copy (a stream synchronisation) of a device box."""
     if box is None:
         return None
-    vals = box.detach().reshape(9).to("cpu", torch.float64).tolist()
+    src = box_host if box_host is not None else box
+    vals = src.detach().reshape(9).to("cpu", torch.float64).tolist()
     return (ctypes.c_double * 9)(*vals)
 
 

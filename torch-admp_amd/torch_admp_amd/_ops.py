@@ -57,15 +57,18 @@ def _ws(nbytes: int, dev) -> Tensor:
 
 # ------------------------------------------------------------------ neighbour pairs
 @torch.library.custom_op("admp::neighbor_pairs", mutates_args=())
-def neighbor_pairs(positions: Tensor, box: Optional[Tensor], cutoff: float) -> Tensor:
+def neighbor_pairs(positions: Tensor, box: Optional[Tensor], cutoff: float,
+                   box_host: Optional[Tensor]) -> Tensor:
     """Ordered pairs (P, 2) int64 with 0.01 < d < cutoff (box) or d < cutoff
     (box None), both orderings, grouped by the first atom
-    (src/torch_admp/nblist.py:335-490).  One stream synchronisation returns P."""
+    (src/torch_admp/nblist.py:335-490).  One stream synchronisation returns P.
+    box_host: the box's values on the host when the caller has them (saves
+    the device-to-host copy), else None; the same holds for every operator."""
     _lib.require_gpu(positions)
     lib = _lib.load()
     pos = _f64(positions)
     n = pos.shape[0]
-    boxh = _lib.box_array(box)
+    boxh = _lib.box_array(box, box_host)
     nbytes = ctypes.c_size_t()
     _lib.check(lib.admp_nblist_workspace_bytes(n, boxh, float(cutoff), ctypes.byref(nbytes)),
                "admp_nblist_workspace_bytes")
@@ -84,7 +87,8 @@ def neighbor_pairs(positions: Tensor, box: Optional[Tensor], cutoff: float) -> T
 
 # ------------------------------------------------------------------ pair distances
 @torch.library.custom_op("admp::pair_distance", mutates_args=())
-def pair_distance(positions: Tensor, box: Optional[Tensor], pairs: Tensor) -> Tensor:
+def pair_distance(positions: Tensor, box: Optional[Tensor], pairs: Tensor,
+                  box_host: Optional[Tensor]) -> Tensor:
     """ds = |pbc_shift(r_j - r_i)| (src/torch_admp/nblist.py:517-551,
     src/torch_admp/spatial.py:15-36), |r_j - r_i| when box is None; (P,) in the
     positions' dtype."""
@@ -94,14 +98,15 @@ def pair_distance(positions: Tensor, box: Optional[Tensor], pairs: Tensor) -> Te
     prs = pairs.detach().to(torch.int64).contiguous()
     npairs = prs.shape[0]
     ds = torch.empty(npairs, dtype=_F64, device=pos.device)
-    _lib.check(lib.admp_pair_ds(_lib.ptr(pos), _lib.ptr(prs), npairs, _lib.box_array(box),
+    _lib.check(lib.admp_pair_ds(_lib.ptr(pos), _lib.ptr(prs), npairs, _lib.box_array(box, box_host),
                                 _lib.ptr(ds), None, _lib.stream_of(pos)), "admp_pair_ds")
     return ds.to(positions.dtype)
 
 
 @torch.library.custom_op("admp::pair_distance_backward", mutates_args=())
 def pair_distance_backward(grad_ds: Tensor, positions: Tensor, box: Optional[Tensor],
-                           pairs: Tensor, ds: Tensor) -> Tuple[Tensor, Tensor]:
+                           pairs: Tensor, ds: Tensor,
+                           box_host: Optional[Tensor]) -> Tuple[Tensor, Tensor]:
     """(dL/dpositions, dL/dbox) of L(ds); dL/dbox is zero when box is None.
     The displacement is recomputed from the positions."""
     _lib.require_gpu(grad_ds, positions, pairs)
@@ -111,7 +116,7 @@ def pair_distance_backward(grad_ds: Tensor, positions: Tensor, box: Optional[Ten
     g = _f64(grad_ds)
     gpos = torch.zeros_like(pos)
     gbox = torch.zeros(3, 3, dtype=_F64, device=pos.device)
-    boxh = _lib.box_array(box)
+    boxh = _lib.box_array(box, box_host)
     nbytes = ctypes.c_size_t()
     _lib.check(lib.admp_pair_ds_backward_workspace_bytes(prs.shape[0], ctypes.byref(nbytes)),
                "admp_pair_ds_backward_workspace_bytes")
@@ -127,18 +132,18 @@ def pair_distance_backward(grad_ds: Tensor, positions: Tensor, box: Optional[Ten
 
 
 def _pd_setup(ctx, inputs, output):
-    positions, box, pairs = inputs
-    ctx.save_for_backward(positions, box, pairs, output)
+    positions, box, pairs, box_host = inputs
+    ctx.save_for_backward(positions, box, pairs, output, box_host)
     ctx.set_materialize_grads(False)
 
 
 def _pd_backward(ctx, g_ds):
-    positions, box, pairs, ds = ctx.saved_tensors
+    positions, box, pairs, ds, box_host = ctx.saved_tensors
     if g_ds is None:
-        return None, None, None
-    gpos, gbox = torch.ops.admp.pair_distance_backward(g_ds, positions, box, pairs, ds)
+        return None, None, None, None
+    gpos, gbox = torch.ops.admp.pair_distance_backward(g_ds, positions, box, pairs, ds, box_host)
     return (gpos if ctx.needs_input_grad[0] else None,
-            gbox if box is not None and ctx.needs_input_grad[1] else None, None)
+            gbox if box is not None and ctx.needs_input_grad[1] else None, None, None)
 
 
 def _pdb_setup(ctx, inputs, output):
@@ -148,7 +153,7 @@ def _pdb_setup(ctx, inputs, output):
 def _pdb_backward(ctx, g_gpos, g_gbox):
     if g_gpos is not None or g_gbox is not None:
         raise NotImplementedError(_SECOND_ORDER.format("the pair distances"))
-    return (None,) * 5
+    return (None,) * 6
 
 
 torch.library.register_autograd("admp::pair_distance", _pd_backward, setup_context=_pd_setup)
@@ -293,8 +298,8 @@ def _pme_ws(lib, km, precision: int, n: int, dev) -> Tuple[Tensor, int]:
 
 @torch.library.custom_op("admp::pme_reciprocal", mutates_args=())
 def pme_reciprocal(positions: Tensor, box: Tensor, q: Tensor, kmesh: List[int], kappa: float,
-                   eps: float, precision: int, convention: int,
-                   need_potential: bool) -> Tuple[Tensor, Tensor, Tensor]:
+                   eps: float, precision: int, convention: int, need_potential: bool,
+                   box_host: Optional[Tensor]) -> Tuple[Tensor, Tensor, Tensor]:
     """One frame of the SPME reciprocal energy (src/torch_admp/pme.py:351-446):
     spread, r2c FFT, influence function, c2r FFT.  Returns (energy (0-dim, the
     positions' dtype), the potential mesh dE/dQ (empty unless
@@ -305,7 +310,7 @@ def pme_reciprocal(positions: Tensor, box: Tensor, q: Tensor, kmesh: List[int], 
     q64 = _f64(q).reshape(-1)
     n = pos.shape[0]
     dev = pos.device
-    boxh = _lib.box_array(box)
+    boxh = _lib.box_array(box, box_host)
     km = _lib.kmesh_array(kmesh)
     mesh_dtype = torch.float32 if precision == _lib.ADMP_F32 else _F64
     mesh = torch.empty(int(kmesh[0]) * int(kmesh[1]) * int(kmesh[2]), dtype=mesh_dtype,
@@ -327,8 +332,8 @@ def pme_reciprocal(positions: Tensor, box: Tensor, q: Tensor, kmesh: List[int], 
 @torch.library.custom_op("admp::pme_reciprocal_grads", mutates_args=())
 def pme_reciprocal_grads(positions: Tensor, box: Tensor, q: Tensor, potential: Tensor,
                          dEdA_k: Tensor, energy: Tensor, kmesh: List[int], kappa: float,
-                         eps: float, precision: int,
-                         convention: int) -> Tuple[Tensor, Tensor, Tensor]:
+                         eps: float, precision: int, convention: int,
+                         box_host: Optional[Tensor]) -> Tuple[Tensor, Tensor, Tensor]:
     """(dE/dpositions (n, 3), dE/dq (n,), dE/dbox (3, 3)) of pme_reciprocal
     from its potential mesh: the gather (the transpose of the spread) plus the
     k-vector and volume terms of dE/dbox."""
@@ -338,7 +343,7 @@ def pme_reciprocal_grads(positions: Tensor, box: Tensor, q: Tensor, potential: T
     q64 = _f64(q).reshape(-1)
     n = pos.shape[0]
     dev = pos.device
-    boxh = _lib.box_array(box)
+    boxh = _lib.box_array(box, box_host)
     km = _lib.kmesh_array(kmesh)
     gpos = torch.empty(n, 3, dtype=_F64, device=dev)
     gq = torch.empty(n, dtype=_F64, device=dev)
@@ -356,50 +361,50 @@ def pme_reciprocal_grads(positions: Tensor, box: Tensor, q: Tensor, potential: T
 
 
 def _pr_setup(ctx, inputs, output):
-    positions, box, q, kmesh, kappa, eps, precision, convention, need_pot = inputs
+    positions, box, q, kmesh, kappa, eps, precision, convention, need_pot, box_host = inputs
     energy, phi, dEdA_k = output
-    ctx.save_for_backward(positions, box, q, phi, dEdA_k, energy)
+    ctx.save_for_backward(positions, box, q, phi, dEdA_k, energy, box_host)
     ctx.args = (list(kmesh), kappa, eps, precision, convention, need_pot)
     ctx.set_materialize_grads(False)
 
 
 def _pr_backward(ctx, g_e, g_phi, g_dA):
-    positions, box, q, phi, dEdA_k, energy = ctx.saved_tensors
+    positions, box, q, phi, dEdA_k, energy, box_host = ctx.saved_tensors
     kmesh, kappa, eps, precision, convention, need_pot = ctx.args
     if g_e is None:
-        return (None,) * 9
+        return (None,) * 10
     if not need_pot:
         raise RuntimeError("pme_reciprocal was called with need_potential=False; its energy "
                            "cannot be differentiated")
     gpos, gq, gbox = torch.ops.admp.pme_reciprocal_grads(positions, box, q, phi, dEdA_k,
                                                          energy.detach(), kmesh, kappa, eps,
-                                                         precision, convention)
+                                                         precision, convention, box_host)
     need = ctx.needs_input_grad
     return (g_e * gpos if need[0] else None, g_e * gbox if need[1] else None,
-            g_e * gq if need[2] else None) + (None,) * 6
+            g_e * gq if need[2] else None) + (None,) * 7
 
 
-def _pme_eval_grads(positions, box, q, kmesh, kappa, eps, precision, convention):
+def _pme_eval_grads(positions, box, q, kmesh, kappa, eps, precision, convention, box_host):
     e, phi, dA = torch.ops.admp.pme_reciprocal(positions, box, q, kmesh, kappa, eps, precision,
-                                               convention, True)
+                                               convention, True, box_host)
     return torch.ops.admp.pme_reciprocal_grads(positions, box, q, phi, dA, e, kmesh, kappa, eps,
-                                               precision, convention)
+                                               precision, convention, box_host)
 
 
 def _prg_setup(ctx, inputs, output):
     positions, box, q = inputs[:3]
-    ctx.save_for_backward(positions, box, q)
-    ctx.args = tuple(inputs[6:])
+    ctx.save_for_backward(positions, box, q, inputs[11])
+    ctx.args = tuple(inputs[6:11])
     ctx.set_materialize_grads(False)
 
 
 def _prg_backward(ctx, g_gpos, g_gq, g_gbox):
     if g_gpos is not None or g_gbox is not None:
         raise NotImplementedError(_SECOND_ORDER.format("the reciprocal-space forces"))
-    nothing = (None,) * 11
+    nothing = (None,) * 12
     if g_gq is None:
         return nothing
-    positions, box, q = ctx.saved_tensors
+    positions, box, q, box_host = ctx.saved_tensors
     kmesh, kappa, eps, precision, convention = ctx.args
     kmesh = list(kmesh)
     need = ctx.needs_input_grad
@@ -407,15 +412,18 @@ def _prg_backward(ctx, g_gpos, g_gq, g_gbox):
     d_pos = d_box = d_q = None
     if need[2]:
         # E is quadratic in q: v^T (dE/dq) differentiates to the potential of charges v
-        d_q = _pme_eval_grads(positions, box, v, kmesh, kappa, eps, precision, convention)[1]
+        d_q = _pme_eval_grads(positions, box, v, kmesh, kappa, eps, precision, convention,
+                              box_host)[1]
     if need[0] or need[1]:
-        gp = _pme_eval_grads(positions, box, q + v, kmesh, kappa, eps, precision, convention)
-        gm = _pme_eval_grads(positions, box, q - v, kmesh, kappa, eps, precision, convention)
+        gp = _pme_eval_grads(positions, box, q + v, kmesh, kappa, eps, precision, convention,
+                             box_host)
+        gm = _pme_eval_grads(positions, box, q - v, kmesh, kappa, eps, precision, convention,
+                             box_host)
         if need[0]:
             d_pos = 0.5 * (gp[0] - gm[0])
         if need[1]:
             d_box = 0.5 * (gp[2] - gm[2])
-    return (d_pos, d_box, d_q) + (None,) * 8
+    return (d_pos, d_box, d_q) + (None,) * 9
 
 
 torch.library.register_autograd("admp::pme_reciprocal", _pr_backward, setup_context=_pr_setup)

@@ -33,11 +33,15 @@ class BaseForceModule(torch.nn.Module):
             _params[k] = v.reshape(nf, -1)
         lc = self.const_lib.length_coeff
         ec = self.const_lib.energy_coeff
-        box_in: Optional[torch.Tensor] = None
-        if _box is not None:
-            box_in = _box * lc
-        energy = self._forward_impl(_pos * lc, box_in, _pairs, _ds * lc, _bs, _params)
-        return energy / ec
+        if lc != 1.0:
+            _pos = _pos * lc
+            _ds = _ds * lc
+            if _box is not None:
+                _box = _box * lc
+        energy = self._forward_impl(_pos, _box, _pairs, _ds, _bs, _params)
+        if ec != 1.0:
+            energy = energy / ec
+        return energy
 
     def _forward_impl(self, positions: torch.Tensor, box: Optional[torch.Tensor],
                       pairs: torch.Tensor, ds: torch.Tensor, buffer_scales: torch.Tensor,

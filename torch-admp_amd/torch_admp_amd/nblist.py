@@ -30,7 +30,7 @@ def check_cutoff(box: torch.Tensor, cutoff: float) -> None:
 def build_pairs(positions: torch.Tensor, box: Optional[torch.Tensor], cutoff: float) -> torch.Tensor:
     """Ordered pair list (both orderings) from the HIP cell list.  One stream
     synchronisation returns the pair count."""
-    return torch.ops.admp.neighbor_pairs(positions, box, float(cutoff))
+    return torch.ops.admp.neighbor_pairs(positions, box, float(cutoff), None)
 
 
 class TorchNeighborList(torch.nn.Module):
@@ -45,25 +45,29 @@ class TorchNeighborList(torch.nn.Module):
         self.ds = torch.empty(1, dtype=torch.float64)
 
     def forward(self, positions: torch.Tensor, box: Optional[torch.Tensor] = None) -> torch.Tensor:
+        # one host copy of the box serves the cutoff check, the pair search and
+        # the distances (each copy of a device box waits for the stream)
+        box_host: Optional[torch.Tensor] = None
         if box is None:
             pairs = self.forward_obc(positions)
             pbc_flag = False
         else:
-            check_cutoff(box, self.cutoff)
-            pairs = self.forward_pbc(positions, box)
+            box_host = box.detach().to(torch.device("cpu"), torch.float64)
+            check_cutoff(box_host, self.cutoff)
+            pairs = torch.ops.admp.neighbor_pairs(positions, box, self.cutoff, box_host)
             pbc_flag = True
         self.pairs = pairs
         self.buffer_scales = self.pairs_buffer_scales(pairs)
-        self.ds = self.pairs_ds(positions, pairs, box, pbc_flag)
+        self.ds = self.pairs_ds(positions, pairs, box, pbc_flag, box_host)
         return pairs
 
     def forward_pbc(self, positions: torch.Tensor, box: torch.Tensor) -> torch.Tensor:
         """Periodic pairs with 0.01 < d < rc (src/torch_admp/nblist.py:335-445)."""
-        return torch.ops.admp.neighbor_pairs(positions, box, self.cutoff)
+        return torch.ops.admp.neighbor_pairs(positions, box, self.cutoff, None)
 
     def forward_obc(self, positions: torch.Tensor) -> torch.Tensor:
         """Open-boundary pairs with d < rc, i != j (src/torch_admp/nblist.py:472-490)."""
-        return torch.ops.admp.neighbor_pairs(positions, None, self.cutoff)
+        return torch.ops.admp.neighbor_pairs(positions, None, self.cutoff, None)
 
     @staticmethod
     def pairs_buffer_scales(pairs: torch.Tensor) -> torch.Tensor:
@@ -72,12 +76,13 @@ class TorchNeighborList(torch.nn.Module):
 
     @staticmethod
     def pairs_ds(positions: torch.Tensor, pairs: torch.Tensor, box: Optional[torch.Tensor] = None,
-                 pbc_flag: bool = True) -> torch.Tensor:
-        """Differentiable pair distances (src/torch_admp/nblist.py:517-551)."""
+                 pbc_flag: bool = True, box_host: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Differentiable pair distances (src/torch_admp/nblist.py:517-551);
+        box_host optionally carries the box's values on the host."""
         if pbc_flag:
             assert box is not None, "Box should be provided for periodic system."
-            return torch.ops.admp.pair_distance(positions, box, pairs)
-        return torch.ops.admp.pair_distance(positions, None, pairs)
+            return torch.ops.admp.pair_distance(positions, box, pairs, box_host)
+        return torch.ops.admp.pair_distance(positions, None, pairs, None)
 
     def set_pairs(self, pairs: torch.Tensor) -> None:
         self.pairs = pairs

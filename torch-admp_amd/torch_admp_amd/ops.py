@@ -28,7 +28,7 @@ def pair_distance(positions: torch.Tensor, box: Optional[torch.Tensor],
                   pairs: torch.Tensor) -> torch.Tensor:
     """ds = |pbc_shift(r_j - r_i)| (src/torch_admp/nblist.py:517-551); open
     boundaries when box is None.  Differentiable in positions and box."""
-    return torch.ops.admp.pair_distance(positions, box, pairs)
+    return torch.ops.admp.pair_distance(positions, box, pairs, None)
 
 
 def pair_energy(q: torch.Tensor, ds: torch.Tensor, pairs: torch.Tensor, bs: torch.Tensor,
@@ -49,7 +49,7 @@ def pme_reciprocal(positions: torch.Tensor, box: torch.Tensor, q: torch.Tensor,
                                         or q.requires_grad)
     return torch.ops.admp.pme_reciprocal(positions, box, q, [int(k) for k in kmesh],
                                          float(kappa), float(eps), int(precision),
-                                         int(convention), need)[0]
+                                         int(convention), need, None)[0]
 
 
 # Host state reused across fused calls: per problem shape the workspace size and

@@ -161,13 +161,15 @@ class CoulombForceModule(BaseForceModule):
         """One reciprocal launch sequence per frame (src/torch_admp/pme.py:351-446)."""
         out: List[torch.Tensor] = []
         for ii in range(positions.size(0)):
-            km = self.mesh_for_box(box[ii])
+            # one host copy of the frame's box for the mesh rule and the kernels
+            box_host = box[ii].detach().to(torch.device("cpu"), torch.float64)
+            km = self.mesh_for_box(box_host)
             need = torch.is_grad_enabled() and (positions.requires_grad or box.requires_grad
                                                 or charges.requires_grad)
             e, _, _ = torch.ops.admp.pme_reciprocal(positions[ii], box[ii], charges[ii], km,
                                                     self.kappa, self.const_lib.dielectric,
                                                     self._precision_code, self._convention_code,
-                                                    need)
+                                                    need, box_host)
             out.append(e)
             self._kmesh = torch.tensor(km, dtype=torch.long)
         return torch.stack(out)
