@@ -74,6 +74,15 @@ int admp_nblist_fill(const double* pos, int64_t n, const double* box, double cut
                      void* workspace, size_t ws_bytes, int64_t* pairs, int64_t capacity,
                      void* stream);
 
+/* Largest displacement max_i |pos_i - ref_i| (minimum image when box is
+ * given) into out[0] (device, float64), for a Verlet-skin neighbour list: a
+ * list built at cutoff + skin from ref holds every pair within cutoff while
+ * the largest displacement stays below skin / 2.  The reference rebuilds its
+ * list on every call (src/torch_admp/nblist.py:304-333); this has no
+ * counterpart there. */
+int admp_max_displacement(const double* pos, const double* ref, int64_t n, const double* box,
+                          double* out, void* stream);
+
 /* Pair distances |pbc_shift(r_i - r_j)| (src/torch_admp/nblist.py:517-551,
  * src/torch_admp/spatial.py:15-36); box == NULL gives |r_j - r_i|.  dr (may
  * be NULL) receives the minimum-image displacement (P, 3). */

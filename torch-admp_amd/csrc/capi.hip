@@ -310,6 +310,14 @@ int admp_nblist_fill(const double* pos, int64_t n, const double* box, double cut
                                  (hipStream_t)stream));
 }
 
+int admp_max_displacement(const double* pos, const double* ref, int64_t n, const double* box,
+                          double* out, void* stream) {
+  if (n < 0 || !out || (n > 0 && (!pos || !ref))) return ADMP_ERR_INVALID;
+  if (box && !box_ok(box)) return ADMP_ERR_INVALID;
+  const CellGrid cg = make_grid(box, 1.0, box == nullptr);
+  return hip_status(launch_max_disp(pos, ref, n, cg, out, (hipStream_t)stream));
+}
+
 int admp_pair_ds(const double* pos, const int64_t* pairs, int64_t n_pairs, const double* box,
                  double* ds, double* dr, void* stream) {
   if (n_pairs < 0 || (n_pairs > 0 && (!pos || !pairs || !ds))) return ADMP_ERR_INVALID;

@@ -666,6 +666,48 @@ __global__ void __launch_bounds__(kBlock) k_pack_sorted(int64_t n, const int* __
 
 
 // ---------------------------------------------------------------------------
+// Largest displacement |r_i - ref_i| (minimum image under PBC) for the Verlet
+// skin test.  Distances are non-negative, so their float64 bit patterns order
+// like unsigned integers: one 64-bit atomic max per wavefront into out (zeroed
+// by the launcher).
+__global__ void __launch_bounds__(kBlock) k_max_disp(const double* __restrict__ pos,
+                                                     const double* __restrict__ ref, int64_t n,
+                                                     CellGrid cg,
+                                                     unsigned long long* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  if (i < n) {
+    double v[3], nimg[3];
+    double x[3];  // x = pos_i - ref_i, then its minimum image
+    for (int k = 0; k < 3; ++k) x[k] = pos[3 * i + k] - ref[3 * i + k];
+    if (cg.obc) {
+      for (int k = 0; k < 3; ++k) v[k] = x[k];
+    } else {
+      double s[3];
+      for (int k = 0; k < 3; ++k) {
+        s[k] = x[0] * cg.a[k] + x[1] * cg.a[3 + k] + x[2] * cg.a[6 + k];
+        nimg[k] = floor(s[k] + 0.5);
+        s[k] -= nimg[k];
+      }
+      for (int k = 0; k < 3; ++k) v[k] = s[0] * cg.h[k] + s[1] * cg.h[3 + k] + s[2] * cg.h[6 + k];
+    }
+    d2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  }
+  double m = sqrt(d2);
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+hipError_t launch_max_disp(const double* pos, const double* ref, int64_t n, const CellGrid& cg,
+                           double* out, hipStream_t s) {
+  k_fill<unsigned long long><<<1, 64, 0, s>>>((unsigned long long*)out, 1, 0ull);
+  if (n > 0)
+    k_max_disp<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        pos, ref, n, cg, (unsigned long long*)out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
 
 void nb_layout(int64_t n, const CellGrid& cg, NbLayout* L) {

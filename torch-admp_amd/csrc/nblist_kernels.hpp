@@ -70,6 +70,9 @@ hipError_t launch_pairs(int mode, int64_t n, const CellGrid& cg, char* ws, const
                         int64_t* pairs, int64_t capacity, hipStream_t s);
 hipError_t launch_pair_ds(const double* pos, const int64_t* pairs, int64_t np,
                           const CellGrid& cg, double* ds, double* dr, hipStream_t s);
+// out[0] = max_i |pos_i - ref_i| (minimum image under PBC), on the stream
+hipError_t launch_max_disp(const double* pos, const double* ref, int64_t n, const CellGrid& cg,
+                           double* out, hipStream_t s);
 // workspace of launch_pair_ds_bwd (per-block partial sums of dL/dbox)
 size_t pair_ds_bwd_scratch_bytes(int64_t np);
 hipError_t launch_pair_ds_bwd(const double* gds, const int64_t* pairs, int64_t np,

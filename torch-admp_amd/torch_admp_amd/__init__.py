@@ -11,7 +11,7 @@ tensors or without the library raise.
 from . import _lib, base_force, electrode, nblist, ops, pme, qeq, serving, utils  # noqa: F401
 from .base_force import BaseForceModule  # noqa: F401
 from .electrode import PolarizableElectrode, infer, setup_from_lammps  # noqa: F401
-from .nblist import TorchNeighborList  # noqa: F401
+from .nblist import TorchNeighborList, VerletNeighborList  # noqa: F401
 from .pme import CoulombForceModule, setup_ewald_parameters  # noqa: F401
 from .qeq import QEqForceModule, matinv_optimize, pgrad_optimize  # noqa: F401
 from .utils import calc_grads, safe_inverse  # noqa: F401

@@ -43,6 +43,7 @@ _SIGNATURES = {
                                          _P(_c_i64), _c_vp]),
     "admp_nblist_fill": (ctypes.c_int, [_c_vp, _c_i64, _c_vp, _c_dbl, _c_vp, _c_sz, _c_vp,
                                         _c_i64, _c_vp]),
+    "admp_max_displacement": (ctypes.c_int, [_c_vp, _c_vp, _c_i64, _c_vp, _c_vp, _c_vp]),
     "admp_pair_ds": (ctypes.c_int, [_c_vp, _c_vp, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
     "admp_pair_ds_backward_workspace_bytes": (ctypes.c_int, [_c_i64, _P(_c_sz)]),
     "admp_pair_ds_backward": (ctypes.c_int, [_c_vp, _c_vp, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp,

@@ -85,6 +85,21 @@ def neighbor_pairs(positions: Tensor, box: Optional[Tensor], cutoff: float,
     return pairs
 
 
+@torch.library.custom_op("admp::max_displacement", mutates_args=())
+def max_displacement(positions: Tensor, reference: Tensor, box: Optional[Tensor],
+                     box_host: Optional[Tensor]) -> Tensor:
+    """max_i |positions_i - reference_i| (minimum image when box is given),
+    a 0-dim float64 device tensor."""
+    _lib.require_gpu(positions, reference)
+    lib = _lib.load()
+    pos, ref = _f64(positions), _f64(reference)
+    out = torch.empty((), dtype=_F64, device=pos.device)
+    _lib.check(lib.admp_max_displacement(_lib.ptr(pos), _lib.ptr(ref), pos.shape[0],
+                                         _lib.box_array(box, box_host), _lib.ptr(out),
+                                         _lib.stream_of(pos)), "admp_max_displacement")
+    return out
+
+
 # ------------------------------------------------------------------ pair distances
 @torch.library.custom_op("admp::pair_distance", mutates_args=())
 def pair_distance(positions: Tensor, box: Optional[Tensor], pairs: Tensor,

@@ -101,3 +101,70 @@ class TorchNeighborList(torch.nn.Module):
 
     def get_ds(self) -> torch.Tensor:
         return self.ds
+
+
+class VerletNeighborList(TorchNeighborList):
+    """Neighbour list with a Verlet skin, for MD steps and repeated requests:
+    the pair search runs at cutoff + skin and its list is kept while the
+    largest displacement since that search stays below skin / 2.
+
+    Same call surface as TorchNeighborList (src/torch_admp/nblist.py:268-617),
+    which rebuilds on every call (nblist.py:304-333).  Rows of the kept list
+    outside 0.01 < d < cutoff at the current positions get buffer scale 0, so
+    every force module gives the energies and forces of a fresh
+    TorchNeighborList(cutoff) (the pair set with scale 1 is the same); the
+    rows themselves are a superset (both orderings, grouped by first atom).
+    A search also runs when the atom count or the box changes, and every call
+    searches at cutoff alone when cutoff + skin exceeds half the smallest box
+    height.  ``builds`` and ``reuses`` count the two cases.
+    """
+
+    def __init__(self, cutoff: float, skin: float = 1.0) -> None:
+        super().__init__(cutoff)
+        if skin < 0.0:
+            raise ValueError(f"skin must be non-negative, got {skin}")
+        self.skin = float(skin)
+        self._ref: Optional[torch.Tensor] = None
+        self._ref_box: Optional[torch.Tensor] = None
+        self._list: Optional[torch.Tensor] = None
+        self._list_skin = 0.0
+        self.builds = 0
+        self.reuses = 0
+
+    def _needs_search(self, positions: torch.Tensor, box: Optional[torch.Tensor],
+                      box_host: Optional[torch.Tensor]) -> bool:
+        if self._list is None or self._ref is None or self._ref.shape != positions.shape:
+            return True
+        if (box is None) != (self._ref_box is None):
+            return True
+        if box_host is not None and not torch.equal(box_host, self._ref_box):
+            return True
+        disp = torch.ops.admp.max_displacement(positions, self._ref, box, box_host)
+        return float(disp) >= 0.5 * self._list_skin
+
+    def forward(self, positions: torch.Tensor, box: Optional[torch.Tensor] = None) -> torch.Tensor:
+        box_host = None if box is None else box.detach().to(torch.device("cpu"), torch.float64)
+        if box_host is not None:
+            check_cutoff(box_host, self.cutoff)
+        if self._needs_search(positions, box, box_host):
+            skin = self.skin
+            if box_host is not None and skin > 0.0:
+                try:
+                    check_cutoff(box_host, self.cutoff + skin)
+                except AssertionError:
+                    skin = 0.0
+            self._list = torch.ops.admp.neighbor_pairs(positions, box, self.cutoff + skin,
+                                                       box_host)
+            self._list_skin = skin
+            self._ref = positions.detach().clone()
+            self._ref_box = box_host
+            self.builds += 1
+        else:
+            self.reuses += 1
+        pairs = self._list
+        self.pairs = pairs
+        self.ds = self.pairs_ds(positions, pairs, box, box is not None, box_host)
+        d = self.ds.detach()
+        inside = (d < self.cutoff) & ((d > 0.01) if box is not None else (d >= 0.0))
+        self.buffer_scales = ((pairs[:, 0] < pairs[:, 1]) & inside).to(torch.long)
+        return pairs
