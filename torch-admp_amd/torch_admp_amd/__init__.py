@@ -8,7 +8,7 @@ The compute path is the HIP library in ``_lib/libadmp_amd.so``; calls on CPU
 tensors or without the library raise.
 """
 
-from . import _lib, base_force, electrode, nblist, ops, pme, qeq, serving, utils  # noqa: F401
+from . import _lib, base_force, electrode, lammps_io, nblist, ops, pme, qeq, serving, utils  # noqa: F401
 from .base_force import BaseForceModule  # noqa: F401
 from .electrode import PolarizableElectrode, infer, setup_from_lammps  # noqa: F401
 from .nblist import TorchNeighborList, VerletNeighborList  # noqa: F401
