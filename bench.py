@@ -289,12 +289,22 @@ def serving_bench(n_requests, torch, dev):
             srv.submit(r)
         srv.drain()
         res[prec] = reqs
-    e_rel = max(abs(a.energy[4] - b.energy[4]) / abs(b.energy[4])
-                for a, b in zip(res["float32"], res["float64"]))
-    rec_rel = max(abs(a.energy[1] - b.energy[1]) / abs(b.energy[1])
-                  for a, b in zip(res["float32"], res["float64"]))
-    f_rmse = max(float(np.sqrt(((a.forces - b.forces) ** 2).mean()))
-                 for a, b in zip(res["float32"], res["float64"]))
+    # Random +-1 ions make the total the small difference of large reciprocal and
+    # self terms, so besides the total-relative error the float32 error is given
+    # per term and against the energy scale |E_real| + |E_recip| + |E_self| +
+    # |E_nn|, with the condition number scale / |E_total| of the worst request.
+    pairs32 = list(zip(res["float32"], res["float64"]))
+    e_rel = max(abs(a.energy[4] - b.energy[4]) / abs(b.energy[4]) for a, b in pairs32)
+    rec_rel = max(abs(a.energy[1] - b.energy[1]) / abs(b.energy[1]) for a, b in pairs32)
+    scale = [float(np.abs(b.energy[:4]).sum()) for _, b in pairs32]
+    e_scale_rel = max(abs(a.energy[4] - b.energy[4]) / sc for (a, b), sc in zip(pairs32, scale))
+    real_scale_rel = max(abs(a.energy[0] - b.energy[0]) / sc
+                         for (a, b), sc in zip(pairs32, scale))
+    worst = max(range(len(pairs32)), key=lambda k: abs(pairs32[k][0].energy[4] -
+                                                       pairs32[k][1].energy[4]) /
+                abs(pairs32[k][1].energy[4]))
+    cond_worst = scale[worst] / abs(pairs32[worst][1].energy[4])
+    f_rmse = max(float(np.sqrt(((a.forces - b.forces) ** 2).mean())) for a, b in pairs32)
     # (5) PME and QEq requests in one stream
     from torch_admp_amd.workloads import qeq_species_box
 
@@ -337,9 +347,132 @@ def serving_bench(n_requests, torch, dev):
         "batched_fp64_vs_fused_single_fp64": round(out["fp64"]["eligible_capacity_req_s"] * t_fused, 2),
         "batched_fp32_vs_plain_fp64": round(out["fp32"]["eligible_capacity_req_s"] * t_plain, 2),
         "fp32_vs_fp64_energy_rel_max": float(e_rel),
+        "fp32_vs_fp64_energy_rel_max_condition": round(cond_worst, 1),
+        "fp32_vs_fp64_energy_scale_rel_max": float(e_scale_rel),
+        "fp32_vs_fp64_real_scale_rel_max": float(real_scale_rel),
         "fp32_vs_fp64_recip_energy_rel_max": float(rec_rel),
         "fp32_vs_fp64_force_rmse_max": float(f_rmse),
     })
+    return out
+
+
+def ionic_bench(torch, dev):
+    """1000 +-1 ions (net zero) in a 20 A cube, rc 6 A, ethresh 1e-5 (kappa
+    0.548224 1/A), order 6, at the reference rule's mesh 74^3 (= 2 x 37: the
+    generic prime FFT pass) and at an explicit 32^3: fused E+F+dE/dbox
+    evaluations per second in float64 and float32, and their error against
+    the float64 oracle."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import nblist_ref, pme_ref
+    from torch_admp_amd import ops
+    from torch_admp_amd.workloads import ionic_box
+
+    pos_np, q_np, box_np = ionic_box(1000, 20.0, seed=0)
+    rc, eth = 6.0, 1e-5
+    kappa = pme_ref.kappa_from_ethresh(rc, eth)
+    pos, q = torch.tensor(pos_np, device=dev), torch.tensor(q_np, device=dev)
+    box = torch.tensor(box_np)
+    p_c, q_c, b_c = (torch.tensor(a) for a in (pos_np, q_np, box_np))
+    pairs = nblist_ref.nblist_pbc(p_c, b_c, rc)
+    bs = nblist_ref.buffer_scales(pairs)
+    out = {"workload": "1000 +-1 ions in a 20 A cube, rc 6 A, ethresh 1e-5, order 6, "
+                       "E+F+dE/dbox (fused path)", "kappa": kappa}
+    for km in (74, 32):
+        orc = pme_ref.CoulombOracle(rc, eth, kmesh=None if km == 74 else km)
+        e_ref, f_ref, _ = pme_ref.energy_forces(
+            p_c, b_c, q_c, pairs, lambda p, b: nblist_ref.pairs_ds(p, pairs, b), bs, orc)
+        leg = {}
+        for prec in ("float64", "float32"):
+            sec, (e, f, _) = timed(lambda prec=prec: ops.pme_fused(pos, box, q, rc, kappa,
+                                                                   [km] * 3, prec),
+                                   50, 10, torch)
+            leg[prec] = {"evals_per_s": round(1.0 / sec, 1),
+                         "energy_rel_vs_oracle": float(abs(float(e[4]) - float(e_ref)) /
+                                                       abs(float(e_ref))),
+                         "force_rmse_vs_oracle": float(((f.cpu() - f_ref) ** 2).mean().sqrt())}
+        out[f"kmesh_{km}"] = leg
+    return out
+
+
+def cpu_baselines(torch, qeq_iters, electrode_iters, n_requests):
+    """The float64 oracle on the host cores, on bounded samples of the QEq,
+    electrode and serving workloads (the dense KKT solves the oracle would use
+    cost thousands of such evaluations, so the iterative solves are priced as
+    per-iteration cost x the GPU solve's iteration count)."""
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT))
+    from oracle import nblist_ref, pme_ref, qeq_ref
+    from torch_admp_amd import serving
+    from torch_admp_amd.workloads import electrode_cell, qeq_species_box
+
+    cores = min(16, os.cpu_count() or 1)
+    torch.set_num_threads(cores)
+    out = {"cores": cores, "kind": "port"}
+
+    def energy_grad_time(orc, pos, box, chi, hard, eta, rc, reps):
+        p, b = torch.tensor(pos), torch.tensor(box)
+        pairs = nblist_ref.nblist_pbc_cells(p, b, rc)
+        ds = nblist_ref.pairs_ds(p, pairs, b)
+        bs = nblist_ref.buffer_scales(pairs)
+        args = [torch.tensor(a) for a in (chi, hard, eta)]
+        rng = np.random.default_rng(0)
+
+        def once():
+            qq = torch.tensor(rng.normal(size=len(pos)) * 0.1, requires_grad=True)
+            e = orc.func_energy(qq, p, b, args[0], args[1], args[2], pairs, ds, bs)
+            return torch.autograd.grad(e.sum(), qq)[0]
+
+        once()
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            once()
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times))
+
+    pos, box, chi, hard, eta = qeq_species_box(3000, 30.0, seed=3)
+    t_it = energy_grad_time(qeq_ref.QEqOracle(6.0, 1e-5, kmesh=[120] * 3), pos, box, chi, hard,
+                            eta, 6.0, 3)
+    out["qeq_3000"] = {"value": round(1.0 / (t_it * qeq_iters), 5), "unit": "solves/s",
+                       "sample": f"median of 3 oracle energy + dE/dq evaluations (one projected "
+                                 f"CG iteration, {t_it:.2f} s) x {qeq_iters} iterations"}
+    pos, q, box = electrode_cell(seed=0)
+    eta = np.zeros(len(pos))
+    eta[:1600] = 0.5
+    zeros = np.zeros(len(pos))
+    t_it = energy_grad_time(qeq_ref.QEqOracle(6.0, 1e-5, kmesh=[150, 150, 450], slab_corr=True),
+                            pos, box, zeros, zeros, eta, 6.0, 2)
+    steps = electrode_iters + 2
+    out["electrode_3600"] = {"value": round(1.0 / (t_it * steps), 5), "unit": "steps/s",
+                             "sample": f"median of 2 oracle energy + dE/dq evaluations "
+                                       f"({t_it:.2f} s) x ({electrode_iters} CG iterations "
+                                       "+ potential + forces)"}
+    reqs = serving.synthetic_stream(n_requests, 1.0, seed=7)[::max(1, n_requests // 16)][:16]
+    lat = []
+    for r in reqs:
+        p, b, c = (torch.tensor(a) for a in (r.positions, r.box, r.charges))
+        t0 = time.perf_counter()
+        vol = float(torch.linalg.det(b).abs())
+        h = min(vol / float(torch.linalg.cross(b[i], b[j]).norm())
+                for i, j in ((1, 2), (2, 0), (0, 1)))
+        orc = pme_ref.CoulombOracle(r.rcut, r.ethresh, kmesh=r.kmesh, kappa=r.kappa)
+        if h < 2 * r.rcut:
+            pairs, _, bs = nblist_ref.multi_image_pairs(p, b, r.rcut)
+            ds_fn = lambda pp, bb, rc=r.rcut: nblist_ref.multi_image_pairs(pp, bb, rc)[1]  # noqa: E731
+        else:
+            pairs = nblist_ref.nblist_pbc_cells(p, b, r.rcut)
+            bs = nblist_ref.buffer_scales(pairs)
+            ds_fn = lambda pp, bb, pr=pairs: nblist_ref.pairs_ds(pp, pr, bb)  # noqa: E731
+        pme_ref.energy_forces(p, b, c, pairs, ds_fn, bs, orc)
+        lat.append(time.perf_counter() - t0)
+    lat = np.array(lat)
+    out["serving_stream"] = {"value": round(len(lat) / lat.sum(), 3), "unit": "requests/s",
+                             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
+                             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),
+                             "sample": f"{len(lat)} requests of the stream (every "
+                                       f"{max(1, n_requests // 16)}th), one at a time: neighbour "
+                                       "list, E+F+dE/dbox by the oracle"}
     return out
 
 
@@ -484,9 +617,45 @@ def main():
             e = mod64(p, box, pairs, nbl.get_ds(), nbl.get_buffer_scales(), {"charge": q})
             return e, -calc_grads(e, p)
 
-        secd, (ed, fd) = timed(dropin, 3, 1, torch)
+        secd, (ed, fd) = timed(dropin, 10, 2, torch)
         extras["dropin_module_fp64_evals_per_s"] = round(1.0 / secd, 3)
         extras["dropin_vs_fused_fp64_energy_rel"] = float(abs(ed.detach()[0] - e64) / abs(e64))
+
+        # MD-like steps (random moves of 0.01 A per atom and step): the pair list
+        # searched at rc + 1 A is kept while displacements stay below 0.5 A
+        from torch_admp_amd.nblist import VerletNeighborList
+
+        vnl = VerletNeighborList(rc, 1.0)
+        gen = torch.Generator(device=dev).manual_seed(3)
+        cur = [pos.clone()]
+
+        def verlet_step():
+            cur[0] = cur[0] + 0.01 * torch.randn(cur[0].shape, device=dev, dtype=cur[0].dtype,
+                                                 generator=gen)
+            p = cur[0].clone().requires_grad_(True)
+            pairs = vnl(p, box)
+            e = mod64(p, box, pairs, vnl.get_ds(), vnl.get_buffer_scales(), {"charge": q})
+            return e, -calc_grads(e, p)
+
+        secv, _ = timed(verlet_step, 20, 2, torch)
+        extras["dropin_verlet_fp64_evals_per_s"] = round(1.0 / secv, 3)
+        extras["dropin_verlet_builds_reuses"] = [vnl.builds, vnl.reuses]
+
+        # the fused evaluation in the library's static scratch (no workspace)
+        km_arr = (ctypes.c_int32 * 3)(kmesh, kmesh, kmesh)
+        out_na = torch.empty(14 + 3 * n, dtype=torch.float64, device=dev)
+
+        def noalloc():
+            _lib.check(lib.admp_pme_fused_noalloc(
+                _lib.ptr(pos), _lib.ptr(q), n, _lib.box_array(box_h), float(rc),
+                float(mod32.kappa), km_arr, _lib.ADMP_F32, 0, 14.399645468667817,
+                _lib.ptr(out_na), _lib.ptr(out_na[14:]), _lib.ptr(out_na[5:14]),
+                _lib.stream_of(pos)), "admp_pme_fused_noalloc")
+
+        secn, _ = timed(noalloc, args.steps, args.warmup // 4, torch)
+        extras["noalloc_fp32_evals_per_s"] = round(1.0 / secn, 2)
+        extras["noalloc_vs_fused_energy_rel"] = float(abs(out_na[4] - e32) / abs(e32))
+        extras["ionic_1000"] = ionic_bench(torch, dev)
 
     if args.serving_requests > 0:
         extras["serving"] = serving_bench(args.serving_requests, torch, dev)
@@ -496,6 +665,11 @@ def main():
     if not args.no_qeq:
         extras["qeq_3000"] = qeq_bench(torch, dev)
         extras["electrode_3600"] = electrode_bench(torch, dev)
+        if not args.no_cpu_baseline:
+            extras["cpu_baselines_other_workloads"] = cpu_baselines(
+                torch, extras["qeq_3000"]["float64"]["cg_iterations"],
+                extras["electrode_3600"]["float64"]["cg_iterations"],
+                max(args.serving_requests, 16))
 
     err32_vs64 = {"energy_rel": float(abs(e32 - e64) / abs(e64)),
                   "force_rmse_eV_per_A": float(((f32 - f64) ** 2).mean().sqrt()),

@@ -156,13 +156,19 @@ __device__ void pass_generic(const cplx<T>* __restrict__ a, cplx<T>* __restrict_
     const int i = rest % nb;
     const int qq = rest / nb;
     const int k = i % p;
-    CT s = cmk<CT>(0, 0);
-    for (int j = 0; j < r; ++j) {
+    // twiddle of term j: w_L^(k j tstep) w_r^(j q) = w_L^(j step) with
+    // step = (k tstep + q rstep) mod L, advanced by one addition and one
+    // conditional subtraction per term (k tstep < L and q rstep < L)
+    int step = k * tstep + qq * rstep;
+    if (step >= L) step -= L;
+    CT s = a[i * C + c];
+    int t = step;
+    for (int j = 1; j < r; ++j) {
       CT v = a[(i + j * nb) * C + c];
-      // twiddle w_L^(k j tstep) times w_r^(j q) = w_L^(k j tstep + j q rstep)
-      const long long t = ((long long)k * j * tstep + (long long)j * qq * rstep) % L;
-      v = cmul(v, tw_get<T>(tw, (int)t, SIGN));
+      v = cmul(v, tw_get<T>(tw, t, SIGN));
       s = cadd(s, v);
+      t += step;
+      if (t >= L) t -= L;
     }
     b[((i - k) * r + k + qq * p) * C + c] = s;
   }
