@@ -100,6 +100,26 @@ def test_batch_mixed_vs_oracle(dev, precision, etol, ftol):
         assert np.abs(gb - gb_ref).max() <= 100 * etol * np.abs(gb_ref).max(), i
 
 
+@pytest.mark.parametrize("precision,etol,ftol", [("float64", 1e-10, 1e-8),
+                                                 ("float32", 1e-6, 1e-4)])
+def test_batch_prime_radix_meshes_vs_oracle(dev, precision, etol, ftol):
+    """Meshes whose line lengths take the register passes of the odd primes
+    7, 11 and 13 (13, 22 = 2 x 11, 35 = 5 x 7, 77 = 7 x 11, 39 = 3 x 13) and
+    the generic pass (17, 74 = 2 x 37), against the oracle."""
+    systems = [_system(300, np.eye(3) * 15.0, 41) + ([13, 22, 35], 0.42),
+               _system(500, TRICLINIC, 42) + ([77, 39, 17], 0.45),
+               _system(400, np.diag([14.0, 16.0, 18.0]), 43) + ([74, 26, 33], 0.40)]
+    energy, forces, dedbox = _run(systems, dev, precision)
+    for i, (pos, q, box, km, kap) in enumerate(systems):
+        e_ref, f_ref, gb_ref = _oracle(pos, q, box, km, kap)
+        e = energy[i].cpu().numpy()
+        assert abs(e[4] - e_ref) <= etol * abs(e_ref), (i, e[4], e_ref)
+        rmse = float(np.sqrt(((forces[i].cpu().numpy() - f_ref) ** 2).mean()))
+        assert rmse < ftol, (i, rmse)
+        gb = dedbox[i].cpu().numpy()
+        assert np.abs(gb - gb_ref).max() <= 100 * etol * np.abs(gb_ref).max(), i
+
+
 @pytest.mark.parametrize("precision", ["float64", "float32"])
 def test_batch_equals_fused(dev, precision):
     systems = [s for s in _mix(11) if len(s[1])]
