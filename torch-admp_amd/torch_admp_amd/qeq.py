@@ -296,7 +296,12 @@ class QEqForceModule(BaseForceModule):
                     reinit_q: bool = False, method: str = "lbfgs"):
         """Projected-gradient solve (src/torch_admp/qeq.py:609-712) by the HIP
         projected CG; returns (energy, q_opt) with q_opt detached, energy
-        differentiable in positions at fixed charges."""
+        differentiable in positions at fixed charges.
+
+        Afterwards ``self.converge_iter`` holds the number of projected-CG
+        iterations the solve took (tens, one Hessian product each), not the
+        reference's count of LBFGS / line-search outer steps (at most
+        ``max_iter``); ``self.residual`` holds the final |P grad E| / n."""
         if method not in ("lbfgs", "quadratic"):
             raise ValueError(f"Method {method} is not supported.")
         n = positions.reshape(-1, 3).shape[0]
