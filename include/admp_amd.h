@@ -69,10 +69,18 @@ int admp_nblist_workspace_bytes(int64_t n, const double* box, double cutoff, siz
 int admp_nblist_count(const double* pos, int64_t n, const double* box, double cutoff,
                       void* workspace, size_t ws_bytes, int64_t* n_pairs, void* stream);
 
-/* Writes the pairs counted by admp_nblist_count; capacity is in pairs. */
+/* Writes the pairs counted by admp_nblist_count; capacity is in pairs.  Rows
+ * are grouped by first atom in ascending atom index.  Optional outputs (each
+ * may be NULL): row_ptr (int64, n + 1) receives the row starts (row i =
+ * pairs [row_ptr[i], row_ptr[i + 1])); reverse (int32, capacity) receives for
+ * each pair (i, j) the index of (j, i), -1 when the list cannot be indexed
+ * (n >= 2^24 atoms or P >= 2^31), and then needs scratch (int32, capacity).
+ * row_ptr and reverse let admp_pair_ds_backward run without per-pair
+ * atomics.  No reference counterpart: TorchNeighborList returns the pairs
+ * alone (src/torch_admp/nblist.py:304-333). */
 int admp_nblist_fill(const double* pos, int64_t n, const double* box, double cutoff,
                      void* workspace, size_t ws_bytes, int64_t* pairs, int64_t capacity,
-                     void* stream);
+                     int64_t* row_ptr, int32_t* reverse, int32_t* scratch, void* stream);
 
 /* Largest displacement max_i |pos_i - ref_i| (minimum image when box is
  * given) into out[0] (device, float64), for a Verlet-skin neighbour list: a
@@ -92,14 +100,18 @@ int admp_pair_ds(const double* pos, const int64_t* pairs, int64_t n_pairs, const
 /* Backward of admp_pair_ds: grad_pos (n, 3) += dL/dds * d ds / d r;
  * grad_box (9) += dL/dds * d ds / d box (may be NULL).  grad_pos and grad_box
  * must be zeroed by the caller.  dr may be NULL (the displacement is then
- * recomputed from pos).  Runs of equal first atoms (lists grouped by i) are
- * summed in registers before the float64 atomics; the box term goes through
+ * recomputed from pos).  With row_ptr and reverse from admp_nblist_fill (both
+ * or neither; n_atoms = rows) each row sums its pairs' first-atom terms and
+ * their reverses' second-atom terms, one float64 atomic per row component;
+ * otherwise runs of equal first atoms are summed in registers before the
+ * atomics and the second atom adds per pair.  The box term goes through
  * per-work-group partials in the workspace (admp_pair_ds_backward_workspace_bytes,
  * needed when grad_box and box are given) and a fixed-order reduction. */
 int admp_pair_ds_backward_workspace_bytes(int64_t n_pairs, size_t* bytes);
 int admp_pair_ds_backward(const double* grad_ds, const int64_t* pairs, int64_t n_pairs,
                           const double* pos, const double* box, const double* dr,
                           const double* ds, double* grad_pos, double* grad_box,
+                          const int64_t* row_ptr, const int32_t* reverse, int64_t n_atoms,
                           void* workspace, size_t ws_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
@@ -155,6 +167,14 @@ int admp_pme_gather(const double* pos, const double* q, int64_t n, const double*
                     const int32_t* kmesh, int32_t precision, const void* phi,
                     double* grad_pos, double* dEdq, double* dEdA, int32_t accumulate,
                     void* workspace, size_t ws_bytes, void* stream);
+
+/* dE/dbox of the reciprocal energy (the autograd box derivative through
+ * recip.py:255-308 and the volume in pme.py:410-443): with A = box^-1,
+ *   grad_box (9) = -A^T (dEdA_k + dEdA_gather) A^T - energy[0] A^T,
+ * dEdA_k and energy from admp_pme_convolve, dEdA_gather (may be NULL) from
+ * admp_pme_gather.  Device inputs and output; no synchronisation. */
+int admp_pme_box_grad(const double* box, const double* dEdA_k, const double* dEdA_gather,
+                      const double* energy, double* grad_box, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Fused Ewald/PME energy + forces for one periodic frame straight from

@@ -119,6 +119,58 @@ def test_pair_ds_and_backward(dev, box, order):
     assert torch.allclose(gb.cpu(), gb_ref, rtol=0, atol=1e-9)
 
 
+@pytest.mark.parametrize("box,rc,n", [(np.eye(3) * 20.0, 6.0, 3000), (BIG_TRICLINIC_NB, 6.0, 3000),
+                                      (np.eye(3) * 13.0, 5.0, 200), (TRICLINIC, 5.0, 300),
+                                      (None, 4.0, 400)])
+def test_row_form_and_reverse_index(dev, box, rc, n):
+    """The search's row form: rows by first atom in ascending index with
+    row_ptr their starts, and reverse[o] the index of (j, i) for o = (i, j)
+    (cell grids of 3+ cells per side, one cell with images, triclinic, open
+    boundaries).  The distance backward through it (no per-pair atomics)
+    against the oracle's autograd, with the buffer-scale zeros and random
+    zeros in the upstream gradient; then with part of the reverse index
+    removed (pairs without a reverse add their second-atom term by atomics)."""
+    pos, _, boxt = _system(n, np.eye(3) * 12.0 if box is None else box, 31)
+    if box is None:
+        boxt = None
+    nbl = TorchNeighborList(rc)
+    pairs = nbl(pos.to(dev), None if boxt is None else boxt.to(dev))
+    row_ptr, rev = nbl.row_ptr.cpu(), nbl.reverse.cpu().long()
+    pc = pairs.cpu()
+    assert bool((pc[1:, 0] >= pc[:-1, 0]).all())
+    counts = torch.bincount(pc[:, 0], minlength=n)
+    assert torch.equal(row_ptr, torch.cat([torch.zeros(1, dtype=torch.long), counts.cumsum(0)]))
+    assert bool((rev >= 0).all())
+    assert torch.equal(pc[rev], pc[:, [1, 0]])
+    assert torch.equal(rev[rev], torch.arange(pc.shape[0]))
+    # backward through the row form against the oracle
+    rng = np.random.default_rng(5)
+    p = pos.clone().requires_grad_(True)
+    b = None if boxt is None else boxt.clone().requires_grad_(True)
+    ds_ref = nblist_ref.pairs_ds(p, pc, b) if b is not None else (p[pc[:, 1]] - p[pc[:, 0]]).norm(dim=1)
+    bs = (pc[:, 0] < pc[:, 1]).to(F64)
+    w = torch.tensor(rng.normal(size=pc.shape[0]) * (rng.random(pc.shape[0]) < 0.8)) * bs
+    wrt = [p] if b is None else [p, b]
+    ref = torch.autograd.grad((ds_ref * w).sum(), wrt)
+    pg = pos.to(dev).requires_grad_(True)
+    bg = None if boxt is None else boxt.to(dev).requires_grad_(True)
+    pairs2 = nbl(pg, bg)
+    ds = nbl.get_ds()
+    assert torch.equal(pairs2.cpu(), pc)
+    got = torch.autograd.grad((ds * w.to(dev)).sum(), [pg] if bg is None else [pg, bg])
+    for g, r in zip(got, ref):
+        assert torch.allclose(g.cpu(), r, rtol=0, atol=1e-10 * max(1.0, float(r.abs().max())))
+    # the same with a third of the reverse index removed (both pairs of each
+    # removed couple), through the operator
+    cut = torch.tensor(rng.random(pc.shape[0]) < 0.33)
+    cut = cut | cut[rev]
+    rev_part = torch.where(cut, torch.full_like(rev, -1), rev).to(torch.int32).to(dev)
+    pg2 = pos.to(dev).requires_grad_(True)
+    ds2 = torch.ops.admp.pair_distance(pg2, bg, pairs2, None, nbl.row_ptr, rev_part)
+    got2 = torch.autograd.grad((ds2 * w.to(dev)).sum(), [pg2])[0]
+    assert torch.allclose(got2.cpu(), ref[0], rtol=0, atol=1e-10 * max(1.0, float(ref[0].abs().max())))
+
+
 def test_real_space_pair_energy(dev):
     pos, q, box = _system(300, np.eye(3) * 14.0, 5)
     rc, kappa = 6.0, 0.4

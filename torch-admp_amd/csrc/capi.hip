@@ -280,7 +280,7 @@ int admp_nblist_count(const double* pos, int64_t n, const double* box, double cu
   char* ws = (char*)workspace;
   hipError_t e = launch_bin(pos, n, cg, ws, L, s);
   if (e != hipSuccess) return ADMP_ERR_HIP;
-  e = launch_pairs(0, n, cg, ws, L, nullptr, 0, s);
+  e = launch_pairs(0, n, cg, ws, L, nullptr, 0, nullptr, s);
   if (e != hipSuccess) return ADMP_ERR_HIP;
   int64_t total = 0;
   if (n > 0) {
@@ -296,18 +296,24 @@ int admp_nblist_count(const double* pos, int64_t n, const double* box, double cu
 
 int admp_nblist_fill(const double* pos, int64_t n, const double* box, double cutoff,
                      void* workspace, size_t ws_bytes, int64_t* pairs, int64_t capacity,
-                     void* stream) {
+                     int64_t* row_ptr, int32_t* reverse, int32_t* scratch, void* stream) {
   if (n < 0 || !(cutoff > 0.0) || capacity < 0 || (n > 0 && (!pos || !workspace)))
     return ADMP_ERR_INVALID;
   if (capacity > 0 && !pairs) return ADMP_ERR_INVALID;
+  if (reverse && capacity > 0 && !scratch) return ADMP_ERR_INVALID;
   if (box && !box_ok(box)) return ADMP_ERR_INVALID;
   const CellGrid cg = make_grid(box, cutoff, box == nullptr);
   NbLayout L;
   nb_layout(n, cg, &L);
   if (ws_bytes < L.bytes) return ADMP_ERR_INVALID;
   (void)pos;
-  return hip_status(launch_pairs(1, n, cg, (char*)workspace, L, pairs, capacity,
-                                 (hipStream_t)stream));
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  int* keys = reverse ? (int*)scratch : nullptr;
+  hipError_t e = launch_pairs(1, n, cg, ws, L, pairs, capacity, keys, s);
+  if (e == hipSuccess && (row_ptr || reverse) && n > 0)
+    e = launch_nb_reverse(n, cg, ws, L, keys, capacity, row_ptr, (int*)reverse, s);
+  return hip_status(e);
 }
 
 int admp_max_displacement(const double* pos, const double* ref, int64_t n, const double* box,
@@ -335,8 +341,11 @@ int admp_pair_ds_backward_workspace_bytes(int64_t n_pairs, size_t* bytes) {
 int admp_pair_ds_backward(const double* grad_ds, const int64_t* pairs, int64_t n_pairs,
                           const double* pos, const double* box, const double* dr,
                           const double* ds, double* grad_pos, double* grad_box,
+                          const int64_t* row_ptr, const int32_t* reverse, int64_t n_atoms,
                           void* workspace, size_t ws_bytes, void* stream) {
   if (n_pairs < 0 || (n_pairs > 0 && (!grad_ds || !pairs || !pos || !ds || !grad_pos)))
+    return ADMP_ERR_INVALID;
+  if ((row_ptr != nullptr) != (reverse != nullptr) || (row_ptr && n_atoms < 0))
     return ADMP_ERR_INVALID;
   if (box && !box_ok(box)) return ADMP_ERR_INVALID;
   if (grad_box && box && n_pairs > 0 &&
@@ -344,7 +353,8 @@ int admp_pair_ds_backward(const double* grad_ds, const int64_t* pairs, int64_t n
     return ADMP_ERR_INVALID;
   const CellGrid cg = make_grid(box, 1.0, box == nullptr);
   return hip_status(launch_pair_ds_bwd(grad_ds, pairs, n_pairs, pos, cg, dr, ds, grad_pos,
-                                       grad_box, (double*)workspace, (hipStream_t)stream));
+                                       grad_box, (double*)workspace, row_ptr,
+                                       (const int*)reverse, n_atoms, (hipStream_t)stream));
 }
 
 // ------------------------------------------------------------- pair energy
@@ -452,6 +462,13 @@ int admp_pme_gather(const double* pos, const double* q, int64_t n, const double*
                                            accumulate, partials, s));
   return hip_status(launch_gather<double>(pos, q, n, g, (const double*)phi, grad_pos, dEdq, dEdA,
                                           accumulate, partials, s));
+}
+
+int admp_pme_box_grad(const double* box, const double* dEdA_k, const double* dEdA_gather,
+                      const double* energy, double* grad_box, void* stream) {
+  if (!box_ok(box) || !dEdA_k || !energy || !grad_box) return ADMP_ERR_INVALID;
+  return hip_status(launch_box_grad(make_geom(box, nullptr), dEdA_k, dEdA_gather, energy,
+                                    grad_box, (hipStream_t)stream));
 }
 
 // --------------------------------------------------------------------- QEq

@@ -142,9 +142,12 @@ __global__ void __launch_bounds__(kBlock) k_nb_rank(int64_t n, const int* __rest
 // compacted by ballot.  The distance test compares squared distances and takes
 // the square root only within 1e-12 (relative) of rc^2 or 1e-4, where it
 // decides exactly as sqrt(d2) < rc and d > 0.01 do.  Both passes enumerate the
-// pairs in the same order (stencil entry, candidate chunk, lane), so pass 1
-// writes each home atom's pairs at the offset pass 0 counted, grouped by home
-// atom in cell order.
+// pairs in the same order (stencil entry, then the candidate's cell slot), so
+// pass 1 writes each home atom's pairs at the offset pass 0 counted: rows
+// grouped by first atom in ascending atom index, each row in ascending
+// (stencil entry, slot).  With `keys`, pass 1 also stores that order as
+// keys[o] = entry << 24 | slot of the second atom, which locates the reverse
+// pair (k_nb_reverse).
 constexpr int kNbThreads = 512;
 constexpr int kNbCand = 2048;     // staged candidates per chunk
 constexpr int kNbHomeBlk = 256;   // home atoms whose counters sit in LDS at once
@@ -167,9 +170,11 @@ template <int MODE>
 __global__ void __launch_bounds__(kNbThreads) k_nb_cell_pairs(
     CellGrid cg, int S, const double* __restrict__ w, const int* __restrict__ start,
     const int* __restrict__ sorted, int64_t* __restrict__ pcount,
-    const int64_t* __restrict__ offsets, int64_t* __restrict__ pairs, int64_t capacity) {
+    const int64_t* __restrict__ offsets, int64_t* __restrict__ pairs, int64_t capacity,
+    int* __restrict__ keys) {
   __shared__ double cx[kNbCand], cy[kNbCand], cz[kNbCand];
   __shared__ int cj[kNbCand];
+  __shared__ int ckey[MODE == 1 ? kNbCand : 1];
   __shared__ int64_t hcnt[kNbHomeBlk];
   __shared__ int st_pre[kMaxStencil + 1];
   __shared__ int st_beg[kMaxStencil];
@@ -202,7 +207,8 @@ __global__ void __launch_bounds__(kNbThreads) k_nb_cell_pairs(
 
   for (int h0 = hb; h0 < he; h0 += kNbHomeBlk) {
     const int nh = min(kNbHomeBlk, he - h0);
-    for (int t = threadIdx.x; t < nh; t += kNbThreads) hcnt[t] = MODE == 1 ? offsets[h0 + t] : 0;
+    for (int t = threadIdx.x; t < nh; t += kNbThreads)
+      hcnt[t] = MODE == 1 ? offsets[sorted[h0 + t]] : 0;
     for (int st0 = 0; st0 < nst; st0 += kMaxStencil) {
       __syncthreads();
       const int m = load_stencil_chunk(cg, 0, ci, st0, nst, start, st_pre, st_beg, st_sh);
@@ -215,7 +221,7 @@ __global__ void __launch_bounds__(kNbThreads) k_nb_cell_pairs(
           const int kg = r0 + threadIdx.x;
           bool keep = false;
           double px = 0.0, py = 0.0, pz = 0.0;
-          int j = 0;
+          int j = 0, key = 0;
           if (kg < min(ctot, k0 + kNbCand)) {
             int lo = 0, hi = m;
             while (hi - lo > 1) {
@@ -223,7 +229,9 @@ __global__ void __launch_bounds__(kNbThreads) k_nb_cell_pairs(
               if (st_pre[mid] <= kg) lo = mid;
               else hi = mid;
             }
-            j = sorted[st_beg[lo] + (kg - st_pre[lo])];
+            const int slot = st_beg[lo] + (kg - st_pre[lo]);
+            j = sorted[slot];
+            key = ((st0 + lo) << 24) | slot;
             px = w[3 * j] + st_sh[lo * 3];
             py = w[3 * j + 1] + st_sh[lo * 3 + 1];
             pz = w[3 * j + 2] + st_sh[lo * 3 + 2];
@@ -250,6 +258,7 @@ __global__ void __launch_bounds__(kNbThreads) k_nb_cell_pairs(
             cy[at] = py;
             cz[at] = pz;
             cj[at] = j;
+            if (MODE == 1) ckey[at] = key;
           }
           cc += tot;
           __syncthreads();
@@ -273,6 +282,7 @@ __global__ void __launch_bounds__(kNbThreads) k_nb_cell_pairs(
               if (o < capacity) {
                 pairs[2 * o] = i;
                 pairs[2 * o + 1] = j;
+                if (keys) keys[o] = ckey[k];
               }
             }
             cnt += __popcll(msk);
@@ -284,7 +294,7 @@ __global__ void __launch_bounds__(kNbThreads) k_nb_cell_pairs(
     }
     __syncthreads();
     if (MODE == 0)
-      for (int t = threadIdx.x; t < nh; t += kNbThreads) pcount[h0 + t] = hcnt[t];
+      for (int t = threadIdx.x; t < nh; t += kNbThreads) pcount[sorted[h0 + t]] = hcnt[t];
   }
 }
 
@@ -557,6 +567,117 @@ __global__ void __launch_bounds__(kBlock) k_pair_ds_bwd(const double* __restrict
   }
 }
 
+// Reverse index of an ordered pair list built by k_nb_cell_pairs (rows by
+// first atom, row i = [off[i], off[i + 1]), each row in ascending key order,
+// keys[o] = stencil entry << 24 | slot of the second atom).  The pair (j, i)
+// of o = (i, j) lies in row j with the mirrored entry nst - 1 - e (the
+// stencil is symmetric about its centre) and the slot of i, found by binary
+// search; rev[o] = its index, -1 if absent.  One wavefront per home slot h.
+__global__ void __launch_bounds__(kBlock) k_nb_reverse(int64_t n, int nst,
+                                                       const int* __restrict__ sorted,
+                                                       const int64_t* __restrict__ off,
+                                                       const int* __restrict__ keys,
+                                                       int* __restrict__ rev) {
+  const int lane = threadIdx.x & 63;
+  const int64_t h = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (h >= n) return;  // whole wavefronts
+  const int i = sorted[h];
+  const int64_t b = off[i], e = off[i + 1];
+  for (int64_t o = b + lane; o < e; o += 64) {
+    const int k = keys[o];
+    const int sj = k & 0xffffff;
+    const int target = ((nst - 1 - (k >> 24)) << 24) | (int)h;
+    const int j = sorted[sj];
+    int64_t lo = off[j], hi = off[j + 1];
+    int found = -1;
+    if (hi > lo) {
+      while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] <= target) lo = mid;
+        else hi = mid;
+      }
+      if (keys[lo] == target) found = (int)lo;
+    }
+    rev[o] = found;
+  }
+}
+
+// dL/dr and dL/dbox of L(ds) for a list in row form: row i holds the pairs
+// whose first atom is i (row_ptr, n + 1 entries) and rev[o] is the index of
+// the reverse pair (j, i) of o = (i, j), or -1.  The second-atom term of pair
+// rev(o) acts on atom i along -u_rev = u_o, so row i sums
+// (g_o + g_rev(o)) d ds_o / d r_i over its own pairs: one wavefront per row,
+// one float64 atomic per row component (grad_pos zeroed by the caller), no
+// per-pair atomics.  A pair without a reverse adds its second-atom term by
+// atomics.  The box term (image pairs, each pair's own g) goes to
+// partial[block][9] as in k_pair_ds_bwd.
+__global__ void __launch_bounds__(kBlock) k_pair_ds_bwd_rows(
+    const double* __restrict__ gds, const int64_t* __restrict__ pairs, int64_t n,
+    const int64_t* __restrict__ row_ptr, const int* __restrict__ rev,
+    const double* __restrict__ pos, CellGrid cg, const double* __restrict__ dr,
+    const double* __restrict__ ds, double* __restrict__ gpos, double* __restrict__ partial) {
+  __shared__ double red[9 * (kBlock / 64)];
+  double gb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int lane = threadIdx.x & 63;
+  const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const double si = cg.obc ? -1.0 : 1.0;  // PBC: v = r_i - r_j - n h; OBC: v = r_j - r_i
+  for (int64_t i = w0; i < n; i += nw) {
+    const int64_t b = row_ptr[i], e = row_ptr[i + 1];
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int64_t o = b + lane; o < e; o += 64) {
+      const double d = ds[o];
+      const double g = gds[o];
+      const int r = rev[o];
+      const double gr = r >= 0 ? gds[r] : 0.0;
+      if (d > 0.0 && (g != 0.0 || gr != 0.0)) {
+        const int64_t j = pairs[2 * o + 1];
+        double v[3], nimg[3];
+        if (dr) {
+          for (int k = 0; k < 3; ++k) v[k] = dr[3 * o + k];
+          if (!cg.obc) {
+            double x[3];
+            for (int k = 0; k < 3; ++k) x[k] = pos[3 * i + k] - pos[3 * j + k];
+            for (int k = 0; k < 3; ++k)
+              nimg[k] = floor(x[0] * cg.a[k] + x[1] * cg.a[3 + k] + x[2] * cg.a[6 + k] + 0.5);
+          }
+        } else {
+          pair_disp(pos, i, j, cg, v, nimg);
+        }
+        const double f = si * (g + gr) / d;
+        sx += f * v[0];
+        sy += f * v[1];
+        sz += f * v[2];
+        if (r < 0 && g != 0.0) {
+          const double fj = si * g / d;
+          unsafeAtomicAdd(&gpos[3 * j], -fj * v[0]);
+          unsafeAtomicAdd(&gpos[3 * j + 1], -fj * v[1]);
+          unsafeAtomicAdd(&gpos[3 * j + 2], -fj * v[2]);
+        }
+        if (!cg.obc && g != 0.0)
+          for (int c = 0; c < 3; ++c)
+            if (nimg[c] != 0.0)
+              for (int k = 0; k < 3; ++k) gb[c * 3 + k] -= g * nimg[c] * v[k] / d;
+      }
+    }
+    if (e > b) {
+      sx = wave_sum(sx);
+      sy = wave_sum(sy);
+      sz = wave_sum(sz);
+      if (lane == 0 && (sx != 0.0 || sy != 0.0 || sz != 0.0)) {
+        unsafeAtomicAdd(&gpos[3 * i], sx);
+        unsafeAtomicAdd(&gpos[3 * i + 1], sy);
+        unsafeAtomicAdd(&gpos[3 * i + 2], sz);
+      }
+    }
+  }
+  if (partial) {
+    block_sum<9>(gb, red);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 9; ++k) partial[(int64_t)blockIdx.x * 9 + k] = gb[k];
+  }
+}
+
 // gbox[k] += sum over blocks of partial[b][k], in block order
 __global__ void __launch_bounds__(kBlock) k_reduce_box(const double* __restrict__ partial,
                                                        int nblocks, double* __restrict__ gbox) {
@@ -766,7 +887,7 @@ hipError_t launch_cell_sort(int64_t n, int64_t ncell, char* ws, const NbLayout& 
 }
 
 hipError_t launch_pairs(int mode, int64_t n, const CellGrid& cg, char* ws, const NbLayout& L,
-                        int64_t* pairs, int64_t capacity, hipStream_t s) {
+                        int64_t* pairs, int64_t capacity, int* keys, hipStream_t s) {
   const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
   if (n == 0) return hipGetLastError();
   const double* w = (const double*)(ws + L.w);
@@ -785,11 +906,42 @@ hipError_t launch_pairs(int mode, int64_t n, const CellGrid& cg, char* ws, const
   const dim3 grid((unsigned)ncell, (unsigned)S);
   if (mode == 0) {
     k_nb_cell_pairs<0><<<grid, kNbThreads, 0, s>>>(cg, S, w, start, sorted, pc, nullptr, nullptr,
-                                                   0);
+                                                   0, nullptr);
     launch_scan<int64_t>(pc, n, off, (int64_t*)(ws + L.scan), s);
   } else {
     k_nb_cell_pairs<1><<<grid, kNbThreads, 0, s>>>(cg, S, w, start, sorted, nullptr, off, pairs,
-                                                   capacity);
+                                                   capacity, keys);
+  }
+  return hipGetLastError();
+}
+
+int nb_stencil_entries(const CellGrid& cg) {
+  int m = 1;
+  for (int d = 0; d < 3; ++d) m *= cg.nc[d] >= 3 ? 3 : 2 * cg.nimg[d] + 1;
+  return m;
+}
+
+bool nb_reverse_supported(int64_t n, const CellGrid& cg, int64_t np) {
+  return n < ((int64_t)1 << 24) && nb_stencil_entries(cg) <= 127 && np < ((int64_t)1 << 31);
+}
+
+hipError_t launch_nb_reverse(int64_t n, const CellGrid& cg, const char* ws, const NbLayout& L,
+                             const int* keys, int64_t np, int64_t* row_ptr, int* rev,
+                             hipStream_t s) {
+  const int64_t* off = (const int64_t*)(ws + L.offsets);
+  if (row_ptr)
+    if (hipMemcpyAsync(row_ptr, off, sizeof(int64_t) * (size_t)(n + 1), hipMemcpyDeviceToDevice,
+                       s) != hipSuccess)
+      return hipGetLastError();
+  if (rev && np > 0) {
+    if (!keys || !nb_reverse_supported(n, cg, np)) {
+      k_fill<int><<<(unsigned)std::min<int64_t>((np + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(
+          rev, np, -1);
+    } else {
+      const int64_t threads = n * 64;
+      k_nb_reverse<<<(unsigned)((threads + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+          n, nb_stencil_entries(cg), (const int*)(ws + L.sorted), off, keys, rev);
+    }
   }
   return hipGetLastError();
 }
@@ -809,12 +961,22 @@ size_t pair_ds_bwd_scratch_bytes(int64_t np) {
 hipError_t launch_pair_ds_bwd(const double* gds, const int64_t* pairs, int64_t np,
                               const double* pos, const CellGrid& cg, const double* dr,
                               const double* ds, double* gpos, double* gbox, double* partial,
+                              const int64_t* row_ptr, const int* rev, int64_t n,
                               hipStream_t s) {
   if (np <= 0) return hipGetLastError();
-  const int nb = (int)std::min<int64_t>((np + kBlock - 1) / kBlock, kDsBwdBlocks);
   const bool box_term = gbox && !cg.obc;
-  k_pair_ds_bwd<<<nb, kBlock, 0, s>>>(gds, pairs, np, pos, cg, dr, ds, gpos,
-                                      box_term ? partial : nullptr);
+  int nb;
+  if (row_ptr && rev) {
+    nb = (int)std::min<int64_t>((n * 64 + kBlock - 1) / kBlock, kDsBwdBlocks);
+    if (nb > 0)
+      k_pair_ds_bwd_rows<<<nb, kBlock, 0, s>>>(gds, pairs, n, row_ptr, rev, pos, cg, dr, ds, gpos,
+                                               box_term ? partial : nullptr);
+  } else {
+    nb = (int)std::min<int64_t>((np + kBlock - 1) / kBlock, kDsBwdBlocks);
+    k_pair_ds_bwd<<<nb, kBlock, 0, s>>>(gds, pairs, np, pos, cg, dr, ds, gpos,
+                                        box_term ? partial : nullptr);
+  }
+  if (nb <= 0) return hipGetLastError();
   if (box_term) k_reduce_box<<<1, kBlock, 0, s>>>(partial, nb, gbox);
   return hipGetLastError();
 }

@@ -64,10 +64,23 @@ hipError_t launch_cell_sort(int64_t n, int64_t ncell, char* ws, const NbLayout& 
 // its first half: start and the unordered cell lists in `unsorted`
 hipError_t launch_cell_scatter(int64_t n, int64_t ncell, char* ws, const NbLayout& L,
                                hipStream_t s);
-// mode 0: count pairs and scan the per-atom counts (total at offsets[n]);
-// mode 1: write the pairs
+// mode 0: count pairs per atom and scan the counts in atom order (row
+// starts at offsets[i], total at offsets[n]); mode 1: write the pairs, and
+// with `keys` (int32 per pair) their in-row order keys
 hipError_t launch_pairs(int mode, int64_t n, const CellGrid& cg, char* ws, const NbLayout& L,
-                        int64_t* pairs, int64_t capacity, hipStream_t s);
+                        int64_t* pairs, int64_t capacity, int* keys, hipStream_t s);
+// stencil entries of the cell grid (3 per dimension of 3+ cells, else the
+// images -nimg..nimg)
+int nb_stencil_entries(const CellGrid& cg);
+// whether the pair keys can encode the list (n < 2^24 atoms, <= 127 stencil
+// entries, P < 2^31)
+bool nb_reverse_supported(int64_t n, const CellGrid& cg, int64_t np);
+// after mode 1 with keys: row_ptr (n + 1, may be NULL) = the row starts, and
+// rev[o] (may be NULL) = index of the reverse pair of o, -1 if absent (every
+// entry -1 when keys is NULL or the list is not supported)
+hipError_t launch_nb_reverse(int64_t n, const CellGrid& cg, const char* ws, const NbLayout& L,
+                             const int* keys, int64_t np, int64_t* row_ptr, int* rev,
+                             hipStream_t s);
 hipError_t launch_pair_ds(const double* pos, const int64_t* pairs, int64_t np,
                           const CellGrid& cg, double* ds, double* dr, hipStream_t s);
 // out[0] = max_i |pos_i - ref_i| (minimum image under PBC), on the stream
@@ -75,9 +88,12 @@ hipError_t launch_max_disp(const double* pos, const double* ref, int64_t n, cons
                            double* out, hipStream_t s);
 // workspace of launch_pair_ds_bwd (per-block partial sums of dL/dbox)
 size_t pair_ds_bwd_scratch_bytes(int64_t np);
+// row_ptr (n + 1) and rev given: the row form (k_pair_ds_bwd_rows), else
+// any pair order (k_pair_ds_bwd)
 hipError_t launch_pair_ds_bwd(const double* gds, const int64_t* pairs, int64_t np,
                               const double* pos, const CellGrid& cg, const double* dr,
                               const double* ds, double* gpos, double* gbox, double* partial,
+                              const int64_t* row_ptr, const int* rev, int64_t n,
                               hipStream_t s);
 hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const double* q,
                               const double* eta, const int64_t* pairs, const double* ds,

@@ -424,6 +424,32 @@ hipError_t launch_convolve(const T* mesh, const Geom& g, int conv, double kappa,
   return hipGetLastError();
 }
 
+// dE/dbox of the reciprocal energy from its two dE/dA parts (k vectors and
+// the gather's s = r A term, 9 values each, row-major) and the energy (the
+// volume term): with A = box^-1, dE/dbox = -A^T (dA1 + dA2) A^T - E A^T.  One
+// thread; every input is on the device, so no stream synchronisation.
+__global__ void k_box_grad(Geom g, const double* __restrict__ dA1, const double* __restrict__ dA2,
+                           const double* __restrict__ energy, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double d[9];
+  for (int k = 0; k < 9; ++k) d[k] = dA1[k] + (dA2 ? dA2[k] : 0.0);
+  const double e = energy[0];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      // (A^T d A^T)[r][c] = sum_kl A[k][r] d[k][l] A[c][l]
+      double acc = 0.0;
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) acc += g.a[3 * k + r] * d[3 * k + l] * g.a[3 * c + l];
+      out[3 * r + c] = -acc - e * g.a[3 * c + r];
+    }
+}
+
+hipError_t launch_box_grad(const Geom& g, const double* dA1, const double* dA2,
+                           const double* energy, double* out, hipStream_t s) {
+  k_box_grad<<<1, 64, 0, s>>>(g, dA1, dA2, energy, out);
+  return hipGetLastError();
+}
+
 size_t convolve_partials_count(const int32_t* k) {
   const int Kzh = k[2] / 2 + 1;
   // the chunk picked for pass 3 is at least 1, so this bounds the block count

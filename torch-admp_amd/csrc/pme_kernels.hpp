@@ -35,6 +35,9 @@ hipError_t launch_convolve(const T* mesh, const Geom& g, int conv, double kappa,
                            cplx<T>* spec, double* partials, hipStream_t s);
 
 // number of doubles the convolution needs for its per-block partials
+// out (9) = -A^T (dA1 + dA2) A^T - energy A^T with A = box^-1 (dA2 may be NULL)
+hipError_t launch_box_grad(const Geom& g, const double* dA1, const double* dA2,
+                           const double* energy, double* out, hipStream_t s);
 size_t convolve_partials_count(const int32_t* k);
 
 }  // namespace admp

@@ -51,38 +51,80 @@ def _f64(t: Tensor) -> Tensor:
     return t.detach().to(_F64).contiguous()
 
 
-def _ws(nbytes: int, dev) -> Tensor:
-    return _lib.workspace(nbytes, dev)
+def _ws(nbytes: int, dev, kind: str) -> Tensor:
+    return _lib.stream_workspace(kind, nbytes, dev)
+
+
+_SIZES: "dict" = {}
+
+
+def _ws_bytes(fn_name: str, *args) -> int:
+    """Workspace size from the library's size query, remembered per argument
+    values (the queries are pure functions of them)."""
+    key = (fn_name,) + tuple(tuple(a) if isinstance(a, ctypes.Array) else a for a in args)
+    nb = _SIZES.get(key)
+    if nb is None:
+        out = ctypes.c_size_t()
+        _lib.check(getattr(_lib.load(), fn_name)(*args, ctypes.byref(out)), fn_name)
+        nb = out.value
+        if len(_SIZES) > 4096:
+            _SIZES.clear()
+        _SIZES[key] = nb
+    return nb
 
 
 # ------------------------------------------------------------------ neighbour pairs
-@torch.library.custom_op("admp::neighbor_pairs", mutates_args=())
-def neighbor_pairs(positions: Tensor, box: Optional[Tensor], cutoff: float,
-                   box_host: Optional[Tensor]) -> Tensor:
-    """Ordered pairs (P, 2) int64 with 0.01 < d < cutoff (box) or d < cutoff
-    (box None), both orderings, grouped by the first atom
-    (src/torch_admp/nblist.py:335-490).  One stream synchronisation returns P.
-    box_host: the box's values on the host when the caller has them (saves
-    the device-to-host copy), else None; the same holds for every operator."""
+def _search(positions: Tensor, box: Optional[Tensor], cutoff: float,
+            box_host: Optional[Tensor], rows: bool) -> Tuple[Tensor, Tensor, Tensor]:
     _lib.require_gpu(positions)
     lib = _lib.load()
     pos = _f64(positions)
     n = pos.shape[0]
     boxh = _lib.box_array(box, box_host)
-    nbytes = ctypes.c_size_t()
-    _lib.check(lib.admp_nblist_workspace_bytes(n, boxh, float(cutoff), ctypes.byref(nbytes)),
-               "admp_nblist_workspace_bytes")
-    ws = _ws(nbytes.value, pos.device)
+    nbytes = _ws_bytes("admp_nblist_workspace_bytes", n, boxh, float(cutoff))
+    ws = _ws(nbytes, pos.device, "nblist")
     stream = _lib.stream_of(pos)
     npairs = ctypes.c_int64()
     _lib.check(lib.admp_nblist_count(_lib.ptr(pos), n, boxh, float(cutoff), _lib.ptr(ws),
-                                     nbytes.value, ctypes.byref(npairs), stream),
+                                     nbytes, ctypes.byref(npairs), stream),
                "admp_nblist_count")
-    pairs = torch.empty(npairs.value, 2, dtype=torch.int64, device=pos.device)
+    p = npairs.value
+    pairs = torch.empty(p, 2, dtype=torch.int64, device=pos.device)
+    if rows:
+        row_ptr = torch.empty(n + 1, dtype=torch.int64, device=pos.device)
+        reverse = torch.empty(p, dtype=torch.int32, device=pos.device)
+        scratch = torch.empty(p, dtype=torch.int32, device=pos.device)
+        if n == 0:
+            row_ptr.zero_()
+        ptrs = (_lib.ptr(row_ptr), _lib.ptr(reverse), _lib.ptr(scratch))
+    else:
+        row_ptr = reverse = torch.empty(0, dtype=torch.int64, device=pos.device)
+        ptrs = (None, None, None)
     _lib.check(lib.admp_nblist_fill(_lib.ptr(pos), n, boxh, float(cutoff), _lib.ptr(ws),
-                                    nbytes.value, _lib.ptr(pairs), npairs.value, stream),
+                                    nbytes, _lib.ptr(pairs), p, *ptrs, stream),
                "admp_nblist_fill")
-    return pairs
+    return pairs, row_ptr, reverse
+
+
+@torch.library.custom_op("admp::neighbor_pairs", mutates_args=())
+def neighbor_pairs(positions: Tensor, box: Optional[Tensor], cutoff: float,
+                   box_host: Optional[Tensor]) -> Tensor:
+    """Ordered pairs (P, 2) int64 with 0.01 < d < cutoff (box) or d < cutoff
+    (box None), both orderings, grouped by the first atom in ascending index
+    (src/torch_admp/nblist.py:335-490).  One stream synchronisation returns P.
+    box_host: the box's values on the host when the caller has them (saves
+    the device-to-host copy), else None; the same holds for every operator."""
+    return _search(positions, box, cutoff, box_host, False)[0]
+
+
+@torch.library.custom_op("admp::neighbor_list", mutates_args=())
+def neighbor_list(positions: Tensor, box: Optional[Tensor], cutoff: float,
+                  box_host: Optional[Tensor]) -> Tuple[Tensor, Tensor, Tensor]:
+    """The pairs of neighbor_pairs with their row form: row_ptr (n + 1,
+    int64; row i = pairs[row_ptr[i]:row_ptr[i + 1]], first atom i) and
+    reverse (P, int32; reverse[o] = index of the pair (j, i) of o = (i, j)).
+    pair_distance given both runs its backward without per-pair atomics."""
+    return _search(positions, box, cutoff, box_host, True)
 
 
 @torch.library.custom_op("admp::max_displacement", mutates_args=())
@@ -103,10 +145,12 @@ def max_displacement(positions: Tensor, reference: Tensor, box: Optional[Tensor]
 # ------------------------------------------------------------------ pair distances
 @torch.library.custom_op("admp::pair_distance", mutates_args=())
 def pair_distance(positions: Tensor, box: Optional[Tensor], pairs: Tensor,
-                  box_host: Optional[Tensor]) -> Tensor:
+                  box_host: Optional[Tensor], row_ptr: Optional[Tensor] = None,
+                  reverse: Optional[Tensor] = None) -> Tensor:
     """ds = |pbc_shift(r_j - r_i)| (src/torch_admp/nblist.py:517-551,
     src/torch_admp/spatial.py:15-36), |r_j - r_i| when box is None; (P,) in the
-    positions' dtype."""
+    positions' dtype.  row_ptr and reverse (from neighbor_list, for these
+    pairs) only change how the backward sums."""
     _lib.require_gpu(positions, pairs)
     lib = _lib.load()
     pos = _f64(positions)
@@ -120,11 +164,14 @@ def pair_distance(positions: Tensor, box: Optional[Tensor], pairs: Tensor,
 
 @torch.library.custom_op("admp::pair_distance_backward", mutates_args=())
 def pair_distance_backward(grad_ds: Tensor, positions: Tensor, box: Optional[Tensor],
-                           pairs: Tensor, ds: Tensor,
-                           box_host: Optional[Tensor]) -> Tuple[Tensor, Tensor]:
+                           pairs: Tensor, ds: Tensor, box_host: Optional[Tensor],
+                           row_ptr: Optional[Tensor] = None,
+                           reverse: Optional[Tensor] = None) -> Tuple[Tensor, Tensor]:
     """(dL/dpositions, dL/dbox) of L(ds); dL/dbox is zero when box is None.
-    The displacement is recomputed from the positions."""
+    The displacement is recomputed from the positions.  With row_ptr and
+    reverse each atom's row is summed in one pass (no per-pair atomics)."""
     _lib.require_gpu(grad_ds, positions, pairs)
+    rows = row_ptr is not None and reverse is not None and row_ptr.numel() == positions.shape[0] + 1
     lib = _lib.load()
     pos = _f64(positions)
     prs = pairs.detach().to(torch.int64).contiguous()
@@ -132,14 +179,18 @@ def pair_distance_backward(grad_ds: Tensor, positions: Tensor, box: Optional[Ten
     gpos = torch.zeros_like(pos)
     gbox = torch.zeros(3, 3, dtype=_F64, device=pos.device)
     boxh = _lib.box_array(box, box_host)
-    nbytes = ctypes.c_size_t()
-    _lib.check(lib.admp_pair_ds_backward_workspace_bytes(prs.shape[0], ctypes.byref(nbytes)),
-               "admp_pair_ds_backward_workspace_bytes")
-    ws = _ws(nbytes.value, pos.device)
+    nbytes = _ws_bytes("admp_pair_ds_backward_workspace_bytes", prs.shape[0])
+    ws = _ws(nbytes, pos.device, "pair_ds")
+    rp = rv = None
+    if rows:
+        assert row_ptr is not None and reverse is not None
+        rp = _lib.ptr(row_ptr.to(torch.int64).contiguous())
+        rv = _lib.ptr(reverse.to(torch.int32).contiguous())
     _lib.check(lib.admp_pair_ds_backward(_lib.ptr(g), _lib.ptr(prs), prs.shape[0], _lib.ptr(pos),
                                          boxh, None, _lib.ptr(_f64(ds)), _lib.ptr(gpos),
                                          _lib.ptr(gbox) if boxh is not None else None,
-                                         _lib.ptr(ws), nbytes.value, _lib.stream_of(pos)),
+                                         rp, rv, pos.shape[0], _lib.ptr(ws), nbytes,
+                                         _lib.stream_of(pos)),
                "admp_pair_ds_backward")
     if box is not None:
         gbox = gbox.to(device=box.device, dtype=box.dtype)
@@ -147,18 +198,20 @@ def pair_distance_backward(grad_ds: Tensor, positions: Tensor, box: Optional[Ten
 
 
 def _pd_setup(ctx, inputs, output):
-    positions, box, pairs, box_host = inputs
-    ctx.save_for_backward(positions, box, pairs, output, box_host)
+    positions, box, pairs, box_host, row_ptr, reverse = inputs
+    ctx.save_for_backward(positions, box, pairs, output, box_host, row_ptr, reverse)
     ctx.set_materialize_grads(False)
 
 
 def _pd_backward(ctx, g_ds):
-    positions, box, pairs, ds, box_host = ctx.saved_tensors
+    positions, box, pairs, ds, box_host, row_ptr, reverse = ctx.saved_tensors
     if g_ds is None:
-        return None, None, None, None
-    gpos, gbox = torch.ops.admp.pair_distance_backward(g_ds, positions, box, pairs, ds, box_host)
+        return None, None, None, None, None, None
+    gpos, gbox = torch.ops.admp.pair_distance_backward(g_ds, positions, box, pairs, ds, box_host,
+                                                       row_ptr, reverse)
     return (gpos if ctx.needs_input_grad[0] else None,
-            gbox if box is not None and ctx.needs_input_grad[1] else None, None, None)
+            gbox if box is not None and ctx.needs_input_grad[1] else None,
+            None, None, None, None)
 
 
 def _pdb_setup(ctx, inputs, output):
@@ -168,7 +221,7 @@ def _pdb_setup(ctx, inputs, output):
 def _pdb_backward(ctx, g_gpos, g_gbox):
     if g_gpos is not None or g_gbox is not None:
         raise NotImplementedError(_SECOND_ORDER.format("the pair distances"))
-    return (None,) * 6
+    return (None,) * 8
 
 
 torch.library.register_autograd("admp::pair_distance", _pd_backward, setup_context=_pd_setup)
@@ -196,13 +249,12 @@ def _pair_call(q: Tensor, ds: Tensor, pairs: Tensor, bs: Tensor, eta: Optional[T
     gds = torch.empty(nf, npairs, dtype=_F64, device=dev) if want_ds else None
     geta = (torch.empty(nf, n, dtype=_F64, device=dev)
             if want_eta and mode == PAIR_DAMPED and eta is not None else None)
-    nbytes = ctypes.c_size_t()
-    _lib.check(lib.admp_pair_energy_workspace_bytes(nf, ctypes.byref(nbytes)), "workspace")
-    ws = _ws(nbytes.value, dev)
+    nbytes = _ws_bytes("admp_pair_energy_workspace_bytes", nf)
+    ws = _ws(nbytes, dev, "pair_energy")
     _lib.check(lib.admp_pair_energy(mode, nf, n, npairs, _lib.ptr(q64), _lib.ptr(eta64),
                                     _lib.ptr(prs), _lib.ptr(ds64), _lib.ptr(bs64), kappa, eps,
                                     _lib.ptr(energy), _lib.ptr(gds), _lib.ptr(gq), _lib.ptr(geta),
-                                    _lib.ptr(ws), nbytes.value, _lib.stream_of(q64)),
+                                    _lib.ptr(ws), nbytes, _lib.stream_of(q64)),
                "admp_pair_energy")
     return energy, gq, gds, geta
 
@@ -299,16 +351,9 @@ torch.library.register_autograd("admp::pair_energy_grads", _peg_backward,
 
 
 # ------------------------------------------------------------------ PME reciprocal space
-def _inverse_transpose(boxh) -> Tensor:
-    b = torch.tensor(list(boxh), dtype=_F64).reshape(3, 3)
-    return torch.linalg.inv(b).transpose(0, 1)
-
-
 def _pme_ws(lib, km, precision: int, n: int, dev) -> Tuple[Tensor, int]:
-    nbytes = ctypes.c_size_t()
-    _lib.check(lib.admp_pme_workspace_bytes(km, precision, n, ctypes.byref(nbytes)),
-               "admp_pme_workspace_bytes")
-    return _ws(nbytes.value, dev), nbytes.value
+    nbytes = _ws_bytes("admp_pme_workspace_bytes", km, precision, n)
+    return _ws(nbytes, dev, "pme"), nbytes
 
 
 @torch.library.custom_op("admp::pme_reciprocal", mutates_args=())
@@ -335,13 +380,14 @@ def pme_reciprocal(positions: Tensor, box: Tensor, q: Tensor, kmesh: List[int], 
     _lib.check(lib.admp_pme_spread(_lib.ptr(pos), _lib.ptr(q64), n, boxh, km, precision,
                                    _lib.ptr(mesh), _lib.ptr(ws), nbytes, stream),
                "admp_pme_spread")
-    out = torch.empty(10, dtype=_F64, device=dev)
+    energy = torch.empty((), dtype=_F64, device=dev)
+    dEdA = torch.empty(9, dtype=_F64, device=dev)
     _lib.check(lib.admp_pme_convolve(_lib.ptr(mesh), boxh, km, precision, convention, kappa,
                                      eps, 1.0, _lib.ptr(mesh) if need_potential else None,
-                                     _lib.ptr(out), _lib.ptr(out[1:]), _lib.ptr(ws), nbytes,
+                                     _lib.ptr(energy), _lib.ptr(dEdA), _lib.ptr(ws), nbytes,
                                      stream), "admp_pme_convolve")
     phi = mesh if need_potential else mesh.new_empty(0)
-    return out[0].to(positions.dtype).clone(), phi, out[1:].clone()
+    return energy.to(positions.dtype), phi, dEdA
 
 
 @torch.library.custom_op("admp::pme_reciprocal_grads", mutates_args=())
@@ -368,9 +414,10 @@ def pme_reciprocal_grads(positions: Tensor, box: Tensor, q: Tensor, potential: T
                                    _lib.ptr(potential.contiguous()), _lib.ptr(gpos), _lib.ptr(gq),
                                    _lib.ptr(dEdA), 0, _lib.ptr(ws), nbytes, _lib.stream_of(pos)),
                "admp_pme_gather")
-    at = _inverse_transpose(boxh).to(dev)                     # A^T with A = box^-1
-    dA = (_f64(dEdA_k) + dEdA).reshape(3, 3)
-    gbox = -(at @ dA @ at) - _f64(energy) * at
+    gbox = torch.empty(3, 3, dtype=_F64, device=dev)
+    _lib.check(lib.admp_pme_box_grad(boxh, _lib.ptr(_f64(dEdA_k).contiguous()), _lib.ptr(dEdA),
+                                     _lib.ptr(_f64(energy).contiguous()), _lib.ptr(gbox),
+                                     _lib.stream_of(pos)), "admp_pme_box_grad")
     return (gpos.to(positions.dtype), gq.to(q.dtype).reshape(q.shape),
             gbox.to(device=box.device, dtype=box.dtype))
 

@@ -11,6 +11,7 @@ from typing import Optional
 
 import torch
 
+from . import _lib
 from . import _ops  # noqa: F401  (registers torch.ops.admp)
 
 
@@ -43,22 +44,29 @@ class TorchNeighborList(torch.nn.Module):
         self.pairs = torch.empty(1, dtype=torch.long)
         self.buffer_scales = torch.empty(1, dtype=torch.long)
         self.ds = torch.empty(1, dtype=torch.float64)
+        # row form of the last search: row starts per first atom and the index
+        # of each pair's reverse; the distance backward then needs no per-pair
+        # atomics
+        self.row_ptr = torch.empty(0, dtype=torch.long)
+        self.reverse = torch.empty(0, dtype=torch.int32)
 
     def forward(self, positions: torch.Tensor, box: Optional[torch.Tensor] = None) -> torch.Tensor:
         # one host copy of the box serves the cutoff check, the pair search and
         # the distances (each copy of a device box waits for the stream)
         box_host: Optional[torch.Tensor] = None
-        if box is None:
-            pairs = self.forward_obc(positions)
-            pbc_flag = False
-        else:
+        if box is not None:
             box_host = box.detach().to(torch.device("cpu"), torch.float64)
+            if not torch.jit.is_scripting():
+                _lib.remember_host_box(box, box_host)
             check_cutoff(box_host, self.cutoff)
-            pairs = torch.ops.admp.neighbor_pairs(positions, box, self.cutoff, box_host)
-            pbc_flag = True
+        pairs, row_ptr, reverse = torch.ops.admp.neighbor_list(positions, box, self.cutoff,
+                                                               box_host)
         self.pairs = pairs
+        self.row_ptr = row_ptr
+        self.reverse = reverse
         self.buffer_scales = self.pairs_buffer_scales(pairs)
-        self.ds = self.pairs_ds(positions, pairs, box, pbc_flag, box_host)
+        self.ds = self.pairs_ds(positions, pairs, box, box is not None, box_host, row_ptr,
+                                reverse)
         return pairs
 
     def forward_pbc(self, positions: torch.Tensor, box: torch.Tensor) -> torch.Tensor:
@@ -76,13 +84,16 @@ class TorchNeighborList(torch.nn.Module):
 
     @staticmethod
     def pairs_ds(positions: torch.Tensor, pairs: torch.Tensor, box: Optional[torch.Tensor] = None,
-                 pbc_flag: bool = True, box_host: Optional[torch.Tensor] = None) -> torch.Tensor:
+                 pbc_flag: bool = True, box_host: Optional[torch.Tensor] = None,
+                 row_ptr: Optional[torch.Tensor] = None,
+                 reverse: Optional[torch.Tensor] = None) -> torch.Tensor:
         """Differentiable pair distances (src/torch_admp/nblist.py:517-551);
-        box_host optionally carries the box's values on the host."""
+        box_host optionally carries the box's values on the host, row_ptr and
+        reverse the row form of these pairs from the search."""
         if pbc_flag:
             assert box is not None, "Box should be provided for periodic system."
-            return torch.ops.admp.pair_distance(positions, box, pairs, box_host)
-        return torch.ops.admp.pair_distance(positions, None, pairs, None)
+            return torch.ops.admp.pair_distance(positions, box, pairs, box_host, row_ptr, reverse)
+        return torch.ops.admp.pair_distance(positions, None, pairs, None, row_ptr, reverse)
 
     def set_pairs(self, pairs: torch.Tensor) -> None:
         self.pairs = pairs
@@ -144,7 +155,8 @@ class VerletNeighborList(TorchNeighborList):
 
     def forward(self, positions: torch.Tensor, box: Optional[torch.Tensor] = None) -> torch.Tensor:
         box_host = None if box is None else box.detach().to(torch.device("cpu"), torch.float64)
-        if box_host is not None:
+        if box is not None and box_host is not None:
+            _lib.remember_host_box(box, box_host)
             check_cutoff(box_host, self.cutoff)
         if self._needs_search(positions, box, box_host):
             skin = self.skin
@@ -153,8 +165,11 @@ class VerletNeighborList(TorchNeighborList):
                     check_cutoff(box_host, self.cutoff + skin)
                 except AssertionError:
                     skin = 0.0
-            self._list = torch.ops.admp.neighbor_pairs(positions, box, self.cutoff + skin,
-                                                       box_host)
+            lst, row_ptr, reverse = torch.ops.admp.neighbor_list(positions, box,
+                                                                 self.cutoff + skin, box_host)
+            self._list = lst
+            self.row_ptr = row_ptr
+            self.reverse = reverse
             self._list_skin = skin
             self._ref = positions.detach().clone()
             self._ref_box = box_host
@@ -163,7 +178,8 @@ class VerletNeighborList(TorchNeighborList):
             self.reuses += 1
         pairs = self._list
         self.pairs = pairs
-        self.ds = self.pairs_ds(positions, pairs, box, box is not None, box_host)
+        self.ds = self.pairs_ds(positions, pairs, box, box is not None, box_host, self.row_ptr,
+                                self.reverse)
         d = self.ds.detach()
         inside = (d < self.cutoff) & ((d > 0.01) if box is not None else (d >= 0.0))
         self.buffer_scales = ((pairs[:, 0] < pairs[:, 1]) & inside).to(torch.long)

@@ -23,6 +23,13 @@ from .ops import pme_fused
 from .utils import safe_inverse  # noqa: F401  (re-export, reference API)
 
 
+def _volume(box: torch.Tensor) -> torch.Tensor:
+    """det of each (3, 3) box as the triple product a . (b x c): the same
+    value and derivative as torch.det, in elementwise kernels rather than a
+    device LU factorisation."""
+    return torch.sum(box[..., 0, :] * torch.cross(box[..., 1, :], box[..., 2, :], dim=-1), dim=-1)
+
+
 class CoulombForceModule(BaseForceModule):
     """Coulomb energy with PME (src/torch_admp/pme.py:24-561).
 
@@ -135,25 +142,35 @@ class CoulombForceModule(BaseForceModule):
     def _forward_pbc(self, charges: torch.Tensor, positions: torch.Tensor, box: torch.Tensor,
                      pairs: torch.Tensor, ds: torch.Tensor,
                      buffer_scales: torch.Tensor) -> torch.Tensor:
-        """Sum of the five terms (src/torch_admp/pme.py:261-310)."""
+        """Sum of the five terms (src/torch_admp/pme.py:261-310); terms that are
+        switched off contribute zero and keep their previous attribute value."""
+        terms: List[torch.Tensor] = []
         if self.rspace_flag:
             self.real_energy = self._forward_pbc_real(charges, pairs, ds, buffer_scales)
+            terms.append(self.real_energy)
         if self.kspace_flag:
             self.reciprocal_energy = self._forward_pbc_reciprocal(charges, positions, box)
             self.self_energy = self._forward_pbc_self(charges)
             self.non_neutral_energy = self._forward_pbc_non_neutral(charges, box)
+            terms.append(self.reciprocal_energy)
+            terms.append(self.self_energy)
+            terms.append(self.non_neutral_energy)
         if self.slab_corr_flag:
             self.slab_corr_energy = self._forward_slab_corr(charges, positions, box, ds)
-        dev = positions.device
-        return (self.real_energy.to(dev) + self.reciprocal_energy.to(dev)
-                + self.self_energy.to(dev) + self.non_neutral_energy.to(dev)
-                + self.slab_corr_energy.to(dev))
+            terms.append(self.slab_corr_energy)
+        if len(terms) == 0:
+            return torch.zeros(charges.size(0), device=positions.device, dtype=positions.dtype)
+        energy = terms[0]
+        for t in terms[1:]:
+            energy = energy + t
+        return energy
 
     def _forward_pbc_real(self, charges: torch.Tensor, pairs: torch.Tensor, ds: torch.Tensor,
                           buffer_scales: torch.Tensor) -> torch.Tensor:
         """eps erfc(kappa d) qi qj / d over the pair rows (src/torch_admp/pme.py:312-349)."""
-        # pair mode 0: erfc-screened pairs
-        return torch.ops.admp.pair_energy(charges, ds, pairs, buffer_scales, None, 0,
+        # pair mode 0: erfc-screened pairs; the scales converted once here serve
+        # the forward and the backward kernels
+        return torch.ops.admp.pair_energy(charges, ds, pairs, buffer_scales.to(ds.dtype), None, 0,
                                           self.kappa, self.const_lib.dielectric)
 
     def _forward_pbc_reciprocal(self, charges: torch.Tensor, positions: torch.Tensor,
@@ -162,7 +179,11 @@ class CoulombForceModule(BaseForceModule):
         out: List[torch.Tensor] = []
         for ii in range(positions.size(0)):
             # one host copy of the frame's box for the mesh rule and the kernels
-            box_host = box[ii].detach().to(torch.device("cpu"), torch.float64)
+            # (the neighbour list's copy of the same box when it has one)
+            if torch.jit.is_scripting():
+                box_host = box[ii].detach().to(torch.device("cpu"), torch.float64)
+            else:
+                box_host = _lib.host_box(box[ii])
             km = self.mesh_for_box(box_host)
             need = torch.is_grad_enabled() and (positions.requires_grad or box.requires_grad
                                                 or charges.requires_grad)
@@ -183,7 +204,7 @@ class CoulombForceModule(BaseForceModule):
 
     def _forward_pbc_non_neutral(self, charges: torch.Tensor, box: torch.Tensor) -> torch.Tensor:
         """-eps pi Q^2 / (2 V kappa^2) (src/torch_admp/pme.py:463-493)."""
-        volume = torch.det(box)
+        volume = _volume(box)
         q_tot = torch.sum(charges, dim=-1)
         coeff = -self.const_lib.pi / (2 * volume * self.kappa ** 2) * self.const_lib.dielectric
         return coeff * q_tot ** 2
@@ -191,7 +212,7 @@ class CoulombForceModule(BaseForceModule):
     def _forward_slab_corr(self, charges: torch.Tensor, positions: torch.Tensor,
                            box: torch.Tensor, ds: torch.Tensor) -> torch.Tensor:
         """Yeh-Berkowitz slab term (src/torch_admp/pme.py:495-527)."""
-        volume = torch.det(box)
+        volume = _volume(box)
         pre = 2 * self.const_lib.pi / volume * self.const_lib.dielectric
         z = positions[:, :, self.slab_axis]
         mz = torch.sum(charges * z, dim=-1)
@@ -204,8 +225,8 @@ class CoulombForceModule(BaseForceModule):
                      buffer_scales: torch.Tensor) -> torch.Tensor:
         """Bare Coulomb over the pair list (src/torch_admp/pme.py:529-561)."""
         # pair mode 1: bare pairs
-        return torch.ops.admp.pair_energy(charges, ds, pairs, buffer_scales, None, 1, 0.0,
-                                          self.const_lib.dielectric)
+        return torch.ops.admp.pair_energy(charges, ds, pairs, buffer_scales.to(ds.dtype), None, 1,
+                                          0.0, self.const_lib.dielectric)
 
     # -- fused single-call path (no pair list crosses the boundary) -------------
     @torch.jit.unused
