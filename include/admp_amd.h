@@ -69,11 +69,12 @@ int admp_nblist_workspace_bytes(int64_t n, const double* box, double cutoff, siz
 int admp_nblist_count(const double* pos, int64_t n, const double* box, double cutoff,
                       void* workspace, size_t ws_bytes, int64_t* n_pairs, void* stream);
 
-/* Writes the pairs counted by admp_nblist_count; capacity is in pairs.  Rows
- * are grouped by first atom in ascending atom index.  Optional outputs (each
- * may be NULL): row_ptr (int64, n + 1) receives the row starts (row i =
- * pairs [row_ptr[i], row_ptr[i + 1])); reverse (int32, capacity) receives for
- * each pair (i, j) the index of (j, i), -1 when the list cannot be indexed
+/* Writes the pairs counted by admp_nblist_count; capacity is in pairs.  The
+ * pairs come in n rows, one per atom in cell order (rows may be empty), each
+ * row's pairs sharing their first atom.  Optional outputs (each may be NULL):
+ * row_ptr (int64, n + 1) receives the row starts (row r = pairs
+ * [row_ptr[r], row_ptr[r + 1])); reverse (int32, capacity) receives for each
+ * pair (i, j) the index of (j, i), -1 when the list cannot be indexed
  * (n >= 2^24 atoms or P >= 2^31), and then needs scratch (int32, capacity).
  * row_ptr and reverse let admp_pair_ds_backward run without per-pair
  * atomics.  No reference counterpart: TorchNeighborList returns the pairs
@@ -101,7 +102,7 @@ int admp_pair_ds(const double* pos, const int64_t* pairs, int64_t n_pairs, const
  * grad_box (9) += dL/dds * d ds / d box (may be NULL).  grad_pos and grad_box
  * must be zeroed by the caller.  dr may be NULL (the displacement is then
  * recomputed from pos).  With row_ptr and reverse from admp_nblist_fill (both
- * or neither; n_atoms = rows) each row sums its pairs' first-atom terms and
+ * or neither; n_atoms = the number of rows) each row sums its pairs' first-atom terms and
  * their reverses' second-atom terms, one float64 atomic per row component;
  * otherwise runs of equal first atoms are summed in registers before the
  * atomics and the second atom adds per pair.  The box term goes through
@@ -167,6 +168,21 @@ int admp_pme_gather(const double* pos, const double* q, int64_t n, const double*
                     const int32_t* kmesh, int32_t precision, const void* phi,
                     double* grad_pos, double* dEdq, double* dEdA, int32_t accumulate,
                     void* workspace, size_t ws_bytes, void* stream);
+
+/* Self and non-neutral Ewald terms of nf frames of n charges
+ * (src/torch_admp/pme.py:448-493): e_self[f] = -dielectric kappa / sqrt(pi)
+ * sum q^2 and e_nonneutral[f] = -dielectric pi Q^2 / (2 V kappa^2), Q = sum q,
+ * V = det(box_f); box is on the device (nf x 9, row vectors), so no host copy
+ * is needed.  The backward writes grad_q (nf, n) and grad_box (nf x 9, may be
+ * NULL) for upstream gradients g_self, g_nonneutral (nf each, device, either
+ * may be NULL for zero). */
+int admp_ewald_self_terms(const double* q, int64_t nf, int64_t n, const double* box,
+                          double kappa, double dielectric, double* e_self, double* e_nonneutral,
+                          void* stream);
+int admp_ewald_self_terms_backward(const double* q, int64_t nf, int64_t n, const double* box,
+                                   double kappa, double dielectric, const double* g_self,
+                                   const double* g_nonneutral, double* grad_q, double* grad_box,
+                                   void* stream);
 
 /* dE/dbox of the reciprocal energy (the autograd box derivative through
  * recip.py:255-308 and the volume in pme.py:410-443): with A = box^-1,

@@ -123,8 +123,9 @@ def test_pair_ds_and_backward(dev, box, order):
                                       (np.eye(3) * 13.0, 5.0, 200), (TRICLINIC, 5.0, 300),
                                       (None, 4.0, 400)])
 def test_row_form_and_reverse_index(dev, box, rc, n):
-    """The search's row form: rows by first atom in ascending index with
-    row_ptr their starts, and reverse[o] the index of (j, i) for o = (i, j)
+    """The search's row form: one row per atom (cell order, empty rows
+    allowed) with row_ptr their starts, each row one first atom, and
+    reverse[o] the index of (j, i) for o = (i, j)
     (cell grids of 3+ cells per side, one cell with images, triclinic, open
     boundaries).  The distance backward through it (no per-pair atomics)
     against the oracle's autograd, with the buffer-scale zeros and random
@@ -137,9 +138,16 @@ def test_row_form_and_reverse_index(dev, box, rc, n):
     pairs = nbl(pos.to(dev), None if boxt is None else boxt.to(dev))
     row_ptr, rev = nbl.row_ptr.cpu(), nbl.reverse.cpu().long()
     pc = pairs.cpu()
-    assert bool((pc[1:, 0] >= pc[:-1, 0]).all())
+    assert row_ptr.shape[0] == n + 1 and int(row_ptr[0]) == 0 and int(row_ptr[-1]) == pc.shape[0]
+    lens = row_ptr[1:] - row_ptr[:-1]
+    assert bool((lens >= 0).all())
+    row_of = torch.repeat_interleave(torch.arange(n), lens)
+    first = torch.full((n,), -1, dtype=torch.long)
+    first[row_of] = pc[:, 0]
+    assert torch.equal(pc[:, 0], first[row_of])  # one first atom per row
     counts = torch.bincount(pc[:, 0], minlength=n)
-    assert torch.equal(row_ptr, torch.cat([torch.zeros(1, dtype=torch.long), counts.cumsum(0)]))
+    assert torch.equal(torch.sort(lens[lens > 0]).values, torch.sort(counts[counts > 0]).values)
+    assert len(torch.unique(first[lens > 0])) == int((lens > 0).sum())  # one row per atom
     assert bool((rev >= 0).all())
     assert torch.equal(pc[rev], pc[:, [1, 0]])
     assert torch.equal(rev[rev], torch.arange(pc.shape[0]))
@@ -357,3 +365,40 @@ def test_kmesh_rule_and_obc(dev):
     f_ref = -torch.autograd.grad(e_ref.sum(), pp)[0]
     assert abs(float(e) - float(e_ref)) < 1e-10 * abs(float(e_ref))
     assert torch.allclose(f.cpu(), f_ref, atol=1e-9)
+
+
+def test_ewald_self_terms_operator(dev):
+    """The fused self and non-neutral terms (two frames, non-neutral charges,
+    triclinic boxes) against the reference's torch expressions
+    (src/torch_admp/pme.py:448-493): energies, the backward kernel (no graph)
+    and the graph-building backward (second derivatives in q and the box)."""
+    rng = np.random.default_rng(9)
+    q = torch.tensor(rng.uniform(-1, 1, (2, 50)) + 0.3, device=dev)
+    box = torch.tensor(np.stack([np.array(TRICLINIC), np.array(SYMMETRIC) * 1.1]), device=dev)
+    kappa, eps = 0.37, 14.399645
+    w = torch.tensor([0.7, -1.3], device=dev, dtype=F64)
+
+    def ref(qq, bb):
+        e_self = -torch.sum(kappa / math.sqrt(math.pi) * qq ** 2, dim=-1) * eps
+        e_nn = -math.pi / (2 * torch.det(bb) * kappa ** 2) * eps * torch.sum(qq, dim=-1) ** 2
+        return e_self, e_nn
+
+    qa, ba = q.clone().requires_grad_(True), box.clone().requires_grad_(True)
+    qb, bb = q.clone().requires_grad_(True), box.clone().requires_grad_(True)
+    es, en = torch.ops.admp.ewald_self_terms(qa, ba, kappa, eps)
+    rs, rn = ref(qb, bb)
+    assert torch.allclose(es, rs, rtol=1e-13, atol=0) and torch.allclose(en, rn, rtol=1e-13, atol=0)
+    ga = torch.autograd.grad((es * w).sum() + (en * w * 2).sum(), [qa, ba])
+    gr = torch.autograd.grad((rs * w).sum() + (rn * w * 2).sum(), [qb, bb])
+    for a, b in zip(ga, gr):
+        assert torch.allclose(a, b, rtol=1e-12, atol=1e-12 * float(b.abs().max()))
+    # second derivatives through the graph-building backward
+    es, en = torch.ops.admp.ewald_self_terms(qa, ba, kappa, eps)
+    rs, rn = ref(qb, bb)
+    (gqa,) = torch.autograd.grad((es + en).sum(), qa, create_graph=True)
+    (gqb,) = torch.autograd.grad((rs + rn).sum(), qb, create_graph=True)
+    v = torch.tensor(rng.normal(size=(2, 50)), device=dev)
+    ha = torch.autograd.grad((gqa * v).sum(), [qa, ba])
+    hb = torch.autograd.grad((gqb * v).sum(), [qb, bb])
+    for a, b in zip(ha, hb):
+        assert torch.allclose(a, b, rtol=1e-12, atol=1e-12 * float(b.abs().max()))

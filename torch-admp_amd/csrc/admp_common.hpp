@@ -163,6 +163,16 @@ __device__ __forceinline__ double read_lane(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// Work-groups are dealt to the 8 XCDs round-robin by index (each XCD has its
+// own L2).  Renumbering them so that XCD x takes one contiguous range of
+// logical indices keeps neighbouring work (rows of nearby cells) in one L2.
+// A bijection of [0, nb).
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
+  const int64_t x = b % kNumXcd, per = nb / kNumXcd, rem = nb % kNumXcd;
+  return x * per + (x < rem ? x : rem) + b / kNumXcd;
+}
+
 // Sum over the 64 lanes, returned in every lane, in a fixed order: pairs
 // (quad_perm 1,0,3,2), quads (2,3,0,1), half rows (row_half_mirror), rows
 // (row_mirror), then row_bcast:15 adds row 0 into row 1 and row 2 into row 3,

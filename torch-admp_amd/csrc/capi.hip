@@ -261,7 +261,7 @@ int admp_nblist_workspace_bytes(int64_t n, const double* box, double cutoff, siz
   if (n < 0 || !(cutoff > 0.0) || !bytes) return ADMP_ERR_INVALID;
   if (box && !box_ok(box)) return ADMP_ERR_INVALID;
   NbLayout L;
-  nb_layout(n, make_grid(box, cutoff, box == nullptr), &L);
+  nb_layout(n, make_grid(box, cutoff, box == nullptr), &L, true);
   *bytes = L.bytes;
   return ADMP_OK;
 }
@@ -274,7 +274,7 @@ int admp_nblist_count(const double* pos, int64_t n, const double* box, double cu
   if (box && !box_ok(box)) return ADMP_ERR_INVALID;
   const CellGrid cg = make_grid(box, cutoff, box == nullptr);
   NbLayout L;
-  nb_layout(n, cg, &L);
+  nb_layout(n, cg, &L, true);
   if (ws_bytes < L.bytes) return ADMP_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)workspace;
@@ -304,7 +304,7 @@ int admp_nblist_fill(const double* pos, int64_t n, const double* box, double cut
   if (box && !box_ok(box)) return ADMP_ERR_INVALID;
   const CellGrid cg = make_grid(box, cutoff, box == nullptr);
   NbLayout L;
-  nb_layout(n, cg, &L);
+  nb_layout(n, cg, &L, true);
   if (ws_bytes < L.bytes) return ADMP_ERR_INVALID;
   (void)pos;
   hipStream_t s = (hipStream_t)stream;
@@ -469,6 +469,24 @@ int admp_pme_box_grad(const double* box, const double* dEdA_k, const double* dEd
   if (!box_ok(box) || !dEdA_k || !energy || !grad_box) return ADMP_ERR_INVALID;
   return hip_status(launch_box_grad(make_geom(box, nullptr), dEdA_k, dEdA_gather, energy,
                                     grad_box, (hipStream_t)stream));
+}
+
+int admp_ewald_self_terms(const double* q, int64_t nf, int64_t n, const double* box,
+                          double kappa, double dielectric, double* e_self, double* e_nonneutral,
+                          void* stream) {
+  if (nf < 0 || n < 0 || (nf > 0 && (!box || !e_self || !e_nonneutral || (n > 0 && !q))))
+    return ADMP_ERR_INVALID;
+  return hip_status(launch_self_terms(q, nf, n, box, kappa, dielectric, e_self, e_nonneutral,
+                                      nullptr, nullptr, nullptr, nullptr, (hipStream_t)stream));
+}
+
+int admp_ewald_self_terms_backward(const double* q, int64_t nf, int64_t n, const double* box,
+                                   double kappa, double dielectric, const double* g_self,
+                                   const double* g_nonneutral, double* grad_q, double* grad_box,
+                                   void* stream) {
+  if (nf < 0 || n < 0 || (nf > 0 && (!box || !grad_q || (n > 0 && !q)))) return ADMP_ERR_INVALID;
+  return hip_status(launch_self_terms(q, nf, n, box, kappa, dielectric, nullptr, nullptr, g_self,
+                                      g_nonneutral, grad_q, grad_box, (hipStream_t)stream));
 }
 
 // --------------------------------------------------------------------- QEq

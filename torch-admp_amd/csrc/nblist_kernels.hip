@@ -143,11 +143,11 @@ __global__ void __launch_bounds__(kBlock) k_nb_rank(int64_t n, const int* __rest
 // the square root only within 1e-12 (relative) of rc^2 or 1e-4, where it
 // decides exactly as sqrt(d2) < rc and d > 0.01 do.  Both passes enumerate the
 // pairs in the same order (stencil entry, then the candidate's cell slot), so
-// pass 1 writes each home atom's pairs at the offset pass 0 counted: rows
-// grouped by first atom in ascending atom index, each row in ascending
-// (stencil entry, slot).  With `keys`, pass 1 also stores that order as
-// keys[o] = entry << 24 | slot of the second atom, which locates the reverse
-// pair (k_nb_reverse).
+// pass 1 writes each home atom's pairs at the offset pass 0 counted: one row
+// per atom in cell order (row h = the atom of sorted slot h), each row in
+// ascending (stencil entry, slot).  With `keys`, pass 1 also stores that order
+// as keys[o] = entry << 24 | slot of the second atom, which locates the
+// reverse pair (k_nb_reverse).
 constexpr int kNbThreads = 512;
 constexpr int kNbCand = 2048;     // staged candidates per chunk
 constexpr int kNbHomeBlk = 256;   // home atoms whose counters sit in LDS at once
@@ -207,8 +207,7 @@ __global__ void __launch_bounds__(kNbThreads) k_nb_cell_pairs(
 
   for (int h0 = hb; h0 < he; h0 += kNbHomeBlk) {
     const int nh = min(kNbHomeBlk, he - h0);
-    for (int t = threadIdx.x; t < nh; t += kNbThreads)
-      hcnt[t] = MODE == 1 ? offsets[sorted[h0 + t]] : 0;
+    for (int t = threadIdx.x; t < nh; t += kNbThreads) hcnt[t] = MODE == 1 ? offsets[h0 + t] : 0;
     for (int st0 = 0; st0 < nst; st0 += kMaxStencil) {
       __syncthreads();
       const int m = load_stencil_chunk(cg, 0, ci, st0, nst, start, st_pre, st_beg, st_sh);
@@ -294,7 +293,7 @@ __global__ void __launch_bounds__(kNbThreads) k_nb_cell_pairs(
     }
     __syncthreads();
     if (MODE == 0)
-      for (int t = threadIdx.x; t < nh; t += kNbThreads) pcount[sorted[h0 + t]] = hcnt[t];
+      for (int t = threadIdx.x; t < nh; t += kNbThreads) pcount[h0 + t] = hcnt[t];
   }
 }
 
@@ -371,7 +370,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(const I* __restrict__ in,
 // Single-block exclusive scan for short arrays (n <= kScanSingleMax); writes
 // out[0..n] with out[n] = total.
 constexpr int kScanSingleThreads = 1024;
-constexpr int kScanSingleItems = 16;
+constexpr int kScanSingleItems = 32;
 constexpr int64_t kScanSingleMax = (int64_t)kScanSingleThreads * kScanSingleItems;
 
 template <typename I>
@@ -567,28 +566,58 @@ __global__ void __launch_bounds__(kBlock) k_pair_ds_bwd(const double* __restrict
   }
 }
 
-// Reverse index of an ordered pair list built by k_nb_cell_pairs (rows by
-// first atom, row i = [off[i], off[i + 1]), each row in ascending key order,
-// keys[o] = stencil entry << 24 | slot of the second atom).  The pair (j, i)
-// of o = (i, j) lies in row j with the mirrored entry nst - 1 - e (the
-// stencil is symmetric about its centre) and the slot of i, found by binary
-// search; rev[o] = its index, -1 if absent.  One wavefront per home slot h.
+// Stencil-entry starts of every row: es[i * (nst + 1) + e] = offset within
+// row i of its first pair whose entry is >= e (keys[o] >> 24), es[.. + nst] =
+// the row length.  One wavefront per row; lane k writes the starts of the
+// entries between the previous pair's entry and its own.
+__global__ void __launch_bounds__(kBlock) k_nb_entry_starts(int64_t n, int nst,
+                                                            const int64_t* __restrict__ off,
+                                                            const int* __restrict__ keys,
+                                                            int* __restrict__ es) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= n) return;  // whole wavefronts
+  const int64_t b = off[i], e = off[i + 1];
+  int* row = es + i * (int64_t)(nst + 1);
+  int prev = -1;  // entry of the pair before this chunk
+  for (int64_t o0 = b; o0 < e; o0 += 64) {
+    const int64_t o = o0 + lane;
+    const int ent = o < e ? keys[o] >> 24 : nst;
+    int pe = __shfl_up(ent, 1, 64);
+    if (lane == 0) pe = prev;
+    if (o < e)
+      for (int x = pe + 1; x <= ent; ++x) row[x] = (int)(o - b);
+    prev = __shfl(ent, (int)min<int64_t>(63, e - 1 - o0), 64);
+  }
+  for (int x = prev + 1 + lane; x <= nst; x += 64) row[x] = (int)(e - b);
+}
+
+// Reverse index of an ordered pair list built by k_nb_cell_pairs: row h (the
+// atom of cell slot h) = [off[h], off[h + 1]), each row in ascending key
+// order, keys[o] = stencil entry << 24 | slot of the second atom.  The pair
+// (j, i) of o = (i, j) lies in the row of j's slot among the pairs of the
+// mirrored entry nst - 1 - e (the stencil is symmetric about its centre; es
+// gives where that entry's pairs start), with the slot of i: a binary search
+// over that group of a few pairs.  rev[o] = its index, -1 if absent.  One
+// wavefront per row; rows of one cell region stay on one XCD, so the rows
+// searched (nearby cells) are mostly in its L2.
 __global__ void __launch_bounds__(kBlock) k_nb_reverse(int64_t n, int nst,
-                                                       const int* __restrict__ sorted,
                                                        const int64_t* __restrict__ off,
                                                        const int* __restrict__ keys,
+                                                       const int* __restrict__ es,
                                                        int* __restrict__ rev) {
   const int lane = threadIdx.x & 63;
-  const int64_t h = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t h = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (h >= n) return;  // whole wavefronts
-  const int i = sorted[h];
-  const int64_t b = off[i], e = off[i + 1];
+  const int64_t b = off[h], e = off[h + 1];
   for (int64_t o = b + lane; o < e; o += 64) {
     const int k = keys[o];
     const int sj = k & 0xffffff;
-    const int target = ((nst - 1 - (k >> 24)) << 24) | (int)h;
-    const int j = sorted[sj];
-    int64_t lo = off[j], hi = off[j + 1];
+    const int em = nst - 1 - (k >> 24);
+    const int target = (em << 24) | (int)h;
+    const int* ej = es + (int64_t)sj * (nst + 1);
+    const int64_t rj = off[sj];
+    int64_t lo = rj + ej[em], hi = rj + ej[em + 1];
     int found = -1;
     if (hi > lo) {
       while (hi - lo > 1) {
@@ -602,15 +631,18 @@ __global__ void __launch_bounds__(kBlock) k_nb_reverse(int64_t n, int nst,
   }
 }
 
-// dL/dr and dL/dbox of L(ds) for a list in row form: row i holds the pairs
-// whose first atom is i (row_ptr, n + 1 entries) and rev[o] is the index of
-// the reverse pair (j, i) of o = (i, j), or -1.  The second-atom term of pair
-// rev(o) acts on atom i along -u_rev = u_o, so row i sums
-// (g_o + g_rev(o)) d ds_o / d r_i over its own pairs: one wavefront per row,
-// one float64 atomic per row component (grad_pos zeroed by the caller), no
-// per-pair atomics.  A pair without a reverse adds its second-atom term by
-// atomics.  The box term (image pairs, each pair's own g) goes to
-// partial[block][9] as in k_pair_ds_bwd.
+// dL/dr and dL/dbox of L(ds) for a list in row form: row r = pairs
+// [row_ptr[r], row_ptr[r + 1]) share their first atom i (nrow rows, some
+// empty) and rev[o] is the index of the reverse pair (j, i) of o = (i, j), or
+// -1.  The second-atom term of pair rev(o) acts on atom i along -u_rev = u_o,
+// so row r sums (g_o + g_rev(o)) d ds_o / d r_i over its own pairs: one
+// wavefront per row, one float64 atomic per row component (grad_pos zeroed by
+// the caller), no per-pair atomics.  A pair without a reverse adds its
+// second-atom term by atomics.  Each wavefront takes a contiguous range of
+// rows and each XCD a contiguous range of wavefronts, so the reverse pairs
+// read (rows of nearby atoms in a cell-ordered list) stay in its L2.  The box
+// term (image pairs, each pair's own g) goes to partial[block][9] as in
+// k_pair_ds_bwd.
 __global__ void __launch_bounds__(kBlock) k_pair_ds_bwd_rows(
     const double* __restrict__ gds, const int64_t* __restrict__ pairs, int64_t n,
     const int64_t* __restrict__ row_ptr, const int* __restrict__ rev,
@@ -619,11 +651,15 @@ __global__ void __launch_bounds__(kBlock) k_pair_ds_bwd_rows(
   __shared__ double red[9 * (kBlock / 64)];
   double gb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const int lane = threadIdx.x & 63;
-  const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int64_t w = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t per = (n + nw - 1) / nw;
+  const int64_t r0 = w * per, r1 = min(n, r0 + per);
   const double si = cg.obc ? -1.0 : 1.0;  // PBC: v = r_i - r_j - n h; OBC: v = r_j - r_i
-  for (int64_t i = w0; i < n; i += nw) {
-    const int64_t b = row_ptr[i], e = row_ptr[i + 1];
+  for (int64_t r = r0; r < r1; ++r) {
+    const int64_t b = row_ptr[r], e = row_ptr[r + 1];
+    if (e <= b) continue;  // wave-uniform
+    const int64_t i = pairs[2 * b];
     double sx = 0.0, sy = 0.0, sz = 0.0;
     for (int64_t o = b + lane; o < e; o += 64) {
       const double d = ds[o];
@@ -660,7 +696,7 @@ __global__ void __launch_bounds__(kBlock) k_pair_ds_bwd_rows(
               for (int k = 0; k < 3; ++k) gb[c * 3 + k] -= g * nimg[c] * v[k] / d;
       }
     }
-    if (e > b) {
+    {
       sx = wave_sum(sx);
       sy = wave_sum(sy);
       sz = wave_sum(sz);
@@ -831,7 +867,7 @@ hipError_t launch_max_disp(const double* pos, const double* ref, int64_t n, cons
 // ---------------------------------------------------------------------------
 // host launchers
 
-void nb_layout(int64_t n, const CellGrid& cg, NbLayout* L) {
+void nb_layout(int64_t n, const CellGrid& cg, NbLayout* L, bool with_entries) {
   size_t off = 0;
   auto take = [&](size_t bytes) {
     const size_t o = off;
@@ -850,6 +886,10 @@ void nb_layout(int64_t n, const CellGrid& cg, NbLayout* L) {
   L->offsets = take(sizeof(int64_t) * (size_t)(n + 1));
   const size_t sc = std::max(scan_scratch_count<int64_t>(n + 1), scan_scratch_count<int>(nkeys));
   L->scan = take(sizeof(int64_t) * sc);
+  L->entry = with_entries
+                 ? take(sizeof(int) * (size_t)std::max<int64_t>(n, 1) *
+                        (size_t)(nb_stencil_entries(cg) + 1))
+                 : 0;
   L->bytes = off;
 }
 
@@ -925,7 +965,7 @@ bool nb_reverse_supported(int64_t n, const CellGrid& cg, int64_t np) {
   return n < ((int64_t)1 << 24) && nb_stencil_entries(cg) <= 127 && np < ((int64_t)1 << 31);
 }
 
-hipError_t launch_nb_reverse(int64_t n, const CellGrid& cg, const char* ws, const NbLayout& L,
+hipError_t launch_nb_reverse(int64_t n, const CellGrid& cg, char* ws, const NbLayout& L,
                              const int* keys, int64_t np, int64_t* row_ptr, int* rev,
                              hipStream_t s) {
   const int64_t* off = (const int64_t*)(ws + L.offsets);
@@ -934,13 +974,15 @@ hipError_t launch_nb_reverse(int64_t n, const CellGrid& cg, const char* ws, cons
                        s) != hipSuccess)
       return hipGetLastError();
   if (rev && np > 0) {
-    if (!keys || !nb_reverse_supported(n, cg, np)) {
+    if (!keys || !L.entry || !nb_reverse_supported(n, cg, np)) {
       k_fill<int><<<(unsigned)std::min<int64_t>((np + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(
           rev, np, -1);
     } else {
-      const int64_t threads = n * 64;
-      k_nb_reverse<<<(unsigned)((threads + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-          n, nb_stencil_entries(cg), (const int*)(ws + L.sorted), off, keys, rev);
+      const unsigned nb = (unsigned)((n * 64 + kBlock - 1) / kBlock);
+      int* es = (int*)(ws + L.entry);
+      const int nst = nb_stencil_entries(cg);
+      k_nb_entry_starts<<<nb, kBlock, 0, s>>>(n, nst, off, keys, es);
+      k_nb_reverse<<<nb, kBlock, 0, s>>>(n, nst, off, keys, es, rev);
     }
   }
   return hipGetLastError();

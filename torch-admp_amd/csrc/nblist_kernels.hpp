@@ -48,11 +48,13 @@ struct alignas(4 * sizeof(T)) P4 {
 
 // byte offsets of the neighbour-list workspace regions
 struct NbLayout {
-  size_t w, cell_of, count, start, cursor, sorted, unsorted, pcount, offsets, scan, bytes;
+  size_t w, cell_of, count, start, cursor, sorted, unsorted, pcount, offsets, scan, entry, bytes;
 };
 
 CellGrid make_grid(const double* box, double cutoff, bool obc);
-void nb_layout(int64_t n, const CellGrid& cg, NbLayout* L);
+// with_entries: also room for the per-row stencil-entry starts of the
+// reverse index (n * (entries + 1) int32)
+void nb_layout(int64_t n, const CellGrid& cg, NbLayout* L, bool with_entries = false);
 
 hipError_t launch_bin(const double* pos, int64_t n, const CellGrid& cg, char* ws,
                       const NbLayout& L, hipStream_t s);
@@ -77,8 +79,9 @@ int nb_stencil_entries(const CellGrid& cg);
 bool nb_reverse_supported(int64_t n, const CellGrid& cg, int64_t np);
 // after mode 1 with keys: row_ptr (n + 1, may be NULL) = the row starts, and
 // rev[o] (may be NULL) = index of the reverse pair of o, -1 if absent (every
-// entry -1 when keys is NULL or the list is not supported)
-hipError_t launch_nb_reverse(int64_t n, const CellGrid& cg, const char* ws, const NbLayout& L,
+// entry -1 when keys is NULL, the layout has no entry region or the list is
+// not supported)
+hipError_t launch_nb_reverse(int64_t n, const CellGrid& cg, char* ws, const NbLayout& L,
                              const int* keys, int64_t np, int64_t* row_ptr, int* rev,
                              hipStream_t s);
 hipError_t launch_pair_ds(const double* pos, const int64_t* pairs, int64_t np,

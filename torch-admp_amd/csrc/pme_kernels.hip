@@ -450,6 +450,77 @@ hipError_t launch_box_grad(const Geom& g, const double* dA1, const double* dA2,
   return hipGetLastError();
 }
 
+// Self and non-neutral Ewald terms (src/torch_admp/pme.py:448-493) of nf
+// frames of n charges with device boxes (row-major, 9 per frame):
+//   e_self = -eps kappa / sqrt(pi) sum q^2,  e_nn = -eps pi Q^2 / (2 V kappa^2),
+// Q = sum q, V = det(box).  With g_self / g_nn given (the upstream gradients,
+// either may be NULL for zero) the same kernel writes the backward instead:
+//   dE/dq_i = g_self (-2 eps kappa / sqrt(pi)) q_i + g_nn (-eps pi Q / (V kappa^2)),
+//   dE/dbox = g_nn (-e_nn / V) cof(box)   (d det / d box = the cofactor matrix).
+// One work-group per frame; the two sums in a fixed order.
+__global__ void __launch_bounds__(kBlock) k_self_terms(const double* __restrict__ q, int64_t n,
+                                                       const double* __restrict__ box,
+                                                       double kappa, double eps,
+                                                       double* __restrict__ e_self,
+                                                       double* __restrict__ e_nn,
+                                                       const double* __restrict__ g_self,
+                                                       const double* __restrict__ g_nn,
+                                                       double* __restrict__ dq,
+                                                       double* __restrict__ dbox) {
+  __shared__ double red[2 * (kBlock / 64)];
+  __shared__ double bcast[2];
+  const int64_t f = blockIdx.x;
+  const double* qf = q + f * n;
+  double acc[2] = {0.0, 0.0};
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = qf[i];
+    acc[0] += v * v;
+    acc[1] += v;
+  }
+  block_sum<2>(acc, red);
+  const double* h = box + 9 * f;
+  double cof[9];
+  cof[0] = h[4] * h[8] - h[5] * h[7];
+  cof[1] = h[5] * h[6] - h[3] * h[8];
+  cof[2] = h[3] * h[7] - h[4] * h[6];
+  cof[3] = h[2] * h[7] - h[1] * h[8];
+  cof[4] = h[0] * h[8] - h[2] * h[6];
+  cof[5] = h[1] * h[6] - h[0] * h[7];
+  cof[6] = h[1] * h[5] - h[2] * h[4];
+  cof[7] = h[2] * h[3] - h[0] * h[5];
+  cof[8] = h[0] * h[4] - h[1] * h[3];
+  const double vol = h[0] * cof[0] + h[1] * cof[1] + h[2] * cof[2];
+  const double cs = -eps * kappa / kSqrtPi;
+  const double cn = -eps * kPi / (2.0 * vol * kappa * kappa);
+  if (threadIdx.x == 0) {
+    bcast[0] = acc[0];
+    bcast[1] = acc[1];
+  }
+  __syncthreads();
+  const double Q = bcast[1];
+  if (!dq) {
+    if (threadIdx.x == 0) {
+      e_self[f] = cs * bcast[0];
+      e_nn[f] = cn * Q * Q;
+    }
+    return;
+  }
+  const double gs = g_self ? g_self[f] : 0.0, gn = g_nn ? g_nn[f] : 0.0;
+  const double a = gs * 2.0 * cs, b = gn * 2.0 * cn * Q;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) dq[f * n + i] = a * qf[i] + b;
+  if (dbox && threadIdx.x < 9) dbox[9 * f + threadIdx.x] = gn * (-(cn * Q * Q) / vol) * cof[threadIdx.x];
+}
+
+hipError_t launch_self_terms(const double* q, int64_t nf, int64_t n, const double* box,
+                             double kappa, double eps, double* e_self, double* e_nn,
+                             const double* g_self, const double* g_nn, double* dq, double* dbox,
+                             hipStream_t s) {
+  if (nf > 0)
+    k_self_terms<<<(unsigned)nf, kBlock, 0, s>>>(q, n, box, kappa, eps, e_self, e_nn, g_self,
+                                                 g_nn, dq, dbox);
+  return hipGetLastError();
+}
+
 size_t convolve_partials_count(const int32_t* k) {
   const int Kzh = k[2] / 2 + 1;
   // the chunk picked for pass 3 is at least 1, so this bounds the block count

@@ -38,6 +38,12 @@ hipError_t launch_convolve(const T* mesh, const Geom& g, int conv, double kappa,
 // out (9) = -A^T (dA1 + dA2) A^T - energy A^T with A = box^-1 (dA2 may be NULL)
 hipError_t launch_box_grad(const Geom& g, const double* dA1, const double* dA2,
                            const double* energy, double* out, hipStream_t s);
+// self and non-neutral Ewald terms per frame (dq == NULL), or their
+// backward into dq (nf, n) and dbox (nf, 9; may be NULL)
+hipError_t launch_self_terms(const double* q, int64_t nf, int64_t n, const double* box,
+                             double kappa, double eps, double* e_self, double* e_nn,
+                             const double* g_self, const double* g_nn, double* dq, double* dbox,
+                             hipStream_t s);
 size_t convolve_partials_count(const int32_t* k);
 
 }  // namespace admp

@@ -61,6 +61,11 @@ _SIGNATURES = {
     "admp_pme_convolve": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_i32, _c_i32, _c_dbl, _c_dbl,
                                          _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp, _c_sz, _c_vp]),
     "admp_pme_box_grad": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "admp_ewald_self_terms": (ctypes.c_int, [_c_vp, _c_i64, _c_i64, _c_vp, _c_dbl, _c_dbl, _c_vp,
+                                             _c_vp, _c_vp]),
+    "admp_ewald_self_terms_backward": (ctypes.c_int, [_c_vp, _c_i64, _c_i64, _c_vp, _c_dbl,
+                                                      _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp,
+                                                      _c_vp]),
     "admp_pme_gather": (ctypes.c_int, [_c_vp, _c_vp, _c_i64, _c_vp, _c_vp, _c_i32, _c_vp, _c_vp,
                                        _c_vp, _c_vp, _c_i32, _c_vp, _c_sz, _c_vp]),
     "admp_pme_fused_workspace_bytes": (ctypes.c_int, [_c_i64, _c_vp, _c_dbl, _c_vp, _c_i32,

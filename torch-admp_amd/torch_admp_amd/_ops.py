@@ -35,6 +35,7 @@ into them are ignored.
 from __future__ import annotations
 
 import ctypes
+import math
 from typing import List, Optional, Tuple
 
 import torch
@@ -121,9 +122,10 @@ def neighbor_pairs(positions: Tensor, box: Optional[Tensor], cutoff: float,
 def neighbor_list(positions: Tensor, box: Optional[Tensor], cutoff: float,
                   box_host: Optional[Tensor]) -> Tuple[Tensor, Tensor, Tensor]:
     """The pairs of neighbor_pairs with their row form: row_ptr (n + 1,
-    int64; row i = pairs[row_ptr[i]:row_ptr[i + 1]], first atom i) and
-    reverse (P, int32; reverse[o] = index of the pair (j, i) of o = (i, j)).
-    pair_distance given both runs its backward without per-pair atomics."""
+    int64; n rows, one per atom in cell order, row r = pairs[row_ptr[r]:
+    row_ptr[r + 1]] sharing one first atom, some empty) and reverse (P, int32;
+    reverse[o] = index of the pair (j, i) of o = (i, j)).  pair_distance
+    given both runs its backward without per-pair atomics."""
     return _search(positions, box, cutoff, box_host, True)
 
 
@@ -348,6 +350,83 @@ def _peg_backward(ctx, g_gq, g_gds, g_geta):
 torch.library.register_autograd("admp::pair_energy", _pe_backward, setup_context=_pe_setup)
 torch.library.register_autograd("admp::pair_energy_grads", _peg_backward,
                                 setup_context=_peg_setup)
+
+
+# ------------------------------------------------------------------ self and non-neutral terms
+@torch.library.custom_op("admp::ewald_self_terms", mutates_args=())
+def ewald_self_terms(q: Tensor, box: Tensor, kappa: float, eps: float) -> Tuple[Tensor, Tensor]:
+    """Per frame (q (nf, n), box (nf, 3, 3) on the device): the self energy
+    -eps kappa / sqrt(pi) sum q^2 and the non-neutral energy
+    -eps pi Q^2 / (2 V kappa^2) (src/torch_admp/pme.py:448-493), in one kernel
+    with no host copy of the box."""
+    _lib.require_gpu(q, box)
+    q64, b64 = _f64(q), _f64(box)
+    nf, n = q64.shape
+    e_self = torch.empty(nf, dtype=_F64, device=q64.device)
+    e_nn = torch.empty(nf, dtype=_F64, device=q64.device)
+    _lib.check(_lib.load().admp_ewald_self_terms(_lib.ptr(q64), nf, n, _lib.ptr(b64), kappa, eps,
+                                                 _lib.ptr(e_self), _lib.ptr(e_nn),
+                                                 _lib.stream_of(q64)), "admp_ewald_self_terms")
+    return e_self.to(q.dtype), e_nn.to(q.dtype)
+
+
+@torch.library.custom_op("admp::ewald_self_terms_backward", mutates_args=())
+def ewald_self_terms_backward(q: Tensor, box: Tensor, kappa: float, eps: float,
+                              g_self: Optional[Tensor], g_nn: Optional[Tensor]
+                              ) -> Tuple[Tensor, Tensor]:
+    """(dL/dq (nf, n), dL/dbox (nf, 3, 3)) of ewald_self_terms for upstream
+    gradients g_self, g_nn (nf each; None for zero)."""
+    _lib.require_gpu(q, box)
+    q64, b64 = _f64(q), _f64(box)
+    nf, n = q64.shape
+    dq = torch.empty(nf, n, dtype=_F64, device=q64.device)
+    dbox = torch.empty(nf, 3, 3, dtype=_F64, device=q64.device)
+    gs = None if g_self is None else _f64(g_self).reshape(nf)
+    gn = None if g_nn is None else _f64(g_nn).reshape(nf)
+    _lib.check(_lib.load().admp_ewald_self_terms_backward(
+        _lib.ptr(q64), nf, n, _lib.ptr(b64), kappa, eps, _lib.ptr(gs), _lib.ptr(gn),
+        _lib.ptr(dq), _lib.ptr(dbox), _lib.stream_of(q64)), "admp_ewald_self_terms_backward")
+    return dq.to(q.dtype), dbox.to(box.dtype)
+
+
+def _st_setup(ctx, inputs, output):
+    q, box, kappa, eps = inputs
+    ctx.save_for_backward(q, box)
+    ctx.args = (kappa, eps)
+    ctx.set_materialize_grads(False)
+
+
+def _st_backward(ctx, g_self, g_nn):
+    q, box = ctx.saved_tensors
+    kappa, eps = ctx.args
+    if g_self is None and g_nn is None:
+        return None, None, None, None
+    if torch.is_grad_enabled():
+        # a graph of the gradient is being built (second derivatives, as the
+        # reference's Hessian takes them): the same formulas as torch
+        # expressions, differentiable in q, box and the upstream gradients
+        cs = -eps * kappa / math.sqrt(math.pi)
+        ra, rb, rc = box[:, 0, :], box[:, 1, :], box[:, 2, :]
+        # d det / d box row by row: b x c, c x a, a x b (the cofactor matrix)
+        dvol = torch.stack([torch.cross(rb, rc, dim=-1), torch.cross(rc, ra, dim=-1),
+                            torch.cross(ra, rb, dim=-1)], dim=1)
+        vol = torch.sum(ra * dvol[:, 0, :], dim=-1)
+        qt = torch.sum(q, dim=-1)
+        cn = -eps * math.pi / (2.0 * vol * kappa * kappa)
+        dq = torch.zeros_like(q)
+        dbox = torch.zeros_like(box)
+        if g_self is not None:
+            dq = dq + (2.0 * cs) * g_self.unsqueeze(-1) * q
+        if g_nn is not None:
+            dq = dq + (2.0 * g_nn * cn * qt).unsqueeze(-1)
+            dbox = dbox - (g_nn * cn * qt * qt / vol).reshape(-1, 1, 1) * dvol
+    else:
+        dq, dbox = torch.ops.admp.ewald_self_terms_backward(q, box, kappa, eps, g_self, g_nn)
+    need = ctx.needs_input_grad
+    return (dq if need[0] else None, dbox if need[1] else None, None, None)
+
+
+torch.library.register_autograd("admp::ewald_self_terms", _st_backward, setup_context=_st_setup)
 
 
 # ------------------------------------------------------------------ PME reciprocal space

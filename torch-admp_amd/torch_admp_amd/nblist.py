@@ -7,7 +7,8 @@ reference criterion 0.01 < d < rc); ``ds`` comes from the pair-distance kernel
 and stays differentiable in positions and box.
 """
 
-from typing import Optional
+import math
+from typing import List, Optional
 
 import torch
 
@@ -15,8 +16,32 @@ from . import _lib
 from . import _ops  # noqa: F401  (registers torch.ops.admp)
 
 
+def _min_half_height(rows: List[float]) -> float:
+    """Half the smallest distance between opposite faces of the cell with
+    rows a, b, c (row-major floats): V / |b x c| etc."""
+    a, b, c = rows[0:3], rows[3:6], rows[6:9]
+
+    def cross(u: List[float], v: List[float]) -> List[float]:
+        return [u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]]
+
+    heights: List[float] = []
+    for x, (u, v) in zip((a, b, c), ((b, c), (c, a), (a, b))):
+        nrm = cross(u, v)
+        length = math.sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2])
+        heights.append(abs(x[0] * nrm[0] + x[1] * nrm[1] + x[2] * nrm[2]) / length)
+    return min(heights) / 2
+
+
 def check_cutoff(box: torch.Tensor, cutoff: float) -> None:
     """Assert rc <= half the smallest face height (src/torch_admp/nblist.py:620-660)."""
+    if not torch.jit.is_scripting():
+        # the same heights from the nine values as Python floats (no tensor ops)
+        min_half = _min_half_height(box.detach().reshape(9).to(torch.device("cpu"),
+                                                               torch.float64).tolist())
+        assert cutoff <= min_half, (
+            f"Cutoff {cutoff} is larger than half the minimum height {min_half} of the box. "
+            "This may lead to unphysical results.")
+        return
     b = box.detach().to(torch.device("cpu"), torch.float64).reshape(3, 3)
     normals = torch.stack([torch.cross(b[1], b[2], dim=-1), torch.cross(b[2], b[0], dim=-1),
                            torch.cross(b[0], b[1], dim=-1)])

@@ -150,8 +150,12 @@ class CoulombForceModule(BaseForceModule):
             terms.append(self.real_energy)
         if self.kspace_flag:
             self.reciprocal_energy = self._forward_pbc_reciprocal(charges, positions, box)
-            self.self_energy = self._forward_pbc_self(charges)
-            self.non_neutral_energy = self._forward_pbc_non_neutral(charges, box)
+            # the terms of _forward_pbc_self and _forward_pbc_non_neutral in one
+            # kernel (and one backward kernel)
+            e_self, e_nn = torch.ops.admp.ewald_self_terms(charges, box, self.kappa,
+                                                           self.const_lib.dielectric)
+            self.self_energy = e_self
+            self.non_neutral_energy = e_nn
             terms.append(self.reciprocal_energy)
             terms.append(self.self_energy)
             terms.append(self.non_neutral_energy)
@@ -192,7 +196,10 @@ class CoulombForceModule(BaseForceModule):
                                                     self._precision_code, self._convention_code,
                                                     need, box_host)
             out.append(e)
-            self._kmesh = torch.tensor(km, dtype=torch.long)
+            if torch.jit.annotate(List[int], self._kmesh.tolist()) != km:
+                self._kmesh = torch.tensor(km, dtype=torch.long)
+        if len(out) == 1:
+            return out[0].reshape(1)
         return torch.stack(out)
 
     def _forward_pbc_self(self, charges: torch.Tensor) -> torch.Tensor:
