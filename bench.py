@@ -56,6 +56,9 @@ def parse():
                     help="run only the 3000-atom QEq timing and print its JSON (for profiling)")
     ap.add_argument("--batch-copies", type=int, default=64,
                     help="copies of the water box in the batched roofline run (0 skips it)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched water-box roofline run and print its JSON "
+                         "(for per-kernel counters)")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay timing")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device copy of this many MiB (a known-size kernel "
@@ -514,6 +517,19 @@ def main():
     box = torch.tensor(box_np, dtype=torch.float64, device=dev)
     box_h = torch.tensor(box_np, dtype=torch.float64)  # launch parameter, kept on the host
 
+    if args.batched_only:
+        if args.calib_copy_mib > 0:
+            src = torch.empty(args.calib_copy_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
+            dst = torch.empty_like(src)
+            torch.cuda.synchronize()
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            del src, dst
+        kappa = CoulombForceModule(rc, ethresh, kmesh=kmesh).kappa
+        print(json.dumps({"roofline_batched": batched_roofline(
+            max(args.batch_copies, 1), pos_np, q_np, box_np, rc, kappa, [kmesh] * 3, torch,
+            dev)}), flush=True)
+        return
     if args.calib_copy_mib > 0:
         src = torch.empty(args.calib_copy_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
