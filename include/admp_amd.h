@@ -253,6 +253,20 @@ int admp_pme_batch(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
                    const double* q, double* energy, double* forces, double* dEdbox,
                    void* workspace, size_t ws_bytes, void* stream);
 
+/* admp_pme_batch with flags.  ADMP_BATCH_NO_OVERLAP keeps every kernel on
+ * `stream`: admp_pme_batch forks the real-space kernel onto one side stream
+ * per device, which callers running batches on several streams at once would
+ * share (their batches then wait on each other's real space); such callers
+ * (a serving loop with batches in flight on two streams) pass this flag and
+ * get their overlap from the concurrent batches instead.  No counterpart in
+ * the reference. */
+#define ADMP_BATCH_NO_OVERLAP 1
+int admp_pme_batch_ex(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                      const double* cutoffs, const double* kappas, const int32_t* kmesh,
+                      int32_t precision, int32_t convention, double dielectric, const double* pos,
+                      const double* q, double* energy, double* forces, double* dEdbox,
+                      int32_t flags, void* workspace, size_t ws_bytes, void* stream);
+
 /* Graph capture of a batch (no counterpart in the reference; for callers that
  * replay one batch composition many times under a HIP graph).  admp_pme_batch
  * copies the system table from host memory on every call, which a captured

@@ -197,3 +197,19 @@ def test_batch_box_far_below_cutoff(dev, precision, etol):
         p, b, c, pairs, lambda pp, bb: nblist_ref.multi_image_pairs(pp, bb, rc)[1], bs, orc)
     assert abs(float(e[0, 4]) - float(e_ref)) <= etol * abs(float(e_ref))
     assert float(((f[0].cpu() - f_ref) ** 2).mean().sqrt()) < (1e-8 if precision == "float64" else 1e-4)
+
+
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-12), ("float32", 1e-6)])
+def test_batch_without_overlap_equals_overlap(dev, precision, tol):
+    """ADMP_BATCH_NO_OVERLAP (every kernel on the caller's stream, as the
+    serving loop runs its batches) gives the results of the default launch,
+    whose real-space kernel runs on the side stream."""
+    systems = [s for s in _mix(41) if len(s[1])]
+    plan = ops.PMEBatchPlan([len(s[1]) for s in systems], [s[2] for s in systems], RC,
+                            [s[4] for s in systems], [s[3] for s in systems], precision)
+    pos = torch.tensor(np.concatenate([s[0] for s in systems]), device=dev)
+    q = torch.tensor(np.concatenate([s[1] for s in systems]), device=dev)
+    a = plan.run(pos, q)
+    b = plan.run(pos, q, overlap=False)
+    for x, y in zip(a, b):
+        assert float((x - y).abs().max()) <= tol * max(1.0, float(x.abs().max()))

@@ -616,7 +616,8 @@ static int batch_call(int32_t n_sys, const int64_t* n_atoms, const double* boxes
                       const double* cutoffs, const double* kappas, const int32_t* kmesh,
                       int32_t precision, int32_t convention, double dielectric,
                       const double* pos, const double* q, double* energy, double* forces,
-                      double* dEdbox, void* workspace, size_t ws_bytes, void* stream, int mode) {
+                      double* dEdbox, void* workspace, size_t ws_bytes, void* stream, int mode,
+                      int32_t flags = 0) {
   // mode 0: upload and run; 1: upload only (synchronous); 2: run on the uploaded table
   const int st = batch_args_ok(n_sys, n_atoms, boxes, cutoffs, kmesh, precision);
   if (st != ADMP_OK) return st;
@@ -636,8 +637,10 @@ static int batch_call(int32_t n_sys, const int64_t* n_atoms, const double* boxes
   if (ws_bytes < P.bytes) return ADMP_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
   if (mode == 1) return n_sys > 1 ? upload_table(P, (char*)workspace, s, true) : ADMP_OK;
+  if (flags & ~ADMP_BATCH_NO_OVERLAP) return ADMP_ERR_INVALID;
   StageMarks pm = stage_marks(s);
-  return run_batch(P, pos, q, energy, forces, dEdbox, (char*)workspace, s, pm.ev, mode == 0);
+  return run_batch(P, pos, q, energy, forces, dEdbox, (char*)workspace, s, pm.ev, mode == 0,
+                   (flags & ADMP_BATCH_NO_OVERLAP) == 0);
 }
 
 int admp_pme_batch(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
@@ -647,6 +650,16 @@ int admp_pme_batch(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
                    void* workspace, size_t ws_bytes, void* stream) {
   return batch_call(n_sys, n_atoms, boxes, cutoffs, kappas, kmesh, precision, convention,
                     dielectric, pos, q, energy, forces, dEdbox, workspace, ws_bytes, stream, 0);
+}
+
+int admp_pme_batch_ex(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                      const double* cutoffs, const double* kappas, const int32_t* kmesh,
+                      int32_t precision, int32_t convention, double dielectric, const double* pos,
+                      const double* q, double* energy, double* forces, double* dEdbox,
+                      int32_t flags, void* workspace, size_t ws_bytes, void* stream) {
+  return batch_call(n_sys, n_atoms, boxes, cutoffs, kappas, kmesh, precision, convention,
+                    dielectric, pos, q, energy, forces, dEdbox, workspace, ws_bytes, stream, 0,
+                    flags);
 }
 
 int admp_pme_batch_upload(int32_t n_sys, const int64_t* n_atoms, const double* boxes,

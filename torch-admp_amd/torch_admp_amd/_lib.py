@@ -24,6 +24,7 @@ ADMP_OK = 0
 ADMP_OVERFLOW = 1
 ADMP_ERR_SCRATCH = -4
 ADMP_NOALLOC_SCRATCH_BYTES = 64 << 20
+ADMP_BATCH_NO_OVERLAP = 1
 ADMP_F64 = 0
 ADMP_F32 = 1
 KCONV = {"reference": 0, "columns": 1}
@@ -79,6 +80,9 @@ _SIGNATURES = {
     "admp_pme_batch": (ctypes.c_int, [_c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i32, _c_i32,
                                       _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_sz,
                                       _c_vp]),
+    "admp_pme_batch_ex": (ctypes.c_int, [_c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i32,
+                                         _c_i32, _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
+                                         _c_i32, _c_vp, _c_sz, _c_vp]),
     "admp_pme_batch_upload": (ctypes.c_int, [_c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i32,
                                              _c_i32, _c_dbl, _c_vp, _c_sz, _c_vp]),
     "admp_pme_batch_run": (ctypes.c_int, [_c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i32,

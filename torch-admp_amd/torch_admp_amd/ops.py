@@ -216,9 +216,11 @@ class PMEBatchPlan:
         self.workspace_bytes = int(nbytes.value)
 
     def run(self, pos: torch.Tensor, q: torch.Tensor, workspace: Optional[torch.Tensor] = None,
-            want_box_grad: bool = True):
+            want_box_grad: bool = True, overlap: bool = True):
         """Energies (n_sys, 5), forces (n_total, 3) and dE/dbox (n_sys, 3, 3)
-        for positions (n_total, 3) and charges (n_total,) stored back to back."""
+        for positions (n_total, 3) and charges (n_total,) stored back to back.
+        overlap=False keeps every kernel on the current stream (no real-space
+        side stream): for callers with batches in flight on several streams."""
         _lib.require_gpu(pos, q)
         pos = _f64(pos)
         q = _f64(q).reshape(-1)
@@ -232,13 +234,15 @@ class PMEBatchPlan:
         dedbox = (torch.empty(self.n_sys, 3, 3, dtype=torch.float64, device=dev)
                   if want_box_grad else None)
         lib = _lib.load()
-        _lib.check(lib.admp_pme_batch(self.n_sys, self._pn, self._pbox, self._prc, self._pkap,
-                                      self._pkm, _lib.PRECISION[self.precision],
-                                      _lib.KCONV[self.convention], self.dielectric,
-                                      _lib.ptr(pos), _lib.ptr(q), _lib.ptr(energy),
-                                      _lib.ptr(forces), _lib.ptr(dedbox), _lib.ptr(workspace),
-                                      self.workspace_bytes, _lib.stream_of(pos)),
-                   "admp_pme_batch")
+        _lib.check(lib.admp_pme_batch_ex(self.n_sys, self._pn, self._pbox, self._prc, self._pkap,
+                                         self._pkm, _lib.PRECISION[self.precision],
+                                         _lib.KCONV[self.convention], self.dielectric,
+                                         _lib.ptr(pos), _lib.ptr(q), _lib.ptr(energy),
+                                         _lib.ptr(forces), _lib.ptr(dedbox),
+                                         0 if overlap else _lib.ADMP_BATCH_NO_OVERLAP,
+                                         _lib.ptr(workspace), self.workspace_bytes,
+                                         _lib.stream_of(pos)),
+                   "admp_pme_batch_ex")
         return energy, forces, dedbox
 
 
