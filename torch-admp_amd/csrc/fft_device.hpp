@@ -142,7 +142,15 @@ __device__ __forceinline__ void pass_r(const cplx<T>* __restrict__ a, cplx<T>* _
 }
 
 // Generic prime radix: each output is a direct sum over the r inputs read
-// from LDS, O(r^2) per butterfly, no per-thread arrays.
+// from LDS, O(r^2) per butterfly, no per-thread arrays.  A thread computes up
+// to kGenOut consecutive outputs q0.. of one butterfly: term j of output q
+// carries w_L^(j step_q) with step_q = (k tstep + q rstep) mod L, so with
+// base = w_L^(j step_q0) and f = w_L^(j rstep) (a root of unity of order r)
+// the outputs take v base, v base f, v base f^2, ...: per input one LDS load
+// and two twiddle loads serve kGenOut outputs (the indices advance by one
+// addition and one conditional subtraction per term: k tstep, q rstep < L).
+constexpr int kGenOut = 4;
+
 template <typename T, int SIGN>
 __device__ void pass_generic(const cplx<T>* __restrict__ a, cplx<T>* __restrict__ b, int L,
                              int C, int p, int r, const cplx<T>* tw) {
@@ -150,27 +158,40 @@ __device__ void pass_generic(const cplx<T>* __restrict__ a, cplx<T>* __restrict_
   const int nb = L / r;
   const int tstep = L / (p * r);
   const int rstep = L / r;
-  for (int idx = threadIdx.x; idx < nb * r * C; idx += blockDim.x) {
+  const int ng = (r + kGenOut - 1) / kGenOut;
+  for (int idx = threadIdx.x; idx < nb * ng * C; idx += blockDim.x) {
     const int c = idx % C;
     const int rest = idx / C;
     const int i = rest % nb;
-    const int qq = rest / nb;
+    const int q0 = (rest / nb) * kGenOut;
+    const int nq = min(kGenOut, r - q0);
     const int k = i % p;
-    // twiddle of term j: w_L^(k j tstep) w_r^(j q) = w_L^(j step) with
-    // step = (k tstep + q rstep) mod L, advanced by one addition and one
-    // conditional subtraction per term (k tstep < L and q rstep < L)
-    int step = k * tstep + qq * rstep;
-    if (step >= L) step -= L;
-    CT s = a[i * C + c];
-    int t = step;
+    int step = k * tstep + q0 * rstep;
+    while (step >= L) step -= L;
+    const CT x0 = a[i * C + c];
+    CT s[kGenOut];
+#pragma unroll
+    for (int m = 0; m < kGenOut; ++m) s[m] = x0;
+    int t = step, tf = rstep;
     for (int j = 1; j < r; ++j) {
-      CT v = a[(i + j * nb) * C + c];
-      v = cmul(v, tw_get<T>(tw, t, SIGN));
-      s = cadd(s, v);
+      const CT v = a[(i + j * nb) * C + c];
+      CT u = cmul(v, tw_get<T>(tw, t, SIGN));
+      const CT f = tw_get<T>(tw, tf, SIGN);
+      s[0] = cadd(s[0], u);
+#pragma unroll
+      for (int m = 1; m < kGenOut; ++m) {
+        u = cmul(u, f);
+        s[m] = cadd(s[m], u);
+      }
       t += step;
       if (t >= L) t -= L;
+      tf += rstep;
+      if (tf >= L) tf -= L;
     }
-    b[((i - k) * r + k + qq * p) * C + c] = s;
+    const int base = (i - k) * r + k;
+#pragma unroll
+    for (int m = 0; m < kGenOut; ++m)
+      if (m < nq) b[(base + (q0 + m) * p) * C + c] = s[m];
   }
 }
 
