@@ -56,6 +56,12 @@ def parse():
                     help="run only the 3000-atom QEq timing and print its JSON (for profiling)")
     ap.add_argument("--batch-copies", type=int, default=64,
                     help="copies of the water box in the batched roofline run (0 skips it)")
+    ap.add_argument("--dropin-only", action="store_true",
+                    help="run only the drop-in module and Verlet-list timings of the water box "
+                         "and print their JSON (for profiling)")
+    ap.add_argument("--serving-capacity-only", action="store_true",
+                    help="run only the serving stream's capacity (all requests queued at once, "
+                         "float64 then float32 engine) and print its JSON (for profiling)")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched water-box roofline run and print its JSON "
                          "(for per-kernel counters)")
@@ -101,6 +107,42 @@ def batched_roofline(copies, pos_np, q_np, box_np, rc, kappa, kmesh, torch, dev)
             "ms_per_batch": round(sec * 1e3, 3), "evals_per_s": round(evals, 1),
             "compulsory_bytes_per_eval": b, "achieved_GBps": round(gbs, 1),
             "peak_GBps": HBM_PEAK_GBS, "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def dropin_rates(torch, dev, pos, q, box, rc, mod64, steps=30):
+    """The reference call surface on the GPU: TorchNeighborList, then
+    CoulombForceModule and autograd forces, per evaluation (float64); then
+    MD-like steps (random moves of 0.01 A per atom and step) through
+    VerletNeighborList, whose list searched at rc + 1 A is kept while
+    displacements stay below 0.5 A."""
+    from torch_admp_amd.nblist import TorchNeighborList, VerletNeighborList
+    from torch_admp_amd.utils import calc_grads
+
+    def dropin():
+        p = pos.clone().requires_grad_(True)
+        nbl = TorchNeighborList(rc)
+        pairs = nbl(p, box)
+        e = mod64(p, box, pairs, nbl.get_ds(), nbl.get_buffer_scales(), {"charge": q})
+        return e, -calc_grads(e, p)
+
+    secd, (ed, _) = timed(dropin, steps, 3, torch)
+    vnl = VerletNeighborList(rc, 1.0)
+    gen = torch.Generator(device=dev).manual_seed(3)
+    cur = [pos.clone()]
+
+    def verlet_step():
+        cur[0] = cur[0] + 0.01 * torch.randn(cur[0].shape, device=dev, dtype=cur[0].dtype,
+                                             generator=gen)
+        p = cur[0].clone().requires_grad_(True)
+        pairs = vnl(p, box)
+        e = mod64(p, box, pairs, vnl.get_ds(), vnl.get_buffer_scales(), {"charge": q})
+        return e, -calc_grads(e, p)
+
+    secv, _ = timed(verlet_step, steps, 3, torch)
+    return {"dropin_module_fp64_evals_per_s": round(1.0 / secd, 3),
+            "dropin_verlet_fp64_evals_per_s": round(1.0 / secv, 3),
+            "dropin_verlet_builds_reuses": [vnl.builds, vnl.reuses],
+            "energy": float(ed.detach()[0])}
 
 
 def qeq_bench(torch, dev):
@@ -517,6 +559,27 @@ def main():
     box = torch.tensor(box_np, dtype=torch.float64, device=dev)
     box_h = torch.tensor(box_np, dtype=torch.float64)  # launch parameter, kept on the host
 
+    if args.dropin_only:
+        mod64 = CoulombForceModule(rc, ethresh, kmesh=kmesh, precision="float64")
+        print(json.dumps(dropin_rates(torch, dev, pos, q, box, rc, mod64)), flush=True)
+        return
+    if args.serving_capacity_only:
+        from torch_admp_amd import serving
+
+        base = serving.synthetic_stream(args.serving_requests or 4096, 1.0, seed=7)
+        out = {}
+        for prec in ("float64", "float32"):
+            def reqs():
+                return [serving.PMERequest(r.positions, r.box, r.charges, r.rcut, r.ethresh)
+                        for r in base]
+            serving.PMEServer(dev, hbm_budget=16 << 30, max_systems=256,
+                              precision=prec).serve(reqs()[:64])
+            cap = serving.PMEServer(dev, hbm_budget=16 << 30, max_systems=256,
+                                    precision=prec).serve(reqs())
+            out[prec] = {"capacity_req_s": round(cap["throughput_req_s"], 2),
+                         "batches": cap["batches"]}
+        print(json.dumps({"serving_capacity": out}), flush=True)
+        return
     if args.batched_only:
         if args.calib_copy_mib > 0:
             src = torch.empty(args.calib_copy_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
@@ -626,36 +689,11 @@ def main():
                          torch)
         extras["fused_fp64_evals_per_s"] = round(1.0 / sec64, 2)
 
-        def dropin():
-            p = pos.clone().requires_grad_(True)
-            nbl = TorchNeighborList(rc)
-            pairs = nbl(p, box)
-            e = mod64(p, box, pairs, nbl.get_ds(), nbl.get_buffer_scales(), {"charge": q})
-            return e, -calc_grads(e, p)
-
-        secd, (ed, fd) = timed(dropin, 10, 2, torch)
-        extras["dropin_module_fp64_evals_per_s"] = round(1.0 / secd, 3)
-        extras["dropin_vs_fused_fp64_energy_rel"] = float(abs(ed.detach()[0] - e64) / abs(e64))
-
-        # MD-like steps (random moves of 0.01 A per atom and step): the pair list
-        # searched at rc + 1 A is kept while displacements stay below 0.5 A
-        from torch_admp_amd.nblist import VerletNeighborList
-
-        vnl = VerletNeighborList(rc, 1.0)
-        gen = torch.Generator(device=dev).manual_seed(3)
-        cur = [pos.clone()]
-
-        def verlet_step():
-            cur[0] = cur[0] + 0.01 * torch.randn(cur[0].shape, device=dev, dtype=cur[0].dtype,
-                                                 generator=gen)
-            p = cur[0].clone().requires_grad_(True)
-            pairs = vnl(p, box)
-            e = mod64(p, box, pairs, vnl.get_ds(), vnl.get_buffer_scales(), {"charge": q})
-            return e, -calc_grads(e, p)
-
-        secv, _ = timed(verlet_step, 20, 2, torch)
-        extras["dropin_verlet_fp64_evals_per_s"] = round(1.0 / secv, 3)
-        extras["dropin_verlet_builds_reuses"] = [vnl.builds, vnl.reuses]
+        dr = dropin_rates(torch, dev, pos, q, box, rc, mod64)
+        extras["dropin_module_fp64_evals_per_s"] = dr["dropin_module_fp64_evals_per_s"]
+        extras["dropin_vs_fused_fp64_energy_rel"] = float(abs(dr["energy"] - e64) / abs(e64))
+        extras["dropin_verlet_fp64_evals_per_s"] = dr["dropin_verlet_fp64_evals_per_s"]
+        extras["dropin_verlet_builds_reuses"] = dr["dropin_verlet_builds_reuses"]
 
         # the fused evaluation in the library's static scratch (no workspace)
         km_arr = (ctypes.c_int32 * 3)(kmesh, kmesh, kmesh)
