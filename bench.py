@@ -377,6 +377,43 @@ def serving_bench(n_requests, torch, dev):
         "qeq_mean_cg_iterations": round(float(np.mean([r.iterations for r in qeq])), 1),
         "wall_s": round(mix["wall_s"], 3)}
 
+    # (6) resident replicas: 64 of the plain-eligible requests kept on the device and
+    # stepped repeatedly (MD replicas): positions moved in place, one graph replay per
+    # step; against a per-step eager batch (planning + launches) and the plain path
+    reps = eligible[:64]
+    n_rep_atoms = int(sum(r.n_atoms for r in reps))
+    res = {"replicas": len(reps), "atoms": n_rep_atoms}
+    for prec in ("float64", "float32"):
+        rsv = serving.ReplicaServer(reps, dev, precision=prec)
+        gen = torch.Generator(device=dev).manual_seed(11)
+        lat = []
+        for step in range(110):
+            rsv.positions.add_(0.01 * torch.randn(rsv.positions.shape, device=dev,
+                                                  dtype=torch.float64, generator=gen))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rsv.step()
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t0)
+        lat = np.array(lat[10:]) * 1e3
+        pos_r, q_r = rsv.positions.clone(), rsv.graph.charges.clone()
+
+        def eager_step():
+            plan = ops.PMEBatchPlan([r.n_atoms for r in reps], [r.box for r in reps],
+                                    [r.rcut for r in reps], [r.kappa for r in reps],
+                                    [r.kmesh for r in reps], prec)
+            return plan.run(pos_r, q_r)
+
+        sec_e, _ = timed(eager_step, 20, 3, torch)
+        tag = "fp64" if prec == "float64" else "fp32"
+        res[tag] = {"step_p50_ms": round(float(np.percentile(lat, 50)), 3),
+                    "step_p99_ms": round(float(np.percentile(lat, 99)), 3),
+                    "replica_evals_per_s": round(len(reps) / (lat.mean() * 1e-3), 1),
+                    "eager_batch_step_ms": round(sec_e * 1e3, 3),
+                    "vs_plain_fp64": round(len(reps) / (lat.mean() * 1e-3) * t_plain, 2)}
+        del rsv
+    out["resident_replicas"] = res
+
     atoms = np.array([r.n_atoms for r in base])
     out.update({
         "workload": f"{n_requests} PME E+F requests, atoms log-uniform [64, 8192] "

@@ -158,3 +158,33 @@ def test_close_cancels_queued_qeq_and_refuses_new(dev):
     pme = serving.synthetic_stream(3, 0.0, seed=2, n_min=100, n_max=300)
     stats = server.serve(pme)
     assert stats["requests"] == 3 and all(r.energy is not None for r in pme)
+
+
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-12), ("float32", 1e-6)])
+def test_replica_server_replays_equal_eager(dev, precision, tol):
+    """Resident replicas stepped by graph replay: after each in-place position
+    update, every replay equals the eager batch of the same positions (eager
+    work between the replays), and a replica equals its system alone."""
+    reqs = serving.synthetic_stream(12, 0.0, seed=17, n_max=2000)
+    rs = serving.ReplicaServer(reqs, dev, precision=precision)
+    gen = torch.Generator(device=dev).manual_seed(5)
+    for step in range(3):
+        rs.positions.add_(0.02 * torch.randn(rs.positions.shape, device=dev, dtype=torch.float64,
+                                             generator=gen))
+        e, f, g = (t.clone() for t in rs.step())
+        pos = rs.positions.clone()
+        q = rs.graph.charges.clone()
+        e1, f1, g1 = rs.plan.run(pos, q)
+        torch.cuda.synchronize()
+        scale = max(1.0, float(e1.abs().max()))
+        assert float((e - e1).abs().max()) <= tol * scale, step
+        assert float((f - f1).abs().max()) <= tol * max(1.0, float(f1.abs().max())), step
+        assert float((g - g1).abs().max()) <= 10 * tol * max(1.0, float(g1.abs().max())), step
+    r0 = reqs[0]
+    n0 = r0.n_atoms
+    e0, f0, _ = ops.pme_fused(rs.positions[:n0].clone(), torch.tensor(r0.box),
+                              rs.graph.charges[:n0].clone(), r0.rcut, r0.kappa, r0.kmesh,
+                              precision=precision)
+    e, f, _ = rs.step()
+    assert abs(float(e[0, 4] - e0[4])) <= tol * max(1.0, abs(float(e0[4])))
+    assert float((f[:n0] - f0).abs().max()) <= tol * max(1.0, float(f0.abs().max()))

@@ -462,6 +462,52 @@ class PMEServer:
         return out
 
 
+class ReplicaServer:
+    """Resident systems evaluated again and again with new positions: MD
+    replicas, or repeated queries of the same systems.
+
+    Boxes, cutoffs, meshes and charges are fixed at construction and stay on
+    the device, with the batch plan's system table; the whole batch is
+    captured once in a HIP graph (ops.PMEGraph).  A step copies the new
+    positions into the graph's position buffer on the device and replays the
+    graph: no host staging, no planning and one graph launch instead of the
+    ~14 kernel launches of a batch.  Results are device tensors of the graph
+    (valid until the next step).  ``replicas`` are PMERequest objects giving
+    the systems (their positions are the initial ones)."""
+
+    def __init__(self, replicas: List[PMERequest], device="cuda", precision: str = "float32",
+                 convention: str = "reference", dielectric: float = DIELECTRIC):
+        from .ops import PMEGraph
+
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise RuntimeError("ReplicaServer runs only on a ROCm GPU (no CPU fallback)")
+        self.replicas = list(replicas)
+        self.plan = PMEBatchPlan([r.n_atoms for r in replicas], [r.box for r in replicas],
+                                 [r.rcut for r in replicas], [r.kappa for r in replicas],
+                                 [r.kmesh for r in replicas], precision, convention, dielectric)
+        self.offsets = self.plan.offsets
+        self.graph = PMEGraph(self.plan, self.device)
+        pos = np.concatenate([r.positions for r in replicas]) if replicas else np.zeros((0, 3))
+        q = np.concatenate([r.charges for r in replicas]) if replicas else np.zeros(0)
+        self.graph.positions.copy_(torch.as_tensor(pos, dtype=torch.float64))
+        self.graph.charges.copy_(torch.as_tensor(q, dtype=torch.float64))
+
+    @property
+    def positions(self) -> torch.Tensor:
+        """The resident positions (n_total, 3), replicas back to back; update
+        them in place (or pass new ones to step)."""
+        return self.graph.positions
+
+    def step(self, positions: Optional[torch.Tensor] = None):
+        """(energies (n_rep, 5), forces (n_total, 3), dE/dbox (n_rep, 3, 3)) for
+        the resident positions, after copying ``positions`` (a device tensor of
+        the same layout) into them when given."""
+        if positions is not None:
+            self.graph.positions.copy_(positions.detach().reshape(-1, 3))
+        return self.graph.replay()
+
+
 def synthetic_stream(n_requests: int, rate_hz: float, seed: int = 0, n_min: int = 64,
                      n_max: int = 8192, density: float = 0.1, rcut_range=(6.0, 8.0),
                      triclinic_fraction: float = 0.5, ethresh: float = 1e-5) -> List[PMERequest]:
