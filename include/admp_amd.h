@@ -75,13 +75,23 @@ int admp_nblist_count(const double* pos, int64_t n, const double* box, double cu
  * row_ptr (int64, n + 1) receives the row starts (row r = pairs
  * [row_ptr[r], row_ptr[r + 1])); reverse (int32, capacity) receives for each
  * pair (i, j) the index of (j, i), -1 when the list cannot be indexed
- * (n >= 2^24 atoms or P >= 2^31), and then needs scratch (int32, capacity).
- * row_ptr and reverse let admp_pair_ds_backward run without per-pair
- * atomics.  No reference counterpart: TorchNeighborList returns the pairs
- * alone (src/torch_admp/nblist.py:304-333). */
+ * (n >= 2^24 atoms or P >= 2^31), and then needs scratch (int32, capacity);
+ * buffer_scales (int64, capacity) receives 1 where i < j else 0, the
+ * reference's pairs_buffer_scales (src/torch_admp/nblist.py:492-515) written
+ * by the same pass.  row_ptr and reverse let admp_pair_ds_backward run
+ * without per-pair atomics; they have no reference counterpart
+ * (TorchNeighborList returns the pairs alone, nblist.py:304-333). */
 int admp_nblist_fill(const double* pos, int64_t n, const double* box, double cutoff,
                      void* workspace, size_t ws_bytes, int64_t* pairs, int64_t capacity,
-                     int64_t* row_ptr, int32_t* reverse, int32_t* scratch, void* stream);
+                     int64_t* row_ptr, int32_t* reverse, int32_t* scratch,
+                     int64_t* buffer_scales, void* stream);
+
+/* Buffer scales of an ordered pair list (P, 2) into scales (int64, P): 1
+ * where i < j (src/torch_admp/nblist.py:492-515), and, when ds (P, float64)
+ * is given, lo < ds < hi, else 0.  A list kept across steps (a Verlet skin)
+ * masks its rows outside the cutoff this way.  No synchronisation. */
+int admp_pair_scales(const int64_t* pairs, int64_t n_pairs, const double* ds, double lo,
+                     double hi, int64_t* scales, void* stream);
 
 /* Largest displacement max_i |pos_i - ref_i| (minimum image when box is
  * given) into out[0] (device, float64), for a Verlet-skin neighbour list: a
@@ -125,10 +135,12 @@ int admp_pair_ds_backward(const double* grad_ds, const int64_t* pairs, int64_t n
  * q is (nf, n); eta (nf, n) for mode 2 else NULL; pairs (nf, P, 2); ds, bs (nf, P).
  * Writes energy (nf) and, when non-NULL, dE/dds (nf, P), dE/dq (nf, n) and,
  * for mode 2 only, dE/deta (nf, n) (dE/dq and dE/deta are overwritten).
- * bs is float64. */
+ * bs is float64, or int64 (the neighbour list's buffer scales as they come,
+ * no conversion pass) when mode carries ADMP_PAIR_BS_INT64. */
+#define ADMP_PAIR_BS_INT64 0x100
 int admp_pair_energy(int32_t mode, int64_t nf, int64_t n, int64_t n_pairs, const double* q,
                      const double* eta, const int64_t* pairs, const double* ds,
-                     const double* bs, double kappa, double dielectric, double* energy,
+                     const void* bs, double kappa, double dielectric, double* energy,
                      double* grad_ds, double* grad_q, double* grad_eta, void* workspace,
                      size_t ws_bytes, void* stream);
 int admp_pair_energy_workspace_bytes(int64_t nf, size_t* bytes);

@@ -179,6 +179,52 @@ def test_row_form_and_reverse_index(dev, box, rc, n):
     assert torch.allclose(got2.cpu(), ref[0], rtol=0, atol=1e-10 * max(1.0, float(ref[0].abs().max())))
 
 
+@pytest.mark.parametrize("n", [1, 2, 65, 1025, 32768, 32769, 50000])
+def test_pair_count_scan_and_buffer_scales(dev, n):
+    """Sizes either side of the single-work-group scan limit (32768 per-atom
+    counts) and the trivial ones: the total pair count against scipy's
+    periodic k-d tree, the row starts against the rows' first atoms, the
+    buffer scales the search writes against i < j, admp::pair_scales with
+    distance bounds against the same mask in torch, and the pair energy with
+    int64 scales bit-equal to the same with float64 scales."""
+    from scipy.spatial import cKDTree
+
+    rc = 4.0
+    side = max((n / 0.05) ** (1.0 / 3.0), 2.0 * rc + 0.5)
+    rng = np.random.default_rng(n)
+    pos_np = rng.random((n, 3)) * side
+    box = torch.eye(3, dtype=F64) * side
+    nbl = TorchNeighborList(rc)
+    pairs = nbl(torch.tensor(pos_np).to(dev), box.to(dev))
+    pc = pairs.cpu()
+    tree = cKDTree(pos_np, boxsize=side)
+    close = tree.query_pairs(0.01)
+    assert pc.shape[0] == 2 * (len(tree.query_pairs(rc)) - len(close))
+    row_ptr = nbl.row_ptr.cpu()
+    lens = row_ptr[1:] - row_ptr[:-1]
+    assert int(row_ptr[0]) == 0 and int(row_ptr[-1]) == pc.shape[0] and bool((lens >= 0).all())
+    row_of = torch.repeat_interleave(torch.arange(n), lens)
+    if pc.shape[0]:
+        first = torch.full((n,), -1, dtype=torch.long)
+        first[row_of] = pc[:, 0]
+        assert torch.equal(pc[:, 0], first[row_of])
+    bs = nbl.get_buffer_scales()
+    assert bs.dtype == torch.long
+    assert torch.equal(bs.cpu(), (pc[:, 0] < pc[:, 1]).long())
+    ds = nbl.get_ds().detach()
+    for lo, hi in [(0.01, rc), (1.5, 3.0), (-1.0, 2.5)]:
+        got = torch.ops.admp.pair_scales(pairs, ds, lo, hi).cpu()
+        d = ds.cpu()
+        assert torch.equal(got, ((pc[:, 0] < pc[:, 1]) & (d > lo) & (d < hi)).long())
+    assert torch.equal(torch.ops.admp.pair_scales(pairs, None, 0.0, 0.0).cpu(),
+                       (pc[:, 0] < pc[:, 1]).long())
+    q = torch.tensor(rng.uniform(-1, 1, n)).to(dev)[None]
+    e_i64 = torch.ops.admp.pair_energy(q, ds[None], pairs[None], bs[None], None, 0, 0.4, 14.4)
+    e_f64 = torch.ops.admp.pair_energy(q, ds[None], pairs[None], bs.to(F64)[None], None, 0,
+                                       0.4, 14.4)
+    assert torch.equal(e_i64, e_f64)
+
+
 def test_real_space_pair_energy(dev):
     pos, q, box = _system(300, np.eye(3) * 14.0, 5)
     rc, kappa = 6.0, 0.4

@@ -280,7 +280,7 @@ int admp_nblist_count(const double* pos, int64_t n, const double* box, double cu
   char* ws = (char*)workspace;
   hipError_t e = launch_bin(pos, n, cg, ws, L, s);
   if (e != hipSuccess) return ADMP_ERR_HIP;
-  e = launch_pairs(0, n, cg, ws, L, nullptr, 0, nullptr, s);
+  e = launch_pairs(0, n, cg, ws, L, nullptr, 0, nullptr, nullptr, s);
   if (e != hipSuccess) return ADMP_ERR_HIP;
   int64_t total = 0;
   if (n > 0) {
@@ -296,7 +296,8 @@ int admp_nblist_count(const double* pos, int64_t n, const double* box, double cu
 
 int admp_nblist_fill(const double* pos, int64_t n, const double* box, double cutoff,
                      void* workspace, size_t ws_bytes, int64_t* pairs, int64_t capacity,
-                     int64_t* row_ptr, int32_t* reverse, int32_t* scratch, void* stream) {
+                     int64_t* row_ptr, int32_t* reverse, int32_t* scratch,
+                     int64_t* buffer_scales, void* stream) {
   if (n < 0 || !(cutoff > 0.0) || capacity < 0 || (n > 0 && (!pos || !workspace)))
     return ADMP_ERR_INVALID;
   if (capacity > 0 && !pairs) return ADMP_ERR_INVALID;
@@ -310,7 +311,7 @@ int admp_nblist_fill(const double* pos, int64_t n, const double* box, double cut
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)workspace;
   int* keys = reverse ? (int*)scratch : nullptr;
-  hipError_t e = launch_pairs(1, n, cg, ws, L, pairs, capacity, keys, s);
+  hipError_t e = launch_pairs(1, n, cg, ws, L, pairs, capacity, keys, buffer_scales, s);
   if (e == hipSuccess && (row_ptr || reverse) && n > 0)
     e = launch_nb_reverse(n, cg, ws, L, keys, capacity, row_ptr, (int*)reverse, s);
   return hip_status(e);
@@ -366,9 +367,11 @@ int admp_pair_energy_workspace_bytes(int64_t nf, size_t* bytes) {
 
 int admp_pair_energy(int32_t mode, int64_t nf, int64_t n, int64_t n_pairs, const double* q,
                      const double* eta, const int64_t* pairs, const double* ds,
-                     const double* bs, double kappa, double dielectric, double* energy,
+                     const void* bs, double kappa, double dielectric, double* energy,
                      double* grad_ds, double* grad_q, double* grad_eta, void* workspace,
                      size_t ws_bytes, void* stream) {
+  const bool bs_i64 = (mode & ADMP_PAIR_BS_INT64) != 0;
+  mode &= ~ADMP_PAIR_BS_INT64;
   if (mode < 0 || mode > 2 || nf < 1 || nf > 65535 || n < 0 || n_pairs < 0 || !energy ||
       !workspace)
     return ADMP_ERR_INVALID;
@@ -378,10 +381,17 @@ int admp_pair_energy(int32_t mode, int64_t nf, int64_t n, int64_t n_pairs, const
   size_t need;
   admp_pair_energy_workspace_bytes(nf, &need);
   if (ws_bytes < need) return ADMP_ERR_INVALID;
-  return hip_status(launch_pair_energy(mode, nf, n, n_pairs, q, eta, pairs, ds, bs, kappa,
+  const double* bsd = bs_i64 ? nullptr : (const double*)bs;
+  const int64_t* bsl = bs_i64 ? (const int64_t*)bs : nullptr;
+  return hip_status(launch_pair_energy(mode, nf, n, n_pairs, q, eta, pairs, ds, bsd, bsl, kappa,
                                        dielectric, energy, grad_ds, grad_q, grad_eta,
-                                       (double*)workspace,
-                                       (hipStream_t)stream));
+                                       (double*)workspace, (hipStream_t)stream));
+}
+
+int admp_pair_scales(const int64_t* pairs, int64_t n_pairs, const double* ds, double lo,
+                     double hi, int64_t* scales, void* stream) {
+  if (n_pairs < 0 || (n_pairs > 0 && (!pairs || !scales))) return ADMP_ERR_INVALID;
+  return hip_status(launch_pair_scales(pairs, n_pairs, ds, lo, hi, scales, (hipStream_t)stream));
 }
 
 // --------------------------------------------------------------------- PME

@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(kNbThreads) k_nb_cell_pairs(
     CellGrid cg, int S, const double* __restrict__ w, const int* __restrict__ start,
     const int* __restrict__ sorted, int64_t* __restrict__ pcount,
     const int64_t* __restrict__ offsets, int64_t* __restrict__ pairs, int64_t capacity,
-    int* __restrict__ keys) {
+    int* __restrict__ keys, int64_t* __restrict__ scales) {
   __shared__ double cx[kNbCand], cy[kNbCand], cz[kNbCand];
   __shared__ int cj[kNbCand];
   __shared__ int ckey[MODE == 1 ? kNbCand : 1];
@@ -282,6 +282,7 @@ __global__ void __launch_bounds__(kNbThreads) k_nb_cell_pairs(
                 pairs[2 * o] = i;
                 pairs[2 * o + 1] = j;
                 if (keys) keys[o] = ckey[k];
+                if (scales) scales[o] = i < j ? 1 : 0;
               }
             }
             cnt += __popcll(msk);
@@ -368,37 +369,72 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(const I* __restrict__ in,
 }
 
 // Single-block exclusive scan for short arrays (n <= kScanSingleMax); writes
-// out[0..n] with out[n] = total.
+// out[0..n] with out[n] = total.  Wave w owns one contiguous segment and reads
+// it 64 consecutive items at a time (coalesced), kScanBatch loads in flight:
+// segment sums first, then each 64-item step is a wave inclusive scan plus the
+// running carry of the segments before it.
 constexpr int kScanSingleThreads = 1024;
 constexpr int kScanSingleItems = 32;
 constexpr int64_t kScanSingleMax = (int64_t)kScanSingleThreads * kScanSingleItems;
+constexpr int kScanBatch = 8;
+
+template <typename I>
+__device__ __forceinline__ I wave_inclusive_scan(I v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const I u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
 
 template <typename I>
 __global__ void __launch_bounds__(kScanSingleThreads) k_scan_single(const I* __restrict__ in,
                                                                     int64_t n,
                                                                     I* __restrict__ out) {
-  __shared__ I lds[kScanSingleThreads];
-  const int per = (int)((n + kScanSingleThreads - 1) / kScanSingleThreads);
-  const int64_t base = (int64_t)threadIdx.x * per;
+  constexpr int kWaves = kScanSingleThreads / 64;
+  __shared__ I wsum[kWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t seg = ((n + kWaves - 1) / kWaves + 63) & ~(int64_t)63;
+  const int64_t lo = (int64_t)w * seg < n ? (int64_t)w * seg : n;
+  const int64_t hi = lo + seg < n ? lo + seg : n;
   I s = 0;
-  for (int k = 0; k < per; ++k)
-    if (base + k < n) s += in[base + k];
-  lds[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = 1; off < kScanSingleThreads; off <<= 1) {
-    const I add = threadIdx.x >= off ? lds[threadIdx.x - off] : I(0);
-    __syncthreads();
-    lds[threadIdx.x] += add;
-    __syncthreads();
-  }
-  I run = lds[threadIdx.x] - s;
-  for (int k = 0; k < per; ++k)
-    if (base + k < n) {
-      const I v = in[base + k];
-      out[base + k] = run;
-      run += v;
+  for (int64_t c0 = lo; c0 < hi; c0 += 64 * kScanBatch) {
+    I v[kScanBatch];
+#pragma unroll
+    for (int b = 0; b < kScanBatch; ++b) {
+      const int64_t i = c0 + b * 64 + lane;
+      v[b] = i < hi ? in[i] : I(0);
     }
-  if (threadIdx.x == kScanSingleThreads - 1) out[n] = lds[threadIdx.x];
+#pragma unroll
+    for (int b = 0; b < kScanBatch; ++b) s += v[b];
+  }
+  s = __shfl(wave_inclusive_scan(s), 63, 64);
+  if (lane == 0) wsum[w] = s;
+  __syncthreads();
+  I carry = 0;
+  for (int k = 0; k < w; ++k) carry += wsum[k];
+  for (int64_t c0 = lo; c0 < hi; c0 += 64 * kScanBatch) {
+    I v[kScanBatch];
+#pragma unroll
+    for (int b = 0; b < kScanBatch; ++b) {
+      const int64_t i = c0 + b * 64 + lane;
+      v[b] = i < hi ? in[i] : I(0);
+    }
+#pragma unroll
+    for (int b = 0; b < kScanBatch; ++b) {
+      const int64_t i = c0 + b * 64 + lane;
+      const I incl = wave_inclusive_scan(v[b]);
+      if (i < hi) out[i] = carry + incl - v[b];
+      carry += __shfl(incl, 63, 64);
+    }
+  }
+  if (threadIdx.x == 0) {
+    I t = 0;
+    for (int k = 0; k < kWaves; ++k) t += wsum[k];
+    out[n] = t;
+  }
 }
 
 template <typename I>
@@ -734,6 +770,7 @@ __global__ void __launch_bounds__(kBlock) k_pair_energy(int mode, int64_t n, int
                                                         const int64_t* __restrict__ pairs,
                                                         const double* __restrict__ ds,
                                                         const double* __restrict__ bs,
+                                                        const int64_t* __restrict__ bsl,
                                                         double kappa, double eps,
                                                         double* __restrict__ partials,
                                                         double* __restrict__ gds,
@@ -749,7 +786,7 @@ __global__ void __launch_bounds__(kBlock) k_pair_energy(int mode, int64_t n, int
        p += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = pf[2 * p], j = pf[2 * p + 1];
     const double d = ds[f * np + p];
-    const double b = bs[f * np + p];
+    const double b = bsl ? (double)bsl[f * np + p] : bs[f * np + p];
     const double qi = qf[i], qj = qf[j];
     const double thr = mode == 1 ? 1e-8 : 1e-4;
     const bool ok = fabs(d) > thr;
@@ -789,6 +826,32 @@ __global__ void __launch_bounds__(kBlock) k_pair_energy(int mode, int64_t n, int
   double arr[1] = {e_acc};
   block_sum<1>(arr, red);
   if (threadIdx.x == 0) partials[f * gridDim.x + blockIdx.x] = arr[0];
+}
+
+// Buffer scales of an ordered pair list (src/torch_admp/nblist.py:492-515):
+// 1 where i < j, else 0; with ds, also 0 unless lo < ds < hi.
+__global__ void __launch_bounds__(kBlock) k_pair_scales(const int64_t* __restrict__ pairs,
+                                                        int64_t np,
+                                                        const double* __restrict__ ds, double lo,
+                                                        double hi, int64_t* __restrict__ out) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    bool keep = pairs[2 * p] < pairs[2 * p + 1];
+    if (ds) {
+      const double d = ds[p];
+      keep = keep && d > lo && d < hi;
+    }
+    out[p] = keep ? 1 : 0;
+  }
+}
+
+hipError_t launch_pair_scales(const int64_t* pairs, int64_t np, const double* ds, double lo,
+                              double hi, int64_t* out, hipStream_t s) {
+  if (np > 0) {
+    const unsigned nb = (unsigned)std::min<int64_t>((np + kBlock - 1) / kBlock, 8192);
+    k_pair_scales<<<nb, kBlock, 0, s>>>(pairs, np, ds, lo, hi, out);
+  }
+  return hipGetLastError();
 }
 
 __global__ void __launch_bounds__(kBlock) k_reduce_frames(const double* __restrict__ p, int nb,
@@ -927,7 +990,8 @@ hipError_t launch_cell_sort(int64_t n, int64_t ncell, char* ws, const NbLayout& 
 }
 
 hipError_t launch_pairs(int mode, int64_t n, const CellGrid& cg, char* ws, const NbLayout& L,
-                        int64_t* pairs, int64_t capacity, int* keys, hipStream_t s) {
+                        int64_t* pairs, int64_t capacity, int* keys, int64_t* scales,
+                        hipStream_t s) {
   const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
   if (n == 0) return hipGetLastError();
   const double* w = (const double*)(ws + L.w);
@@ -946,11 +1010,11 @@ hipError_t launch_pairs(int mode, int64_t n, const CellGrid& cg, char* ws, const
   const dim3 grid((unsigned)ncell, (unsigned)S);
   if (mode == 0) {
     k_nb_cell_pairs<0><<<grid, kNbThreads, 0, s>>>(cg, S, w, start, sorted, pc, nullptr, nullptr,
-                                                   0, nullptr);
+                                                   0, nullptr, nullptr);
     launch_scan<int64_t>(pc, n, off, (int64_t*)(ws + L.scan), s);
   } else {
     k_nb_cell_pairs<1><<<grid, kNbThreads, 0, s>>>(cg, S, w, start, sorted, nullptr, off, pairs,
-                                                   capacity, keys);
+                                                   capacity, keys, scales);
   }
   return hipGetLastError();
 }
@@ -1025,9 +1089,9 @@ hipError_t launch_pair_ds_bwd(const double* gds, const int64_t* pairs, int64_t n
 
 hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const double* q,
                               const double* eta, const int64_t* pairs, const double* ds,
-                              const double* bs, double kappa, double eps, double* energy,
-                              double* gds, double* gq, double* geta, double* partials,
-                              hipStream_t s) {
+                              const double* bs, const int64_t* bsl, double kappa, double eps,
+                              double* energy, double* gds, double* gq, double* geta,
+                              double* partials, hipStream_t s) {
   const int bpf = kPairBlocksPerFrame;
   const int64_t m = nf * n;
   if (m > 0) {
@@ -1036,8 +1100,8 @@ hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const
     if (geta) k_fill<double><<<nb, kBlock, 0, s>>>(geta, m, 0.0);
   }
   dim3 grid(bpf, (unsigned)nf);
-  k_pair_energy<<<grid, kBlock, 0, s>>>(mode, n, np, q, eta, pairs, ds, bs, kappa, eps, partials,
-                                        gds, gq, geta);
+  k_pair_energy<<<grid, kBlock, 0, s>>>(mode, n, np, q, eta, pairs, ds, bs, bsl, kappa, eps,
+                                        partials, gds, gq, geta);
   k_reduce_frames<<<(unsigned)nf, kBlock, 0, s>>>(partials, bpf, energy);
   return hipGetLastError();
 }

@@ -68,9 +68,11 @@ hipError_t launch_cell_scatter(int64_t n, int64_t ncell, char* ws, const NbLayou
                                hipStream_t s);
 // mode 0: count pairs per atom and scan the counts in atom order (row
 // starts at offsets[i], total at offsets[n]); mode 1: write the pairs, and
-// with `keys` (int32 per pair) their in-row order keys
+// with `keys` (int32 per pair) their in-row order keys and with `scales`
+// (int64 per pair) the buffer scales i < j
 hipError_t launch_pairs(int mode, int64_t n, const CellGrid& cg, char* ws, const NbLayout& L,
-                        int64_t* pairs, int64_t capacity, int* keys, hipStream_t s);
+                        int64_t* pairs, int64_t capacity, int* keys, int64_t* scales,
+                        hipStream_t s);
 // stencil entries of the cell grid (3 per dimension of 3+ cells, else the
 // images -nimg..nimg)
 int nb_stencil_entries(const CellGrid& cg);
@@ -98,11 +100,15 @@ hipError_t launch_pair_ds_bwd(const double* gds, const int64_t* pairs, int64_t n
                               const double* ds, double* gpos, double* gbox, double* partial,
                               const int64_t* row_ptr, const int* rev, int64_t n,
                               hipStream_t s);
+// bs float64, or bsl int64 when bs is null
 hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const double* q,
                               const double* eta, const int64_t* pairs, const double* ds,
-                              const double* bs, double kappa, double eps, double* energy,
-                              double* gds, double* gq, double* geta, double* partials,
-                              hipStream_t s);
+                              const double* bs, const int64_t* bsl, double kappa, double eps,
+                              double* energy, double* gds, double* gq, double* geta,
+                              double* partials, hipStream_t s);
+// buffer scales (i < j, and lo < ds < hi when ds is given) as int64
+hipError_t launch_pair_scales(const int64_t* pairs, int64_t np, const double* ds, double lo,
+                              double hi, int64_t* out, hipStream_t s);
 hipError_t launch_pack(const double* q, int64_t n, char* ws, const NbLayout& L, P4<double>* ps,
                        hipStream_t s);
 }  // namespace admp

@@ -457,21 +457,20 @@ hipError_t launch_box_grad(const Geom& g, const double* dA1, const double* dA2,
 // either may be NULL for zero) the same kernel writes the backward instead:
 //   dE/dq_i = g_self (-2 eps kappa / sqrt(pi)) q_i + g_nn (-eps pi Q / (V kappa^2)),
 //   dE/dbox = g_nn (-e_nn / V) cof(box)   (d det / d box = the cofactor matrix).
-// One work-group per frame; the two sums in a fixed order.
-__global__ void __launch_bounds__(kBlock) k_self_terms(const double* __restrict__ q, int64_t n,
-                                                       const double* __restrict__ box,
-                                                       double kappa, double eps,
-                                                       double* __restrict__ e_self,
-                                                       double* __restrict__ e_nn,
-                                                       const double* __restrict__ g_self,
-                                                       const double* __restrict__ g_nn,
-                                                       double* __restrict__ dq,
-                                                       double* __restrict__ dbox) {
-  __shared__ double red[2 * (kBlock / 64)];
+// One work-group of kSelfThreads per frame; the two sums in a fixed order.
+constexpr int kSelfThreads = 1024;
+__global__ void __launch_bounds__(kSelfThreads)
+    k_self_terms(const double* __restrict__ q, int64_t n, const double* __restrict__ box,
+                 double kappa, double eps, double* __restrict__ e_self,
+                 double* __restrict__ e_nn, const double* __restrict__ g_self,
+                 const double* __restrict__ g_nn, double* __restrict__ dq,
+                 double* __restrict__ dbox) {
+  __shared__ double red[2 * (kSelfThreads / 64)];
   __shared__ double bcast[2];
   const int64_t f = blockIdx.x;
   const double* qf = q + f * n;
   double acc[2] = {0.0, 0.0};
+#pragma unroll 4
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
     const double v = qf[i];
     acc[0] += v * v;
@@ -516,8 +515,8 @@ hipError_t launch_self_terms(const double* q, int64_t nf, int64_t n, const doubl
                              const double* g_self, const double* g_nn, double* dq, double* dbox,
                              hipStream_t s) {
   if (nf > 0)
-    k_self_terms<<<(unsigned)nf, kBlock, 0, s>>>(q, n, box, kappa, eps, e_self, e_nn, g_self,
-                                                 g_nn, dq, dbox);
+    k_self_terms<<<(unsigned)nf, kSelfThreads, 0, s>>>(q, n, box, kappa, eps, e_self, e_nn,
+                                                       g_self, g_nn, dq, dbox);
   return hipGetLastError();
 }
 

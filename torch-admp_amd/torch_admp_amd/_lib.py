@@ -45,7 +45,8 @@ _SIGNATURES = {
     "admp_nblist_count": (ctypes.c_int, [_c_vp, _c_i64, _c_vp, _c_dbl, _c_vp, _c_sz,
                                          _P(_c_i64), _c_vp]),
     "admp_nblist_fill": (ctypes.c_int, [_c_vp, _c_i64, _c_vp, _c_dbl, _c_vp, _c_sz, _c_vp,
-                                        _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
+                                        _c_i64, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "admp_pair_scales": (ctypes.c_int, [_c_vp, _c_i64, _c_vp, _c_dbl, _c_dbl, _c_vp, _c_vp]),
     "admp_max_displacement": (ctypes.c_int, [_c_vp, _c_vp, _c_i64, _c_vp, _c_vp, _c_vp]),
     "admp_pair_ds": (ctypes.c_int, [_c_vp, _c_vp, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
     "admp_pair_ds_backward_workspace_bytes": (ctypes.c_int, [_c_i64, _P(_c_sz)]),

@@ -84,12 +84,13 @@ class TorchNeighborList(torch.nn.Module):
             if not torch.jit.is_scripting():
                 _lib.remember_host_box(box, box_host)
             check_cutoff(box_host, self.cutoff)
-        pairs, row_ptr, reverse = torch.ops.admp.neighbor_list(positions, box, self.cutoff,
-                                                               box_host)
+        # the search pass also writes the buffer scales (i < j)
+        pairs, row_ptr, reverse, scales = torch.ops.admp.neighbor_list(positions, box,
+                                                                       self.cutoff, box_host)
         self.pairs = pairs
         self.row_ptr = row_ptr
         self.reverse = reverse
-        self.buffer_scales = self.pairs_buffer_scales(pairs)
+        self.buffer_scales = scales
         self.ds = self.pairs_ds(positions, pairs, box, box is not None, box_host, row_ptr,
                                 reverse)
         return pairs
@@ -190,8 +191,8 @@ class VerletNeighborList(TorchNeighborList):
                     check_cutoff(box_host, self.cutoff + skin)
                 except AssertionError:
                     skin = 0.0
-            lst, row_ptr, reverse = torch.ops.admp.neighbor_list(positions, box,
-                                                                 self.cutoff + skin, box_host)
+            lst, row_ptr, reverse, _ = torch.ops.admp.neighbor_list(positions, box,
+                                                                    self.cutoff + skin, box_host)
             self._list = lst
             self.row_ptr = row_ptr
             self.reverse = reverse
@@ -205,7 +206,8 @@ class VerletNeighborList(TorchNeighborList):
         self.pairs = pairs
         self.ds = self.pairs_ds(positions, pairs, box, box is not None, box_host, self.row_ptr,
                                 self.reverse)
-        d = self.ds.detach()
-        inside = (d < self.cutoff) & ((d > 0.01) if box is not None else (d >= 0.0))
-        self.buffer_scales = ((pairs[:, 0] < pairs[:, 1]) & inside).to(torch.long)
+        # i < j and inside 0.01 < d < cutoff (d < cutoff without a box), one kernel
+        self.buffer_scales = torch.ops.admp.pair_scales(pairs, self.ds.detach(),
+                                                        0.01 if box is not None else -1.0,
+                                                        self.cutoff)
         return pairs

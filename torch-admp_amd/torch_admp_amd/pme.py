@@ -172,9 +172,9 @@ class CoulombForceModule(BaseForceModule):
     def _forward_pbc_real(self, charges: torch.Tensor, pairs: torch.Tensor, ds: torch.Tensor,
                           buffer_scales: torch.Tensor) -> torch.Tensor:
         """eps erfc(kappa d) qi qj / d over the pair rows (src/torch_admp/pme.py:312-349)."""
-        # pair mode 0: erfc-screened pairs; the scales converted once here serve
-        # the forward and the backward kernels
-        return torch.ops.admp.pair_energy(charges, ds, pairs, buffer_scales.to(ds.dtype), None, 0,
+        # pair mode 0: erfc-screened pairs (the kernel reads int64 scales as
+        # they are)
+        return torch.ops.admp.pair_energy(charges, ds, pairs, buffer_scales, None, 0,
                                           self.kappa, self.const_lib.dielectric)
 
     def _forward_pbc_reciprocal(self, charges: torch.Tensor, positions: torch.Tensor,
@@ -232,7 +232,7 @@ class CoulombForceModule(BaseForceModule):
                      buffer_scales: torch.Tensor) -> torch.Tensor:
         """Bare Coulomb over the pair list (src/torch_admp/pme.py:529-561)."""
         # pair mode 1: bare pairs
-        return torch.ops.admp.pair_energy(charges, ds, pairs, buffer_scales.to(ds.dtype), None, 1,
+        return torch.ops.admp.pair_energy(charges, ds, pairs, buffer_scales, None, 1,
                                           0.0, self.const_lib.dielectric)
 
     # -- fused single-call path (no pair list crosses the boundary) -------------
