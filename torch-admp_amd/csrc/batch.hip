@@ -116,8 +116,9 @@ __global__ void __launch_bounds__(kBlock) k_b_bin(int64_t ntot, int nsys,
 }
 
 // Cell order and packing, one thread per atom: slot = start[key] + a cursor
-// atomic (key = cell * kSub + i mod kSub), then sorted[slot] = i and
-// ps[slot] = (wrapped position, charge).  The order within a cell is that of
+// atomic (key = cell * kSub + i mod kSub), then sorted[slot] = i,
+// ps[slot] = (wrapped position, charge) and, for systems spread by bricks,
+// mref[slot] = the mesh reference point m0 of the wrapped position (x | y << 16, z).  The order within a cell is that of
 // the atomics; every consumer (centre-cell pair test, spread, gather) works
 // with any order.  With start == nullptr each work-group forms the exclusive
 // scan of the ncount counters in LDS itself (ncount <= kScatterScanMax) and
@@ -133,7 +134,10 @@ __global__ void __launch_bounds__(kBlock) k_b_scatter_pack(int64_t ntot, int nco
                                                            const double* __restrict__ w,
                                                            const double* __restrict__ q,
                                                            int* __restrict__ sorted,
-                                                           P4<double>* __restrict__ ps) {
+                                                           P4<double>* __restrict__ ps,
+                                                           const int* __restrict__ sys_of,
+                                                           const SysDesc* __restrict__ tab,
+                                                           int2* __restrict__ mref) {
   __shared__ int st[kScatterScanMax + 1];
   __shared__ int part[kBlock];
   if (!start) {
@@ -180,6 +184,14 @@ __global__ void __launch_bounds__(kBlock) k_b_scatter_pack(int64_t ntot, int nco
   v.z = w[3 * i + 2];
   v.q = q[i];
   ps[slot] = v;
+  // the mesh reference point of the wrapped position, for the brick spread's scan
+  const SysDesc* D = tab + sys_of[i];
+  if (D->brick) {
+    int m0[3];
+    double f[3];
+    mesh_ref(D->g, v.x, v.y, v.z, m0, f);
+    mref[slot] = make_int2(m0[0] | (m0[1] << 16), m0[2]);
+  }
 }
 
 // ------------------------------------------------------------------ real space
@@ -224,10 +236,34 @@ __global__ void __launch_bounds__(kSpreadThreads) k_b_spread_tiles(int nsys,
                                                            T* __restrict__ mesh) {
   const int64_t c = blockIdx.x;
   const SysDesc* D = tab + find_seg(cell_off, nsys, c);
-  if (!D->tg.ok) return;
+  if (!D->tg.ok || D->brick) return;
   const int lc = (int)(c - D->cell0);
   spread_cell_tile<T>(D->g, D->tg, lc, start[c * kSub], start[(c + 1) * kSub], sorted, pos, qv,
                       mesh + D->mesh0, stm, stw);
+}
+
+// systems with every mesh dimension >= kBrickMinK: one work-group per mesh
+// brick (global brick index, XCD-local), see spread_brick
+template <typename T>
+__global__ void __launch_bounds__(kBrickThreads) k_b_spread_bricks(int nsys,
+                                                        const int64_t* __restrict__ brick_off,
+                                                        const SysDesc* __restrict__ tab,
+                                                        const P4<double>* __restrict__ ps,
+                                                        const int2* __restrict__ mref,
+                                                        const int* __restrict__ sorted,
+                                                        const int* __restrict__ start,
+                                                        int4* __restrict__ stm,
+                                                        T* __restrict__ stw,
+                                                        T* __restrict__ mesh) {
+  __shared__ BrickStage<T> S;
+  const int64_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int sid = find_seg(brick_off, nsys, b);
+  const SysDesc* D = tab + sid;
+  const Geom g = D->g;
+  const int nc[3] = {D->cg.nc[0], D->cg.nc[1], D->cg.nc[2]};
+  const int nbr[3] = {D->nbr[0], D->nbr[1], D->nbr[2]};
+  spread_brick<T>(g, nc, D->cell0, nbr, b - brick_off[sid], ps, mref, sorted, start, stm, stw,
+                  mesh + D->mesh0, S);
 }
 
 // atoms of systems whose tiles do not fit: global atomics into their mesh
@@ -245,7 +281,7 @@ __global__ void __launch_bounds__(kBlock) k_b_spread_atoms(int64_t ntot,
   const int a = (int)(t & 7);
   if (i >= ntot || a == 7) return;
   const SysDesc* D = tab + sys_of[i];
-  if (D->tg.ok) return;
+  if (D->tg.ok || D->brick) return;
   if (a == 6)  // the stencil record for the gather
     stencil_record<T>(D->g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], stm + i,
                       stw + (int64_t)kStencilVals * i);
@@ -561,6 +597,7 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
   P.lds_plane = 0;
   P.any_plane = P.any_cols = 0;
   P.any_untiled = 0;
+  P.any_brick = P.any_nonbrick = 0;
   int64_t spec = 0;
   for (int s = 0; s < nsys; ++s) {
     SysDesc& D = P.sys[s];
@@ -579,6 +616,11 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
       D.rad[d] = factorize(km[d]);
     }
     D.tg = make_tile_geom(D.cg, km, es);
+    D.brick = km[0] >= kBrickMinK && km[1] >= kBrickMinK && km[2] >= kBrickMinK &&
+              km[0] < 32768 && km[1] < 32768 ? 1 : 0;
+    D.nbr[0] = D.brick ? (km[0] + kBrickX - 1) / kBrickX : 0;
+    D.nbr[1] = D.brick ? (km[1] + kBrickY - 1) / kBrickY : 0;
+    D.nbr[2] = D.brick ? (km[2] + kBrickZ - 1) / kBrickZ : 0;
     D.kappa = kappas ? kappas[s] : 0.0;
     D.eps = eps;
     D.conv = convention;
@@ -606,6 +648,7 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
     P.off[kSegFftX][s + 1] =
         P.off[kSegFftX][s] + (D.plane ? Kzh : (int64_t)K1 * ((Kzh + D.fchunk[2] - 1) / D.fchunk[2]));
     P.off[kSegGather][s + 1] = P.off[kSegGather][s] + (D.n + kGatherAtomsPerBlock - 1) / kGatherAtomsPerBlock;
+    P.off[kSegBrick][s + 1] = P.off[kSegBrick][s] + (int64_t)D.nbr[0] * D.nbr[1] * D.nbr[2];
     P.lds_fft[0] = std::max(P.lds_fft[0], (size_t)(K2 + 2 * K2 * D.fchunk[0]) * ces);
     if (D.plane) {
       P.lds_plane = std::max(P.lds_plane, lds_plane);
@@ -615,8 +658,12 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
       P.lds_fft[2] = std::max(P.lds_fft[2], (size_t)(K0 + 2 * K0 * D.fchunk[2]) * ces);
       P.any_cols = 1;
     }
-    if (D.tg.ok) P.lds_tile = std::max(P.lds_tile, (size_t)D.tg.ntile * sizeof(double));
-    else P.any_untiled = 1;
+    if (D.brick) P.any_brick = 1;
+    else {
+      P.any_nonbrick = 1;
+      if (D.tg.ok) P.lds_tile = std::max(P.lds_tile, (size_t)D.tg.ntile * sizeof(double));
+      else P.any_untiled = 1;
+    }
   }
   P.ntot = P.off[kSegAtom][nsys];
   P.ctot = P.off[kSegCell][nsys];
@@ -637,6 +684,7 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
   P.o_nb = take(P.nb.bytes);
   P.o_sys_of = take(sizeof(int) * std::max<int64_t>(P.ntot, 1));
   P.o_ps = take(sizeof(P4<double>) * std::max<int64_t>(P.ntot, 1));
+  P.o_mref = take(sizeof(int2) * std::max<int64_t>(P.ntot, 1));
   P.o_stm = take(sizeof(int4) * std::max<int64_t>(P.ntot, 1));
   P.o_stw = take(es * kStencilVals * std::max<int64_t>(P.ntot, 1));
   P.o_grad = take(sizeof(double) * 3 * std::max<int64_t>(P.ntot, 1));
@@ -763,9 +811,10 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   mark(0);
   // ---- prologue, binning, in-cell order
   const int64_t ncount = 2 * (P.ctot * kSub + 1);  // counters and scatter cursors
-  const int64_t nfill = std::max(std::max(ncount, P.mtot), 3 * ntot);
+  const int64_t nfill = std::max(std::max(ncount, P.any_nonbrick ? P.mtot : 0), 3 * ntot);
   k_b_prologue<T><<<(unsigned)std::min<int64_t>((nfill / 4 + kBlock) / kBlock, 8192), kBlock, 0, s>>>(
-      nsys == 1, one, tab, offs, count, ncount, mesh, P.mtot, grad, 3 * ntot, tickets, nsys);
+      nsys == 1, one, tab, offs, count, ncount, mesh, P.any_nonbrick ? P.mtot : 0, grad, 3 * ntot,
+      tickets, nsys);
   if (ntot > 0)
     k_b_bin<<<nb_atoms, kBlock, 0, s>>>(ntot, nsys, seg(kSegAtom), tab, pos, (double*)(nbws + L.w),
                                         (int*)(nbws + L.cell_of), (int*)(ws + P.o_sys_of), count);
@@ -778,7 +827,8 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
                        s>>>(ntot, (int)std::min<int64_t>(nkeys, INT32_MAX), count, start,
                             scan_inline ? nullptr : start, (int*)(nbws + L.cursor),
                             (const int*)(nbws + L.cell_of), (const double*)(nbws + L.w), q,
-                            (int*)(nbws + L.sorted), ps);
+                            (int*)(nbws + L.sorted), ps, (const int*)(ws + P.o_sys_of), tab,
+                            (int2*)(ws + P.o_mref));
   }
   mark(1);
   // ---- real space (also for empty batches: its partial rows feed the reductions),
@@ -808,7 +858,13 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   };
   if (!join && !launch_real()) return ADMP_ERR_HIP;
   mark(2);
-  // ---- spread: LDS tiles added to the mesh (tiled systems) or per-atom atomics (the others)
+  // ---- spread: mesh bricks (every point stored, also with no atoms: the
+  // mesh is not zeroed for them), LDS tiles added to the mesh, or per-atom atomics
+  const int64_t nbricks = P.off[kSegBrick][nsys];
+  if (P.any_brick && nbricks > 0)
+    k_b_spread_bricks<T><<<(unsigned)nbricks, kBrickThreads, 0, s>>>(
+        nsys, seg(kSegBrick), tab, ps, (const int2*)(ws + P.o_mref),
+        (const int*)(nbws + L.sorted), start, stm, stw, mesh);
   if (ntot > 0) {
     if (P.lds_tile > 0)
       k_b_spread_tiles<T><<<(unsigned)P.ctot, kSpreadThreads, P.lds_tile, s>>>(

@@ -28,13 +28,17 @@ struct SysDesc {
   int fchunk[3];  // lines per block: z passes (row pairs), y passes, x pass
   int plane;      // the y and x passes run as one LDS-resident plane per kz
                   // (the half spectrum is then stored kz-major)
+  int brick;      // spread by mesh bricks (owner computes, no atomics)
+  int nbr[3];     // bricks per mesh axis
   int64_t atom0, n;
   int64_t cell0, ncell;
   int64_t mesh0, spec0;
 };
 
 // Segment offsets (n_sys + 1 each) used to map a global index to its system.
-enum SegKind { kSegAtom = 0, kSegCell, kSegMesh, kSegFftZ, kSegFftY, kSegFftX, kSegGather, kSegCount };
+enum SegKind {
+  kSegAtom = 0, kSegCell, kSegMesh, kSegFftZ, kSegFftY, kSegFftX, kSegGather, kSegBrick, kSegCount
+};
 
 struct BatchPlan {
   int nsys;
@@ -47,11 +51,13 @@ struct BatchPlan {
   size_t lds_plane;    // dynamic LDS of the plane pass
   int any_plane, any_cols;  // some system takes the plane pass / the line passes
   int any_untiled;     // some system spreads with global atomics
+  int any_brick;       // some system spreads by mesh bricks
+  int any_nonbrick;    // some system spreads into a zeroed mesh (tiles or atomics)
   int nsplit;          // real-space candidate splits per cell
   NbLayout nb;         // binning workspace over all cells
   CellGrid cg_all;     // only ncell is used by the shared binning helpers
   // workspace byte offsets
-  size_t o_tab, o_off, o_nb, o_sys_of, o_ps, o_stm, o_stw, o_grad, o_mesh, o_spec, o_rpart,
+  size_t o_tab, o_off, o_nb, o_sys_of, o_ps, o_mref, o_stm, o_stw, o_grad, o_mesh, o_spec, o_rpart,
       o_cpart, o_gpart, o_red, o_tickets, bytes;
   int status;          // ADMP_OK or an error code from planning
 };

@@ -201,6 +201,341 @@ __device__ __forceinline__ void spread_cell_tile(const Geom& g, const TileGeom& 
   flush_tile(tg, tile, lo0, lo1, lo2, mesh);
 }
 
+// ---- owner-computes spread over mesh bricks
+// A brick is a block of kBrickX x kBrickY x kBrickZ mesh points (smaller at the
+// upper mesh faces), owned by one work-group of kBrickWaves wavefronts; lane
+// (x, y) of every wavefront holds a column of kBrickZ points in float64
+// registers.  The work-group collects every atom whose 6-point stencil
+// reaches the brick from the cells of the binned atoms (packed positions in
+// cell order), lays each collected atom's weights onto the brick's x, y and z
+// lines, and each wavefront adds q wx wy wz of its share of them to its
+// columns.  The wavefronts' sums are added in LDS and every point is stored
+// once: no atomics, and the mesh needs no zeroing.  The brick holding an
+// atom's reference point m0 (mod K) writes its stencil record for the gather.
+constexpr int kBrickX = 8, kBrickY = 8, kBrickZ = 8;
+constexpr int kBrickWaves = 4;
+constexpr int kBrickThreads = 64 * kBrickWaves;
+constexpr int kBrickPer = 4;      // candidates tested per thread and segment
+constexpr int kBrickSeg = kBrickPer * kBrickThreads;  // candidates per segment
+constexpr int kBrickList = kBrickSeg + kBrickThreads;  // collected atoms held (a pass runs
+                                                       // once kBrickThreads are pending)
+constexpr int kBrickCells = 64;   // cells whose candidates are scanned together
+// Smallest mesh dimension of the brick path: with K >= kBrickZ + 6 a stencil
+// meets a brick in one run of points (no second meeting through the wrap).
+constexpr int kBrickMinK = 14;
+static_assert(kBrickX == 8 && kBrickY == 8 && kBrickZ == 8,
+              "lane (x, y) of a wavefront owns one column; lines of 8 points");
+
+__host__ __device__ inline int floordiv(long long a, long long b) {
+  return (int)(a >= 0 ? a / b : -((-a + b - 1) / b));
+}
+
+// Inclusive prefix sum over the 64 lanes (row shifts, then row broadcasts).
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  v += dpp_move<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_move<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_move<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_move<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
+// Collected atoms whose weights are laid out at once: one per thread, half
+// the threads in float64 (the LDS of the lines).
+template <typename T>
+constexpr int brick_chunk() { return sizeof(T) == 4 ? kBrickThreads : kBrickThreads / 2; }
+
+template <typename T>
+struct BrickStage {
+  int slot[kBrickList];             // sorted slot of each collected atom
+  int rel[kBrickList];              // stencil start - brick origin + 8, one byte per axis
+  int cpre[kBrickCells + 1];        // prefix of the scanned cells' atom counts
+  int cbeg[kBrickCells];            // first sorted slot of each scanned cell
+  int wcnt[kBrickWaves];
+  union {
+    struct {
+      // weights laid onto the brick's lines (zero where the stencil misses)
+      __align__(16) T qx[brick_chunk<T>()][kBrickX];  // q wx
+      __align__(16) T wy[brick_chunk<T>()][kBrickY];
+      __align__(16) T wz[brick_chunk<T>()][kBrickZ];
+    } w;
+    double red[kBrickWaves - 1][kBrickZ * 64];  // the other wavefronts' column sums
+  } u;
+};
+
+// Wrapped positions give 0 <= m0 <= K, so m0 - 3 - lo lies in (-2K, K).
+__device__ __forceinline__ int brick_rel(int m0, int lo, int K) {
+  int r = m0 - 3 - lo;
+  r += r < 0 ? K : 0;
+  r += r < 0 ? K : 0;
+  return r;
+}
+
+// w (6 weights from offset s of an 8-point line) laid onto the line, times scale.
+template <typename T>
+__device__ __forceinline__ void lay_line(const T w[6], int s, T scale, T* __restrict__ line) {
+  typedef T tv4 __attribute__((ext_vector_type(4)));
+  T v[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    T t = T(0);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) t = (u - s == c) ? scale * w[c] : t;
+    v[u] = t;
+  }
+  tv4* l4 = reinterpret_cast<tv4*>(line);
+  l4[0] = tv4{v[0], v[1], v[2], v[3]};
+  l4[1] = tv4{v[4], v[5], v[6], v[7]};
+}
+
+// The collected atoms [0, cnt), brick_chunk<T>() at a time: weights (float64
+// B-splines rounded once to T, as every other spread path) laid onto the
+// brick's lines, stencil records of the atoms this brick owns, then each
+// wavefront's contiguous share of the column sums.  Ends with a barrier.
+template <typename T>
+__device__ __forceinline__ void brick_pass(const Geom& g, const int lo[3], const int W[3], int cnt,
+                                           BrickStage<T>& S, const P4<double>* __restrict__ ps,
+                                           const int* __restrict__ sorted, int4* __restrict__ stm,
+                                           T* __restrict__ stw, double (&acc)[kBrickZ]) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int x = lane >> 3, y = lane & 7;
+  typedef T tv4 __attribute__((ext_vector_type(4)));
+  constexpr int kChunk = brick_chunk<T>();
+  for (int c0 = 0; c0 < cnt; c0 += kChunk) {
+    const int k = c0 + tid;
+    if (tid < kChunk && k < cnt) {
+      const int slot = S.slot[k], rel = S.rel[k];
+      const P4<double> p = ps[slot];
+      const int iv = sorted[slot];
+      int m0[3];
+      double f[3];
+      mesh_ref(g, p.x, p.y, p.z, m0, f);
+      bool own = true;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int o = (m0[d] == g.k[d] ? 0 : m0[d]) - lo[d];
+        own = own && o >= 0 && o < W[d];
+      }
+      T* rec = stw + (int64_t)kStencilVals * iv;
+      // one axis at a time (register pressure); the owner stores the record
+      // piecewise: weights at 6 d, derivatives at 18 + 6 d
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        T w[6], dw[6];
+        bspline6_rounded<T>(f[d], w, dw);
+        const int s = ((rel >> (8 * d)) & 255) - 8;
+        if (d == 0) lay_line<T>(w, s, (T)p.q, S.u.w.qx[tid]);
+        else if (d == 1) lay_line<T>(w, s, T(1), S.u.w.wy[tid]);
+        else lay_line<T>(w, s, T(1), S.u.w.wz[tid]);
+        if (own) {
+          typedef T tv2 __attribute__((ext_vector_type(2)));
+          tv2* pw = reinterpret_cast<tv2*>(rec + 6 * d);
+          tv2* pd = reinterpret_cast<tv2*>(rec + 18 + 6 * d);
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            pw[j] = tv2{w[2 * j], w[2 * j + 1]};
+            pd[j] = tv2{dw[2 * j], dw[2 * j + 1]};
+          }
+        }
+      }
+      if (own) stm[iv] = make_int4(m0[0], m0[1], m0[2], 0);
+    }
+    __syncthreads();
+    const int nk = min(kChunk, cnt - c0);
+    const int qn = (nk + kBrickWaves - 1) / kBrickWaves;
+    const int e1 = min((wave + 1) * qn, nk);
+    int e = wave * qn;
+    // four collected atoms a step, their LDS reads issued together
+    for (; e + 4 <= e1; e += 4) {
+      T a[4], b[4];
+      tv4 z0[4], z1[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = S.u.w.qx[e + u][x];
+        b[u] = S.u.w.wy[e + u][y];
+        z0[u] = reinterpret_cast<const tv4*>(S.u.w.wz[e + u])[0];
+        z1[u] = reinterpret_cast<const tv4*>(S.u.w.wz[e + u])[1];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const T v = a[u] * b[u];
+        acc[0] += (double)(v * z0[u].x);
+        acc[1] += (double)(v * z0[u].y);
+        acc[2] += (double)(v * z0[u].z);
+        acc[3] += (double)(v * z0[u].w);
+        acc[4] += (double)(v * z1[u].x);
+        acc[5] += (double)(v * z1[u].y);
+        acc[6] += (double)(v * z1[u].z);
+        acc[7] += (double)(v * z1[u].w);
+      }
+    }
+    for (; e < e1; ++e) {
+      const T v = S.u.w.qx[e][x] * S.u.w.wy[e][y];
+      const tv4 z0 = reinterpret_cast<const tv4*>(S.u.w.wz[e])[0];
+      const tv4 z1 = reinterpret_cast<const tv4*>(S.u.w.wz[e])[1];
+      acc[0] += (double)(v * z0.x);
+      acc[1] += (double)(v * z0.y);
+      acc[2] += (double)(v * z0.z);
+      acc[3] += (double)(v * z0.w);
+      acc[4] += (double)(v * z1.x);
+      acc[5] += (double)(v * z1.y);
+      acc[6] += (double)(v * z1.z);
+      acc[7] += (double)(v * z1.w);
+    }
+    __syncthreads();
+  }
+}
+
+// Brick lb (x-major over the brick grid nbr) of one system's mesh; launched
+// with kBrickThreads threads.  Atoms come binned: cell c (global index cell0 +
+// local, nc cells per axis of the fractional grid) holds the sorted slots
+// [start[c * kSub], start[(c + 1) * kSub]) with packed wrapped positions
+// ps[slot] and their mesh reference points mref[slot] (x | y << 16, z).  Cell c holds atoms with ceil(K c / nc) - 1 <= m0 <= ceil(K (c +
+// 1) / nc) up to the rounding of the binning, and a brick [lo, lo + W)
+// receives from atoms with lo - 2 <= m0 <= lo + W + 2: the cells scanned cover
+// that range with one more point on each side.
+template <typename T>
+__device__ __forceinline__ void spread_brick(const Geom& g, const int nc[3], int64_t cell0,
+                                             const int nbr[3], int64_t lb,
+                                             const P4<double>* __restrict__ ps,
+                                             const int2* __restrict__ mref,
+                                             const int* __restrict__ sorted,
+                                             const int* __restrict__ start, int4* __restrict__ stm,
+                                             T* __restrict__ stw, T* __restrict__ mesh,
+                                             BrickStage<T>& S) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int K[3] = {g.k[0], g.k[1], g.k[2]};
+  const int bz = (int)(lb % nbr[2]), by = (int)((lb / nbr[2]) % nbr[1]),
+            bx = (int)(lb / ((int64_t)nbr[1] * nbr[2]));
+  const int lo[3] = {bx * kBrickX, by * kBrickY, bz * kBrickZ};
+  const int W[3] = {min(kBrickX, K[0] - lo[0]), min(kBrickY, K[1] - lo[1]),
+                    min(kBrickZ, K[2] - lo[2])};
+  int cmin[3], clen[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int a = floordiv((long long)(lo[d] - 4) * nc[d], K[d]);
+    const int e = floordiv((long long)(lo[d] + W[d] + 3) * nc[d], K[d]);
+    clen[d] = min(e - a + 1, nc[d]);
+    cmin[d] = clen[d] == nc[d] ? 0 : a;
+  }
+  double acc[kBrickZ];
+#pragma unroll
+  for (int z = 0; z < kBrickZ; ++z) acc[z] = 0.0;
+  int cnt = 0;  // collected atoms not yet summed (the same in every thread)
+  const int ncl = clen[0] * clen[1] * clen[2];
+  for (int g0 = 0; g0 < ncl; g0 += kBrickCells) {
+    const int m = min(kBrickCells, ncl - g0);
+    if (wave == 0) {
+      // cells g0 .. g0 + m - 1 and the inclusive prefix of their counts
+      int n = 0;
+      if (lane < m) {
+        const int cc = g0 + lane;
+        const int o2 = cc % clen[2], o1 = (cc / clen[2]) % clen[1], o0 = cc / (clen[1] * clen[2]);
+        const int c0 = pmod32(cmin[0] + o0, nc[0]), c1 = pmod32(cmin[1] + o1, nc[1]),
+                  c2 = pmod32(cmin[2] + o2, nc[2]);
+        const int64_t cid = cell0 + ((int64_t)c0 * nc[1] + c1) * nc[2] + c2;
+        const int b = start[cid * kSub];
+        S.cbeg[lane] = b;
+        n = start[(cid + 1) * kSub] - b;
+      }
+      n = wave_incl_scan(n);
+      if (lane < m) S.cpre[lane + 1] = n;
+      if (lane == 0) S.cpre[0] = 0;
+    }
+    __syncthreads();
+    const int tot = S.cpre[m];
+    // segments of kBrickSeg candidates; thread tid tests candidates
+    // s0 + kBrickPer tid + r with their loads in flight together
+    for (int s0 = 0; s0 < tot; s0 += kBrickSeg) {
+      const int ns = min(kBrickSeg, tot - s0);
+      const int j0 = kBrickPer * tid;
+      int slot4[kBrickPer], rel4[kBrickPer];
+      int nh = 0;
+#pragma unroll
+      for (int r = 0; r < kBrickPer; ++r) rel4[r] = -1;
+      if (j0 < ns) {
+        const int t0 = s0 + j0;
+        int a = 0, b = m;  // largest e with cpre[e] <= t0
+        while (b - a > 1) {
+          const int mid = (a + b) >> 1;
+          if (S.cpre[mid] <= t0) a = mid;
+          else b = mid;
+        }
+        int2 mr[kBrickPer];
+#pragma unroll
+        for (int r = 0; r < kBrickPer; ++r) {
+          const int t = t0 + r;
+          while (a + 1 < m && S.cpre[a + 1] <= t) ++a;
+          slot4[r] = S.cbeg[a] + (t - S.cpre[a]);
+          if (j0 + r < ns) mr[r] = mref[slot4[r]];
+        }
+#pragma unroll
+        for (int r = 0; r < kBrickPer; ++r) {
+          if (j0 + r >= ns) continue;
+          const int m0[3] = {mr[r].x & 0xffff, (int)((unsigned)mr[r].x >> 16), mr[r].y};
+          bool hit = true;
+          int rel = 0;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const int q = brick_rel(m0[d], lo[d], K[d]);
+            hit = hit && (q < W[d] || q + 5 >= K[d]);
+            rel |= ((q < W[d] ? q : q - K[d]) + 8) << (8 * d);
+          }
+          rel4[r] = hit ? rel : -1;
+          nh += hit ? 1 : 0;
+        }
+      }
+      // collected atoms appended in candidate order
+      const int inc = wave_incl_scan(nh);
+      if (lane == 63) S.wcnt[wave] = inc;
+      __syncthreads();
+      int at = cnt + inc - nh, add = 0;
+#pragma unroll
+      for (int w = 0; w < kBrickWaves; ++w) {
+        at += w < wave ? S.wcnt[w] : 0;
+        add += S.wcnt[w];
+      }
+#pragma unroll
+      for (int r = 0; r < kBrickPer; ++r)
+        if (rel4[r] >= 0) {
+          S.slot[at] = slot4[r];
+          S.rel[at] = rel4[r];
+          ++at;
+        }
+      cnt += add;
+      __syncthreads();
+      if (cnt >= kBrickThreads) {
+        brick_pass<T>(g, lo, W, cnt, S, ps, sorted, stm, stw, acc);
+        cnt = 0;
+      }
+    }
+    __syncthreads();  // every thread has read cpre before the next group rewrites it
+  }
+  if (cnt > 0) brick_pass<T>(g, lo, W, cnt, S, ps, sorted, stm, stw, acc);
+  // the wavefronts' column sums; wavefront 0 stores the brick
+  if (wave > 0) {
+#pragma unroll
+    for (int z = 0; z < kBrickZ; ++z) S.u.red[wave - 1][z * 64 + lane] = acc[z];
+  }
+  __syncthreads();
+  if (wave == 0) {
+    const int x = lane >> 3, y = lane & 7;
+#pragma unroll
+    for (int z = 0; z < kBrickZ; ++z)
+#pragma unroll
+      for (int w = 0; w < kBrickWaves - 1; ++w) acc[z] += S.u.red[w][z * 64 + lane];
+    if (x < W[0] && y < W[1]) {
+      T* col = mesh + ((int64_t)(lo[0] + x) * K[1] + lo[1] + y) * K[2] + lo[2];
+#pragma unroll
+      for (int z = 0; z < kBrickZ; ++z)
+        if (z < W[2]) col[z] = (T)acc[z];
+    }
+  }
+}
+
 // Six consecutive mesh values (element-aligned only) in two or three vector loads.
 __device__ __forceinline__ void load6(const float* r, float p[6]) {
   typedef float f4 __attribute__((ext_vector_type(4), aligned(4)));
