@@ -325,7 +325,7 @@ def serving_bench(n_requests, torch, dev):
     t_fused = per_request(fused)
 
     # float32 engine against the float64 engine (which matches the oracle to 1e-10)
-    chk = base[:64]
+    chk = base[:256]
     res = {}
     for prec in ("float64", "float32"):
         srv = server(prec, max_systems=64)
@@ -350,6 +350,10 @@ def serving_bench(n_requests, torch, dev):
                 abs(pairs32[k][1].energy[4]))
     cond_worst = scale[worst] / abs(pairs32[worst][1].energy[4])
     f_rmse = max(float(np.sqrt(((a.forces - b.forces) ** 2).mean())) for a, b in pairs32)
+    # per-result stated bounds (serving.energy_error_bound) against the errors
+    rel_tot = np.array([abs(a.energy[4] - b.energy[4]) / abs(b.energy[4]) for a, b in pairs32])
+    cover = max(abs(a.energy[4] - b.energy[4]) / a.energy_error for a, b in pairs32)
+    stated_rel = np.array([a.energy_error / abs(b.energy[4]) for a, b in pairs32])
     # (5) PME and QEq requests in one stream
     from torch_admp_amd.workloads import qeq_species_box
 
@@ -434,6 +438,11 @@ def serving_bench(n_requests, torch, dev):
         "fp32_vs_fp64_real_scale_rel_max": float(real_scale_rel),
         "fp32_vs_fp64_recip_energy_rel_max": float(rec_rel),
         "fp32_vs_fp64_force_rmse_max": float(f_rmse),
+        "fp32_vs_fp64_energy_rel_quantiles_50_90_99": [float(np.quantile(rel_tot, p))
+                                                        for p in (0.5, 0.9, 0.99)],
+        "fp32_frac_energy_rel_above_1e-6": float((rel_tot > 1e-6).mean()),
+        "fp32_stated_bound_max_ratio": float(cover),
+        "fp32_frac_stated_bound_within_1e-6": float((stated_rel <= 1e-6).mean()),
     })
     return out
 

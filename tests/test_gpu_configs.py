@@ -233,6 +233,7 @@ def test_serving_stream_full_range(dev):
         e = r64[i].energy
         scale = abs(real_ref) + abs(recip_ref) + abs(e[2]) + abs(e[3])
         assert abs(e[4] - e_ref) <= 1e-10 * abs(e_ref) + 1e-12 * scale, (i, reqs[i].n_atoms)
+        assert abs(e[4] - e_ref) <= r64[i].energy_error, i  # the stated bound holds
         assert abs(e[0] - real_ref) <= 1e-10 * scale and abs(e[1] - recip_ref) <= 1e-10 * scale
     # the float32 engine against the float64 engine on every request
     worst = 0.0
@@ -242,6 +243,9 @@ def test_serving_stream_full_range(dev):
         assert abs(a.energy[0] - b.energy[0]) <= 1e-6 * scale, i               # real space
         d_total = abs(a.energy[4] - b.energy[4])
         assert d_total <= 1e-6 * scale, i
+        # every float32 result states a bound on its total's error, and the
+        # error stays within it (the float64 engine's own error is ~1e-10 of it)
+        assert d_total <= a.energy_error, (i, d_total, a.energy_error)
         worst = max(worst, d_total / scale)
         rmse = float(np.sqrt(((a.forces - b.forces) ** 2).mean()))
         assert rmse <= 1e-4, i

@@ -52,3 +52,17 @@ def test_qeq_request_checks_and_cost():
     assert serving.PMEServer.qeq_cost(r) == int(64 * pairs + 512 * 300 + 48 * mesh + (1 << 20))
     with pytest.raises(ValueError, match="eta"):
         serving.QEqRequest(pos, box, chi, hard, eta[:-1], rcut=6.0)
+
+
+def test_energy_error_bound_forms():
+    """The stated bound of a result: float32 scales with the sum of the term
+    magnitudes plus the reciprocal term, float64 with the term magnitudes."""
+    from torch_admp_amd import serving
+
+    e = np.array([-100.0, 2000.0, -2500.0, -1.0, -601.0])
+    b32 = serving.energy_error_bound(e, "float32")
+    assert b32 == pytest.approx(serving.F32_SCALE_ERROR * 4601.0 + serving.F32_RECIP_ERROR * 2000.0)
+    assert serving.energy_error_bound(e, "float64") == pytest.approx(1e-10 * 4601.0)
+    # the total-relative bound grows as the terms cancel
+    e2 = np.array([-100.0, 2000.0, -1899.0, 0.0, 1.0])
+    assert serving.energy_error_bound(e2, "float32") / 1.0 > 1e-4
