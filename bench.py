@@ -109,6 +109,62 @@ def batched_roofline(copies, pos_np, q_np, box_np, rc, kappa, kmesh, torch, dev)
             "peak_GBps": HBM_PEAK_GBS, "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def qeq_batch_rates(reqs, torch, dev, repeats=3):
+    """The QEq requests solved one after another (QEqForceModule.solve_implicit
+    on each) against all of them at once (qeq.solve_batch), both to each
+    request's tolerance; wall time per set, median of `repeats`."""
+    import numpy as np
+
+    from torch_admp_amd.nblist import TorchNeighborList
+    from torch_admp_amd.qeq import QEqForceModule, solve_batch
+
+    f64 = torch.float64
+    t = lambda a: torch.tensor(a, dtype=f64, device=dev)  # noqa: E731
+    sysm = [(t(r.positions), t(r.box), t(r.chi), t(r.hardness), t(r.eta)) for r in reqs]
+
+    def one_by_one():
+        out = []
+        for r, (pos, box, chi, hard, eta) in zip(reqs, sysm):
+            mod = QEqForceModule(r.rcut, r.ethresh, kappa=r.kappa, kmesh=r.kmesh)
+            nbl = TorchNeighborList(r.rcut)
+            with torch.no_grad():
+                pairs = nbl(pos, box)
+                A = torch.ones(1, r.n_atoms, dtype=f64, device=dev)
+                b = torch.full((1,), float(r.total_charge), dtype=f64, device=dev)
+                out.append(mod.solve_implicit(pos, box, chi, hard, eta, pairs, nbl.get_ds(),
+                                              nbl.get_buffer_scales(), A, b, tol=r.tol))
+        torch.cuda.synchronize()
+        return out
+
+    def batched():
+        with torch.no_grad():
+            qs, iters, _ = solve_batch([s[0] for s in sysm], [s[1] for s in sysm],
+                                       [s[2] for s in sysm], [s[3] for s in sysm],
+                                       [s[4] for s in sysm], [r.rcut for r in reqs],
+                                       total_charge=[float(r.total_charge) for r in reqs],
+                                       tol=[r.tol for r in reqs], kappa=[r.kappa for r in reqs],
+                                       kmesh=[r.kmesh for r in reqs])
+        torch.cuda.synchronize()
+        return qs, iters
+
+    def timed_ms(fn):
+        fn()
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            res = fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e3, res
+
+    t_one, q1 = timed_ms(one_by_one)
+    t_bat, (qb, iters) = timed_ms(batched)
+    dq = max(float((a - b).abs().max()) for a, b in zip(q1, qb))
+    return {"requests": len(reqs), "atoms": int(sum(r.n_atoms for r in reqs)),
+            "one_by_one_ms": round(t_one, 2), "batched_ms": round(t_bat, 2),
+            "speedup": round(t_one / t_bat, 2), "batched_max_iterations": int(max(iters)),
+            "charges_max_abs_diff": dq}
+
+
 def dropin_rates(torch, dev, pos, q, box, rc, mod64, steps=30):
     """The reference call surface on the GPU: TorchNeighborList, then
     CoulombForceModule and autograd forces, per evaluation (float64); then
@@ -374,6 +430,7 @@ def serving_bench(n_requests, torch, dev):
         mix = srv.serve([clone(r, g / rate) for r, g in zip(base, gaps)] + qeq)
     finally:
         srv.close()
+    out["qeq_batch_16"] = qeq_batch_rates(qeq, torch, dev)
     out["mixed_pme_qeq_fp64"] = {
         "pme_p50_ms": round(mix["pme"]["p50_ms"], 3), "pme_p99_ms": round(mix["pme"]["p99_ms"], 3),
         "qeq_requests": len(qeq), "qeq_p50_ms": round(mix["qeq"]["p50_ms"], 3),

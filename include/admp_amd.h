@@ -351,6 +351,36 @@ int admp_qeq_solve(const admp_qeq_problem* problem, const double* q0, double tol
                    int32_t max_iter, int32_t check_every, double* q, int32_t* iters,
                    double* residual, void* workspace, size_t ws_bytes, void* stream);
 
+/* Batched charge equilibration: n_sys independent periodic systems (atoms back
+ * to back in pos, pairs with batch-wide atom indices), each with Ewald
+ * real-space and reciprocal Coulomb terms (kappas, kmesh per system), Gaussian
+ * damping when flags has ADMP_QEQ_DAMPING, diagonal terms hdiag (2 hardness +
+ * self terms) and the one constraint sum q = total_charge[s] (host, n_sys).
+ * Every iteration forms the Hessian products of all systems in one launch
+ * sequence (pair rows, then the batched spread / FFT / influence function /
+ * gather of admp_pme_batch), then one projected-CG step per system; a system
+ * stops on its own at |P grad E| / n_s < tols[s] (host array; a device flag,
+ * the others go on)
+ * and the host reads the flags every check_every iterations.  iters[s] is the
+ * iteration at convergence or -1, residuals[s] = |P grad E| / n_s (host
+ * arrays).  The same fixed point as admp_qeq_solve on each system alone.  No
+ * counterpart in the reference, which solves one frame per call
+ * (src/torch_admp/qeq.py:609-765); for serving loops that solve many systems
+ * at once. */
+int admp_qeq_solve_batch_workspace_bytes(int32_t n_sys, const int64_t* n_atoms,
+                                         const double* boxes, const double* cutoffs,
+                                         const int32_t* kmesh, int32_t precision,
+                                         int64_t n_pairs, size_t* bytes);
+int admp_qeq_solve_batch(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                         const double* cutoffs, const double* kappas, const int32_t* kmesh,
+                         int32_t precision, int32_t convention, double dielectric, int32_t flags,
+                         const double* pos, const int64_t* pairs, const double* ds,
+                         const double* bs, int64_t n_pairs, const double* eta,
+                         const double* hdiag, const double* chi, const double* precond,
+                         const double* total_charge, const double* q0, const double* tols,
+                         int32_t max_iter, int32_t check_every, double* q, int32_t* iters,
+                         double* residuals, void* workspace, size_t ws_bytes, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Stage timing of admp_pme_fused (a profiling aid with no counterpart in the
  * reference; single-threaded use).  admp_profile_begin arms hipEvent pairs

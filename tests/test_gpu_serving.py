@@ -89,9 +89,11 @@ def test_admission_respects_budget(dev):
 
 def test_qeq_requests_served_beside_pme(dev):
     """QEq solves run on the server's worker stream while PME batches are in
-    flight: charges equal the same solve made directly with QEqForceModule
-    (1e-9 absolute; both stop at the same residual), the constraint holds, and
-    the PME results still equal the request evaluated alone."""
+    flight, queued solves together (qeq.solve_batch): charges equal the same
+    solve made directly with QEqForceModule within 1e-7 (both stop below the
+    same residual, the batched solve at the first iteration below it, the
+    direct one at its check every 4 iterations), the constraint holds, and the
+    PME results still equal the request evaluated alone."""
     from torch_admp_amd.nblist import TorchNeighborList
     from torch_admp_amd.qeq import QEqForceModule
     from torch_admp_amd.workloads import qeq_species_box
@@ -110,6 +112,7 @@ def test_qeq_requests_served_beside_pme(dev):
     assert stats["requests"] == len(pme) + len(qeq)
     assert stats["qeq"]["requests"] == len(qeq) and stats["pme"]["requests"] == len(pme)
     assert server.bytes_in_flight == 0
+    assert 1 <= server.qeq_batches <= len(qeq)
     # PME batches and QEq solves reserve from one budget under one lock
     assert 0 < server.peak_bytes_in_flight <= server.hbm_budget
     for r in qeq:
@@ -124,7 +127,7 @@ def test_qeq_requests_served_beside_pme(dev):
             q = mod.solve_implicit(pos, box, t(r.chi), t(r.hardness), t(r.eta), pairs,
                                    nbl.get_ds(), nbl.get_buffer_scales(), A, b, tol=r.tol)
         q = q.cpu().numpy()
-        assert np.abs(r.charges - q).max() <= 1e-9, r.rid
+        assert np.abs(r.charges - q).max() <= 1e-7, r.rid
         assert abs(r.charges.sum() - r.total_charge) <= 1e-9, r.rid
         assert r.iterations > 0 and r.residual <= r.tol, r.rid
         assert r.t_done >= r.t_admit >= r.arrival

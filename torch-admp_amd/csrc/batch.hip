@@ -252,6 +252,7 @@ __global__ void __launch_bounds__(kBrickThreads) k_b_spread_bricks(int nsys,
                                                         const int2* __restrict__ mref,
                                                         const int* __restrict__ sorted,
                                                         const int* __restrict__ start,
+                                                        const double* __restrict__ qv,
                                                         int4* __restrict__ stm,
                                                         T* __restrict__ stw,
                                                         T* __restrict__ mesh) {
@@ -262,8 +263,8 @@ __global__ void __launch_bounds__(kBrickThreads) k_b_spread_bricks(int nsys,
   const Geom g = D->g;
   const int nc[3] = {D->cg.nc[0], D->cg.nc[1], D->cg.nc[2]};
   const int nbr[3] = {D->nbr[0], D->nbr[1], D->nbr[2]};
-  spread_brick<T>(g, nc, D->cell0, nbr, b - brick_off[sid], ps, mref, sorted, start, stm, stw,
-                  mesh + D->mesh0, S);
+  spread_brick<T>(g, nc, D->cell0, nbr, b - brick_off[sid], ps, mref, sorted, start, qv, stm,
+                  stw, mesh + D->mesh0, S);
 }
 
 // atoms of systems whose tiles do not fit: global atomics into their mesh
@@ -770,11 +771,12 @@ int upload_table(const BatchPlan& P, char* ws, hipStream_t s, bool sync) {
   return hipLaunchHostFunc(s, free_host, host) == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
 }
 
+// Table (written by the prologue for a batch of one, uploaded otherwise),
+// prologue, binning and the cell-order pass (sorted slots, packed positions
+// and charges, mesh reference points).
 template <typename T>
-static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, double* energy,
-                       double* forces, double* dEdbox, char* ws, hipStream_t s, hipEvent_t* marks,
-                       bool upload, bool fork) {
-  using CT = cplx<T>;
+static int batch_prep(const BatchPlan& P, const double* pos, const double* q, char* ws,
+                      hipStream_t s, bool upload) {
   const int nsys = P.nsys;
   SysDesc* tab = (SysDesc*)(ws + P.o_tab);
   int64_t* offs = (int64_t*)(ws + P.o_off);
@@ -788,48 +790,122 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
     const int st = upload_table(P, ws, s, false);
     if (st != ADMP_OK) return st;
   }
-  auto seg = [&](int k) { return (const int64_t*)offs + (int64_t)(nsys + 1) * k; };
-  auto mark = [&](int k) {
-    if (marks) (void)hipEventRecord(marks[k], s);
-  };
   const int64_t ntot = P.ntot;
   char* nbws = ws + P.o_nb;
   const NbLayout& L = P.nb;
   int* count = (int*)(nbws + L.count);
   int* start = (int*)(nbws + L.start);
-  double* grad = (double*)(ws + P.o_grad);
+  const int64_t ncount = 2 * (P.ctot * kSub + 1);  // counters and scatter cursors
+  const int64_t nmesh = P.any_nonbrick ? P.mtot : 0;
+  const int64_t nfill = std::max(std::max(ncount, nmesh), 3 * ntot);
+  k_b_prologue<T><<<(unsigned)std::min<int64_t>((nfill / 4 + kBlock) / kBlock, 8192), kBlock, 0, s>>>(
+      nsys == 1, one, tab, offs, count, ncount, (T*)(ws + P.o_mesh), nmesh,
+      (double*)(ws + P.o_grad), 3 * ntot, (int*)(ws + P.o_tickets), nsys);
+  if (ntot > 0)
+    k_b_bin<<<(unsigned)std::max<int64_t>((ntot + kBlock - 1) / kBlock, 1), kBlock, 0, s>>>(
+        ntot, nsys, offs + (int64_t)(nsys + 1) * kSegAtom, tab, pos, (double*)(nbws + L.w),
+        (int*)(nbws + L.cell_of), (int*)(ws + P.o_sys_of), count);
+  // cell starts (scanned in the scatter kernel when the counters fit LDS), scatter + pack
+  const int64_t nkeys = P.ctot * kSub;
+  const bool scan_inline = nkeys <= kScatterScanMax;
+  if (!scan_inline) launch_scan<int>(count, nkeys, start, (int*)(nbws + L.scan), s);
+  k_b_scatter_pack<<<(unsigned)std::max<int64_t>((ntot + kBlock - 1) / kBlock, 1), kBlock, 0,
+                     s>>>(ntot, (int)std::min<int64_t>(nkeys, INT32_MAX), count, start,
+                          scan_inline ? nullptr : start, (int*)(nbws + L.cursor),
+                          (const int*)(nbws + L.cell_of), (const double*)(nbws + L.w), q,
+                          (int*)(nbws + L.sorted), (P4<double>*)(ws + P.o_ps),
+                          (const int*)(ws + P.o_sys_of), tab, (int2*)(ws + P.o_mref));
+  return ADMP_OK;
+}
+
+// Spread of the charges q (per atom; the brick path reads them through qv when
+// given, else from the packed ps): mesh bricks (every point stored, also with
+// no atoms: the mesh is not zeroed for them), LDS tiles added to the mesh, or
+// per-atom atomics.  Writes every atom's stencil record.
+template <typename T>
+static void batch_spread(const BatchPlan& P, const double* pos, const double* q,
+                         const double* qv, char* ws, hipStream_t s) {
+  const int nsys = P.nsys;
+  const SysDesc* tab = (const SysDesc*)(ws + P.o_tab);
+  const int64_t* offs = (const int64_t*)(ws + P.o_off);
+  auto seg = [&](int k) { return offs + (int64_t)(nsys + 1) * k; };
+  char* nbws = ws + P.o_nb;
+  const NbLayout& L = P.nb;
+  const int* start = (const int*)(nbws + L.start);
+  const int* sorted = (const int*)(nbws + L.sorted);
+  int4* stm = (int4*)(ws + P.o_stm);
+  T* stw = (T*)(ws + P.o_stw);
+  T* mesh = (T*)(ws + P.o_mesh);
+  const int64_t ntot = P.ntot;
+  const int64_t nbricks = P.off[kSegBrick][nsys];
+  if (P.any_brick && nbricks > 0)
+    k_b_spread_bricks<T><<<(unsigned)nbricks, kBrickThreads, 0, s>>>(
+        nsys, seg(kSegBrick), tab, (const P4<double>*)(ws + P.o_ps),
+        (const int2*)(ws + P.o_mref), sorted, start, qv, stm, stw, mesh);
+  if (ntot > 0) {
+    if (P.lds_tile > 0)
+      k_b_spread_tiles<T><<<(unsigned)P.ctot, kSpreadThreads, P.lds_tile, s>>>(
+          nsys, seg(kSegCell), tab, pos, q, sorted, start, stm, stw, mesh);
+    if (P.any_untiled)
+      k_b_spread_atoms<T><<<(unsigned)((ntot * 8 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+          ntot, (const int*)(ws + P.o_sys_of), tab, pos, q, stm, stw, mesh);
+  }
+}
+
+// r2c FFT, influence function (energy and dE/dA partials), c2r FFT: the mesh
+// becomes the potential mesh dE/dQ.
+template <typename T>
+static void batch_fft(const BatchPlan& P, char* ws, hipStream_t s) {
+  using CT = cplx<T>;
+  const int nsys = P.nsys;
+  const SysDesc* tab = (const SysDesc*)(ws + P.o_tab);
+  const int64_t* offs = (const int64_t*)(ws + P.o_off);
+  auto seg = [&](int k) { return offs + (int64_t)(nsys + 1) * k; };
   T* mesh = (T*)(ws + P.o_mesh);
   CT* spec = (CT*)(ws + P.o_spec);
-  P4<double>* ps = (P4<double>*)(ws + P.o_ps);
+  const int64_t rows_x = P.off[kSegFftX][nsys];
+  const unsigned bz = (unsigned)P.off[kSegFftZ][nsys], by = (unsigned)P.off[kSegFftY][nsys],
+                 bx = (unsigned)rows_x;
+  double* cpart = (double*)(ws + P.o_cpart);
+  k_b_r2c<T><<<bz, kBlock, P.lds_fft[0], s>>>(nsys, seg(kSegFftZ), tab, mesh, spec);
+  if (by > 0)
+    k_b_cols_y<T, -1><<<by, kBlock, P.lds_fft[1], s>>>(nsys, seg(kSegFftY), tab, spec);
+  if (P.any_cols)
+    k_b_cols_x<T><<<bx, kBlock, P.lds_fft[2], s>>>(nsys, seg(kSegFftX), tab, spec, cpart, rows_x);
+  if (P.any_plane)
+    k_b_plane<T><<<bx, kPlaneThreads, P.lds_plane, s>>>(nsys, seg(kSegFftX), tab, spec, cpart,
+                                                        rows_x);
+  if (by > 0)
+    k_b_cols_y<T, 1><<<by, kBlock, P.lds_fft[1], s>>>(nsys, seg(kSegFftY), tab, spec);
+  k_b_c2r<T><<<bz, kBlock, P.lds_fft[0], s>>>(nsys, seg(kSegFftZ), tab, spec, mesh);
+}
+
+template <typename T>
+static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, double* energy,
+                       double* forces, double* dEdbox, char* ws, hipStream_t s, hipEvent_t* marks,
+                       bool upload, bool fork) {
+  const int nsys = P.nsys;
+  const SysDesc* tab = (const SysDesc*)(ws + P.o_tab);
+  const int64_t* offs = (const int64_t*)(ws + P.o_off);
+  auto seg = [&](int k) { return offs + (int64_t)(nsys + 1) * k; };
+  auto mark = [&](int k) {
+    if (marks) (void)hipEventRecord(marks[k], s);
+  };
+  char* nbws = ws + P.o_nb;
+  const NbLayout& L = P.nb;
+  const int* start = (const int*)(nbws + L.start);
+  double* grad = (double*)(ws + P.o_grad);
+  T* mesh = (T*)(ws + P.o_mesh);
   int4* stm = (int4*)(ws + P.o_stm);
   T* stw = (T*)(ws + P.o_stw);
   int* tickets = (int*)(ws + P.o_tickets);
-  const unsigned nb_atoms = (unsigned)std::max<int64_t>((ntot + kBlock - 1) / kBlock, 1);
   const int64_t rows_real = P.ctot * P.nsplit;
   const int64_t rows_x = P.off[kSegFftX][nsys], rows_g = P.off[kSegGather][nsys];
 
   mark(0);
   // ---- prologue, binning, in-cell order
-  const int64_t ncount = 2 * (P.ctot * kSub + 1);  // counters and scatter cursors
-  const int64_t nfill = std::max(std::max(ncount, P.any_nonbrick ? P.mtot : 0), 3 * ntot);
-  k_b_prologue<T><<<(unsigned)std::min<int64_t>((nfill / 4 + kBlock) / kBlock, 8192), kBlock, 0, s>>>(
-      nsys == 1, one, tab, offs, count, ncount, mesh, P.any_nonbrick ? P.mtot : 0, grad, 3 * ntot,
-      tickets, nsys);
-  if (ntot > 0)
-    k_b_bin<<<nb_atoms, kBlock, 0, s>>>(ntot, nsys, seg(kSegAtom), tab, pos, (double*)(nbws + L.w),
-                                        (int*)(nbws + L.cell_of), (int*)(ws + P.o_sys_of), count);
-  {
-    // cell starts (scanned in the scatter kernel when the counters fit LDS), scatter + pack
-    const int64_t nkeys = P.ctot * kSub;
-    const bool scan_inline = nkeys <= kScatterScanMax;
-    if (!scan_inline) launch_scan<int>(count, nkeys, start, (int*)(nbws + L.scan), s);
-    k_b_scatter_pack<<<(unsigned)std::max<int64_t>((ntot + kBlock - 1) / kBlock, 1), kBlock, 0,
-                       s>>>(ntot, (int)std::min<int64_t>(nkeys, INT32_MAX), count, start,
-                            scan_inline ? nullptr : start, (int*)(nbws + L.cursor),
-                            (const int*)(nbws + L.cell_of), (const double*)(nbws + L.w), q,
-                            (int*)(nbws + L.sorted), ps, (const int*)(ws + P.o_sys_of), tab,
-                            (int2*)(ws + P.o_mref));
-  }
+  const int st = batch_prep<T>(P, pos, q, ws, s, upload);
+  if (st != ADMP_OK) return st;
   mark(1);
   // ---- real space (also for empty batches: its partial rows feed the reductions),
   // forked onto the side stream unless stages are being timed or the caller asks not to
@@ -851,45 +927,20 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   auto launch_real = [&]() -> bool {
     if (P.ctot > 0) {
       dim3 grid((unsigned)P.ctot, (unsigned)P.nsplit);
-      k_b_real<T><<<grid, kRealBlock, 0, rs>>>(nsys, seg(kSegCell), tab, ps,
+      k_b_real<T><<<grid, kRealBlock, 0, rs>>>(nsys, seg(kSegCell), tab,
+                                               (const P4<double>*)(ws + P.o_ps),
                                                (const int*)(nbws + L.sorted), start, grad, rpart);
     }
     return !join || hipEventRecord(join, rs) == hipSuccess;
   };
   if (!join && !launch_real()) return ADMP_ERR_HIP;
   mark(2);
-  // ---- spread: mesh bricks (every point stored, also with no atoms: the
-  // mesh is not zeroed for them), LDS tiles added to the mesh, or per-atom atomics
-  const int64_t nbricks = P.off[kSegBrick][nsys];
-  if (P.any_brick && nbricks > 0)
-    k_b_spread_bricks<T><<<(unsigned)nbricks, kBrickThreads, 0, s>>>(
-        nsys, seg(kSegBrick), tab, ps, (const int2*)(ws + P.o_mref),
-        (const int*)(nbws + L.sorted), start, stm, stw, mesh);
-  if (ntot > 0) {
-    if (P.lds_tile > 0)
-      k_b_spread_tiles<T><<<(unsigned)P.ctot, kSpreadThreads, P.lds_tile, s>>>(
-          nsys, seg(kSegCell), tab, pos, q, (const int*)(nbws + L.sorted), start, stm, stw, mesh);
-    if (P.any_untiled)
-      k_b_spread_atoms<T><<<(unsigned)((ntot * 8 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-          ntot, (const int*)(ws + P.o_sys_of), tab, pos, q, stm, stw, mesh);
-  }
+  // ---- spread
+  batch_spread<T>(P, pos, q, nullptr, ws, s);
   if (join && !launch_real()) return ADMP_ERR_HIP;
   mark(3);
   // ---- FFT, influence function, inverse FFT
-  const unsigned bz = (unsigned)P.off[kSegFftZ][nsys], by = (unsigned)P.off[kSegFftY][nsys],
-                 bx = (unsigned)rows_x;
-  double* cpart = (double*)(ws + P.o_cpart);
-  k_b_r2c<T><<<bz, kBlock, P.lds_fft[0], s>>>(nsys, seg(kSegFftZ), tab, mesh, spec);
-  if (by > 0)
-    k_b_cols_y<T, -1><<<by, kBlock, P.lds_fft[1], s>>>(nsys, seg(kSegFftY), tab, spec);
-  if (P.any_cols)
-    k_b_cols_x<T><<<bx, kBlock, P.lds_fft[2], s>>>(nsys, seg(kSegFftX), tab, spec, cpart, rows_x);
-  if (P.any_plane)
-    k_b_plane<T><<<bx, kPlaneThreads, P.lds_plane, s>>>(nsys, seg(kSegFftX), tab, spec, cpart,
-                                                        rows_x);
-  if (by > 0)
-    k_b_cols_y<T, 1><<<by, kBlock, P.lds_fft[1], s>>>(nsys, seg(kSegFftY), tab, spec);
-  k_b_c2r<T><<<bz, kBlock, P.lds_fft[0], s>>>(nsys, seg(kSegFftZ), tab, spec, mesh);
+  batch_fft<T>(P, ws, s);
   mark(4);
   if (join && hipStreamWaitEvent(s, join, 0) != hipSuccess) return ADMP_ERR_HIP;
   // ---- gather with the final forces
@@ -901,7 +952,7 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   // ---- per-system reductions and outputs
   PartialSets sets;
   sets.real = rpart;
-  sets.recip = cpart;
+  sets.recip = (double*)(ws + P.o_cpart);
   sets.gath = gpart;
   sets.rows_real = rows_real;
   sets.rows_recip = rows_x;
@@ -911,6 +962,68 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
       nsys, offs, tab, sets, (double*)(ws + P.o_red), tickets, energy, dEdbox);
   mark(6);
   return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
+}
+
+// ------------------------------------------------------------------ QEq pieces
+// out[i] += sum_g phi(g) theta_i(g): the reciprocal-space potential of every
+// atom from the potential mesh and the stencil records (8 lanes per atom, as
+// the gather).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_b_potential(int nsys, const int64_t* __restrict__ off,
+                                                        const SysDesc* __restrict__ tab,
+                                                        const T* __restrict__ meshes,
+                                                        const int4* __restrict__ stm,
+                                                        const T* __restrict__ stw,
+                                                        double* __restrict__ out) {
+  const int sid = find_seg(off, nsys, blockIdx.x);
+  const SysDesc* D = tab + sid;
+  const int a = threadIdx.x & 7;
+  const int64_t atom =
+      D->atom0 + (blockIdx.x - off[sid]) * kGatherAtomsPerBlock + (threadIdx.x >> 3);
+  const bool valid = atom < D->atom0 + D->n;
+  T sx = 0, sy = 0, sz = 0, sp = 0;
+  if (valid)
+    gather_lane_rec<T>(D->g.k[0], D->g.k[1], D->g.k[2], stm[atom],
+                       stw + (int64_t)kStencilVals * atom, a, meshes + D->mesh0, &sx, &sy, &sz,
+                       &sp);
+  sp = group8_sum(sp);  // every lane takes part
+  if (valid && a == 0) out[atom] += (double)sp;
+}
+
+int batch_geometry(const BatchPlan& P, const double* pos, const double* q, char* ws,
+                   hipStream_t s) {
+  if (P.status != ADMP_OK) return P.status;
+  const int st = P.precision == ADMP_F32 ? batch_prep<float>(P, pos, q, ws, s, true)
+                                         : batch_prep<double>(P, pos, q, ws, s, true);
+  if (st != ADMP_OK) return st;
+  return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
+}
+
+template <typename T>
+static int batch_potential_t(const BatchPlan& P, const double* pos, const double* v, double* out,
+                             char* ws, hipStream_t s) {
+  const int nsys = P.nsys;
+  T* mesh = (T*)(ws + P.o_mesh);
+  if (P.any_nonbrick)
+    k_fill<T><<<(unsigned)std::max<int64_t>(std::min<int64_t>((P.mtot + kBlock - 1) / kBlock, 4096),
+                                            1),
+                kBlock, 0, s>>>(mesh, P.mtot, T(0));
+  batch_spread<T>(P, pos, v, v, ws, s);
+  batch_fft<T>(P, ws, s);
+  const int64_t rows_g = P.off[kSegGather][nsys];
+  if (rows_g > 0)
+    k_b_potential<T><<<(unsigned)rows_g, kBlock, 0, s>>>(
+        nsys, (const int64_t*)(ws + P.o_off) + (int64_t)(nsys + 1) * kSegGather,
+        (const SysDesc*)(ws + P.o_tab), mesh, (const int4*)(ws + P.o_stm),
+        (const T*)(ws + P.o_stw), out);
+  return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
+}
+
+int batch_potential(const BatchPlan& P, const double* pos, const double* v, double* out, char* ws,
+                    hipStream_t s) {
+  if (P.status != ADMP_OK) return P.status;
+  return P.precision == ADMP_F32 ? batch_potential_t<float>(P, pos, v, out, ws, s)
+                                 : batch_potential_t<double>(P, pos, v, out, ws, s);
 }
 
 int run_batch(const BatchPlan& plan, const double* pos, const double* q, double* energy,

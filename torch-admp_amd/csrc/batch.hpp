@@ -76,6 +76,16 @@ int run_batch(const BatchPlan& plan, const double* pos, const double* q, double*
               double* forces, double* dEdbox, char* ws, hipStream_t s, hipEvent_t* marks,
               bool upload = true, bool fork = true);
 
+// The pieces of a batch that a charge solve repeats.  batch_geometry: table,
+// binning and cell order of the positions (q: any per-atom values, packed
+// beside them).  batch_potential: out[i] += the reciprocal-space potential
+// dE_recip/dq_i of the charges v (spread, FFT and influence function, gather),
+// on the geometry of the last batch_geometry.
+int batch_geometry(const BatchPlan& plan, const double* pos, const double* q, char* ws,
+                   hipStream_t s);
+int batch_potential(const BatchPlan& plan, const double* pos, const double* v, double* out,
+                    char* ws, hipStream_t s);
+
 // Copy the system table and segment offsets into the workspace; `sync`
 // waits for the copy.
 int upload_table(const BatchPlan& plan, char* ws, hipStream_t s, bool sync);

@@ -672,6 +672,78 @@ int admp_pme_batch_ex(int32_t n_sys, const int64_t* n_atoms, const double* boxes
                     flags);
 }
 
+// ------------------------------------------------------------------- batched QEq
+static int qeq_batch_args(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                          const double* cutoffs, const int32_t* kmesh, int32_t precision,
+                          int64_t n_pairs, BatchPlan* P, size_t* bytes,
+                          const double* kappas = nullptr, int32_t convention = ADMP_KCONV_REFERENCE,
+                          double dielectric = 1.0) {
+  const int st = batch_args_ok(n_sys, n_atoms, boxes, cutoffs, kmesh, precision);
+  if (st != ADMP_OK) return st;
+  if (n_pairs < 0) return ADMP_ERR_INVALID;
+  *P = plan_batch(n_sys, n_atoms, boxes, cutoffs, kappas, kmesh, precision, convention,
+                  dielectric);
+  const int ps = batch_plan_ok(*P);
+  if (ps != ADMP_OK) return ps;
+  const size_t extra = qeq_batch_extra_bytes(*P, n_pairs);
+  if (extra == 0) return ADMP_ERR_UNSUPPORTED;
+  *bytes = P->bytes + extra;
+  return ADMP_OK;
+}
+
+int admp_qeq_solve_batch_workspace_bytes(int32_t n_sys, const int64_t* n_atoms,
+                                         const double* boxes, const double* cutoffs,
+                                         const int32_t* kmesh, int32_t precision,
+                                         int64_t n_pairs, size_t* bytes) {
+  if (!bytes) return ADMP_ERR_INVALID;
+  BatchPlan P;
+  return qeq_batch_args(n_sys, n_atoms, boxes, cutoffs, kmesh, precision, n_pairs, &P, bytes);
+}
+
+int admp_qeq_solve_batch(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                         const double* cutoffs, const double* kappas, const int32_t* kmesh,
+                         int32_t precision, int32_t convention, double dielectric, int32_t flags,
+                         const double* pos, const int64_t* pairs, const double* ds,
+                         const double* bs, int64_t n_pairs, const double* eta,
+                         const double* hdiag, const double* chi, const double* precond,
+                         const double* total_charge, const double* q0, const double* tols,
+                         int32_t max_iter, int32_t check_every, double* q, int32_t* iters,
+                         double* residuals, void* workspace, size_t ws_bytes, void* stream) {
+  if (!kappas || !total_charge || !tols || !iters || !residuals || !workspace || max_iter < 0 ||
+      check_every < 1 || !(dielectric > 0.0) ||
+      (flags & ~ADMP_QEQ_DAMPING) != 0 ||
+      (convention != ADMP_KCONV_REFERENCE && convention != ADMP_KCONV_COLUMNS))
+    return ADMP_ERR_INVALID;
+  for (int32_t k = 0; k < n_sys; ++k)
+    if (!(kappas[k] > 0.0) || !std::isfinite(kappas[k]) || !std::isfinite(total_charge[k]) ||
+        !(tols[k] >= 0.0))
+      return ADMP_ERR_INVALID;
+  BatchPlan P;
+  size_t need = 0;
+  const int st = qeq_batch_args(n_sys, n_atoms, boxes, cutoffs, kmesh, precision, n_pairs, &P,
+                                &need, kappas, convention, dielectric);
+  if (st != ADMP_OK) return st;
+  if (ws_bytes < need) return ADMP_ERR_INVALID;
+  if (P.ntot > 0 && (!pos || !hdiag || !chi || !q0 || !q || ((flags & ADMP_QEQ_DAMPING) && !eta)))
+    return ADMP_ERR_INVALID;
+  if (n_pairs > 0 && (!pairs || !ds || !bs)) return ADMP_ERR_INVALID;
+  QeqBatch B;
+  B.pos = pos;
+  B.pairs = pairs;
+  B.ds = ds;
+  B.bs = bs;
+  B.n_pairs = n_pairs;
+  B.eta = eta;
+  B.hdiag = hdiag;
+  B.chi = chi;
+  B.precond = precond;
+  B.total_charge = total_charge;
+  B.damping = (flags & ADMP_QEQ_DAMPING) ? 1 : 0;
+  return qeq_solve_batch(P, B, q0, tols, max_iter, check_every, q, iters, residuals,
+                         (char*)workspace, (hipStream_t)stream);
+}
+
+
 int admp_pme_batch_upload(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
                           const double* cutoffs, const double* kappas, const int32_t* kmesh,
                           int32_t precision, int32_t convention, double dielectric,

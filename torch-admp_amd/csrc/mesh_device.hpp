@@ -296,7 +296,8 @@ __device__ __forceinline__ void lay_line(const T w[6], int s, T scale, T* __rest
 template <typename T>
 __device__ __forceinline__ void brick_pass(const Geom& g, const int lo[3], const int W[3], int cnt,
                                            BrickStage<T>& S, const P4<double>* __restrict__ ps,
-                                           const int* __restrict__ sorted, int4* __restrict__ stm,
+                                           const int* __restrict__ sorted,
+                                           const double* __restrict__ qv, int4* __restrict__ stm,
                                            T* __restrict__ stw, double (&acc)[kBrickZ]) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -326,7 +327,7 @@ __device__ __forceinline__ void brick_pass(const Geom& g, const int lo[3], const
         T w[6], dw[6];
         bspline6_rounded<T>(f[d], w, dw);
         const int s = ((rel >> (8 * d)) & 255) - 8;
-        if (d == 0) lay_line<T>(w, s, (T)p.q, S.u.w.qx[tid]);
+        if (d == 0) lay_line<T>(w, s, (T)(qv ? qv[iv] : p.q), S.u.w.qx[tid]);
         else if (d == 1) lay_line<T>(w, s, T(1), S.u.w.wy[tid]);
         else lay_line<T>(w, s, T(1), S.u.w.wz[tid]);
         if (own) {
@@ -392,7 +393,8 @@ __device__ __forceinline__ void brick_pass(const Geom& g, const int lo[3], const
 // with kBrickThreads threads.  Atoms come binned: cell c (global index cell0 +
 // local, nc cells per axis of the fractional grid) holds the sorted slots
 // [start[c * kSub], start[(c + 1) * kSub]) with packed wrapped positions
-// ps[slot] and their mesh reference points mref[slot] (x | y << 16, z).  Cell c holds atoms with ceil(K c / nc) - 1 <= m0 <= ceil(K (c +
+// ps[slot] and their mesh reference points mref[slot] (x | y << 16, z); the
+// charges are ps[slot].q, or qv[atom] when qv is given.  Cell c holds atoms with ceil(K c / nc) - 1 <= m0 <= ceil(K (c +
 // 1) / nc) up to the rounding of the binning, and a brick [lo, lo + W)
 // receives from atoms with lo - 2 <= m0 <= lo + W + 2: the cells scanned cover
 // that range with one more point on each side.
@@ -402,7 +404,8 @@ __device__ __forceinline__ void spread_brick(const Geom& g, const int nc[3], int
                                              const P4<double>* __restrict__ ps,
                                              const int2* __restrict__ mref,
                                              const int* __restrict__ sorted,
-                                             const int* __restrict__ start, int4* __restrict__ stm,
+                                             const int* __restrict__ start,
+                                             const double* __restrict__ qv, int4* __restrict__ stm,
                                              T* __restrict__ stw, T* __restrict__ mesh,
                                              BrickStage<T>& S) {
   const int tid = threadIdx.x, lane = tid & 63;
@@ -508,13 +511,13 @@ __device__ __forceinline__ void spread_brick(const Geom& g, const int nc[3], int
       cnt += add;
       __syncthreads();
       if (cnt >= kBrickThreads) {
-        brick_pass<T>(g, lo, W, cnt, S, ps, sorted, stm, stw, acc);
+        brick_pass<T>(g, lo, W, cnt, S, ps, sorted, qv, stm, stw, acc);
         cnt = 0;
       }
     }
     __syncthreads();  // every thread has read cpre before the next group rewrites it
   }
-  if (cnt > 0) brick_pass<T>(g, lo, W, cnt, S, ps, sorted, stm, stw, acc);
+  if (cnt > 0) brick_pass<T>(g, lo, W, cnt, S, ps, sorted, qv, stm, stw, acc);
   // the wavefronts' column sums; wavefront 0 stores the brick
   if (wave > 0) {
 #pragma unroll

@@ -13,6 +13,7 @@
 // P = I - A^T (A A^T)^-1 A; all scalars (alpha, beta, norms) stay on the
 // device and the host reads the residual every `check_every` iterations.
 #include <cmath>
+#include <vector>
 
 #include "nblist_kernels.hpp"
 #include "pme_kernels.hpp"
@@ -1027,6 +1028,345 @@ int qeq_solve_impl(const QeqProblem& P, const double* q0, double tol, int max_it
   (void)sv;
   *iters = converged ? it : -1;
   *resid = res;
+  return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
+}
+
+// ------------------------------------------------------------------ batched solves
+// Several systems' solves advance together: one Hessian product for the whole
+// batch per iteration (pair rows over the concatenated atoms, the batch
+// engine's spread / FFT / influence function / potential gather), then one
+// work-group per system for the vector step.  With the single constraint
+// sum q = Q the projection is x - mean(x).  Each system stops on its own: the
+// vector step compares |r| / n with tol and writes the iteration into a
+// device flag; a converged system's charges stay as they are while the others
+// go on, and the host reads the flags every check_every iterations.
+//
+// Per-system scalars (sc + 8 s): rz (even / odd iterations), |r|^2, sum of the
+// vector the next product multiplies, the neutralising coefficient
+// -eps pi / (V kappa^2), the total charge, the tolerance.
+constexpr int kQbRz0 = 0, kQbRz1 = 1, kQbRr = 2, kQbSumV = 3, kQbCnn = 4, kQbQ = 5, kQbTol = 6,
+              kQbStride = 8;
+
+__global__ void __launch_bounds__(kBlock) k_qb_pair_coeff(
+    int64_t np, const int64_t* __restrict__ pairs, const double* __restrict__ ds,
+    const double* __restrict__ bs, const double* __restrict__ eta, const int* __restrict__ sys_of,
+    const SysDesc* __restrict__ tab, int damp, double eps, double* __restrict__ c) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const double d = ds[p], b = bs[p];
+  double v = 0.0;
+  if (b != 0.0) {
+    const int64_t i = pairs[2 * p], j = pairs[2 * p + 1];
+    const double kappa = tab[sys_of[i]].kappa;
+    const double inv4 = fabs(d) > 1e-4 ? 1.0 / d : 0.0;
+    v += erfc(kappa * d) * inv4;
+    if (damp) {
+      double eij = sqrt((eta[i] * eta[i] + eta[j] * eta[j]) * 0.5);
+      if (eij == 0.0) eij = 1e-10;
+      v -= erfc(d / (2.0 * eij)) * inv4;
+    }
+    v *= eps * b;
+  }
+  c[p] = v;
+}
+
+// out[row] = hdiag v + sum over row of c v[col] + c_nn(s) sum(v over s) (overwrite)
+__global__ void __launch_bounds__(kBlock) k_qb_rows(int64_t n, const int* __restrict__ rowptr,
+                                                    const int* __restrict__ col,
+                                                    const double* __restrict__ val,
+                                                    const double* __restrict__ d,
+                                                    const double* __restrict__ v,
+                                                    const int* __restrict__ sys_of,
+                                                    const double* __restrict__ sc,
+                                                    double* __restrict__ out) {
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  const int sub = threadIdx.x & 15;
+  double acc = 0.0;
+  if (row < n) {
+    const int e = rowptr[row + 1];
+    for (int k = rowptr[row] + sub; k < e; k += 16) acc += val[k] * v[col[k]];
+  }
+  acc = group16_sum(acc);  // every lane of the wavefront takes part
+  if (row < n && sub == 0) {
+    const double* ss = sc + (int64_t)kQbStride * sys_of[row];
+    out[row] = d[row] * v[row] + acc + ss[kQbCnn] * ss[kQbSumV];
+  }
+}
+
+// Feasible start of system blockIdx.x: q = q0 - (sum q0 - Q) / n
+__global__ void __launch_bounds__(kPcgThreads) k_qb_start(const int64_t* __restrict__ aoff,
+                                                          const double* __restrict__ q0,
+                                                          double* __restrict__ q,
+                                                          double* __restrict__ sc) {
+  __shared__ double red[kPcgThreads / 64];
+  __shared__ double res[1];
+  const int sid = blockIdx.x;
+  const int64_t a0 = aoff[sid], a1 = aoff[sid + 1], n = a1 - a0;
+  double* ss = sc + (int64_t)kQbStride * sid;
+  double t[1] = {0.0};
+  for (int64_t i = a0 + threadIdx.x; i < a1; i += blockDim.x) t[0] += q0[i];
+  block_allsum<1>(t, red, res);
+  const double shift = n > 0 ? (t[0] - ss[kQbQ]) / (double)n : 0.0;
+  for (int64_t i = a0 + threadIdx.x; i < a1; i += blockDim.x) q[i] = q0[i] - shift;
+  if (threadIdx.x == 0) ss[kQbSumV] = ss[kQbQ];
+}
+
+// r = -P (H q + chi), z = P M^-1 r, p = z, rz, |r|^2, sum p; converged at
+// iteration 0 when |r| / n < tol
+__global__ void __launch_bounds__(kPcgThreads) k_qb_init(
+    const int64_t* __restrict__ aoff, const double* __restrict__ w, const double* __restrict__ chi,
+    const double* __restrict__ m, double* __restrict__ r, double* __restrict__ z,
+    double* __restrict__ p, double* __restrict__ sc, int* __restrict__ done) {
+  __shared__ double red[3 * (kPcgThreads / 64)];
+  __shared__ double res[3];
+  const int sid = blockIdx.x;
+  const int64_t a0 = aoff[sid], a1 = aoff[sid + 1], n = a1 - a0;
+  const double inv_n = n > 0 ? 1.0 / (double)n : 0.0;
+  double* ss = sc + (int64_t)kQbStride * sid;
+  double t[1] = {0.0};
+  for (int64_t i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+    const double ri = -(w[i] + chi[i]);
+    r[i] = ri;
+    t[0] += ri;
+  }
+  block_allsum<1>(t, red, res);
+  const double mr = t[0] * inv_n;
+  t[0] = 0.0;
+  for (int64_t i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+    const double ri = r[i] - mr;
+    r[i] = ri;
+    const double zi = m ? ri / m[i] : ri;
+    z[i] = zi;
+    t[0] += zi;
+  }
+  block_allsum<1>(t, red, res);
+  const double mz = t[0] * inv_n;
+  double u[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+    const double zi = z[i] - mz, ri = r[i];
+    z[i] = zi;
+    p[i] = zi;
+    u[0] += ri * zi;
+    u[1] += ri * ri;
+    u[2] += zi;
+  }
+  block_allsum<3>(u, red, res);
+  if (threadIdx.x == 0) {
+    ss[kQbRz0] = u[0];
+    ss[kQbRr] = u[1];
+    ss[kQbSumV] = u[2];
+    done[sid] = sqrt(fmax(u[1], 0.0)) * inv_n < ss[kQbTol] ? 0 : -1;
+  }
+}
+
+// One projected PCG iteration of system blockIdx.x (w = H p formed), the
+// arithmetic of k_pcg_vector with A = ones: alpha, q, r, |r|^2; at |r| / n <
+// tol the flag takes the iteration and the system stops; else z = P M^-1 r,
+// r.z, beta and the new direction p with its sum.
+__global__ void __launch_bounds__(kPcgThreads) k_qb_step(
+    const int64_t* __restrict__ aoff, double* __restrict__ q, double* __restrict__ r,
+    double* __restrict__ p, const double* __restrict__ w, double* __restrict__ z,
+    const double* __restrict__ m, double* __restrict__ sc, int* __restrict__ done, int it,
+    int rz_old, int rz_new) {
+  __shared__ double red[3 * (kPcgThreads / 64)];
+  __shared__ double res[3];
+  const int sid = blockIdx.x;
+  if (done[sid] >= 0) return;  // the whole work-group: the flag is only written below
+  const int64_t a0 = aoff[sid], a1 = aoff[sid + 1], n = a1 - a0;
+  const double inv_n = n > 0 ? 1.0 / (double)n : 0.0;
+  double* ss = sc + (int64_t)kQbStride * sid;
+  const double rzo = ss[rz_old];
+  double u[2] = {0.0, 0.0};
+  for (int64_t i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+    const double wi = w[i];
+    u[0] += wi * p[i];
+    u[1] += wi;
+  }
+  block_allsum<2>(u, red, res);
+  const double alpha = u[0] != 0.0 ? rzo / u[0] : 0.0;
+  const double mw = u[1] * inv_n;
+  double e[1] = {0.0};
+  for (int64_t i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+    q[i] += alpha * p[i];
+    const double ri = r[i] - alpha * (w[i] - mw);
+    r[i] = ri;
+    e[0] += ri * ri;
+  }
+  block_allsum<1>(e, red, res);
+  const double rr = e[0];
+  if (sqrt(fmax(rr, 0.0)) * inv_n < ss[kQbTol]) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      ss[kQbRr] = rr;
+      done[sid] = it;
+    }
+    return;
+  }
+  double t[1] = {0.0};
+  for (int64_t i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+    const double zi = m ? r[i] / m[i] : r[i];
+    z[i] = zi;
+    t[0] += zi;
+  }
+  block_allsum<1>(t, red, res);
+  const double mz = t[0] * inv_n;
+  e[0] = 0.0;
+  for (int64_t i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+    const double zi = z[i] - mz;
+    z[i] = zi;
+    e[0] += r[i] * zi;
+  }
+  block_allsum<1>(e, red, res);
+  const double rzn = e[0];
+  const double beta = rzo != 0.0 ? rzn / rzo : 0.0;
+  t[0] = 0.0;
+  for (int64_t i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+    const double pi = z[i] + beta * p[i];
+    p[i] = pi;
+    t[0] += pi;
+  }
+  block_allsum<1>(t, red, res);
+  __syncthreads();  // every thread has read sc[rz_old] before it can be overwritten
+  if (threadIdx.x == 0) {
+    ss[rz_new] = rzn;
+    ss[kQbRr] = rr;
+    ss[kQbSumV] = t[0];
+  }
+}
+
+namespace {
+struct QbLayout {
+  size_t c, rowcnt, rowcur, rowptr, col, val, scan, r, z, p, w, sc, done, bytes;
+  bool ok;
+};
+QbLayout qb_layout(const BatchPlan& P, int64_t n_pairs) {
+  QbLayout L;
+  size_t off = P.bytes;
+  auto take = [&](size_t b) {
+    const size_t o = off;
+    off += (b + 255) & ~size_t(255);
+    return o;
+  };
+  const int64_t n = std::max<int64_t>(P.ntot, 1), np = std::max<int64_t>(n_pairs, 1);
+  L.ok = 2 * np < (int64_t)INT32_MAX && n < (int64_t)INT32_MAX;
+  L.c = take(sizeof(double) * np);
+  L.rowcnt = take(sizeof(int) * n);
+  L.rowcur = take(sizeof(int) * n);
+  L.rowptr = take(sizeof(int) * (n + 1));
+  L.col = take(sizeof(int) * 2 * np);
+  L.val = take(sizeof(double) * 2 * np);
+  L.scan = take(sizeof(int) * (n / kBlock + 2));
+  L.r = take(sizeof(double) * n);
+  L.z = take(sizeof(double) * n);
+  L.p = take(sizeof(double) * n);
+  L.w = take(sizeof(double) * n);
+  L.sc = take(sizeof(double) * kQbStride * P.nsys);
+  L.done = take(sizeof(int) * P.nsys);
+  L.bytes = off;
+  return L;
+}
+}  // namespace
+
+size_t qeq_batch_extra_bytes(const BatchPlan& P, int64_t n_pairs) {
+  const QbLayout L = qb_layout(P, n_pairs);
+  return L.ok ? L.bytes - P.bytes : 0;
+}
+
+int qeq_solve_batch(const BatchPlan& P, const QeqBatch& B, const double* q0, const double* tol,
+                    int max_iter, int check_every, double* q, int32_t* iters, double* resid,
+                    char* ws, hipStream_t s) {
+  const QbLayout L = qb_layout(P, B.n_pairs);
+  if (!L.ok) return ADMP_ERR_UNSUPPORTED;
+  const int nsys = P.nsys;
+  const int64_t n = P.ntot;
+  for (int k = 0; k < nsys; ++k) {
+    iters[k] = -1;
+    resid[k] = 0.0;
+  }
+  // geometry: table, binning, cell order and mesh reference points
+  int st = batch_geometry(P, B.pos, B.chi, ws, s);
+  if (st != ADMP_OK) return st;
+  const SysDesc* tab = (const SysDesc*)(ws + P.o_tab);
+  const int64_t* aoff = (const int64_t*)(ws + P.o_off) + (int64_t)(nsys + 1) * kSegAtom;
+  const int* sys_of = (const int*)(ws + P.o_sys_of);
+  double* c = (double*)(ws + L.c);
+  int* rowptr = (int*)(ws + L.rowptr);
+  int* col = (int*)(ws + L.col);
+  double* val = (double*)(ws + L.val);
+  double* r = (double*)(ws + L.r);
+  double* z = (double*)(ws + L.z);
+  double* p = (double*)(ws + L.p);
+  double* w = (double*)(ws + L.w);
+  double* sc = (double*)(ws + L.sc);
+  int* done = (int*)(ws + L.done);
+  // pair rows (both orientations of every pair), built once
+  const unsigned gp = (unsigned)std::max<int64_t>((B.n_pairs + kBlock - 1) / kBlock, 1);
+  const unsigned gn = grid_for(n);
+  if (B.n_pairs > 0)
+    k_qb_pair_coeff<<<gp, kBlock, 0, s>>>(B.n_pairs, B.pairs, B.ds, B.bs, B.eta, sys_of, tab,
+                                          B.damping, P.sys.empty() ? 0.0 : P.sys[0].eps, c);
+  k_fill<int><<<gn, kBlock, 0, s>>>((int*)(ws + L.rowcnt), n, 0);
+  k_fill<int><<<gn, kBlock, 0, s>>>((int*)(ws + L.rowcur), n, 0);
+  if (B.n_pairs > 0)
+    k_csr_count<<<gp, kBlock, 0, s>>>(B.n_pairs, B.pairs, c, (int*)(ws + L.rowcnt));
+  launch_scan<int>((int*)(ws + L.rowcnt), n, rowptr, (int*)(ws + L.scan), s);
+  if (B.n_pairs > 0)
+    k_csr_fill<<<gp, kBlock, 0, s>>>(B.n_pairs, B.pairs, c, rowptr, (int*)(ws + L.rowcur), col,
+                                     val);
+  // per-system constants
+  std::vector<double> sch((size_t)kQbStride * nsys, 0.0);
+  for (int k = 0; k < nsys; ++k) {
+    const SysDesc& D = P.sys[k];
+    sch[(size_t)kQbStride * k + kQbCnn] = -D.eps * kPi / (D.g.vol * D.kappa * D.kappa);
+    sch[(size_t)kQbStride * k + kQbQ] = B.total_charge[k];
+    sch[(size_t)kQbStride * k + kQbTol] = tol[k];
+  }
+  if (hipMemcpyAsync(sc, sch.data(), sizeof(double) * sch.size(), hipMemcpyHostToDevice, s) !=
+      hipSuccess)
+    return ADMP_ERR_HIP;
+  auto product = [&](const double* v, double* out) -> int {
+    k_qb_rows<<<(unsigned)((16 * n + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        n, rowptr, col, val, B.hdiag, v, sys_of, sc, out);
+    return batch_potential(P, B.pos, v, out, ws, s);
+  };
+  // start: q feasible, r = -P (H q + chi), z, p
+  k_qb_start<<<nsys, kPcgThreads, 0, s>>>(aoff, q0, q, sc);
+  if ((st = product(q, w)) != ADMP_OK) return st;
+  k_qb_init<<<nsys, kPcgThreads, 0, s>>>(aoff, w, B.chi, B.precond, r, z, p, sc, done);
+  std::vector<int> flags(nsys);
+  auto all_done = [&]() -> int {
+    if (hipMemcpyAsync(flags.data(), done, sizeof(int) * nsys, hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return -1;
+    for (int k = 0; k < nsys; ++k)
+      if (flags[k] < 0) return 0;
+    return 1;
+  };
+  int ad = all_done();
+  if (ad < 0) return ADMP_ERR_HIP;
+  for (int it = 1; it <= max_iter && !ad; ++it) {
+    if ((st = product(p, w)) != ADMP_OK) return st;
+    k_qb_step<<<nsys, kPcgThreads, 0, s>>>(aoff, q, r, p, w, z, B.precond, sc, done, it,
+                                            (it & 1) ? kQbRz0 : kQbRz1,
+                                            (it & 1) ? kQbRz1 : kQbRz0);
+    if (it % check_every == 0 || it == max_iter) {
+      ad = all_done();
+      if (ad < 0) return ADMP_ERR_HIP;
+    }
+  }
+  // flags and residuals
+  if (hipMemcpyAsync(flags.data(), done, sizeof(int) * nsys, hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipMemcpyAsync(sch.data(), sc, sizeof(double) * sch.size(), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return ADMP_ERR_HIP;
+  for (int k = 0; k < nsys; ++k) {
+    const double nk = (double)std::max<int64_t>(P.sys[k].n, 1);
+    iters[k] = flags[k];
+    resid[k] = std::sqrt(std::max(sch[(size_t)kQbStride * k + kQbRr], 0.0)) / nk;
+  }
   return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
 }
 

@@ -4,6 +4,7 @@
 #include "admp_common.hpp"
 #include "nblist_kernels.hpp"
 #include "tiles.hpp"
+#include "batch.hpp"
 
 namespace admp {
 
@@ -64,5 +65,29 @@ template <typename T>
 int qeq_solve_impl(const QeqProblem& P, const double* q0, double tol, int max_iter,
                    int check_every, double* q, int* iters, double* resid, void* workspace,
                    hipStream_t s);
+
+// Batched solves: independent periodic systems side by side (atoms back to
+// back, pairs with batch-wide atom indices), each with Ewald real and
+// reciprocal Coulomb terms, optional Gaussian damping and the one constraint
+// sum q = total_charge[s].  The batch plan gives boxes, kappas and meshes.
+struct QeqBatch {
+  const double* pos;
+  const int64_t* pairs;
+  const double* ds;
+  const double* bs;
+  int64_t n_pairs;
+  const double* eta;
+  const double* hdiag;
+  const double* chi;
+  const double* precond;       // NULL: no preconditioner
+  const double* total_charge;  // host, one per system
+  int damping;
+};
+// Workspace bytes after the batch plan's own (0 when unsupported: row lists
+// beyond 32-bit indices).
+size_t qeq_batch_extra_bytes(const BatchPlan& plan, int64_t n_pairs);
+int qeq_solve_batch(const BatchPlan& plan, const QeqBatch& B, const double* q0, const double* tol,
+                    int max_iter, int check_every, double* q, int32_t* iters, double* resid,
+                    char* ws, hipStream_t s);
 
 }  // namespace admp

@@ -93,6 +93,13 @@ _SIGNATURES = {
     "admp_qeq_matvec": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_sz, _c_vp]),
     "admp_qeq_solve": (ctypes.c_int, [_c_vp, _c_vp, _c_dbl, _c_i32, _c_i32, _c_vp, _P(_c_i32),
                                       _P(_c_dbl), _c_vp, _c_sz, _c_vp]),
+    "admp_qeq_solve_batch_workspace_bytes": (ctypes.c_int, [_c_i32, _c_vp, _c_vp, _c_vp, _c_vp,
+                                                            _c_i32, _c_i64, _P(_c_sz)]),
+    "admp_qeq_solve_batch": (ctypes.c_int, [_c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i32,
+                                            _c_i32, _c_dbl, _c_i32, _c_vp, _c_vp, _c_vp, _c_vp,
+                                            _c_i64, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
+                                            _c_vp, _c_i32, _c_i32, _c_vp, _c_vp, _c_vp, _c_vp,
+                                            _c_sz, _c_vp]),
     "admp_profile_begin": (ctypes.c_int, [_c_i32]),
     "admp_profile_collect": (ctypes.c_int, [_P(_c_dbl), _P(_c_i32)]),
 }

@@ -492,3 +492,117 @@ class _QEqImplicit(torch.autograd.Function):
             for (i, t), gr in zip(want, out):
                 grads[i] = None if gr is None else -gr
         return tuple(grads)
+
+
+def solve_batch(positions: List[torch.Tensor], boxes: List[torch.Tensor],
+                chi: List[torch.Tensor], hardness: List[torch.Tensor], eta: List[torch.Tensor],
+                rcut: Union[float, List[float]], ethresh: Union[float, List[float]] = 1e-5,
+                total_charge: Optional[List[float]] = None,
+                tol: Union[float, List[float]] = 1e-8, max_iter: int = 1000, check_every: int = 4,
+                damping: bool = True, kappa: Optional[List[float]] = None,
+                kmesh: Optional[List[List[int]]] = None, precision: str = "float64",
+                convention: str = "reference", q0: Optional[List[torch.Tensor]] = None):
+    """Charge equilibration of several periodic systems at once.
+
+    System s has positions (n_s, 3), box (3, 3) rows, chi, hardness (the q^2
+    coefficient) and Gaussian widths eta (n_s,) on the device, in the
+    reference's default units (Angstrom, eV).  Each is solved for the minimum
+    of QEqForceModule's energy (Ewald real and reciprocal space with
+    CoulombForceModule's kappa and mesh rule for ``ethresh``, Gaussian damping
+    when ``damping``) on sum q = total_charge[s], to |P grad E| / n_s < tol.
+    ``rcut``, ``ethresh`` and ``tol`` are one value or one per system.
+    All systems advance together through admp_qeq_solve_batch: one Hessian
+    product for the whole batch per iteration, one CG step per system, and
+    each system stops when it converges.  Returns (charges per system,
+    iterations per system (-1 if not converged), residuals per system).
+
+    The reference solves one frame per call (src/torch_admp/qeq.py:609-765);
+    each system's fixed point is the one QEqForceModule.solve_pgrad reaches.
+    """
+    from .nblist import TorchNeighborList
+    from .utils import DIELECTRIC
+
+    nsys = len(positions)
+    if nsys == 0:
+        return [], [], []
+    _lib.require_gpu(*positions)
+    dev = positions[0].device
+    f64 = torch.float64
+    eps = DIELECTRIC
+    per = lambda v: [float(x) for x in v] if isinstance(v, (list, tuple)) else [float(v)] * nsys  # noqa: E731
+    rcs, eths, tols = per(rcut), per(ethresh), per(tol)
+    n_atoms, pos_l, pairs_l, ds_l, bs_l, hd_l, pc_l, chi_l, eta_l, q0_l = ([] for _ in range(10))
+    boxes_h, kap_h, km_h = [], [], []
+    off = 0
+    for s in range(nsys):
+        pos = positions[s].detach().reshape(-1, 3).to(f64).contiguous()
+        box = boxes[s].detach().reshape(3, 3).to(f64)
+        n = pos.shape[0]
+        # CoulombForceModule's rule (src/torch_admp/pme.py:113-115,383-392)
+        kap = (float(kappa[s]) if kappa is not None
+               else math.sqrt(-math.log(2.0 * eths[s])) / rcs[s])
+        km = ([int(k) for k in kmesh[s]] if kmesh is not None else
+              [int(math.ceil(2.0 * kap * float(box[d, d]) / (3.0 * eths[s] ** 0.2)))
+               for d in range(3)])
+        nbl = TorchNeighborList(rcs[s])
+        with torch.no_grad():
+            prs = nbl(pos, box.to(dev)).to(torch.int64).reshape(-1, 2)
+            ds = nbl.get_ds().reshape(-1).to(f64)
+            bs = nbl.get_buffer_scales().reshape(-1).to(f64)
+        hard = hardness[s].detach().reshape(-1).to(f64)
+        et = eta[s].detach().reshape(-1).to(f64)
+        hdiag = 2 * hard - 2 * eps * kap / math.sqrt(math.pi)
+        pc = 2 * hard
+        if damping:
+            dself = eps * safe_inverse(et, 1e-4) / math.sqrt(math.pi)
+            hdiag = hdiag + dself
+            pc = pc + dself
+        n_atoms.append(n)
+        pos_l.append(pos)
+        pairs_l.append(prs + off)
+        ds_l.append(ds)
+        bs_l.append(bs)
+        hd_l.append(hdiag)
+        pc_l.append(pc)
+        chi_l.append(chi[s].detach().reshape(-1).to(f64))
+        eta_l.append(et)
+        q0_l.append(torch.zeros(n, dtype=f64, device=dev) if q0 is None
+                    else q0[s].detach().reshape(-1).to(f64))
+        boxes_h.append(box.cpu().numpy().reshape(-1))
+        kap_h.append(kap)
+        km_h.append(km)
+        off += n
+    cat = lambda xs: torch.cat(xs).contiguous()  # noqa: E731
+    pos_a, prs_a, ds_a, bs_a = cat(pos_l), cat(pairs_l), cat(ds_l), cat(bs_l)
+    hd_a, chi_a, eta_a, q0_a = cat(hd_l), cat(chi_l), cat(eta_l), cat(q0_l)
+    pc_a = cat(pc_l)
+    precond = pc_a if bool((pc_a > 0).all()) else None
+    import numpy as np
+
+    n_arr = np.asarray(n_atoms, dtype=np.int64)
+    box_arr = np.ascontiguousarray(np.stack(boxes_h), dtype=np.float64)
+    rc_arr = np.asarray(rcs, dtype=np.float64)
+    tol_arr = np.asarray(tols, dtype=np.float64)
+    kap_arr = np.asarray(kap_h, dtype=np.float64)
+    km_arr = np.ascontiguousarray(np.asarray(km_h, dtype=np.int32))
+    tq = np.asarray([0.0] * nsys if total_charge is None else total_charge, dtype=np.float64)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    lib = _lib.load()
+    prec = _lib.PRECISION[precision]
+    nbytes = ctypes.c_size_t()
+    _lib.check(lib.admp_qeq_solve_batch_workspace_bytes(nsys, p(n_arr), p(box_arr), p(rc_arr),
+                                                        p(km_arr), prec, prs_a.shape[0],
+                                                        ctypes.byref(nbytes)),
+               "admp_qeq_solve_batch_workspace_bytes")
+    ws = _lib.workspace(nbytes.value, dev)
+    q = torch.empty_like(q0_a)
+    iters = np.zeros(nsys, dtype=np.int32)
+    resid = np.zeros(nsys, dtype=np.float64)
+    _lib.check(lib.admp_qeq_solve_batch(
+        nsys, p(n_arr), p(box_arr), p(rc_arr), p(kap_arr), p(km_arr), prec,
+        _lib.KCONV[convention], eps, QEQ_DAMPING if damping else 0, _lib.ptr(pos_a),
+        _lib.ptr(prs_a), _lib.ptr(ds_a), _lib.ptr(bs_a), prs_a.shape[0], _lib.ptr(eta_a),
+        _lib.ptr(hd_a), _lib.ptr(chi_a), _lib.ptr(precond), p(tq), _lib.ptr(q0_a), p(tol_arr),
+        int(max_iter), int(check_every), _lib.ptr(q), p(iters), p(resid), _lib.ptr(ws),
+        nbytes.value, _lib.stream_of(pos_a)), "admp_qeq_solve_batch")
+    return (list(torch.split(q, n_atoms)), [int(i) for i in iters], [float(r) for r in resid])

@@ -12,12 +12,12 @@ stream and memory free up, so their work overlaps on the device with the
 batches already there.  Each request's result is the same whether it runs
 alone or in a mix (the batched kernels treat every system independently).
 
-QEq requests (charge equilibration) share the HBM budget.  A QEq solve checks
-its residual on the host every few iterations, so it is not folded into a
-PME launch: a worker thread admits QEq requests in arrival order as the
-budget allows and solves each on its own stream with the library's projected
-CG (QEqForceModule.solve_implicit, forward only), concurrently with the PME
-batches.
+QEq requests (charge equilibration) share the HBM budget.  A worker thread
+admits the queued QEq requests in arrival order as the budget allows (up to
+``max_qeq_batch`` at a time) and solves them together on its own stream,
+concurrently with the PME batches: admp_qeq_solve_batch forms all their
+Hessian products in one launch sequence per iteration and each solve stops at
+its own tolerance (a device-side flag, read every few iterations).
 
 The reference evaluates one frame per CoulombForceModule call
 (src/torch_admp/pme.py:261-310); this module has no counterpart there.
@@ -195,7 +195,7 @@ class PMEServer:
 
     def __init__(self, device="cuda", hbm_budget: Optional[int] = None, max_systems: int = 1024,
                  streams: int = 2, precision: str = "float32", convention: str = "reference",
-                 dielectric: float = DIELECTRIC):
+                 dielectric: float = DIELECTRIC, max_qeq_batch: int = 64):
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise RuntimeError("PMEServer runs only on a ROCm GPU (no CPU fallback)")
@@ -204,6 +204,8 @@ class PMEServer:
             hbm_budget = int(free * 0.5)
         self.hbm_budget = int(hbm_budget)
         self.max_systems = int(max_systems)
+        self.max_qeq_batch = max(1, int(max_qeq_batch))
+        self.qeq_batches = 0
         self.precision = precision
         self.convention = convention
         self.dielectric = float(dielectric)
@@ -286,44 +288,57 @@ class PMEServer:
                     self._cv.wait(0.01)
                 if self._stop:
                     return
-                req = self.qeq_queue.popleft()
-                self._reserve(req.cost)
+                # every queued solve that fits the budget, in arrival order
+                take: List[QEqRequest] = []
+                cost = 0
+                while (self.qeq_queue and len(take) < self.max_qeq_batch and
+                       self.bytes_in_flight + cost + self.qeq_queue[0].cost <= self.hbm_budget):
+                    r = self.qeq_queue.popleft()
+                    take.append(r)
+                    cost += r.cost
+                self._reserve(cost)
                 self._qeq_busy = True
-            req.t_admit = self.now()
+            t = self.now()
+            for r in take:
+                r.t_admit = t
             try:
                 with torch.cuda.stream(stream):
-                    self._solve_qeq(req)
+                    self._solve_qeq_batch(take)
             except Exception as exc:  # reported by pump() on the serving thread
-                req.error = repr(exc)
-            req.t_done = self.now()
+                for r in take:
+                    r.error = repr(exc)
+            t = self.now()
+            for r in take:
+                r.t_done = t
             with self._cv:
-                self.bytes_in_flight -= req.cost
+                self.bytes_in_flight -= cost
                 self._qeq_busy = False
-                self._qeq_done.append(req)
+                self._qeq_done.extend(take)
+                self.qeq_batches += 1
                 self._cv.notify_all()
 
-    def _solve_qeq(self, req: QEqRequest) -> None:
-        from .nblist import TorchNeighborList
-        from .qeq import QEqForceModule
+    def _solve_qeq_batch(self, reqs: List[QEqRequest]) -> None:
+        """The solves of `reqs` together (qeq.solve_batch): one Hessian product
+        for all of them per iteration, each stopping at its own tolerance."""
+        from .qeq import solve_batch
 
         def dev(a):
             return torch.tensor(a, dtype=torch.float64, device=self.device)
 
-        n = req.n_atoms
-        pos, box = dev(req.positions), dev(req.box)
-        chi, hard, eta = dev(req.chi), dev(req.hardness), dev(req.eta)
-        mod = QEqForceModule(req.rcut, req.ethresh, kappa=req.kappa, kmesh=req.kmesh,
-                             precision="float64", convention=self.convention)
-        nbl = TorchNeighborList(req.rcut)
         with torch.no_grad():
-            pairs = nbl(pos, box)
-            ds, bs = nbl.get_ds(), nbl.get_buffer_scales()
-            A = torch.ones(1, n, dtype=torch.float64, device=self.device)
-            b = torch.full((1,), float(req.total_charge), dtype=torch.float64, device=self.device)
-            q = mod.solve_implicit(pos, box, chi, hard, eta, pairs, ds, bs, A, b, tol=req.tol)
-        req.charges = q.detach().cpu().numpy()  # waits for this stream only
-        req.iterations = int(mod.converge_iter)
-        req.residual = float(mod.residual)
+            qs, iters, resid = solve_batch(
+                [dev(r.positions) for r in reqs], [dev(r.box) for r in reqs],
+                [dev(r.chi) for r in reqs], [dev(r.hardness) for r in reqs],
+                [dev(r.eta) for r in reqs], [r.rcut for r in reqs],
+                total_charge=[float(r.total_charge) for r in reqs], tol=[r.tol for r in reqs],
+                kappa=[r.kappa for r in reqs], kmesh=[r.kmesh for r in reqs],
+                convention=self.convention)
+        for r, q, it, res in zip(reqs, qs, iters, resid):
+            r.charges = q.cpu().numpy()  # waits for this stream only
+            r.iterations = it
+            r.residual = res
+            if it < 0:
+                r.error = f"QEq solve did not converge (|P grad E| / n = {res:.3e})"
 
     def __enter__(self) -> "PMEServer":
         return self
