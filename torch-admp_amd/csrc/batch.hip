@@ -243,8 +243,9 @@ __global__ void __launch_bounds__(kSpreadThreads) k_b_spread_tiles(int nsys,
 }
 
 // systems with every mesh dimension >= kBrickMinK: one work-group per mesh
-// brick (global brick index, XCD-local), see spread_brick
-template <typename T>
+// brick (global brick index, XCD-local), see spread_brick; ZW = the system
+// table's brick value (the launch covers only the systems of that kind)
+template <typename T, int ZW>
 __global__ void __launch_bounds__(kBrickThreads) k_b_spread_bricks(int nsys,
                                                         const int64_t* __restrict__ brick_off,
                                                         const SysDesc* __restrict__ tab,
@@ -256,15 +257,16 @@ __global__ void __launch_bounds__(kBrickThreads) k_b_spread_bricks(int nsys,
                                                         int4* __restrict__ stm,
                                                         T* __restrict__ stw,
                                                         T* __restrict__ mesh) {
-  __shared__ BrickStage<T> S;
+  __shared__ BrickStage<T, ZW> S;
   const int64_t b = xcd_block(blockIdx.x, gridDim.x);
   const int sid = find_seg(brick_off, nsys, b);
   const SysDesc* D = tab + sid;
+  if (D->brick != ZW) return;
   const Geom g = D->g;
   const int nc[3] = {D->cg.nc[0], D->cg.nc[1], D->cg.nc[2]};
   const int nbr[3] = {D->nbr[0], D->nbr[1], D->nbr[2]};
-  spread_brick<T>(g, nc, D->cell0, nbr, b - brick_off[sid], ps, mref, sorted, start, qv, stm,
-                  stw, mesh + D->mesh0, S);
+  spread_brick<T, ZW>(g, nc, D->cell0, nbr, b - brick_off[sid], ps, mref, sorted, start, qv,
+                      stm, stw, mesh + D->mesh0, S);
 }
 
 // atoms of systems whose tiles do not fit: global atomics into their mesh
@@ -337,12 +339,14 @@ __global__ void __launch_bounds__(kBlock) k_b_cols_y(int nsys, const int64_t* __
 // times eps, value-major with nrows rows)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_b_cols_x(int nsys, const int64_t* __restrict__ off,
+                                                     const int64_t* __restrict__ xoff,
                                                      const SysDesc* __restrict__ tab,
                                                      cplx<T>* __restrict__ specs,
                                                      double* __restrict__ partials, int64_t nrows) {
   const int sid = find_seg(off, nsys, blockIdx.x);
   const SysDesc* D = tab + sid;
   if (D->plane) return;
+  const int64_t row = xoff[sid] + (blockIdx.x - off[sid]);  // the system's partial row
   InflParams P;
   P.g = D->g;
   P.kappa = D->kappa;
@@ -357,14 +361,15 @@ __global__ void __launch_bounds__(kBlock) k_b_cols_x(int nsys, const int64_t* __
   x_convolve_lines<T>(specs + D->spec0, (int)(lb / nchunk), (int)(lb % nchunk) * C, C, D->rad[0],
                       P, vals);
   __syncthreads();
-  if (threadIdx.x < 10) partials[(int64_t)threadIdx.x * nrows + blockIdx.x] = vals[threadIdx.x] * P.eps;
+  if (threadIdx.x < 10) partials[(int64_t)threadIdx.x * nrows + row] = vals[threadIdx.x] * P.eps;
 }
 
-// one kz plane per work-group (systems with plane set; same partial rows as
-// k_b_cols_x, whose segment the planes share)
+// one kz plane per work-group (systems with plane set; their partial rows are
+// the system's x-pass rows xoff[sid] + kz)
 template <typename T>
 __global__ void __launch_bounds__(kPlaneThreads) k_b_plane(int nsys,
                                                          const int64_t* __restrict__ off,
+                                                         const int64_t* __restrict__ xoff,
                                                          const SysDesc* __restrict__ tab,
                                                          cplx<T>* __restrict__ specs,
                                                          double* __restrict__ partials,
@@ -372,6 +377,7 @@ __global__ void __launch_bounds__(kPlaneThreads) k_b_plane(int nsys,
   const int sid = find_seg(off, nsys, blockIdx.x);
   const SysDesc* D = tab + sid;
   if (!D->plane) return;
+  const int64_t row = xoff[sid] + (blockIdx.x - off[sid]);
   InflParams P;
   P.g = D->g;
   P.kappa = D->kappa;
@@ -383,7 +389,7 @@ __global__ void __launch_bounds__(kPlaneThreads) k_b_plane(int nsys,
   plane_convolve<T>(specs + D->spec0, (int)(blockIdx.x - off[sid]), D->rad[0], D->rad[1], P,
                     vals);
   __syncthreads();
-  if (threadIdx.x < 10) partials[(int64_t)threadIdx.x * nrows + blockIdx.x] = vals[threadIdx.x] * P.eps;
+  if (threadIdx.x < 10) partials[(int64_t)threadIdx.x * nrows + row] = vals[threadIdx.x] * P.eps;
 }
 
 // ------------------------------------------------------------------ gather
@@ -619,9 +625,17 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
     D.tg = make_tile_geom(D.cg, km, es);
     D.brick = km[0] >= kBrickMinK && km[1] >= kBrickMinK && km[2] >= kBrickMinK &&
               km[0] < 32768 && km[1] < 32768 ? 1 : 0;
+    // deep bricks where a brick of kBrickX kBrickY kBrickZ points would see
+    // few atoms (>= kDeepBrickPts mesh points per atom) and a deep brick
+    // plus a stencil fits along z
+    const double pts = (double)km[0] * km[1] * km[2];
+    if (D.brick && pts >= kDeepBrickPts * (double)std::max<int64_t>(n_atoms[s], 1) &&
+        km[2] >= kBrickZ * kBrickWaves + 6)
+      D.brick = kBrickWaves;
+    const int zb = kBrickZ * D.brick;
     D.nbr[0] = D.brick ? (km[0] + kBrickX - 1) / kBrickX : 0;
     D.nbr[1] = D.brick ? (km[1] + kBrickY - 1) / kBrickY : 0;
-    D.nbr[2] = D.brick ? (km[2] + kBrickZ - 1) / kBrickZ : 0;
+    D.nbr[2] = D.brick ? (km[2] + zb - 1) / zb : 0;
     D.kappa = kappas ? kappas[s] : 0.0;
     D.eps = eps;
     D.conv = convention;
@@ -646,10 +660,14 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
     D.plane = plane_fits(K0, K1, D.rad[0], D.rad[1], lds_plane) ? 1 : 0;
     P.off[kSegFftY][s + 1] =
         P.off[kSegFftY][s] + (D.plane ? 0 : (int64_t)K0 * ((Kzh + D.fchunk[1] - 1) / D.fchunk[1]));
-    P.off[kSegFftX][s + 1] =
-        P.off[kSegFftX][s] + (D.plane ? Kzh : (int64_t)K1 * ((Kzh + D.fchunk[2] - 1) / D.fchunk[2]));
+    const int64_t xrows = D.plane ? Kzh : (int64_t)K1 * ((Kzh + D.fchunk[2] - 1) / D.fchunk[2]);
+    P.off[kSegFftX][s + 1] = P.off[kSegFftX][s] + xrows;
+    P.off[kSegPlane][s + 1] = P.off[kSegPlane][s] + (D.plane ? xrows : 0);
+    P.off[kSegColsX][s + 1] = P.off[kSegColsX][s] + (D.plane ? 0 : xrows);
     P.off[kSegGather][s + 1] = P.off[kSegGather][s] + (D.n + kGatherAtomsPerBlock - 1) / kGatherAtomsPerBlock;
-    P.off[kSegBrick][s + 1] = P.off[kSegBrick][s] + (int64_t)D.nbr[0] * D.nbr[1] * D.nbr[2];
+    const int64_t nbrk = (int64_t)D.nbr[0] * D.nbr[1] * D.nbr[2];
+    P.off[kSegBrick][s + 1] = P.off[kSegBrick][s] + (D.brick == 1 ? nbrk : 0);
+    P.off[kSegBrickZ][s + 1] = P.off[kSegBrickZ][s] + (D.brick == kBrickWaves ? nbrk : 0);
     P.lds_fft[0] = std::max(P.lds_fft[0], (size_t)(K2 + 2 * K2 * D.fchunk[0]) * ces);
     if (D.plane) {
       P.lds_plane = std::max(P.lds_plane, lds_plane);
@@ -837,10 +855,14 @@ static void batch_spread(const BatchPlan& P, const double* pos, const double* q,
   T* stw = (T*)(ws + P.o_stw);
   T* mesh = (T*)(ws + P.o_mesh);
   const int64_t ntot = P.ntot;
-  const int64_t nbricks = P.off[kSegBrick][nsys];
+  const int64_t nbricks = P.off[kSegBrick][nsys], ndeep = P.off[kSegBrickZ][nsys];
   if (P.any_brick && nbricks > 0)
-    k_b_spread_bricks<T><<<(unsigned)nbricks, kBrickThreads, 0, s>>>(
+    k_b_spread_bricks<T, 1><<<(unsigned)nbricks, kBrickThreads, 0, s>>>(
         nsys, seg(kSegBrick), tab, (const P4<double>*)(ws + P.o_ps),
+        (const int2*)(ws + P.o_mref), sorted, start, qv, stm, stw, mesh);
+  if (P.any_brick && ndeep > 0)
+    k_b_spread_bricks<T, kBrickWaves><<<(unsigned)ndeep, kBrickThreads, 0, s>>>(
+        nsys, seg(kSegBrickZ), tab, (const P4<double>*)(ws + P.o_ps),
         (const int2*)(ws + P.o_mref), sorted, start, qv, stm, stw, mesh);
   if (ntot > 0) {
     if (P.lds_tile > 0)
@@ -866,15 +888,18 @@ static void batch_fft(const BatchPlan& P, char* ws, hipStream_t s) {
   const int64_t rows_x = P.off[kSegFftX][nsys];
   const unsigned bz = (unsigned)P.off[kSegFftZ][nsys], by = (unsigned)P.off[kSegFftY][nsys],
                  bx = (unsigned)rows_x;
+  (void)bx;
   double* cpart = (double*)(ws + P.o_cpart);
   k_b_r2c<T><<<bz, kBlock, P.lds_fft[0], s>>>(nsys, seg(kSegFftZ), tab, mesh, spec);
   if (by > 0)
     k_b_cols_y<T, -1><<<by, kBlock, P.lds_fft[1], s>>>(nsys, seg(kSegFftY), tab, spec);
-  if (P.any_cols)
-    k_b_cols_x<T><<<bx, kBlock, P.lds_fft[2], s>>>(nsys, seg(kSegFftX), tab, spec, cpart, rows_x);
-  if (P.any_plane)
-    k_b_plane<T><<<bx, kPlaneThreads, P.lds_plane, s>>>(nsys, seg(kSegFftX), tab, spec, cpart,
-                                                        rows_x);
+  const unsigned ncols = (unsigned)P.off[kSegColsX][nsys], nplane = (unsigned)P.off[kSegPlane][nsys];
+  if (P.any_cols && ncols > 0)
+    k_b_cols_x<T><<<ncols, kBlock, P.lds_fft[2], s>>>(nsys, seg(kSegColsX), seg(kSegFftX), tab,
+                                                      spec, cpart, rows_x);
+  if (P.any_plane && nplane > 0)
+    k_b_plane<T><<<nplane, kPlaneThreads, P.lds_plane, s>>>(nsys, seg(kSegPlane), seg(kSegFftX),
+                                                            tab, spec, cpart, rows_x);
   if (by > 0)
     k_b_cols_y<T, 1><<<by, kBlock, P.lds_fft[1], s>>>(nsys, seg(kSegFftY), tab, spec);
   k_b_c2r<T><<<bz, kBlock, P.lds_fft[0], s>>>(nsys, seg(kSegFftZ), tab, spec, mesh);

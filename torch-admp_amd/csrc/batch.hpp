@@ -28,7 +28,8 @@ struct SysDesc {
   int fchunk[3];  // lines per block: z passes (row pairs), y passes, x pass
   int plane;      // the y and x passes run as one LDS-resident plane per kz
                   // (the half spectrum is then stored kz-major)
-  int brick;      // spread by mesh bricks (owner computes, no atomics)
+  int brick;      // spread by mesh bricks (owner computes, no atomics): 1 bricks of
+                  // kBrickZ points along z, kBrickWaves deeper bricks (dense meshes)
   int nbr[3];     // bricks per mesh axis
   int64_t atom0, n;
   int64_t cell0, ncell;
@@ -36,8 +37,13 @@ struct SysDesc {
 };
 
 // Segment offsets (n_sys + 1 each) used to map a global index to its system.
+// kSegFftX numbers the x-pass partial rows of every system (Kzh planes or the
+// line work-groups); kSegPlane and kSegColsX number the work-groups of the
+// plane pass and of the x line pass only, so that neither kernel is launched
+// over the other's systems.
 enum SegKind {
-  kSegAtom = 0, kSegCell, kSegMesh, kSegFftZ, kSegFftY, kSegFftX, kSegGather, kSegBrick, kSegCount
+  kSegAtom = 0, kSegCell, kSegMesh, kSegFftZ, kSegFftY, kSegFftX, kSegGather, kSegBrick,
+  kSegPlane, kSegColsX, kSegBrickZ, kSegCount
 };
 
 struct BatchPlan {

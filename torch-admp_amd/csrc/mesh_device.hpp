@@ -223,6 +223,10 @@ constexpr int kBrickCells = 64;   // cells whose candidates are scanned together
 // Smallest mesh dimension of the brick path: with K >= kBrickZ + 6 a stencil
 // meets a brick in one run of points (no second meeting through the wrap).
 constexpr int kBrickMinK = 14;
+// Mesh points per atom from which a system's bricks are kBrickWaves times
+// deeper along z (the serving stream's rule meshes have ~300, the 24k-atom
+// water box at 64^3 has 11).
+constexpr int kDeepBrickPts = 64;
 static_assert(kBrickX == 8 && kBrickY == 8 && kBrickZ == 8,
               "lane (x, y) of a wavefront owns one column; lines of 8 points");
 
@@ -242,11 +246,20 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 }
 
 // Collected atoms whose weights are laid out at once: one per thread, half
-// the threads in float64 (the LDS of the lines).
-template <typename T>
-constexpr int brick_chunk() { return sizeof(T) == 4 ? kBrickThreads : kBrickThreads / 2; }
+// the threads in float64 (the LDS of the lines).  With the brick deepened
+// along z (ZW > 1, dense meshes: a few atoms per brick) 64 in both precisions.
+template <typename T, int ZW = 1>
+constexpr int brick_chunk() {
+  return ZW > 1 ? 64 : (sizeof(T) == 4 ? kBrickThreads : kBrickThreads / 2);
+}
 
-template <typename T>
+// ZW = 1: a brick of kBrickX x kBrickY x kBrickZ points, the work-group's
+// wavefronts split the collected atoms and their column sums are added in
+// LDS.  ZW = kBrickWaves: a brick ZW times deeper along z, wavefront w owns
+// the points [w kBrickZ, (w + 1) kBrickZ) of every column and adds every
+// collected atom (no cross-wavefront sums); for meshes with many points per
+// atom this takes ZW times fewer work-groups through the same scan.
+template <typename T, int ZW = 1>
 struct BrickStage {
   int slot[kBrickList];             // sorted slot of each collected atom
   int rel[kBrickList];              // stencil start - brick origin + 8, one byte per axis
@@ -256,9 +269,9 @@ struct BrickStage {
   union {
     struct {
       // weights laid onto the brick's lines (zero where the stencil misses)
-      __align__(16) T qx[brick_chunk<T>()][kBrickX];  // q wx
-      __align__(16) T wy[brick_chunk<T>()][kBrickY];
-      __align__(16) T wz[brick_chunk<T>()][kBrickZ];
+      __align__(16) T qx[brick_chunk<T, ZW>()][kBrickX];  // q wx
+      __align__(16) T wy[brick_chunk<T, ZW>()][kBrickY];
+      __align__(16) T wz[brick_chunk<T, ZW>()][kBrickZ * ZW];
     } w;
     double red[kBrickWaves - 1][kBrickZ * 64];  // the other wavefronts' column sums
   } u;
@@ -289,13 +302,26 @@ __device__ __forceinline__ void lay_line(const T w[6], int s, T scale, T* __rest
   l4[1] = tv4{v[4], v[5], v[6], v[7]};
 }
 
+// The same onto a line of N points (a multiple of 4): zeros, then the six
+// weights that fall inside the line (one thread writes the whole line).
+template <typename T, int N>
+__device__ __forceinline__ void lay_line_n(const T w[6], int s, T scale, T* __restrict__ line) {
+  typedef T tv4 __attribute__((ext_vector_type(4)));
+  tv4* l4 = reinterpret_cast<tv4*>(line);
+#pragma unroll
+  for (int u = 0; u < N / 4; ++u) l4[u] = tv4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int c = 0; c < 6; ++c)
+    if (s + c >= 0 && s + c < N) line[s + c] = scale * w[c];
+}
+
 // The collected atoms [0, cnt), brick_chunk<T>() at a time: weights (float64
 // B-splines rounded once to T, as every other spread path) laid onto the
 // brick's lines, stencil records of the atoms this brick owns, then each
 // wavefront's contiguous share of the column sums.  Ends with a barrier.
-template <typename T>
+template <typename T, int ZW>
 __device__ __forceinline__ void brick_pass(const Geom& g, const int lo[3], const int W[3], int cnt,
-                                           BrickStage<T>& S, const P4<double>* __restrict__ ps,
+                                           BrickStage<T, ZW>& S, const P4<double>* __restrict__ ps,
                                            const int* __restrict__ sorted,
                                            const double* __restrict__ qv, int4* __restrict__ stm,
                                            T* __restrict__ stw, double (&acc)[kBrickZ]) {
@@ -303,7 +329,7 @@ __device__ __forceinline__ void brick_pass(const Geom& g, const int lo[3], const
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int x = lane >> 3, y = lane & 7;
   typedef T tv4 __attribute__((ext_vector_type(4)));
-  constexpr int kChunk = brick_chunk<T>();
+  constexpr int kChunk = brick_chunk<T, ZW>();
   for (int c0 = 0; c0 < cnt; c0 += kChunk) {
     const int k = c0 + tid;
     if (tid < kChunk && k < cnt) {
@@ -329,7 +355,8 @@ __device__ __forceinline__ void brick_pass(const Geom& g, const int lo[3], const
         const int s = ((rel >> (8 * d)) & 255) - 8;
         if (d == 0) lay_line<T>(w, s, (T)(qv ? qv[iv] : p.q), S.u.w.qx[tid]);
         else if (d == 1) lay_line<T>(w, s, T(1), S.u.w.wy[tid]);
-        else lay_line<T>(w, s, T(1), S.u.w.wz[tid]);
+        else if (ZW == 1) lay_line<T>(w, s, T(1), S.u.w.wz[tid]);
+        else lay_line_n<T, kBrickZ * ZW>(w, s, T(1), S.u.w.wz[tid]);
         if (own) {
           typedef T tv2 __attribute__((ext_vector_type(2)));
           tv2* pw = reinterpret_cast<tv2*>(rec + 6 * d);
@@ -345,9 +372,12 @@ __device__ __forceinline__ void brick_pass(const Geom& g, const int lo[3], const
     }
     __syncthreads();
     const int nk = min(kChunk, cnt - c0);
-    const int qn = (nk + kBrickWaves - 1) / kBrickWaves;
-    const int e1 = min((wave + 1) * qn, nk);
-    int e = wave * qn;
+    // ZW = 1: wavefront w takes its share of the atoms; ZW > 1: every atom,
+    // at the z points [w kBrickZ, (w + 1) kBrickZ) of the lines
+    const int qn = ZW > 1 ? nk : (nk + kBrickWaves - 1) / kBrickWaves;
+    const int e1 = ZW > 1 ? nk : min((wave + 1) * qn, nk);
+    int e = ZW > 1 ? 0 : wave * qn;
+    const int zo = ZW > 1 ? wave * kBrickZ : 0;
     // four collected atoms a step, their LDS reads issued together
     for (; e + 4 <= e1; e += 4) {
       T a[4], b[4];
@@ -356,8 +386,8 @@ __device__ __forceinline__ void brick_pass(const Geom& g, const int lo[3], const
       for (int u = 0; u < 4; ++u) {
         a[u] = S.u.w.qx[e + u][x];
         b[u] = S.u.w.wy[e + u][y];
-        z0[u] = reinterpret_cast<const tv4*>(S.u.w.wz[e + u])[0];
-        z1[u] = reinterpret_cast<const tv4*>(S.u.w.wz[e + u])[1];
+        z0[u] = reinterpret_cast<const tv4*>(S.u.w.wz[e + u] + zo)[0];
+        z1[u] = reinterpret_cast<const tv4*>(S.u.w.wz[e + u] + zo)[1];
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -374,8 +404,8 @@ __device__ __forceinline__ void brick_pass(const Geom& g, const int lo[3], const
     }
     for (; e < e1; ++e) {
       const T v = S.u.w.qx[e][x] * S.u.w.wy[e][y];
-      const tv4 z0 = reinterpret_cast<const tv4*>(S.u.w.wz[e])[0];
-      const tv4 z1 = reinterpret_cast<const tv4*>(S.u.w.wz[e])[1];
+      const tv4 z0 = reinterpret_cast<const tv4*>(S.u.w.wz[e] + zo)[0];
+      const tv4 z1 = reinterpret_cast<const tv4*>(S.u.w.wz[e] + zo)[1];
       acc[0] += (double)(v * z0.x);
       acc[1] += (double)(v * z0.y);
       acc[2] += (double)(v * z0.z);
@@ -398,7 +428,7 @@ __device__ __forceinline__ void brick_pass(const Geom& g, const int lo[3], const
 // 1) / nc) up to the rounding of the binning, and a brick [lo, lo + W)
 // receives from atoms with lo - 2 <= m0 <= lo + W + 2: the cells scanned cover
 // that range with one more point on each side.
-template <typename T>
+template <typename T, int ZW>
 __device__ __forceinline__ void spread_brick(const Geom& g, const int nc[3], int64_t cell0,
                                              const int nbr[3], int64_t lb,
                                              const P4<double>* __restrict__ ps,
@@ -407,15 +437,15 @@ __device__ __forceinline__ void spread_brick(const Geom& g, const int nc[3], int
                                              const int* __restrict__ start,
                                              const double* __restrict__ qv, int4* __restrict__ stm,
                                              T* __restrict__ stw, T* __restrict__ mesh,
-                                             BrickStage<T>& S) {
+                                             BrickStage<T, ZW>& S) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int K[3] = {g.k[0], g.k[1], g.k[2]};
   const int bz = (int)(lb % nbr[2]), by = (int)((lb / nbr[2]) % nbr[1]),
             bx = (int)(lb / ((int64_t)nbr[1] * nbr[2]));
-  const int lo[3] = {bx * kBrickX, by * kBrickY, bz * kBrickZ};
+  const int lo[3] = {bx * kBrickX, by * kBrickY, bz * kBrickZ * ZW};
   const int W[3] = {min(kBrickX, K[0] - lo[0]), min(kBrickY, K[1] - lo[1]),
-                    min(kBrickZ, K[2] - lo[2])};
+                    min(kBrickZ * ZW, K[2] - lo[2])};
   int cmin[3], clen[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -511,13 +541,24 @@ __device__ __forceinline__ void spread_brick(const Geom& g, const int nc[3], int
       cnt += add;
       __syncthreads();
       if (cnt >= kBrickThreads) {
-        brick_pass<T>(g, lo, W, cnt, S, ps, sorted, qv, stm, stw, acc);
+        brick_pass<T, ZW>(g, lo, W, cnt, S, ps, sorted, qv, stm, stw, acc);
         cnt = 0;
       }
     }
     __syncthreads();  // every thread has read cpre before the next group rewrites it
   }
-  if (cnt > 0) brick_pass<T>(g, lo, W, cnt, S, ps, sorted, qv, stm, stw, acc);
+  if (cnt > 0) brick_pass<T, ZW>(g, lo, W, cnt, S, ps, sorted, qv, stm, stw, acc);
+  if (ZW > 1) {
+    // every wavefront stores its own z points of the columns
+    const int x = lane >> 3, y = lane & 7, z0 = wave * kBrickZ;
+    if (x < W[0] && y < W[1] && z0 < W[2]) {
+      T* col = mesh + ((int64_t)(lo[0] + x) * K[1] + lo[1] + y) * K[2] + lo[2] + z0;
+#pragma unroll
+      for (int z = 0; z < kBrickZ; ++z)
+        if (z0 + z < W[2]) col[z] = (T)acc[z];
+    }
+    return;
+  }
   // the wavefronts' column sums; wavefront 0 stores the brick
   if (wave > 0) {
 #pragma unroll
