@@ -203,18 +203,23 @@ __global__ void __launch_bounds__(kBlock) k_b_scatter_pack(int64_t ntot, int nco
 // 28 us beside it instead of 14.
 template <typename T>
 __global__ void __launch_bounds__(kRealBlock) __attribute__((amdgpu_num_vgpr(104))) k_b_real(int nsys,
-                                                   const int64_t* __restrict__ cell_off,
+                                                   const int64_t* __restrict__ real_off,
                                                    const SysDesc* __restrict__ tab,
                                                    const P4<double>* __restrict__ ps,
                                                    const int* __restrict__ sorted,
                                                    const int* __restrict__ start,
                                                    double* __restrict__ grad,
-                                                   double* __restrict__ partials) {
-  const int64_t c = blockIdx.x;
-  const SysDesc* D = tab + find_seg(cell_off, nsys, c);
-  real_cell_body<T>(D->cg, D->cell0, (int)(c - D->cell0), (int)blockIdx.y, (int)gridDim.y,
-                    D->kappa, (T)D->eps, ps, sorted, start, grad,
-                    partials + c * gridDim.y + blockIdx.y, (int64_t)gridDim.x * gridDim.y);
+                                                   double* __restrict__ partials,
+                                                   int64_t rows) {
+  // work-group = (cell, split) of a system with its own split count; the
+  // items of a system are numbered cell-major from real_off[sid]
+  const int64_t item = blockIdx.x;
+  const int sid = find_seg(real_off, nsys, item);
+  const SysDesc* D = tab + sid;
+  const int S = D->nsplit;
+  const int64_t local = item - real_off[sid];
+  real_cell_body<T>(D->cg, D->cell0, (int)(local / S), (int)(local % S), S, D->kappa, (T)D->eps,
+                    ps, sorted, start, grad, partials + item, rows);
 }
 
 // ------------------------------------------------------------------ spread
@@ -471,7 +476,7 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __
 // red[s][kSysValues]; the last work-group of a system to finish (a ticket
 // counter) turns the values into energies and dE/dbox.
 struct PartialSets {
-  const double* real;  // kRealNV values x rows_real (rows = cells x splits)
+  const double* real;  // kRealNV values x rows_real (rows = (cell, split) items)
   const double* recip; // 10 values x rows_recip (x-pass blocks)
   const double* gath;  // kGatherNV values x rows_gath (gather blocks)
   int64_t rows_real, rows_recip, rows_gath;
@@ -489,13 +494,13 @@ __global__ void __launch_bounds__(kBlock) k_b_reduce_finalize(
   {
     const double* p;
     int64_t r0, r1;
-    const int64_t* cell = offs + (int64_t)(nsys + 1) * kSegCell;
+    const int64_t* cell = offs + (int64_t)(nsys + 1) * kSegReal;  // real-space rows
     const int64_t* xo = offs + (int64_t)(nsys + 1) * kSegFftX;
     const int64_t* go = offs + (int64_t)(nsys + 1) * kSegGather;
     if (v < kRealNV) {
       p = ps.real + (int64_t)v * ps.rows_real;
-      r0 = cell[s] * ps.nsplit;
-      r1 = cell[s + 1] * ps.nsplit;
+      r0 = cell[s];
+      r1 = cell[s + 1];
     } else if (v < kRealNV + 10) {
       p = ps.recip + (int64_t)(v - kRealNV) * ps.rows_recip;
       r0 = xo[s];
@@ -688,6 +693,21 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
   P.ctot = P.off[kSegCell][nsys];
   P.mtot = P.off[kSegMesh][nsys];
   P.stot = spec;
+  // real-space splits: the batch's fewest (enough work-groups for small
+  // batches), more for systems whose cells hold many candidate tests
+  // (home atoms x half-stencil candidates per cell at the mean occupancy)
+  P.nsplit = real_splits(P.ctot);
+  for (int s = 0; s < nsys; ++s) {
+    SysDesc& D = P.sys[s];
+    const CellGrid& cg = D.cg;
+    int nst = 1;
+    for (int d = 0; d < 3; ++d) nst *= cg.nc[d] >= 3 ? 3 : 2 * cg.nimg[d] + 1;
+    const double per_cell = (double)D.n / (double)std::max<int64_t>(D.ncell, 1);
+    const double tests = per_cell * per_cell * (double)((nst - 1) / 2 + 1);
+    const int want = (int)std::min<double>(std::ceil(tests / kRealBudget), (double)kRealSplitMax);
+    D.nsplit = std::max(P.nsplit, want);
+    P.off[kSegReal][s + 1] = P.off[kSegReal][s] + D.ncell * D.nsplit;
+  }
   std::memset(&P.cg_all, 0, sizeof(P.cg_all));
   P.cg_all.ncell = (int)std::min<int64_t>(P.ctot, INT32_MAX);
   nb_layout(P.ntot, P.cg_all, &P.nb);
@@ -709,8 +729,7 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
   P.o_grad = take(sizeof(double) * 3 * std::max<int64_t>(P.ntot, 1));
   P.o_mesh = take(es * std::max<int64_t>(P.mtot, 1));
   P.o_spec = take(ces * std::max<int64_t>(P.stot, 1));
-  P.nsplit = real_splits(P.ctot);
-  P.o_rpart = take(sizeof(double) * kRealNV * P.nsplit * std::max<int64_t>(P.ctot, 1));
+  P.o_rpart = take(sizeof(double) * kRealNV * std::max<int64_t>(P.off[kSegReal][nsys], 1));
   P.o_cpart = take(sizeof(double) * 10 * std::max<int64_t>(rows_x, 1));
   P.o_gpart = take(sizeof(double) * kGatherNV * std::max<int64_t>(rows_g, 1));
   P.o_red = take(sizeof(double) * kSysValues * nsys);
@@ -924,7 +943,7 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   int4* stm = (int4*)(ws + P.o_stm);
   T* stw = (T*)(ws + P.o_stw);
   int* tickets = (int*)(ws + P.o_tickets);
-  const int64_t rows_real = P.ctot * P.nsplit;
+  const int64_t rows_real = P.off[kSegReal][nsys];
   const int64_t rows_x = P.off[kSegFftX][nsys], rows_g = P.off[kSegGather][nsys];
 
   mark(0);
@@ -951,10 +970,9 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   // 0.144 ms a 24k-atom step) instead of ahead of it.
   auto launch_real = [&]() -> bool {
     if (P.ctot > 0) {
-      dim3 grid((unsigned)P.ctot, (unsigned)P.nsplit);
-      k_b_real<T><<<grid, kRealBlock, 0, rs>>>(nsys, seg(kSegCell), tab,
-                                               (const P4<double>*)(ws + P.o_ps),
-                                               (const int*)(nbws + L.sorted), start, grad, rpart);
+      k_b_real<T><<<(unsigned)rows_real, kRealBlock, 0, rs>>>(
+          nsys, seg(kSegReal), tab, (const P4<double>*)(ws + P.o_ps),
+          (const int*)(nbws + L.sorted), start, grad, rpart, rows_real);
     }
     return !join || hipEventRecord(join, rs) == hipSuccess;
   };

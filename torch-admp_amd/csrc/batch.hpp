@@ -31,6 +31,8 @@ struct SysDesc {
   int brick;      // spread by mesh bricks (owner computes, no atomics): 1 bricks of
                   // kBrickZ points along z, kBrickWaves deeper bricks (dense meshes)
   int nbr[3];     // bricks per mesh axis
+  int nsplit;     // real-space candidate splits (work-groups) per home cell
+  int pad_;
   int64_t atom0, n;
   int64_t cell0, ncell;
   int64_t mesh0, spec0;
@@ -43,7 +45,7 @@ struct SysDesc {
 // over the other's systems.
 enum SegKind {
   kSegAtom = 0, kSegCell, kSegMesh, kSegFftZ, kSegFftY, kSegFftX, kSegGather, kSegBrick,
-  kSegPlane, kSegColsX, kSegBrickZ, kSegCount
+  kSegPlane, kSegColsX, kSegBrickZ, kSegReal, kSegCount
 };
 
 struct BatchPlan {
@@ -59,7 +61,8 @@ struct BatchPlan {
   int any_untiled;     // some system spreads with global atomics
   int any_brick;       // some system spreads by mesh bricks
   int any_nonbrick;    // some system spreads into a zeroed mesh (tiles or atomics)
-  int nsplit;          // real-space candidate splits per cell
+  int nsplit;          // fewest real-space candidate splits per cell (each system
+                       // takes at least this many, SysDesc::nsplit)
   NbLayout nb;         // binning workspace over all cells
   CellGrid cg_all;     // only ncell is used by the shared binning helpers
   // workspace byte offsets

@@ -24,6 +24,13 @@ constexpr int kRealSplit = 8;     // most candidate splits (work-groups) per hom
 // groups) spill past the 1024 resident slots, and a persistent grid pulling
 // (cell, split) items from a counter measured 10-50% slower at every item count
 constexpr int64_t kRealTargetBlocks = 512;
+// A batch system whose cells hold many candidate tests (few cells: a box
+// under three cutoffs wide is one cell with its periodic images as the
+// stencil) takes more splits, up to kRealSplitMax, so that no work-group
+// tests more than about kRealBudget (home, candidate) pairs: one such
+// work-group otherwise sets the kernel's duration.
+constexpr int kRealSplitMax = 64;
+constexpr double kRealBudget = 262144.0;
 constexpr int kFilterUnroll = 4;  // candidates tested per lane between queue checks
 // per-wavefront pair ring: holds < 64 + 64 * kFilterUnroll entries, and new
 // entries never reach the slots of the 64 being evaluated
