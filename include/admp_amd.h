@@ -265,6 +265,16 @@ int admp_pme_batch(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
                    const double* q, double* energy, double* forces, double* dEdbox,
                    void* workspace, size_t ws_bytes, void* stream);
 
+/* bytes[i] = admp_pme_batch_workspace_bytes of system i alone, for n_sys
+ * systems in one call: the per-request workspace share an admission loop
+ * charges against its device-memory budget, without one host call (and one
+ * batch plan) per request.  Same arguments and errors as
+ * admp_pme_batch_workspace_bytes.  No counterpart in the reference (it
+ * evaluates one frame per call and admits nothing). */
+int admp_pme_batch_system_bytes(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                                const double* cutoffs, const int32_t* kmesh, int32_t precision,
+                                size_t* bytes);
+
 /* admp_pme_batch with flags.  ADMP_BATCH_NO_OVERLAP keeps every kernel on
  * `stream`: admp_pme_batch forks the real-space kernel onto one side stream
  * per device, which callers running batches on several streams at once would

@@ -622,6 +622,22 @@ int admp_pme_batch_workspace_bytes(int32_t n_sys, const int64_t* n_atoms, const 
   return ADMP_OK;
 }
 
+int admp_pme_batch_system_bytes(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
+                                const double* cutoffs, const int32_t* kmesh, int32_t precision,
+                                size_t* bytes) {
+  if (!bytes) return ADMP_ERR_INVALID;
+  const int st = batch_args_ok(n_sys, n_atoms, boxes, cutoffs, kmesh, precision);
+  if (st != ADMP_OK) return st;
+  for (int32_t s = 0; s < n_sys; ++s) {
+    const BatchPlan P = plan_batch(1, n_atoms + s, boxes + 9 * (int64_t)s, cutoffs + s, nullptr,
+                                   kmesh + 3 * (int64_t)s, precision, ADMP_KCONV_REFERENCE, 1.0);
+    const int ps = batch_plan_ok(P);
+    if (ps != ADMP_OK) return ps;
+    bytes[s] = P.bytes;
+  }
+  return ADMP_OK;
+}
+
 static int batch_call(int32_t n_sys, const int64_t* n_atoms, const double* boxes,
                       const double* cutoffs, const double* kappas, const int32_t* kmesh,
                       int32_t precision, int32_t convention, double dielectric,

@@ -78,6 +78,8 @@ _SIGNATURES = {
                                               _c_i32, _c_i32, _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp]),
     "admp_pme_batch_workspace_bytes": (ctypes.c_int, [_c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_i32,
                                                       _P(_c_sz)]),
+    "admp_pme_batch_system_bytes": (ctypes.c_int, [_c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_i32,
+                                                   _c_vp]),
     "admp_pme_batch": (ctypes.c_int, [_c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i32, _c_i32,
                                       _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_sz,
                                       _c_vp]),

@@ -233,9 +233,25 @@ class PMEServer:
     def request_cost(self, req: PMERequest) -> int:
         """HBM bytes a request holds while in flight: its share of the batch
         workspace plus positions, charges and results."""
-        plan = PMEBatchPlan([req.n_atoms], [req.box], req.rcut, req.kappa, [req.kmesh],
-                            self.precision, self.convention, self.dielectric)
-        return plan.workspace_bytes + 56 * req.n_atoms + 8 * 14 + 4096
+        return self.request_costs([req])[0]
+
+    def request_costs(self, reqs: List[PMERequest]) -> List[int]:
+        """request_cost of many requests from one library call."""
+        n = len(reqs)
+        if n == 0:
+            return []
+        n_atoms = np.fromiter((r.n_atoms for r in reqs), dtype=np.int64, count=n)
+        boxes = np.empty((n, 9), dtype=np.float64)
+        for k, r in enumerate(reqs):
+            boxes[k] = np.asarray(r.box, dtype=np.float64).reshape(9)
+        cut = np.fromiter((r.rcut for r in reqs), dtype=np.float64, count=n)
+        km = np.asarray([r.kmesh for r in reqs], dtype=np.int32).reshape(n, 3)
+        out = np.empty(n, dtype=np.uint64)
+        lib = _lib.load()
+        _lib.check(lib.admp_pme_batch_system_bytes(
+            n, n_atoms.ctypes.data, boxes.ctypes.data, cut.ctypes.data, km.ctypes.data,
+            _lib.PRECISION[self.precision], out.ctypes.data), "admp_pme_batch_system_bytes")
+        return [int(b) + 56 * int(a) + 8 * 14 + 4096 for b, a in zip(out, n_atoms)]
 
     def now(self) -> float:
         return time.perf_counter() - self._t0
@@ -250,8 +266,17 @@ class PMEServer:
         mesh = int(np.prod(req.kmesh))
         return int(64 * pairs + 512 * n + 48 * mesh + (1 << 20))
 
-    def submit(self, req: Union[PMERequest, QEqRequest]) -> int:
-        req.cost = self.qeq_cost(req) if isinstance(req, QEqRequest) else self.request_cost(req)
+    def submit_many(self, reqs: List[Union[PMERequest, QEqRequest]]) -> List[int]:
+        """submit() of several requests, the PME costs from one library call."""
+        pme = [r for r in reqs if not isinstance(r, QEqRequest)]
+        for r, c in zip(pme, self.request_costs(pme)):
+            r.cost = c
+        return [self.submit(r, _costed=not isinstance(r, QEqRequest)) for r in reqs]
+
+    def submit(self, req: Union[PMERequest, QEqRequest], _costed: bool = False) -> int:
+        if not _costed:
+            req.cost = (self.qeq_cost(req) if isinstance(req, QEqRequest)
+                        else self.request_cost(req))
         if req.cost > self.hbm_budget:
             raise ValueError(f"request of {req.n_atoms} atoms needs {req.cost} bytes, more than "
                              f"the HBM budget of {self.hbm_budget}")
@@ -492,9 +517,13 @@ class PMEServer:
         while i < len(order) or not self.idle():
             now = self.now()
             i0 = i
-            while i < len(order) and order[i].arrival <= now:
-                self.submit(order[i])
+            # arrivals due by now, at most one batch's worth per pass so that
+            # launching overlaps the admission of a large backlog
+            while (i < len(order) and order[i].arrival <= now and
+                   i - i0 < self.max_systems):
                 i += 1
+            if i > i0:
+                self.submit_many(order[i0:i])
             if not self.pump() and i == i0 and self._worker is not None:
                 time.sleep(5e-5)  # nothing moved: let the QEq worker have the interpreter
         wall = self.now()
