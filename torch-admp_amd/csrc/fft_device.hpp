@@ -141,14 +141,17 @@ __device__ __forceinline__ void pass_r(const cplx<T>* __restrict__ a, cplx<T>* _
   }
 }
 
-// Generic prime radix: each output is a direct sum over the r inputs read
-// from LDS, O(r^2) per butterfly, no per-thread arrays.  A thread computes up
-// to kGenOut consecutive outputs q0.. of one butterfly: term j of output q
-// carries w_L^(j step_q) with step_q = (k tstep + q rstep) mod L, so with
-// base = w_L^(j step_q0) and f = w_L^(j rstep) (a root of unity of order r)
-// the outputs take v base, v base f, v base f^2, ...: per input one LDS load
-// and two twiddle loads serve kGenOut outputs (the indices advance by one
-// addition and one conditional subtraction per term: k tstep, q rstep < L).
+// Generic odd radix r (the primes >= 7 that factorize leaves): inputs j and
+// r - j enter output q as x_j w^(jq) + x_(r-j) w^(-jq) = c a_j + i s b_j with
+// a_j = x_j + x_(r-j), b_j = x_j - x_(r-j) and w^(jq) = c + i s, and output
+// r - q as c a_j - i s b_j.  A thread takes kGenOut values q0.. of one
+// butterfly and produces the outputs q and r - q of each (plus output 0 for
+// the first group): per pair (j, r - j) two LDS loads, the inter-pass
+// twiddles, and per q two real-times-complex products into A_q and B_q,
+// w^(j(q+1)) from w^(jq) by one product with w^j.  Against a direct sum over
+// all r inputs per output this is half the loads and a quarter of the
+// multiplies.  Twiddle indices advance by additions (k j tstep and j rstep
+// stay below L; j q0 rstep wraps once at most per step).
 constexpr int kGenOut = 4;
 
 template <typename T, int SIGN>
@@ -158,40 +161,66 @@ __device__ void pass_generic(const cplx<T>* __restrict__ a, cplx<T>* __restrict_
   const int nb = L / r;
   const int tstep = L / (p * r);
   const int rstep = L / r;
-  const int ng = (r + kGenOut - 1) / kGenOut;
+  const int h = (r - 1) / 2;
+  const int ng = (h + kGenOut - 1) / kGenOut;
   for (int idx = threadIdx.x; idx < nb * ng * C; idx += blockDim.x) {
     const int c = idx % C;
     const int rest = idx / C;
     const int i = rest % nb;
-    const int q0 = (rest / nb) * kGenOut;
-    const int nq = min(kGenOut, r - q0);
+    const int g = rest / nb;
+    const int q0 = 1 + g * kGenOut;
+    const int nq = min(kGenOut, h + 1 - q0);
     const int k = i % p;
-    int step = k * tstep + q0 * rstep;
-    while (step >= L) step -= L;
+    const int dk = k * tstep;      // twiddle index of input j is k j tstep
+    const int tr = k * r * tstep;  // k L / p < L; input r - j takes tr - k j tstep
+    const int dq = q0 * rstep;     // < L
     const CT x0 = a[i * C + c];
-    CT s[kGenOut];
+    CT A[kGenOut], B[kGenOut];
 #pragma unroll
-    for (int m = 0; m < kGenOut; ++m) s[m] = x0;
-    int t = step, tf = rstep;
-    for (int j = 1; j < r; ++j) {
-      const CT v = a[(i + j * nb) * C + c];
-      CT u = cmul(v, tw_get<T>(tw, t, SIGN));
-      const CT f = tw_get<T>(tw, tf, SIGN);
-      s[0] = cadd(s[0], u);
-#pragma unroll
-      for (int m = 1; m < kGenOut; ++m) {
-        u = cmul(u, f);
-        s[m] = cadd(s[m], u);
-      }
-      t += step;
-      if (t >= L) t -= L;
+    for (int m = 0; m < kGenOut; ++m) {
+      A[m] = cmk<CT>(0, 0);
+      B[m] = cmk<CT>(0, 0);
+    }
+    CT s0 = cmk<CT>(0, 0);
+    int tj = 0, tq = 0, tf = 0;
+    for (int j = 1; j <= h; ++j) {
+      tj += dk;
+      tq += dq;
+      if (tq >= L) tq -= L;
       tf += rstep;
-      if (tf >= L) tf -= L;
+      CT xj = a[(i + j * nb) * C + c];
+      CT xm = a[(i + (r - j) * nb) * C + c];
+      if (k > 0) {
+        xj = cmul(xj, tw_get<T>(tw, tj, SIGN));
+        xm = cmul(xm, tw_get<T>(tw, tr - tj, SIGN));
+      }
+      const CT aj = cadd(xj, xm), bj = csub(xj, xm);
+      s0 = cadd(s0, aj);
+      CT w = tw_get<T>(tw, tq, SIGN);
+      const CT f = tw_get<T>(tw, tf, SIGN);
+#pragma unroll
+      for (int m = 0; m < kGenOut; ++m) {
+        A[m].x += w.x * aj.x;
+        A[m].y += w.x * aj.y;
+        B[m].x += w.y * bj.x;
+        B[m].y += w.y * bj.y;
+        if (m + 1 < kGenOut) w = cmul(w, f);
+      }
     }
     const int base = (i - k) * r + k;
+    if (g == 0) b[base * C + c] = cadd(x0, s0);
 #pragma unroll
     for (int m = 0; m < kGenOut; ++m)
-      if (m < nq) b[(base + (q0 + m) * p) * C + c] = s[m];
+      if (m < nq) {
+        const int q = q0 + m;
+        CT u, v;  // x0 + A +- i B
+        u.x = x0.x + A[m].x - B[m].y;
+        u.y = x0.y + A[m].y + B[m].x;
+        v.x = x0.x + A[m].x + B[m].y;
+        v.y = x0.y + A[m].y - B[m].x;
+        b[(base + q * p) * C + c] = u;
+        b[(base + (r - q) * p) * C + c] = v;
+      }
   }
 }
 
