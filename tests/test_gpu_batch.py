@@ -103,9 +103,9 @@ def test_batch_mixed_vs_oracle(dev, precision, etol, ftol):
 @pytest.mark.parametrize("precision,etol,ftol", [("float64", 1e-10, 1e-8),
                                                  ("float32", 1e-6, 1e-4)])
 def test_batch_prime_radix_meshes_vs_oracle(dev, precision, etol, ftol):
-    """Meshes whose line lengths take the register passes of the odd primes
-    7, 11 and 13 (13, 22 = 2 x 11, 35 = 5 x 7, 77 = 7 x 11, 39 = 3 x 13) and
-    the generic pass (17, 74 = 2 x 37), against the oracle."""
+    """Meshes whose line lengths take the generic odd-radix pass (13,
+    22 = 2 x 11, 35 = 5 x 7, 77 = 7 x 11, 39 = 3 x 13, 17, 74 = 2 x 37),
+    against the oracle."""
     systems = [_system(300, np.eye(3) * 15.0, 41) + ([13, 22, 35], 0.42),
                _system(500, TRICLINIC, 42) + ([77, 39, 17], 0.45),
                _system(400, np.diag([14.0, 16.0, 18.0]), 43) + ([74, 26, 33], 0.40)]
@@ -213,3 +213,42 @@ def test_batch_without_overlap_equals_overlap(dev, precision, tol):
     b = plan.run(pos, q, overlap=False)
     for x, y in zip(a, b):
         assert float((x - y).abs().max()) <= tol * max(1.0, float(x.abs().max()))
+
+
+@pytest.mark.parametrize("precision,etol,ftol", [("float64", 1e-10, 1e-8),
+                                                 ("float32", 1e-6, 1e-4)])
+def test_batch_serving_shaped_systems_vs_oracle(dev, precision, etol, ftol):
+    """The batch forms that serving-stream requests take, against the oracle
+    and against each system alone:
+    * a box under three cutoffs wide (one cell, its images the stencil) whose
+      cell holds ~5M candidate tests, so it takes ~20 work-groups, more than
+      the batch's fewest splits per cell (8 here);
+    * dense meshes (>= 64 mesh points per atom, K_z >= 38): bricks four times
+      deeper along z;
+    * a 202 x 200 x 199 mesh (199 prime: the generic odd-radix pass with
+      r = 199), whose half spectrum alone exceeds the 4M elements from which
+      the line passes take 512 elements per work-group;
+    * a triclinic box with prime mesh dimensions."""
+    systems = [_system(600, np.eye(3) * 14.4, 51) + ([24, 25, 27], 0.42),
+               _system(400, np.eye(3) * 16.0, 52) + ([48, 45, 52], 0.45),
+               _system(1500, np.eye(3) * 22.0, 53) + ([202, 200, 199], 0.45),
+               _system(500, TRICLINIC, 54) + ([37, 41, 43], 0.45)]
+    energy, forces, dedbox = _run(systems, dev, precision)
+    for i, (pos, q, box, km, kap) in enumerate(systems):
+        e_ref, f_ref, gb_ref = _oracle(pos, q, box, km, kap)
+        e = energy[i].cpu().numpy()
+        # float32: relative to the magnitude of the terms (random charges can
+        # give a total ~1/100 of them, which float32 sums do not resolve to
+        # 1e-6 of the total; the serving bound states the same scale)
+        scale = abs(e_ref) if precision == "float64" else float(np.abs(e[:4]).sum())
+        assert abs(e[4] - e_ref) <= etol * scale, (i, e[4], e_ref)
+        rmse = float(np.sqrt(((forces[i].cpu().numpy() - f_ref) ** 2).mean()))
+        assert rmse < ftol, (i, rmse)
+        gb = dedbox[i].cpu().numpy()
+        assert np.abs(gb - gb_ref).max() <= 100 * etol * np.abs(gb_ref).max(), i
+        # the same system alone (its own split counts, brick depth and line
+        # chunks are chosen per system or from its own spectrum size)
+        e1, f1, g1 = _run([systems[i]], dev, precision)
+        tol = 1e-12 if precision == "float64" else 1e-6
+        assert abs(float(e1[0, 4]) - e[4]) <= tol * scale, i
+        assert float((f1[0] - forces[i]).abs().max()) <= tol * max(1.0, float(forces[i].abs().max())) * 10, i
