@@ -66,6 +66,8 @@ def parse():
                     help="run only the batched water-box roofline run and print its JSON "
                          "(for per-kernel counters)")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay timing")
+    ap.add_argument("--electrode-only", action="store_true",
+                    help="only the CONP electrode step (for its kernel profile)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device copy of this many MiB (a known-size kernel "
                          "for calibrating PMC byte counters)")
@@ -651,8 +653,24 @@ def main():
     from torch_admp_amd.workloads import spc_water
 
     dev = torch.device("cuda:0")
+
+    def calib_copy():
+        # a device copy of known size, whose FETCH_SIZE / WRITE_SIZE scale the counters
+        if args.calib_copy_mib > 0:
+            src = torch.empty(args.calib_copy_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
+            dst = torch.empty_like(src)
+            torch.cuda.synchronize()
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            del src, dst
+
     if args.qeq_only:
+        calib_copy()
         print(json.dumps({"qeq_3000": qeq_bench(torch, dev)}), flush=True)
+        return
+    if args.electrode_only:
+        calib_copy()
+        print(json.dumps({"electrode_3600": electrode_bench(torch, dev)}), flush=True)
         return
     n_mol, rc, ethresh, kmesh = 8000, 8.0, 1e-5, 64
     pos_np, q_np, box_np = spc_water(n_mol, seed=2024)
@@ -684,25 +702,13 @@ def main():
         print(json.dumps({"serving_capacity": out}), flush=True)
         return
     if args.batched_only:
-        if args.calib_copy_mib > 0:
-            src = torch.empty(args.calib_copy_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
-            dst = torch.empty_like(src)
-            torch.cuda.synchronize()
-            dst.copy_(src)
-            torch.cuda.synchronize()
-            del src, dst
+        calib_copy()
         kappa = CoulombForceModule(rc, ethresh, kmesh=kmesh).kappa
         print(json.dumps({"roofline_batched": batched_roofline(
             max(args.batch_copies, 1), pos_np, q_np, box_np, rc, kappa, [kmesh] * 3, torch,
             dev)}), flush=True)
         return
-    if args.calib_copy_mib > 0:
-        src = torch.empty(args.calib_copy_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+    calib_copy()
 
     mod32 = CoulombForceModule(rc, ethresh, kmesh=kmesh, precision="float32")
     mod64 = CoulombForceModule(rc, ethresh, kmesh=kmesh, precision="float64")

@@ -310,7 +310,7 @@ __global__ void __launch_bounds__(kBlock) k_b_r2c(int nsys, const int64_t* __res
   const T* mesh = meshes + D->mesh0;
   const int64_t nrows = (int64_t)D->g.k[0] * D->g.k[1];
   r2c_rows<T>([=](int64_t row, int z) { return mesh[row * L + z]; }, specs + D->spec0, nrows,
-              (blockIdx.x - off[sid]) * 2 * C, L, C, D->rad[2], D->plane != 0);
+              (blockIdx.x - off[sid]) * 2 * C, L, C, D->rad[2], D->plane != 0, D->kzp);
 }
 
 template <typename T>
@@ -322,10 +322,10 @@ __global__ void __launch_bounds__(kBlock) k_b_c2r(int nsys, const int64_t* __res
   const SysDesc* D = tab + sid;
   const int C = D->fchunk[0];
   c2r_rows<T>(specs + D->spec0, meshes + D->mesh0, (int64_t)D->g.k[0] * D->g.k[1],
-              (blockIdx.x - off[sid]) * 2 * C, D->g.k[2], C, D->rad[2], D->plane != 0);
+              (blockIdx.x - off[sid]) * 2 * C, D->g.k[2], C, D->rad[2], D->plane != 0, D->kzp);
 }
 
-// lines along y: element (x, y, kz) at (x * K1 + y) * Kzh + kz
+// lines along y: element (x, y, kz) at (x * K1 + y) * kzp + kz
 template <typename T, int SIGN>
 __global__ void __launch_bounds__(kBlock) k_b_cols_y(int nsys, const int64_t* __restrict__ off,
                                                      const SysDesc* __restrict__ tab,
@@ -336,7 +336,8 @@ __global__ void __launch_bounds__(kBlock) k_b_cols_y(int nsys, const int64_t* __
   const int nchunk = (Kzh + C - 1) / C;
   const int64_t lb = blockIdx.x - off[sid];
   const int x = (int)(lb / nchunk), chunk = (int)(lb % nchunk);
-  strided_lines<T, SIGN>(specs + D->spec0 + (int64_t)x * K1 * Kzh + (int64_t)chunk * C, K1, Kzh,
+  strided_lines<T, SIGN>(specs + D->spec0 + (int64_t)x * K1 * D->kzp + (int64_t)chunk * C, K1,
+                         D->kzp,
                          min(C, Kzh - chunk * C), C, D->rad[1]);
 }
 
@@ -364,7 +365,7 @@ __global__ void __launch_bounds__(kBlock) k_b_cols_x(int nsys, const int64_t* __
   const int64_t lb = blockIdx.x - off[sid];
   __shared__ double vals[10];
   x_convolve_lines<T>(specs + D->spec0, (int)(lb / nchunk), (int)(lb % nchunk) * C, C, D->rad[0],
-                      P, vals);
+                      P, vals, D->kzp);
   __syncthreads();
   if (threadIdx.x < 10) partials[(int64_t)threadIdx.x * nrows + row] = vals[threadIdx.x] * P.eps;
 }
@@ -669,14 +670,20 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
     D.cell0 = P.off[kSegCell][s];
     D.ncell = D.cg.ncell;
     D.mesh0 = P.off[kSegMesh][s];
-    D.spec0 = spec;
-    spec += (int64_t)K0 * K1 * Kzh;
     P.off[kSegAtom][s + 1] = D.atom0 + D.n;
     P.off[kSegCell][s + 1] = D.cell0 + D.ncell;
     P.off[kSegMesh][s + 1] = D.mesh0 + (int64_t)K0 * K1 * K2;
     P.off[kSegFftZ][s + 1] = P.off[kSegFftZ][s] + ((int64_t)K0 * K1 + 2 * D.fchunk[0] - 1) / (2 * D.fchunk[0]);
     const size_t lds_plane = plane_lds_bytes(K0, K1, ces);
     D.plane = plane_fits(K0, K1, D.rad[0], D.rad[1], lds_plane) ? 1 : 0;
+    // line layout: rows padded to whole 128-byte lines, so that the line passes'
+    // chunks of consecutive kz do not straddle cache lines; every system's
+    // spectrum starts on a 128-byte boundary
+    const int line_elems = (int)(128 / ces);
+    D.spec0 = spec;
+    D.kzp = D.plane ? Kzh : (Kzh + line_elems - 1) / line_elems * line_elems;
+    spec += (int64_t)K0 * K1 * D.kzp;
+    spec = (spec + line_elems - 1) / line_elems * line_elems;
     P.off[kSegFftY][s + 1] =
         P.off[kSegFftY][s] + (D.plane ? 0 : (int64_t)K0 * ((Kzh + D.fchunk[1] - 1) / D.fchunk[1]));
     const int64_t xrows = D.plane ? Kzh : (int64_t)K1 * ((Kzh + D.fchunk[2] - 1) / D.fchunk[2]);

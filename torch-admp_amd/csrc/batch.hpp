@@ -32,7 +32,8 @@ struct SysDesc {
                   // kBrickZ points along z, kBrickWaves deeper bricks (dense meshes)
   int nbr[3];     // bricks per mesh axis
   int nsplit;     // real-space candidate splits (work-groups) per home cell
-  int pad_;
+  int kzp;        // row pitch of the half spectrum (line layout: K2/2 + 1 rounded up to
+                  // whole 128-byte lines; plane layout: K2/2 + 1)
   int64_t atom0, n;
   int64_t cell0, ncell;
   int64_t mesh0, spec0;

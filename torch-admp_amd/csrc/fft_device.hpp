@@ -338,11 +338,11 @@ __device__ __forceinline__ void add_dEdA(double acc[10], const InflParams& P, in
 // the FFT A_k = (Z_k + conj Z_{L-k}) / 2 and B_k = (Z_k - conj Z_{L-k}) / (2i).
 // load(row, z) supplies the real input of row `row`, element z.
 // kz_major: store element (row, k) at out[k * nrows + row] (planes of one k
-// contiguous) instead of out[row * (L / 2 + 1) + k].
+// contiguous) instead of out[row * pitch + k] (pitch 0: L / 2 + 1).
 template <typename T, typename Load>
 __device__ __forceinline__ void r2c_rows(Load load, cplx<T>* __restrict__ out, int64_t nrows,
                                          int64_t row0, int L, int C, const Radix& rd,
-                                         bool kz_major = false) {
+                                         bool kz_major = false, int pitch = 0) {
   using CT = cplx<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   CT* tw = reinterpret_cast<CT*>(smem_raw);
@@ -377,7 +377,7 @@ __device__ __forceinline__ void r2c_rows(Load load, cplx<T>* __restrict__ out, i
       o.x = T(0.5) * (zk.y + zm.y);
       o.y = T(-0.5) * (zk.x - zm.x);
     }
-    out[kz_major ? (int64_t)k * nrows + row : row * Lh + k] = o;
+    out[kz_major ? (int64_t)k * nrows + row : row * (pitch ? pitch : Lh) + k] = o;
   }
 }
 
@@ -387,14 +387,14 @@ __device__ __forceinline__ void r2c_rows(Load load, cplx<T>* __restrict__ out, i
 template <typename T>
 __device__ __forceinline__ void c2r_rows(const cplx<T>* __restrict__ in, T* __restrict__ out,
                                          int64_t nrows, int64_t row0, int L, int C,
-                                         const Radix& rd, bool kz_major = false) {
+                                         const Radix& rd, bool kz_major = false, int pitch = 0) {
   using CT = cplx<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   CT* tw = reinterpret_cast<CT*>(smem_raw);
   CT* a = tw + L;
   CT* b = a + (size_t)L * C;
   build_twiddles<T>(tw, L);
-  const int Lh = L / 2 + 1;
+  const int Lh = L / 2 + 1, Lp = pitch ? pitch : Lh;
   for (int idx = threadIdx.x; idx < C * L; idx += blockDim.x) {
     int c, k;
     if (kz_major) divmod_u(idx, C, k, c);
@@ -403,8 +403,8 @@ __device__ __forceinline__ void c2r_rows(const cplx<T>* __restrict__ in, T* __re
     const int kk = k < Lh ? k : L - k;
     const T sg = k < Lh ? T(1) : T(-1);
     CT A = cmk<CT>(0, 0), B = cmk<CT>(0, 0);
-    if (ra < nrows) A = in[kz_major ? (int64_t)kk * nrows + ra : ra * Lh + kk];
-    if (rb < nrows) B = in[kz_major ? (int64_t)kk * nrows + rb : rb * Lh + kk];
+    if (ra < nrows) A = in[kz_major ? (int64_t)kk * nrows + ra : ra * Lp + kk];
+    if (rb < nrows) B = in[kz_major ? (int64_t)kk * nrows + rb : rb * Lp + kk];
     A.y *= sg;
     B.y *= sg;
     CT z;
@@ -455,22 +455,23 @@ __device__ __forceinline__ void strided_lines(cplx<T>* __restrict__ base, int L,
 // into partial_dst when it is not null.  Gsym = (G(m) + G(m')) / 2 with m'
 // the index of -m, which makes the result exact for either k convention; the
 // half-spectrum weight w is 2 for interior kz.
+// pitch: elements between consecutive (x, y) rows of the spectrum (0: K2/2 + 1).
 template <typename T>
 __device__ __forceinline__ void x_convolve_lines(cplx<T>* __restrict__ data, int iy, int z0,
                                                  int C, const Radix& rd, const InflParams& P,
-                                                 double* __restrict__ partial_dst) {
+                                                 double* __restrict__ partial_dst, int pitch = 0) {
   using CT = cplx<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double red[10 * (kBlock / 64)];
   const int K0 = P.g.k[0], K1 = P.g.k[1], K2 = P.g.k[2];
-  const int Kzh = K2 / 2 + 1;
+  const int Kzh = K2 / 2 + 1, Kzp = pitch ? pitch : Kzh;
   const int L = K0;
-  const int64_t stride = (int64_t)K1 * Kzh;
+  const int64_t stride = (int64_t)K1 * Kzp;
   CT* tw = reinterpret_cast<CT*>(smem_raw);
   CT* a = tw + L;
   CT* b = a + (size_t)L * C;
   build_twiddles<T>(tw, L);
-  CT* base = data + (int64_t)iy * Kzh + z0;
+  CT* base = data + (int64_t)iy * Kzp + z0;
   const int cmax = min(C, Kzh - z0);
   for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
     int e, c;
