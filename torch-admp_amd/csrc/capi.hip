@@ -78,7 +78,7 @@ struct PmeLayout {
 
 PmeLayout pme_layout(const int32_t* k, int32_t precision, int64_t n = 0) {
   PmeLayout L;
-  const size_t nspec = (size_t)k[0] * k[1] * (k[2] / 2 + 1);
+  const size_t nspec = spec_elems(k, 2 * elem_size(precision));
   L.spec = 0;
   size_t off = align256(nspec * 2 * elem_size(precision));
   L.partials = off;

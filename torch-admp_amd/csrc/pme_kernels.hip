@@ -93,16 +93,16 @@ __global__ void __launch_bounds__(kBlock) k_gather(const double* __restrict__ po
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_fft_r2c_z(const T* __restrict__ in,
                                                       cplx<T>* __restrict__ out, int64_t nrows,
-                                                      int L, int C, Radix rd) {
+                                                      int L, int C, Radix rd, int pitch) {
   r2c_rows<T>([=](int64_t row, int z) { return in[row * L + z]; }, out, nrows,
-              (int64_t)blockIdx.x * 2 * C, L, C, rd);
+              (int64_t)blockIdx.x * 2 * C, L, C, rd, false, pitch);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_fft_c2r_z(const cplx<T>* __restrict__ in,
                                                       T* __restrict__ out, int64_t nrows, int L,
-                                                      int C, Radix rd) {
-  c2r_rows<T>(in, out, nrows, (int64_t)blockIdx.x * 2 * C, L, C, rd);
+                                                      int C, Radix rd, int pitch) {
+  c2r_rows<T>(in, out, nrows, (int64_t)blockIdx.x * 2 * C, L, C, rd, false, pitch);
 }
 
 template <typename T, int SIGN>
@@ -116,10 +116,12 @@ __global__ void __launch_bounds__(kBlock) k_fft_cols(cplx<T>* __restrict__ data,
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_fft_x_convolve(cplx<T>* __restrict__ data,
                                                            int C, Radix rd, InflParams P,
-                                                           double* __restrict__ partials) {
+                                                           double* __restrict__ partials,
+                                                           int pitch) {
   x_convolve_lines<T>(data, blockIdx.y, blockIdx.x * C, C, rd, P,
                       partials ? partials + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 10
-                               : nullptr);
+                               : nullptr,
+                      pitch);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,6 +366,9 @@ hipError_t launch_convolve(const T* mesh, const Geom& g, int conv, double kappa,
   const Radix r0 = factorize(K0), r1 = factorize(K1), r2 = factorize(K2);
   const int64_t nrows = (int64_t)K0 * K1;
   const size_t yz = yz_lds(K1, K2, sizeof(CT));
+  // the slab kernels store whole [y][kz] slabs (pitch Kzh); the line passes use
+  // rows padded to 128-byte lines
+  const int Kzp = yz ? Kzh : spec_row_pitch(K2, sizeof(CT));
   if (yz) {
     // passes 1-2 in one work-group per x slab
     k_fft_yz_fwd<T><<<K0, kYzThreads, yz, s>>>(mesh, spec, K1, K2, r1, r2);
@@ -373,14 +378,14 @@ hipError_t launch_convolve(const T* mesh, const Geom& g, int conv, double kappa,
     const int C = pick_chunk(K2, (nrows + 1) / 2, sizeof(CT));
     const unsigned blocks = (unsigned)((nrows + 2 * C - 1) / (2 * C));
     const size_t sh = (size_t)(K2 + 2 * K2 * C) * sizeof(CT);
-    k_fft_r2c_z<T><<<blocks, kBlock, sh, s>>>(mesh, spec, nrows, K2, C, r2);
+    k_fft_r2c_z<T><<<blocks, kBlock, sh, s>>>(mesh, spec, nrows, K2, C, r2, Kzp);
   }
   // pass 2: forward along y
   {
     const int C = pick_chunk(K1, (int64_t)K0 * Kzh, sizeof(CT));
     dim3 grid((Kzh + C - 1) / C, K0);
     const size_t sh = (size_t)(K1 + 2 * K1 * C) * sizeof(CT);
-    k_fft_cols<T, -1><<<grid, kBlock, sh, s>>>(spec, K1, Kzh, Kzh, (int64_t)K1 * Kzh, C, r1);
+    k_fft_cols<T, -1><<<grid, kBlock, sh, s>>>(spec, K1, Kzp, Kzh, (int64_t)K1 * Kzp, C, r1);
   }
   }
   // pass 3: along x with the influence function
@@ -397,7 +402,7 @@ hipError_t launch_convolve(const T* mesh, const Geom& g, int conv, double kappa,
   {
     const size_t sh = (size_t)(K0 + 2 * K0 * C3) * sizeof(CT);
     k_fft_x_convolve<T><<<grid3, kBlock, sh, s>>>(spec, C3, r0, P,
-                                                  (energy || dEdA) ? partials : nullptr);
+                                                  (energy || dEdA) ? partials : nullptr, Kzp);
   }
   const int nb3 = grid3.x * grid3.y;
   if (energy || dEdA)
@@ -411,14 +416,14 @@ hipError_t launch_convolve(const T* mesh, const Geom& g, int conv, double kappa,
       const int C = pick_chunk(K1, (int64_t)K0 * Kzh, sizeof(CT));
       dim3 grid((Kzh + C - 1) / C, K0);
       const size_t sh = (size_t)(K1 + 2 * K1 * C) * sizeof(CT);
-      k_fft_cols<T, 1><<<grid, kBlock, sh, s>>>(spec, K1, Kzh, Kzh, (int64_t)K1 * Kzh, C, r1);
+      k_fft_cols<T, 1><<<grid, kBlock, sh, s>>>(spec, K1, Kzp, Kzh, (int64_t)K1 * Kzp, C, r1);
     }
     // pass 5: c2r along z
     {
       const int C = pick_chunk(K2, (nrows + 1) / 2, sizeof(CT));
       const unsigned blocks = (unsigned)((nrows + 2 * C - 1) / (2 * C));
       const size_t sh = (size_t)(K2 + 2 * K2 * C) * sizeof(CT);
-      k_fft_c2r_z<T><<<blocks, kBlock, sh, s>>>(spec, phi, nrows, K2, C, r2);
+      k_fft_c2r_z<T><<<blocks, kBlock, sh, s>>>(spec, phi, nrows, K2, C, r2, Kzp);
     }
   }
   return hipGetLastError();

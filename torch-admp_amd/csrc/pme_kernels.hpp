@@ -10,6 +10,18 @@ namespace admp {
 constexpr int kMaxPasses = 24;
 constexpr int kMaxMesh = 1024;  // largest mesh dimension (LDS line limit)
 
+// Row pitch of a half spectrum whose rows are read in chunks of consecutive
+// kz (line layout): K2/2 + 1 complex elements of celem bytes rounded up to whole
+// 128-byte lines, so that no chunk straddles two cache lines.
+inline int spec_row_pitch(int K2, size_t celem) {
+  const int e = (int)(128 / celem);
+  return (K2 / 2 + 1 + e - 1) / e * e;
+}
+// Complex elements of a K0 x K1 x K2 mesh's half spectrum at that pitch.
+inline size_t spec_elems(const int32_t* k, size_t celem) {
+  return (size_t)k[0] * k[1] * spec_row_pitch(k[2], celem);
+}
+
 // Factorisation of an FFT length into Stockham passes: radices 8, 4, 2, 3, 5
 // first, then any remaining primes (handled by a direct-sum pass).
 struct Radix {

@@ -684,7 +684,7 @@ QeqLayout qeq_layout(const QeqProblem& P) {
       L.pbscan = take(sizeof(int) * (nbins / kBlock + 2));
     }
     L.mesh = take((size_t)P.kmesh[0] * P.kmesh[1] * P.kmesh[2] * es);
-    L.spec = take((size_t)P.kmesh[0] * P.kmesh[1] * (P.kmesh[2] / 2 + 1) * 2 * es);
+    L.spec = take(spec_elems(P.kmesh, 2 * es) * 2 * es);
     L.ppart = take(sizeof(double) * (std::max(convolve_partials_count(P.kmesh), (size_t)kRedBlocks * 9) + 16));
   } else {
     L.tg.ok = 0;
