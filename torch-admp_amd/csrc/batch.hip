@@ -476,6 +476,9 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(int nsys, const int64_t* __
 // order: thread t sums rows t, t + 256, ..., then a block tree) into
 // red[s][kSysValues]; the last work-group of a system to finish (a ticket
 // counter) turns the values into energies and dE/dbox.
+// Each system's ticket counter on a 128-byte line of its own.
+constexpr int kTicketStride = 32;
+
 struct PartialSets {
   const double* real;  // kRealNV values x rows_real (rows = (cell, split) items)
   const double* recip; // 10 values x rows_recip (x-pass blocks)
@@ -533,7 +536,7 @@ __global__ void __launch_bounds__(kBlock) k_b_reduce_finalize(
   __threadfence();
   __syncthreads();
   __shared__ int last;
-  if (threadIdx.x == 0) last = atomicAdd(&tickets[s], 1) == (int)gridDim.y - 1;
+  if (threadIdx.x == 0) last = atomicAdd(&tickets[s * kTicketStride], 1) == (int)gridDim.y - 1;
   __syncthreads();
   if (last && threadIdx.x == 0) {
     __threadfence();
@@ -554,6 +557,66 @@ __global__ void __launch_bounds__(kBlock) k_b_reduce_finalize(
                     energy + 5 * s, dEdbox ? dEdbox + 9 * s : nullptr);
   }
 }
+
+// The same reductions and finalisation with one work-group per system (no
+// tickets, no fences): wavefront w sums values w, w + 4, ... over their rows,
+// then thread 0 finalises.  For batches of many systems, where the per-value
+// work-groups' ticket atomics and fences cost ~72 ns each, serialised (a batch
+// of 256 systems: 6656 work-groups, 0.48 ms).
+__global__ void __launch_bounds__(kBlock) k_b_reduce_finalize_sys(
+    int nsys, const int64_t* __restrict__ offs, const SysDesc* __restrict__ tab, PartialSets ps,
+    double* __restrict__ energy, double* __restrict__ dEdbox) {
+  __shared__ double vals[kSysValues];
+  const int s = blockIdx.x;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int64_t* cell = offs + (int64_t)(nsys + 1) * kSegReal;
+  const int64_t* xo = offs + (int64_t)(nsys + 1) * kSegFftX;
+  const int64_t* go = offs + (int64_t)(nsys + 1) * kSegGather;
+  for (int v = wid; v < kSysValues; v += kBlock / 64) {
+    const double* p;
+    int64_t r0, r1;
+    if (v < kRealNV) {
+      p = ps.real + (int64_t)v * ps.rows_real;
+      r0 = cell[s];
+      r1 = cell[s + 1];
+    } else if (v < kRealNV + 10) {
+      p = ps.recip + (int64_t)(v - kRealNV) * ps.rows_recip;
+      r0 = xo[s];
+      r1 = xo[s + 1];
+    } else {
+      p = ps.gath + (int64_t)(v - kRealNV - 10) * ps.rows_gath;
+      r0 = go[s];
+      r1 = go[s + 1];
+    }
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int64_t r = r0 + lane;
+    for (; r + 3 * 64 < r1; r += 4 * 64) {
+      a0 += p[r];
+      a1 += p[r + 64];
+      a2 += p[r + 128];
+      a3 += p[r + 192];
+    }
+    for (; r < r1; r += 64) a0 += p[r];
+    const double t = wave_sum((a0 + a1) + (a2 + a3));
+    if (lane == 0) vals[v] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const SysDesc* D = tab + s;
+    double real[18];
+    for (int k = 0; k < 7; ++k) real[k] = vals[k];
+    for (int k = 7; k < 16; ++k) real[k] = 0.0;
+    real[16] = vals[7];
+    real[17] = vals[8];
+    finalize_system(real, vals + kRealNV, vals + kRealNV + 10, D->g, D->kappa, D->eps,
+                    energy + 5 * s, dEdbox ? dEdbox + 9 * s : nullptr);
+  }
+}
+
+// Batches of at least this many systems finalise with one work-group per
+// system; smaller ones (a single system: 26 work-groups in parallel) keep the
+// per-value work-groups.
+constexpr int kFinalizePerSystem = 16;
 
 // ------------------------------------------------------------------ host
 // Lines per FFT work-group: enough for about one point per thread, within the
@@ -754,7 +817,7 @@ BatchPlan plan_batch(int nsys, const int64_t* n_atoms, const double* boxes, cons
   P.o_cpart = take(sizeof(double) * 10 * std::max<int64_t>(rows_x, 1));
   P.o_gpart = take(sizeof(double) * kGatherNV * std::max<int64_t>(rows_g, 1));
   P.o_red = take(sizeof(double) * kSysValues * nsys);
-  P.o_tickets = take(sizeof(int) * nsys);
+  P.o_tickets = take(sizeof(int) * kTicketStride * nsys);
   P.bytes = off;
   return P;
 }
@@ -858,7 +921,7 @@ static int batch_prep(const BatchPlan& P, const double* pos, const double* q, ch
   const int64_t nfill = std::max(std::max(ncount, nmesh), 3 * ntot);
   k_b_prologue<T><<<(unsigned)std::min<int64_t>((nfill / 4 + kBlock) / kBlock, 8192), kBlock, 0, s>>>(
       nsys == 1, one, tab, offs, count, ncount, (T*)(ws + P.o_mesh), nmesh,
-      (double*)(ws + P.o_grad), 3 * ntot, (int*)(ws + P.o_tickets), nsys);
+      (double*)(ws + P.o_grad), 3 * ntot, (int*)(ws + P.o_tickets), kTicketStride * nsys);
   if (ntot > 0)
     k_b_bin<<<(unsigned)std::max<int64_t>((ntot + kBlock - 1) / kBlock, 1), kBlock, 0, s>>>(
         ntot, nsys, offs + (int64_t)(nsys + 1) * kSegAtom, tab, pos, (double*)(nbws + L.w),
@@ -1022,8 +1085,11 @@ static int run_batch_t(const BatchPlan& P, const double* pos, const double* q, d
   sets.rows_recip = rows_x;
   sets.rows_gath = rows_g;
   sets.nsplit = P.nsplit;
-  k_b_reduce_finalize<<<dim3(nsys, kSysValues), kBlock, 0, s>>>(
-      nsys, offs, tab, sets, (double*)(ws + P.o_red), tickets, energy, dEdbox);
+  if (nsys >= kFinalizePerSystem)
+    k_b_reduce_finalize_sys<<<nsys, kBlock, 0, s>>>(nsys, offs, tab, sets, energy, dEdbox);
+  else
+    k_b_reduce_finalize<<<dim3(nsys, kSysValues), kBlock, 0, s>>>(
+        nsys, offs, tab, sets, (double*)(ws + P.o_red), tickets, energy, dEdbox);
   mark(6);
   return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
 }
