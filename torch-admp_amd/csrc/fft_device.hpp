@@ -102,6 +102,13 @@ __device__ __forceinline__ cplx<T> tw_get(const cplx<T>* tw, int t, int sign) {
   return w;
 }
 
+// q = x / d, r = x mod d with the multiplier m = div_magic(d) (x, d < 2^16;
+// m = 0 for d = 1): one high multiply instead of an integer division.
+__device__ __forceinline__ void divmod_m(int x, int d, uint32_t m, int& q, int& r) {
+  q = m ? (int)__umulhi((uint32_t)x, m) : x;
+  r = x - q * d;
+}
+
 // q = x / d, r = x mod d (x >= 0, d > 0 and the same in every lane): shifts
 // when d is a power of two (the usual mesh sizes), else a division.
 __device__ __forceinline__ void divmod_u(int x, int d, int& q, int& r) {
@@ -118,14 +125,14 @@ __device__ __forceinline__ void divmod_u(int x, int d, int& q, int& r) {
 // One Stockham pass with a radix known at compile time (registers only).
 template <typename T, int SIGN, int R>
 __device__ __forceinline__ void pass_r(const cplx<T>* __restrict__ a, cplx<T>* __restrict__ b,
-                                       int L, int C, int p, const cplx<T>* tw) {
+                                       int L, int C, int p, uint32_t mp, const cplx<T>* tw) {
   using CT = cplx<T>;
   const int nb = L / R;
   const int tstep = L / (p * R);
   for (int idx = threadIdx.x; idx < nb * C; idx += blockDim.x) {
     int c, i, k, ip;
     divmod_u(idx, C, i, c);
-    divmod_u(i, p, ip, k);
+    divmod_m(i, p, mp, ip, k);
     CT u[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -156,7 +163,8 @@ constexpr int kGenOut = 4;
 
 template <typename T, int SIGN>
 __device__ void pass_generic(const cplx<T>* __restrict__ a, cplx<T>* __restrict__ b, int L,
-                             int C, int p, int r, const cplx<T>* tw) {
+                             int C, int p, int r, uint32_t mp, uint32_t mnb,
+                             const cplx<T>* tw) {
   using CT = cplx<T>;
   const int nb = L / r;
   const int tstep = L / (p * r);
@@ -164,13 +172,12 @@ __device__ void pass_generic(const cplx<T>* __restrict__ a, cplx<T>* __restrict_
   const int h = (r - 1) / 2;
   const int ng = (h + kGenOut - 1) / kGenOut;
   for (int idx = threadIdx.x; idx < nb * ng * C; idx += blockDim.x) {
-    const int c = idx % C;
-    const int rest = idx / C;
-    const int i = rest % nb;
-    const int g = rest / nb;
+    int c, rest, i, g, k, ip;
+    divmod_u(idx, C, rest, c);
+    divmod_m(rest, nb, mnb, g, i);
+    divmod_m(i, p, mp, ip, k);
     const int q0 = 1 + g * kGenOut;
     const int nq = min(kGenOut, h + 1 - q0);
-    const int k = i % p;
     const int dk = k * tstep;      // twiddle index of input j is k j tstep
     const int tr = k * r * tstep;  // k L / p < L; input r - j takes tr - k j tstep
     const int dq = q0 * rstep;     // < L
@@ -230,13 +237,14 @@ __device__ __forceinline__ cplx<T>* stockham(cplx<T>* a, cplx<T>* b, int L, int 
   int p = 1;
   for (int s = 0; s < rd.n; ++s) {
     const int r = rd.r[s];
+    const uint32_t mp = rd.mp[s];
     switch (r) {
-      case 2: pass_r<T, SIGN, 2>(a, b, L, C, p, tw); break;
-      case 3: pass_r<T, SIGN, 3>(a, b, L, C, p, tw); break;
-      case 4: pass_r<T, SIGN, 4>(a, b, L, C, p, tw); break;
-      case 5: pass_r<T, SIGN, 5>(a, b, L, C, p, tw); break;
-      case 8: pass_r<T, SIGN, 8>(a, b, L, C, p, tw); break;
-      default: pass_generic<T, SIGN>(a, b, L, C, p, r, tw); break;
+      case 2: pass_r<T, SIGN, 2>(a, b, L, C, p, mp, tw); break;
+      case 3: pass_r<T, SIGN, 3>(a, b, L, C, p, mp, tw); break;
+      case 4: pass_r<T, SIGN, 4>(a, b, L, C, p, mp, tw); break;
+      case 5: pass_r<T, SIGN, 5>(a, b, L, C, p, mp, tw); break;
+      case 8: pass_r<T, SIGN, 8>(a, b, L, C, p, mp, tw); break;
+      default: pass_generic<T, SIGN>(a, b, L, C, p, r, mp, rd.mnb[s], tw); break;
     }
     __syncthreads();
     cplx<T>* t = a;
@@ -351,7 +359,7 @@ __device__ __forceinline__ void r2c_rows(Load load, cplx<T>* __restrict__ out, i
   build_twiddles<T>(tw, L);
   for (int idx = threadIdx.x; idx < 2 * C * L; idx += blockDim.x) {
     int rl, z;
-    divmod_u(idx, L, rl, z);
+    divmod_m(idx, L, rd.mlen, rl, z);
     const int64_t row = row0 + rl;
     const T v = row < nrows ? load(row, z) : T(0);
     T* dst = reinterpret_cast<T*>(&a[z * C + (rl >> 1)]);
@@ -364,7 +372,7 @@ __device__ __forceinline__ void r2c_rows(Load load, cplx<T>* __restrict__ out, i
     // consecutive threads take consecutive k (row-major store) or rows (kz-major)
     int rl, k;
     if (kz_major) divmod_u(idx, 2 * C, k, rl);
-    else divmod_u(idx, Lh, rl, k);
+    else divmod_m(idx, Lh, rd.mhalf, rl, k);
     const int64_t row = row0 + rl;
     if (row >= nrows) continue;
     const CT zk = res[k * C + (rl >> 1)];
@@ -398,7 +406,7 @@ __device__ __forceinline__ void c2r_rows(const cplx<T>* __restrict__ in, T* __re
   for (int idx = threadIdx.x; idx < C * L; idx += blockDim.x) {
     int c, k;
     if (kz_major) divmod_u(idx, C, k, c);
-    else divmod_u(idx, L, c, k);
+    else divmod_m(idx, L, rd.mlen, c, k);
     const int64_t ra = row0 + 2 * c, rb = ra + 1;
     const int kk = k < Lh ? k : L - k;
     const T sg = k < Lh ? T(1) : T(-1);
@@ -416,7 +424,7 @@ __device__ __forceinline__ void c2r_rows(const cplx<T>* __restrict__ in, T* __re
   CT* res = stockham<T, 1>(a, b, L, C, rd, tw);
   for (int idx = threadIdx.x; idx < 2 * C * L; idx += blockDim.x) {
     int rl, z;
-    divmod_u(idx, L, rl, z);
+    divmod_m(idx, L, rd.mlen, rl, z);
     const int64_t row = row0 + rl;
     if (row >= nrows) continue;
     const T* src = reinterpret_cast<const T*>(&res[z * C + (rl >> 1)]);

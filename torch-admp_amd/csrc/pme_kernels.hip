@@ -270,6 +270,14 @@ Radix factorize(int L) {
     }
   }
   if (n != 1) rd.n = -1;
+  int p = 1;
+  for (int s = 0; s < rd.n; ++s) {
+    rd.mp[s] = div_magic((uint32_t)p);
+    rd.mnb[s] = div_magic((uint32_t)(L / rd.r[s]));
+    p *= rd.r[s];
+  }
+  rd.mlen = div_magic((uint32_t)L);
+  rd.mhalf = div_magic((uint32_t)(L / 2 + 1));
   return rd;
 }
 

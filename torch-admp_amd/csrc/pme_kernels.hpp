@@ -7,7 +7,7 @@
 
 namespace admp {
 
-constexpr int kMaxPasses = 24;
+constexpr int kMaxPasses = 12;  // a length <= kMaxMesh has at most 7 factors
 constexpr int kMaxMesh = 1024;  // largest mesh dimension (LDS line limit)
 
 // Row pitch of a half spectrum whose rows are read in chunks of consecutive
@@ -23,11 +23,22 @@ inline size_t spec_elems(const int32_t* k, size_t celem) {
 }
 
 // Factorisation of an FFT length into Stockham passes: radices 8, 4, 2, 3, 5
-// first, then any remaining primes (handled by a direct-sum pass).
+// first, then any remaining primes (handled by a direct-sum pass).  With it
+// the multipliers for dividing by the pass strides without a division
+// instruction: q = mulhi(x, m) = floor(x / d) with m = floor(2^32 / d) + 1,
+// exact for x, d < 2^16 (x m / 2^32 exceeds x / d by at most 2^-16 < 1/d);
+// m = 0 stands for d = 1.
 struct Radix {
   int n;
   int r[kMaxPasses];
+  uint32_t mp[kMaxPasses];   // for p = r[0] ... r[s - 1], the product before pass s
+  uint32_t mnb[kMaxPasses];  // for L / r[s], the butterflies of pass s
+  uint32_t mlen;             // for L itself
+  uint32_t mhalf;            // for L / 2 + 1
 };
+inline uint32_t div_magic(uint32_t d) {
+  return d <= 1 ? 0u : (uint32_t)((((uint64_t)1) << 32) / d + 1);
+}
 
 Radix factorize(int L);
 bool radix_ok(const Radix& rd);
