@@ -7,6 +7,15 @@
 
 namespace admp {
 
+// Work-group size of every kernel that runs the out-of-place Stockham
+// passes and the line bodies below (r2c_rows, c2r_rows, strided_lines,
+// x_convolve_lines): all of them are launched with kBlock threads.  A
+// constant stride lets their loops run without reading the work-group size,
+// which the compiler loads from the dispatch's hidden arguments with a vector
+// memory load before each loop.
+constexpr int kFftThreads = kBlock;
+
+
 // ---------------------------------------------------------------------------
 // Stockham FFT on LDS.  C interleaved lines of length L, element e of line c
 // at buf[e * C + c].  One pass of radix r (p = product of the earlier radices,
@@ -129,7 +138,7 @@ __device__ __forceinline__ void pass_r(const cplx<T>* __restrict__ a, cplx<T>* _
   using CT = cplx<T>;
   const int nb = L / R;
   const int tstep = L / (p * R);
-  for (int idx = threadIdx.x; idx < nb * C; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < nb * C; idx += kFftThreads) {
     int c, i, k, ip;
     divmod_u(idx, C, i, c);
     divmod_m(i, p, mp, ip, k);
@@ -171,7 +180,7 @@ __device__ void pass_generic(const cplx<T>* __restrict__ a, cplx<T>* __restrict_
   const int rstep = L / r;
   const int h = (r - 1) / 2;
   const int ng = (h + kGenOut - 1) / kGenOut;
-  for (int idx = threadIdx.x; idx < nb * ng * C; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < nb * ng * C; idx += kFftThreads) {
     int c, rest, i, g, k, ip;
     divmod_u(idx, C, rest, c);
     divmod_m(rest, nb, mnb, g, i);
@@ -255,9 +264,10 @@ __device__ __forceinline__ cplx<T>* stockham(cplx<T>* a, cplx<T>* b, int L, int 
   return a;
 }
 
+// nt: the work-group's thread count (kFftThreads in the line passes).
 template <typename T>
-__device__ void build_twiddles(cplx<T>* tw, int L) {
-  for (int t = threadIdx.x; t < L; t += blockDim.x) {
+__device__ void build_twiddles(cplx<T>* tw, int L, int nt) {
+  for (int t = threadIdx.x; t < L; t += nt) {
     double s, c;
     sincospi(2.0 * t / L, &s, &c);
     tw[t].x = (T)c;
@@ -356,8 +366,8 @@ __device__ __forceinline__ void r2c_rows(Load load, cplx<T>* __restrict__ out, i
   CT* tw = reinterpret_cast<CT*>(smem_raw);
   CT* a = tw + L;
   CT* b = a + (size_t)L * C;
-  build_twiddles<T>(tw, L);
-  for (int idx = threadIdx.x; idx < 2 * C * L; idx += blockDim.x) {
+  build_twiddles<T>(tw, L, kFftThreads);
+  for (int idx = threadIdx.x; idx < 2 * C * L; idx += kFftThreads) {
     int rl, z;
     divmod_m(idx, L, rd.mlen, rl, z);
     const int64_t row = row0 + rl;
@@ -368,7 +378,7 @@ __device__ __forceinline__ void r2c_rows(Load load, cplx<T>* __restrict__ out, i
   __syncthreads();
   CT* res = stockham<T, -1>(a, b, L, C, rd, tw);
   const int Lh = L / 2 + 1;
-  for (int idx = threadIdx.x; idx < 2 * C * Lh; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < 2 * C * Lh; idx += kFftThreads) {
     // consecutive threads take consecutive k (row-major store) or rows (kz-major)
     int rl, k;
     if (kz_major) divmod_u(idx, 2 * C, k, rl);
@@ -401,9 +411,9 @@ __device__ __forceinline__ void c2r_rows(const cplx<T>* __restrict__ in, T* __re
   CT* tw = reinterpret_cast<CT*>(smem_raw);
   CT* a = tw + L;
   CT* b = a + (size_t)L * C;
-  build_twiddles<T>(tw, L);
+  build_twiddles<T>(tw, L, kFftThreads);
   const int Lh = L / 2 + 1, Lp = pitch ? pitch : Lh;
-  for (int idx = threadIdx.x; idx < C * L; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < C * L; idx += kFftThreads) {
     int c, k;
     if (kz_major) divmod_u(idx, C, k, c);
     else divmod_m(idx, L, rd.mlen, c, k);
@@ -422,7 +432,7 @@ __device__ __forceinline__ void c2r_rows(const cplx<T>* __restrict__ in, T* __re
   }
   __syncthreads();
   CT* res = stockham<T, 1>(a, b, L, C, rd, tw);
-  for (int idx = threadIdx.x; idx < 2 * C * L; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < 2 * C * L; idx += kFftThreads) {
     int rl, z;
     divmod_m(idx, L, rd.mlen, rl, z);
     const int64_t row = row0 + rl;
@@ -442,15 +452,15 @@ __device__ __forceinline__ void strided_lines(cplx<T>* __restrict__ base, int L,
   CT* tw = reinterpret_cast<CT*>(smem_raw);
   CT* a = tw + L;
   CT* b = a + (size_t)L * C;
-  build_twiddles<T>(tw, L);
-  for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
+  build_twiddles<T>(tw, L, kFftThreads);
+  for (int idx = threadIdx.x; idx < L * C; idx += kFftThreads) {
     int e, c;
     divmod_u(idx, C, e, c);
     a[idx] = c < cmax ? base[e * stride + c] : cmk<CT>(0, 0);
   }
   __syncthreads();
   CT* res = stockham<T, SIGN>(a, b, L, C, rd, tw);
-  for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < L * C; idx += kFftThreads) {
     int e, c;
     divmod_u(idx, C, e, c);
     if (c < cmax) base[e * stride + c] = res[idx];
@@ -478,10 +488,10 @@ __device__ __forceinline__ void x_convolve_lines(cplx<T>* __restrict__ data, int
   CT* tw = reinterpret_cast<CT*>(smem_raw);
   CT* a = tw + L;
   CT* b = a + (size_t)L * C;
-  build_twiddles<T>(tw, L);
+  build_twiddles<T>(tw, L, kFftThreads);
   CT* base = data + (int64_t)iy * Kzp + z0;
   const int cmax = min(C, Kzh - z0);
-  for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < L * C; idx += kFftThreads) {
     int e, c;
     divmod_u(idx, C, e, c);
     a[idx] = c < cmax ? base[e * stride + c] : cmk<CT>(0, 0);
@@ -491,7 +501,7 @@ __device__ __forceinline__ void x_convolve_lines(cplx<T>* __restrict__ data, int
   CT* oth = (res == a) ? b : a;
   // B-spline moduli theta(m) = prod_d theta_d(m_d) from per-dimension factors
   double* thx = reinterpret_cast<double*>(oth);  // the idle Stockham buffer holds the x factors
-  for (int ix = threadIdx.x; ix < L; ix += blockDim.x) thx[ix] = theta1(fft_m(ix, K0), K0);
+  for (int ix = threadIdx.x; ix < L; ix += kFftThreads) thx[ix] = theta1(fft_m(ix, K0), K0);
   __syncthreads();
   double acc[10];
 #pragma unroll
@@ -499,7 +509,7 @@ __device__ __forceinline__ void x_convolve_lines(cplx<T>* __restrict__ data, int
   const int my = fft_m(iy, K1);
   const int myp = fft_m((K1 - iy) % K1, K1);
   const double thy = theta1(my, K1);
-  for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < L * C; idx += kFftThreads) {
     int ix, c;
     divmod_u(idx, C, ix, c);
     if (c >= cmax) continue;
@@ -534,7 +544,7 @@ __device__ __forceinline__ void x_convolve_lines(cplx<T>* __restrict__ data, int
   }
   __syncthreads();
   CT* out = stockham<T, 1>(res, oth, L, C, rd, tw);
-  for (int idx = threadIdx.x; idx < L * C; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < L * C; idx += kFftThreads) {
     int e, c;
     divmod_u(idx, C, e, c);
     if (c < cmax) base[e * stride + c] = out[idx];

@@ -147,8 +147,8 @@ __global__ void __launch_bounds__(kYzThreads) k_fft_yz_fwd(const T* __restrict__
   CT* buf = reinterpret_cast<CT*>(smem_raw);
   CT* twz = buf + (size_t)K1 * Kzh;
   CT* twy = twz + K2;
-  build_twiddles<T>(twz, K2);
-  build_twiddles<T>(twy, K1);
+  build_twiddles<T>(twz, K2, blockDim.x);
+  build_twiddles<T>(twy, K1, blockDim.x);
   const T* src = mesh + (int64_t)blockIdx.x * K1 * K2;
   for (int idx = threadIdx.x; idx < np * K2; idx += blockDim.x) {
     int pr, z;
@@ -203,8 +203,8 @@ __global__ void __launch_bounds__(kYzThreads) k_fft_yz_inv(const cplx<T>* __rest
   CT* buf = reinterpret_cast<CT*>(smem_raw);
   CT* twz = buf + (size_t)K1 * Kzh;
   CT* twy = twz + K2;
-  build_twiddles<T>(twz, K2);
-  build_twiddles<T>(twy, K1);
+  build_twiddles<T>(twz, K2, blockDim.x);
+  build_twiddles<T>(twy, K1, blockDim.x);
   const CT* src = spec + (int64_t)blockIdx.x * K1 * Kzh;
   for (int idx = threadIdx.x; idx < K1 * Kzh; idx += blockDim.x) buf[idx] = src[idx];
   __syncthreads();
