@@ -506,6 +506,10 @@ __global__ void __launch_bounds__(kBlock) k_pair_ds(const double* __restrict__ p
 // pairs only) is summed per work-group into partial[block][9], reduced in a
 // fixed order by k_reduce_box.
 constexpr int kDsBwdBlocks = 1024;
+// the row form: up to one row per wavefront for 32k rows (a row's pairs are
+// chains of dependent loads; more wavefronts in flight hide them: 24k rows
+// 140 -> 120 us)
+constexpr int kDsBwdRowBlocks = 8192;
 
 __global__ void __launch_bounds__(kBlock) k_pair_ds_bwd(const double* __restrict__ gds,
                                                         const int64_t* __restrict__ pairs,
@@ -1061,7 +1065,7 @@ hipError_t launch_pair_ds(const double* pos, const int64_t* pairs, int64_t np, c
 
 size_t pair_ds_bwd_scratch_bytes(int64_t np) {
   (void)np;
-  return sizeof(double) * 9 * kDsBwdBlocks;
+  return sizeof(double) * 9 * std::max(kDsBwdBlocks, kDsBwdRowBlocks);
 }
 
 hipError_t launch_pair_ds_bwd(const double* gds, const int64_t* pairs, int64_t np,
@@ -1073,7 +1077,7 @@ hipError_t launch_pair_ds_bwd(const double* gds, const int64_t* pairs, int64_t n
   const bool box_term = gbox && !cg.obc;
   int nb;
   if (row_ptr && rev) {
-    nb = (int)std::min<int64_t>((n * 64 + kBlock - 1) / kBlock, kDsBwdBlocks);
+    nb = (int)std::min<int64_t>((n * 64 + kBlock - 1) / kBlock, kDsBwdRowBlocks);
     if (nb > 0)
       k_pair_ds_bwd_rows<<<nb, kBlock, 0, s>>>(gds, pairs, n, row_ptr, rev, pos, cg, dr, ds, gpos,
                                                box_term ? partial : nullptr);
@@ -1092,7 +1096,10 @@ hipError_t launch_pair_energy(int mode, int64_t nf, int64_t n, int64_t np, const
                               const double* bs, const int64_t* bsl, double kappa, double eps,
                               double* energy, double* gds, double* gq, double* geta,
                               double* partials, hipStream_t s) {
-  const int bpf = kPairBlocksPerFrame;
+  // one pair per thread up to kPairBlocksPerFrame work-groups a frame (the
+  // partial sums are then reduced in that fixed order)
+  const int bpf = (int)std::max<int64_t>(
+      1, std::min<int64_t>((np + kBlock - 1) / kBlock, kPairBlocksPerFrame));
   const int64_t m = nf * n;
   if (m > 0) {
     const unsigned nb = (unsigned)std::min<int64_t>((m + kBlock - 1) / kBlock, 4096);

@@ -8,7 +8,7 @@
 namespace admp {
 
 constexpr int kRealNV = 9;                // fused real-space partial values per block
-constexpr int kPairBlocksPerFrame = 1024;  // fixed grid of the pair-energy reduction
+constexpr int kPairBlocksPerFrame = 4096;  // most work-groups (partial sums) per pair-energy frame
 constexpr int kSub = 8;                   // sub-counters per cell (atomic contention)
 constexpr int64_t kMaxCells = 1 << 21;    // cap on the cell-grid size
 // staged candidates per pass of the cell kernel (32 and 64 bytes each; 512
