@@ -742,7 +742,10 @@ def main():
     names = ["bin", "real_space", "spread", "fft_convolve", "gather", "finalize"]
     stage_ms = {nm: stage[i] / max(ncalls.value, 1) for i, nm in enumerate(names)}
 
-    # algorithmic bytes per launch (fp32 bulk): see DESIGN.md
+    # algorithmic bytes per launch (fp32 bulk): spread and gather touch 216
+    # float32 mesh points per atom, the FFT reads and writes the real mesh and
+    # the half spectrum twice, real space moves 28 B per atom (xyz, q in;
+    # forces out)
     K3 = kmesh ** 3
     alg_bytes = {
         "spread": n * 216 * 4,                                    # stencil-expanded mesh updates
