@@ -88,7 +88,8 @@ __global__ void __launch_bounds__(kBlock) k_b_bin(int64_t ntot, int nsys,
                                                   const double* __restrict__ pos,
                                                   double* __restrict__ w, int* __restrict__ cell_of,
                                                   int* __restrict__ sys_of,
-                                                  int* __restrict__ count) {
+                                                  int* __restrict__ count,
+                                                  int* __restrict__ rank) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ntot) return;
   const int s = find_seg(atom_off, nsys, i);
@@ -112,11 +113,14 @@ __global__ void __launch_bounds__(kBlock) k_b_bin(int64_t ntot, int nsys,
   const int cid = (int)tab[s].cell0 + (c[0] * cg.nc[1] + c[1]) * cg.nc[2] + c[2];
   cell_of[i] = cid;
   sys_of[i] = s;
-  atomicAdd(&count[cid * kSub + (int)(i & (kSub - 1))], 1);
+  // the atom's place among its counter's atoms (the cell-order pass adds the
+  // counter's start)
+  rank[i] = atomicAdd(&count[cid * kSub + (int)(i & (kSub - 1))], 1);
 }
 
-// Cell order and packing, one thread per atom: slot = start[key] + a cursor
-// atomic (key = cell * kSub + i mod kSub), then sorted[slot] = i,
+// Cell order and packing, one thread per atom: slot = start[key] + the
+// atom's rank from the binning's counter atomic (key = cell * kSub + i mod
+// kSub), then sorted[slot] = i,
 // ps[slot] = (wrapped position, charge) and, for systems spread by bricks,
 // mref[slot] = the mesh reference point m0 of the wrapped position (x | y << 16, z).  The order within a cell is that of
 // the atomics; every consumer (centre-cell pair test, spread, gather) works
@@ -129,7 +133,7 @@ __global__ void __launch_bounds__(kBlock) k_b_scatter_pack(int64_t ntot, int nco
                                                            const int* __restrict__ count,
                                                            int* __restrict__ start_out,
                                                            const int* __restrict__ start,
-                                                           int* __restrict__ cursor,
+                                                           const int* __restrict__ rank,
                                                            const int* __restrict__ cell_of,
                                                            const double* __restrict__ w,
                                                            const double* __restrict__ q,
@@ -176,7 +180,7 @@ __global__ void __launch_bounds__(kBlock) k_b_scatter_pack(int64_t ntot, int nco
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ntot) return;
   const int key = cell_of[i] * kSub + (int)(i & (kSub - 1));
-  const int slot = (start ? start[key] : st[key]) + atomicAdd(&cursor[key], 1);
+  const int slot = (start ? start[key] : st[key]) + rank[i];
   sorted[slot] = (int)i;
   P4<double> v;
   v.x = w[3 * i];
@@ -925,14 +929,14 @@ static int batch_prep(const BatchPlan& P, const double* pos, const double* q, ch
   if (ntot > 0)
     k_b_bin<<<(unsigned)std::max<int64_t>((ntot + kBlock - 1) / kBlock, 1), kBlock, 0, s>>>(
         ntot, nsys, offs + (int64_t)(nsys + 1) * kSegAtom, tab, pos, (double*)(nbws + L.w),
-        (int*)(nbws + L.cell_of), (int*)(ws + P.o_sys_of), count);
+        (int*)(nbws + L.cell_of), (int*)(ws + P.o_sys_of), count, (int*)(nbws + L.unsorted));
   // cell starts (scanned in the scatter kernel when the counters fit LDS), scatter + pack
   const int64_t nkeys = P.ctot * kSub;
   const bool scan_inline = nkeys <= kScatterScanMax;
   if (!scan_inline) launch_scan<int>(count, nkeys, start, (int*)(nbws + L.scan), s);
   k_b_scatter_pack<<<(unsigned)std::max<int64_t>((ntot + kBlock - 1) / kBlock, 1), kBlock, 0,
                      s>>>(ntot, (int)std::min<int64_t>(nkeys, INT32_MAX), count, start,
-                          scan_inline ? nullptr : start, (int*)(nbws + L.cursor),
+                          scan_inline ? nullptr : start, (const int*)(nbws + L.unsorted),
                           (const int*)(nbws + L.cell_of), (const double*)(nbws + L.w), q,
                           (int*)(nbws + L.sorted), (P4<double>*)(ws + P.o_ps),
                           (const int*)(ws + P.o_sys_of), tab, (int2*)(ws + P.o_mref));
